@@ -1,0 +1,1728 @@
+// MI355X HOS-NWT time-stepping core: device-resident RK4(3) stepper, one
+// solveHOS2 right-hand side per stage, first-order wavemaker potential.
+//
+// Reference path (LHEEA/HOS-NWT sources/): runge4_adapt2
+// (timeintegration/runge_kutta.f90:81-360) -> solveHOS2 (main/resol_HOS.f90:270-656)
+// -> phisxy_etaxy / HOSphis_modes (resol_HOS.f90:665-1078), dealias / filter_ext
+// (main/dealiasing.f90), solveHOS_SWEET_1 / phiadd_SL (main/resol_wmkr.f90),
+// pos_1st (main/wavemaking.f90:259-304), FFTW r2r transforms
+// (fourier/fourier_FFTW.f90).
+//
+// The right-hand side runs the necessary transform sequence: identity round
+// trips (S2F after F2S), truncations whose result is only summed into a
+// filter_ext'ed total, and the dead SWENSE/PRESENT(c) branches are removed;
+// each removal is exact in exact arithmetic because truncation is a linear
+// projection.  The extended grid is the smallest N' >= ((M+1) n)/2 with a
+// 2,3,5-smooth 2(N'-1): with dealiasing factor (M+1)/2 the retained modes of
+// every product do not depend on the extended grid size (verified against the
+// FP64 oracle), so the padding is exact up to rounding.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <rocblas/rocblas.h>
+
+#include "../../include/hosnwt.h"
+#include "r2r.h"
+
+using namespace hosnwt;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+constexpr double kCut = 44.0;  // wavemaker sums: terms with k_add*x > kCut are below 1e-19
+constexpr int L_ADD = 2;       // common_vars.f90:47
+
+// ---------------------------------------------------------------------------
+// small device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_k(int i1, int i2, double pix, double piy) {
+  const double kx = i1 * pix, ky = i2 * piy;
+  return sqrt(kx * kx + ky * ky);
+}
+__device__ __forceinline__ double wave_omega(double k) { return sqrt(k * tanh(k)); }
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// block reductions for blockDim.x <= 1024 (multiple of 64)
+__device__ double block_sum(double v) {
+  __shared__ double red[16];
+  v = wsum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  double r = 0.0;
+  if (threadIdx.x < 64) {
+    r = (l < nw) ? red[l] : 0.0;
+    r = wsum(r);
+  }
+  __syncthreads();
+  return r;
+}
+__device__ double block_max(double v) {
+  __shared__ double red[16];
+  v = wmax(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  double r = 0.0;
+  if (threadIdx.x < 64) {
+    r = (l < nw) ? red[l] : 0.0;
+    r = wmax(r);
+  }
+  __syncthreads();
+  return r;
+}
+
+inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// spectral-state kernels (double unless noted)
+// ---------------------------------------------------------------------------
+// GtoF + division by omega at the start of solveHOS2 (resol_HOS.f90:344-351);
+// also partial sums of eta_hat for the mean-volume fix (resol_HOS.f90:353-364).
+template <typename T>
+__global__ void k_gtof_in(int n1, int n2, double pix, double piy, double dt, const double* __restrict__ G1,
+                          const double* __restrict__ G2, T* eta_hat, T* phis_hat, double* partial) {
+  const size_t n = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < n) {
+    const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+    const double w = wave_omega(wave_k(i1, i2, pix, piy));
+    const double c = cos(-w * dt), sn = sin(-w * dt);
+    const double F1 = G1[i] * c - G2[i] * sn;
+    const double F2 = G2[i] * c + G1[i] * sn;
+    eta_hat[i] = (T)F1;
+    phis_hat[i] = (T)((i == 0) ? F2 : F2 / w);
+    s = F1;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Mean-volume mode of eta from the paddle-corner value (resol_HOS.f90:358-364,
+// runge_kutta.f90:306-312): a single block sums the partials.
+template <typename T>
+__global__ void k_volfix(int npart, const double* partial, T* eta_hat, double* eta_hat_d, int mode, double alpha,
+                         double pos_top, double xlen, double d_hinge) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < npart; i += blockDim.x) s += partial[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    const double x = alpha * pos_top / xlen;
+    double v = 0.0;
+    if (mode == 1)
+      v = x + s * x;
+    else if (mode == 2)
+      v = 0.5 * alpha * (1 - d_hinge) * pos_top / xlen + s * x;
+    if (eta_hat) eta_hat[0] = (T)v;
+    if (eta_hat_d) eta_hat_d[0] = v;
+  }
+}
+
+// eta^(j+1)/(j+1)! = eta^j/j! / (j+1) * eta (resol_HOS.f90:974-982); |grad eta|^2.
+template <typename T>
+struct PtrSet {
+  T* p[10];
+};
+
+template <typename T>
+__global__ void k_powers(size_t n, int M, PtrSet<T> e, const T* __restrict__ ex, const T* __restrict__ ey,
+                         T* g2) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T e1 = e.p[1][i];
+  T ej = e1;
+  for (int j = 1; j < M - 1; ++j) {  // eta^M only feeds the dead last-order phimext
+    ej = ej / (T)(j + 1) * e1;
+    e.p[j + 1][i] = ej;
+  }
+  const T x = ex[i], y = ey[i];
+  g2[i] = x * x + y * y;
+}
+
+template <typename T>
+__global__ void k_sum_orders(size_t n, int M, PtrSet<T> z, T* out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T s = z.p[1][i];
+  for (int j = 2; j <= M; ++j) s += z.p[j][i];
+  out[i] = s;
+}
+
+// Free-surface products on the extended grid (resol_HOS.f90:376-392, :534-555 and
+// the phiz2 / geta2phiz / geta2phiz2 assembly of :840-905):
+//   X = -grad(phis).grad(eta) + geta2phiz - grad(eta).grad(phi_add)
+//   Y = 1/2 (-|grad phis|^2 + geta2phiz2 + phiz2)
+//       - (grad(phis).grad(phi_add) + 1/2 |grad phi_add|^2)
+// With p = M none of the reference's dealias calls inside these sums is active
+// before the final filter_ext, so all terms are pointwise.
+// The products of a sine-type slope with a cosine-type paddle-potential
+// gradient (eta_x phi_add,x, phis_x phi_add,x and their y twins) are not
+// band-limited in the cosine basis, so their filtered value depends on the
+// extended grid; they are returned separately (XA, YA: sine in x; XB, YB:
+// sine in y) and converted exactly to the reference grid afterwards.
+template <typename T>
+__global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, const T* __restrict__ py,
+                       const T* __restrict__ ex, const T* __restrict__ ey, const T* __restrict__ g2,
+                       const T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ az, T* X, T* Y,
+                       T* XA, T* YA, T* XB, T* YB) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T z[10];
+  for (int j = 1; j <= M; ++j) z[j] = zz.p[j][i];
+  const T g = g2[i];
+  T sgz = (T)0;
+  for (int j = 1; j <= M - 3; ++j) sgz += z[j];
+  if (M >= 3) sgz += z[M - 2 >= 1 ? M - 2 : 1];
+  T t1 = (T)0, gg2 = (T)0;
+  for (int j = 1; j <= (M - 2) / 2; ++j) {
+    const T t = z[j] * z[j];
+    t1 += t;
+    gg2 += g * t;
+  }
+  for (int j = 2; j <= M - 3; ++j) {
+    const int qm = min(M - 2 - j, j - 1);
+    for (int q = 1; q <= qm; ++q) {
+      const T t = (T)2 * z[j] * z[q];
+      t1 += t;
+      gg2 += g * t;
+    }
+  }
+  if (M >= 2) t1 += z[M / 2] * z[M / 2];
+  for (int j = 2; j <= M - 1; ++j) {
+    const int q0 = max(0, min(M - 2 - j, j - 1)) + 1, q1 = min(M - j, j - 1);
+    for (int q = q0; q <= q1; ++q) t1 += (T)2 * z[j] * z[q];
+  }
+  const T vpx = px[i], vpy = py[i], vex = ex[i], vey = ey[i];
+  const T vax = ax[i], vay = ay[i], vaz = az[i];
+  const T deta = -vpx * vex - vpy * vey + g * sgz;
+  const T dphis = -vpx * vpx - vpy * vpy + gg2 + t1;
+  X[i] = deta;
+  Y[i] = (T)0.5 * dphis - (T)0.5 * (vax * vax + vay * vay + vaz * vaz);
+  XA[i] = -vex * vax;
+  YA[i] = -vpx * vax;
+  if (XB) {
+    XB[i] = -vey * vay;
+    YB[i] = -vpy * vay;
+  }
+}
+
+// D1 += RA_X + RB_X, D2 += RA_Y + RB_Y (exact reference-grid filter of the
+// mixed-parity products).
+template <typename T>
+__global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* D2) {
+  const size_t n = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+  T a = RA[i], b = RA[n + i];
+  if (RB) {
+    a += RB[i1 + (size_t)2 * n1 * i2];
+    b += RB[n1 + i1 + (size_t)2 * n1 * i2];
+  }
+  D1[i] += a;
+  D2[i] += b;
+}
+
+template <typename T>
+__global__ void k_add3(size_t n, const T* a, const T* b, T* out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + b[i];
+}
+
+// dphin (physical) = F2S(D1+P) + phiz_add ; Z = -phit_add - nu dphin
+template <typename T>
+__global__ void k_dphin(size_t n, const T* __restrict__ fq, const double* __restrict__ phiz_add,
+                        const double* __restrict__ phit_add, const double* __restrict__ nu, T* Z) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double dphin = (double)fq[i] + phiz_add[i];
+  Z[i] = (T)(-phit_add[i] - (nu ? nu[i] : 0.0) * dphin);
+}
+
+// NL1 = D1 + P - phis_hat kth1 + S2F(phiz_add) ; NL2 = D2 + S2F(Z) ; expon
+// (resol_HOS.f90:623-644, variablechange.f90:37-80); k_G2(1,1) = 0
+// (runge_kutta.f90:150,199,205).
+template <typename T>
+__global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, const T* D1, const T* P,
+                            const T* phis_hat, const T* phizh, const T* D2, const T* Zh, double* dG1, double* dG2) {
+  const size_t n = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+  const double k = wave_k(i1, i2, pix, piy);
+  const double w = wave_omega(k);
+  const double kth1 = k * tanh(k);
+  const double nl1 = (D1 ? (double)D1[i] : 0.0) + (double)P[i] - (double)phis_hat[i] * kth1 + (double)phizh[i];
+  const double nl2 = ((D2 ? (double)D2[i] : 0.0) + (double)Zh[i]) * w;
+  const double c = cos(w * dt), s = sin(w * dt);
+  dG1[i] = nl1 * c - nl2 * s;
+  dG2[i] = (i == 0) ? 0.0 : nl2 * c + nl1 * s;
+}
+
+// ---------------------------------------------------------------------------
+// wavemaker kernels (double)
+// ---------------------------------------------------------------------------
+// sum over harmonics of pos_ini(i2,f) e^{i w t} (i w)^d, d = 0,1,2, times g(z)
+// (wavemaking.f90:259-304).  pos arrays are (n3, n2).
+__global__ void k_pos(int n2, int n3, int nf, double t, const double* __restrict__ freq,
+                      const double* __restrict__ pre, const double* __restrict__ pim, const double* __restrict__ gz,
+                      double* X, double* Xt, double* Xtt) {
+  const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i2 >= n2) return;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int f = 0; f < nf; ++f) {
+    const double w = 2.0 * M_PI * freq[f];
+    double sn, cs;
+    sincos(w * t, &sn, &cs);
+    const double pr = pre[i2 + (size_t)n2 * f], pi = pim[i2 + (size_t)n2 * f];
+    const double rr = pr * cs - pi * sn, ri = pr * sn + pi * cs;  // pos e^{iwt}
+    s0 += rr;
+    s1 += -w * ri;       // Re(i w z) = -w Im z
+    s2 += -w * w * rr;   // Re(-w^2 z)
+  }
+  for (int i3 = 0; i3 < n3; ++i3) {
+    const size_t o = i3 + (size_t)n3 * i2;
+    X[o] = gz[i3] * s0;
+    Xt[o] = gz[i3] * s1;
+    Xtt[o] = gz[i3] * s2;
+  }
+}
+
+// Paddle RHS columns and their C2 continuation (resol_wmkr.f90:148-156 and
+// match_surface_wmk :1655-1741): f1 = c1x X + c1t Xt + c1tt Xtt,
+// f2 = c2x X + c2t Xt + c2tt Xtt on i3 < n3, continued to n3_add.
+struct Cont {
+  double m[3][4];  // c_r = m[0][r] f + m[1][r] df + m[2][r] d2f
+};
+__global__ void k_paddle_rhs(int n2, int n3, int n3a, double delz, const double* X, const double* Xt,
+                             const double* Xtt, double c1x, double c1t, double c1tt, double c2x, double c2t,
+                             double c2tt, Cont cm, double* f1, double* f2) {
+  const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i2 >= n2) return;
+  for (int which = 0; which < 2; ++which) {
+    double* f = which ? f2 : f1;
+    if (!f) continue;
+    const double cx = which ? c2x : c1x, ct = which ? c2t : c1t, ctt = which ? c2tt : c1tt;
+    // evaluation order of the reference: - a Xtt - 2 a' Xt - a'' X
+    for (int i3 = 0; i3 < n3; ++i3) {
+      const size_t o = i3 + (size_t)n3 * i2;
+      f[i3 + (size_t)n3a * i2] = -ctt * Xtt[o] - ct * Xt[o] - cx * X[o];
+    }
+    const double* col = f + (size_t)n3a * i2;
+    const double fm3 = col[n3 - 4], fm2 = col[n3 - 3], fm1 = col[n3 - 2], f0 = col[n3 - 1];
+    const double df = (fm2 - 4.0 * fm1 + 3.0 * f0) / (2.0 * delz);
+    const double d2f = (-fm3 + 4.0 * fm2 - 5.0 * fm1 + 2.0 * f0) / (delz * delz);
+    double c[4];
+    for (int r = 0; r < 4; ++r) c[r] = cm.m[0][r] * f0 + cm.m[1][r] * df + cm.m[2][r] * d2f;
+    for (int i3 = n3; i3 < n3a; ++i3) {
+      const double xx = (double)i3 / (double)(n3 - 1) - (double)L_ADD;
+      const double xx2 = xx * xx;
+      const double pxx = (((c[0] * xx2 + c[1]) * xx2 + c[2]) * xx2 + c[3]) * xx * exp(-6.0 * (xx + 1.0));
+      f[i3 + (size_t)n3a * i2] = pxx;
+    }
+  }
+}
+
+// space_2_Fourier_add('cos','cos') then / k_add tanh(k_add xlen)
+// (fourier_FFTW.f90:432-447, resol_wmkr.f90:160-171): z REDFT01, y REDFT00.
+// Stage 1: z transform per column (table czt[k*n3a + j] = cos(pi j (2k+1)/(2 n3a))).
+__global__ void k_add_z(int n2, int n3a, const double* __restrict__ czt, const double* in1, const double* in2,
+                        double* out1, double* out2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i2 = blockIdx.y;
+  if (k >= n3a) return;
+  for (int w = 0; w < 2; ++w) {
+    const double* in = w ? in2 : in1;
+    double* out = w ? out2 : out1;
+    if (!in) continue;
+    const double* col = in + (size_t)n3a * i2;
+    double s = 0.0;
+    for (int j = 1; j < n3a; ++j) s += col[j] * czt[k * n3a + j];
+    out[k + (size_t)n3a * i2] = col[0] + 2.0 * s;
+  }
+}
+// Stage 2: y DCT-I, normalisation and division by k_add tanh(k_add xlen).
+__global__ void k_add_y(int n2, int n3a, double xlen, double piy, const double* __restrict__ cyt, const double* in1,
+                        const double* in2, double* out1, double* out2) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (k >= n3a) return;
+  const double kxa = (2 * k + 1) * M_PI / (2.0 * L_ADD);
+  const double ky = m * piy;
+  const double ka = sqrt(kxa * kxa + ky * ky);
+  const double div = ka * tanh(ka * xlen);
+  double ys2f = 1.0;
+  if (n2 > 1) ys2f = ((m == 0 || m == n2 - 1) ? 1.0 : 2.0) / (2.0 * (n2 - 1));
+  const double nrm = ys2f / (double)n3a;
+  for (int w = 0; w < 2; ++w) {
+    const double* in = w ? in2 : in1;
+    double* out = w ? out2 : out1;
+    if (!in) continue;
+    double s;
+    if (n2 == 1) {
+      s = in[k];
+    } else {
+      const int Lc = 2 * (n2 - 1);
+      s = in[k] + ((m & 1) ? -1.0 : 1.0) * in[k + (size_t)n3a * (n2 - 1)];
+      double acc = 0.0;
+      int idx = 0;
+      for (int j = 1; j < n2 - 1; ++j) {
+        idx += m;
+        if (idx >= Lc) idx -= Lc;
+        acc += in[k + (size_t)n3a * j] * cyt[idx];
+      }
+      s += 2.0 * acc;
+    }
+    out[k + (size_t)n3a * m] = s * nrm / div;
+  }
+}
+
+// csh_add_x and k_add_sh_add_x / k (HOS-NWT.f90:364-405) for one (x, k).
+__device__ __forceinline__ void csh_sh(double k, double x, double xlen, double& csh, double& sh) {
+  const double k2 = 2.0 * k;
+  const double kx3 = k2 * xlen;
+  const double e3 = (kx3 <= 700.0) ? exp(-kx3) : 0.0;
+  const double kx1 = k * x;
+  const double e1 = (kx1 <= 700.0) ? exp(-kx1) / (e3 + 1.0) : 0.0;
+  const double kx2 = k2 * (xlen - x);
+  const double e2 = (kx2 <= 700.0) ? exp(-kx2) : 0.0;
+  const double e12 = (kx1 + kx2 <= 700.0) ? e1 * e2 : 0.0;
+  csh = e1 + e12;
+  sh = e1 - e12;
+}
+
+// Additional potential on the free surface, 3-D (resol_wmkr.f90:1149-1175;
+// z = 0 variant: phiadd_1st_SWEET :485-532).  One workgroup per x-node i1,
+// threads over y-nodes.  For each wavemaker mode i3 the y-sum over j2 is a
+// dense cosine / sine sum accumulated from an LDS chunk of
+// a(i3,j2) C(i1,j2,i3); the i3 sum then applies cos / sin(kx_add (eta+1)).
+// Modes whose decay k_add x exceeds kCut contribute below 1e-19 and are skipped.
+constexpr int kPhiCh = 128;
+constexpr int kPhiMaxPer = 4;
+template <typename T, bool SURF>
+__global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, double xlen, double piy, double dx,
+                                                  const double* __restrict__ a, const double* __restrict__ da,
+                                                  const T* __restrict__ eta, double* o_phi, double* o_phit,
+                                                  double* o_phix, double* o_phiy, double* o_phiz) {
+  extern __shared__ double sm[];
+  const int Lc = 2 * (n2 - 1);
+  double* ctab = sm;
+  double* stab = sm + Lc;
+  double* A = sm + 2 * Lc;  // [5][kPhiCh]
+  const int i1 = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  for (int m = tid; m < Lc; m += nt) {
+    ctab[m] = cospi((double)m / (double)(n2 - 1));
+    stab[m] = sinpi((double)m / (double)(n2 - 1));
+  }
+  const double x = i1 * dx;
+  double out[5][kPhiMaxPer];
+  double ep1[kPhiMaxPer];
+#pragma unroll
+  for (int q = 0; q < kPhiMaxPer; ++q) {
+    const int i2 = tid + q * nt;
+    ep1[q] = 1.0;
+    if (SURF && i2 < n2) ep1[q] = (double)eta[i1 + (size_t)n1 * i2] + 1.0;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) out[f][q] = 0.0;
+  }
+  __syncthreads();
+  for (int i3 = 0; i3 < n3a; ++i3) {
+    const double kxa = (2 * i3 + 1) * M_PI / (2.0 * L_ADD);
+    if (x * kxa > kCut) break;
+    int jmax = n2;
+    if (x > 0.0) {
+      const double r = kCut / x;
+      const double ky2 = r * r - kxa * kxa;
+      jmax = (piy > 0.0) ? min(n2, (int)floor(sqrt(ky2) / piy) + 1) : n2;
+    }
+    double B[5][kPhiMaxPer];
+#pragma unroll
+    for (int f = 0; f < 5; ++f)
+#pragma unroll
+      for (int q = 0; q < kPhiMaxPer; ++q) B[f][q] = 0.0;
+    for (int j0 = 0; j0 < jmax; j0 += kPhiCh) {
+      const int jn = min(kPhiCh, jmax - j0);
+      for (int jj = tid; jj < jn; jj += nt) {
+        const int j2 = j0 + jj;
+        const double ky = j2 * piy;
+        const double k = sqrt(kxa * kxa + ky * ky);
+        double csh, sh;
+        csh_sh(k, x, xlen, csh, sh);
+        const double av = a[i3 + (size_t)n3a * j2];
+        const double dav = da[i3 + (size_t)n3a * j2];
+        A[0 * kPhiCh + jj] = av * csh;
+        A[1 * kPhiCh + jj] = dav * csh;
+        A[2 * kPhiCh + jj] = -av * sh * k;
+        A[3 * kPhiCh + jj] = -av * csh * ky;
+        A[4 * kPhiCh + jj] = -av * csh * kxa;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kPhiMaxPer; ++q) {
+        const int i2 = tid + q * nt;
+        if (i2 >= n2) continue;
+        int idx = (int)(((long)j0 * i2) % Lc);
+        for (int jj = 0; jj < jn; ++jj) {
+          const double c = ctab[idx], s = stab[idx];
+          if (SURF) {
+            B[0][q] += A[0 * kPhiCh + jj] * c;
+            B[2][q] += A[2 * kPhiCh + jj] * c;
+            B[3][q] += A[3 * kPhiCh + jj] * s;
+          }
+          B[1][q] += A[1 * kPhiCh + jj] * c;
+          B[4][q] += A[4 * kPhiCh + jj] * c;
+          idx += i2;
+          if (idx >= Lc) idx -= Lc;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < kPhiMaxPer; ++q) {
+      double sn, cs;
+      sincos(kxa * ep1[q], &sn, &cs);
+      out[0][q] += cs * B[0][q];
+      out[1][q] += cs * B[1][q];
+      out[2][q] += cs * B[2][q];
+      out[3][q] += cs * B[3][q];
+      out[4][q] += sn * B[4][q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPhiMaxPer; ++q) {
+    const int i2 = tid + q * nt;
+    if (i2 >= n2) continue;
+    const size_t o = i1 + (size_t)n1 * i2;
+    if (SURF) {
+      o_phi[o] = out[0][q];
+      o_phix[o] = out[2][q];
+      o_phiy[o] = out[3][q];
+    }
+    o_phit[o] = out[1][q];
+    o_phiz[o] = out[4][q];
+  }
+}
+
+// 2-D tank (n2 = 1): one thread per x-node.
+template <typename T, bool SURF>
+__global__ void k_phiadd2d(int n1, int n3a, double xlen, double dx, const double* __restrict__ a,
+                           const double* __restrict__ da, const T* __restrict__ eta, double* o_phi, double* o_phit,
+                           double* o_phix, double* o_phiy, double* o_phiz) {
+  const int i1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i1 >= n1) return;
+  const double x = i1 * dx;
+  const double ep1 = SURF ? (double)eta[i1] + 1.0 : 1.0;
+  double phi = 0, phit = 0, phix = 0, phiz = 0;
+  for (int i3 = 0; i3 < n3a; ++i3) {
+    const double kxa = (2 * i3 + 1) * M_PI / (2.0 * L_ADD);
+    if (x * kxa > kCut) break;
+    double csh, sh;
+    csh_sh(kxa, x, xlen, csh, sh);
+    double sn, cs;
+    sincos(kxa * ep1, &sn, &cs);
+    const double av = a[i3], dav = da[i3];
+    phi += cs * (av * csh);
+    phit += cs * (dav * csh);
+    phix += cs * (-av * sh * kxa);
+    phiz += sn * (-av * csh * kxa);
+  }
+  if (SURF) {
+    o_phi[i1] = phi;
+    o_phix[i1] = phix;
+    o_phiy[i1] = 0.0;
+  }
+  o_phit[i1] = phit;
+  o_phiz[i1] = phiz;
+}
+
+// First-order free surface RHS (resol_wmkr.f90:186-193 via phidphiz :583-636):
+// deta1 = F2S(a1 kth1) + phiz_add_1 ; tmp = -eta1 - phit_add_1 - iabsnb nu deta1.
+__global__ void k_first_order(size_t n, const double* w1, const double* phiz1, const double* phit1,
+                              const double* eta1, const double* nu, double iabs, double* deta1, double* tmp) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = w1[i] + phiz1[i];
+  deta1[i] = d;
+  tmp[i] = -eta1[i] - phit1[i] - iabs * (nu ? nu[i] : 0.0) * d;
+}
+
+// ---------------------------------------------------------------------------
+// RK kernels (double)
+// ---------------------------------------------------------------------------
+struct KSet {
+  const double* k[5];
+  double c[5];
+  int nk;
+};
+__global__ void k_axpyk(size_t n, double* out, const double* base, double h, KSet ks) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int s = 0; s < ks.nk; ++s) acc = acc + ks.k[s][i] * ks.c[s];
+  out[i] = base[i] + h * acc;
+}
+// X_new = X + h sum b k ; err = |(sum e k - sum b k) h| ; block maxima of err and |X_new|
+__global__ void k_final_err(size_t n, double* out, const double* base, double h, KSet kb, KSet ke,
+                            double* pmax_err, double* pmax_x) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double me = 0.0, mx = 0.0;
+  if (i < n) {
+    double db = 0.0, de = 0.0;
+    for (int s = 0; s < kb.nk; ++s) db = db + kb.k[s][i] * kb.c[s];
+    for (int s = 0; s < ke.nk; ++s) de = de + ke.k[s][i] * ke.c[s];
+    const double xn = base[i] + h * db;
+    out[i] = xn;
+    me = fabs((de - db) * h);
+    mx = fabs(xn);
+  }
+  me = block_max(me);
+  mx = block_max(mx);
+  if (threadIdx.x == 0) {
+    pmax_err[blockIdx.x] = me;
+    pmax_x[blockIdx.x] = mx;
+  }
+}
+__global__ void k_reduce_max(int n, const double* p, double* out) {
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, p[i]);
+  m = block_max(m);
+  if (threadIdx.x == 0) *out = m;
+}
+
+// End of runge4_adapt2 (runge_kutta.f90:281-299): GtoF, /omega, phis(1,1)
+// restored, deta = GtoF(k_s) + phis omega^2; partial sums for the volume fix.
+__global__ void k_step_end(int n1, int n2, double pix, double piy, double h, const double* G1, const double* G2,
+                           const double* kG1, const double* kG2, const double* phis_hat_cur, double* eta_hat,
+                           double* phis_hat, double* deta_hat, double* partial) {
+  const size_t n = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < n) {
+    const double phis0 = phis_hat_cur[0];
+    const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+    const double w = wave_omega(wave_k(i1, i2, pix, piy));
+    const double c = cos(-w * h), sn = sin(-w * h);
+    const double F1 = G1[i] * c - G2[i] * sn;
+    const double F2 = G2[i] * c + G1[i] * sn;
+    const double D1 = kG1[i] * c - kG2[i] * sn;
+    const double ph = (i == 0) ? phis0 : F2 / w;
+    eta_hat[i] = F1;
+    phis_hat[i] = ph;
+    deta_hat[i] = (i == 0) ? D1 : D1 + ph * w * w;
+    s = F1;
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// G1 = eta_hat, G2 = phis_hat * omega at the start of runge4_adapt2
+// (runge_kutta.f90:128-141); G2(1,1) = 0 since omega(1,1) = 0.
+__global__ void k_step_begin(int n1, int n2, double pix, double piy, const double* eta_hat,
+                             const double* phis_hat, double* G1, double* G2) {
+  const size_t n = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+  const double w = wave_omega(wave_k(i1, i2, pix, piy));
+  G1[i] = eta_hat[i];
+  G2[i] = phis_hat[i] * w;
+}
+
+__global__ void k_sub(size_t n, double* a, const double* b) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] -= b[i];
+}
+
+template <typename A, typename B>
+__global__ void k_convert(size_t n, const A* in, B* out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (B)in[i];
+}
+
+__global__ void k_absmax2(size_t n, const void* a, const void* b, int is_f32, double* pa, double* pb) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double ma = 0.0, mb = 0.0;
+  if (i < n) {
+    if (is_f32) {
+      ma = fabs((double)((const float*)a)[i]);
+      mb = fabs((double)((const float*)b)[i]);
+    } else {
+      ma = fabs(((const double*)a)[i]);
+      mb = fabs(((const double*)b)[i]);
+    }
+  }
+  ma = block_max(ma);
+  mb = block_max(mb);
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = ma;
+    pb[blockIdx.x] = mb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+class Arena {
+ public:
+  template <typename T>
+  T* get(size_t n) {
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    HIP_OK(hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    ptrs_.push_back(p);
+    bytes_ += n * sizeof(T);
+    return reinterpret_cast<T*>(p);
+  }
+  ~Arena() {
+    for (void* p : ptrs_) (void)hipFree(p);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  std::vector<void*> ptrs_;
+  size_t bytes_ = 0;
+};
+
+int pad_smooth(int N) {
+  int n = N;
+  while (!smooth235(2L * (n - 1))) ++n;
+  return n;
+}
+
+// ramp (resol_wmkr.f90:1268-1397)
+void ramp_eval(const hos_params& p, double t, double& a, double& at, double& att) {
+  const double Tr = p.Tramp, Ts = p.T_stop;
+  if (p.iramp == 11) {
+    const double Tsw = 3.0 / std::sqrt(5.0 / 9.81);
+    if (t <= Tsw) {
+      a = t / Tsw; at = 1.0 / Tsw; att = 0.0;
+    } else {
+      a = 1.0; at = 0.0; att = 0.0;
+    }
+    return;
+  }
+  if (p.iramp == 33) {
+    const double Tsw = -std::log(0.1) / (10.0 * p.T_mono);
+    a = 1.0 - std::exp(-Tsw * t);
+    at = (1.0 - a) * Tsw;
+    att = (1.0 - a) * Tsw * Tsw;
+    return;
+  }
+  if (t <= Tr) {
+    switch (p.iramp) {
+      case 0: a = 1.0; at = 0.0; att = 0.0; return;
+      case 1: a = t / Tr; at = 1.0 / Tr; att = 0.0; return;
+      case 2: { const double s = t / Tr; a = s * s; at = 2.0 * s / Tr; att = 2.0 / (Tr * Tr); return; }
+      case 21: {
+        const double T2 = Tr * Tr;
+        if (t <= Tr / 2.0) { a = 2.0 * t * t / T2; at = 4.0 * t / T2; att = 4.0 / T2; }
+        else { const double ts = t / Tr - 1.0; a = 1.0 - 2.0 * ts * ts; at = -4.0 * ts / Tr; att = -4.0 / T2; }
+        return;
+      }
+      case 4: { const double s = t / Tr, s2 = s * s; a = s2 * s2; at = 4.0 * s2 * s / Tr; att = 12.0 * s2 / (Tr * Tr); return; }
+      case 41: {
+        const double T2 = Tr * Tr, rap = 8.0 * t / Tr, rap2 = t * t / T2, rap22 = 3.0 * rap2;
+        a = (rap22 - rap + 6.0) * rap2;
+        at = (4.0 * rap22 - 3.0 * rap + 12.0) * t / T2;
+        att = (12.0 * rap22 - 6.0 * rap + 12.0) / T2;
+        return;
+      }
+      default: throw std::runtime_error("Unknown ramp");
+    }
+  }
+  if (t >= Ts + Tr) { a = at = att = 0.0; return; }
+  if (t >= Ts) {
+    const double ti = Ts + Tr - t;
+    switch (p.iramp) {
+      case 1: a = ti / Tr; at = -1.0 / Tr; att = 0.0; return;
+      case 2: { const double s = ti / Tr; a = s * s; at = -2.0 * s / Tr; att = 2.0 / (Tr * Tr); return; }
+      case 21: {
+        const double T2 = Tr * Tr;
+        if (ti <= Tr / 2.0) { a = 2.0 * ti * ti / T2; at = -4.0 * ti / T2; att = 4.0 / T2; }
+        else { const double ts = ti / Tr - 1.0; a = 1.0 - 2.0 * ts * ts; at = 4.0 * ts / Tr; att = -4.0 / T2; }
+        return;
+      }
+      case 4: { const double s = ti / Tr, s2 = s * s; a = s2 * s2; at = -4.0 * s2 * s / Tr; att = 12.0 * s2 / (Tr * Tr); return; }
+      case 41: {
+        const double T2 = Tr * Tr, rap = 8.0 * ti / Tr, rap2 = ti * ti / T2, rap22 = 3.0 * rap2;
+        a = (rap22 - rap + 6.0) * rap2;
+        at = -(4.0 * rap22 - 3.0 * rap + 12.0) * ti / T2;
+        att = (12.0 * rap22 - 6.0 * rap + 12.0) / T2;
+        return;
+      }
+      default: a = 1.0; at = 0.0; att = 0.0; return;
+    }
+  }
+  a = 1.0; at = 0.0; att = 0.0;
+}
+
+// paddle shape g(z) (wavemaking.f90:184-215)
+double gzed(double z, int igeom, double d) {
+  const double delta = std::min(d, (1.0 - d) / 10.0);
+  if (igeom == 1) return 1.0;
+  if (z >= d - 1.0 + delta) return (z + 1.0 - d) / (1.0 - d);
+  if (z <= d - 1.0 - delta) return 0.0;
+  const double u = (z - (d - 1.0)) / delta;
+  return delta / (2.0 * (1.0 - d)) *
+         (5.0 / 16.0 + u + 15.0 / 16.0 * (u * u - std::pow(u, 4) / 3.0 + std::pow(u, 6) / 15.0));
+}
+
+// Inverse of the 4x4 system P(-1)=f, P'(-1)=df, P''(-1)=d2f, P(-2/3)=0 for
+// P(x) = (c1 x^7 + c2 x^5 + c3 x^3 + c4 x) e^{-6(x+1)}  (match_surface_wmk).
+Cont continuation_matrix() {
+  double A[4][8] = {};
+  const double x0 = -1.0, a = 6.0, xb = -2.0 / 3.0;
+  const int pw[4] = {7, 5, 3, 1};
+  for (int c = 0; c < 4; ++c) {
+    const int n = pw[c];
+    const double q = std::pow(x0, n), q1 = n * std::pow(x0, n - 1), q2 = n >= 2 ? n * (n - 1) * std::pow(x0, n - 2) : 0;
+    A[0][c] = q;
+    A[1][c] = q1 - a * q;
+    A[2][c] = q2 - 2 * a * q1 + a * a * q;
+    A[3][c] = std::pow(xb, n) * std::exp(-a * (xb + 1.0));
+  }
+  for (int r = 0; r < 4; ++r) A[r][4 + r] = 1.0;
+  for (int c = 0; c < 4; ++c) {  // Gauss-Jordan with partial pivoting
+    int piv = c;
+    for (int r = c + 1; r < 4; ++r)
+      if (std::fabs(A[r][c]) > std::fabs(A[piv][c])) piv = r;
+    for (int k = 0; k < 8; ++k) std::swap(A[c][k], A[piv][k]);
+    const double d = A[c][c];
+    for (int k = 0; k < 8; ++k) A[c][k] /= d;
+    for (int r = 0; r < 4; ++r) {
+      if (r == c) continue;
+      const double f = A[r][c];
+      for (int k = 0; k < 8; ++k) A[r][k] -= f * A[c][k];
+    }
+  }
+  Cont cm{};
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 3; ++k) cm.m[k][r] = A[r][4 + k];  // c_r = sum_k inv[r][k] rhs_k
+  return cm;
+}
+
+// Sine-to-cosine conversion on the reference's N-point extended grid:
+// T[k][s] = S2F_N(sin(s pi x / L) at the N nodes)_k for k < n, s < K
+//         = s2f_k (C(s+k) + C(s-k)),  C(q) = cot(pi q / (2(N-1))) for odd q, 0 for even q,
+// from sum_{j=1}^{N-2} sin(pi q j/(N-1)) = cot(pi q/(2(N-1))) (q odd).  Stored
+// column-major (ld = n).
+std::vector<double> sin_to_cos_matrix(int N, int n, int K) {
+  const int M = N - 1;
+  auto C = [&](long q) -> double {
+    if (q % 2 == 0) return 0.0;
+    return 1.0 / std::tan(M_PI * (double)q / (2.0 * M));
+  };
+  std::vector<double> t((size_t)n * K);
+  for (int s = 0; s < K; ++s)
+    for (int k = 0; k < n; ++k) {
+      const double s2f = (k == 0 || k == N - 1) ? 1.0 / (2.0 * M) : 1.0 / M;
+      t[k + (size_t)n * s] = s2f * (C((long)s + k) + C((long)s - k));
+    }
+  return t;
+}
+
+void blas_ok(rocblas_status st, const char* what) {
+  if (st != rocblas_status_success) throw std::runtime_error(std::string("rocBLAS error in ") + what);
+}
+
+inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double* A, int lda,
+                 const double* B, int ldb, double* Cm, int ldc) {
+  const double one = 1.0, zero = 0.0;
+  blas_ok(rocblas_dgemm(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, Cm, ldc), "dgemm");
+}
+inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const float* A, int lda,
+                 const float* B, int ldb, float* Cm, int ldc) {
+  const float one = 1.0f, zero = 0.0f;
+  blas_ok(rocblas_sgemm(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, Cm, ldc), "sgemm");
+}
+
+}  // namespace
+
+// ===========================================================================
+// engine: the right-hand side in transform type T
+// ===========================================================================
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  virtual void rhs(double t, double dt, const double* G1, const double* G2, const double* a1, const double* e1,
+                   const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
+                   double* kadr) = 0;
+  virtual void slope_max(double& sx, double& sy) = 0;
+};
+
+struct hos_ctx {
+  hos_params p{};
+  int n1, n2, n3, n3a, M, N1b, N2b;
+  double pix, piy, dx, dy, delz;
+  bool wmk;
+  hipStream_t st = nullptr;
+  Arena ar;
+  Grid2<double> gs64;
+  std::unique_ptr<EngineBase> eng;
+  // host copies for the paddle-top displacement pos1st(n3,1)
+  std::vector<double> freq, pre0, pim0;
+  double gz_top = 1.0;
+  // wavemaker globals of the last evaluation (alpha, pos1st(n3,1))
+  double last_alpha = 0.0, last_pos_top = 0.0;
+  // device state
+  double *eta_hat, *phis_hat, *a1st_add, *a_add_ramp, *a1st, *eta1st;
+  double *eta_hat_t, *phis_hat_t, *a1st_add_t, *a_add_ramp_t, *a1st_t, *eta1st_t, *deta_hat, *deta_hat_t;
+  double *G1, *G2, *G1m, *G2m, *aadm, *a1m, *e1m;
+  double *kG1[5], *kG2[5], *kad1[5], *kadr[5], *ka1[5], *ke1[5];
+  double *partial, *pmax_e, *pmax_x, *red6;
+  double *phys_tmp, *nu_d;
+  double *phi_add_last;
+  hos_stats stats{};
+  bool have_trial = false;
+  int endramp = 0;
+  // rk constants (runge_kutta.f90:46-65)
+  double A[5][5] = {{0}, {0.5}, {0, 0.5}, {0, 0, 1.0}, {-1.0, 2.0}};
+  double B[5] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 0.0};
+  double C[5] = {0.0, 0.5, 0.5, 1.0, 1.0};
+  double E[5] = {1.0 / 6.0, 2.0 / 3.0, 0.0, 0.0, 1.0 / 6.0};
+
+  double pos_top(double t) const {
+    double s = 0.0;
+    for (size_t f = 0; f < freq.size(); ++f) {
+      const double w = 2.0 * M_PI * freq[f];
+      s += pre0[f] * std::cos(w * t) - pim0[f] * std::sin(w * t);
+    }
+    return gz_top * s;
+  }
+  int volmode() const {
+    if (!wmk) return 0;
+    return p.igeom == 1 ? 1 : (p.igeom == 2 ? 2 : 0);
+  }
+};
+
+template <typename T>
+struct Engine : EngineBase {
+  hos_ctx* c;
+  Grid2<T> gb;   // extended grid
+  Grid2<T> gsT;  // base grid in T
+  size_t nb, ns, na;
+  // base (T)
+  T *eta_hat, *phis_hat, *eta_phys, *P, *D1, *D2, *Q, *Zp, *Zh, *phixh, *phiyh, *phizh, *tmpS;
+  // big (T)
+  std::vector<T*> etapm, phizm, aHOS;
+  T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
+  // wavemaker / first order (double)
+  double *Xp, *Xpt, *Xptt, *f1, *f2, *fz1, *fz2, *gz, *czt, *cyt, *freq, *pre, *pim;
+  double *phi_add, *phix_add, *phiy_add, *phiz_add, *phit_add, *phit1, *phiz1, *w1, *tmp1;
+  T *phix_addT, *phiy_addT, *phiz_addT;
+  Cont cont;
+  // mixed-parity products: spectra, conversion matrices, results
+  int K1 = 0, K2 = 0;
+  T *XA, *YA, *XB, *YB, *PA, *PB, *RA, *RB, *T1, *T2;
+  rocblas_handle blas = nullptr;
+
+  explicit Engine(hos_ctx* ctx) : c(ctx) {
+    Arena& ar = c->ar;
+    gb.build(c->N1b, c->N2b);
+    gsT.build(c->n1, c->n2);
+    nb = (size_t)c->N1b * c->N2b;
+    ns = (size_t)c->n1 * c->n2;
+    na = (size_t)c->n3a * c->n2;
+    const int M = c->M;
+    eta_hat = ar.get<T>(ns); phis_hat = ar.get<T>(ns); eta_phys = ar.get<T>(ns);
+    P = ar.get<T>(ns); D1 = ar.get<T>(ns); D2 = ar.get<T>(ns); Q = ar.get<T>(ns);
+    Zp = ar.get<T>(ns); Zh = ar.get<T>(ns); phixh = ar.get<T>(ns); phiyh = ar.get<T>(ns);
+    phizh = ar.get<T>(ns); tmpS = ar.get<T>(ns);
+    phix_addT = ar.get<T>(ns); phiy_addT = ar.get<T>(ns); phiz_addT = ar.get<T>(ns);
+    etapm.assign(M + 1, nullptr);
+    phizm.assign(M + 1, nullptr);
+    aHOS.assign(M + 1, nullptr);
+    for (int j = 1; j <= M; ++j) {
+      if (j < M) etapm[j] = ar.get<T>(nb);
+      phizm[j] = ar.get<T>(nb);
+      if (j >= 2) aHOS[j] = ar.get<T>(nb);
+    }
+    etax = ar.get<T>(nb); etay = ar.get<T>(nb); phisx = ar.get<T>(nb); phisy = ar.get<T>(nb);
+    g2 = ar.get<T>(nb); phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
+    phiza = ar.get<T>(nb); X = ar.get<T>(nb); Y = ar.get<T>(nb); tmpB = ar.get<T>(nb);
+    // wavemaker
+    const int n2 = c->n2, n3 = c->n3, n3a = c->n3a;
+    Xp = ar.get<double>((size_t)n3 * n2); Xpt = ar.get<double>((size_t)n3 * n2); Xptt = ar.get<double>((size_t)n3 * n2);
+    f1 = ar.get<double>(na); f2 = ar.get<double>(na); fz1 = ar.get<double>(na); fz2 = ar.get<double>(na);
+    phi_add = ar.get<double>(ns); phix_add = ar.get<double>(ns); phiy_add = ar.get<double>(ns);
+    phiz_add = ar.get<double>(ns); phit_add = ar.get<double>(ns); phit1 = ar.get<double>(ns);
+    phiz1 = ar.get<double>(ns); w1 = ar.get<double>(ns); tmp1 = ar.get<double>(ns);
+    std::vector<double> h_gz(n3), h_czt((size_t)n3a * n3a), h_cyt(std::max(1, 2 * (n2 - 1)));
+    for (int i3 = 0; i3 < n3; ++i3) h_gz[i3] = gzed(i3 * c->delz - 1.0, c->p.igeom, c->p.d_hinge);
+    for (int k = 0; k < n3a; ++k)
+      for (int j = 0; j < n3a; ++j) h_czt[(size_t)k * n3a + j] = std::cos(M_PI * j * (2 * k + 1) / (2.0 * n3a));
+    for (int m = 0; m < (int)h_cyt.size(); ++m) h_cyt[m] = n2 > 1 ? std::cos(M_PI * m / (double)(n2 - 1)) : 1.0;
+    gz = upload(h_gz);
+    czt = upload(h_czt);
+    cyt = upload(h_cyt);
+    const int nf = c->p.n_freq;
+    std::vector<double> hf(c->p.freq, c->p.freq + nf);
+    std::vector<double> hr((size_t)n2 * nf), hi((size_t)n2 * nf);
+    if (nf) {
+      std::memcpy(hr.data(), c->p.pos_ini_re, hr.size() * sizeof(double));
+      std::memcpy(hi.data(), c->p.pos_ini_im, hi.size() * sizeof(double));
+    }
+    freq = upload(hf);
+    pre = upload(hr);
+    pim = upload(hi);
+    cont = continuation_matrix();
+    if (c->n2 > 1 && c->n2 > kPhiMaxPer * 256) throw std::runtime_error("n2 too large for the wavemaker kernel");
+    // reference extended grid (common_vars.f90:38-39) for the exact mixed-parity filter
+    const int n1 = c->n1;
+    const int Nd1 = ((c->p.p1 + 1) * n1) / 2, Nd2 = ((c->p.p2 + 1) * n2) / 2;
+    XA = ar.get<T>(nb); YA = ar.get<T>(nb);
+    XB = YB = PB = RB = T2 = nullptr;
+    if (M >= 2 && c->wmk) {
+      if (M < 3) throw std::runtime_error("the wavemaker coupling needs mHOS >= 3 (or mHOS = 1)");
+      K1 = std::min(2 * n1, c->N1b);
+      PA = ar.get<T>((size_t)K1 * 2 * n2);
+      RA = ar.get<T>((size_t)n1 * 2 * n2);
+      T1 = upload_t(sin_to_cos_matrix(Nd1, n1, K1));
+      if (n2 > 1) {
+        K2 = std::min(2 * n2, c->N2b);
+        XB = ar.get<T>(nb); YB = ar.get<T>(nb);
+        PB = ar.get<T>((size_t)2 * n1 * K2);
+        RB = ar.get<T>((size_t)2 * n1 * n2);
+        T2 = upload_t(sin_to_cos_matrix(Nd2, n2, K2));
+      }
+      blas_ok(rocblas_create_handle(&blas), "create_handle");
+      blas_ok(rocblas_set_stream(blas, c->st), "set_stream");
+    } else {
+      PA = RA = T1 = nullptr;
+    }
+  }
+  ~Engine() override {
+    if (blas) (void)rocblas_destroy_handle(blas);
+  }
+
+  T* upload_t(const std::vector<double>& v) {
+    std::vector<T> h(v.begin(), v.end());
+    T* d = c->ar.get<T>(h.size());
+    HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  double* upload(const std::vector<double>& v) {
+    double* d = c->ar.get<double>(v.size());
+    if (!v.empty()) HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    return d;
+  }
+
+  // ---- transform helpers ------------------------------------------------
+  template <class S, class D>
+  void big(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
+    r2r2d<T>(gb, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
+    c->stats.n_big++;
+  }
+  // base-grid transforms use tmpB (large enough) as their intermediate
+  template <class S, class D>
+  void base(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
+    r2r2d<T>(gsT, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
+    c->stats.n_base++;
+  }
+  template <class S, class D>
+  void base64(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
+    r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, c->st);
+    c->stats.n_base++;
+  }
+
+  void rhs(double t, double dt, const double* G1, const double* G2, const double* a1, const double* e1,
+           const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
+           double* kadr) override {
+    hos_ctx& C = *c;
+    hipStream_t st = C.st;
+    const int n1 = C.n1, n2 = C.n2, M = C.M, N1b = C.N1b, N2b = C.N2b;
+    const double pix = C.pix, piy = C.piy;
+    const int ldb = N1b;
+    C.stats.n_rhs++;
+    // --- GtoF, /omega, volume fix (uses the previous wavemaker evaluation) ---
+    const unsigned nbs = nblk(ns);
+    hipLaunchKernelGGL(k_gtof_in<T>, dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt, G1, G2, eta_hat,
+                       phis_hat, C.partial);
+    hipLaunchKernelGGL(k_volfix<T>, dim3(1), dim3(256), 0, st, (int)nbs, C.partial, eta_hat, (double*)nullptr,
+                       C.volmode(), C.last_alpha, C.last_pos_top, C.p.xlen, C.p.d_hinge);
+    // --- physical eta on the base grid ---
+    base(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), n1, n2, Dst<T>{eta_phys, n1, 0});
+    // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
+    big(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{etapm[1] ? etapm[1] : tmpB2(), ldb, 0});
+    big(KSIN, KCOS, INV, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0});
+    big(KSIN, KCOS, INV, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b,
+        Dst<T>{phisx, ldb, 0});
+    if (n2 > 1) {
+      big(KCOS, KSIN, INV, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+          Dst<T>{etay, ldb, 0});
+      big(KCOS, KSIN, INV, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+          Dst<T>{phisy, ldb, 0});
+    }
+    PtrSet<T> ep{};
+    for (int j = 1; j < M; ++j) ep.p[j] = etapm[j];
+    if (M >= 2) {
+      hipLaunchKernelGGL(k_powers<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, ep, etax, etay, g2);
+    } else {
+      hipLaunchKernelGGL(k_powers<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, 2, ep_one(), etax, etay, g2);
+    }
+    // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
+    for (int iH = 1; iH <= M; ++iH) {
+      for (int j = iH; j >= 1; --j) {
+        const int j2 = iH - j + 1;
+        DstHos<T> d{phim, phizm[iH], (j < M) ? etapm[j] : nullptr, (j >= 2) ? etapm[j - 1] : nullptr, ldb,
+                    j == iH ? 1 : 0, iH < M ? 1 : 0};
+        if (j2 == 1) {
+          big(KCOS, KCOS, INV, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, j, pix, piy), N1b, N2b, d);
+        } else {
+          const bool trunc = (j2 == M);
+          big(KCOS, KCOS, INV,
+              src_mul<T>(aHOS[j2], trunc ? n1 : ldb, trunc ? n1 : N1b, trunc ? n2 : N2b, MUL_KTH, j, pix, piy),
+              N1b, N2b, d);
+        }
+      }
+      if (iH < M) {
+        // aHOS(iH+1) = S2F_big(phimext), truncated to n1 x n2 modes when iH+1 = M
+        if (iH + 1 == M)
+          big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
+        else
+          big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), N1b, N2b, Dst<T>{aHOS[iH + 1], ldb, 0});
+      }
+    }
+    // phiz = filter_ext(sum_j phizm(j)) -> modal P
+    PtrSet<T> zz{};
+    for (int j = 1; j <= M; ++j) zz.p[j] = phizm[j];
+    hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, X);
+    big(KCOS, KCOS, FWD, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0});
+    // --- wavemaker (solveHOS_SWEET_1 + phiadd_SL) ---
+    wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr);
+    // --- free-surface products, filtered to the base modes ---
+    if (M >= 2) {
+      if (n2 == 1) {
+        HIP_OK(hipMemsetAsync(etay, 0, nb * sizeof(T), st));
+        HIP_OK(hipMemsetAsync(phisy, 0, nb * sizeof(T), st));
+      }
+      hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
+                         phixa, phiya, phiza, X, Y, XA, YA, XB, YB);
+      big(KCOS, KCOS, FWD, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0});
+      big(KCOS, KCOS, FWD, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0});
+      if (C.wmk) mixed_parity();
+      hipLaunchKernelGGL(k_add3<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)D1, (const T*)P, Q);
+    } else {
+      HIP_OK(hipMemcpyAsync(Q, P, ns * sizeof(T), hipMemcpyDeviceToDevice, st));
+    }
+    // dphin and the final base-grid RHS
+    base(KCOS, KCOS, INV, src_plain<T>(Q, n1, n1, n2), n1, n2, Dst<T>{tmpS, n1, 0});
+    hipLaunchKernelGGL(k_dphin<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)phiz_add,
+                       (const double*)phit_add, (const double*)C.nu_d, Zp);
+    base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
+    hipLaunchKernelGGL(k_expon_out<T>, dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
+                       (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
+                       (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, kG1, kG2);
+    HIP_OK(hipMemcpyAsync(C.phi_add_last, phi_add, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+
+  // filter_ext of the mixed-parity products on the reference grid: exact sine
+  // spectra on the padded grid, then the reference grid's sine-to-cosine map.
+  void mixed_parity() {
+    hos_ctx& C = *c;
+    const int n1 = C.n1, n2 = C.n2, N1b = C.N1b, N2b = C.N2b, ldb = N1b;
+    big(KSIN, KCOS, FWD, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0});
+    big(KSIN, KCOS, FWD, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0});
+    gemm(blas, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
+    if (n2 > 1) {
+      big(KCOS, KSIN, FWD, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, Dst<T>{PB, 2 * n1, 0});
+      big(KCOS, KSIN, FWD, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0});
+      gemm(blas, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
+    }
+    hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, C.st, n1, n2, (const T*)RA,
+                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2);
+  }
+
+  PtrSet<T> ep_one() {
+    PtrSet<T> e{};
+    e.p[1] = tmpB2();
+    return e;
+  }
+  T* tmpB2() { return X; }
+
+  void wavemaker(double t, const double* a1, const double* e1, const double* aad, double* ka1, double* ke1,
+                 double* kad1, double* kadr) {
+    hos_ctx& C = *c;
+    hipStream_t st = C.st;
+    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a, N1b = C.N1b, N2b = C.N2b;
+    const int ldb = N1b;
+    if (!C.wmk) {
+      HIP_OK(hipMemsetAsync(phi_add, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(phix_add, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(phiy_add, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(phiz_add, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(phit_add, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(phizh, 0, ns * sizeof(T), st));
+      HIP_OK(hipMemsetAsync(phixa, 0, nb * sizeof(T), st));
+      HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+      HIP_OK(hipMemsetAsync(phiza, 0, nb * sizeof(T), st));
+      if (ka1) HIP_OK(hipMemsetAsync(ka1, 0, ns * sizeof(double), st));
+      if (ke1) HIP_OK(hipMemsetAsync(ke1, 0, ns * sizeof(double), st));
+      if (kad1) HIP_OK(hipMemsetAsync(kad1, 0, na * sizeof(double), st));
+      if (kadr) HIP_OK(hipMemsetAsync(kadr, 0, na * sizeof(double), st));
+      return;
+    }
+    double al, alt, altt;
+    ramp_eval(C.p, t, al, alt, altt);
+    C.last_alpha = al;
+    C.last_pos_top = C.pos_top(t);
+    hipLaunchKernelGGL(k_pos, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
+                       (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt);
+    // f1 = -alpha X_tt - 2 alpha' X_t - alpha'' X ; f2 = -alpha' X_t - alpha'' X
+    hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
+                       (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2);
+    hipLaunchKernelGGL(k_add_z, dim3(nblk(n3a, 64), n2), dim3(64), 0, st, n2, n3a, (const double*)czt,
+                       (const double*)f1, (const double*)f2, fz1, fz2);
+    hipLaunchKernelGGL(k_add_y, dim3(nblk(n3a, 64), n2), dim3(64), 0, st, n2, n3a, C.p.xlen, C.piy,
+                       (const double*)cyt, (const double*)fz1, (const double*)fz2, kad1, kadr);
+    // first-order potential at z = 0 and the first-order FS RHS
+    phiadd(aad, kad1, (const T*)nullptr, false);
+    base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
+           Dst<double>{w1, n1, 0});
+    hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
+                       (const double*)phiz1, (const double*)phit1, e1, (const double*)C.nu_d,
+                       (double)C.p.iabsnb, ke1, tmp1);
+    base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp1, n1, n1, n2), n1, n2, Dst<double>{ka1, n1, 0});
+    // total additional potential at z = eta
+    phiadd(aad, kad1, eta_phys, true);
+    // grad phi_add onto the extended grid (resol_HOS.f90:534-544)
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
+    big(KCOS, KCOS, INV, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0});
+    if (n2 > 1) big(KCOS, KCOS, INV, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0});
+    else HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+    big(KCOS, KCOS, INV, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0});
+  }
+
+  void phiadd(const double* a, const double* da, const T* eta, bool surf) {
+    hos_ctx& C = *c;
+    hipStream_t st = C.st;
+    const int n1 = C.n1, n2 = C.n2, n3a = C.n3a;
+    if (n2 == 1) {
+      if (surf)
+        hipLaunchKernelGGL((k_phiadd2d<T, true>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx, a,
+                           da, eta, phi_add, phit_add, phix_add, phiy_add, phiz_add);
+      else
+        hipLaunchKernelGGL((k_phiadd2d<T, false>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx,
+                           a, da, eta, (double*)nullptr, phit1, (double*)nullptr, (double*)nullptr, phiz1);
+    } else {
+      int nt = 64;
+      while (nt < 256 && nt < n2) nt *= 2;
+      const size_t lds = (size_t)(2 * 2 * (n2 - 1) + 5 * kPhiCh) * sizeof(double);
+      if (surf)
+        hipLaunchKernelGGL((k_phiadd3d<T, true>), dim3(n1), dim3(nt), lds, st, n1, n2, n3a, C.p.xlen, C.piy, C.dx,
+                           a, da, eta, phi_add, phit_add, phix_add, phiy_add, phiz_add);
+      else
+        hipLaunchKernelGGL((k_phiadd3d<T, false>), dim3(n1), dim3(nt), lds, st, n1, n2, n3a, C.p.xlen, C.piy,
+                           C.dx, a, da, eta, (double*)nullptr, phit1, (double*)nullptr, (double*)nullptr, phiz1);
+    }
+    HIP_OK(hipGetLastError());
+  }
+
+  void slope_max(double& sx, double& sy) override {
+    hos_ctx& C = *c;
+    const unsigned nbk = nblk(nb);
+    double* pa = C.ar.get<double>(nbk);
+    double* pb = C.ar.get<double>(nbk);
+    hipLaunchKernelGGL(k_absmax2, dim3(nbk), dim3(256), 0, C.st, nb, (const void*)etax, (const void*)etay,
+                       sizeof(T) == 4 ? 1 : 0, pa, pb);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, C.st, (int)nbk, (const double*)pa, C.red6);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, C.st, (int)nbk, (const double*)pb, C.red6 + 1);
+    double h[2];
+    HIP_OK(hipMemcpyAsync(h, C.red6, 2 * sizeof(double), hipMemcpyDeviceToHost, C.st));
+    HIP_OK(hipStreamSynchronize(C.st));
+    sx = h[0];
+    sy = h[1];
+  }
+};
+
+// ===========================================================================
+// context life cycle and RK driver
+// ===========================================================================
+namespace {
+
+void check_params(const hos_params& p) {
+  if (p.n1 < 4) throw std::runtime_error("n1 must be >= 4");
+  if (p.n2 < 1 || p.n2 == 2 || p.n2 == 3) throw std::runtime_error("n2 must be 1 or >= 4");
+  if (p.n3 < 5) throw std::runtime_error("n3 must be >= 5");
+  if (p.mHOS < 1 || p.mHOS > 8) throw std::runtime_error("mHOS must be in 1..8");
+  const int p2ok = (p.n2 == 1) ? 1 : p.mHOS;
+  if (p.p1 != p.mHOS || (p.p2 != p2ok && !(p.n2 == 1 && p.p2 == p.mHOS)))
+    throw std::runtime_error("supported dealiasing: p1 = mHOS, p2 = mHOS (3-D) or 1 (2-D)");
+  if (p.i_wmk != 1 && (p.icase != 1)) throw std::runtime_error("supported wavemaker order: i_wmk = 1");
+  if (p.precision != 0 && p.precision != 1) throw std::runtime_error("precision must be 0 or 1");
+  if (p.xlen <= 0.0 || (p.n2 > 1 && p.ylen <= 0.0)) throw std::runtime_error("tank size must be positive");
+  if (p.n_freq < 0 || (p.n_freq > 0 && (!p.freq || !p.pos_ini_re || !p.pos_ini_im)))
+    throw std::runtime_error("wavemaker spectrum arrays missing");
+}
+
+// Stage time t + h c with two roundings, as the reference computes it
+// (runge_kutta.f90:186): a fused multiply-add can move a stage across a ramp
+// boundary such as t = Tramp.
+double stage_time(double t, double h, double cs) {
+#pragma clang fp contract(off)
+  const double hc = h * cs;
+  return t + hc;
+}
+
+void rhs_eval(hos_ctx* c, double t, double dt, const double* G1, const double* G2, const double* a1,
+              const double* e1, const double* aad, int s) {
+  c->eng->rhs(t, dt, G1, G2, a1, e1, aad, c->kG1[s], c->kG2[s], c->ka1[s], c->ke1[s], c->kad1[s], c->kadr[s]);
+}
+
+void rk_attempt(hos_ctx* c, double t, double h, double* err) {
+  const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+  hipStream_t st = c->st;
+  hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
+                     (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
+  rhs_eval(c, t, h * c->C[0], c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0);
+  for (int jl = 1; jl < 5; ++jl) {
+    auto comb = [&](double* out, const double* base, double* const* karr, size_t n) {
+      KSet k{};
+      k.nk = jl;
+      for (int s = 0; s < jl; ++s) {
+        k.k[s] = karr[s];
+        k.c[s] = c->A[jl][s];
+      }
+      hipLaunchKernelGGL(k_axpyk, dim3(nblk(n)), dim3(256), 0, st, n, out, base, h, k);
+    };
+    comb(c->G1m, c->G1, c->kG1, ns);
+    comb(c->G2m, c->G2, c->kG2, ns);
+    comb(c->aadm, c->a1st_add, c->kad1, na);
+    comb(c->a1m, c->a1st, c->ka1, ns);
+    comb(c->e1m, c->eta1st, c->ke1, ns);
+    rhs_eval(c, stage_time(t, h, c->C[jl]), h * c->C[jl], c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, jl);
+  }
+  // final combination and embedded error (runge_kutta.f90:225-279)
+  auto kset = [&](double* const* karr, const double* w) {
+    KSet k{};
+    k.nk = 5;
+    for (int s = 0; s < 5; ++s) {
+      k.k[s] = karr[s];
+      k.c[s] = w[s];
+    }
+    return k;
+  };
+  const unsigned nbs = nblk(ns), nba = nblk(na);
+  double* Gn1 = c->G1m;  // reuse stage buffers for the new modal G
+  double* Gn2 = c->G2m;
+  hipLaunchKernelGGL(k_final_err, dim3(nbs), dim3(256), 0, st, ns, Gn1, (const double*)c->G1, h, kset(c->kG1, c->B),
+                     kset(c->kG1, c->E), c->pmax_e, c->pmax_x);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_e, c->red6 + 0);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_x, c->red6 + 1);
+  hipLaunchKernelGGL(k_final_err, dim3(nbs), dim3(256), 0, st, ns, Gn2, (const double*)c->G2, h, kset(c->kG2, c->B),
+                     kset(c->kG2, c->E), c->pmax_e, c->pmax_x);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_e, c->red6 + 2);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_x, c->red6 + 3);
+  hipLaunchKernelGGL(k_final_err, dim3(nba), dim3(256), 0, st, na, c->a1st_add_t, (const double*)c->a1st_add, h,
+                     kset(c->kad1, c->B), kset(c->kad1, c->E), c->pmax_e, c->pmax_x);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_e, c->red6 + 4);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_x, c->red6 + 5);
+  hipLaunchKernelGGL(k_axpyk, dim3(nba), dim3(256), 0, st, na, c->a_add_ramp_t, (const double*)c->a_add_ramp, h,
+                     kset(c->kadr, c->B));
+  hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->a1st_t, (const double*)c->a1st, h,
+                     kset(c->ka1, c->B));
+  hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->eta1st_t, (const double*)c->eta1st, h,
+                     kset(c->ke1, c->B));
+  // GtoF back, deta, volume fix with the last stage's wavemaker globals
+  hipLaunchKernelGGL(k_step_end, dim3(nbs), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy, h, (const double*)Gn1,
+                     (const double*)Gn2, (const double*)c->kG1[4], (const double*)c->kG2[4],
+                     (const double*)c->phis_hat, c->eta_hat_t,
+                     c->phis_hat_t, c->deta_hat_t, c->partial);
+  hipLaunchKernelGGL(k_volfix<double>, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->partial,
+                     (double*)nullptr, c->eta_hat_t, c->volmode(), c->last_alpha, c->last_pos_top, c->p.xlen,
+                     c->p.d_hinge);
+  double r[6];
+  HIP_OK(hipMemcpyAsync(r, c->red6, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  const double tiny = 2.220446049250313e-16;
+  double e = r[0] / (r[1] + tiny);
+  e = std::max(e, r[2] / (r[3] + tiny));
+  e = std::max(e, r[4] / (r[5] + tiny));
+  *err = e;
+  c->have_trial = true;
+}
+
+void commit(hos_ctx* c) {
+  if (!c->have_trial) throw std::runtime_error("no trial state to commit");
+  std::swap(c->eta_hat, c->eta_hat_t);
+  std::swap(c->phis_hat, c->phis_hat_t);
+  std::swap(c->a1st_add, c->a1st_add_t);
+  std::swap(c->a_add_ramp, c->a_add_ramp_t);
+  std::swap(c->a1st, c->a1st_t);
+  std::swap(c->eta1st, c->eta1st_t);
+  std::swap(c->deta_hat, c->deta_hat_t);
+  c->have_trial = false;
+}
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return 1;
+  }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+const char* hos_last_error(void) { return g_last_error.c_str(); }
+
+int hos_create(const hos_params* pin, hos_ctx** out) {
+  return guard([&] {
+    if (!pin || !out) throw std::runtime_error("null argument");
+    check_params(*pin);
+    int ndev = 0;
+    HIP_OK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) throw std::runtime_error("no GPU device");
+    auto c = std::make_unique<hos_ctx>();
+    c->p = *pin;
+    const hos_params& p = c->p;
+    c->n1 = p.n1;
+    c->n2 = p.n2;
+    c->n3 = p.n3;
+    c->n3a = L_ADD * (p.n3 - 1);
+    c->M = p.mHOS;
+    const int Nd1 = ((p.p1 + 1) * p.n1) / 2;
+    const int Nd2 = p.n2 > 1 ? ((p.p2 + 1) * p.n2) / 2 : 1;
+    c->N1b = pad_smooth(std::max(Nd1, 2));
+    c->N2b = p.n2 > 1 ? pad_smooth(Nd2) : 1;
+    c->pix = M_PI / p.xlen;
+    c->piy = p.n2 > 1 ? M_PI / p.ylen : 0.0;
+    c->dx = p.xlen / (p.n1 - 1);
+    c->dy = p.n2 > 1 ? p.ylen / (p.n2 - 1) : 0.0;
+    c->delz = 1.0 / (p.n3 - 1);
+    c->wmk = (p.icase == 2 || p.icase == 3 || p.icase == 31 || p.icase == 32 || p.icase == 33 || p.icase == 4 ||
+              p.icase == 41 || p.icase == 42) &&
+             (p.igeom == 1 || p.igeom == 2);
+    c->stats.big_n1 = c->N1b;
+    c->stats.big_n2 = c->N2b;
+    HIP_OK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    c->gs64.build(p.n1, p.n2);
+    const size_t ns = (size_t)p.n1 * p.n2, na = (size_t)c->n3a * p.n2;
+    Arena& ar = c->ar;
+    c->eta_hat = ar.get<double>(ns); c->phis_hat = ar.get<double>(ns);
+    c->a1st_add = ar.get<double>(na); c->a_add_ramp = ar.get<double>(na);
+    c->a1st = ar.get<double>(ns); c->eta1st = ar.get<double>(ns);
+    c->eta_hat_t = ar.get<double>(ns); c->phis_hat_t = ar.get<double>(ns);
+    c->a1st_add_t = ar.get<double>(na); c->a_add_ramp_t = ar.get<double>(na);
+    c->a1st_t = ar.get<double>(ns); c->eta1st_t = ar.get<double>(ns);
+    c->deta_hat = ar.get<double>(ns); c->deta_hat_t = ar.get<double>(ns);
+    c->G1 = ar.get<double>(ns); c->G2 = ar.get<double>(ns); c->G1m = ar.get<double>(ns); c->G2m = ar.get<double>(ns);
+    c->aadm = ar.get<double>(na); c->a1m = ar.get<double>(ns); c->e1m = ar.get<double>(ns);
+    for (int s = 0; s < 5; ++s) {
+      c->kG1[s] = ar.get<double>(ns); c->kG2[s] = ar.get<double>(ns);
+      c->kad1[s] = ar.get<double>(na); c->kadr[s] = ar.get<double>(na);
+      c->ka1[s] = ar.get<double>(ns); c->ke1[s] = ar.get<double>(ns);
+    }
+    const size_t nbk = nblk(std::max(ns, na)) + 1;
+    c->partial = ar.get<double>(nbk);
+    c->pmax_e = ar.get<double>(nbk);
+    c->pmax_x = ar.get<double>(nbk);
+    c->red6 = ar.get<double>(8);
+    c->phys_tmp = ar.get<double>(ns);
+    c->phi_add_last = ar.get<double>(ns);
+    c->nu_d = nullptr;
+    if (p.nu) {
+      c->nu_d = ar.get<double>(ns);
+      HIP_OK(hipMemcpy(c->nu_d, p.nu, ns * sizeof(double), hipMemcpyHostToDevice));
+    }
+    c->p.nu = nullptr;
+    c->freq.assign(p.freq, p.freq + p.n_freq);
+    c->pre0.resize(p.n_freq);
+    c->pim0.resize(p.n_freq);
+    for (int f = 0; f < p.n_freq; ++f) {
+      c->pre0[f] = p.pos_ini_re[(size_t)p.n2 * f];
+      c->pim0[f] = p.pos_ini_im[(size_t)p.n2 * f];
+    }
+    c->gz_top = gzed(0.0, p.igeom, p.d_hinge);
+    if (p.precision == 0)
+      c->eng = std::make_unique<Engine<double>>(c.get());
+    else
+      c->eng = std::make_unique<Engine<float>>(c.get());
+    c->p.freq = nullptr;
+    c->p.pos_ini_re = c->p.pos_ini_im = nullptr;
+    HIP_OK(hipDeviceSynchronize());
+    *out = c.release();
+  });
+}
+
+int hos_destroy(hos_ctx* c) {
+  return guard([&] {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->st);
+    c->eng.reset();
+    c->gs64.release();
+    hipStream_t st = c->st;
+    delete c;
+    (void)hipStreamDestroy(st);
+  });
+}
+
+int hos_initialize(hos_ctx* c) {
+  return guard([&] {
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    hipStream_t st = c->st;
+    HIP_OK(hipMemsetAsync(c->a1st, 0, ns * sizeof(double), st));
+    HIP_OK(hipMemsetAsync(c->eta1st, 0, ns * sizeof(double), st));
+    HIP_OK(hipMemsetAsync(c->a1st_add, 0, na * sizeof(double), st));
+    HIP_OK(hipMemsetAsync(c->a_add_ramp, 0, na * sizeof(double), st));
+    HIP_OK(hipMemsetAsync(c->deta_hat, 0, ns * sizeof(double), st));
+    c->endramp = 0;
+    if (!c->wmk) {
+      c->last_alpha = 0.0;
+      c->last_pos_top = 0.0;
+      HIP_OK(hipStreamSynchronize(st));
+      return;
+    }
+    // a1st_add from -alpha X_t - alpha' X (initial_condition.f90:372-392)
+    Engine<double>* e64 = dynamic_cast<Engine<double>*>(c->eng.get());
+    Engine<float>* e32 = dynamic_cast<Engine<float>*>(c->eng.get());
+    double al, alt, altt;
+    ramp_eval(c->p, 0.0, al, alt, altt);
+    auto run = [&](auto* e) {
+      hipLaunchKernelGGL(k_pos, dim3(nblk(c->n2, 64)), dim3(64), 0, st, c->n2, c->n3, c->p.n_freq, 0.0,
+                         (const double*)e->freq, (const double*)e->pre, (const double*)e->pim, (const double*)e->gz,
+                         e->Xp, e->Xpt, e->Xptt);
+      hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(c->n2, 64)), dim3(64), 0, st, c->n2, c->n3, c->n3a, c->delz,
+                         (const double*)e->Xp, (const double*)e->Xpt, (const double*)e->Xptt, alt, al, 0.0, 0.0, 0.0,
+                         0.0, e->cont, e->f1, (double*)nullptr);
+      hipLaunchKernelGGL(k_add_z, dim3(nblk(c->n3a, 64), c->n2), dim3(64), 0, st, c->n2, c->n3a,
+                         (const double*)e->czt, (const double*)e->f1, (const double*)nullptr, e->fz1, (double*)nullptr);
+      hipLaunchKernelGGL(k_add_y, dim3(nblk(c->n3a, 64), c->n2), dim3(64), 0, st, c->n2, c->n3a, c->p.xlen, c->piy,
+                         (const double*)e->cyt, (const double*)e->fz1, (const double*)nullptr, c->a1st_add,
+                         (double*)nullptr);
+    };
+    if (e64) run(e64);
+    else run(e32);
+    HIP_OK(hipMemcpyAsync(c->a_add_ramp, c->a1st_add, na * sizeof(double), hipMemcpyDeviceToDevice, st));
+    // one RHS at t = 0 sets phi_add (phiadd_SL with da1st_add) and the wavemaker globals
+    hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
+                       (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
+    rhs_eval(c, 0.0, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0);
+    HIP_OK(hipStreamSynchronize(st));
+  });
+}
+
+int hos_set_state(hos_ctx* c, const double* eta, const double* phis, const double* a1st_add,
+                  const double* a_add_ramp, const double* a1st, const double* eta1st) {
+  return guard([&] {
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    hipStream_t st = c->st;
+    // eta, phis physical -> modal (runge_kutta.f90:128-134)
+    HIP_OK(hipMemcpyAsync(c->phys_tmp, eta, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    r2r2d<double>(c->gs64, KCOS, KCOS, FWD, src_plain<double, double>(c->phys_tmp, c->n1, c->n1, c->n2), c->n1,
+                  c->n2, Dst<double>{c->eta_hat, c->n1, 0}, c->G1m, st);
+    HIP_OK(hipMemcpyAsync(c->phys_tmp, phis, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+    r2r2d<double>(c->gs64, KCOS, KCOS, FWD, src_plain<double, double>(c->phys_tmp, c->n1, c->n1, c->n2), c->n1,
+                  c->n2, Dst<double>{c->phis_hat, c->n1, 0}, c->G1m, st);
+    if (a1st_add) HIP_OK(hipMemcpyAsync(c->a1st_add, a1st_add, na * sizeof(double), hipMemcpyHostToDevice, st));
+    if (a_add_ramp) HIP_OK(hipMemcpyAsync(c->a_add_ramp, a_add_ramp, na * sizeof(double), hipMemcpyHostToDevice, st));
+    if (a1st) HIP_OK(hipMemcpyAsync(c->a1st, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    if (eta1st) HIP_OK(hipMemcpyAsync(c->eta1st, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+    c->have_trial = false;
+  });
+}
+
+static void to_phys(hos_ctx* c, const double* modal, double* host) {
+  const size_t ns = (size_t)c->n1 * c->n2;
+  r2r2d<double>(c->gs64, KCOS, KCOS, INV, src_plain<double, double>(modal, c->n1, c->n1, c->n2), c->n1, c->n2,
+                Dst<double>{c->phys_tmp, c->n1, 0}, c->G1m, c->st);
+  HIP_OK(hipMemcpyAsync(host, c->phys_tmp, ns * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIP_OK(hipStreamSynchronize(c->st));
+}
+
+int hos_get_state(hos_ctx* c, double* eta, double* phis, double* a1st_add, double* a_add_ramp, double* a1st,
+                  double* eta1st) {
+  return guard([&] {
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    if (eta) to_phys(c, c->eta_hat, eta);
+    if (phis) to_phys(c, c->phis_hat, phis);
+    if (a1st_add) HIP_OK(hipMemcpy(a1st_add, c->a1st_add, na * sizeof(double), hipMemcpyDeviceToHost));
+    if (a_add_ramp) HIP_OK(hipMemcpy(a_add_ramp, c->a_add_ramp, na * sizeof(double), hipMemcpyDeviceToHost));
+    if (a1st) HIP_OK(hipMemcpy(a1st, c->a1st, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (eta1st) HIP_OK(hipMemcpy(eta1st, c->eta1st, ns * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int hos_get_physical(hos_ctx* c, double* eta, double* phis, double* deta, double* phi_add) {
+  return guard([&] {
+    const size_t ns = (size_t)c->n1 * c->n2;
+    if (eta) to_phys(c, c->eta_hat, eta);
+    if (phis) to_phys(c, c->phis_hat, phis);
+    if (deta) to_phys(c, c->deta_hat, deta);
+    if (phi_add) {
+      HIP_OK(hipMemcpyAsync(phi_add, c->phi_add_last, ns * sizeof(double), hipMemcpyDeviceToHost, c->st));
+      HIP_OK(hipStreamSynchronize(c->st));
+    }
+  });
+}
+
+int hos_rk_step(hos_ctx* c, double t, double h, double* err) {
+  return guard([&] { rk_attempt(c, t, h, err); });
+}
+
+int hos_commit(hos_ctx* c) {
+  return guard([&] { commit(c); });
+}
+
+int hos_end_ramp(hos_ctx* c) {
+  return guard([&] {
+    const size_t na = (size_t)c->n3a * c->n2;
+    hipLaunchKernelGGL(k_sub, dim3(nblk(na)), dim3(256), 0, c->st, na, c->a1st_add, (const double*)c->a_add_ramp);
+    HIP_OK(hipStreamSynchronize(c->st));
+  });
+}
+
+int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
+                double delt_out, int* n_acc, int* n_rej) {
+  return guard([&] {
+    const double tiny = 2.220446049250313e-16;
+    double t = *t_cur, h = *h_rk;
+    int na = 0, nr = 0;
+    const hos_params& p = c->p;
+    while (t < t_next) {
+      double hl = h;
+      if (t_next - t < h) hl = t_next - t;
+      double err = 0.0;
+      rk_attempt(c, t, hl, &err);
+      double corr;
+      if (err > toler && h > 0.001 * 0.95 * dt_stab) {
+        corr = 0.9 * std::pow(err / toler, -(1.0 / 4.0));
+        ++nr;
+        if (std::fabs(p.Tramp - (t + hl)) <= tiny && p.Tramp > t) c->endramp = 0;
+        c->have_trial = false;
+      } else {
+        commit(c);
+        if (p.iramp == 1 && p.Tramp - (t + hl) <= tiny && p.Tramp > t && c->endramp != 1) {
+          const size_t n = (size_t)c->n3a * c->n2;
+          hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, c->st, n, c->a1st_add, (const double*)c->a_add_ramp);
+          c->endramp = 1;
+        }
+        t = t + hl;
+        corr = (std::fabs(err) < tiny) ? 2.0 : 0.9 * std::pow(err / toler, -(1.0 / 4.0));
+        ++na;
+      }
+      h = h * corr;
+      h = std::min({h, 0.95 * dt_stab, delt_out});
+      h = std::max({h, 0.0001 * 0.95 * dt_stab, 0.0001 * delt_out});
+    }
+    HIP_OK(hipStreamSynchronize(c->st));
+    *t_cur = t;
+    *h_rk = h;
+    if (n_acc) *n_acc = na;
+    if (n_rej) *n_rej = nr;
+  });
+}
+
+int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
+            const double* eta1st, const double* a_add_1, double* dG1, double* dG2, double* da1st, double* deta1st,
+            double* da_add_1, double* da_add_ramp) {
+  return guard([&] {
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    hipStream_t st = c->st;
+    HIP_OK(hipMemcpyAsync(c->G1m, G1, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->G2m, G2, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->a1m, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->e1m, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->aadm, a_add_1, na * sizeof(double), hipMemcpyHostToDevice, st));
+    rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0);
+    HIP_OK(hipMemcpyAsync(dG1, c->kG1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(dG2, c->kG2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da1st) HIP_OK(hipMemcpyAsync(da1st, c->ka1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (deta1st) HIP_OK(hipMemcpyAsync(deta1st, c->ke1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da_add_1) HIP_OK(hipMemcpyAsync(da_add_1, c->kad1[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da_add_ramp) HIP_OK(hipMemcpyAsync(da_add_ramp, c->kadr[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+  });
+}
+
+int hos_get_slope_max(hos_ctx* c, double* sx, double* sy) {
+  return guard([&] { c->eng->slope_max(*sx, *sy); });
+}
+
+int hos_get_stats(hos_ctx* c, hos_stats* s) {
+  return guard([&] { *s = c->stats; });
+}
+
+int hos_synchronize(hos_ctx* c) {
+  return guard([&] { HIP_OK(hipStreamSynchronize(c->st)); });
+}
+
+static int standalone(int n1, int n2, int kx, int ky, int prec, int dir, const double* in, double* out) {
+  return guard([&] {
+    if (n1 < 2 || n2 < 1) throw std::runtime_error("bad grid");
+    const size_t n = (size_t)n1 * n2;
+    hipStream_t st;
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    Arena ar;
+    double* din = ar.get<double>(n);
+    double* dout = ar.get<double>(n);
+    HIP_OK(hipMemcpy(din, in, n * sizeof(double), hipMemcpyHostToDevice));
+    if (prec == 0) {
+      Grid2<double> g;
+      g.build(n1, n2);
+      double* tmp = ar.get<double>(n);
+      r2r2d<double>(g, kx, ky, dir, src_plain<double, double>(din, n1, n1, n2), n1, n2, Dst<double>{dout, n1, 0},
+                    tmp, st);
+      HIP_OK(hipStreamSynchronize(st));
+      g.release();
+    } else {
+      Grid2<float> g;
+      g.build(n1, n2);
+      float* tmp = ar.get<float>(n);
+      float* fo = ar.get<float>(n);
+      r2r2d<float>(g, kx, ky, dir, src_plain<float, double>(din, n1, n1, n2), n1, n2, Dst<float>{fo, n1, 0}, tmp,
+                   st);
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n)), dim3(256), 0, st, n, (const float*)fo, dout);
+      HIP_OK(hipStreamSynchronize(st));
+      g.release();
+    }
+    HIP_OK(hipMemcpy(out, dout, n * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipStreamDestroy(st);
+  });
+}
+
+int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out) {
+  return standalone(n1, n2, kind_x, kind_y, precision, FWD, in, out);
+}
+
+int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out) {
+  return standalone(n1, n2, kind_x, kind_y, precision, INV, in, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,412 @@
+// Batched DCT-I / DST-I transforms on column-major 2-D grids (x fastest).
+//
+// A 2-D transform is two line passes.  The x pass reads whole contiguous
+// x-lines; the y pass reads tiles of TX adjacent columns so every row segment
+// is a coalesced TX-wide load.  Two real lines are packed as the real and
+// imaginary parts of one complex line, extended evenly (cos) or oddly (sin) to
+// length 2(N-1), and transformed by one LDS FFT; the even/odd symmetry makes
+// both halves recoverable from the real and imaginary parts of the result.
+//
+// Normalisations follow fourier_FFTW.f90:178-229:
+//   forward (space -> Fourier): out_k * s2f_k, s2f = 1/(2(N-1)) at the ends,
+//                               1/(N-1) inside;
+//   inverse (Fourier -> space): in_j * f2s_j, f2s = 1 at the ends, 1/2 inside,
+// so that the inverse evaluates sum_k a_k cos(k pi x / L) at the nodes.
+// Sine transforms use interior points only with the end values forced to
+// zero (fourier_FFTW.f90:284-299).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fft_lds.h"
+
+namespace hosnwt {
+
+using hosfft::C2;
+using hosfft::FftPlan;
+
+#define HIP_OK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +   \
+                               __FILE__ + ":" + std::to_string(__LINE__) + " (" #expr ")");  \
+  } while (0)
+
+enum Kind { KCOS = 0, KSIN = 1 };
+enum Dir { FWD = 1, INV = -1 };
+enum Mul { MUL_NONE = 0, MUL_NEG_KX = 1, MUL_NEG_KY = 2, MUL_KTH = 3 };
+
+// ---------------------------------------------------------------------------
+// source / destination functors
+// ---------------------------------------------------------------------------
+// Real field on a column-major grid, zero outside [0,nx) x [0,ny); optional
+// spectral multiplier evaluated from the mode indices (kx = ix*pix, ky = iy*piy).
+// U is the stored type, T the transform type.
+template <typename T, typename U = T>
+struct Src {
+  const U* p;
+  int ld, nx, ny;
+  int mul, jpow;
+  double pix, piy;
+  __device__ __forceinline__ T get(int ix, int iy) const {
+    if (ix >= nx || iy >= ny) return (T)0;
+    const U v = p[ix + (size_t)ld * iy];
+    if (mul == MUL_NONE) return (T)v;
+    double m;
+    if (mul == MUL_NEG_KX) {
+      m = -(double)ix * pix;
+    } else if (mul == MUL_NEG_KY) {
+      m = -(double)iy * piy;
+    } else {
+      // kth: k^j tanh(k) for odd j, k^j for even j (HOS-NWT.f90:311-327)
+      const double kx = (double)ix * pix, ky = (double)iy * piy;
+      const double k = sqrt(kx * kx + ky * ky);
+      double kj = k;
+      for (int q = 1; q < jpow; ++q) kj *= k;
+      if (jpow & 1) kj *= (k < 20.0 ? tanh(k) : 1.0);
+      m = kj;
+    }
+    return (T)(m * (double)v);
+  }
+};
+
+template <typename T, typename U = T>
+inline Src<T, U> src_plain(const U* p, int ld, int nx, int ny) {
+  return Src<T, U>{p, ld, nx, ny, MUL_NONE, 0, 0.0, 0.0};
+}
+template <typename T, typename U = T>
+inline Src<T, U> src_mul(const U* p, int ld, int nx, int ny, int mul, int jpow, double pix, double piy) {
+  return Src<T, U>{p, ld, nx, ny, mul, jpow, pix, piy};
+}
+
+// Plain store: p[ix + ld*iy] = v (mode 0) or += v (mode 1).
+template <typename T>
+struct Dst {
+  T* p;
+  int ld, mode;
+  __device__ __forceinline__ void put(int ix, int iy, T v) const {
+    const size_t i = ix + (size_t)ld * iy;
+    if (mode == 0)
+      p[i] = v;
+    else
+      p[i] += v;
+  }
+};
+
+// HOS recursion epilogue (resol_HOS.f90:737-769): with phizi = v,
+//   phimext -= phizi * eta^j/j!   and   phizm(iHOS) += phizi * eta^(j-1)/(j-1)!
+// (`first` overwrites instead of accumulating; ejm1 == nullptr means factor 1).
+template <typename T>
+struct DstHos {
+  T* phim;
+  T* phizm;
+  const T* ej;
+  const T* ejm1;
+  int ld, first, do_phim;
+  __device__ __forceinline__ void put(int ix, int iy, T v) const {
+    const size_t i = ix + (size_t)ld * iy;
+    if (do_phim) {
+      const T t = v * ej[i];
+      phim[i] = first ? -t : phim[i] - t;
+    }
+    const T u = ejm1 ? v * ejm1[i] : v;
+    phizm[i] = first ? u : phizm[i] + u;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// line kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T f2s_fac(int j, int N) { return (j == 0 || j == N - 1) ? (T)1 : (T)0.5; }
+
+// x pass: workgroup `pair` transforms lines 2*pair and 2*pair+1 (< nlines).
+template <typename T, class S, class D>
+__global__ void r2r_x_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines, S src, D dst,
+                             FftPlan<T> pl) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int la = 2 * blockIdx.x, lb = la + 1;
+  const bool hb = lb < nlines;
+  const int L = 2 * (N - 1);
+  for (int j = tid; j < N; j += nt) {
+    T a = (T)0, b = (T)0;
+    if (j < n_in) {
+      a = src.get(j, la);
+      if (hb) b = src.get(j, lb);
+    }
+    if (dir == INV) {
+      const T f = f2s_fac<T>(j, N);
+      a *= f;
+      b *= f;
+    }
+    C2<T> z = hosfft::mk<T>(a, b);
+    if (kind == KSIN && (j == 0 || j == N - 1)) z = hosfft::mk<T>((T)0, (T)0);
+    buf[j] = z;
+    if (j >= 1 && j <= N - 2) buf[L - j] = (kind == KCOS) ? z : hosfft::mk<T>(-z.x, -z.y);
+  }
+  __syncthreads();
+  hosfft::fft_lines<T>(buf, 1, pl.L, pl, tid, nt);
+  const T inv = (T)(1.0 / (double)L);
+  for (int k = tid; k < n_out; k += nt) {
+    const C2<T> Z = buf[k];
+    T A, B;
+    if (kind == KCOS) {
+      A = Z.x;
+      B = Z.y;
+    } else {
+      A = -Z.y;
+      B = Z.x;
+      if (k == 0 || k == N - 1) A = B = (T)0;
+    }
+    if (dir == FWD) {
+      const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
+      A *= f;
+      B *= f;
+    }
+    dst.put(k, la, A);
+    if (hb) dst.put(k, lb, B);
+  }
+}
+
+// y pass: workgroup handles columns [c0, c0+TX) (< ncols), packed in pairs.
+template <typename T, class S, class D>
+__global__ void r2r_y_kernel(int N, int kind, int dir, int n_in, int n_out, int ncols, int TX, S src, D dst,
+                             FftPlan<T> pl) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
+  T* bufr = reinterpret_cast<T*>(smem_raw);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int c0 = blockIdx.x * TX;
+  const int npairs = TX / 2;
+  const int ls = pl.L;
+  const int L = 2 * (N - 1);
+  for (int idx = tid; idx < TX * N; idx += nt) {
+    const int c = idx % TX, j = idx / TX;
+    const int col = c0 + c;
+    T v = (T)0;
+    if (j < n_in && col < ncols) v = src.get(col, j);
+    if (dir == INV) v *= f2s_fac<T>(j, N);
+    if (kind == KSIN && (j == 0 || j == N - 1)) v = (T)0;
+    bufr[2 * ((c >> 1) * ls + j) + (c & 1)] = v;
+  }
+  __syncthreads();
+  for (int idx = tid; N > 2 && idx < npairs * (N - 2); idx += nt) {
+    const int q = idx / (N - 2), j = 1 + idx % (N - 2);
+    const C2<T> z = buf[q * ls + j];
+    buf[q * ls + L - j] = (kind == KCOS) ? z : hosfft::mk<T>(-z.x, -z.y);
+  }
+  __syncthreads();
+  hosfft::fft_lines<T>(buf, npairs, ls, pl, tid, nt);
+  const T inv = (T)(1.0 / (double)L);
+  for (int idx = tid; idx < TX * n_out; idx += nt) {
+    const int c = idx % TX, k = idx / TX;
+    const int col = c0 + c;
+    if (col >= ncols) continue;
+    const C2<T> Z = buf[(c >> 1) * ls + k];
+    T v;
+    if (kind == KCOS) {
+      v = (c & 1) ? Z.y : Z.x;
+    } else {
+      v = (c & 1) ? Z.x : -Z.y;
+      if (k == 0 || k == N - 1) v = (T)0;
+    }
+    if (dir == FWD) v *= (k == 0 || k == N - 1) ? inv : (T)2 * inv;
+    dst.put(col, k, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side plans
+// ---------------------------------------------------------------------------
+inline bool smooth235(long L) {
+  for (int r : {2, 3, 5})
+    while (L % r == 0) L /= r;
+  return L == 1;
+}
+
+inline void host_fft_pow2(std::vector<std::complex<double>>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len) {
+      for (size_t k = 0; k < len / 2; ++k) {
+        const double ang = -2.0 * M_PI * (double)k / (double)len;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const std::complex<double> u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+// Device tables for one logical line length (complex FFT length Lorig).
+template <typename T>
+struct LinePlan {
+  FftPlan<T> dev{};
+  std::vector<void*> allocs;
+  void build(int Lorig) {
+    dev.Lorig = Lorig;
+    long L = Lorig;
+    dev.bluestein = smooth235(L) ? 0 : 1;
+    if (dev.bluestein) {
+      L = 1;
+      while (L < 2L * Lorig - 1) L <<= 1;
+    }
+    dev.L = (int)L;
+    // radix sequence: 8s first, then 4/2, 3, 5
+    long r = L;
+    int np = 0;
+    while (r % 8 == 0) { dev.radix[np++] = 8; r /= 8; }
+    while (r % 4 == 0) { dev.radix[np++] = 4; r /= 4; }
+    while (r % 2 == 0) { dev.radix[np++] = 2; r /= 2; }
+    while (r % 3 == 0) { dev.radix[np++] = 3; r /= 3; }
+    while (r % 5 == 0) { dev.radix[np++] = 5; r /= 5; }
+    if (r != 1 || np > hosfft::kMaxPass) throw std::runtime_error("unsupported FFT length");
+    dev.npass = np;
+    std::vector<C2<T>> tw(L);
+    for (long m = 0; m < L; ++m) {
+      const double a = -2.0 * M_PI * (double)m / (double)L;
+      tw[m].x = (T)std::cos(a);
+      tw[m].y = (T)std::sin(a);
+    }
+    dev.tw = upload(tw);
+    if (dev.bluestein) {
+      std::vector<std::complex<double>> c(Lorig), b(L, 0.0);
+      for (long j = 0; j < Lorig; ++j) {
+        const long long q = ((long long)j * j) % (2LL * Lorig);
+        const double a = -M_PI * (double)q / (double)Lorig;
+        c[j] = std::complex<double>(std::cos(a), std::sin(a));
+      }
+      for (long m = 0; m < Lorig; ++m) {
+        b[m] = std::conj(c[m]);
+        if (m > 0) b[L - m] = std::conj(c[m]);
+      }
+      host_fft_pow2(b);
+      std::vector<C2<T>> ct(Lorig), bt(L);
+      for (long j = 0; j < Lorig; ++j) {
+        ct[j].x = (T)c[j].real();
+        ct[j].y = (T)c[j].imag();
+      }
+      for (long j = 0; j < L; ++j) {
+        bt[j].x = (T)(b[j].real() / (double)L);
+        bt[j].y = (T)(b[j].imag() / (double)L);
+      }
+      dev.chirp = upload(ct);
+      dev.bhat = upload(bt);
+    }
+  }
+  const C2<T>* upload(const std::vector<C2<T>>& v) {
+    void* d = nullptr;
+    HIP_OK(hipMalloc(&d, v.size() * sizeof(C2<T>)));
+    HIP_OK(hipMemcpy(d, v.data(), v.size() * sizeof(C2<T>), hipMemcpyHostToDevice));
+    allocs.push_back(d);
+    return reinterpret_cast<const C2<T>*>(d);
+  }
+  void release() {
+    for (void* p : allocs) (void)hipFree(p);
+    allocs.clear();
+  }
+};
+
+// Transforms on one N1 x N2 grid (N2 == 1: 2-D tank, x lines only).
+template <typename T>
+struct Grid2 {
+  int N1 = 0, N2 = 1;
+  int TX = 8;
+  LinePlan<T> px, py;
+  void build(int n1, int n2) {
+    N1 = n1;
+    N2 = n2;
+    px.build(2 * (N1 - 1));
+    if (N2 > 1) py.build(2 * (N2 - 1));
+    TX = sizeof(T) == 8 ? 8 : 16;
+    // keep the y tile within the LDS budget
+    while (TX > 2 && (size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 && N2 > 1) TX /= 2;
+  }
+  void release() {
+    px.release();
+    py.release();
+  }
+};
+
+inline int pick_threads(long vals) {
+  long nt = (vals + hosfft::kValsPerThread - 1) / hosfft::kValsPerThread;
+  nt = ((nt + 63) / 64) * 64;
+  if (nt < 64) nt = 64;
+  if (nt > 1024) throw std::runtime_error("line batch too large for one workgroup");
+  return (int)nt;
+}
+
+constexpr size_t kLdsMax = 160 * 1024;
+
+template <typename K>
+inline void allow_lds(K kernel, size_t bytes) {
+  static std::map<const void*, size_t> granted;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  if (bytes > 64 * 1024 && bytes > granted[key]) {
+    HIP_OK(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    granted[key] = bytes;
+  }
+}
+
+template <typename T, class S, class D>
+void launch_x(const LinePlan<T>& pl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
+              const D& d, hipStream_t st) {
+  if (nlines <= 0 || n_out <= 0) return;
+  const size_t lds = (size_t)pl.dev.L * sizeof(C2<T>);
+  if (lds > kLdsMax) throw std::runtime_error("x line exceeds LDS: N=" + std::to_string(N));
+  const int nt = pick_threads(pl.dev.L);
+  auto kern = r2r_x_kernel<T, S, D>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3((nlines + 1) / 2), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, s, d,
+                     pl.dev);
+  HIP_OK(hipGetLastError());
+}
+
+template <typename T, class S, class D>
+void launch_y(const LinePlan<T>& pl, int TX, int N, int kind, int dir, int n_in, int n_out, int ncols, const S& s,
+              const D& d, hipStream_t st) {
+  if (ncols <= 0 || n_out <= 0) return;
+  const size_t lds = (size_t)(TX / 2) * pl.dev.L * sizeof(C2<T>);
+  if (lds > kLdsMax) throw std::runtime_error("y tile exceeds LDS: N=" + std::to_string(N));
+  const int nt = pick_threads((long)(TX / 2) * pl.dev.L);
+  auto kern = r2r_y_kernel<T, S, D>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3((ncols + TX - 1) / TX), dim3(nt), lds, st, N, kind, dir, n_in, n_out, ncols, TX, s,
+                     d, pl.dev);
+  HIP_OK(hipGetLastError());
+}
+
+// 2-D r2r: src (valid extents src.nx, src.ny) -> dst with output extents
+// nout1 x nout2.  tmp must hold nout1 * src.ny values.
+template <typename T, class S, class D>
+void r2r2d(const Grid2<T>& g, int kx, int ky, int dir, const S& src, int nout1, int nout2, const D& dst, T* tmp,
+           hipStream_t st) {
+  if (g.N2 == 1) {
+    launch_x<T>(g.px, g.N1, kx, dir, src.nx, nout1, 1, src, dst, st);
+    return;
+  }
+  const int nl = src.ny < g.N2 ? src.ny : g.N2;
+  Dst<T> dt{tmp, nout1, 0};
+  launch_x<T>(g.px, g.N1, kx, dir, src.nx, nout1, nl, src, dt, st);
+  Src<T, T> s2 = src_plain<T, T>(tmp, nout1, nout1, nl);
+  launch_y<T>(g.py, g.TX, g.N2, ky, dir, nl, nout2, nout1, s2, dst, st);
+}
+
+}  // namespace hosnwt

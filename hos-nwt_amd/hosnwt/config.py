@@ -1,0 +1,275 @@
+"""Driver setup on the host: input parameters, wave components, paddle motion
+amplitudes, numerical beach and the RK4 stability bound.
+
+Restates the reference's setup (HOS-NWT.f90:95-255, initial_condition.f90:128-310,
+wavemaking.f90:37-175, wave/wave_def.f90:54-329, wave/linear_wave.f90:119-334,
+initial_condition.f90:724-794) so that the GPU context receives the same
+pos_ini / freq / nu arrays the reference builds.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+G = 9.81
+TWOPI = 2.0 * math.pi
+L_ADD = 2
+
+
+def wave_number(f: float, prec: float = 1e-15) -> float:
+    """Nondimensional k with (2 pi f)^2 = k tanh k (linear_wave.f90:137-151).
+
+    Bisection on [0, 1e6] to relative accuracy ``prec`` like the reference's
+    dichoto (linear_wave.f90:427-466); returns 0 for f = 0.
+    """
+    w2 = (TWOPI * f) ** 2
+    if w2 == 0.0:
+        return 0.0
+    lo, hi = 0.0, 1.0e6
+    x3 = hi
+    while abs(lo - hi) > prec * abs(hi):
+        x3 = 0.5 * (lo + hi)
+        v = x3 * math.tanh(x3) - w2
+        if v == 0.0:
+            return x3
+        if (0.0 - w2) * v < 0.0:
+            hi = x3
+        else:
+            lo = x3
+    return x3
+
+
+def transfer_function(typ: str, k: float, hinge: float = 0.0) -> complex:
+    """Paddle transfer function b = TF a (linear_wave.f90:277-334)."""
+    if typ == "piston":
+        if k < 50.0:
+            return complex(0.0, -(2.0 * k + math.sinh(2.0 * k)) / (4.0 * math.sinh(k) * math.sinh(k)))
+        return complex(0.0, -0.5)
+    if typ == "hinged":
+        d = hinge
+        if k < 50 and k * d < 50:
+            num = -1j * k * (1.0 - d) * (2.0 * k + math.sinh(2.0 * k)) / (4.0 * math.sinh(k))
+            return num / (k * (1.0 - d) * math.sinh(k) + math.cosh(k * d) - math.cosh(k))
+        if k >= 50 and k * d < 50:
+            return -1j * k * (1.0 - d) / (2.0 * k * (1.0 - d) - 2.0 + 4.0 * math.cosh(k * d) * math.exp(-k))
+        if k * (1.0 - d) >= 20:
+            return -0.5j
+        return -1j * k * (1.0 - d) / (2.0 * k * (1.0 - d) - 2.0 + 2.0 * math.exp(k * (d - 1.0)))
+    return 1.0 + 0j
+
+
+@dataclass
+class WaveComponents:
+    """Nondimensional wave components (the reference's TYPE(wave), wave_def.f90)."""
+
+    harmo: np.ndarray
+    freq: np.ndarray
+    ampli: np.ndarray
+    angle: np.ndarray  # radians
+    phase: np.ndarray
+    k: np.ndarray = field(default=None)
+    TF: np.ndarray = field(default=None)
+    rnum: int = 1
+    clock: float = 1.0
+
+    def init(self, typ: str, hinge: float):
+        """wave_init (wave_def.f90:282-329)."""
+        n = len(self.freq)
+        self.k = np.zeros(n)
+        self.TF = np.ones(n, complex)
+        for i in range(n):
+            kk = wave_number(self.freq[i])
+            self.k[i] = kk
+            self.TF[i] = transfer_function(typ, kk, hinge) if abs(kk) > np.finfo(float).eps else 1.0
+            if self.freq[i] < 1e-8:
+                self.k[i] = 0.0
+                self.TF[i] = 1.0
+        return self
+
+
+@dataclass
+class TankConfig:
+    """Dimensional inputs in the order of input_HOS-NWT.dat (input.f90:62-133)."""
+
+    n1: int
+    n2: int
+    mHOS: int
+    xlen: float
+    ylen: float = 1.0
+    h: float = 1.0
+    n3: int = 33
+    icase: int = 2
+    islosh: int = 1
+    aslosh: float = 0.0
+    amp_mono: float = 0.0
+    nu_mono: float = 1.0
+    theta_mono: float = 0.0
+    ph_mono: float = 0.0
+    ibat: int = 2
+    i_wmk: int = 1
+    igeom: int = 1
+    d_hinge: float = 0.0
+    iramp: int = 1
+    Tramp: float = 0.0
+    iabsnb: int = 1
+    xabsf: float = 0.8
+    coeffabsf: float = 1.0
+    T_stop: float = 10.0
+    toler: float = 1e-7
+    f_out: float = 1.0
+    precision: int = 0
+    # icase = 3: components in SI (freq Hz, ampli m, angle deg, phase rad)
+    components: dict | None = None
+
+
+@dataclass
+class Setup:
+    """Nondimensional run description shared by the GPU context and the oracle."""
+
+    cfg: TankConfig
+    xlen: float
+    ylen: float
+    Tscale: float
+    Tramp: float
+    T_stop: float
+    T_mono: float
+    f_out: float
+    freq: np.ndarray
+    pos_ini: np.ndarray  # (n2, n_freq) complex
+    nu: np.ndarray  # (n1, n2)
+    dt_stab: float
+    delt_out: float
+    waves: WaveComponents | None
+
+
+def gzed(z: float, igeom: int, d: float) -> float:
+    """Paddle shape (wavemaking.f90:184-215)."""
+    delta = min(d, (1.0 - d) / 10.0)
+    if igeom == 1:
+        return 1.0
+    if z >= d - 1.0 + delta:
+        return (z + 1.0 - d) / (1.0 - d)
+    if z <= d - 1.0 - delta:
+        return 0.0
+    u = (z - (d - 1.0)) / delta
+    return delta / (2.0 * (1.0 - d)) * (5.0 / 16.0 + u + 15.0 / 16.0 * (u ** 2 - u ** 4 / 3.0 + u ** 6 / 15.0))
+
+
+def build(cfg: TankConfig) -> Setup:
+    """Nondimensionalise and build the wavemaker and beach inputs."""
+    h = cfg.h
+    Tscale = 1.0 / math.sqrt(G / h)
+    xlen = cfg.xlen / h
+    ylen = cfg.ylen / h if cfg.n2 != 1 else 1.0
+    Tramp = (cfg.Tramp if cfg.iramp != 0 else 0.0) / Tscale
+    T_stop = cfg.T_stop / Tscale
+    f_out = cfg.f_out * Tscale
+    nu_mono = cfg.nu_mono * Tscale
+    T_mono = 1.0 / nu_mono
+    typ = "piston" if cfg.igeom == 1 else "hinged"
+    d_hinge = cfg.d_hinge / h
+    waves = None
+    if cfg.icase == 2:
+        # front(): harmo 1 on clock 2 nu, rnum 1 (initial_condition.f90:148-166)
+        theta = 0.0 if cfg.n2 == 1 else cfg.theta_mono * math.pi / 180.0
+        waves = WaveComponents(harmo=np.array([1]), freq=np.array([nu_mono]),
+                               ampli=np.array([cfg.amp_mono / h]), angle=np.array([theta]),
+                               phase=np.array([cfg.ph_mono]), rnum=1, clock=2.0 * nu_mono).init(typ, d_hinge)
+    elif cfg.icase == 3:
+        c = cfg.components
+        waves = WaveComponents(harmo=np.asarray(c["harmo"], int), freq=np.asarray(c["freq"], float) * Tscale,
+                               ampli=np.asarray(c["ampli"], float) / h,
+                               angle=np.asarray(c["angle"], float) * math.pi / 180.0,
+                               phase=np.asarray(c["phase"], float), rnum=int(c["rnum"]),
+                               clock=float(c["clock"]) * Tscale).init(typ, d_hinge)
+    n1, n2 = cfg.n1, cfg.n2
+    dely = ylen / (n2 - 1) if n2 != 1 else 0.0
+    cy = np.arange(n2) * dely
+    if waves is not None:
+        freq, pos_ini = wmk_ini(waves, cy)
+    else:
+        freq, pos_ini = np.zeros(0), np.zeros((n2, 0), complex)
+    nu = absorb(cfg, xlen)
+    stab = dt_stab(n1, n2, cfg.n3, xlen, ylen, cfg.icase)
+    return Setup(cfg=cfg, xlen=xlen, ylen=ylen, Tscale=Tscale, Tramp=Tramp, T_stop=T_stop, T_mono=T_mono,
+                 f_out=f_out, freq=freq, pos_ini=pos_ini, nu=nu, dt_stab=stab, delt_out=1.0 / f_out,
+                 waves=waves)
+
+
+def wmk_ini(w: WaveComponents, cy: np.ndarray):
+    """Per-frequency paddle amplitudes pos_ini(y, f) (wavemaking.f90:104-155), full-width paddle."""
+    harms = sorted(set(int(x) for x in w.harmo))
+    freq = []
+    cols = []
+    for hm in harms:
+        sel = np.nonzero(w.harmo == hm)[0]
+        freq.append(w.freq[sel[0]])
+        col = np.zeros(len(cy), complex)
+        for i in sel:
+            col += (w.TF[i] * math.cos(w.angle[i]) * w.ampli[i]
+                    * np.exp(1j * (-w.k[i] * math.sin(w.angle[i]) * cy + w.phase[i])))
+        cols.append(col)
+    return np.array(freq), (np.array(cols).T if cols else np.zeros((len(cy), 0), complex))
+
+
+def absorb(cfg: TankConfig, xlen: float) -> np.ndarray:
+    """Parabolic front beach nu(x) (initial_condition.f90:724-794; no back or side beach)."""
+    nu = np.zeros((cfg.n1, cfg.n2))
+    if cfg.iabsnb == 0:
+        return nu
+    x0 = xlen * cfg.xabsf
+    if abs(cfg.xabsf - 1.0) > np.finfo(float).eps:
+        x2 = xlen - x0
+        cx = np.arange(cfg.n1) * xlen / (cfg.n1 - 1)
+        sel = cx >= x0
+        x3 = (cx[sel] - x0) / x2
+        nu[sel, :] = (cfg.coeffabsf * x3 * x3)[:, None]
+    return nu
+
+
+def dt_stab(n1, n2, n3, xlen, ylen, icase) -> float:
+    """RK4 stability bound (HOS-NWT.f90:132-176)."""
+    c = math.sqrt(8.0 / math.pi ** 2)
+    d1 = c * xlen / (n1 - 1)
+    d2 = c * ylen / (n2 - 1) if n2 != 1 else d1 + 1.0
+    d3 = c * (2.0 * L_ADD) / (L_ADD * (n3 - 1))
+    if icase == 1:
+        return min(d1, d2)
+    if n2 == 1:
+        return min(d1, d3)
+    return min(d1, d2, d3)
+
+
+def jonswap_components(Hs: float, Tp: float, gamma: float, n_freq: int, seed: int, rnum: int = 10,
+                       clock: float = 32.0, n_dir: int = 1, spread_s: float = 10.0):
+    """Seeded JONSWAP components on the clock/2^rnum harmonic grid (SI units).
+
+    Spectrum as in initial_condition.f90:255-283 (sigma 0.07/0.09, Hs rescaled);
+    n_freq harmonics starting at the one nearest 0.5/Tp, phases U[0, 2 pi) from
+    numpy's PCG64 with ``seed``.  n_dir > 1 spreads each frequency over n_dir
+    directions with cos^(2s) weights on [-pi/2, pi/2].
+    """
+    rng = np.random.default_rng(seed)
+    f_base = clock / 2.0 ** rnum
+    h0 = max(1, int(round(0.5 / Tp / f_base)))
+    harmo = np.arange(h0, h0 + n_freq)
+    f = harmo * f_base
+    sigma = np.where(f < 1.0 / Tp, 0.07, 0.09)
+    S = (1.0 / TWOPI ** 4 * G ** 2 / f ** 5 * np.exp(-1.25 * (f * Tp) ** -4)
+         * gamma ** np.exp(-((f - 1.0 / Tp) ** 2) / (2.0 * (sigma / Tp) ** 2)) * f_base)
+    Hs_calc = 4.0 * math.sqrt(S.sum())
+    amp = np.sqrt(2.0 * S) * Hs / Hs_calc
+    if n_dir == 1:
+        return dict(harmo=harmo, freq=f, ampli=amp, angle=np.zeros(n_freq),
+                    phase=rng.uniform(0.0, TWOPI, n_freq), rnum=rnum, clock=clock)
+    th = (np.arange(n_dir) + 0.5) / n_dir * math.pi - math.pi / 2.0
+    wgt = np.cos(th) ** (2 * spread_s)
+    wgt /= wgt.sum()
+    H = np.repeat(harmo, n_dir)
+    F = np.repeat(f, n_dir)
+    A = (amp[:, None] * np.sqrt(wgt)[None, :]).ravel()
+    TH = np.tile(th * 180.0 / math.pi, n_freq)
+    PH = rng.uniform(0.0, TWOPI, n_freq * n_dir)
+    return dict(harmo=H, freq=F, ampli=A, angle=TH, phase=PH, rnum=rnum, clock=clock)

@@ -1,0 +1,112 @@
+"""ctypes binding of libhosnwt.so (the C-ABI declared in include/hosnwt.h).
+
+The library is built in-tree by ``hos-nwt_amd/build.sh`` (or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhosnwt.so")
+
+_dp = C.POINTER(C.c_double)
+
+
+class HosParams(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int), ("n2", C.c_int), ("n3", C.c_int), ("mHOS", C.c_int),
+        ("p1", C.c_int), ("p2", C.c_int), ("icase", C.c_int), ("igeom", C.c_int),
+        ("i_wmk", C.c_int), ("iramp", C.c_int), ("iabsnb", C.c_int), ("precision", C.c_int),
+        ("xlen", C.c_double), ("ylen", C.c_double), ("d_hinge", C.c_double),
+        ("Tramp", C.c_double), ("T_stop", C.c_double), ("T_mono", C.c_double),
+        ("nu", _dp), ("n_freq", C.c_int), ("freq", _dp),
+        ("pos_ini_re", _dp), ("pos_ini_im", _dp),
+    ]
+
+
+class HosStats(C.Structure):
+    _fields_ = [
+        ("n_rhs", C.c_longlong), ("n_big", C.c_longlong), ("n_base", C.c_longlong),
+        ("big_bytes", C.c_longlong), ("big_n1", C.c_int), ("big_n2", C.c_int),
+    ]
+
+
+EXPORTS = [
+    "hos_create", "hos_destroy", "hos_last_error", "hos_initialize", "hos_set_state",
+    "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
+    "hos_get_physical", "hos_get_slope_max", "hos_get_stats", "hos_synchronize",
+    "hos_space_2_fourier", "hos_fourier_2_space",
+]
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libhosnwt.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run hos-nwt_amd/build.sh")
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    lib.hos_last_error.restype = C.c_char_p
+    lib.hos_create.argtypes = [C.POINTER(HosParams), C.POINTER(vp)]
+    lib.hos_destroy.argtypes = [vp]
+    lib.hos_initialize.argtypes = [vp]
+    lib.hos_set_state.argtypes = [vp] + [_dp] * 6
+    lib.hos_get_state.argtypes = [vp] + [_dp] * 6
+    lib.hos_rk_step.argtypes = [vp, C.c_double, C.c_double, _dp]
+    lib.hos_commit.argtypes = [vp]
+    lib.hos_end_ramp.argtypes = [vp]
+    lib.hos_advance.argtypes = [vp, _dp, C.c_double, _dp, C.c_double, C.c_double, C.c_double,
+                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.hos_rhs.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 11
+    lib.hos_get_physical.argtypes = [vp] + [_dp] * 4
+    lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
+    lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
+    lib.hos_synchronize.argtypes = [vp]
+    for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
+        getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
+    for f in EXPORTS:
+        if f != "hos_last_error":
+            getattr(lib, f).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError("hosnwt: " + load().hos_last_error().decode())
+
+
+def ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
+    return a.ctypes.data_as(_dp)
+
+
+def fortran(a) -> np.ndarray:
+    """Column-major float64 copy (the reference's array layout)."""
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def r2r(grid_in: np.ndarray, kind_x: str, kind_y: str, forward: bool, precision: int = 0) -> np.ndarray:
+    """SPACE_2_FOURIER / FOURIER_2_SPACE on the GPU for an (n1, n2) array."""
+    lib = load()
+    a = fortran(grid_in)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1, order="F")
+    n1, n2 = a.shape
+    out = np.zeros((n1, n2), dtype=np.float64, order="F")
+    kx = 0 if kind_x == "cos" else 1
+    ky = 0 if kind_y == "cos" else 1
+    f = lib.hos_space_2_fourier if forward else lib.hos_fourier_2_space
+    check(f(n1, n2, kx, ky, precision, ptr(a), ptr(out)))
+    return out

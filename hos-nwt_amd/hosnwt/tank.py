@@ -1,0 +1,184 @@
+"""``Tank``: the reference driver loop around the GPU time-stepping core.
+
+Mirrors HOS-NWT.f90:446-581: per output interval, the adaptive RK4(3) loop
+(on the GPU, through hos_advance), then volume and energy.  The operator names
+follow the reference: ``runge4_adapt2`` (one attempt), ``solve_hos2`` (one
+RHS), ``space_2_fourier`` / ``fourier_2_space``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import config as cfgmod
+from .lib import HosParams, HosStats, check, fortran, load, ptr
+
+
+class Tank:
+    """One GPU context holding the solver state of a tank."""
+
+    def __init__(self, setup: cfgmod.Setup, precision: int | None = None):
+        self.s = setup
+        cfg = setup.cfg
+        self.cfg = cfg
+        self.lib = load()
+        self.n1, self.n2, self.n3 = cfg.n1, cfg.n2, cfg.n3
+        self.n3a = 2 * (cfg.n3 - 1)
+        prec = cfg.precision if precision is None else precision
+        self._keep = []
+        nu = fortran(setup.nu)
+        pr = fortran(setup.pos_ini.real)
+        pi = fortran(setup.pos_ini.imag)
+        fr = np.ascontiguousarray(setup.freq, dtype=np.float64)
+        self._keep += [nu, pr, pi, fr]
+        p = HosParams()
+        p.n1, p.n2, p.n3, p.mHOS = cfg.n1, cfg.n2, cfg.n3, cfg.mHOS
+        p.p1 = cfg.mHOS
+        p.p2 = cfg.mHOS if cfg.n2 != 1 else 1
+        p.icase, p.igeom, p.i_wmk, p.iramp, p.iabsnb = cfg.icase, cfg.igeom, cfg.i_wmk, cfg.iramp, cfg.iabsnb
+        p.precision = prec
+        p.xlen, p.ylen, p.d_hinge = setup.xlen, setup.ylen, cfg.d_hinge / cfg.h
+        p.Tramp, p.T_stop, p.T_mono = setup.Tramp, setup.T_stop, setup.T_mono
+        p.nu = ptr(nu)
+        p.n_freq = len(fr)
+        p.freq = ptr(fr) if len(fr) else None
+        p.pos_ini_re = ptr(pr) if len(fr) else None
+        p.pos_ini_im = ptr(pi) if len(fr) else None
+        h = C.c_void_p()
+        check(self.lib.hos_create(C.byref(p), C.byref(h)))
+        self.h = h
+        self.time = 0.0
+        self.n_rk_tot = 0
+        self.n_er_tot = 0
+
+    def close(self):
+        if self.h:
+            check(self.lib.hos_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state -----------------------------------------------------------
+    def initialize(self):
+        """Initial wavemaker modes from rest (initial_condition.f90:314-466)."""
+        check(self.lib.hos_initialize(self.h))
+        self.time = 0.0
+
+    def set_state(self, eta, phis, a1st_add=None, a_add_ramp=None, a1st=None, eta1st=None):
+        arrs = [fortran(eta), fortran(phis)] + [None if a is None else fortran(a)
+                                                 for a in (a1st_add, a_add_ramp, a1st, eta1st)]
+        check(self.lib.hos_set_state(self.h, *[ptr(a) for a in arrs]))
+
+    def get_state(self):
+        n1, n2, na = self.n1, self.n2, self.n3a
+        out = [np.zeros((n1, n2), order="F"), np.zeros((n1, n2), order="F"), np.zeros((na, n2), order="F"),
+               np.zeros((na, n2), order="F"), np.zeros((n1, n2), order="F"), np.zeros((n1, n2), order="F")]
+        check(self.lib.hos_get_state(self.h, *[ptr(a) for a in out]))
+        return dict(zip(("eta", "phis", "a1st_add", "a_add_ramp", "a1st", "eta1st"), out))
+
+    def physical(self):
+        n1, n2 = self.n1, self.n2
+        out = [np.zeros((n1, n2), order="F") for _ in range(4)]
+        check(self.lib.hos_get_physical(self.h, *[ptr(a) for a in out]))
+        return dict(zip(("eta", "phis", "deta", "phi_add"), out))
+
+    # ---- operators ---------------------------------------------------------
+    def runge4_adapt2(self, t: float, h: float) -> float:
+        """One attempt; the trial state stays on the GPU until commit()."""
+        err = C.c_double()
+        check(self.lib.hos_rk_step(self.h, t, h, C.byref(err)))
+        return err.value
+
+    def commit(self):
+        check(self.lib.hos_commit(self.h))
+
+    def end_ramp(self):
+        check(self.lib.hos_end_ramp(self.h))
+
+    def solve_hos2(self, t, delta_t, G1, G2, a1st, eta1st, a_add_1):
+        n1, n2, na = self.n1, self.n2, self.n3a
+        ins = [fortran(x) for x in (G1, G2, a1st, eta1st, a_add_1)]
+        outs = [np.zeros((n1, n2), order="F") for _ in range(4)] + [np.zeros((na, n2), order="F") for _ in range(2)]
+        check(self.lib.hos_rhs(self.h, t, delta_t, *[ptr(a) for a in ins], *[ptr(a) for a in outs]))
+        return dict(zip(("dG1", "dG2", "da1st", "deta1st", "da_add_1", "da_add_ramp"), outs))
+
+    def advance(self, t_next: float, h_rk: float):
+        """Adaptive loop to t_next (HOS-NWT.f90:498-571); returns the next step proposal."""
+        t = C.c_double(self.time)
+        h = C.c_double(h_rk)
+        na, nr = C.c_int(), C.c_int()
+        check(self.lib.hos_advance(self.h, C.byref(t), t_next, C.byref(h), self.cfg.toler, self.s.dt_stab,
+                                   self.s.delt_out, C.byref(na), C.byref(nr)))
+        self.time = t.value
+        self.n_rk_tot += na.value
+        self.n_er_tot += nr.value
+        return h.value, na.value, nr.value
+
+    def slope_max(self):
+        sx, sy = C.c_double(), C.c_double()
+        check(self.lib.hos_get_slope_max(self.h, C.byref(sx), C.byref(sy)))
+        return sx.value, sy.value
+
+    def stats(self):
+        s = HosStats()
+        check(self.lib.hos_get_stats(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in HosStats._fields_}
+
+    def synchronize(self):
+        check(self.lib.hos_synchronize(self.h))
+
+    # ---- diagnostics (vol_energy.f90:36-119) --------------------------------
+    def volume(self, eta):
+        n1, n2 = self.n1, self.n2
+        delx = self.s.xlen / (n1 - 1)
+        w1 = np.ones(n1)
+        w1[[0, -1]] = 0.5
+        if n2 == 1:
+            return float((eta[:, 0] * w1).sum() * delx / self.s.xlen)
+        w2 = np.ones(n2)
+        w2[[0, -1]] = 0.5
+        dely = self.s.ylen / (n2 - 1)
+        return float((eta * w2[None, :]).sum(axis=1) @ w1 * delx / self.s.xlen / self.s.ylen * dely)
+
+    def energy(self, eta, phis, deta):
+        n1, n2 = self.n1, self.n2
+        delx = self.s.xlen / (n1 - 1)
+        w1 = np.ones(n1)
+        w1[[0, -1]] = 0.5
+
+        def integ(f):
+            if n2 == 1:
+                return float((f[:, 0] * w1).sum() * delx / 2.0)
+            w2 = np.ones(n2)
+            w2[[0, -1]] = 0.5
+            dely = self.s.ylen / (n2 - 1)
+            return float((f * w2[None, :]).sum(axis=1) @ w1 * delx / 2.0 * dely)
+        ep, ek = integ(eta * eta), integ(phis * deta)
+        return ep, ek, ep + ek
+
+    # ---- the driver loop -------------------------------------------------------
+    def run(self, T_stop: float | None = None, on_output=None):
+        """Time loop with outputs every 1/f_out (HOS-NWT.f90:446-581)."""
+        s = self.s
+        T_stop = s.T_stop if T_stop is None else T_stop
+        delt_out = s.delt_out
+        delt = min(0.95 * s.dt_stab, delt_out)
+        records = []
+        while self.time < T_stop + delt_out:
+            ph = self.physical()
+            vol = self.volume(ph["eta"])
+            en = self.energy(ph["eta"], ph["phis"] + ph["phi_add"], ph["deta"])
+            rec = dict(t=self.time, volume=vol, energy=en, n_rk=self.n_rk_tot, n_er=self.n_er_tot)
+            records.append(rec)
+            if on_output:
+                on_output(rec, ph)
+            if abs(self.time - T_stop) <= np.finfo(float).eps:
+                break
+            t_next = min(self.time + delt_out, T_stop + delt_out)
+            delt, _, _ = self.advance(t_next, delt)
+        return records

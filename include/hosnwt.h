@@ -1,0 +1,122 @@
+/*
+ * hosnwt.h - C-ABI of the MI355X HOS-NWT time-stepping core.
+ *
+ * The library owns one GPU context per call to hos_create.  The context holds
+ * the RK state resident in HBM (modal eta and phis, first-order wavemaker and
+ * free-surface quantities) and every scratch buffer; host arrays passed in or
+ * out are plain column-major double arrays (i1 = x index fastest), exactly the
+ * layout of the reference's REAL(RP) (m1,m2) / (m3_add,m2) arrays.  No pointer
+ * is retained across calls.  All entry points return 0 on success and a
+ * nonzero status otherwise; hos_last_error() gives the message.
+ *
+ * Reference interfaces replaced (LHEEA/HOS-NWT, sources/...):
+ *   hos_create        : setup of HOS-NWT.f90:269-406 (spectral and wavemaker
+ *                       tables) plus Fourier_ini (fourier/fourier_FFTW.f90:89)
+ *   hos_initialize    : initialize, first-order wavemaker part
+ *                       (main/initial_condition.f90:314-466)
+ *   hos_set_state /
+ *   hos_get_state     : the driver's state arrays eta, phis, a1st_add,
+ *                       a_add_ramp, a1st, eta1st (main/HOS-NWT.f90:506-515)
+ *   hos_rk_step       : runge4_adapt2 (timeintegration/runge_kutta.f90:81-360)
+ *                       minus the commit; the trial state stays on the GPU
+ *   hos_commit        : the accept branch copy-back (main/HOS-NWT.f90:534-543)
+ *   hos_end_ramp      : end-of-ramp correction (main/HOS-NWT.f90:546-550)
+ *   hos_advance       : the inner loop DO WHILE(time_cur < time_next) with the
+ *                       accept/reject controller (main/HOS-NWT.f90:498-571)
+ *   hos_rhs           : one solveHOS2 evaluation (main/resol_HOS.f90:270-656)
+ *   hos_get_physical  : eta, phis, deta, phi_add as read by energy()
+ *                       (main/HOS-NWT.f90:579, IO/vol_energy.f90:69)
+ *   hos_get_slope_max : max |eta_x|, |eta_y| read by check_slope
+ *                       (main/initial_condition.f90:694-696)
+ *   hos_space_2_fourier / hos_fourier_2_space (+ _big):
+ *                       SPACE_2_FOURIER / FOURIER_2_SPACE (_BIG) operators
+ *                       (fourier/fourier_FFTW.f90:35-63)
+ */
+#ifndef HOSNWT_H
+#define HOSNWT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hos_ctx hos_ctx;
+
+/* Run parameters, nondimensional (lengths / h, times / sqrt(h/g)) exactly as
+ * the reference holds them after HOS-NWT.f90:99-119. */
+typedef struct hos_params {
+  int n1, n2, n3;     /* modes (common_vars.f90:31-33); n2 = 1 for a 2-D tank */
+  int mHOS;           /* HOS order M (common_vars.f90:35) */
+  int p1, p2;         /* dealiasing parameters; supported: p1 = M, p2 = M (3-D) or 1 */
+  int icase;          /* 1 sloshing (no wavemaker), 2/3/31/32/33/4/41 wavemaker */
+  int igeom;          /* 1 piston, 2 hinged flap */
+  int i_wmk;          /* wavemaker nonlinear order; supported: 1 */
+  int iramp;          /* time ramp type (resol_wmkr.f90:1268-1397) */
+  int iabsnb;         /* numerical beach on/off */
+  int precision;      /* 0: FP64 everywhere; 1: FP32 transforms and products */
+  double xlen, ylen;  /* tank length and width */
+  double d_hinge;     /* hinge depth */
+  double Tramp, T_stop, T_mono;
+  const double* nu;   /* beach coefficient nu(x,y), n1*n2, may be NULL */
+  int n_freq;         /* number of distinct paddle frequencies */
+  const double* freq; /* n_freq frequencies (wavemaking.f90:136) */
+  const double* pos_ini_re; /* pos_ini(i2,f): n2*n_freq, i2 fastest */
+  const double* pos_ini_im;
+} hos_params;
+
+/* Kernel statistics of the last call (transform counts, bytes). */
+typedef struct hos_stats {
+  long long n_rhs;          /* solveHOS2 evaluations */
+  long long n_big;          /* extended-grid 2-D transforms */
+  long long n_base;         /* base-grid 2-D transforms */
+  long long big_bytes;      /* algorithmic HBM bytes of the big transforms */
+  int big_n1, big_n2;       /* extended grid actually used (>= reference Nd) */
+} hos_stats;
+
+int hos_create(const hos_params* p, hos_ctx** out);
+int hos_destroy(hos_ctx* c);
+const char* hos_last_error(void);
+
+/* Initial wavemaker modes and potential from rest at t = 0
+ * (initial_condition.f90:367-466). */
+int hos_initialize(hos_ctx* c);
+
+/* state: eta, phis, a1st, eta1st: n1*n2; a1st_add, a_add_ramp: n3_add*n2
+ * (n3_add = 2*(n3-1)).  eta, phis, eta1st physical; a1st modal. */
+int hos_set_state(hos_ctx* c, const double* eta, const double* phis, const double* a1st_add,
+                  const double* a_add_ramp, const double* a1st, const double* eta1st);
+int hos_get_state(hos_ctx* c, double* eta, double* phis, double* a1st_add, double* a_add_ramp, double* a1st,
+                  double* eta1st);
+
+/* One RK4(3) attempt from time t with step h; writes the embedded error. */
+int hos_rk_step(hos_ctx* c, double t, double h, double* err);
+int hos_commit(hos_ctx* c);
+int hos_end_ramp(hos_ctx* c);
+
+/* Adaptive loop from *t_cur to t_next.  h_rk: in/out step proposal.
+ * Counts accepted/rejected attempts. */
+int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
+                double delt_out, int* n_acc, int* n_rej);
+
+/* One RHS evaluation (solveHOS2) on modal inputs G1, G2 (n1*n2) and the
+ * stage values a1st (modal), eta1st (physical), a_add_1 (n3_add*n2) at
+ * time t with integrating-factor shift delta_t.  Outputs dG1, dG2, da1st,
+ * deta1st (n1*n2), da_add_1, da_add_ramp (n3_add*n2). */
+int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
+            const double* eta1st, const double* a_add_1, double* dG1, double* dG2, double* da1st,
+            double* deta1st, double* da_add_1, double* da_add_ramp);
+
+int hos_get_physical(hos_ctx* c, double* eta, double* phis, double* deta, double* phi_add);
+int hos_get_slope_max(hos_ctx* c, double* sx, double* sy);
+int hos_get_stats(hos_ctx* c, hos_stats* s);
+int hos_synchronize(hos_ctx* c);
+
+/* Standalone transforms on host arrays (copy in, transform on the GPU, copy
+ * out).  kind_x / kind_y: 0 = cos, 1 = sin.  Grid n1 x n2 (n2 = 1: 1-D).
+ * precision: 0 FP64, 1 FP32 on the GPU (host data stays double). */
+int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
+int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HOSNWT_H */

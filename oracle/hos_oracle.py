@@ -1,0 +1,1002 @@
+"""FP64 CPU oracle for the HOS-NWT time-stepping core.
+
+TEST INFRASTRUCTURE ONLY.  This module restates the reference's algorithm in
+numpy/scipy so that the GPU library can be checked against it.  Only the test
+suite, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py``
+may import it, and only as the checker / timed CPU baseline.  The product path
+(``hos-nwt_amd``) never imports it.
+
+It follows the reference's literal transform sequence ("reference mode"),
+including the round trips and the dealias calls the GPU path proves redundant,
+so that the GPU path's redundancy elimination is checked by it, not assumed.
+
+Array convention: every 2-D field is a numpy array of shape (n1, n2) indexed
+[i1-1, i2-1] (reference i1 = x index, i2 = y index).  Paddle-grid fields have
+shape (n3_add, n2).
+
+Pinning of each function (tests/test_oracle_*.py):
+  * r2r kinds / normalisations   -> O(N^2) closed-form FFTW definitions
+  * S2F/F2S                        -> identity round trips, F2S(a)=sum a_k cos
+  * dealias/filter_ext             -> projection identities, product identities
+  * phisxy_etaxy derivatives       -> single-mode closed forms
+  * hosphis_modes                  -> M=1 closed form W = sum a k tanh k;
+                                      convergence to the exact Laplace
+                                      solution at order (kA)^(M+1)
+  * expon/gtof                     -> closed-form rotation, inverse pair
+  * runge4_adapt2                  -> y'=lambda*y RK4 polynomial + Kutta-3
+                                      embedded difference (sloshing M=1 case)
+  * phiadd_sl separable form       -> brute-force quadruple loop on tiny grids
+  * match_surface_wmk              -> its four defining conditions
+  * end to end                     -> linear sloshing closed form, energy and
+                                      volume conservation
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy import fft as sfft
+
+# ---------------------------------------------------------------------------
+# constants (type.f90:38-50)
+# ---------------------------------------------------------------------------
+G = 9.81
+TINY = np.finfo(np.float64).eps
+TWOPI = 2.0 * math.pi
+L_ADD = 2  # common_vars.f90:47
+K_ADD_X_MAX = 700.0  # HOS-NWT.f90:359
+
+
+# ---------------------------------------------------------------------------
+# r2r kinds (FFTW definitions; fourier_FFTW.f90:130-167 selects them)
+# ---------------------------------------------------------------------------
+def r2r(x: np.ndarray, kind: str, axis: int) -> np.ndarray:
+    """Unnormalised FFTW r2r transform along ``axis``.
+
+    REDFT00 = DCT-I, RODFT00 = DST-I, REDFT01 = DCT-III, REDFT10 = DCT-II,
+    RODFT01 = DST-III, RODFT10 = DST-II.  scipy's unnormalised types 1-3 use
+    exactly FFTW's definitions (factor 2 on the interior sums).
+    """
+    if kind == "REDFT00":
+        return sfft.dct(x, type=1, axis=axis)
+    if kind == "RODFT00":
+        return sfft.dst(x, type=1, axis=axis)
+    if kind == "REDFT01":
+        return sfft.dct(x, type=3, axis=axis)
+    if kind == "REDFT10":
+        return sfft.dct(x, type=2, axis=axis)
+    if kind == "RODFT01":
+        return sfft.dst(x, type=3, axis=axis)
+    if kind == "RODFT10":
+        return sfft.dst(x, type=2, axis=axis)
+    raise ValueError(kind)
+
+
+def _s2f_vec(n: int) -> np.ndarray:
+    """Per-index S2F factor: 1/(2(n-1)) at ends, doubled inside (fourier_FFTW.f90:178-182)."""
+    v = np.full(n, 1.0 / max(1, 2 * (n - 1)))
+    v[1:n - 1] *= 2.0
+    return v
+
+
+def _f2s_vec(n: int) -> np.ndarray:
+    """Per-index F2S factor: 1 at ends, 1/2 inside (fourier_FFTW.f90:205-210)."""
+    v = np.ones(n)
+    v[1:n - 1] *= 0.5
+    return v
+
+
+class Fourier:
+    """The reference's transform operators on one grid (fourier_FFTW.f90:35-63).
+
+    ``grid`` is (N1, N2); N2 == 1 means a 2-D tank (1-D transforms).
+    """
+
+    def __init__(self, n1: int, n2: int):
+        self.n1, self.n2 = n1, n2
+        self.two_d = n2 != 1
+        self.s2f = _s2f_vec(n1)[:, None] * (_s2f_vec(n2)[None, :] if self.two_d else 1.0)
+        self.f2s = _f2s_vec(n1)[:, None] * (_f2s_vec(n2)[None, :] if self.two_d else 1.0)
+
+    def _execute(self, inp: np.ndarray, cx: str, cy: str) -> np.ndarray:
+        """execute / execute_big (fourier_FFTW.f90:271-302, :621-653)."""
+        a = np.array(inp, dtype=np.float64, copy=True)
+        n1, n2 = self.n1, self.n2
+        if self.two_d and cy == "sin":
+            a[:, 0] = 0.0
+            a[:, n2 - 1] = 0.0
+        if cx == "cos":
+            a = r2r(a, "REDFT00", 0)
+        else:
+            b = np.zeros_like(a)
+            b[1:n1 - 1, :] = r2r(a[1:n1 - 1, :], "RODFT00", 0)
+            a = b
+        if self.two_d:
+            if cy == "cos":
+                a = r2r(a, "REDFT00", 1)
+            else:
+                b = np.zeros_like(a)
+                b[:, 1:n2 - 1] = r2r(a[:, 1:n2 - 1], "RODFT00", 1)
+                a = b
+        return a
+
+    def space_2_fourier(self, f, cx="cos", cy="cos"):
+        """DCT_FFTW / DCT_FFTW_big (fourier_FFTW.f90:235-247, :558-571)."""
+        return self._execute(f, cx, cy) * self.s2f
+
+    def fourier_2_space(self, a, cx="cos", cy="cos"):
+        """IDCT_FFTW / IDCT_FFTW_big (fourier_FFTW.f90:251-267, :575-588)."""
+        return self._execute(np.asarray(a) * self.f2s, cx, cy)
+
+
+class FourierAdd:
+    """Paddle-grid transforms (n3_add x n2): z DCT/DST-II/III tensor y DCT/DST-I.
+
+    DCT_FFTW_add / IDCT_FFTW_add / execute_add (fourier_FFTW.f90:432-492).
+    Forward uses REDFT01/RODFT01 along z, inverse REDFT10/RODFT10.
+    """
+
+    def __init__(self, n3_add: int, n2: int):
+        self.n3_add, self.n2 = n3_add, n2
+        self.two_d = n2 != 1
+        ys2f = _s2f_vec(n2)[None, :] if self.two_d else 1.0
+        yf2s = _f2s_vec(n2)[None, :] if self.two_d else 1.0
+        self.s2f = np.full((n3_add, 1), 1.0 / n3_add) * ys2f
+        self.f2s = np.full((n3_add, 1), 0.5) * yf2s
+        if not self.two_d:
+            self.s2f = np.full((n3_add, 1), 1.0 / n3_add)
+            self.f2s = np.full((n3_add, 1), 0.5)
+
+    def _execute(self, inp, cz, cy, fb):
+        a = np.array(inp, dtype=np.float64, copy=True)
+        n2 = self.n2
+        if self.two_d and cy == "sin":
+            a[:, 0] = 0.0
+            a[:, n2 - 1] = 0.0
+        if fb == 1:
+            kind = "REDFT01" if cz == "cos" else "RODFT01"
+        else:
+            kind = "REDFT10" if cz == "cos" else "RODFT10"
+        a = r2r(a, kind, 0)
+        if self.two_d:
+            if cy == "cos":
+                a = r2r(a, "REDFT00", 1)
+            else:
+                b = np.zeros_like(a)
+                b[:, 1:n2 - 1] = r2r(a[:, 1:n2 - 1], "RODFT00", 1)
+                a = b
+        return a
+
+    def space_2_fourier(self, f, cz="cos", cy="cos"):
+        return self._execute(f, cz, cy, 1) * self.s2f
+
+    def fourier_2_space(self, a, cz="cos", cy="cos"):
+        return self._execute(np.asarray(a) * self.f2s, cz, cy, -1)
+
+
+def fourier_2_space_y_axis(a: np.ndarray, cy: str, axis: int) -> np.ndarray:
+    """IDCT_FFTW_y applied along ``axis`` (fourier_FFTW.f90:362-380, :384-407).
+
+    For a length-n2 line: cos -> sum_j a_j cos(pi j i/(n2-1));
+    sin -> sum_j a_j sin(pi j i/(n2-1)) with end modes and end nodes zero.
+    """
+    a = np.moveaxis(np.asarray(a, dtype=np.float64), axis, -1)
+    n2 = a.shape[-1]
+    if n2 == 1:  # 2-D tank: the y "transform" is the identity (ky = 0)
+        out = a.copy() if cy == "cos" else np.zeros_like(a)
+        return np.moveaxis(out, -1, axis)
+    v = a * _f2s_vec(n2)
+    if cy == "cos":
+        out = r2r(v, "REDFT00", -1)
+    else:
+        out = np.zeros_like(v)
+        out[..., 1:n2 - 1] = r2r(v[..., 1:n2 - 1], "RODFT00", -1)
+    return np.moveaxis(out, -1, axis)
+
+
+# ---------------------------------------------------------------------------
+# parameters and static tables (HOS-NWT.f90:99-406)
+# ---------------------------------------------------------------------------
+@dataclass
+class Params:
+    """Nondimensional run parameters (input.f90:62-133 after HOS-NWT.f90:99-119)."""
+
+    n1: int
+    n2: int
+    mHOS: int
+    xlen: float
+    ylen: float = 1.0
+    n3: int = 33
+    p1: int | None = None  # default: mHOS  (dealiasing factor (p+1)/2)
+    p2: int | None = None  # default: mHOS in 3-D, 1 in 2-D
+    Nd1: int | None = None  # extended grid; default ((p1+1)*n1)//2
+    Nd2: int | None = None
+    icase: int = 2
+    igeom: int = 1
+    d_hinge: float = 0.0
+    iramp: int = 1
+    Tramp: float = 0.0
+    T_stop: float = 1.0e30
+    T_mono: float = 1.0
+    iabsnb: int = 1
+    xabsf: float = 0.8
+    coeffabsf: float = 1.0
+    i_wmk: int = 1
+    # wavemaker spectral content: pos_ini (n2 x n_freq complex) and freq (n_freq)
+    freq: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    pos_ini: np.ndarray = field(default_factory=lambda: np.zeros((1, 0), complex))
+
+    def __post_init__(self):
+        if self.p1 is None:
+            self.p1 = self.mHOS
+        if self.p2 is None:
+            self.p2 = self.mHOS if self.n2 != 1 else 1
+        if self.Nd1 is None:
+            self.Nd1 = ((self.p1 + 1) * self.n1) // 2
+        if self.Nd2 is None:
+            self.Nd2 = ((self.p2 + 1) * self.n2) // 2
+        if self.n2 == 1:
+            self.ylen = 1.0
+        self.n3_add = L_ADD * (self.n3 - 1)
+        self.has_wmk = self.icase in (2, 3, 31, 32, 33, 4, 41, 42) and self.igeom in (1, 2)
+
+
+class Tables:
+    """Wavenumbers, omega, kth, kth_ext and wavemaker tables (HOS-NWT.f90:269-406)."""
+
+    def __init__(self, p: Params):
+        self.p = p
+        n1, n2, M = p.n1, p.n2, p.mHOS
+        pixlen = math.pi / p.xlen
+        piylen = math.pi / p.ylen if n2 != 1 else 0.0
+        self.kx = np.arange(n1) * pixlen
+        self.ky = np.arange(n2) * piylen
+        self.k = np.sqrt(self.kx[:, None] ** 2 + self.ky[None, :] ** 2)
+        self.omega = np.sqrt(self.k * np.tanh(self.k))
+        self.kth = self._kth(self.k, M)
+        kxe = np.arange(p.Nd1) * pixlen
+        kye = np.arange(p.Nd2) * piylen
+        ke = np.sqrt(kxe[:, None] ** 2 + kye[None, :] ** 2)
+        self.kth_ext = self._kth(ke, M)
+        # mesh
+        self.delx = p.xlen / (n1 - 1)
+        self.dely = p.ylen / (n2 - 1) if n2 != 1 else 0.0
+        self.cx = np.arange(n1) * self.delx
+        self.cy = np.arange(n2) * self.dely
+        self.delz = 1.0 / (p.n3 - 1)
+        self.cz = np.arange(p.n3) * self.delz - 1.0
+        # wavemaker modes (HOS-NWT.f90:330-339)
+        xlen_add = 2.0 * L_ADD
+        pixlen_add = math.pi / xlen_add
+        n3a = p.n3_add
+        self.kx_add = (2 * np.arange(1, n3a + 1) - 1) * pixlen_add
+        self.k_add = np.sqrt(self.kx_add[:, None] ** 2 + self.ky[None, :] ** 2)
+        self.k_add_thk_add = self.k_add * np.tanh(self.k_add * p.xlen)
+        # beach (initial_condition.f90:724-794, full-width wavemaker, no back beach)
+        self.nu = np.zeros((n1, n2))
+        if p.iabsnb != 0:
+            x0 = p.xlen * p.xabsf
+            front = abs(p.xabsf - 1.0) > TINY
+            if front:
+                x2 = p.xlen - x0
+                sel = self.cx >= x0
+                x3 = (self.cx[sel] - x0) / x2
+                self.nu[sel, :] = (p.coeffabsf * x3 * x3)[:, None]
+        # wavemaker vertical shape (wavemaking.f90:184-215)
+        self.gz = np.array([gzed(z, p.igeom, p.d_hinge) for z in self.cz])
+
+    @staticmethod
+    def _kth(k: np.ndarray, M: int) -> np.ndarray:
+        out = np.zeros(k.shape + (M,))
+        thk = np.tanh(k)
+        for j in range(1, M + 1):
+            out[..., j - 1] = k ** j * thk if j % 2 == 1 else k ** j
+        return out
+
+    def csh_slice(self, i3: int):
+        """csh_add_x(:,:,i3) and k_add_sh_add_x(:,:,i3) (HOS-NWT.f90:364-405)."""
+        p = self.p
+        k = self.k_add[i3][None, :]  # (1, n2)
+        k2 = 2.0 * k
+        cx = self.cx[:, None]
+        k_add_x = k2 * p.xlen
+        expon3 = np.where(k_add_x <= K_ADD_X_MAX, np.exp(-np.minimum(k_add_x, K_ADD_X_MAX)), 0.0)
+        expon3p1 = expon3 + 1.0
+        k_add_x1 = k * cx
+        expon1 = np.where(k_add_x1 <= K_ADD_X_MAX, np.exp(-np.minimum(k_add_x1, K_ADD_X_MAX)) / expon3p1, 0.0)
+        k_add_x2 = k2 * (p.xlen - cx)
+        expon2 = np.where(k_add_x2 <= K_ADD_X_MAX, np.exp(-np.minimum(k_add_x2, K_ADD_X_MAX)), 0.0)
+        expon12 = np.where(k_add_x1 + k_add_x2 <= K_ADD_X_MAX, expon1 * expon2, 0.0)
+        csh = expon1 + expon12
+        sh = expon1 - expon12
+        return csh, sh * k
+
+
+def gzed(z: float, igeom: int, d_hinge: float) -> float:
+    """Paddle shape g(z) (wavemaking.f90:184-215)."""
+    delta = min(d_hinge, (1.0 - d_hinge) / 10.0)
+    if igeom == 1:
+        return 1.0
+    if z >= d_hinge - 1.0 + delta:
+        return (z + 1.0 - d_hinge) / (1.0 - d_hinge)
+    if z <= d_hinge - 1.0 - delta:
+        return 0.0
+    u = (z - (d_hinge - 1.0)) / delta
+    return delta / (2.0 * (1.0 - d_hinge)) * (5.0 / 16.0 + u + 15.0 / 16.0 * (u ** 2 - u ** 4 / 3.0 + u ** 6 / 15.0))
+
+
+# ---------------------------------------------------------------------------
+# wavemaker motion per stage (resol_wmkr.f90:1268-1397, wavemaking.f90:259-304)
+# ---------------------------------------------------------------------------
+def ramp(t: float, p: Params):
+    """alpha, alpha_t, alpha_tt of the time ramp (resol_wmkr.f90:1268-1397)."""
+    Tr, Ts = p.Tramp, p.T_stop
+    if p.iramp == 11:
+        Tsw = 3.0 / math.sqrt(5.0 / 9.81)
+        return (t / Tsw, 1.0 / Tsw, 0.0) if t <= Tsw else (1.0, 0.0, 0.0)
+    if p.iramp == 33:
+        Tsw = -math.log(0.1) / (10.0 * p.T_mono)
+        a = 1.0 - math.exp(-Tsw * t)
+        return a, (1.0 - a) * Tsw, (1.0 - a) * Tsw * Tsw
+    if t <= Tr:
+        if p.iramp == 0:
+            return 1.0, 0.0, 0.0
+        if p.iramp == 1:
+            return t / Tr, 1.0 / Tr, 0.0
+        if p.iramp == 2:
+            s = t / Tr
+            return s * s, 2.0 * s / Tr, 2.0 / (Tr * Tr)
+        if p.iramp == 21:
+            T2 = Tr ** 2
+            if t <= Tr / 2.0:
+                return 2.0 * t * t / T2, 4.0 * t / T2, 4.0 / T2
+            ts = t / Tr - 1.0
+            return 1.0 - 2.0 * ts ** 2, -4.0 * ts / Tr, -4.0 / T2
+        if p.iramp == 4:
+            s = t / Tr
+            s2 = s * s
+            return s2 * s2, 4.0 * s2 * s / Tr, 12.0 * s2 / (Tr * Tr)
+        if p.iramp == 41:
+            T2 = Tr ** 2
+            rap = 8.0 * t / Tr
+            rap2 = t * t / T2
+            rap22 = 3.0 * rap2
+            return ((rap22 - rap + 6.0) * rap2, (4.0 * rap22 - 3.0 * rap + 12.0) * t / T2,
+                    (12.0 * rap22 - 6.0 * rap + 12.0) / T2)
+        raise ValueError("Unknown ramp")
+    if t >= Ts + Tr:
+        return 0.0, 0.0, 0.0
+    if t >= Ts:
+        ti = Ts + Tr - t
+        if p.iramp == 1:
+            return ti / Tr, -1.0 / Tr, 0.0
+        if p.iramp == 2:
+            s = ti / Tr
+            return s * s, -2.0 * s / Tr, 2.0 / (Tr * Tr)
+        if p.iramp == 21:
+            T2 = Tr ** 2
+            if ti <= Tr / 2.0:
+                return 2.0 * ti * ti / T2, -4.0 * ti / T2, 4.0 / T2
+            ts = ti / Tr - 1.0
+            return 1.0 - 2.0 * ts ** 2, 4.0 * ts / Tr, -4.0 / T2
+        if p.iramp == 4:
+            s = ti / Tr
+            s2 = s * s
+            return s2 * s2, -4.0 * s2 * s / Tr, 12.0 * s2 / (Tr * Tr)
+        if p.iramp == 41:
+            T2 = Tr ** 2
+            rap = 8.0 * ti / Tr
+            rap2 = ti * ti / T2
+            rap22 = 3.0 * rap2
+            return ((rap22 - rap + 6.0) * rap2, -(4.0 * rap22 - 3.0 * rap + 12.0) * ti / T2,
+                    (12.0 * rap22 - 6.0 * rap + 12.0) / T2)
+        # the reference leaves alpha unchanged for iramp 0 here; iramp 0 has Tr=0
+        return 1.0, 0.0, 0.0
+    return 1.0, 0.0, 0.0
+
+
+def pos_1st(t: float, p: Params, tb: Tables):
+    """X, X_t, X_tt on the paddle, shape (n3, n2) (wavemaking.f90:259-304)."""
+    w = TWOPI * p.freq  # (nf,)
+    e = np.exp(1j * w * t)  # (nf,)
+    # sum over frequencies of pos_ini(i2,f) e^{i w t} (i w)^d, then times g(z)
+    base = p.pos_ini * e[None, :]  # (n2, nf)
+    s0 = base.sum(axis=1)
+    s1 = (base * (1j * w)[None, :]).sum(axis=1)
+    s2 = (base * (1j * w)[None, :] ** 2).sum(axis=1)
+    gz = tb.gz[:, None]
+    return (gz * s0[None, :]).real, (gz * s1[None, :]).real, (gz * s2[None, :]).real
+
+
+def match_surface_coeffs(f, df, d2f):
+    """Coefficients c1..c4 of the C2 continuation polynomial.
+
+    P(xx) = (c1 xx^7 + c2 xx^5 + c3 xx^3 + c4 xx) exp(-6 (xx + 1)) with
+    P(-1)=f, P'(-1)=df, P''(-1)=d2f and P(-2/3)=0 (resol_wmkr.f90:1655-1741,
+    l_add = 2).  Solved as the 4x4 linear system those conditions define.
+    """
+    f = np.atleast_1d(np.asarray(f, float))
+    A = np.zeros((4, 4))
+    x0, a = -1.0, 6.0
+    powers = [7, 5, 3, 1]
+    E0 = math.exp(-a * (x0 + 1.0))
+    for col, n in enumerate(powers):
+        # q = x^n ; P = q E ; P' = (q' - a q)E ; P'' = (q'' - 2a q' + a^2 q)E
+        q = x0 ** n
+        q1 = n * x0 ** (n - 1)
+        q2 = n * (n - 1) * x0 ** (n - 2) if n >= 2 else 0.0
+        A[0, col] = q * E0
+        A[1, col] = (q1 - a * q) * E0
+        A[2, col] = (q2 - 2 * a * q1 + a * a * q) * E0
+        xb = -2.0 / 3.0
+        A[3, col] = xb ** n * math.exp(-a * (xb + 1.0))
+    rhs = np.stack([f, np.atleast_1d(df), np.atleast_1d(d2f), np.zeros_like(f)])
+    return np.linalg.solve(A, rhs)  # (4, n)
+
+
+def match_surface_wmk(data: np.ndarray, n3: int, delz: float) -> np.ndarray:
+    """Continue a paddle RHS from i3=n3 up to n3_add (resol_wmkr.f90:1655-1741)."""
+    d = data.copy()
+    n3a = d.shape[0]
+    # reference rows n3-3..n3 are python n3-4..n3-1
+    fm3, fm2, fm1, f0 = d[n3 - 4], d[n3 - 3], d[n3 - 2], d[n3 - 1]
+    df = (fm2 - 4.0 * fm1 + 3.0 * f0) / (2.0 * delz)
+    d2f = (-fm3 + 4.0 * fm2 - 5.0 * fm1 + 2.0 * f0) / delz ** 2
+    c = match_surface_coeffs(f0, df, d2f)  # (4, n2)
+    i3 = np.arange(n3 + 1, n3a + 1)
+    xx = (i3 - 1) / (n3 - 1) - float(L_ADD)
+    xx2 = xx * xx
+    ex = np.exp(-6.0 * (xx + 1.0))
+    pxx = (((c[0][None, :] * xx2[:, None] + c[1][None, :]) * xx2[:, None] + c[2][None, :]) * xx2[:, None]
+           + c[3][None, :]) * xx[:, None] * ex[:, None]
+    d[n3:n3a, :] = pxx
+    return d
+
+
+# ---------------------------------------------------------------------------
+# additional potential on the free surface
+# ---------------------------------------------------------------------------
+def phiadd_sl(a_add, da_add, eta, tb: Tables, need=("phi", "phix", "phiy", "phiz", "phit")):
+    """Separable evaluation of phiadd_SL (resol_wmkr.f90:1124-1187).
+
+    phi(i1,i2) = sum_{i3} cos(kx_add(i3)(eta+1)) B(i1,i3,i2) with
+    B(i1,i3,.) = F2S_y[ a(i3,.) C(i1,.,i3) ]; identical to the reference's
+    quadruple loop in exact arithmetic (pinned by brute force in the tests).
+    Pass eta=None for z=0 (phiadd_1st_SWEET, resol_wmkr.f90:485-532).
+    """
+    p = tb.p
+    n1, n2 = p.n1, p.n2
+    out = {k: np.zeros((n1, n2)) for k in need}
+    etap1 = np.ones((n1, n2)) if eta is None else eta + 1.0
+    for i3 in range(p.n3_add):
+        csh, ksh = tb.csh_slice(i3)  # (n1, n2) indexed [i1, j2]
+        kxa = tb.kx_add[i3]
+        c = np.cos(kxa * etap1)
+        s = np.sin(kxa * etap1)
+        a = a_add[i3][None, :]
+        da = da_add[i3][None, :]
+        if "phi" in need:
+            out["phi"] += c * fourier_2_space_y_axis(a * csh, "cos", 1)
+        if "phit" in need:
+            out["phit"] += c * fourier_2_space_y_axis(da * csh, "cos", 1)
+        if "phix" in need:
+            out["phix"] -= c * fourier_2_space_y_axis(a * ksh, "cos", 1)
+        if "phiy" in need:
+            out["phiy"] -= c * fourier_2_space_y_axis(a * csh * tb.ky[None, :], "sin", 1)
+        if "phiz" in need:
+            out["phiz"] -= s * fourier_2_space_y_axis(a * csh * kxa, "cos", 1)
+    return out
+
+
+def phiadd_sl_bruteforce(a_add, da_add, eta, tb: Tables):
+    """The reference's literal quadruple loop (resol_wmkr.f90:1149-1175); tiny grids only."""
+    p = tb.p
+    n1, n2 = p.n1, p.n2
+    phi, phit, phix, phiy, phiz = (np.zeros((n1, n2)) for _ in range(5))
+    for i2 in range(n2):
+        for i1 in range(n1):
+            for i3 in range(p.n3_add):
+                csh, ksh = tb.csh_slice(i3)
+                e = 0.0 if eta is None else eta[i1, i2]
+                kx_e = tb.kx_add[i3] * (e + 1.0)
+                ce, se = math.cos(kx_e), math.sin(kx_e)
+                for j2 in range(n2):
+                    cy = math.cos(tb.ky[j2] * tb.cy[i2])
+                    sy = math.sin(tb.ky[j2] * tb.cy[i2])
+                    phi[i1, i2] += a_add[i3, j2] * ce * csh[i1, j2] * cy
+                    phit[i1, i2] += da_add[i3, j2] * ce * csh[i1, j2] * cy
+                    phix[i1, i2] -= a_add[i3, j2] * ce * ksh[i1, j2] * cy
+                    phiy[i1, i2] -= a_add[i3, j2] * ce * csh[i1, j2] * tb.ky[j2] * sy
+                    phiz[i1, i2] -= a_add[i3, j2] * se * csh[i1, j2] * tb.kx_add[i3] * cy
+    return dict(phi=phi, phit=phit, phix=phix, phiy=phiy, phiz=phiz)
+
+
+# ---------------------------------------------------------------------------
+# solver state
+# ---------------------------------------------------------------------------
+@dataclass
+class State:
+    """Driver state copied into runge4_adapt2 each attempt (HOS-NWT.f90:506-515)."""
+
+    eta: np.ndarray
+    phis: np.ndarray
+    a1st_add: np.ndarray
+    a_add_ramp: np.ndarray
+    a1st: np.ndarray
+    eta1st: np.ndarray
+
+    def copy(self):
+        return State(*(x.copy() for x in (self.eta, self.phis, self.a1st_add, self.a_add_ramp,
+                                          self.a1st, self.eta1st)))
+
+
+class Oracle:
+    """Reference-mode restatement of solveHOS2 / runge4_adapt2 and the driver loop."""
+
+    def __init__(self, p: Params, wmk_direct: bool = False):
+        self.p = p
+        self.tb = Tables(p)
+        self.F = Fourier(p.n1, p.n2)
+        self.FB = Fourier(p.Nd1, p.Nd2)
+        self.FA = FourierAdd(p.n3_add, p.n2)
+        self.wmk_direct = wmk_direct
+        # module globals written inside the step (resol_wmkr.f90:1268-1397, wavemaking.f90:277-295)
+        self.alpha = self.alphat = self.alphatt = 0.0
+        self.pos1st = np.zeros((p.n3, p.n2))
+        self.dpos1stdt = np.zeros((p.n3, p.n2))
+        self.d2pos1stdt2 = np.zeros((p.n3, p.n2))
+        self.time = 0.0
+        self.etax = self.etay = None
+        self.phi_add = np.zeros((p.n1, p.n2))
+        self.deta = np.zeros((p.n1, p.n2))
+        self.n_rhs = 0
+        self.count = {"big": 0, "base": 0}
+        self.endramp = 0  # HOS-NWT.f90:527,546-549
+
+    # ----- extended-grid helpers -----------------------------------------
+    def extend(self, a):
+        """extend (variables.f90:243-266)."""
+        p = self.p
+        e = np.zeros((p.Nd1, p.Nd2))
+        e[:p.n1, :p.n2] = a
+        return e
+
+    def s2f_big(self, f, cx="cos", cy="cos"):
+        self.count["big"] += 1
+        return self.FB.space_2_fourier(f, cx, cy)
+
+    def f2s_big(self, a, cx="cos", cy="cos"):
+        self.count["big"] += 1
+        return self.FB.fourier_2_space(a, cx, cy)
+
+    def s2f(self, f, cx="cos", cy="cos"):
+        self.count["base"] += 1
+        return self.F.space_2_fourier(f, cx, cy)
+
+    def f2s(self, a, cx="cos", cy="cos"):
+        self.count["base"] += 1
+        return self.F.fourier_2_space(a, cx, cy)
+
+    def _dealias_active(self, order):
+        p = self.p
+        return ((p.p1 != 1 and (order - 1) % (p.p1 - 1) == 0)
+                or (p.p2 != 1 and (order - 1) % (p.p2 - 1) == 0))
+
+    def dealias(self, order, x, cx="cos", cy="cos"):
+        """dealias (dealiasing.f90:39-92)."""
+        p = self.p
+        if not self._dealias_active(order):
+            return x
+        a = self.s2f_big(x, cx, cy)
+        if p.p1 != 1 and (order - 1) % (p.p1 - 1) == 0:
+            a[p.n1:p.Nd1, :] = 0.0
+        if p.n2 != 1 and p.p2 != 1 and (order - 1) % (p.p2 - 1) == 0:
+            a[:, p.n2:p.Nd2] = 0.0
+        return self.f2s_big(a, cx, cy)
+
+    def filter_ext(self, x, cx="cos", cy="cos"):
+        """filter_ext (dealiasing.f90:96-134)."""
+        p = self.p
+        a = self.s2f_big(x, cx, cy)
+        return self.f2s(a[:p.n1, :p.n2], cx, cy)
+
+    # ----- integrating factor (variablechange.f90:37-130) ----------------
+    def expon(self, n1_, n2_, dt):
+        c = np.cos(self.tb.omega * dt)
+        s = np.sin(self.tb.omega * dt)
+        return n1_ * c - n2_ * s, n2_ * c + n1_ * s
+
+    def gtof(self, g1, g2, dt):
+        c = np.cos(-self.tb.omega * dt)
+        s = np.sin(-self.tb.omega * dt)
+        return g1 * c - g2 * s, g2 * c + g1 * s
+
+    # ----- phisxy_etaxy (resol_HOS.f90:924-1078) --------------------------
+    def phisxy_etaxy(self, phisrk, etark):
+        p, tb = self.p, self.tb
+        M = p.mHOS
+        as_ = self.s2f(etark)
+        etapm = [None] * (M + 1)
+        etapm[1] = self.f2s_big(self.extend(as_))
+        for j in range(1, M):
+            e = etapm[j] / (j + 1) * etapm[1]
+            etapm[j + 1] = self.dealias(j + 1, e)
+        etax = self.f2s_big(self.extend(-tb.kx[:, None] * as_), "sin", "cos")
+        if p.n2 != 1:
+            etay = self.f2s_big(self.extend(-tb.ky[None, :] * as_), "cos", "sin")
+        else:
+            etay = np.zeros_like(etax)
+        gradeta2 = self.dealias(2, etax * etax + etay * etay)
+        as_ = self.s2f(phisrk)
+        phisx = self.f2s_big(self.extend(-tb.kx[:, None] * as_), "sin", "cos")
+        if p.n2 != 1:
+            phisy = self.f2s_big(self.extend(-tb.ky[None, :] * as_), "cos", "sin")
+        else:
+            phisy = np.zeros_like(phisx)
+        return etapm, etax, etay, gradeta2, phisx, phisy
+
+    # ----- HOSphis_modes (resol_HOS.f90:665-915) --------------------------
+    def hosphis_modes(self, phisrk, etapm, gradeta2):
+        p, tb = self.p, self.tb
+        M = p.mHOS
+        kth_ext = tb.kth_ext
+        phim = self.s2f(phisrk)
+        aHOS = [None] * (M + 2)
+        aHOS[1] = self.extend(phim)
+        phim = self.f2s(phim)  # result unused in the reference too
+        phizm = [None] + [np.zeros((p.Nd1, p.Nd2)) for _ in range(M)]
+        for iHOS in range(1, M + 1):
+            phimext = np.zeros((p.Nd1, p.Nd2))
+            for j in range(iHOS, 1, -1):
+                jm1 = j - 1
+                j2 = iHOS - jm1
+                phizi = self.f2s_big(aHOS[j2] * kth_ext[:, :, j - 1])
+                phimext = phimext - phizi * etapm[j]
+                phizm[iHOS] = phizm[iHOS] + phizi * etapm[jm1]
+            phizi = self.f2s_big(aHOS[iHOS] * kth_ext[:, :, 0])
+            phimext = phimext - phizi * etapm[1]
+            phizm[iHOS] = phizm[iHOS] + phizi
+            if iHOS > 1:
+                phizm[iHOS] = self.dealias(iHOS, phizm[iHOS])
+            if iHOS < M:
+                phimext = self.dealias(iHOS + 1, phimext)
+                aHOS[iHOS + 1] = self.s2f_big(phimext)
+        if M >= 2:
+            phizMm2 = np.zeros((p.Nd1, p.Nd2))
+            for j in range(1, M - 1):
+                phizMm2 = phizMm2 + phizm[j]
+            phizMm1 = phizMm2 + phizm[M - 1]
+            phiz = self.filter_ext(phizMm1.copy()) + self.filter_ext(phizm[M].copy())
+        else:
+            phizMm1 = np.zeros((p.Nd1, p.Nd2))
+            phiz = self.filter_ext(phizm[1].copy())
+        W1 = self.filter_ext(phizm[1].copy())
+        geta2phiz = np.zeros((p.Nd1, p.Nd2))
+        for j in range(1, M - 2):
+            geta2phiz = geta2phiz + self.dealias(j + 2, gradeta2 * phizm[j])
+        if M >= 3:
+            geta2phiz = geta2phiz + self.dealias(M, gradeta2 * phizm[max(M - 2, 1)])
+        geta2phiz2 = np.zeros((p.Nd1, p.Nd2))
+        temp1 = np.zeros((p.Nd1, p.Nd2))
+        for j in range(1, (M - 2) // 2 + 1):
+            temp2 = self.dealias(2 * j, phizm[j] * phizm[j])
+            temp1 = temp1 + temp2
+            geta2phiz2 = geta2phiz2 + self.dealias(2 * j + 2, gradeta2 * temp2)
+        for j in range(2, M - 2):
+            for q in range(1, min(M - 2 - j, j - 1) + 1):
+                temp2 = self.dealias(j + q, 2.0 * phizm[j] * phizm[q])
+                temp1 = temp1 + temp2
+                geta2phiz2 = geta2phiz2 + self.dealias(j + q + 2, gradeta2 * temp2)
+        if M >= 2:
+            temp2 = self.dealias(2 * (M // 2), phizm[M // 2] * phizm[M // 2])
+            temp1 = temp1 + temp2
+        for j in range(2, M):
+            for q in range(max(0, min(M - 2 - j, j - 1)) + 1, min(M - j, j - 1) + 1):
+                temp2 = self.dealias(j + q, 2.0 * phizm[j] * phizm[q])
+                temp1 = temp1 + temp2
+        phiz2 = temp1
+        return dict(phiz=phiz, W1=W1, phizMm1=phizMm1, geta2phiz=geta2phiz,
+                    geta2phiz2=geta2phiz2, phiz2=phiz2)
+
+    # ----- wavemaker first order (resol_wmkr.f90:109-210) ----------------
+    def solve_sweet_1(self, a1st, eta1st, a_add_1):
+        p, tb = self.p, self.tb
+        self.alpha, self.alphat, self.alphatt = ramp(self.time, p)
+        self.pos1st, self.dpos1stdt, self.d2pos1stdt2 = pos_1st(self.time, p, tb)
+        al, alt, altt = self.alpha, self.alphat, self.alphatt
+        n3, n3a = p.n3, p.n3_add
+        d1 = np.zeros((n3a, p.n2))
+        dr = np.zeros((n3a, p.n2))
+        d1[:n3] = -al * self.d2pos1stdt2 - 2.0 * alt * self.dpos1stdt - altt * self.pos1st
+        dr[:n3] = -alt * self.dpos1stdt - altt * self.pos1st
+        dr = match_surface_wmk(dr, n3, tb.delz)
+        d1 = match_surface_wmk(d1, n3, tb.delz)
+        d1 = self.FA.space_2_fourier(d1) / tb.k_add_thk_add
+        dr = self.FA.space_2_fourier(dr) / tb.k_add_thk_add
+        # phiadd_1st_SWEET at z=0: only phi_add_1, phit_add_1 and phiz_add_1 reach the RK path
+        if self.wmk_direct:
+            sl = phiadd_sl_bruteforce(a_add_1, d1, None, tb)
+        else:
+            sl = phiadd_sl(a_add_1, d1, None, tb, need=("phit", "phiz"))
+        # phidphiz (resol_wmkr.f90:583-636)
+        phiz1st = self.f2s(a1st * tb.kth[:, :, 0]) + sl["phiz"]
+        da1 = -eta1st - sl["phit"] - p.iabsnb * tb.nu * phiz1st
+        deta1 = phiz1st
+        da1 = self.s2f(da1)
+        return da1, deta1, d1, dr
+
+    # ----- solveHOS2 (resol_HOS.f90:270-656) ------------------------------
+    def solve_hos2(self, a1st, eta1st, G2, G1, a_add_1, delta_t):
+        p, tb = self.p, self.tb
+        self.n_rhs += 1
+        etark, phisrk = self.gtof(G1, G2, delta_t)
+        om = tb.omega.copy()
+        om[0, 0] = 1.0
+        ph = phisrk / om
+        ph[0, 0] = phisrk[0, 0]
+        phisrk = ph
+        NL1 = self.f2s(etark)
+        etark = etark.copy()
+        etark[0, 0] = self._volume_mode(NL1[0, 0])
+        etark = self.f2s(etark)
+        phisrk = self.f2s(phisrk)
+        etapm, etax, etay, gradeta2, phisx, phisy = self.phisxy_etaxy(phisrk, etark)
+        self.etax, self.etay = etax, etay
+        h = self.hosphis_modes(phisrk, etapm, gradeta2)
+        detarkext = -phisx * etax - phisy * etay
+        dphisrkext = -phisx * phisx - phisy * phisy
+        detarkext = self.dealias(2, detarkext)
+        dphisrkext = self.dealias(2, dphisrkext)
+        detarkext = detarkext + h["geta2phiz"]
+        dphisrkext = dphisrkext + h["geta2phiz2"] + h["phiz2"]
+        detark = self.filter_ext(detarkext)
+        dphisrk = self.filter_ext(dphisrkext)
+        n1, n2, n3a = p.n1, p.n2, p.n3_add
+        if p.has_wmk:
+            da1, deta1, d1, dr = self.solve_sweet_1(a1st, eta1st, a_add_1)
+            if self.wmk_direct:
+                sl = phiadd_sl_bruteforce(a_add_1, d1, etark, tb)
+            else:
+                sl = phiadd_sl(a_add_1, d1, etark, tb)
+            phi_add, phix_add, phiy_add = sl["phi"], sl["phix"], sl["phiy"]
+            phiz_add, phit_add = sl["phiz"], sl["phit"]
+        else:
+            da1 = np.zeros((n1, n2))
+            deta1 = np.zeros((n1, n2))
+            d1 = np.zeros((n3a, n2))
+            dr = np.zeros((n3a, n2))
+            phi_add = phix_add = phiy_add = phiz_add = phit_add = np.zeros((n1, n2))
+        self.phi_add = phi_add
+        # phiz_etax_ext / phiz_etay_ext are filtered only for PRESENT(c) (resol_HOS.f90:522-530)
+        self.filter_ext(etax * h["phizMm1"], "sin", "cos")
+        self.filter_ext(etay * h["phizMm1"], "cos", "sin")
+        phix_add = self.s2f(phix_add)
+        phiy_add = self.s2f(phiy_add)
+        phiz_add = self.s2f(phiz_add)
+        phix_add_ext = self.extend(phix_add)
+        phiy_add_ext = self.extend(phiy_add)
+        phiz_add_ext = self.extend(phiz_add)
+        phix_add = self.f2s(phix_add)
+        phiy_add = self.f2s(phiy_add)
+        phiz_add = self.f2s(phiz_add)
+        phix_add_ext = self.f2s_big(phix_add_ext)
+        phiy_add_ext = self.f2s_big(phiy_add_ext)
+        phiz_add_ext = self.f2s_big(phiz_add_ext)
+        gg_ext = etax * phix_add_ext + etay * phiy_add_ext
+        gp_ext = (phisx * phix_add_ext + phisy * phiy_add_ext
+                  + 0.5 * (phix_add_ext * phix_add_ext + phiy_add_ext * phiy_add_ext
+                           + phiz_add_ext * phiz_add_ext))
+        gradeta_gradphi = self.filter_ext(gg_ext)
+        gradphis_gradphi = self.filter_ext(gp_ext)
+        if p.mHOS == 1:
+            dphin = h["phiz"] + phiz_add
+            detark = dphin - h["W1"]
+            dphisrk = -phit_add - tb.nu * dphin
+        else:
+            dphin = detark + h["phiz"] + phiz_add - gradeta_gradphi
+            detark = dphin - h["W1"]
+            dphisrk = 0.5 * dphisrk - phit_add - tb.nu * dphin - gradphis_gradphi
+        NL1 = self.s2f(detark)
+        NL2 = self.s2f(dphisrk)
+        dphisrk_1 = NL2[0, 0]
+        NL2 = NL2 * tb.omega
+        dG1, dG2 = self.expon(NL1, NL2, delta_t)
+        dG2[0, 0] = dphisrk_1
+        return dG1, dG2, da1, deta1, d1, dr
+
+    def _volume_mode(self, nl11):
+        """Mean-volume mode from the paddle-corner value (resol_HOS.f90:358-364)."""
+        p = self.p
+        x = self.alpha * self.pos1st[p.n3 - 1, 0] / p.xlen
+        if p.igeom == 2:
+            return 0.5 * self.alpha * (1 - p.d_hinge) * self.pos1st[p.n3 - 1, 0] / p.xlen + nl11 * x
+        if p.igeom == 1:
+            return x + nl11 * x
+        return 0.0
+
+    # ----- runge4_adapt2 (runge_kutta.f90:81-360) ----------------------------
+    A = np.array([[0, 0, 0, 0, 0], [0.5, 0, 0, 0, 0], [0, 0.5, 0, 0, 0], [0, 0, 1.0, 0, 0],
+                  [-1.0, 2.0, 0, 0, 0]])
+    B = np.array([1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 0.0])
+    C = np.array([0.0, 0.5, 0.5, 1.0, 1.0])
+    E = np.array([1.0 / 6.0, 2.0 / 3.0, 0.0, 0.0, 1.0 / 6.0])
+    RK_P = 4
+
+    def runge4_adapt2(self, t_o, st: State, h):
+        """One RK4(3) attempt; returns (new State, err, deta) without committing."""
+        p, tb = self.p, self.tb
+        G1 = self.s2f(st.eta)
+        G2 = self.s2f(st.phis)
+        phisrk0 = G2[0, 0]
+        G2 = G2 * tb.omega
+        s = 5
+        kG1, kG2, ka1, ke1, kad1, kadr = ([None] * s for _ in range(6))
+        self.time = t_o
+        out = self.solve_hos2(st.a1st, st.eta1st, G2, G1, st.a1st_add, h * self.C[0])
+        kG1[0], kG2[0], ka1[0], ke1[0], kad1[0], kadr[0] = out
+        kG2[0][0, 0] = 0.0
+        for jl in range(1, s):
+            def comb(arr_list, base):
+                acc = np.zeros_like(base)
+                for ns in range(jl):
+                    acc = acc + arr_list[ns] * self.A[jl, ns]
+                return base + h * acc
+            G1m = comb(kG1, G1)
+            G2m = comb(kG2, G2)
+            aad = comb(kad1, st.a1st_add)
+            a1m = comb(ka1, st.a1st)
+            e1m = comb(ke1, st.eta1st)
+            self.time = t_o + h * self.C[jl]
+            out = self.solve_hos2(a1m, e1m, G2m, G1m, aad, h * self.C[jl])
+            kG1[jl], kG2[jl], ka1[jl], ke1[jl], kad1[jl], kadr[jl] = out
+            kG2[jl][0, 0] = 0.0
+
+        def bsum(arr_list, w):
+            acc = np.zeros_like(arr_list[0])
+            for j in range(s):
+                acc = acc + arr_list[j] * w[j]
+            return acc
+        dG1m, dG2m = bsum(kG1, self.B), bsum(kG2, self.B)
+        dad1, dadr = bsum(kad1, self.B), bsum(kadr, self.B)
+        da1, de1 = bsum(ka1, self.B), bsum(ke1, self.B)
+        er1, er2, era = bsum(kG1, self.E), bsum(kG2, self.E), bsum(kad1, self.E)
+        G1 = G1 + h * dG1m
+        G2 = G2 + h * dG2m
+        a_add_1 = st.a1st_add + h * dad1
+        a_add_r = st.a_add_ramp + h * dadr
+        a1st = st.a1st + h * da1
+        eta1st = st.eta1st + h * de1
+        er1 = np.abs((er1 - dG1m) * h) / (np.max(np.abs(G1)) + TINY)
+        er2 = np.abs((er2 - dG2m) * h) / (np.max(np.abs(G2)) + TINY)
+        era = np.abs((era - dad1) * h) / (np.max(np.abs(a_add_1)) + TINY)
+        # a_add_2 and a_add_3 are identically zero for a first-order wavemaker: their error
+        # terms are 0/(0+tiny) = 0 (runge_kutta.f90:273-279)
+        err = max(er1.max(), er2.max(), era.max())
+        etark, phisrk = self.gtof(G1, G2, h)
+        deta, _ = self.gtof(kG1[s - 1], kG2[s - 1], h)
+        om = tb.omega.copy()
+        om[0, 0] = 1.0
+        ph = phisrk / om
+        ph[0, 0] = phisrk0
+        d = deta + ph * tb.omega ** 2
+        d[0, 0] = deta[0, 0]
+        modesFS = self.f2s(etark)
+        etark = etark.copy()
+        etark[0, 0] = self._volume_mode(modesFS[0, 0])
+        self.modesFS = etark.copy()
+        new = State(self.f2s(etark), self.f2s(ph), a_add_1, a_add_r, a1st, eta1st)
+        deta = self.f2s(d)
+        return new, err, deta
+
+    # ----- initial state (initial_condition.f90:314-482) -------------------------
+    def initialize(self, eta0=None, phis0=None):
+        p, tb = self.p, self.tb
+        n1, n2 = p.n1, p.n2
+        eta = np.zeros((n1, n2)) if eta0 is None else np.array(eta0, float)
+        phis = np.zeros((n1, n2)) if phis0 is None else np.array(phis0, float)
+        z = np.zeros((n1, n2))
+        za = np.zeros((p.n3_add, n2))
+        self.time = 0.0
+        if p.has_wmk:
+            self.alpha, self.alphat, self.alphatt = ramp(0.0, p)
+            self.pos1st, self.dpos1stdt, self.d2pos1stdt2 = pos_1st(0.0, p, tb)
+            a = np.zeros((p.n3_add, n2))
+            a[:p.n3] = -self.alpha * self.dpos1stdt - self.alphat * self.pos1st
+            a = match_surface_wmk(a, p.n3, tb.delz)
+            a = self.FA.space_2_fourier(a) / tb.k_add_thk_add
+            da1, deta1, d1, dr = self.solve_sweet_1(z, z, a)
+            sl = phiadd_sl(a, d1, eta, tb, need=("phi",))
+            self.phi_add = sl["phi"]
+            return State(eta, phis, a, a.copy(), z.copy(), z.copy())
+        return State(eta, phis, za, za.copy(), z.copy(), z.copy())
+
+    # ----- volume / energy (vol_energy.f90:36-119) ----------------------------
+    def volume(self, eta):
+        p, tb = self.p, self.tb
+        w2 = np.ones(p.n2)
+        if p.n2 > 1:
+            w2[0] = w2[-1] = 0.5
+        sx = (eta * w2[None, :]).sum(axis=1) if p.n2 > 1 else eta[:, 0]
+        w1 = np.ones(p.n1)
+        w1[0] = w1[-1] = 0.5
+        v = (sx * w1).sum() * tb.delx / p.xlen
+        if p.n2 != 1:
+            v = v / p.ylen * tb.dely
+        return v
+
+    def energy(self, eta, phis, deta):
+        p, tb = self.p, self.tb
+        def integ(f):
+            w2 = np.ones(p.n2)
+            if p.n2 > 1:
+                w2[0] = w2[-1] = 0.5
+            sx = (f * w2[None, :]).sum(axis=1) if p.n2 > 1 else f[:, 0]
+            w1 = np.ones(p.n1)
+            w1[0] = w1[-1] = 0.5
+            v = (sx * w1).sum() * tb.delx / 2.0
+            return v * tb.dely if p.n2 != 1 else v
+        ep = integ(eta * eta)
+        ek = integ(phis * deta)
+        return ep, ek, ep + ek
+
+
+# ---------------------------------------------------------------------------
+# adaptive controller (HOS-NWT.f90:498-563)
+# ---------------------------------------------------------------------------
+def dt_stab(p: Params) -> float:
+    """RK4 stability bound dt_rk4 (HOS-NWT.f90:132-176)."""
+    c = math.sqrt(8.0 / math.pi ** 2)
+    d1 = c * p.xlen / (p.n1 - 1)
+    d2 = c * p.ylen / (p.n2 - 1) if p.n2 != 1 else d1 + 1.0
+    d3 = c * (2.0 * L_ADD) / (L_ADD * (p.n3 - 1))
+    if p.icase == 1:
+        return min(d1, d2)
+    if p.n2 == 1:
+        return min(d1, d3)
+    return min(d1, d2, d3)
+
+
+def advance(orc: Oracle, st: State, t_cur: float, t_next: float, h_rk: float, toler: float,
+            stab: float, delt_out: float, log=None, replay=None):
+    """Inner loop DO WHILE(time_cur < time_next) with accept/reject (HOS-NWT.f90:498-571).
+
+    ``replay``: optional list of (h_loc, accept) forcing the decisions (parity replay).
+    Returns (state, t_cur, h_rk, n_acc, n_rej, deta).
+    """
+    n_acc = n_rej = 0
+    deta = None
+    k = 0
+    p = orc.p
+    while t_cur < t_next:
+        h_loc = h_rk
+        if t_next - t_cur < h_rk:
+            h_loc = t_next - t_cur
+        new, err, d = orc.runge4_adapt2(t_cur, st, h_loc)
+        if replay is not None:
+            accept = replay[k][1]
+        else:
+            accept = not (err > toler and h_rk > 0.001 * 0.95 * stab)
+        if log is not None:
+            log.append((h_loc, accept, err))
+        k += 1
+        if not accept:
+            dt_correc = 0.9 * (err / toler) ** (-(1.0 / orc.RK_P))
+            n_rej += 1
+            if abs(p.Tramp - (t_cur + h_loc)) <= TINY and p.Tramp > t_cur:
+                orc.endramp = 0
+        else:
+            st = new
+            deta = d
+            orc.deta = d
+            if (p.iramp == 1 and p.Tramp - (t_cur + h_loc) <= TINY and p.Tramp > t_cur
+                    and orc.endramp != 1):
+                st.a1st_add = st.a1st_add - st.a_add_ramp
+                orc.endramp = 1
+            t_cur = t_cur + h_loc
+            dt_correc = 2.0 if abs(err) < TINY else 0.9 * (err / toler) ** (-(1.0 / orc.RK_P))
+            n_acc += 1
+        h_rk = h_rk * dt_correc
+        h_rk = min(h_rk, 0.95 * stab, delt_out)
+        h_rk = max(h_rk, 0.0001 * 0.95 * stab, 0.0001 * delt_out)
+    return st, t_cur, h_rk, n_acc, n_rej, deta
