@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""RK4(3) steps/s of the HOS-NWT time-stepping core on one MI355X.
+
+Workload: 3-D directional sea, 40 x 10 basin (h = 1 m), n1 = 1024 x n2 = 256 modes,
+HOS order M = 5, n3 = 33 paddle nodes, snake wavemaker (one paddle per y-node)
+fed a seeded JONSWAP spectrum (Hs 0.06 m, Tp 1.4 s, gamma 3.3) with cos^2s
+spreading (s = 10) over 256 frequencies x 32 directions, beach on the last 15 %.
+
+A step is one runge4_adapt2 attempt (5 solveHOS2 evaluations + combination +
+embedded error), accepted or rejected by the reference controller; the state
+stays resident in HBM for the whole timed region.  Both precisions are timed:
+FP64 everywhere, and FP32 transforms/products with FP64 RK state ("mixed").
+`value` is the mixed path's steps/s when its eta stays within 1e-5 relative L2
+of the FP64 path after the timed steps, otherwise the FP64 path's.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "hos-nwt_amd")]
+
+import numpy as np  # noqa: E402
+
+from hosnwt import config as hc  # noqa: E402
+
+WORKLOAD = dict(n1=1024, n2=256, M=5, n3=33, xlen=40.0, ylen=10.0, Hs=0.06, Tp=1.4, gamma=3.3, n_freq=256,
+                n_dir=32, s=10, xabsf=0.85, seed=2024)
+
+
+def make_setup(precision: int, wl=WORKLOAD) -> hc.Setup:
+    comps = hc.jonswap_components(wl["Hs"], wl["Tp"], wl["gamma"], n_freq=wl["n_freq"], seed=wl["seed"], rnum=12,
+                                  clock=32.0, n_dir=wl["n_dir"], spread_s=wl["s"])
+    cfg = hc.TankConfig(n1=wl["n1"], n2=wl["n2"], mHOS=wl["M"], xlen=wl["xlen"], ylen=wl["ylen"], n3=wl["n3"],
+                        icase=3, igeom=1, iramp=1, Tramp=2.0 * wl["Tp"], xabsf=wl["xabsf"], T_stop=50 * wl["Tp"],
+                        toler=1e-7, f_out=1.0 / wl["Tp"], precision=precision, components=comps)
+    return hc.build(cfg)
+
+
+def run_steps(tank, s, n, t0, h0):
+    """n attempts with the reference accept/reject rule (HOS-NWT.f90:518-563)."""
+    t, h = t0, h0
+    toler, stab = s.cfg.toler, s.dt_stab
+    n_acc = n_rej = 0
+    for _ in range(n):
+        err = tank.runge4_adapt2(t, h)
+        if err > toler and h > 0.001 * 0.95 * stab:
+            corr = 0.9 * (err / toler) ** -0.25
+            n_rej += 1
+        else:
+            tank.commit()
+            t += h
+            corr = 2.0 if err < 2.2e-16 else 0.9 * (err / toler) ** -0.25
+            n_acc += 1
+        h = max(min(h * corr, 0.95 * stab, s.delt_out), 1e-4 * 0.95 * stab, 1e-4 * s.delt_out)
+    return t, h, n_acc, n_rej
+
+
+def time_path(precision, steps, warmup, spin_up):
+    from hosnwt.tank import Tank
+    s = make_setup(precision)
+    tank = Tank(s)
+    tank.initialize()
+    h0 = min(0.95 * s.dt_stab, s.delt_out)
+    t, h, _, _ = run_steps(tank, s, spin_up + warmup, 0.0, h0)
+    tank.synchronize()
+    st0 = tank.stats()
+    t_start = time.perf_counter()
+    t, h, na, nr = run_steps(tank, s, steps, t, h)
+    tank.synchronize()
+    dt = time.perf_counter() - t_start
+    st1 = tank.stats()
+    eta = tank.get_state()["eta"]
+    out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, n_acc=na, n_rej=nr, t_end=t, eta=eta,
+               big_per_step=(st1["n_big"] - st0["n_big"]) / steps, base_per_step=(st1["n_base"] - st0["n_base"]) / steps,
+               big_grid=(st1["big_n1"], st1["big_n2"]), tank=tank, setup=s)
+    return out
+
+
+def roofline(tank, precision):
+    """Live HIP-event timing of the dominant kernel family (extended-grid r2r passes)."""
+    r = tank.time_big_transform(reps=20)
+    achieved = r["bytes"] / (r["ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": r["kernel"],
+            "bytes_per_launch": r["bytes"], "avg_ms": round(r["ms"], 5)}
+
+
+def cpu_baseline(wl=WORKLOAD):
+    """FP64 oracle (numpy/scipy, one thread) on one solveHOS2 evaluation of the same workload."""
+    from oracle import hos_oracle as ho
+    s = make_setup(0, wl)
+    c = s.cfg
+    p = ho.Params(n1=c.n1, n2=c.n2, mHOS=c.mHOS, xlen=s.xlen, ylen=s.ylen, n3=c.n3, icase=c.icase, igeom=c.igeom,
+                  d_hinge=0.0, iramp=c.iramp, Tramp=s.Tramp, T_stop=s.T_stop, T_mono=s.T_mono, iabsnb=c.iabsnb,
+                  xabsf=c.xabsf, coeffabsf=c.coeffabsf, freq=s.freq, pos_ini=s.pos_ini)
+    orc = ho.Oracle(p)
+    rng = np.random.default_rng(0)
+    kx = np.arange(c.n1) * math.pi / s.xlen
+    ky = np.arange(c.n2) * math.pi / s.ylen
+    k = np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2)
+    G1 = 1e-3 * rng.standard_normal((c.n1, c.n2)) * np.exp(-0.5 * k)
+    G2 = 1e-3 * rng.standard_normal((c.n1, c.n2)) * np.exp(-0.5 * k)
+    z = np.zeros((c.n1, c.n2))
+    za = np.zeros((2 * (c.n3 - 1), c.n2))
+    orc.time = 1.0
+    t0 = time.perf_counter()
+    orc.solve_hos2(z, z, G2, G1, za, 0.01)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / (5.0 * dt), 5), "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": f"one solveHOS2 evaluation (1/5 of an RK4(3) step) of the 1024x256 M=5 workload "
+                      f"in reference mode: {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--spin-up", type=int, default=0, help="untimed accepted/rejected attempts before warmup")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(ensembles and a single field both fit one MI355X).", file=sys.stderr)
+        sys.exit(2)
+    print(f"[bench] FP64 path: {args.warmup} warmup + {args.steps} timed attempts", file=sys.stderr, flush=True)
+    r64 = time_path(0, args.steps, args.warmup, args.spin_up)
+    print(f"[bench] FP64 {r64['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+    roof = roofline(r64["tank"], 0)
+    r32 = None
+    if not args.fp64_only:
+        print("[bench] mixed path", file=sys.stderr, flush=True)
+        r32 = time_path(1, args.steps, args.warmup, args.spin_up)
+        print(f"[bench] mixed {r32['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
+    eta_err = None
+    if r32 is not None:
+        d = np.linalg.norm(r32["eta"] - r64["eta"])
+        eta_err = float(d / max(np.linalg.norm(r64["eta"]), 1e-300))
+    use_mixed = r32 is not None and eta_err is not None and eta_err <= 1e-5
+    best = r32 if use_mixed else r64
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+    for r in (r64, r32):
+        if r is not None:
+            r["tank"].close()
+    line = {
+        "metric": "RK4(5) steps/s (3D 1024x256, M=5) mixed/learned vs FP64 GPU at eta err <=1e-5",
+        "value": round(best["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 transforms + f64 RK state" if use_mixed else "f64",
+        "data": "synthetic (seeded JONSWAP directional sea)",
+        "config": {"workload": "3D directional sea 40x10, n1=1024 x n2=256, M=5, n3=33, 256 freq x 32 dir",
+                   "path": "mixed" if use_mixed else "fp64", "big_grid": list(r64["big_grid"]),
+                   "big_transforms_per_step": r64["big_per_step"], "base_transforms_per_step": r64["base_per_step"]},
+        "fp64_steps_per_s": round(r64["steps_per_s"], 3),
+        "mixed_steps_per_s": round(r32["steps_per_s"], 3) if r32 else None,
+        "mixed_vs_fp64": round(r32["steps_per_s"] / r64["steps_per_s"], 3) if r32 else None,
+        "mixed_eta_rel_err": eta_err,
+        "accepted_rejected": [r64["n_acc"], r64["n_rej"]],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

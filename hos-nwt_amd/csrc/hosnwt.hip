@@ -871,6 +871,9 @@ struct EngineBase {
                    const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
                    double* kadr) = 0;
   virtual void slope_max(double& sx, double& sy) = 0;
+  // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
+  // and one y pass (full inverse cos-cos transform), timed with HIP events
+  virtual void time_big(int reps, double* out4) = 0;
 };
 
 struct hos_ctx {
@@ -1242,6 +1245,37 @@ struct Engine : EngineBase {
                            C.dx, a, da, eta, (double*)nullptr, phit1, (double*)nullptr, (double*)nullptr, phiz1);
     }
     HIP_OK(hipGetLastError());
+  }
+
+  void time_big(int reps, double* out4) override {
+    hos_ctx& C = *c;
+    const int N1b = C.N1b, N2b = C.N2b;
+    hipEvent_t e0, e1, e2;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventCreate(&e2));
+    Src<T, T> sx = src_plain<T>(X, N1b, N1b, N2b);
+    Dst<T> dx{N2b > 1 ? tmpB : Y, N1b, 0};
+    HIP_OK(hipEventRecord(e0, C.st));
+    for (int r = 0; r < reps; ++r) launch_x<T>(gb.px, N1b, KCOS, INV, N1b, N1b, N2b, sx, dx, C.st);
+    HIP_OK(hipEventRecord(e1, C.st));
+    if (N2b > 1) {
+      Src<T, T> sy = src_plain<T>(tmpB, N1b, N1b, N2b);
+      for (int r = 0; r < reps; ++r)
+        launch_y<T>(gb.py, gb.TX, N2b, KCOS, INV, N2b, N2b, N1b, sy, Dst<T>{Y, N1b, 0}, C.st);
+    }
+    HIP_OK(hipEventRecord(e2, C.st));
+    HIP_OK(hipEventSynchronize(e2));
+    float mx = 0.f, my = 0.f;
+    HIP_OK(hipEventElapsedTime(&mx, e0, e1));
+    HIP_OK(hipEventElapsedTime(&my, e1, e2));
+    out4[0] = mx / reps;
+    out4[1] = 2.0 * (double)nb * sizeof(T);
+    out4[2] = N2b > 1 ? my / reps : 0.0;
+    out4[3] = N2b > 1 ? 2.0 * (double)nb * sizeof(T) : 0.0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
   }
 
   void slope_max(double& sx, double& sy) override {
@@ -1673,6 +1707,13 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
 
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy) {
   return guard([&] { c->eng->slope_max(*sx, *sy); });
+}
+
+int hos_time_big_transform(hos_ctx* c, int reps, double* out4) {
+  return guard([&] {
+    if (reps < 1) throw std::runtime_error("reps must be >= 1");
+    c->eng->time_big(reps, out4);
+  });
 }
 
 int hos_get_stats(hos_ctx* c, hos_stats* s) {

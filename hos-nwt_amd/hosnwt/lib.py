@@ -40,7 +40,7 @@ EXPORTS = [
     "hos_create", "hos_destroy", "hos_last_error", "hos_initialize", "hos_set_state",
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
     "hos_get_physical", "hos_get_slope_max", "hos_get_stats", "hos_synchronize",
-    "hos_space_2_fourier", "hos_fourier_2_space",
+    "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform",
 ]
 
 _lib = None
@@ -71,6 +71,7 @@ def load() -> C.CDLL:
     lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
     lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
     lib.hos_synchronize.argtypes = [vp]
+    lib.hos_time_big_transform.argtypes = [vp, C.c_int, _dp]
     for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
         getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
     for f in EXPORTS:

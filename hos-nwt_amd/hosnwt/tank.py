@@ -129,6 +129,16 @@ class Tank:
         check(self.lib.hos_get_stats(self.h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in HosStats._fields_}
 
+    def time_big_transform(self, reps: int = 20):
+        """HIP-event timing of one extended-grid x pass and y pass; returns the slower one."""
+        out = np.zeros(4)
+        check(self.lib.hos_time_big_transform(self.h, reps, ptr(out)))
+        if out[2] > out[0]:
+            return dict(kernel="r2r_y_kernel (extended grid, inverse cos-cos)", ms=out[2], bytes=out[3],
+                        other=dict(ms=out[0], bytes=out[1]))
+        return dict(kernel="r2r_x_kernel (extended grid, inverse cos-cos)", ms=out[0], bytes=out[1],
+                    other=dict(ms=out[2], bytes=out[3]))
+
     def synchronize(self):
         check(self.lib.hos_synchronize(self.h))
 

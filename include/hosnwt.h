@@ -108,6 +108,11 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
 int hos_get_physical(hos_ctx* c, double* eta, double* phis, double* deta, double* phi_add);
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy);
 int hos_get_stats(hos_ctx* c, hos_stats* s);
+/* Measurement helper (no reference counterpart): average duration in ms and
+ * algorithmic HBM bytes of one extended-grid x pass and one y pass of a full
+ * inverse cos-cos transform, timed with HIP events on the context's stream.
+ * out4 = {ms_x, bytes_x, ms_y, bytes_y}. */
+int hos_time_big_transform(hos_ctx* c, int reps, double* out4);
 int hos_synchronize(hos_ctx* c);
 
 /* Standalone transforms on host arrays (copy in, transform on the GPU, copy

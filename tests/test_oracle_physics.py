@@ -1,0 +1,253 @@
+"""Pins the oracle's physics against what the mathematics fixes.
+
+Each check is chosen so that a plausible slip (dropped term, wrong sign,
+transposed operand, wrong index) fails it:
+  * the separable wavemaker sum equals the reference's literal quadruple loop;
+  * the continuation polynomial meets its four defining conditions;
+  * HOS W at order M converges to the exact Laplace solution like (kA)^(M+1);
+  * with eta = 0 every order above 1 vanishes: W = F2S(phis_hat k tanh k);
+  * GtoF inverts expon; linear sloshing (M=1) is integrated exactly;
+  * the nonlinear (M=3) sloshing step conserves energy and volume;
+  * partial dealiasing with p = M makes cos-cos products grid independent,
+    and the sine-to-cosine conversion reproduces the reference grid exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import hos_oracle as ho
+
+
+def tiny_params(**kw):
+    base = dict(n1=6, n2=5, mHOS=3, xlen=2.0, ylen=1.5, n3=5, icase=2, iramp=0)
+    base.update(kw)
+    return ho.Params(**base)
+
+
+def test_phiadd_separable_equals_quadruple_loop():
+    p = tiny_params()
+    tb = ho.Tables(p)
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((p.n3_add, p.n2))
+    da = rng.standard_normal((p.n3_add, p.n2))
+    eta = 0.05 * rng.standard_normal((p.n1, p.n2))
+    for e in (None, eta):
+        sep = ho.phiadd_sl(a, da, e, tb)
+        bf = ho.phiadd_sl_bruteforce(a, da, e, tb)
+        for k in sep:
+            np.testing.assert_allclose(sep[k], bf[k], rtol=1e-11, atol=1e-12, err_msg=k)
+
+
+def test_phiadd_separable_2d():
+    p = tiny_params(n1=9, n2=1)
+    tb = ho.Tables(p)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((p.n3_add, 1))
+    da = rng.standard_normal((p.n3_add, 1))
+    eta = 0.05 * rng.standard_normal((p.n1, 1))
+    sep = ho.phiadd_sl(a, da, eta, tb)
+    bf = ho.phiadd_sl_bruteforce(a, da, eta, tb)
+    for k in sep:
+        np.testing.assert_allclose(sep[k], bf[k], rtol=1e-11, atol=1e-12, err_msg=k)
+
+
+def test_continuation_conditions():
+    f, df, d2f = 0.3, -0.7, 1.9
+    c = ho.match_surface_coeffs(f, df, d2f)[:, 0]
+
+    def P(x, d=0):
+        # derivatives of (c1 x^7 + c2 x^5 + c3 x^3 + c4 x) e^{-6(x+1)} by finite differences of high order
+        fx = lambda t: (((c[0] * t * t + c[1]) * t * t + c[2]) * t * t + c[3]) * t * math.exp(-6 * (t + 1))
+        h = 1e-3
+        if d == 0:
+            return fx(x)
+        if d == 1:
+            return (-fx(x + 2 * h) + 8 * fx(x + h) - 8 * fx(x - h) + fx(x - 2 * h)) / (12 * h)
+        return (-fx(x + 2 * h) + 16 * fx(x + h) - 30 * fx(x) + 16 * fx(x - h) - fx(x - 2 * h)) / (12 * h * h)
+    assert abs(P(-1.0) - f) < 1e-12
+    assert abs(P(-1.0, 1) - df) < 1e-7
+    assert abs(P(-1.0, 2) - d2f) < 1e-5
+    assert abs(P(-2.0 / 3.0)) < 1e-12
+
+
+def test_match_surface_uses_one_sided_stencils():
+    """For quadratic paddle data the reference stencils are exact, so the continuation starts from f, f', f''."""
+    n3 = 9
+    delz = 1.0 / (n3 - 1)
+    z = np.arange(2 * (n3 - 1)) * delz
+    q = lambda t: 0.2 - 0.7 * t + 1.1 * t * t
+    col = np.zeros((2 * (n3 - 1), 1))
+    col[:n3, 0] = q(z[:n3])
+    out = ho.match_surface_wmk(col, n3, delz)
+    c = ho.match_surface_coeffs(q(1.0), -0.7 + 2.2, 2.2)[:, 0]
+    xx = np.arange(n3 + 1, 2 * (n3 - 1) + 1 - 0) - 1
+    xx = xx / (n3 - 1) - 2.0
+    P = (((c[0] * xx ** 2 + c[1]) * xx ** 2 + c[2]) * xx ** 2 + c[3]) * xx * np.exp(-6 * (xx + 1))
+    np.testing.assert_allclose(out[n3:, 0], P, rtol=1e-10, atol=1e-13)
+    np.testing.assert_array_equal(out[:n3, 0], col[:n3, 0])
+
+
+def _hos_w(M, A, n1=128, xlen=2 * math.pi, m=4):
+    """W from hosphis_modes for eta = A cos(kx), phis = exact Laplace potential at z = eta."""
+    p = ho.Params(n1=n1, n2=1, mHOS=M, xlen=xlen, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    x = np.arange(n1) * xlen / (n1 - 1)
+    k = m * math.pi / xlen
+    eta = (A * np.cos(k * x))[:, None]
+    phis = (np.cosh(k * (eta[:, 0] + 1)) / np.cosh(k) * np.cos(k * x))[:, None]
+    w_exact = k * np.sinh(k * (eta[:, 0] + 1)) / np.cosh(k) * np.cos(k * x)
+    etapm, etax, etay, g2, px, py = o.phisxy_etaxy(phis, eta)
+    h = o.hosphis_modes(phis, etapm, g2)
+    return np.max(np.abs(h["phiz"][:, 0] - w_exact)) / np.max(np.abs(w_exact))
+
+
+def test_hos_w_converges_with_order():
+    """|W_M - W_exact| ~ (kA)^(M+1): each order gains about a factor kA."""
+    A = 0.02
+    errs = [_hos_w(M, A) for M in (1, 2, 3, 4, 5)]
+    k = 4 * math.pi / (2 * math.pi)
+    for a, b in zip(errs[:-1], errs[1:]):
+        assert b < a * 3 * k * A + 1e-13, errs
+    assert errs[-1] < 1e-7
+
+
+def test_w_at_flat_surface_is_linear():
+    p = ho.Params(n1=33, n2=9, mHOS=5, xlen=3.0, ylen=2.0, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((33, 9)) * np.exp(-0.5 * o.tb.k)
+    phis = o.F.fourier_2_space(a)
+    eta = np.zeros((33, 9))
+    etapm, etax, etay, g2, px, py = o.phisxy_etaxy(phis, eta)
+    h = o.hosphis_modes(phis, etapm, g2)
+    w = o.F.fourier_2_space(a * o.tb.k * np.tanh(o.tb.k))
+    np.testing.assert_allclose(h["phiz"], w, atol=1e-12)
+    np.testing.assert_allclose(h["W1"], w, atol=1e-12)
+
+
+def test_gtof_inverts_expon():
+    p = ho.Params(n1=17, n2=1, mHOS=1, xlen=3.0, icase=1)
+    o = ho.Oracle(p)
+    rng = np.random.default_rng(4)
+    a, b = rng.standard_normal((17, 1)), rng.standard_normal((17, 1))
+    f1, f2 = o.expon(a, b, 0.37)
+    g1, g2 = o.gtof(f1, f2, 0.37)
+    np.testing.assert_allclose(g1, a, atol=1e-14)
+    np.testing.assert_allclose(g2, b, atol=1e-14)
+
+
+def test_linear_sloshing_is_exact():
+    """M=1 without wavemaker: eta = a cos(kx) cos(w t), w^2 = k tanh k, to rounding."""
+    n1, xlen, m = 33, 4.0, 3
+    p = ho.Params(n1=n1, n2=1, mHOS=1, xlen=xlen, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    x = np.arange(n1) * xlen / (n1 - 1)
+    k = m * math.pi / xlen
+    w = math.sqrt(k * math.tanh(k))
+    a = 0.01
+    st = o.initialize(eta0=(a * np.cos(k * x))[:, None])
+    t = 0.0
+    for h in (0.05, 0.11, 0.07):
+        st, err, _ = o.runge4_adapt2(t, st, h)
+        t += h
+    np.testing.assert_allclose(st.eta[:, 0], a * np.cos(k * x) * math.cos(w * t), atol=1e-15)
+    assert err < 1e-13
+
+
+def test_nonlinear_sloshing_conserves_energy_and_volume():
+    n1, xlen, m = 64, 4.0, 2
+    p = ho.Params(n1=n1, n2=1, mHOS=3, xlen=xlen, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    x = np.arange(n1) * xlen / (n1 - 1)
+    k = m * math.pi / xlen
+    st = o.initialize(eta0=(0.05 * np.cos(k * x))[:, None])
+    e0 = o.energy(st.eta, st.phis, np.zeros_like(st.eta))[2]
+    t = 0.0
+    for _ in range(30):
+        st, err, deta = o.runge4_adapt2(t, st, 0.05)
+        t += 0.05
+    e1 = o.energy(st.eta, st.phis, deta)[2]
+    # HOS at order M conserves energy to O((kA)^(M+1)) ~ 4e-5 here; a dropped term gives O(kA)
+    assert abs(e1 - e0) / e0 < 1e-4
+    assert abs(o.volume(st.eta)) < 1e-15
+
+
+def test_volume_energy_closed_forms():
+    p = ho.Params(n1=41, n2=21, mHOS=1, xlen=4.0, ylen=2.0, icase=1)
+    o = ho.Oracle(p)
+    one = np.ones((41, 21))
+    assert abs(o.volume(one) - 1.0) < 1e-14
+    x = np.arange(41) * 4.0 / 40
+    c = np.repeat(np.cos(math.pi * x / 4.0)[:, None], 21, axis=1)
+    assert abs(o.volume(c)) < 1e-15
+    # potential energy of eta = 1: 1/2 * area (trapezoid of a constant is exact)
+    assert abs(o.energy(one, 0 * one, 0 * one)[0] - 0.5 * 4.0 * 2.0) < 1e-13
+
+
+def test_padded_grid_is_exact_for_cos_products():
+    """Design basis: with p = M, retained modes of the HOS RHS do not depend on the extended grid."""
+    for n2 in (1, 8):
+        outs = []
+        for pad in (0, 1, 3):
+            p = ho.Params(n1=24, n2=n2, mHOS=5, xlen=4.0, ylen=2.0, icase=1, iabsnb=0)
+            p.Nd1 += pad
+            if n2 > 1:
+                p.Nd2 += pad
+            o = ho.Oracle(p)
+            rng = np.random.default_rng(5)
+            G1 = 0.2 * rng.standard_normal((24, n2)) * np.exp(-0.4 * o.tb.k)
+            G2 = 0.2 * rng.standard_normal((24, n2)) * np.exp(-0.4 * o.tb.k) * o.tb.omega
+            z = np.zeros((24, n2))
+            outs.append(o.solve_hos2(z, z, G2, G1, np.zeros((p.n3_add, n2)), 0.02))
+        for o2 in outs[1:]:
+            for a, b in zip(outs[0][:2], o2[:2]):
+                assert np.linalg.norm(a - b) / np.linalg.norm(a) < 1e-12
+
+
+def sin_to_cos(N, n, K):
+    M = N - 1
+    s2f = ho._s2f_vec(N)[:n]
+    q = np.arange(K)[None, :] + np.arange(n)[:, None]
+    r = np.arange(K)[None, :] - np.arange(n)[:, None]
+
+    def C(v):
+        out = np.zeros(v.shape)
+        odd = v % 2 != 0
+        out[odd] = 1.0 / np.tan(math.pi * v[odd] / (2 * M))
+        return out
+    return s2f[:, None] * (C(q) + C(r))
+
+
+@pytest.mark.parametrize("n,M", [(32, 3), (48, 4), (40, 5)])
+def test_sine_to_cosine_conversion_is_exact(n, M):
+    """filter_ext on the reference grid of (sine series) x (cosine series) from any finer grid."""
+    Nd = ((M + 1) * n) // 2
+    rng = np.random.default_rng(n)
+    e = rng.standard_normal(n)
+    a = rng.standard_normal(n)
+
+    def ext(v, N):
+        z = np.zeros((N, 1))
+        z[:n, 0] = v
+        return z
+    F = ho.Fourier(Nd, 1)
+    ref = F.space_2_fourier(F.fourier_2_space(ext(e, Nd), "sin") * F.fourier_2_space(ext(a, Nd)))[:n, 0]
+    Np = Nd + 7
+    Fp = ho.Fourier(Np, 1)
+    K = min(2 * n, Np)
+    P = Fp.space_2_fourier(Fp.fourier_2_space(ext(e, Np), "sin") * Fp.fourier_2_space(ext(a, Np)), "sin")[:K, 0]
+    got = sin_to_cos(Nd, n, K) @ P
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-13
+    naive = Fp.space_2_fourier(Fp.fourier_2_space(ext(e, Np), "sin") * Fp.fourier_2_space(ext(a, Np)))[:n, 0]
+    assert np.linalg.norm(naive - ref) / np.linalg.norm(ref) > 1e-8  # the conversion is needed
+
+
+def test_ramp_closed_forms():
+    p = ho.Params(n1=8, n2=1, mHOS=1, xlen=1.0, iramp=2, Tramp=3.0, T_stop=100.0)
+    a, at, att = ho.ramp(1.5, p)
+    assert (a, at, att) == (0.25, 2 * 0.5 / 3.0, 2 / 9.0)
+    assert ho.ramp(5.0, p) == (1.0, 0.0, 0.0)
+    assert ho.ramp(104.0, p) == (0.0, 0.0, 0.0)
+    p.iramp = 1
+    assert ho.ramp(1.5, p) == (0.5, 1.0 / 3.0, 0.0)
