@@ -1705,6 +1705,13 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
   });
 }
 
+int hos_get_wmk(hos_ctx* c, double* alpha, double* pos_top) {
+  return guard([&] {
+    *alpha = c->last_alpha;
+    *pos_top = c->last_pos_top;
+  });
+}
+
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy) {
   return guard([&] { c->eng->slope_max(*sx, *sy); });
 }

@@ -1,0 +1,109 @@
+! iso_c_binding interface to libhosnwt (include/hosnwt.h).
+!
+! A Fortran caller of the reference's solver loop (main/HOS-NWT.f90:446-581)
+! replaces CALL runge4_adapt2(...) by hos_rk_step / hos_commit, or the whole
+! inner DO WHILE(time_cur < time_next) loop by hos_advance.  Arrays are the
+! reference's column-major REAL(RP) (m1,m2) / (m3_add,m2) arrays, passed by
+! reference.
+MODULE hosnwt_gpu
+USE, INTRINSIC :: iso_c_binding
+IMPLICIT NONE
+!
+TYPE, BIND(C) :: hos_params
+   INTEGER(C_INT) :: n1, n2, n3, mHOS, p1, p2, icase, igeom, i_wmk, iramp, iabsnb, precision
+   REAL(C_DOUBLE) :: xlen, ylen, d_hinge, Tramp, T_stop, T_mono
+   TYPE(C_PTR)    :: nu
+   INTEGER(C_INT) :: n_freq
+   TYPE(C_PTR)    :: freq, pos_ini_re, pos_ini_im
+END TYPE hos_params
+!
+INTERFACE
+   INTEGER(C_INT) FUNCTION hos_create(p, ctx) BIND(C, NAME='hos_create')
+      IMPORT :: C_INT, C_PTR, hos_params
+      TYPE(hos_params), INTENT(IN) :: p
+      TYPE(C_PTR), INTENT(OUT)     :: ctx
+   END FUNCTION hos_create
+   INTEGER(C_INT) FUNCTION hos_destroy(ctx) BIND(C, NAME='hos_destroy')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+   END FUNCTION hos_destroy
+   TYPE(C_PTR) FUNCTION hos_last_error() BIND(C, NAME='hos_last_error')
+      IMPORT :: C_PTR
+   END FUNCTION hos_last_error
+   INTEGER(C_INT) FUNCTION hos_initialize(ctx) BIND(C, NAME='hos_initialize')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+   END FUNCTION hos_initialize
+   INTEGER(C_INT) FUNCTION hos_set_state(ctx, eta, phis, a1st_add, a_add_ramp, a1st, eta1st) &
+        BIND(C, NAME='hos_set_state')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: eta(*), phis(*), a1st_add(*), a_add_ramp(*), a1st(*), eta1st(*)
+   END FUNCTION hos_set_state
+   INTEGER(C_INT) FUNCTION hos_get_state(ctx, eta, phis, a1st_add, a_add_ramp, a1st, eta1st) &
+        BIND(C, NAME='hos_get_state')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: eta(*), phis(*), a1st_add(*), a_add_ramp(*), a1st(*), eta1st(*)
+   END FUNCTION hos_get_state
+   INTEGER(C_INT) FUNCTION hos_rk_step(ctx, t, h, err) BIND(C, NAME='hos_rk_step')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: t, h
+      REAL(C_DOUBLE), INTENT(OUT) :: err
+   END FUNCTION hos_rk_step
+   INTEGER(C_INT) FUNCTION hos_commit(ctx) BIND(C, NAME='hos_commit')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+   END FUNCTION hos_commit
+   INTEGER(C_INT) FUNCTION hos_end_ramp(ctx) BIND(C, NAME='hos_end_ramp')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+   END FUNCTION hos_end_ramp
+   INTEGER(C_INT) FUNCTION hos_advance(ctx, t_cur, t_next, h_rk, toler, dt_stab, delt_out, n_acc, n_rej) &
+        BIND(C, NAME='hos_advance')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(INOUT) :: t_cur, h_rk
+      REAL(C_DOUBLE), VALUE :: t_next, toler, dt_stab, delt_out
+      INTEGER(C_INT), INTENT(OUT) :: n_acc, n_rej
+   END FUNCTION hos_advance
+   INTEGER(C_INT) FUNCTION hos_get_physical(ctx, eta, phis, deta, phi_add) BIND(C, NAME='hos_get_physical')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: eta(*), phis(*), deta(*), phi_add(*)
+   END FUNCTION hos_get_physical
+   INTEGER(C_INT) FUNCTION hos_get_wmk(ctx, alpha, pos_top) BIND(C, NAME='hos_get_wmk')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: alpha, pos_top
+   END FUNCTION hos_get_wmk
+   INTEGER(C_INT) FUNCTION hos_get_slope_max(ctx, sx, sy) BIND(C, NAME='hos_get_slope_max')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: sx, sy
+   END FUNCTION hos_get_slope_max
+END INTERFACE
+!
+CONTAINS
+!
+! Map a nonzero status to ERROR STOP with the library's message
+! (the reference STOPs on bad input, e.g. initial_condition.f90:41-59).
+SUBROUTINE hos_check(rc, what)
+   INTEGER(C_INT), INTENT(IN)   :: rc
+   CHARACTER(LEN=*), INTENT(IN) :: what
+   CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+   CHARACTER(LEN=512) :: text
+   INTEGER :: i
+   IF (rc == 0) RETURN
+   CALL C_F_POINTER(hos_last_error(), msg, [512])
+   text = ' '
+   DO i = 1, 512
+      IF (msg(i) == C_NULL_CHAR) EXIT
+      text(i:i) = msg(i)
+   END DO
+   WRITE(*,'(A,A,A)') TRIM(what), ': ', TRIM(text)
+   ERROR STOP 'libhosnwt call failed'
+END SUBROUTINE hos_check
+!
+END MODULE hosnwt_gpu

@@ -39,7 +39,7 @@ class HosStats(C.Structure):
 EXPORTS = [
     "hos_create", "hos_destroy", "hos_last_error", "hos_initialize", "hos_set_state",
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
-    "hos_get_physical", "hos_get_slope_max", "hos_get_stats", "hos_synchronize",
+    "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
     "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform",
 ]
 
@@ -69,6 +69,7 @@ def load() -> C.CDLL:
     lib.hos_rhs.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 11
     lib.hos_get_physical.argtypes = [vp] + [_dp] * 4
     lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
+    lib.hos_get_wmk.argtypes = [vp, _dp, _dp]
     lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
     lib.hos_synchronize.argtypes = [vp]
     lib.hos_time_big_transform.argtypes = [vp, C.c_int, _dp]

@@ -28,6 +28,8 @@
  *                       (main/HOS-NWT.f90:579, IO/vol_energy.f90:69)
  *   hos_get_slope_max : max |eta_x|, |eta_y| read by check_slope
  *                       (main/initial_condition.f90:694-696)
+ *   hos_get_wmk       : alpha and pos1st(n3,1) read by the volume output
+ *                       (IO/output.f90:251-256)
  *   hos_space_2_fourier / hos_fourier_2_space (+ _big):
  *                       SPACE_2_FOURIER / FOURIER_2_SPACE (_BIG) operators
  *                       (fourier/fourier_FFTW.f90:35-63)
@@ -107,6 +109,10 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
 
 int hos_get_physical(hos_ctx* c, double* eta, double* phis, double* deta, double* phi_add);
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy);
+/* Ramp value alpha and paddle-top displacement pos1st(n3,1) of the last
+ * wavemaker evaluation, the module globals output.f90:251-256 subtracts
+ * from the volume (resol_wmkr.f90:1268, wavemaking.f90:277). */
+int hos_get_wmk(hos_ctx* c, double* alpha, double* pos_top);
 int hos_get_stats(hos_ctx* c, hos_stats* s);
 /* Measurement helper (no reference counterpart): average duration in ms and
  * algorithmic HBM bytes of one extended-grid x pass and one y pass of a full

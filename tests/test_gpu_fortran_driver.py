@@ -1,0 +1,103 @@
+"""The Fortran driver (iso_c_binding into libhosnwt) reproduces the oracle's
+volume/energy trace for a reference-format input file.
+
+Results/vol_energy.dat carries 5 significant digits (ES12.5, output.f90:262),
+so the comparison is at 2e-4 relative to the largest energy value.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import make_setup, oracle_params
+from oracle import hos_oracle as ho
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "hos-nwt_amd", "lib", "hos_nwt_gpu")
+
+INPUT = """----------------------- Tank dimensions -----------------------
+Length(m) of the wavetank    :: xlen             :: 20.d0
+Beam(m) of the wavetank      :: ylen             :: 1.d0
+Depth(m) of the wavetank     :: h                :: 1.d0
+------------------------ Select case --------------------------
+Choice of computed case      :: icase            :: 2
+------------------ Sloshing case (icase = 1) ------------------
+Number of the natural mode   :: islosh           :: 3
+Amplitude (m) of the mode    :: aslosh           :: 0.02d0
+--------------- Monochromatic case (icase = 2) ----------------
+Amplitude (m) of wave train  :: amp_mono         :: {amp!r}
+Frequency (Hz) of wave train :: nu_mono          :: {nu!r}
+Angle (deg) from x-axis      :: theta_mono       :: 0.d0
+Phasis (rad) of wave train   :: ph_mono          :: 0.d0
+Directional wmk type         :: ibat             :: 2
+Wave target distance (ibat=3):: xd_mono          :: 18.d0
+-------------------- File case (icase = 3) --------------------
+Name of frequency file       :: file_name        :: none
+Frequency cut-off            :: i_cut            :: 0
+Low cut_off frequency (Hz)   :: nuc_low          :: 0.3d0
+High cut_off frequency (Hz)  :: nuc_high         :: 3.d0
+------------------ Irregular wave (icase=4) -------------------
+Significant wave height (m)  :: Hs               :: 0.05d0
+Peak period (s)              :: Tp               :: 1.d0
+Shape factor (Jonswap)       :: gamma            :: 3.3d0
+Seed number for random numb. :: iseed            :: 513
+-------------------- Wavemaker definition ---------------------
+Nonlinear order of wavemaker :: i_wmk            :: 1
+Type (1: piston, 2: hinged)  :: igeom            :: 1
+Rotation axis distance       :: d_hinge          :: 0.d0
+Time ramp                    :: iramp            :: 1
+Time ramp duration (s)       :: Tramp            :: 4.5d0
+----------------------- Numerical beach -----------------------
+Absorption numerical beach   :: iabsnb           :: 1
+Beginning front num. beach   :: xabsf            :: 0.8d0
+Absorption strength front    :: coeffabsf        :: 1.d0
+------------- Elevation/Velocity-Pressure probes --------------
+Use of probes                :: iprobes          :: 0
+Filename of probe positions  :: pro_file         :: prob.inp
+---------------------- Time-integration -----------------------
+Duration of the simulation   :: T_stop           :: {tstop!r}
+Time tolerance: RK 4(5)      :: toler            :: 1.e-7
+Output frequency             :: f_out            :: {fout!r}
+--------------------------- Output ----------------------------
+Output: 1-dim. ; 0-nondim.   :: idim             :: 0
+free surface plot            :: i3d              :: 0
+modes plot                   :: imodes           :: 0
+wavemaker motion plot        :: iwmk             :: 0
+Swense output 1='yes',0='no' :: i_sw             :: 0
+"""
+
+
+@pytest.mark.gpu
+def test_fortran_driver_matches_oracle(tmp_path):
+    assert os.path.exists(EXE), "build the Fortran driver with hos-nwt_amd/build.sh"
+    T, tstop, n1, M = 1.5, 3.0, 64, 3
+    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T)
+    amp = s.cfg.amp_mono
+    # values written with Python repr are read back exactly by ES25.16
+    text = INPUT.format(amp=amp, nu=1.0 / T, tstop=tstop, fout=2.0 / T)
+    (tmp_path / "input_HOS-NWT.dat").write_text(text)
+    r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    rows = [list(map(float, ln.split())) for ln in (tmp_path / "Results" / "vol_energy.dat").read_text().split("\n")[2:]
+            if ln.strip()]
+    got = np.array(rows)
+    orc = ho.Oracle(oracle_params(s))
+    st = orc.initialize()
+    deta = np.zeros_like(st.eta)
+    t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+    ref = []
+    while t < s.T_stop + s.delt_out:
+        ep, ek, et = orc.energy(st.eta, st.phis + orc.phi_add, deta)
+        ref.append((t, ep, ek, et))
+        if abs(t - s.T_stop) <= 2.2e-16:
+            break
+        tn = min(t + s.delt_out, s.T_stop + s.delt_out)
+        st, t, h, _, _, d = ho.advance(orc, st, t, tn, h, s.cfg.toler, s.dt_stab, s.delt_out)
+        deta = d if d is not None else deta
+    ref = np.array(ref)
+    assert got.shape[0] == ref.shape[0]
+    np.testing.assert_allclose(got[:, 0], ref[:, 0], rtol=1e-4, atol=1e-4)
+    scale = np.abs(ref[:, 3]).max()
+    np.testing.assert_allclose(got[:, 2:5], ref[:, 1:4], rtol=0, atol=2e-4 * scale)
