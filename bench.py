@@ -42,14 +42,30 @@ def make_setup(precision: int, wl=WORKLOAD) -> hc.Setup:
     return hc.build(cfg)
 
 
-def run_steps(tank, s, n, t0, h0):
-    """n attempts with the reference accept/reject rule (HOS-NWT.f90:518-563)."""
+def run_steps(tank, s, n, t0, h0, log=None, replay=None):
+    """n attempts with the reference accept/reject rule (HOS-NWT.f90:518-563).
+
+    ``replay``: list of (h, accept) decisions to repeat instead (same physical times).
+    """
     t, h = t0, h0
     toler, stab = s.cfg.toler, s.dt_stab
     n_acc = n_rej = 0
-    for _ in range(n):
+    for i in range(n):
+        if replay is not None:
+            h, acc = replay[i]
+            tank.runge4_adapt2(t, h)
+            if acc:
+                tank.commit()
+                t += h
+                n_acc += 1
+            else:
+                n_rej += 1
+            continue
         err = tank.runge4_adapt2(t, h)
-        if err > toler and h > 0.001 * 0.95 * stab:
+        acc = not (err > toler and h > 0.001 * 0.95 * stab)
+        if log is not None:
+            log.append((h, acc))
+        if not acc:
             corr = 0.9 * (err / toler) ** -0.25
             n_rej += 1
         else:
@@ -61,24 +77,26 @@ def run_steps(tank, s, n, t0, h0):
     return t, h, n_acc, n_rej
 
 
-def time_path(precision, steps, warmup, spin_up):
+def time_path(precision, steps, warmup, spin_up, replay=None):
     from hosnwt.tank import Tank
     s = make_setup(precision)
     tank = Tank(s)
     tank.initialize()
     h0 = min(0.95 * s.dt_stab, s.delt_out)
-    t, h, _, _ = run_steps(tank, s, spin_up + warmup, 0.0, h0)
+    n0 = spin_up + warmup
+    log = [] if replay is None else None
+    t, h, _, _ = run_steps(tank, s, n0, 0.0, h0, log=log, replay=None if replay is None else replay[:n0])
     tank.synchronize()
     st0 = tank.stats()
     t_start = time.perf_counter()
-    t, h, na, nr = run_steps(tank, s, steps, t, h)
+    t, h, na, nr = run_steps(tank, s, steps, t, h, log=log, replay=None if replay is None else replay[n0:])
     tank.synchronize()
     dt = time.perf_counter() - t_start
     st1 = tank.stats()
     eta = tank.get_state()["eta"]
     out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, n_acc=na, n_rej=nr, t_end=t, eta=eta,
                big_per_step=(st1["n_big"] - st0["n_big"]) / steps, base_per_step=(st1["n_base"] - st0["n_base"]) / steps,
-               big_grid=(st1["big_n1"], st1["big_n2"]), tank=tank, setup=s)
+               big_grid=(st1["big_n1"], st1["big_n2"]), tank=tank, setup=s, log=log)
     return out
 
 
@@ -137,7 +155,7 @@ def main():
     r32 = None
     if not args.fp64_only:
         print("[bench] mixed path", file=sys.stderr, flush=True)
-        r32 = time_path(1, args.steps, args.warmup, args.spin_up)
+        r32 = time_path(1, args.steps, args.warmup, args.spin_up, replay=r64["log"])
         print(f"[bench] mixed {r32['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     eta_err = None
     if r32 is not None:

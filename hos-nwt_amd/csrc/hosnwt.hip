@@ -282,28 +282,36 @@ __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, c
 // wavemaker kernels (double)
 // ---------------------------------------------------------------------------
 // sum over harmonics of pos_ini(i2,f) e^{i w t} (i w)^d, d = 0,1,2, times g(z)
-// (wavemaking.f90:259-304).  pos arrays are (n3, n2).
+// (wavemaking.f90:259-304).  One block per y-node, threads over frequencies.
+// pos arrays are (n3, n2).
 __global__ void k_pos(int n2, int n3, int nf, double t, const double* __restrict__ freq,
                       const double* __restrict__ pre, const double* __restrict__ pim, const double* __restrict__ gz,
                       double* X, double* Xt, double* Xtt) {
-  const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i2 >= n2) return;
+  const int i2 = blockIdx.x;
   double s0 = 0, s1 = 0, s2 = 0;
-  for (int f = 0; f < nf; ++f) {
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
     const double w = 2.0 * M_PI * freq[f];
     double sn, cs;
     sincos(w * t, &sn, &cs);
     const double pr = pre[i2 + (size_t)n2 * f], pi = pim[i2 + (size_t)n2 * f];
     const double rr = pr * cs - pi * sn, ri = pr * sn + pi * cs;  // pos e^{iwt}
     s0 += rr;
-    s1 += -w * ri;       // Re(i w z) = -w Im z
-    s2 += -w * w * rr;   // Re(-w^2 z)
+    s1 += -w * ri;      // Re(i w z) = -w Im z
+    s2 += -w * w * rr;  // Re(-w^2 z)
   }
-  for (int i3 = 0; i3 < n3; ++i3) {
+  __shared__ double tot[3];
+  s0 = block_sum(s0);
+  if (threadIdx.x == 0) tot[0] = s0;
+  s1 = block_sum(s1);
+  if (threadIdx.x == 0) tot[1] = s1;
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) tot[2] = s2;
+  __syncthreads();
+  for (int i3 = threadIdx.x; i3 < n3; i3 += blockDim.x) {
     const size_t o = i3 + (size_t)n3 * i2;
-    X[o] = gz[i3] * s0;
-    Xt[o] = gz[i3] * s1;
-    Xtt[o] = gz[i3] * s2;
+    X[o] = gz[i3] * tot[0];
+    Xt[o] = gz[i3] * tot[1];
+    Xtt[o] = gz[i3] * tot[2];
   }
 }
 
@@ -343,57 +351,24 @@ __global__ void k_paddle_rhs(int n2, int n3, int n3a, double delz, const double*
 }
 
 // space_2_Fourier_add('cos','cos') then / k_add tanh(k_add xlen)
-// (fourier_FFTW.f90:432-447, resol_wmkr.f90:160-171): z REDFT01, y REDFT00.
-// Stage 1: z transform per column (table czt[k*n3a + j] = cos(pi j (2k+1)/(2 n3a))).
-__global__ void k_add_z(int n2, int n3a, const double* __restrict__ czt, const double* in1, const double* in2,
-                        double* out1, double* out2) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i2 = blockIdx.y;
-  if (k >= n3a) return;
-  for (int w = 0; w < 2; ++w) {
-    const double* in = w ? in2 : in1;
-    double* out = w ? out2 : out1;
-    if (!in) continue;
-    const double* col = in + (size_t)n3a * i2;
-    double s = 0.0;
-    for (int j = 1; j < n3a; ++j) s += col[j] * czt[k * n3a + j];
-    out[k + (size_t)n3a * i2] = col[0] + 2.0 * s;
-  }
-}
-// Stage 2: y DCT-I, normalisation and division by k_add tanh(k_add xlen).
-__global__ void k_add_y(int n2, int n3a, double xlen, double piy, const double* __restrict__ cyt, const double* in1,
-                        const double* in2, double* out1, double* out2) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = blockIdx.y;
-  if (k >= n3a) return;
+// (fourier_FFTW.f90:432-447, resol_wmkr.f90:160-171): the z REDFT01 and y
+// REDFT00 are dense products with fixed matrices (rocBLAS); this kernel applies
+// the normalisation s_2_f_add and the division.
+__global__ void k_add_scale(int n2, int n3a, int nfields, double xlen, double piy, const double* in, double* out1,
+                            double* out2) {
+  const size_t na = (size_t)n3a * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= na * nfields) return;
+  const int w = (int)(i / na);
+  const size_t r = i - (size_t)w * na;
+  const int k = (int)(r % n3a), m = (int)(r / n3a);
   const double kxa = (2 * k + 1) * M_PI / (2.0 * L_ADD);
   const double ky = m * piy;
   const double ka = sqrt(kxa * kxa + ky * ky);
   const double div = ka * tanh(ka * xlen);
   double ys2f = 1.0;
   if (n2 > 1) ys2f = ((m == 0 || m == n2 - 1) ? 1.0 : 2.0) / (2.0 * (n2 - 1));
-  const double nrm = ys2f / (double)n3a;
-  for (int w = 0; w < 2; ++w) {
-    const double* in = w ? in2 : in1;
-    double* out = w ? out2 : out1;
-    if (!in) continue;
-    double s;
-    if (n2 == 1) {
-      s = in[k];
-    } else {
-      const int Lc = 2 * (n2 - 1);
-      s = in[k] + ((m & 1) ? -1.0 : 1.0) * in[k + (size_t)n3a * (n2 - 1)];
-      double acc = 0.0;
-      int idx = 0;
-      for (int j = 1; j < n2 - 1; ++j) {
-        idx += m;
-        if (idx >= Lc) idx -= Lc;
-        acc += in[k + (size_t)n3a * j] * cyt[idx];
-      }
-      s += 2.0 * acc;
-    }
-    out[k + (size_t)n3a * m] = s * nrm / div;
-  }
+  (w ? out2 : out1)[r] = in[i] * (ys2f / (double)n3a) / div;
 }
 
 // csh_add_x and k_add_sh_add_x / k (HOS-NWT.f90:364-405) for one (x, k).
@@ -410,49 +385,48 @@ __device__ __forceinline__ void csh_sh(double k, double x, double xlen, double& 
   sh = e1 - e12;
 }
 
-// Additional potential on the free surface, 3-D (resol_wmkr.f90:1149-1175;
-// z = 0 variant: phiadd_1st_SWEET :485-532).  One workgroup per x-node i1,
-// threads over y-nodes.  For each wavemaker mode i3 the y-sum over j2 is a
-// dense cosine / sine sum accumulated from an LDS chunk of
-// a(i3,j2) C(i1,j2,i3); the i3 sum then applies cos / sin(kx_add (eta+1)).
-// Modes whose decay k_add x exceeds kCut contribute below 1e-19 and are skipped.
-constexpr int kPhiCh = 128;
+// Additional potential on the free surface (resol_wmkr.f90:1149-1175) and,
+// from the same sums, its first-order part at z = 0 (phiadd_1st_SWEET
+// :485-532): with B_f(i1,i3,i2) = sum_j2 A_f(i1,j2,i3) cos|sin(ky(j2) y(i2)),
+//   phi  = sum_i3 cos(kx_add (eta+1)) B_phi,  phiz = sum_i3 sin(kx_add (eta+1)) B_z
+// and at z = 0 the same sums with cos(kx_add), sin(kx_add).  Workgroup
+// (i1, i3-chunk), threads over y-nodes; A_f of a y-mode chunk is staged in LDS;
+// cos(ky y) advances by rotation.  Modes with k_add x > kCut contribute below
+// 1e-19 and are skipped.  Partial sums per chunk go to `part`
+// [chunk][field][i1 + n1 i2]; k_phiadd_reduce adds the chunks in order.
+constexpr int kPhiCh = 256;
 constexpr int kPhiMaxPer = 4;
-template <typename T, bool SURF>
+constexpr int kPhiI3 = 8;   // wavemaker modes per workgroup
+constexpr int kPhiF = 7;    // phi, phit, phix, phiy, phiz, phit(z=0), phiz(z=0)
+__device__ __forceinline__ double kx_add_of(int i3) { return (2 * i3 + 1) * M_PI / (2.0 * L_ADD); }
+
+template <typename T>
 __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, double xlen, double piy, double dx,
                                                   const double* __restrict__ a, const double* __restrict__ da,
-                                                  const T* __restrict__ eta, double* o_phi, double* o_phit,
-                                                  double* o_phix, double* o_phiy, double* o_phiz) {
-  extern __shared__ double sm[];
-  const int Lc = 2 * (n2 - 1);
-  double* ctab = sm;
-  double* stab = sm + Lc;
-  double* A = sm + 2 * Lc;  // [5][kPhiCh]
-  const int i1 = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
-  for (int m = tid; m < Lc; m += nt) {
-    ctab[m] = cospi((double)m / (double)(n2 - 1));
-    stab[m] = sinpi((double)m / (double)(n2 - 1));
-  }
+                                                  const T* __restrict__ eta, double* part) {
+  __shared__ double A[5][kPhiCh];
+  const int i1 = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
   const double x = i1 * dx;
-  double out[5][kPhiMaxPer];
+  const int i3a = ch * kPhiI3, i3b = min(n3a, i3a + kPhiI3);
+  if (x * kx_add_of(i3a) > kCut) return;  // the reduction skips this chunk too
+  const size_t ns = (size_t)n1 * n2;
+  double out[kPhiF][kPhiMaxPer];
   double ep1[kPhiMaxPer];
 #pragma unroll
   for (int q = 0; q < kPhiMaxPer; ++q) {
     const int i2 = tid + q * nt;
-    ep1[q] = 1.0;
-    if (SURF && i2 < n2) ep1[q] = (double)eta[i1 + (size_t)n1 * i2] + 1.0;
+    ep1[q] = (i2 < n2) ? (double)eta[i1 + (size_t)n1 * i2] + 1.0 : 1.0;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) out[f][q] = 0.0;
+    for (int f = 0; f < kPhiF; ++f) out[f][q] = 0.0;
   }
-  __syncthreads();
-  for (int i3 = 0; i3 < n3a; ++i3) {
-    const double kxa = (2 * i3 + 1) * M_PI / (2.0 * L_ADD);
+  const double th = 1.0 / (double)(n2 - 1);
+  for (int i3 = i3a; i3 < i3b; ++i3) {
+    const double kxa = kx_add_of(i3);
     if (x * kxa > kCut) break;
     int jmax = n2;
     if (x > 0.0) {
       const double r = kCut / x;
-      const double ky2 = r * r - kxa * kxa;
-      jmax = (piy > 0.0) ? min(n2, (int)floor(sqrt(ky2) / piy) + 1) : n2;
+      jmax = min(n2, (int)floor(sqrt(r * r - kxa * kxa) / piy) + 1);
     }
     double B[5][kPhiMaxPer];
 #pragma unroll
@@ -468,34 +442,38 @@ __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, doubl
         double csh, sh;
         csh_sh(k, x, xlen, csh, sh);
         const double av = a[i3 + (size_t)n3a * j2];
-        const double dav = da[i3 + (size_t)n3a * j2];
-        A[0 * kPhiCh + jj] = av * csh;
-        A[1 * kPhiCh + jj] = dav * csh;
-        A[2 * kPhiCh + jj] = -av * sh * k;
-        A[3 * kPhiCh + jj] = -av * csh * ky;
-        A[4 * kPhiCh + jj] = -av * csh * kxa;
+        A[0][jj] = av * csh;
+        A[1][jj] = da[i3 + (size_t)n3a * j2] * csh;
+        A[2][jj] = -av * sh * k;
+        A[3][jj] = -av * csh * ky;
+        A[4][jj] = -av * csh * kxa;
       }
       __syncthreads();
 #pragma unroll
       for (int q = 0; q < kPhiMaxPer; ++q) {
         const int i2 = tid + q * nt;
         if (i2 >= n2) continue;
-        int idx = (int)(((long)j0 * i2) % Lc);
+        // (c, s) = (cos, sin)(pi j2 i2 / (n2-1)), advanced by the rotation (cr, sr)
+        const double ang = (double)(((long)j0 * i2) % (2 * (n2 - 1))) * th;
+        double c = cospi(ang), sn = sinpi(ang);
+        const double cr = cospi(i2 * th), sr = sinpi(i2 * th);
+        double b0 = B[0][q], b1 = B[1][q], b2 = B[2][q], b3 = B[3][q], b4 = B[4][q];
         for (int jj = 0; jj < jn; ++jj) {
-          const double c = ctab[idx], s = stab[idx];
-          if (SURF) {
-            B[0][q] += A[0 * kPhiCh + jj] * c;
-            B[2][q] += A[2 * kPhiCh + jj] * c;
-            B[3][q] += A[3 * kPhiCh + jj] * s;
-          }
-          B[1][q] += A[1 * kPhiCh + jj] * c;
-          B[4][q] += A[4 * kPhiCh + jj] * c;
-          idx += i2;
-          if (idx >= Lc) idx -= Lc;
+          b0 += A[0][jj] * c;
+          b1 += A[1][jj] * c;
+          b2 += A[2][jj] * c;
+          b3 += A[3][jj] * sn;
+          b4 += A[4][jj] * c;
+          const double cn = c * cr - sn * sr;
+          sn = sn * cr + c * sr;
+          c = cn;
         }
+        B[0][q] = b0; B[1][q] = b1; B[2][q] = b2; B[3][q] = b3; B[4][q] = b4;
       }
       __syncthreads();
     }
+    double s0, c0;
+    sincos(kxa, &s0, &c0);
 #pragma unroll
     for (int q = 0; q < kPhiMaxPer; ++q) {
       double sn, cs;
@@ -505,6 +483,8 @@ __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, doubl
       out[2][q] += cs * B[2][q];
       out[3][q] += cs * B[3][q];
       out[4][q] += sn * B[4][q];
+      out[5][q] += c0 * B[1][q];
+      out[6][q] += s0 * B[4][q];
     }
   }
 #pragma unroll
@@ -512,46 +492,67 @@ __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, doubl
     const int i2 = tid + q * nt;
     if (i2 >= n2) continue;
     const size_t o = i1 + (size_t)n1 * i2;
-    if (SURF) {
-      o_phi[o] = out[0][q];
-      o_phix[o] = out[2][q];
-      o_phiy[o] = out[3][q];
-    }
-    o_phit[o] = out[1][q];
-    o_phiz[o] = out[4][q];
+#pragma unroll
+    for (int f = 0; f < kPhiF; ++f) part[((size_t)ch * kPhiF + f) * ns + o] = out[f][q];
   }
 }
 
-// 2-D tank (n2 = 1): one thread per x-node.
-template <typename T, bool SURF>
+// Sum the active chunks of each x-node in order and write the fields.
+__global__ void k_phiadd_reduce(int n1, int n2, int nch, double dx, const double* __restrict__ part, double* phi,
+                                double* phit, double* phix, double* phiy, double* phiz, double* phit1,
+                                double* phiz1) {
+  const size_t ns = (size_t)n1 * n2;
+  const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (o >= ns) return;
+  const double x = (double)(o % n1) * dx;
+  double acc[kPhiF] = {0, 0, 0, 0, 0, 0, 0};
+  for (int ch = 0; ch < nch; ++ch) {
+    if (x * kx_add_of(ch * kPhiI3) > kCut) break;
+    const double* pc = part + (size_t)ch * kPhiF * ns;
+    for (int f = 0; f < kPhiF; ++f) acc[f] += pc[f * ns + o];
+  }
+  phi[o] = acc[0];
+  phit[o] = acc[1];
+  phix[o] = acc[2];
+  phiy[o] = acc[3];
+  phiz[o] = acc[4];
+  phit1[o] = acc[5];
+  phiz1[o] = acc[6];
+}
+
+// 2-D tank (n2 = 1): one thread per x-node, all seven fields.
+template <typename T>
 __global__ void k_phiadd2d(int n1, int n3a, double xlen, double dx, const double* __restrict__ a,
                            const double* __restrict__ da, const T* __restrict__ eta, double* o_phi, double* o_phit,
-                           double* o_phix, double* o_phiy, double* o_phiz) {
+                           double* o_phix, double* o_phiy, double* o_phiz, double* o_phit1, double* o_phiz1) {
   const int i1 = blockIdx.x * blockDim.x + threadIdx.x;
   if (i1 >= n1) return;
   const double x = i1 * dx;
-  const double ep1 = SURF ? (double)eta[i1] + 1.0 : 1.0;
-  double phi = 0, phit = 0, phix = 0, phiz = 0;
+  const double ep1 = (double)eta[i1] + 1.0;
+  double phi = 0, phit = 0, phix = 0, phiz = 0, phit1 = 0, phiz1 = 0;
   for (int i3 = 0; i3 < n3a; ++i3) {
-    const double kxa = (2 * i3 + 1) * M_PI / (2.0 * L_ADD);
+    const double kxa = kx_add_of(i3);
     if (x * kxa > kCut) break;
     double csh, sh;
     csh_sh(kxa, x, xlen, csh, sh);
-    double sn, cs;
+    double sn, cs, s0, c0;
     sincos(kxa * ep1, &sn, &cs);
+    sincos(kxa, &s0, &c0);
     const double av = a[i3], dav = da[i3];
     phi += cs * (av * csh);
     phit += cs * (dav * csh);
     phix += cs * (-av * sh * kxa);
     phiz += sn * (-av * csh * kxa);
+    phit1 += c0 * (dav * csh);
+    phiz1 += s0 * (-av * csh * kxa);
   }
-  if (SURF) {
-    o_phi[i1] = phi;
-    o_phix[i1] = phix;
-    o_phiy[i1] = 0.0;
-  }
+  o_phi[i1] = phi;
   o_phit[i1] = phit;
+  o_phix[i1] = phix;
+  o_phiy[i1] = 0.0;
   o_phiz[i1] = phiz;
+  o_phit1[i1] = phit1;
+  o_phiz1[i1] = phiz1;
 }
 
 // First-order free surface RHS (resol_wmkr.f90:186-193 via phidphiz :583-636):
@@ -933,7 +934,8 @@ struct Engine : EngineBase {
   std::vector<T*> etapm, phizm, aHOS;
   T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
   // wavemaker / first order (double)
-  double *Xp, *Xpt, *Xptt, *f1, *f2, *fz1, *fz2, *gz, *czt, *cyt, *freq, *pre, *pim;
+  double *Xp, *Xpt, *Xptt, *f1, *f2, *fz1, *fz2, *gz, *Wz, *Wy, *freq, *pre, *pim, *part;
+  int nch = 1;
   double *phi_add, *phix_add, *phiy_add, *phiz_add, *phit_add, *phit1, *phiz1, *w1, *tmp1;
   T *phix_addT, *phiy_addT, *phiz_addT;
   Cont cont;
@@ -969,18 +971,29 @@ struct Engine : EngineBase {
     // wavemaker
     const int n2 = c->n2, n3 = c->n3, n3a = c->n3a;
     Xp = ar.get<double>((size_t)n3 * n2); Xpt = ar.get<double>((size_t)n3 * n2); Xptt = ar.get<double>((size_t)n3 * n2);
-    f1 = ar.get<double>(na); f2 = ar.get<double>(na); fz1 = ar.get<double>(na); fz2 = ar.get<double>(na);
+    f1 = ar.get<double>(2 * na); f2 = f1 + na; fz1 = ar.get<double>(2 * na); fz2 = ar.get<double>(2 * na);
+    nch = (n3a + kPhiI3 - 1) / kPhiI3;
+    part = (n2 > 1) ? ar.get<double>((size_t)nch * kPhiF * ns) : nullptr;
     phi_add = ar.get<double>(ns); phix_add = ar.get<double>(ns); phiy_add = ar.get<double>(ns);
     phiz_add = ar.get<double>(ns); phit_add = ar.get<double>(ns); phit1 = ar.get<double>(ns);
     phiz1 = ar.get<double>(ns); w1 = ar.get<double>(ns); tmp1 = ar.get<double>(ns);
-    std::vector<double> h_gz(n3), h_czt((size_t)n3a * n3a), h_cyt(std::max(1, 2 * (n2 - 1)));
+    // REDFT01 along z and REDFT00 along y as column-major matrices
+    std::vector<double> h_gz(n3), h_wz((size_t)n3a * n3a), h_wy((size_t)n2 * n2);
     for (int i3 = 0; i3 < n3; ++i3) h_gz[i3] = gzed(i3 * c->delz - 1.0, c->p.igeom, c->p.d_hinge);
     for (int k = 0; k < n3a; ++k)
-      for (int j = 0; j < n3a; ++j) h_czt[(size_t)k * n3a + j] = std::cos(M_PI * j * (2 * k + 1) / (2.0 * n3a));
-    for (int m = 0; m < (int)h_cyt.size(); ++m) h_cyt[m] = n2 > 1 ? std::cos(M_PI * m / (double)(n2 - 1)) : 1.0;
+      for (int j = 0; j < n3a; ++j)
+        h_wz[k + (size_t)n3a * j] = (j == 0 ? 1.0 : 2.0) * std::cos(M_PI * j * (2 * k + 1) / (2.0 * n3a));
+    for (int j = 0; j < n2; ++j)
+      for (int m = 0; m < n2; ++m)
+        h_wy[j + (size_t)n2 * m] =
+            n2 > 1 ? ((j == 0 || j == n2 - 1) ? 1.0 : 2.0) * std::cos(M_PI * (double)(((long)j * m) % (2 * (n2 - 1))) /
+                                                                       (double)(n2 - 1))
+                   : 1.0;
     gz = upload(h_gz);
-    czt = upload(h_czt);
-    cyt = upload(h_cyt);
+    Wz = upload(h_wz);
+    Wy = upload(h_wy);
+    blas_ok(rocblas_create_handle(&blas), "create_handle");
+    blas_ok(rocblas_set_stream(blas, c->st), "set_stream");
     const int nf = c->p.n_freq;
     std::vector<double> hf(c->p.freq, c->p.freq + nf);
     std::vector<double> hr((size_t)n2 * nf), hi((size_t)n2 * nf);
@@ -1011,8 +1024,6 @@ struct Engine : EngineBase {
         RB = ar.get<T>((size_t)2 * n1 * n2);
         T2 = upload_t(sin_to_cos_matrix(Nd2, n2, K2));
       }
-      blas_ok(rocblas_create_handle(&blas), "create_handle");
-      blas_ok(rocblas_set_stream(blas, c->st), "set_stream");
     } else {
       PA = RA = T1 = nullptr;
     }
@@ -1193,25 +1204,21 @@ struct Engine : EngineBase {
     ramp_eval(C.p, t, al, alt, altt);
     C.last_alpha = al;
     C.last_pos_top = C.pos_top(t);
-    hipLaunchKernelGGL(k_pos, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
+    hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
                        (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt);
     // f1 = -alpha X_tt - 2 alpha' X_t - alpha'' X ; f2 = -alpha' X_t - alpha'' X
     hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
                        (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2);
-    hipLaunchKernelGGL(k_add_z, dim3(nblk(n3a, 64), n2), dim3(64), 0, st, n2, n3a, (const double*)czt,
-                       (const double*)f1, (const double*)f2, fz1, fz2);
-    hipLaunchKernelGGL(k_add_y, dim3(nblk(n3a, 64), n2), dim3(64), 0, st, n2, n3a, C.p.xlen, C.piy,
-                       (const double*)cyt, (const double*)fz1, (const double*)fz2, kad1, kadr);
-    // first-order potential at z = 0 and the first-order FS RHS
-    phiadd(aad, kad1, (const T*)nullptr, false);
+    paddle_modes(2, kad1, kadr);
+    // additional potential at z = eta and its first-order part at z = 0
+    phiadd(aad, kad1, eta_phys);
+    // first-order free-surface RHS (phidphiz)
     base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
            Dst<double>{w1, n1, 0});
     hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
                        (const double*)phiz1, (const double*)phit1, e1, (const double*)C.nu_d,
                        (double)C.p.iabsnb, ke1, tmp1);
     base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp1, n1, n1, n2), n1, n2, Dst<double>{ka1, n1, 0});
-    // total additional potential at z = eta
-    phiadd(aad, kad1, eta_phys, true);
     // grad phi_add onto the extended grid (resol_HOS.f90:534-544)
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
@@ -1222,27 +1229,42 @@ struct Engine : EngineBase {
     big(KCOS, KCOS, INV, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0});
   }
 
-  void phiadd(const double* a, const double* da, const T* eta, bool surf) {
+  // S2F_add of nfields paddle columns stored contiguously from f1, divided by
+  // k_add tanh(k_add xlen): z transform Wz * F, y transform (.) * Wy.
+  void paddle_modes(int nfields, double* out1, double* out2) {
+    hos_ctx& C = *c;
+    const int n2 = C.n2, n3a = C.n3a;
+    const double one = 1.0, zero = 0.0;
+    blas_ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n3a, nfields * n2, n3a, &one, Wz, n3a,
+                          f1, n3a, &zero, fz1, n3a),
+            "dgemm paddle z");
+    const double* zin = fz1;
+    if (n2 > 1) {
+      blas_ok(rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, n3a, n2, n2, &one,
+                                            fz1, n3a, (rocblas_stride)na, Wy, n2, 0, &zero, fz2, n3a,
+                                            (rocblas_stride)na, nfields),
+              "dgemm paddle y");
+      zin = fz2;
+    }
+    hipLaunchKernelGGL(k_add_scale, dim3(nblk(na * nfields)), dim3(256), 0, C.st, n2, n3a, nfields, C.p.xlen, C.piy,
+                       zin, out1, out2);
+    HIP_OK(hipGetLastError());
+  }
+
+  void phiadd(const double* a, const double* da, const T* eta) {
     hos_ctx& C = *c;
     hipStream_t st = C.st;
     const int n1 = C.n1, n2 = C.n2, n3a = C.n3a;
     if (n2 == 1) {
-      if (surf)
-        hipLaunchKernelGGL((k_phiadd2d<T, true>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx, a,
-                           da, eta, phi_add, phit_add, phix_add, phiy_add, phiz_add);
-      else
-        hipLaunchKernelGGL((k_phiadd2d<T, false>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx,
-                           a, da, eta, (double*)nullptr, phit1, (double*)nullptr, (double*)nullptr, phiz1);
+      hipLaunchKernelGGL((k_phiadd2d<T>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx, a, da, eta,
+                         phi_add, phit_add, phix_add, phiy_add, phiz_add, phit1, phiz1);
     } else {
       int nt = 64;
       while (nt < 256 && nt < n2) nt *= 2;
-      const size_t lds = (size_t)(2 * 2 * (n2 - 1) + 5 * kPhiCh) * sizeof(double);
-      if (surf)
-        hipLaunchKernelGGL((k_phiadd3d<T, true>), dim3(n1), dim3(nt), lds, st, n1, n2, n3a, C.p.xlen, C.piy, C.dx,
-                           a, da, eta, phi_add, phit_add, phix_add, phiy_add, phiz_add);
-      else
-        hipLaunchKernelGGL((k_phiadd3d<T, false>), dim3(n1), dim3(nt), lds, st, n1, n2, n3a, C.p.xlen, C.piy,
-                           C.dx, a, da, eta, (double*)nullptr, phit1, (double*)nullptr, (double*)nullptr, phiz1);
+      hipLaunchKernelGGL((k_phiadd3d<T>), dim3(n1, nch), dim3(nt), 0, st, n1, n2, n3a, C.p.xlen, C.piy, C.dx, a, da,
+                         eta, part);
+      hipLaunchKernelGGL(k_phiadd_reduce, dim3(nblk(ns)), dim3(256), 0, st, n1, n2, nch, C.dx, (const double*)part,
+                         phi_add, phit_add, phix_add, phiy_add, phiz_add, phit1, phiz1);
     }
     HIP_OK(hipGetLastError());
   }
@@ -1547,17 +1569,12 @@ int hos_initialize(hos_ctx* c) {
     double al, alt, altt;
     ramp_eval(c->p, 0.0, al, alt, altt);
     auto run = [&](auto* e) {
-      hipLaunchKernelGGL(k_pos, dim3(nblk(c->n2, 64)), dim3(64), 0, st, c->n2, c->n3, c->p.n_freq, 0.0,
-                         (const double*)e->freq, (const double*)e->pre, (const double*)e->pim, (const double*)e->gz,
-                         e->Xp, e->Xpt, e->Xptt);
+      hipLaunchKernelGGL(k_pos, dim3(c->n2), dim3(256), 0, st, c->n2, c->n3, c->p.n_freq, 0.0, (const double*)e->freq,
+                         (const double*)e->pre, (const double*)e->pim, (const double*)e->gz, e->Xp, e->Xpt, e->Xptt);
       hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(c->n2, 64)), dim3(64), 0, st, c->n2, c->n3, c->n3a, c->delz,
                          (const double*)e->Xp, (const double*)e->Xpt, (const double*)e->Xptt, alt, al, 0.0, 0.0, 0.0,
                          0.0, e->cont, e->f1, (double*)nullptr);
-      hipLaunchKernelGGL(k_add_z, dim3(nblk(c->n3a, 64), c->n2), dim3(64), 0, st, c->n2, c->n3a,
-                         (const double*)e->czt, (const double*)e->f1, (const double*)nullptr, e->fz1, (double*)nullptr);
-      hipLaunchKernelGGL(k_add_y, dim3(nblk(c->n3a, 64), c->n2), dim3(64), 0, st, c->n2, c->n3a, c->p.xlen, c->piy,
-                         (const double*)e->cyt, (const double*)e->fz1, (const double*)nullptr, c->a1st_add,
-                         (double*)nullptr);
+      e->paddle_modes(1, c->a1st_add, (double*)nullptr);
     };
     if (e64) run(e64);
     else run(e32);
