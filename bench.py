@@ -9,7 +9,9 @@ spreading (s = 10) over 256 frequencies x 32 directions, beach on the last 15 %.
 A step is one runge4_adapt2 attempt (5 solveHOS2 evaluations + combination +
 embedded error), accepted or rejected by the reference controller; the state
 stays resident in HBM for the whole timed region.  Both precisions are timed:
-FP64 everywhere, and FP32 transforms/products with FP64 RK state ("mixed").
+FP64 everywhere, and FP32 transforms/products with FP64 RK state ("mixed");
+the mixed path repeats the FP64 path's (h, accept) decisions so that both end
+at the same physical time.
 `value` is the mixed path's steps/s when its eta stays within 1e-5 relative L2
 of the FP64 path after the timed steps, otherwise the FP64 path's.
 """
