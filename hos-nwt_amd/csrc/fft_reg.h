@@ -1,0 +1,255 @@
+// Register-resident batched DCT-I / DST-I line transforms for smooth lengths.
+//
+// Each complex line (two real lines packed as real and imaginary parts, evenly
+// or oddly extended to L = 2(N-1)) is held in registers, V values per thread,
+// across all Stockham passes.  Between passes the values move through LDS, the
+// real parts first and then the imaginary parts, so the LDS footprint is one
+// real value per point (half of a complex buffer) and two workgroups fit on a
+// CU at the extended-grid sizes.  The first pass reads its inputs straight
+// from HBM in the layout it needs (element j + r L/R for butterfly j), and the
+// last pass writes its outputs straight to HBM from the same layout, so HBM is
+// touched once per element in each direction.  Twiddles come from a two-level
+// table in LDS: w^m = T1[m / 64] * T0[m % 64].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.h"
+
+namespace hosfft2 {
+
+using hosfft::C2;
+using hosfft::cmul;
+using hosfft::mk;
+
+constexpr int kMaxPass2 = 8;
+constexpr int hosnwt_kind_cos = 0;
+constexpr int hosnwt_kind_sin = 1;
+
+template <typename T>
+struct Plan2 {
+  int L;
+  int npass;
+  int radix[kMaxPass2];
+  int ntl;              // threads per complex line = L / V
+  int lp;               // LDS stride per line (reals), padded
+  const C2<T>* tw0;     // 64 entries: w^m, m < 64
+  const C2<T>* tw1;     // ceil(L/64) entries: w^(64 m)
+};
+
+__host__ __device__ __forceinline__ int padx(int p) { return p + (p >> 3); }
+
+template <typename T>
+__device__ __forceinline__ C2<T> twiddle(const C2<T>* t0, const C2<T>* t1, int m) {
+  return cmul<T>(t1[m >> 6], t0[m & 63]);
+}
+
+// Butterflies of radix R on the V/R groups held by this thread.
+template <typename T, int V, int R>
+__device__ __forceinline__ void pass_compute(C2<T>* v, int tl, int ntl, int Ns, int L, const C2<T>* t0,
+                                             const C2<T>* t1) {
+  const int twstep = L / (Ns * R);
+#pragma unroll
+  for (int b = 0; b < V / R; ++b) {
+    const int j = tl + b * ntl;
+    const int k = j % Ns;
+    if (Ns > 1) {
+      const int st = twstep * k;
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[b * R + r] = cmul<T>(v[b * R + r], twiddle<T>(t0, t1, r * st));
+    }
+    hosfft::Bfly<T, R>::run(v + b * R);
+  }
+}
+
+// Write this pass's outputs (component c of each value) to LDS at their
+// Stockham positions (j - k) R + k + r Ns.
+template <typename T, int V, int R>
+__device__ __forceinline__ void pass_write(const C2<T>* v, T* line, int tl, int ntl, int Ns, int comp) {
+#pragma unroll
+  for (int b = 0; b < V / R; ++b) {
+    const int j = tl + b * ntl;
+    const int k = j % Ns;
+    const int o = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) line[padx(o + r * Ns)] = comp ? v[b * R + r].y : v[b * R + r].x;
+  }
+}
+
+// Read the next pass's inputs (radix R2): element j + r L/R2 for j = tl + b ntl.
+template <typename T, int V, int R2>
+__device__ __forceinline__ void pass_read(C2<T>* v, const T* line, int tl, int ntl, int L, int comp) {
+  const int stride = L / R2;
+#pragma unroll
+  for (int b = 0; b < V / R2; ++b) {
+    const int j = tl + b * ntl;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) {
+      const T x = line[padx(j + r * stride)];
+      if (comp)
+        v[b * R2 + r].y = x;
+      else
+        v[b * R2 + r].x = x;
+    }
+  }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void dispatch_compute(int R, C2<T>* v, int tl, int ntl, int Ns, int L, const C2<T>* t0,
+                                                 const C2<T>* t1) {
+  switch (R) {
+    case 8: if constexpr (V % 8 == 0) pass_compute<T, V, 8>(v, tl, ntl, Ns, L, t0, t1); break;
+    case 4: if constexpr (V % 4 == 0) pass_compute<T, V, 4>(v, tl, ntl, Ns, L, t0, t1); break;
+    case 2: if constexpr (V % 2 == 0) pass_compute<T, V, 2>(v, tl, ntl, Ns, L, t0, t1); break;
+    case 3: if constexpr (V % 3 == 0) pass_compute<T, V, 3>(v, tl, ntl, Ns, L, t0, t1); break;
+    default: break;
+  }
+}
+template <typename T, int V>
+__device__ __forceinline__ void dispatch_write(int R, const C2<T>* v, T* line, int tl, int ntl, int Ns, int comp) {
+  switch (R) {
+    case 8: if constexpr (V % 8 == 0) pass_write<T, V, 8>(v, line, tl, ntl, Ns, comp); break;
+    case 4: if constexpr (V % 4 == 0) pass_write<T, V, 4>(v, line, tl, ntl, Ns, comp); break;
+    case 2: if constexpr (V % 2 == 0) pass_write<T, V, 2>(v, line, tl, ntl, Ns, comp); break;
+    case 3: if constexpr (V % 3 == 0) pass_write<T, V, 3>(v, line, tl, ntl, Ns, comp); break;
+    default: break;
+  }
+}
+template <typename T, int V>
+__device__ __forceinline__ void dispatch_read(int R2, C2<T>* v, const T* line, int tl, int ntl, int L, int comp) {
+  switch (R2) {
+    case 8: if constexpr (V % 8 == 0) pass_read<T, V, 8>(v, line, tl, ntl, L, comp); break;
+    case 4: if constexpr (V % 4 == 0) pass_read<T, V, 4>(v, line, tl, ntl, L, comp); break;
+    case 2: if constexpr (V % 2 == 0) pass_read<T, V, 2>(v, line, tl, ntl, L, comp); break;
+    case 3: if constexpr (V % 3 == 0) pass_read<T, V, 3>(v, line, tl, ntl, L, comp); break;
+    default: break;
+  }
+}
+
+// Run all passes on the values in v (first-pass layout on entry, last-pass
+// layout on exit: value b*R_last + r sits at position tl + b ntl + r L/R_last).
+// `line` is this line's LDS region; every thread of the workgroup must call.
+template <typename T, int V>
+__device__ __forceinline__ void fft_reg(C2<T>* v, T* line, int tl, const Plan2<T>& pl, const C2<T>* t0,
+                                        const C2<T>* t1) {
+  int Ns = 1;
+  for (int p = 0; p < pl.npass; ++p) {
+    const int R = pl.radix[p];
+    dispatch_compute<T, V>(R, v, tl, pl.ntl, Ns, pl.L, t0, t1);
+    if (p + 1 < pl.npass) {
+      const int R2 = pl.radix[p + 1];
+      dispatch_write<T, V>(R, v, line, tl, pl.ntl, Ns, 0);
+      __syncthreads();
+      dispatch_read<T, V>(R2, v, line, tl, pl.ntl, pl.L, 0);
+      __syncthreads();
+      dispatch_write<T, V>(R, v, line, tl, pl.ntl, Ns, 1);
+      __syncthreads();
+      dispatch_read<T, V>(R2, v, line, tl, pl.ntl, pl.L, 1);
+      __syncthreads();
+    }
+    Ns *= R;
+  }
+}
+
+// Position of value i (= b*R + r) in the layout of radix R.
+__device__ __forceinline__ int layout_pos(int i, int R, int tl, int ntl, int L) {
+  const int b = i / R, r = i - (i / R) * R;
+  return tl + b * ntl + r * (L / R);
+}
+
+// Line kernel.  YP = false: x pass, one or more complex lines per workgroup,
+// threads of a line contiguous; each complex line packs real lines 2q, 2q+1.
+// YP = true: y pass over TX = 2*ncl adjacent columns, lines interleaved across
+// threads so that a row segment of TX columns is one coalesced access.
+template <typename T, int V, bool YP, class S, class D>
+__global__ void __launch_bounds__(512) r2r_reg_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines,
+                                                      int ncl, S src, D dst, Plan2<T> pl) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  C2<T>* t0 = reinterpret_cast<C2<T>*>(lds + (size_t)ncl * pl.lp);
+  C2<T>* t1 = t0 + 64;
+  const int tid = threadIdx.x;
+  const int ntl = pl.ntl, L = pl.L;
+  int cl, tl;
+  if (YP) {
+    cl = tid % ncl;
+    tl = tid / ncl;
+  } else {
+    cl = tid / ntl;
+    tl = tid - cl * ntl;
+  }
+  // workgroup -> group of complex lines; y pass: XCD-aware order so that
+  // neighbouring column tiles share an L2
+  int grp = blockIdx.x;
+  if (YP) {
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) grp = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+  }
+  const int la = 2 * (grp * ncl + cl), lb = la + 1;  // real lines of this complex line
+  const bool ha = la < nlines, hb = lb < nlines;
+  // load the evenly / oddly extended line straight into the first-pass layout
+  C2<T> v[V];
+  const int R0 = pl.radix[0];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int p = layout_pos(i, R0, tl, ntl, L);
+    int e = p;
+    T sg = (T)1;
+    if (p >= N) {
+      e = L - p;
+      if (kind == hosnwt_kind_sin) sg = (T)-1;
+    }
+    T a = (T)0, b = (T)0;
+    const bool edge = (e == 0 || e == N - 1);
+    if (e < n_in && !(kind == hosnwt_kind_sin && edge)) {
+      if (YP) {
+        if (ha) a = src.get(la, e);
+        if (hb) b = src.get(lb, e);
+      } else {
+        if (ha) a = src.get(e, la);
+        if (hb) b = src.get(e, lb);
+      }
+      if (dir < 0) {
+        const T f = edge ? (T)1 : (T)0.5;
+        a *= f;
+        b *= f;
+      }
+    }
+    v[i] = mk<T>(sg * a, sg * b);
+  }
+  for (int m = tid; m < 64; m += blockDim.x) t0[m] = pl.tw0[m];
+  for (int m = tid; m < (L + 63) / 64; m += blockDim.x) t1[m] = pl.tw1[m];
+  __syncthreads();
+  fft_reg<T, V>(v, lds + (size_t)cl * pl.lp, tl, pl, t0, t1);
+  // store straight from the last-pass layout
+  const int Rl = pl.radix[pl.npass - 1];
+  const T inv = (T)(1.0 / (double)L);
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int k = layout_pos(i, Rl, tl, ntl, L);
+    if (k >= n_out) continue;
+    const C2<T> Z = v[i];
+    T A, B;
+    if (kind == hosnwt_kind_cos) {
+      A = Z.x;
+      B = Z.y;
+    } else {
+      A = -Z.y;
+      B = Z.x;
+      if (k == 0 || k == N - 1) A = B = (T)0;
+    }
+    if (dir > 0) {
+      const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
+      A *= f;
+      B *= f;
+    }
+    if (YP) {
+      if (ha) dst.put(la, k, A);
+      if (hb) dst.put(lb, k, B);
+    } else {
+      if (ha) dst.put(k, la, A);
+      if (hb) dst.put(k, lb, B);
+    }
+  }
+}
+
+}  // namespace hosfft2

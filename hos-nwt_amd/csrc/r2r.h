@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "fft_lds.h"
+#include "fft_reg.h"
 
 namespace hosnwt {
 
@@ -259,8 +260,48 @@ inline void host_fft_pow2(std::vector<std::complex<double>>& a) {
 template <typename T>
 struct LinePlan {
   FftPlan<T> dev{};
+  hosfft2::Plan2<T> dev2{};
+  int V2 = 0;  // values per thread of the register-resident engine (0: not usable)
   std::vector<void*> allocs;
+  // Register-resident plan: lengths 2^a 3^b with radices 8/4/2/3 dividing V.
+  void build2(int L) {
+    long r = L;
+    int a = 0, b = 0;
+    while (r % 2 == 0) { r /= 2; ++a; }
+    while (r % 3 == 0) { r /= 3; ++b; }
+    if (r != 1) return;
+    const int V = b ? 24 : 8;
+    if (L % V != 0 || L / V < 8 || L / V > 512) return;
+    int np = 0;
+    long q = L;
+    while (q % 8 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 8; q /= 8; }
+    while (q % 4 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 4; q /= 4; }
+    while (q % 2 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 2; q /= 2; }
+    while (q % 3 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 3; q /= 3; }
+    if (q != 1) return;
+    dev2.L = L;
+    dev2.npass = np;
+    dev2.ntl = L / V;
+    int lp = hosfft2::padx(L - 1) + 1;
+    lp += lp & 1;
+    dev2.lp = lp;
+    std::vector<C2<T>> t0(64), t1((L + 63) / 64);
+    for (int m = 0; m < 64; ++m) {
+      const double ang = -2.0 * M_PI * (double)m / (double)L;
+      t0[m].x = (T)std::cos(ang);
+      t0[m].y = (T)std::sin(ang);
+    }
+    for (int m = 0; m < (L + 63) / 64; ++m) {
+      const double ang = -2.0 * M_PI * (double)(64L * m % L) / (double)L;
+      t1[m].x = (T)std::cos(ang);
+      t1[m].y = (T)std::sin(ang);
+    }
+    dev2.tw0 = upload(t0);
+    dev2.tw1 = upload(t1);
+    V2 = V;
+  }
   void build(int Lorig) {
+    build2(Lorig);
     dev.Lorig = Lorig;
     long L = Lorig;
     dev.bluestein = smooth235(L) ? 0 : 1;
@@ -338,6 +379,11 @@ struct Grid2 {
     TX = sizeof(T) == 8 ? 8 : 16;
     // keep the y tile within the LDS budget
     while (TX > 2 && (size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 && N2 > 1) TX /= 2;
+    if (N2 > 1 && py.V2) {
+      TX = sizeof(T) == 8 ? 8 : 16;
+      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 64 * 1024 || (TX / 2) * py.dev2.ntl > 512))
+        TX /= 2;
+    }
   }
   void release() {
     px.release();
@@ -365,10 +411,35 @@ inline void allow_lds(K kernel, size_t bytes) {
   }
 }
 
+template <typename T, bool YP, class S, class D>
+void launch_reg(const LinePlan<T>& pl, int ncl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
+                const D& d, hipStream_t st) {
+  const size_t lds = (size_t)ncl * pl.dev2.lp * sizeof(T) + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
+  const int nt = ncl * pl.dev2.ntl;
+  int groups = (nlines + 2 * ncl - 1) / (2 * ncl);
+  if (YP) groups = (groups + 7) / 8 * 8;  // XCD-aware tile order needs a multiple of 8
+  if (pl.V2 == 24) {
+    auto kern = hosfft2::r2r_reg_kernel<T, 24, YP, S, D>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(groups), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, ncl, s, d, pl.dev2);
+  } else {
+    auto kern = hosfft2::r2r_reg_kernel<T, 8, YP, S, D>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3(groups), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, ncl, s, d, pl.dev2);
+  }
+  HIP_OK(hipGetLastError());
+}
+
 template <typename T, class S, class D>
 void launch_x(const LinePlan<T>& pl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
               const D& d, hipStream_t st) {
   if (nlines <= 0 || n_out <= 0) return;
+  if (pl.V2) {
+    int ncl = 1;
+    while (2 * ncl * pl.dev2.ntl <= 256 && 2 * ncl <= (nlines + 1) / 2) ncl *= 2;
+    launch_reg<T, false>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st);
+    return;
+  }
   const size_t lds = (size_t)pl.dev.L * sizeof(C2<T>);
   if (lds > kLdsMax) throw std::runtime_error("x line exceeds LDS: N=" + std::to_string(N));
   const int nt = pick_threads(pl.dev.L);
@@ -383,6 +454,10 @@ template <typename T, class S, class D>
 void launch_y(const LinePlan<T>& pl, int TX, int N, int kind, int dir, int n_in, int n_out, int ncols, const S& s,
               const D& d, hipStream_t st) {
   if (ncols <= 0 || n_out <= 0) return;
+  if (pl.V2) {
+    launch_reg<T, true>(pl, TX / 2, N, kind, dir, n_in, n_out, ncols, s, d, st);
+    return;
+  }
   const size_t lds = (size_t)(TX / 2) * pl.dev.L * sizeof(C2<T>);
   if (lds > kLdsMax) throw std::runtime_error("y tile exceeds LDS: N=" + std::to_string(N));
   const int nt = pick_threads((long)(TX / 2) * pl.dev.L);
