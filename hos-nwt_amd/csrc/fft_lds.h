@@ -177,9 +177,13 @@ __device__ __forceinline__ void stockham_pass(C2<T>* buf, int L, int Ns, int nli
       const int j = idx - line * nb;
       const int k = j % Ns;
       if (Ns > 1) {
-        const int st = twstep * k;
+        const C2<T> w1 = tw[twstep * k];
+        C2<T> w = w1;
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[b][r] = cmul<T>(v[b][r], tw[r * st]);
+        for (int r = 1; r < R; ++r) {
+          v[b][r] = cmul<T>(v[b][r], w);
+          if (r + 1 < R) w = cmul<T>(w, w1);
+        }
       }
       Bfly<T, R>::run(v[b]);
       C2<T>* dst = buf + line * lstride + (j - k) * R + k;

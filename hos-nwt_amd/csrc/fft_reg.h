@@ -53,9 +53,14 @@ __device__ __forceinline__ void pass_compute(C2<T>* v, int tl, int ntl, int Ns, 
     const int j = tl + b * ntl;
     const int k = j % Ns;
     if (Ns > 1) {
-      const int st = twstep * k;
+      // one table lookup per butterfly; higher powers by repeated products
+      const C2<T> w1 = twiddle<T>(t0, t1, twstep * k);
+      C2<T> w = w1;
 #pragma unroll
-      for (int r = 1; r < R; ++r) v[b * R + r] = cmul<T>(v[b * R + r], twiddle<T>(t0, t1, r * st));
+      for (int r = 1; r < R; ++r) {
+        v[b * R + r] = cmul<T>(v[b * R + r], w);
+        if (r + 1 < R) w = cmul<T>(w, w1);
+      }
     }
     hosfft::Bfly<T, R>::run(v + b * R);
   }
@@ -151,9 +156,74 @@ __device__ __forceinline__ void fft_reg(C2<T>* v, T* line, int tl, const Plan2<T
 }
 
 // Position of value i (= b*R + r) in the layout of radix R.
-__device__ __forceinline__ int layout_pos(int i, int R, int tl, int ntl, int L) {
+template <int R>
+__device__ __forceinline__ int layout_pos(int i, int tl, int ntl, int L) {
   const int b = i / R, r = i - (i / R) * R;
   return tl + b * ntl + r * (L / R);
+}
+
+template <typename T, int V, int R, bool YP, class S>
+__device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int ntl, int L, int N, int kind, int dir,
+                                          int n_in, int la, int lb, bool ha, bool hb) {
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int p = layout_pos<R>(i, tl, ntl, L);
+    int e = p;
+    T sg = (T)1;
+    if (p >= N) {
+      e = L - p;
+      if (kind == hosnwt_kind_sin) sg = (T)-1;
+    }
+    T a = (T)0, b = (T)0;
+    const bool edge = (e == 0 || e == N - 1);
+    if (e < n_in && !(kind == hosnwt_kind_sin && edge)) {
+      if (YP) {
+        if (ha) a = src.get(la, e);
+        if (hb) b = src.get(lb, e);
+      } else {
+        if (ha) a = src.get(e, la);
+        if (hb) b = src.get(e, lb);
+      }
+      if (dir < 0 && !edge) {
+        a *= (T)0.5;
+        b *= (T)0.5;
+      }
+    }
+    v[i] = mk<T>(sg * a, sg * b);
+  }
+}
+
+template <typename T, int V, int R, bool YP, class D>
+__device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl, int ntl, int L, int N, int kind,
+                                           int dir, int n_out, int la, int lb, bool ha, bool hb) {
+  const T inv = (T)(1.0 / (double)L);
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int k = layout_pos<R>(i, tl, ntl, L);
+    if (k >= n_out) continue;
+    const C2<T> Z = v[i];
+    T A, B;
+    if (kind == hosnwt_kind_cos) {
+      A = Z.x;
+      B = Z.y;
+    } else {
+      A = -Z.y;
+      B = Z.x;
+      if (k == 0 || k == N - 1) A = B = (T)0;
+    }
+    if (dir > 0) {
+      const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
+      A *= f;
+      B *= f;
+    }
+    if (YP) {
+      if (ha) dst.put(la, k, A);
+      if (hb) dst.put(lb, k, B);
+    } else {
+      if (ha) dst.put(k, la, A);
+      if (hb) dst.put(k, lb, B);
+    }
+  }
 }
 
 // Line kernel.  YP = false: x pass, one or more complex lines per workgroup,
@@ -161,7 +231,7 @@ __device__ __forceinline__ int layout_pos(int i, int R, int tl, int ntl, int L) 
 // YP = true: y pass over TX = 2*ncl adjacent columns, lines interleaved across
 // threads so that a row segment of TX columns is one coalesced access.
 template <typename T, int V, bool YP, class S, class D>
-__global__ void __launch_bounds__(512) r2r_reg_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines,
+__global__ void __launch_bounds__(256) r2r_reg_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines,
                                                       int ncl, S src, D dst, Plan2<T> pl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
@@ -188,67 +258,24 @@ __global__ void __launch_bounds__(512) r2r_reg_kernel(int N, int kind, int dir, 
   const bool ha = la < nlines, hb = lb < nlines;
   // load the evenly / oddly extended line straight into the first-pass layout
   C2<T> v[V];
-  const int R0 = pl.radix[0];
-#pragma unroll
-  for (int i = 0; i < V; ++i) {
-    const int p = layout_pos(i, R0, tl, ntl, L);
-    int e = p;
-    T sg = (T)1;
-    if (p >= N) {
-      e = L - p;
-      if (kind == hosnwt_kind_sin) sg = (T)-1;
-    }
-    T a = (T)0, b = (T)0;
-    const bool edge = (e == 0 || e == N - 1);
-    if (e < n_in && !(kind == hosnwt_kind_sin && edge)) {
-      if (YP) {
-        if (ha) a = src.get(la, e);
-        if (hb) b = src.get(lb, e);
-      } else {
-        if (ha) a = src.get(e, la);
-        if (hb) b = src.get(e, lb);
-      }
-      if (dir < 0) {
-        const T f = edge ? (T)1 : (T)0.5;
-        a *= f;
-        b *= f;
-      }
-    }
-    v[i] = mk<T>(sg * a, sg * b);
+  switch (pl.radix[0]) {
+    case 8: if constexpr (V % 8 == 0) load_line<T, V, 8, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
+    case 4: if constexpr (V % 4 == 0) load_line<T, V, 4, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
+    case 2: if constexpr (V % 2 == 0) load_line<T, V, 2, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
+    case 3: if constexpr (V % 3 == 0) load_line<T, V, 3, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
+    default: break;
   }
   for (int m = tid; m < 64; m += blockDim.x) t0[m] = pl.tw0[m];
   for (int m = tid; m < (L + 63) / 64; m += blockDim.x) t1[m] = pl.tw1[m];
   __syncthreads();
   fft_reg<T, V>(v, lds + (size_t)cl * pl.lp, tl, pl, t0, t1);
   // store straight from the last-pass layout
-  const int Rl = pl.radix[pl.npass - 1];
-  const T inv = (T)(1.0 / (double)L);
-#pragma unroll
-  for (int i = 0; i < V; ++i) {
-    const int k = layout_pos(i, Rl, tl, ntl, L);
-    if (k >= n_out) continue;
-    const C2<T> Z = v[i];
-    T A, B;
-    if (kind == hosnwt_kind_cos) {
-      A = Z.x;
-      B = Z.y;
-    } else {
-      A = -Z.y;
-      B = Z.x;
-      if (k == 0 || k == N - 1) A = B = (T)0;
-    }
-    if (dir > 0) {
-      const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
-      A *= f;
-      B *= f;
-    }
-    if (YP) {
-      if (ha) dst.put(la, k, A);
-      if (hb) dst.put(lb, k, B);
-    } else {
-      if (ha) dst.put(k, la, A);
-      if (hb) dst.put(k, lb, B);
-    }
+  switch (pl.radix[pl.npass - 1]) {
+    case 8: if constexpr (V % 8 == 0) store_line<T, V, 8, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
+    case 4: if constexpr (V % 4 == 0) store_line<T, V, 4, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
+    case 2: if constexpr (V % 2 == 0) store_line<T, V, 2, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
+    case 3: if constexpr (V % 3 == 0) store_line<T, V, 3, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
+    default: break;
   }
 }
 

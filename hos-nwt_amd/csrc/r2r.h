@@ -131,7 +131,7 @@ __device__ __forceinline__ T f2s_fac(int j, int N) { return (j == 0 || j == N - 
 
 // x pass: workgroup `pair` transforms lines 2*pair and 2*pair+1 (< nlines).
 template <typename T, class S, class D>
-__global__ void r2r_x_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines, S src, D dst,
+__global__ void __launch_bounds__(256) r2r_x_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines, S src, D dst,
                              FftPlan<T> pl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
@@ -181,7 +181,7 @@ __global__ void r2r_x_kernel(int N, int kind, int dir, int n_in, int n_out, int 
 
 // y pass: workgroup handles columns [c0, c0+TX) (< ncols), packed in pairs.
 template <typename T, class S, class D>
-__global__ void r2r_y_kernel(int N, int kind, int dir, int n_in, int n_out, int ncols, int TX, S src, D dst,
+__global__ void __launch_bounds__(256) r2r_y_kernel(int N, int kind, int dir, int n_in, int n_out, int ncols, int TX, S src, D dst,
                              FftPlan<T> pl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C2<T>* buf = reinterpret_cast<C2<T>*>(smem_raw);
@@ -271,7 +271,7 @@ struct LinePlan {
     while (r % 3 == 0) { r /= 3; ++b; }
     if (r != 1) return;
     const int V = b ? 24 : 8;
-    if (L % V != 0 || L / V < 8 || L / V > 512) return;
+    if (L % V != 0 || L / V < 8 || L / V > 256) return;
     int np = 0;
     long q = L;
     while (q % 8 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 8; q /= 8; }
@@ -378,10 +378,12 @@ struct Grid2 {
     if (N2 > 1) py.build(2 * (N2 - 1));
     TX = sizeof(T) == 8 ? 8 : 16;
     // keep the y tile within the LDS budget
-    while (TX > 2 && (size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 && N2 > 1) TX /= 2;
+    while (TX > 2 && N2 > 1 &&
+           ((size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 || (TX / 2) * py.dev.L > 256 * hosfft::kValsPerThread))
+      TX /= 2;
     if (N2 > 1 && py.V2) {
       TX = sizeof(T) == 8 ? 8 : 16;
-      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 64 * 1024 || (TX / 2) * py.dev2.ntl > 512))
+      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 64 * 1024 || (TX / 2) * py.dev2.ntl > 256))
         TX /= 2;
     }
   }
@@ -395,7 +397,7 @@ inline int pick_threads(long vals) {
   long nt = (vals + hosfft::kValsPerThread - 1) / hosfft::kValsPerThread;
   nt = ((nt + 63) / 64) * 64;
   if (nt < 64) nt = 64;
-  if (nt > 1024) throw std::runtime_error("line batch too large for one workgroup");
+  if (nt > 256) throw std::runtime_error("line batch too large for one workgroup");
   return (int)nt;
 }
 
