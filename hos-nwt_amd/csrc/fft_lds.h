@@ -42,8 +42,8 @@ template <typename T>
 __device__ __forceinline__ C2<T> cscale(C2<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
 
 constexpr int kMaxPass = 16;
-// Upper bound on complex values a thread stages per pass; the host sizes the
-// workgroup so that nlines * L <= kValsPerThread * blockDim.x.
+// Complex values a thread stages per pass (VPT template parameter): the host
+// sizes the workgroup so that nlines * L <= VPT * blockDim.x.
 constexpr int kValsPerThread = 16;
 
 // Device-side plan of one line length.  For Bluestein plans `L` is the
@@ -149,10 +149,10 @@ struct Bfly<T, 5> {
 // One Stockham pass of radix R on `nlines` lines of length L.
 // Inputs of butterfly j sit at j + r*(L/R); outputs go to
 // (j/Ns)*Ns*R + (j%Ns) + r*Ns, after twiddling input r by W_{Ns R}^{r (j%Ns)}.
-template <typename T, int R>
+template <typename T, int R, int VPT>
 __device__ __forceinline__ void stockham_pass(C2<T>* buf, int L, int Ns, int nlines, int lstride,
                                               const C2<T>* __restrict__ tw, int tid, int nt) {
-  constexpr int BMAX = (kValsPerThread + R - 1) / R;
+  constexpr int BMAX = (VPT + R - 1) / R;
   const int nb = L / R;
   const int total = nb * nlines;
   const int twstep = L / (Ns * R);
@@ -195,17 +195,17 @@ __device__ __forceinline__ void stockham_pass(C2<T>* buf, int L, int Ns, int nli
 }
 
 // Smooth-length FFT of all lines (in place, natural order out).
-template <typename T>
+template <typename T, int VPT>
 __device__ void fft_smooth(C2<T>* buf, int nlines, int lstride, const FftPlan<T>& pl, int tid, int nt) {
   int Ns = 1;
   for (int p = 0; p < pl.npass; ++p) {
     const int R = pl.radix[p];
     switch (R) {
-      case 8: stockham_pass<T, 8>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
-      case 4: stockham_pass<T, 4>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
-      case 2: stockham_pass<T, 2>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
-      case 3: stockham_pass<T, 3>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
-      case 5: stockham_pass<T, 5>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
+      case 8: stockham_pass<T, 8, VPT>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
+      case 4: stockham_pass<T, 4, VPT>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
+      case 2: stockham_pass<T, 2, VPT>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
+      case 3: stockham_pass<T, 3, VPT>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
+      case 5: stockham_pass<T, 5, VPT>(buf, pl.L, Ns, nlines, lstride, pl.tw, tid, nt); break;
       default: break;
     }
     Ns *= R;
@@ -215,10 +215,10 @@ __device__ void fft_smooth(C2<T>* buf, int nlines, int lstride, const FftPlan<T>
 // FFT of logical length pl.Lorig.  On entry each line holds Lorig values
 // (lstride >= pl.L); on exit it holds the Lorig DFT outputs.  Must be called
 // by the whole workgroup after a barrier.
-template <typename T>
+template <typename T, int VPT>
 __device__ void fft_lines(C2<T>* buf, int nlines, int lstride, const FftPlan<T>& pl, int tid, int nt) {
   if (!pl.bluestein) {
-    fft_smooth<T>(buf, nlines, lstride, pl, tid, nt);
+    fft_smooth<T, VPT>(buf, nlines, lstride, pl, tid, nt);
     return;
   }
   const int P = pl.L, Lo = pl.Lorig;
@@ -228,14 +228,14 @@ __device__ void fft_lines(C2<T>* buf, int nlines, int lstride, const FftPlan<T>&
     *e = (j < Lo) ? cmul<T>(*e, pl.chirp[j]) : mk<T>((T)0, (T)0);
   }
   __syncthreads();
-  fft_smooth<T>(buf, nlines, lstride, pl, tid, nt);
+  fft_smooth<T, VPT>(buf, nlines, lstride, pl, tid, nt);
   for (int idx = tid; idx < nlines * P; idx += nt) {
     const int line = idx / P, j = idx - line * P;
     C2<T>* e = buf + line * lstride + j;
     *e = cconj<T>(cmul<T>(*e, pl.bhat[j]));
   }
   __syncthreads();
-  fft_smooth<T>(buf, nlines, lstride, pl, tid, nt);
+  fft_smooth<T, VPT>(buf, nlines, lstride, pl, tid, nt);
   for (int idx = tid; idx < nlines * Lo; idx += nt) {
     const int line = idx / Lo, j = idx - line * Lo;
     C2<T>* e = buf + line * lstride + j;

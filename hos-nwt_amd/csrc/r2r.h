@@ -130,7 +130,7 @@ template <typename T>
 __device__ __forceinline__ T f2s_fac(int j, int N) { return (j == 0 || j == N - 1) ? (T)1 : (T)0.5; }
 
 // x pass: workgroup `pair` transforms lines 2*pair and 2*pair+1 (< nlines).
-template <typename T, class S, class D>
+template <typename T, int VPT, class S, class D>
 __global__ void __launch_bounds__(256) r2r_x_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines, S src, D dst,
                              FftPlan<T> pl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) r2r_x_kernel(int N, int kind, int dir, in
     if (j >= 1 && j <= N - 2) buf[L - j] = (kind == KCOS) ? z : hosfft::mk<T>(-z.x, -z.y);
   }
   __syncthreads();
-  hosfft::fft_lines<T>(buf, 1, pl.L, pl, tid, nt);
+  hosfft::fft_lines<T, VPT>(buf, 1, pl.L, pl, tid, nt);
   const T inv = (T)(1.0 / (double)L);
   for (int k = tid; k < n_out; k += nt) {
     const C2<T> Z = buf[k];
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(256) r2r_x_kernel(int N, int kind, int dir, in
 }
 
 // y pass: workgroup handles columns [c0, c0+TX) (< ncols), packed in pairs.
-template <typename T, class S, class D>
+template <typename T, int VPT, class S, class D>
 __global__ void __launch_bounds__(256) r2r_y_kernel(int N, int kind, int dir, int n_in, int n_out, int ncols, int TX, S src, D dst,
                              FftPlan<T> pl) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(256) r2r_y_kernel(int N, int kind, int dir, in
     buf[q * ls + L - j] = (kind == KCOS) ? z : hosfft::mk<T>(-z.x, -z.y);
   }
   __syncthreads();
-  hosfft::fft_lines<T>(buf, npairs, ls, pl, tid, nt);
+  hosfft::fft_lines<T, VPT>(buf, npairs, ls, pl, tid, nt);
   const T inv = (T)(1.0 / (double)L);
   for (int idx = tid; idx < TX * n_out; idx += nt) {
     const int c = idx % TX, k = idx / TX;
@@ -393,8 +393,9 @@ struct Grid2 {
   }
 };
 
-inline int pick_threads(long vals) {
-  long nt = (vals + hosfft::kValsPerThread - 1) / hosfft::kValsPerThread;
+// Threads for `vals` complex values per workgroup at `vpt` values per thread.
+inline int pick_threads(long vals, int vpt = hosfft::kValsPerThread) {
+  long nt = (vals + vpt - 1) / vpt;
   nt = ((nt + 63) / 64) * 64;
   if (nt < 64) nt = 64;
   if (nt > 256) throw std::runtime_error("line batch too large for one workgroup");
@@ -444,11 +445,19 @@ void launch_x(const LinePlan<T>& pl, int N, int kind, int dir, int n_in, int n_o
   }
   const size_t lds = (size_t)pl.dev.L * sizeof(C2<T>);
   if (lds > kLdsMax) throw std::runtime_error("x line exceeds LDS: N=" + std::to_string(N));
-  const int nt = pick_threads(pl.dev.L);
-  auto kern = r2r_x_kernel<T, S, D>;
-  allow_lds(kern, lds);
-  hipLaunchKernelGGL(kern, dim3((nlines + 1) / 2), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, s, d,
-                     pl.dev);
+  if (pl.dev.L <= 256 * 16) {
+    const int nt = pick_threads(pl.dev.L, 16);
+    auto kern = r2r_x_kernel<T, 16, S, D>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((nlines + 1) / 2), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, s, d,
+                       pl.dev);
+  } else {
+    const int nt = pick_threads(pl.dev.L, 32);
+    auto kern = r2r_x_kernel<T, 32, S, D>;
+    allow_lds(kern, lds);
+    hipLaunchKernelGGL(kern, dim3((nlines + 1) / 2), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, s, d,
+                       pl.dev);
+  }
   HIP_OK(hipGetLastError());
 }
 
@@ -462,8 +471,8 @@ void launch_y(const LinePlan<T>& pl, int TX, int N, int kind, int dir, int n_in,
   }
   const size_t lds = (size_t)(TX / 2) * pl.dev.L * sizeof(C2<T>);
   if (lds > kLdsMax) throw std::runtime_error("y tile exceeds LDS: N=" + std::to_string(N));
-  const int nt = pick_threads((long)(TX / 2) * pl.dev.L);
-  auto kern = r2r_y_kernel<T, S, D>;
+  const int nt = pick_threads((long)(TX / 2) * pl.dev.L, 16);
+  auto kern = r2r_y_kernel<T, 16, S, D>;
   allow_lds(kern, lds);
   hipLaunchKernelGGL(kern, dim3((ncols + TX - 1) / TX), dim3(nt), lds, st, N, kind, dir, n_in, n_out, ncols, TX, s,
                      d, pl.dev);
