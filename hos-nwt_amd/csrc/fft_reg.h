@@ -165,31 +165,41 @@ __device__ __forceinline__ int layout_pos(int i, int tl, int ntl, int L) {
 template <typename T, int V, int R, bool YP, class S>
 __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int ntl, int L, int N, int kind, int dir,
                                           int n_in, int la, int lb, bool ha, bool hb) {
+  // phase 1: issue every load (branch-free); phase 2: masks, multipliers, signs
+  decltype(src.raw(0, 0)) ra[V], rb[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     const int p = layout_pos<R>(i, tl, ntl, L);
-    int e = p;
-    T sg = (T)1;
-    if (p >= N) {
-      e = L - p;
-      if (kind == hosnwt_kind_sin) sg = (T)-1;
+    const int e = (p >= N) ? L - p : p;
+    const int ec = e < n_in ? e : 0;
+    if (YP) {
+      ra[i] = src.raw(ha ? la : 0, ec);
+      rb[i] = src.raw(hb ? lb : 0, ec);
+    } else {
+      ra[i] = src.raw(ec, ha ? la : 0);
+      rb[i] = src.raw(ec, hb ? lb : 0);
     }
-    T a = (T)0, b = (T)0;
+  }
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int p = layout_pos<R>(i, tl, ntl, L);
+    const bool mir = p >= N;
+    const int e = mir ? L - p : p;
     const bool edge = (e == 0 || e == N - 1);
-    if (e < n_in && !(kind == hosnwt_kind_sin && edge)) {
-      if (YP) {
-        if (ha) a = src.get(la, e);
-        if (hb) b = src.get(lb, e);
-      } else {
-        if (ha) a = src.get(e, la);
-        if (hb) b = src.get(e, lb);
-      }
-      if (dir < 0 && !edge) {
-        a *= (T)0.5;
-        b *= (T)0.5;
-      }
+    const bool use = (e < n_in) & !(kind == hosnwt_kind_sin && edge);
+    T a, b;
+    if (YP) {
+      a = src.apply(ra[i], la, e);
+      b = src.apply(rb[i], lb, e);
+    } else {
+      a = src.apply(ra[i], e, la);
+      b = src.apply(rb[i], e, lb);
     }
-    v[i] = mk<T>(sg * a, sg * b);
+    T f = (dir < 0 && !edge) ? (T)0.5 : (T)1;
+    if (mir && kind == hosnwt_kind_sin) f = -f;
+    a = (use && ha) ? a * f : (T)0;
+    b = (use && hb) ? b * f : (T)0;
+    v[i] = mk<T>(a, b);
   }
 }
 
@@ -197,31 +207,52 @@ template <typename T, int V, int R, bool YP, class D>
 __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl, int ntl, int L, int N, int kind,
                                            int dir, int n_out, int la, int lb, bool ha, bool hb) {
   const T inv = (T)(1.0 / (double)L);
+  constexpr int CH = 8;  // outputs whose loads are issued together
 #pragma unroll
-  for (int i = 0; i < V; ++i) {
-    const int k = layout_pos<R>(i, tl, ntl, L);
-    if (k >= n_out) continue;
-    const C2<T> Z = v[i];
-    T A, B;
-    if (kind == hosnwt_kind_cos) {
-      A = Z.x;
-      B = Z.y;
-    } else {
-      A = -Z.y;
-      B = Z.x;
-      if (k == 0 || k == N - 1) A = B = (T)0;
+  for (int i0 = 0; i0 < V; i0 += CH) {
+    typename D::Pre qa[CH], qb[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int i = i0 + c;
+      if (i >= V) break;
+      const int k = layout_pos<R>(i, tl, ntl, L);
+      const bool ok = k < n_out;
+      if (YP) {
+        qa[c] = dst.pre(la, k, ok && ha);
+        qb[c] = dst.pre(lb, k, ok && hb);
+      } else {
+        qa[c] = dst.pre(k, la, ok && ha);
+        qb[c] = dst.pre(k, lb, ok && hb);
+      }
     }
-    if (dir > 0) {
-      const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
-      A *= f;
-      B *= f;
-    }
-    if (YP) {
-      if (ha) dst.put(la, k, A);
-      if (hb) dst.put(lb, k, B);
-    } else {
-      if (ha) dst.put(k, la, A);
-      if (hb) dst.put(k, lb, B);
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+      const int i = i0 + c;
+      if (i >= V) break;
+      const int k = layout_pos<R>(i, tl, ntl, L);
+      if (k >= n_out) continue;
+      const C2<T> Z = v[i];
+      T A, B;
+      if (kind == hosnwt_kind_cos) {
+        A = Z.x;
+        B = Z.y;
+      } else {
+        A = -Z.y;
+        B = Z.x;
+        if (k == 0 || k == N - 1) A = B = (T)0;
+      }
+      if (dir > 0) {
+        const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
+        A *= f;
+        B *= f;
+      }
+      if (YP) {
+        if (ha) dst.put(la, k, A, qa[c]);
+        if (hb) dst.put(lb, k, B, qb[c]);
+      } else {
+        if (ha) dst.put(k, la, A, qa[c]);
+        if (hb) dst.put(k, lb, B, qb[c]);
+      }
     }
   }
 }

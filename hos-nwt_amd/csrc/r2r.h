@@ -53,30 +53,35 @@ enum Mul { MUL_NONE = 0, MUL_NEG_KX = 1, MUL_NEG_KY = 2, MUL_KTH = 3 };
 // U is the stored type, T the transform type.
 template <typename T, typename U = T>
 struct Src {
-  const U* p;
+  const U* __restrict__ p;
   int ld, nx, ny;
   int mul, jpow;
   double pix, piy;
-  __device__ __forceinline__ T get(int ix, int iy) const {
-    if (ix >= nx || iy >= ny) return (T)0;
-    const U v = p[ix + (size_t)ld * iy];
-    if (mul == MUL_NONE) return (T)v;
-    double m;
-    if (mul == MUL_NEG_KX) {
-      m = -(double)ix * pix;
-    } else if (mul == MUL_NEG_KY) {
-      m = -(double)iy * piy;
-    } else {
-      // kth: k^j tanh(k) for odd j, k^j for even j (HOS-NWT.f90:311-327)
-      const double kx = (double)ix * pix, ky = (double)iy * piy;
-      const double k = sqrt(kx * kx + ky * ky);
-      double kj = k;
-      for (int q = 1; q < jpow; ++q) kj *= k;
-      if (jpow & 1) kj *= (k < 20.0 ? tanh(k) : 1.0);
-      m = kj;
-    }
-    return (T)(m * (double)v);
+  // Branch-free load: the index is clamped to a valid element and the value
+  // masked, so that a thread can keep many loads in flight.
+  __device__ __forceinline__ U raw(int ix, int iy) const {
+    const bool ok = (ix < nx) & (iy < ny);
+    const U v = p[ok ? ix + (size_t)ld * iy : 0];
+    return ok ? v : (U)0;
   }
+  // Spectral multiplier of mode (ix, iy).
+  __device__ __forceinline__ double mulval(int ix, int iy) const {
+    if (mul == MUL_NONE) return 1.0;
+    if (mul == MUL_NEG_KX) return -(double)ix * pix;
+    if (mul == MUL_NEG_KY) return -(double)iy * piy;
+    // kth: k^j tanh(k) for odd j, k^j for even j (HOS-NWT.f90:311-327)
+    const double kx = (double)ix * pix, ky = (double)iy * piy;
+    const double k = sqrt(kx * kx + ky * ky);
+    double kj = k;
+    for (int q = 1; q < jpow; ++q) kj *= k;
+    if (jpow & 1) kj *= (k < 20.0 ? tanh(k) : 1.0);
+    return kj;
+  }
+  __device__ __forceinline__ T apply(U v, int ix, int iy) const {
+    if (mul == MUL_NONE) return (T)v;
+    return (T)(mulval(ix, iy) * (double)v);
+  }
+  __device__ __forceinline__ T get(int ix, int iy) const { return apply(raw(ix, iy), ix, iy); }
 };
 
 template <typename T, typename U = T>
@@ -89,17 +94,24 @@ inline Src<T, U> src_mul(const U* p, int ld, int nx, int ny, int mul, int jpow, 
 }
 
 // Plain store: p[ix + ld*iy] = v (mode 0) or += v (mode 1).
+// Stores go through a two-phase interface: pre() issues the loads an output
+// needs (branch-free, clamped), put() combines and stores.
 template <typename T>
 struct Dst {
-  T* p;
+  T* __restrict__ p;
   int ld, mode;
-  __device__ __forceinline__ void put(int ix, int iy, T v) const {
-    const size_t i = ix + (size_t)ld * iy;
-    if (mode == 0)
-      p[i] = v;
-    else
-      p[i] += v;
+  struct Pre {
+    T old;
+  };
+  __device__ __forceinline__ Pre pre(int ix, int iy, bool ok) const {
+    Pre q;
+    q.old = (mode == 0) ? (T)0 : p[ok ? ix + (size_t)ld * iy : 0];
+    return q;
   }
+  __device__ __forceinline__ void put(int ix, int iy, T v, const Pre& q) const {
+    p[ix + (size_t)ld * iy] = (mode == 0) ? v : q.old + v;
+  }
+  __device__ __forceinline__ void put(int ix, int iy, T v) const { put(ix, iy, v, pre(ix, iy, true)); }
 };
 
 // HOS recursion epilogue (resol_HOS.f90:737-769): with phizi = v,
@@ -107,20 +119,29 @@ struct Dst {
 // (`first` overwrites instead of accumulating; ejm1 == nullptr means factor 1).
 template <typename T>
 struct DstHos {
-  T* phim;
-  T* phizm;
-  const T* ej;
-  const T* ejm1;
+  T* __restrict__ phim;
+  T* __restrict__ phizm;
+  const T* __restrict__ ej;
+  const T* __restrict__ ejm1;
   int ld, first, do_phim;
-  __device__ __forceinline__ void put(int ix, int iy, T v) const {
-    const size_t i = ix + (size_t)ld * iy;
-    if (do_phim) {
-      const T t = v * ej[i];
-      phim[i] = first ? -t : phim[i] - t;
-    }
-    const T u = ejm1 ? v * ejm1[i] : v;
-    phizm[i] = first ? u : phizm[i] + u;
+  struct Pre {
+    T e, e1, pm, pz;
+  };
+  __device__ __forceinline__ Pre pre(int ix, int iy, bool ok) const {
+    const size_t i = ok ? ix + (size_t)ld * iy : 0;
+    Pre q;
+    q.e = do_phim ? ej[i] : (T)0;
+    q.e1 = ejm1 ? ejm1[i] : (T)1;
+    q.pm = (do_phim && !first) ? phim[i] : (T)0;
+    q.pz = first ? (T)0 : phizm[i];
+    return q;
   }
+  __device__ __forceinline__ void put(int ix, int iy, T v, const Pre& q) const {
+    const size_t i = ix + (size_t)ld * iy;
+    if (do_phim) phim[i] = q.pm - v * q.e;
+    phizm[i] = q.pz + v * q.e1;
+  }
+  __device__ __forceinline__ void put(int ix, int iy, T v) const { put(ix, iy, v, pre(ix, iy, true)); }
 };
 
 // ---------------------------------------------------------------------------
