@@ -38,6 +38,25 @@ struct Plan2 {
 
 __host__ __device__ __forceinline__ int padx(int p) { return p + (p >> 3); }
 
+template <int A, int... B>
+struct hosfft2_first {
+  static constexpr int value = A;
+};
+template <int... Rs>
+struct hosfft2_last;
+template <int A>
+struct hosfft2_last<A> {
+  static constexpr int value = A;
+};
+template <int A, int... B>
+struct hosfft2_last<A, B...> {
+  static constexpr int value = hosfft2_last<B...>::value;
+};
+template <int... Rs>
+struct hosfft2_prod {
+  static constexpr int value = (Rs * ... * 1);
+};
+
 template <typename T>
 __device__ __forceinline__ C2<T> twiddle(const C2<T>* t0, const C2<T>* t1, int m) {
   return cmul<T>(t1[m >> 6], t0[m & 63]);
@@ -98,60 +117,25 @@ __device__ __forceinline__ void pass_read(C2<T>* v, const T* line, int tl, int n
   }
 }
 
-template <typename T, int V>
-__device__ __forceinline__ void dispatch_compute(int R, C2<T>* v, int tl, int ntl, int Ns, int L, const C2<T>* t0,
-                                                 const C2<T>* t1) {
-  switch (R) {
-    case 8: if constexpr (V % 8 == 0) pass_compute<T, V, 8>(v, tl, ntl, Ns, L, t0, t1); break;
-    case 4: if constexpr (V % 4 == 0) pass_compute<T, V, 4>(v, tl, ntl, Ns, L, t0, t1); break;
-    case 2: if constexpr (V % 2 == 0) pass_compute<T, V, 2>(v, tl, ntl, Ns, L, t0, t1); break;
-    case 3: if constexpr (V % 3 == 0) pass_compute<T, V, 3>(v, tl, ntl, Ns, L, t0, t1); break;
-    default: break;
-  }
-}
-template <typename T, int V>
-__device__ __forceinline__ void dispatch_write(int R, const C2<T>* v, T* line, int tl, int ntl, int Ns, int comp) {
-  switch (R) {
-    case 8: if constexpr (V % 8 == 0) pass_write<T, V, 8>(v, line, tl, ntl, Ns, comp); break;
-    case 4: if constexpr (V % 4 == 0) pass_write<T, V, 4>(v, line, tl, ntl, Ns, comp); break;
-    case 2: if constexpr (V % 2 == 0) pass_write<T, V, 2>(v, line, tl, ntl, Ns, comp); break;
-    case 3: if constexpr (V % 3 == 0) pass_write<T, V, 3>(v, line, tl, ntl, Ns, comp); break;
-    default: break;
-  }
-}
-template <typename T, int V>
-__device__ __forceinline__ void dispatch_read(int R2, C2<T>* v, const T* line, int tl, int ntl, int L, int comp) {
-  switch (R2) {
-    case 8: if constexpr (V % 8 == 0) pass_read<T, V, 8>(v, line, tl, ntl, L, comp); break;
-    case 4: if constexpr (V % 4 == 0) pass_read<T, V, 4>(v, line, tl, ntl, L, comp); break;
-    case 2: if constexpr (V % 2 == 0) pass_read<T, V, 2>(v, line, tl, ntl, L, comp); break;
-    case 3: if constexpr (V % 3 == 0) pass_read<T, V, 3>(v, line, tl, ntl, L, comp); break;
-    default: break;
-  }
-}
-
-// Run all passes on the values in v (first-pass layout on entry, last-pass
-// layout on exit: value b*R_last + r sits at position tl + b ntl + r L/R_last).
-// `line` is this line's LDS region; every thread of the workgroup must call.
-template <typename T, int V>
-__device__ __forceinline__ void fft_reg(C2<T>* v, T* line, int tl, const Plan2<T>& pl, const C2<T>* t0,
-                                        const C2<T>* t1) {
-  int Ns = 1;
-  for (int p = 0; p < pl.npass; ++p) {
-    const int R = pl.radix[p];
-    dispatch_compute<T, V>(R, v, tl, pl.ntl, Ns, pl.L, t0, t1);
-    if (p + 1 < pl.npass) {
-      const int R2 = pl.radix[p + 1];
-      dispatch_write<T, V>(R, v, line, tl, pl.ntl, Ns, 0);
-      __syncthreads();
-      dispatch_read<T, V>(R2, v, line, tl, pl.ntl, pl.L, 0);
-      __syncthreads();
-      dispatch_write<T, V>(R, v, line, tl, pl.ntl, Ns, 1);
-      __syncthreads();
-      dispatch_read<T, V>(R2, v, line, tl, pl.ntl, pl.L, 1);
-      __syncthreads();
-    }
-    Ns *= R;
+// All passes of a compile-time radix sequence.  On entry v holds the
+// first-pass layout, on exit the last-pass layout (value b*R_last + r at
+// position tl + b ntl + r L/R_last).  `line` is this line's LDS region; every
+// thread of the workgroup must call.
+template <typename T, int V, int L, int Ns, int R, int... Rest>
+__device__ __forceinline__ void run_passes(C2<T>* v, T* line, int tl, const C2<T>* t0, const C2<T>* t1) {
+  constexpr int ntl = L / V;
+  pass_compute<T, V, R>(v, tl, ntl, Ns, L, t0, t1);
+  if constexpr (sizeof...(Rest) > 0) {
+    constexpr int R2 = hosfft2_first<Rest...>::value;
+    pass_write<T, V, R>(v, line, tl, ntl, Ns, 0);
+    __syncthreads();
+    pass_read<T, V, R2>(v, line, tl, ntl, L, 0);
+    __syncthreads();
+    pass_write<T, V, R>(v, line, tl, ntl, Ns, 1);
+    __syncthreads();
+    pass_read<T, V, R2>(v, line, tl, ntl, L, 1);
+    __syncthreads();
+    run_passes<T, V, L, Ns * R, Rest...>(v, line, tl, t0, t1);
   }
 }
 
@@ -164,7 +148,7 @@ __device__ __forceinline__ int layout_pos(int i, int tl, int ntl, int L) {
 
 template <typename T, int V, int R, bool YP, class S>
 __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int ntl, int L, int N, int kind, int dir,
-                                          int n_in, int la, int lb, bool ha, bool hb) {
+                                          int n_in, int la, int lb, bool ha, bool hb, const T* mult) {
   // phase 1: issue every load (branch-free); phase 2: masks, multipliers, signs
   decltype(src.raw(0, 0)) ra[V], rb[V];
 #pragma unroll
@@ -187,13 +171,12 @@ __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int nt
     const int e = mir ? L - p : p;
     const bool edge = (e == 0 || e == N - 1);
     const bool use = (e < n_in) & !(kind == hosnwt_kind_sin && edge);
-    T a, b;
-    if (YP) {
-      a = src.apply(ra[i], la, e);
-      b = src.apply(rb[i], lb, e);
-    } else {
-      a = src.apply(ra[i], e, la);
-      b = src.apply(rb[i], e, lb);
+    T a = (T)ra[i], b = (T)rb[i];
+    if (mult) {
+      // multipliers of both real lines, precomputed in LDS: [2][N]
+      const int ec = e < n_in ? e : 0;
+      a = (T)((double)mult[ec] * (double)ra[i]);
+      b = (T)((double)mult[N + ec] * (double)rb[i]);
     }
     T f = (dir < 0 && !edge) ? (T)0.5 : (T)1;
     if (mir && kind == hosnwt_kind_sin) f = -f;
@@ -257,19 +240,26 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
   }
 }
 
-// Line kernel.  YP = false: x pass, one or more complex lines per workgroup,
-// threads of a line contiguous; each complex line packs real lines 2q, 2q+1.
+// Line kernel for a compile-time radix sequence Rs (L = prod Rs).
+// YP = false: x pass, ncl complex lines per workgroup, threads of a line
+// contiguous; complex line q packs real lines 2q, 2q+1 of the batch.
 // YP = true: y pass over TX = 2*ncl adjacent columns, lines interleaved across
-// threads so that a row segment of TX columns is one coalesced access.
-template <typename T, int V, bool YP, class S, class D>
-__global__ void __launch_bounds__(256) r2r_reg_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines,
+// threads so that a row segment of TX columns is one coalesced access; the
+// column tiles are visited in an XCD-aware order so that neighbouring tiles
+// share an L2.  Spectral multipliers (kth, -kx, -ky) are evaluated once per
+// element in a rolled loop into LDS before the loads.
+template <typename T, int V, bool YP, class S, class D, int... Rs>
+__global__ void __launch_bounds__(256) r2r_seq_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines,
                                                       int ncl, S src, D dst, Plan2<T> pl) {
+  constexpr int L = hosfft2_prod<Rs...>::value;
+  constexpr int ntl = L / V;
+  constexpr int R0 = hosfft2_first<Rs...>::value;
+  constexpr int Rl = hosfft2_last<Rs...>::value;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   C2<T>* t0 = reinterpret_cast<C2<T>*>(lds + (size_t)ncl * pl.lp);
   C2<T>* t1 = t0 + 64;
   const int tid = threadIdx.x;
-  const int ntl = pl.ntl, L = pl.L;
   int cl, tl;
   if (YP) {
     cl = tid % ncl;
@@ -278,36 +268,33 @@ __global__ void __launch_bounds__(256) r2r_reg_kernel(int N, int kind, int dir, 
     cl = tid / ntl;
     tl = tid - cl * ntl;
   }
-  // workgroup -> group of complex lines; y pass: XCD-aware order so that
-  // neighbouring column tiles share an L2
   int grp = blockIdx.x;
   if (YP) {
     const int nb = gridDim.x;
     if ((nb & 7) == 0) grp = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
   }
-  const int la = 2 * (grp * ncl + cl), lb = la + 1;  // real lines of this complex line
+  const int la = 2 * (grp * ncl + cl), lb = la + 1;
   const bool ha = la < nlines, hb = lb < nlines;
-  // load the evenly / oddly extended line straight into the first-pass layout
-  C2<T> v[V];
-  switch (pl.radix[0]) {
-    case 8: if constexpr (V % 8 == 0) load_line<T, V, 8, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
-    case 4: if constexpr (V % 4 == 0) load_line<T, V, 4, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
-    case 2: if constexpr (V % 2 == 0) load_line<T, V, 2, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
-    case 3: if constexpr (V % 3 == 0) load_line<T, V, 3, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb); break;
-    default: break;
+  T* line = lds + (size_t)cl * pl.lp;
+  const T* mult = nullptr;
+  if (src.mul != 0) {
+    // multipliers of the two real lines of every complex line: line[q*N + e]
+    for (int idx = tid; idx < ncl * 2 * n_in; idx += blockDim.x) {
+      const int c2 = idx / (2 * n_in), rem = idx - c2 * 2 * n_in;
+      const int h = rem / n_in, e = rem - h * n_in;
+      const int ln = 2 * (grp * ncl + c2) + h;
+      lds[(size_t)c2 * pl.lp + h * N + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
+    }
+    __syncthreads();
+    mult = line;
   }
+  C2<T> v[V];
+  load_line<T, V, R0, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb, mult);
   for (int m = tid; m < 64; m += blockDim.x) t0[m] = pl.tw0[m];
   for (int m = tid; m < (L + 63) / 64; m += blockDim.x) t1[m] = pl.tw1[m];
   __syncthreads();
-  fft_reg<T, V>(v, lds + (size_t)cl * pl.lp, tl, pl, t0, t1);
-  // store straight from the last-pass layout
-  switch (pl.radix[pl.npass - 1]) {
-    case 8: if constexpr (V % 8 == 0) store_line<T, V, 8, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
-    case 4: if constexpr (V % 4 == 0) store_line<T, V, 4, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
-    case 2: if constexpr (V % 2 == 0) store_line<T, V, 2, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
-    case 3: if constexpr (V % 3 == 0) store_line<T, V, 3, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb); break;
-    default: break;
-  }
+  run_passes<T, V, L, 1, Rs...>(v, line, tl, t0, t1);
+  store_line<T, V, Rl, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb);
 }
 
 }  // namespace hosfft2

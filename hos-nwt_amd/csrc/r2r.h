@@ -250,6 +250,22 @@ __global__ void __launch_bounds__(256) r2r_y_kernel(int N, int kind, int dir, in
 // ---------------------------------------------------------------------------
 // host-side plans
 // ---------------------------------------------------------------------------
+// Extended-grid line lengths with a compiled radix sequence (values per thread V).
+inline bool seq_supported(int L) {
+  switch (L) {
+    case 6144: case 3072: case 1536: case 768: case 4096: case 2048: case 1024: case 512: return true;
+    default: return false;
+  }
+}
+inline int seq_values(int L) {
+  switch (L) {
+    case 6144: case 3072: case 1536: case 768: return 24;
+    case 4096: case 2048: return 16;
+    default: return 8;
+  }
+}
+
+
 inline bool smooth235(long L) {
   for (int r : {2, 3, 5})
     while (L % r == 0) L /= r;
@@ -291,15 +307,11 @@ struct LinePlan {
     while (r % 2 == 0) { r /= 2; ++a; }
     while (r % 3 == 0) { r /= 3; ++b; }
     if (r != 1) return;
-    const int V = b ? 24 : 8;
-    if (L % V != 0 || L / V < 8 || L / V > 256) return;
+    (void)a;
+    (void)b;
+    if (!seq_supported(L)) return;
+    const int V = seq_values(L);
     int np = 0;
-    long q = L;
-    while (q % 8 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 8; q /= 8; }
-    while (q % 4 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 4; q /= 4; }
-    while (q % 2 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 2; q /= 2; }
-    while (q % 3 == 0 && np < hosfft2::kMaxPass2) { dev2.radix[np++] = 3; q /= 3; }
-    if (q != 1) return;
     dev2.L = L;
     dev2.npass = np;
     dev2.ntl = L / V;
@@ -435,23 +447,33 @@ inline void allow_lds(K kernel, size_t bytes) {
   }
 }
 
-template <typename T, bool YP, class S, class D>
-void launch_reg(const LinePlan<T>& pl, int ncl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
+template <typename T, int V, bool YP, class S, class D, int... Rs>
+void launch_seq(const LinePlan<T>& pl, int ncl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
                 const D& d, hipStream_t st) {
   const size_t lds = (size_t)ncl * pl.dev2.lp * sizeof(T) + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
   const int nt = ncl * pl.dev2.ntl;
   int groups = (nlines + 2 * ncl - 1) / (2 * ncl);
   if (YP) groups = (groups + 7) / 8 * 8;  // XCD-aware tile order needs a multiple of 8
-  if (pl.V2 == 24) {
-    auto kern = hosfft2::r2r_reg_kernel<T, 24, YP, S, D>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(groups), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, ncl, s, d, pl.dev2);
-  } else {
-    auto kern = hosfft2::r2r_reg_kernel<T, 8, YP, S, D>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3(groups), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, ncl, s, d, pl.dev2);
-  }
+  auto kern = hosfft2::r2r_seq_kernel<T, V, YP, S, D, Rs...>;
+  allow_lds(kern, lds);
+  hipLaunchKernelGGL(kern, dim3(groups), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, ncl, s, d, pl.dev2);
   HIP_OK(hipGetLastError());
+}
+
+template <typename T, bool YP, class S, class D>
+void launch_reg(const LinePlan<T>& pl, int ncl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
+                const D& d, hipStream_t st) {
+  switch (pl.dev2.L) {
+    case 6144: launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 3072: launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 1536: launch_seq<T, 24, YP, S, D, 8, 8, 8, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 768: launch_seq<T, 24, YP, S, D, 8, 8, 4, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 4096: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 2048: launch_seq<T, 16, YP, S, D, 8, 8, 8, 4>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 1024: launch_seq<T, 8, YP, S, D, 8, 8, 8, 2>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 512: launch_seq<T, 8, YP, S, D, 8, 8, 8>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    default: throw std::runtime_error("no compiled radix sequence for L=" + std::to_string(pl.dev2.L));
+  }
 }
 
 template <typename T, class S, class D>
