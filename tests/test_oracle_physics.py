@@ -251,3 +251,44 @@ def test_ramp_closed_forms():
     assert ho.ramp(104.0, p) == (0.0, 0.0, 0.0)
     p.iramp = 1
     assert ho.ramp(1.5, p) == (0.5, 1.0 / 3.0, 0.0)
+
+
+def test_rk_tableau_on_linear_rhs():
+    """runge4_adapt2 with a diagonal linear RHS f(G) = lam * G (lam per mode and per field).
+
+    The classical RK4 step multiplies each mode by 1 + z + z^2/2 + z^3/6 + z^4/24
+    (z = lam h) and Kutta's third-order companion by 1 + z + z^2/2 + z^3/6, so the
+    embedded error of every mode is |z^4/24 G| over max|G_new|.  A wrong A, b or e
+    entry, or a stage fed the wrong predecessor, changes one of the polynomials.
+    """
+    n1, n2 = 9, 6
+    p = ho.Params(n1=n1, n2=n2, mHOS=1, xlen=3.0, ylen=2.0, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    rng = np.random.default_rng(21)
+    lam1 = -0.8 + 1.3 * rng.standard_normal((n1, n2))
+    lam2 = 0.4 * rng.standard_normal((n1, n2))
+
+    def linear_rhs(a1st, eta1st, G2, G1, a_add, delta_t):
+        za = np.zeros_like(a_add)
+        return lam1 * G1, lam2 * G2, np.zeros_like(a1st), np.zeros_like(eta1st), za, za.copy()
+
+    o.solve_hos2 = linear_rhs
+    eta = rng.standard_normal((n1, n2)) * 0.01
+    phis = rng.standard_normal((n1, n2)) * 0.01
+    st = o.initialize(eta0=eta, phis0=phis)
+    h = 0.23
+    new, err, _ = o.runge4_adapt2(0.0, st, h)
+
+    def p4(z):
+        return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+    G1 = o.s2f(eta)
+    G2 = o.s2f(phis) * o.tb.omega
+    G1n, G2n = p4(lam1 * h) * G1, p4(lam2 * h) * G2
+    G2n[0, 0] = 0.0  # k_G2(1,1) = 0 and omega(1,1) = 0
+    e1 = np.abs((lam1 * h) ** 4 / 24 * G1).max() / (np.abs(G1n).max() + ho.TINY)
+    e2 = np.abs((lam2 * h) ** 4 / 24 * G2).max() / (np.abs(G2n).max() + ho.TINY)
+    assert abs(err - max(e1, e2)) <= 1e-12 * max(e1, e2)
+    etark, _ = o.gtof(G1n, G2n, h)
+    etark[0, 0] = 0.0  # no wavemaker: the mean-volume mode is reset to 0
+    np.testing.assert_allclose(new.eta, o.f2s(etark), atol=1e-15, rtol=0)
