@@ -861,6 +861,103 @@ inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, co
   blas_ok(rocblas_sgemm(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, Cm, ldc), "sgemm");
 }
 
+// ---------------------------------------------------------------------------
+// base-grid transforms as dense products
+// ---------------------------------------------------------------------------
+// At the reference sizes the base-grid lengths 2(n-1) carry large primes
+// (1024: 2046 = 2.3.11.31, 256: 510 = 2.3.5.17), so an FFT needs Bluestein's
+// doubled length; on a grid this small the transform is latency-bound, and the
+// N x N DCT-I/DST-I matrix product (rocBLAS, matrix cores) is both faster and
+// exact to rounding.  Column-major N x N, out_r = sum_c M[r + N c] in_c, with
+// the normalisations of fourier_FFTW.f90:178-229 folded in:
+//   cos forward  a_r = s2f_r (x_0 + (-1)^r x_{N-1} + 2 sum_{0<c<N-1} x_c cos(pi r c/(N-1)))
+//   cos inverse  x_r = sum_c a_c cos(pi r c/(N-1))
+//   sin forward  a_r = 2/(N-1) sum_{0<c<N-1} x_c sin(pi r c/(N-1)), zero end rows
+//   sin inverse  x_r = sum_{0<c<N-1} a_c sin(pi r c/(N-1)), zero end rows
+std::vector<double> r2r_matrix(int N, int kind, int dir) {
+  std::vector<double> m((size_t)N * N, 0.0);
+  if (N == 1) {
+    m[0] = (kind == KCOS) ? 1.0 : 0.0;
+    return m;
+  }
+  const long P = 2L * (N - 1);
+  for (int c = 0; c < N; ++c)
+    for (int r = 0; r < N; ++r) {
+      const double ang = M_PI * (double)(((long)r * c) % P) / (double)(N - 1);
+      const bool er = (r == 0 || r == N - 1), ec = (c == 0 || c == N - 1);
+      double v;
+      if (kind == KCOS)
+        v = (dir == FWD) ? (er ? 1.0 : 2.0) / (double)P * (ec ? 1.0 : 2.0) * std::cos(ang) : std::cos(ang);
+      else
+        v = (er || ec) ? 0.0 : ((dir == FWD) ? 2.0 / (double)(N - 1) : 1.0) * std::sin(ang);
+      m[r + (size_t)N * c] = v;
+    }
+  return m;
+}
+
+// Dense path applies when a base-grid direction has no FFT plan without
+// Bluestein and the matrices stay small (N <= 4096).
+inline bool dense_base_grid(int n1, int n2) {
+  auto dense = [](int n) { return n > 1 && !seq_supported(2 * (n - 1)); };
+  return n1 <= 4096 && n2 <= 4096 && (dense(n1) || (n2 > 1 && dense(n2)));
+}
+
+template <typename U, typename S>
+__global__ void k_dense_in(int n1, int n2, S src, U* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)n1 * n2) return;
+  const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+  out[i] = (U)src.get(i1, i2);
+}
+
+template <typename U>
+struct DenseGrid {
+  int n1 = 0, n2 = 1;
+  U* mx[2][2] = {};  // [kind][0 forward, 1 inverse]
+  U* my[2][2] = {};
+  U* stage = nullptr;  // source after multiplier / conversion
+  U* mid = nullptr;    // after the x product
+};
+
+template <typename U>
+U* upload_matrix(Arena& ar, const std::vector<double>& m) {
+  std::vector<U> h(m.begin(), m.end());
+  U* d = ar.get<U>(h.size());
+  HIP_OK(hipMemcpy(d, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
+  return d;
+}
+
+template <typename U>
+void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
+  g.n1 = n1;
+  g.n2 = n2;
+  for (int k = 0; k < 2; ++k)
+    for (int d = 0; d < 2; ++d) {
+      g.mx[k][d] = upload_matrix<U>(ar, r2r_matrix(n1, k, d == 0 ? FWD : INV));
+      if (n2 > 1) g.my[k][d] = upload_matrix<U>(ar, r2r_matrix(n2, k, d == 0 ? FWD : INV));
+    }
+  g.stage = ar.get<U>((size_t)n1 * n2);
+  g.mid = ar.get<U>((size_t)n1 * n2);
+}
+
+// out (nout1 x nout2, plain store) = My[ky] (Mx[kx] src)^T restricted to the
+// requested output extents.
+template <typename U, class S>
+void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx, int ky, int dir, const S& src,
+             int nout1, int nout2, const Dst<U>& d) {
+  if (d.mode != 0) throw std::runtime_error("dense base transform: plain stores only");
+  const int n1 = g.n1, n2 = g.n2, di = (dir == FWD) ? 0 : 1;
+  const size_t n = (size_t)n1 * n2;
+  hipLaunchKernelGGL((k_dense_in<U, S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n1, n2, src, g.stage);
+  HIP_OK(hipGetLastError());
+  if (n2 == 1) {
+    gemm(blas, rocblas_operation_none, nout1, 1, n1, g.mx[kx][di], n1, g.stage, n1, d.p, d.ld);
+    return;
+  }
+  gemm(blas, rocblas_operation_none, nout1, n2, n1, g.mx[kx][di], n1, g.stage, n1, g.mid, n1);
+  gemm(blas, rocblas_operation_transpose, nout1, nout2, n2, g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -943,6 +1040,10 @@ struct Engine : EngineBase {
   int K1 = 0, K2 = 0;
   T *XA, *YA, *XB, *YB, *PA, *PB, *RA, *RB, *T1, *T2;
   rocblas_handle blas = nullptr;
+  // base grid as dense products (lengths without a Bluestein-free FFT plan)
+  bool dense = false;
+  DenseGrid<T> dT;
+  DenseGrid<double> d64;
 
   explicit Engine(hos_ctx* ctx) : c(ctx) {
     Arena& ar = c->ar;
@@ -994,6 +1095,11 @@ struct Engine : EngineBase {
     Wy = upload(h_wy);
     blas_ok(rocblas_create_handle(&blas), "create_handle");
     blas_ok(rocblas_set_stream(blas, c->st), "set_stream");
+    dense = dense_base_grid(c->n1, c->n2);
+    if (dense) {
+      build_dense<T>(dT, ar, c->n1, c->n2);
+      if (sizeof(T) != sizeof(double)) build_dense<double>(d64, ar, c->n1, c->n2);
+    }
     const int nf = c->p.n_freq;
     std::vector<double> hf(c->p.freq, c->p.freq + nf);
     std::vector<double> hr((size_t)n2 * nf), hi((size_t)n2 * nf);
@@ -1054,12 +1160,22 @@ struct Engine : EngineBase {
   // base-grid transforms use tmpB (large enough) as their intermediate
   template <class S, class D>
   void base(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
-    r2r2d<T>(gsT, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
+    if (dense)
+      dense2d<T>(dT, blas, c->st, kx, ky, dir, s, o1, o2, d);
+    else
+      r2r2d<T>(gsT, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
     c->stats.n_base++;
   }
   template <class S, class D>
   void base64(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
-    r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, c->st);
+    if (dense) {
+      if constexpr (sizeof(T) == sizeof(double))
+        dense2d<double>(dT, blas, c->st, kx, ky, dir, s, o1, o2, d);
+      else
+        dense2d<double>(d64, blas, c->st, kx, ky, dir, s, o1, o2, d);
+    } else {
+      r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, c->st);
+    }
     c->stats.n_base++;
   }
 

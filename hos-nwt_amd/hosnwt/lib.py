@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhosnwt.so")
+# HOSNWT_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("HOSNWT_LIB") or os.path.join(os.path.dirname(_HERE), "lib", "libhosnwt.so")
 
 _dp = C.POINTER(C.c_double)
 
