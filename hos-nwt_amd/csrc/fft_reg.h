@@ -248,10 +248,29 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
 // column tiles are visited in an XCD-aware order so that neighbouring tiles
 // share an L2.  Spectral multipliers (kth, -kx, -ky) are evaluated once per
 // element in a rolled loop into LDS before the loads.
+// Independent line transforms sharing one launch (blockIdx.y = item): same
+// length, direction and functor types; kind, extents, source and destination
+// per item.  A workgroup whose lines lie beyond its item's extent exits at
+// once, before any barrier.
+constexpr int kMaxBatch = 8;
+template <class S, class D>
+struct LineBatch {
+  int n = 0;
+  int kind[kMaxBatch] = {};
+  int n_in[kMaxBatch] = {};
+  int n_out[kMaxBatch] = {};
+  int nlines[kMaxBatch] = {};
+  S src[kMaxBatch];
+  D dst[kMaxBatch];
+};
+
 template <typename T, int V, bool YP, class S, class D, int... Rs>
-__global__ void __launch_bounds__(256) r2r_seq_kernel(int N, int kind, int dir, int n_in, int n_out, int nlines,
-                                                      int ncl, S src, D dst, Plan2<T> pl) {
+__global__ void __launch_bounds__(256) r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt, Plan2<T> pl) {
   constexpr int L = hosfft2_prod<Rs...>::value;
+  const int item = blockIdx.y;
+  const int kind = bt.kind[item], n_in = bt.n_in[item], n_out = bt.n_out[item], nlines = bt.nlines[item];
+  const S& src = bt.src[item];
+  const D& dst = bt.dst[item];
   constexpr int ntl = L / V;
   constexpr int R0 = hosfft2_first<Rs...>::value;
   constexpr int Rl = hosfft2_last<Rs...>::value;
@@ -273,6 +292,7 @@ __global__ void __launch_bounds__(256) r2r_seq_kernel(int N, int kind, int dir, 
     const int nb = gridDim.x;
     if ((nb & 7) == 0) grp = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
   }
+  if (2 * grp * ncl >= nlines) return;  // uniform over the workgroup
   const int la = 2 * (grp * ncl + cl), lb = la + 1;
   const bool ha = la < nlines, hb = lb < nlines;
   T* line = lds + (size_t)cl * pl.lp;

@@ -1029,6 +1029,7 @@ struct Engine : EngineBase {
   T *eta_hat, *phis_hat, *eta_phys, *P, *D1, *D2, *Q, *Zp, *Zh, *phixh, *phiyh, *phizh, *tmpS;
   // big (T)
   std::vector<T*> etapm, phizm, aHOS;
+  std::vector<T*> tmpX;  // x-pass intermediates of batched transforms
   T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
   // wavemaker / first order (double)
   double *Xp, *Xpt, *Xptt, *f1, *f2, *fz1, *fz2, *gz, *Wz, *Wy, *freq, *pre, *pim, *part;
@@ -1066,6 +1067,7 @@ struct Engine : EngineBase {
       phizm[j] = ar.get<T>(nb);
       if (j >= 2) aHOS[j] = ar.get<T>(nb);
     }
+    for (int i = 0; i < kMaxBatch && i < std::max(M, 5); ++i) tmpX.push_back(ar.get<T>(nb));
     etax = ar.get<T>(nb); etay = ar.get<T>(nb); phisx = ar.get<T>(nb); phisy = ar.get<T>(nb);
     g2 = ar.get<T>(nb); phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
     phiza = ar.get<T>(nb); X = ar.get<T>(nb); Y = ar.get<T>(nb); tmpB = ar.get<T>(nb);
@@ -1157,6 +1159,12 @@ struct Engine : EngineBase {
     r2r2d<T>(gb, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
     c->stats.n_big++;
   }
+  // independent extended-grid transforms in one launch per pass
+  template <class S, class D>
+  void bigB(int dir, const Batch2<T, S, D>& b, bool serial_y = false) {
+    r2r2d_batch<T>(gb, dir, b, c->st, serial_y);
+    c->stats.n_big += b.n;
+  }
   // base-grid transforms use tmpB (large enough) as their intermediate
   template <class S, class D>
   void base(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
@@ -1197,15 +1205,21 @@ struct Engine : EngineBase {
     // --- physical eta on the base grid ---
     base(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), n1, n2, Dst<T>{eta_phys, n1, 0});
     // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
-    big(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{etapm[1] ? etapm[1] : tmpB2(), ldb, 0});
-    big(KSIN, KCOS, INV, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0});
-    big(KSIN, KCOS, INV, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b,
-        Dst<T>{phisx, ldb, 0});
-    if (n2 > 1) {
-      big(KCOS, KSIN, INV, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
-          Dst<T>{etay, ldb, 0});
-      big(KCOS, KSIN, INV, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
-          Dst<T>{phisy, ldb, 0});
+    {
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      b.add(KCOS, KCOS, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{etapm[1] ? etapm[1] : tmpB2(), ldb, 0},
+            tmpX[0]);
+      b.add(KSIN, KCOS, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0},
+            tmpX[1]);
+      b.add(KSIN, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b,
+            Dst<T>{phisx, ldb, 0}, tmpX[2]);
+      if (n2 > 1) {
+        b.add(KCOS, KSIN, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+              Dst<T>{etay, ldb, 0}, tmpX[3]);
+        b.add(KCOS, KSIN, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+              Dst<T>{phisy, ldb, 0}, tmpX[4]);
+      }
+      bigB(INV, b);
     }
     PtrSet<T> ep{};
     for (int j = 1; j < M; ++j) ep.p[j] = etapm[j];
@@ -1216,19 +1230,23 @@ struct Engine : EngineBase {
     }
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
     for (int iH = 1; iH <= M; ++iH) {
+      // the x passes of every order j in one launch; the y passes accumulate
+      // into phim / phizm(iH) and run in the reference's order j = iH..1
+      Batch2<T, Src<T, T>, DstHos<T>> b;
       for (int j = iH; j >= 1; --j) {
         const int j2 = iH - j + 1;
         DstHos<T> d{phim, phizm[iH], (j < M) ? etapm[j] : nullptr, (j >= 2) ? etapm[j - 1] : nullptr, ldb,
                     j == iH ? 1 : 0, iH < M ? 1 : 0};
         if (j2 == 1) {
-          big(KCOS, KCOS, INV, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, j, pix, piy), N1b, N2b, d);
+          b.add(KCOS, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, j, pix, piy), N1b, N2b, d, tmpX[b.n]);
         } else {
           const bool trunc = (j2 == M);
-          big(KCOS, KCOS, INV,
-              src_mul<T>(aHOS[j2], trunc ? n1 : ldb, trunc ? n1 : N1b, trunc ? n2 : N2b, MUL_KTH, j, pix, piy),
-              N1b, N2b, d);
+          b.add(KCOS, KCOS,
+                src_mul<T>(aHOS[j2], trunc ? n1 : ldb, trunc ? n1 : N1b, trunc ? n2 : N2b, MUL_KTH, j, pix, piy),
+                N1b, N2b, d, tmpX[b.n]);
         }
       }
+      bigB(INV, b, true);
       if (iH < M) {
         // aHOS(iH+1) = S2F_big(phimext), truncated to n1 x n2 modes when iH+1 = M
         if (iH + 1 == M)
@@ -1252,8 +1270,12 @@ struct Engine : EngineBase {
       }
       hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
                          phixa, phiya, phiza, X, Y, XA, YA, XB, YB);
-      big(KCOS, KCOS, FWD, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0});
-      big(KCOS, KCOS, FWD, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0});
+      {
+        Batch2<T, Src<T, T>, Dst<T>> b;
+        b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[0]);
+        b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[1]);
+        bigB(FWD, b);
+      }
       if (C.wmk) mixed_parity();
       hipLaunchKernelGGL(k_add3<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)D1, (const T*)P, Q);
     } else {
@@ -1275,14 +1297,16 @@ struct Engine : EngineBase {
   void mixed_parity() {
     hos_ctx& C = *c;
     const int n1 = C.n1, n2 = C.n2, N1b = C.N1b, N2b = C.N2b, ldb = N1b;
-    big(KSIN, KCOS, FWD, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0});
-    big(KSIN, KCOS, FWD, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0});
-    gemm(blas, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
+    Batch2<T, Src<T, T>, Dst<T>> b;
+    b.add(KSIN, KCOS, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0}, tmpX[0]);
+    b.add(KSIN, KCOS, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0}, tmpX[1]);
     if (n2 > 1) {
-      big(KCOS, KSIN, FWD, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, Dst<T>{PB, 2 * n1, 0});
-      big(KCOS, KSIN, FWD, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0});
-      gemm(blas, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
+      b.add(KCOS, KSIN, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, Dst<T>{PB, 2 * n1, 0}, tmpX[2]);
+      b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0}, tmpX[3]);
     }
+    bigB(FWD, b);
+    gemm(blas, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
+    if (n2 > 1) gemm(blas, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
     hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, C.st, n1, n2, (const T*)RA,
                        (const T*)(n2 > 1 ? RB : nullptr), D1, D2);
   }
@@ -1339,10 +1363,14 @@ struct Engine : EngineBase {
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
-    big(KCOS, KCOS, INV, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0});
-    if (n2 > 1) big(KCOS, KCOS, INV, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0});
-    else HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
-    big(KCOS, KCOS, INV, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0});
+    {
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, tmpX[0]);
+      if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, tmpX[1]);
+      b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, tmpX[2]);
+      bigB(INV, b);
+      if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+    }
   }
 
   // S2F_add of nfields paddle columns stored contiguously from f1, divided by
@@ -1395,12 +1423,27 @@ struct Engine : EngineBase {
     Src<T, T> sx = src_plain<T>(X, N1b, N1b, N2b);
     Dst<T> dx{N2b > 1 ? tmpB : Y, N1b, 0};
     HIP_OK(hipEventRecord(e0, C.st));
-    for (int r = 0; r < reps; ++r) launch_x<T>(gb.px, N1b, KCOS, INV, N1b, N1b, N2b, sx, dx, C.st);
+    LineBatch<Src<T, T>, Dst<T>> bx;
+    bx.n = 1;
+    bx.kind[0] = KCOS;
+    bx.n_in[0] = N1b;
+    bx.n_out[0] = N1b;
+    bx.nlines[0] = N2b;
+    bx.src[0] = sx;
+    bx.dst[0] = dx;
+    for (int r = 0; r < reps; ++r) launch_x<T>(gb.px, N1b, INV, bx, C.st);
     HIP_OK(hipEventRecord(e1, C.st));
     if (N2b > 1) {
       Src<T, T> sy = src_plain<T>(tmpB, N1b, N1b, N2b);
-      for (int r = 0; r < reps; ++r)
-        launch_y<T>(gb.py, gb.TX, N2b, KCOS, INV, N2b, N2b, N1b, sy, Dst<T>{Y, N1b, 0}, C.st);
+      LineBatch<Src<T, T>, Dst<T>> by;
+      by.n = 1;
+      by.kind[0] = KCOS;
+      by.n_in[0] = N2b;
+      by.n_out[0] = N2b;
+      by.nlines[0] = N1b;
+      by.src[0] = sy;
+      by.dst[0] = Dst<T>{Y, N1b, 0};
+      for (int r = 0; r < reps; ++r) launch_y<T>(gb.py, gb.TX, N2b, INV, by, C.st);
     }
     HIP_OK(hipEventRecord(e2, C.st));
     HIP_OK(hipEventSynchronize(e2));

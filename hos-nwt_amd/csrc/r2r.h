@@ -17,6 +17,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -457,69 +458,83 @@ inline void allow_lds(K kernel, size_t bytes) {
   }
 }
 
+using hosfft2::kMaxBatch;
+using hosfft2::LineBatch;
+
+template <class S, class D>
+inline int batch_max_lines(const LineBatch<S, D>& b) {
+  int m = 0;
+  for (int i = 0; i < b.n; ++i) m = std::max(m, (b.n_out[i] > 0) ? b.nlines[i] : 0);
+  return m;
+}
+
 template <typename T, int V, bool YP, class S, class D, int... Rs>
-void launch_seq(const LinePlan<T>& pl, int ncl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
-                const D& d, hipStream_t st) {
+void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   const size_t lds = (size_t)ncl * pl.dev2.lp * sizeof(T) + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
   const int nt = ncl * pl.dev2.ntl;
-  int groups = (nlines + 2 * ncl - 1) / (2 * ncl);
+  const int nl = batch_max_lines(b);
+  if (nl <= 0) return;
+  int groups = (nl + 2 * ncl - 1) / (2 * ncl);
   if (YP) groups = (groups + 7) / 8 * 8;  // XCD-aware tile order needs a multiple of 8
   auto kern = hosfft2::r2r_seq_kernel<T, V, YP, S, D, Rs...>;
   allow_lds(kern, lds);
-  hipLaunchKernelGGL(kern, dim3(groups), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, ncl, s, d, pl.dev2);
+  hipLaunchKernelGGL(kern, dim3(groups, b.n), dim3(nt), lds, st, N, dir, ncl, b, pl.dev2);
   HIP_OK(hipGetLastError());
 }
 
 template <typename T, bool YP, class S, class D>
-void launch_reg(const LinePlan<T>& pl, int ncl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
-                const D& d, hipStream_t st) {
+void launch_reg(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   switch (pl.dev2.L) {
-    case 6144: launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 3072: launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 1536: launch_seq<T, 24, YP, S, D, 8, 8, 8, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 768: launch_seq<T, 24, YP, S, D, 8, 8, 4, 3>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 4096: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 2048: launch_seq<T, 16, YP, S, D, 8, 8, 8, 4>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 1024: launch_seq<T, 8, YP, S, D, 8, 8, 8, 2>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
-    case 512: launch_seq<T, 8, YP, S, D, 8, 8, 8>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st); break;
+    case 6144: launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
+    case 3072: launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, dir, b, st); break;
+    case 1536: launch_seq<T, 24, YP, S, D, 8, 8, 8, 3>(pl, ncl, N, dir, b, st); break;
+    case 768: launch_seq<T, 24, YP, S, D, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
+    case 4096: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8>(pl, ncl, N, dir, b, st); break;
+    case 2048: launch_seq<T, 16, YP, S, D, 8, 8, 8, 4>(pl, ncl, N, dir, b, st); break;
+    case 1024: launch_seq<T, 8, YP, S, D, 8, 8, 8, 2>(pl, ncl, N, dir, b, st); break;
+    case 512: launch_seq<T, 8, YP, S, D, 8, 8, 8>(pl, ncl, N, dir, b, st); break;
     default: throw std::runtime_error("no compiled radix sequence for L=" + std::to_string(pl.dev2.L));
   }
 }
 
+// x passes of every item of a batch (lines of item i: nlines[i] x-lines).
 template <typename T, class S, class D>
-void launch_x(const LinePlan<T>& pl, int N, int kind, int dir, int n_in, int n_out, int nlines, const S& s,
-              const D& d, hipStream_t st) {
-  if (nlines <= 0 || n_out <= 0) return;
+void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+  const int nl = batch_max_lines(b);
+  if (nl <= 0) return;
   if (pl.V2) {
     int ncl = 1;
-    while (2 * ncl * pl.dev2.ntl <= 256 && 2 * ncl <= (nlines + 1) / 2) ncl *= 2;
-    launch_reg<T, false>(pl, ncl, N, kind, dir, n_in, n_out, nlines, s, d, st);
+    while (2 * ncl * pl.dev2.ntl <= 256 && 2 * ncl <= (nl + 1) / 2) ncl *= 2;
+    launch_reg<T, false>(pl, ncl, N, dir, b, st);
     return;
   }
   const size_t lds = (size_t)pl.dev.L * sizeof(C2<T>);
   if (lds > kLdsMax) throw std::runtime_error("x line exceeds LDS: N=" + std::to_string(N));
-  if (pl.dev.L <= 256 * 16) {
-    const int nt = pick_threads(pl.dev.L, 16);
-    auto kern = r2r_x_kernel<T, 16, S, D>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((nlines + 1) / 2), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, s, d,
-                       pl.dev);
-  } else {
-    const int nt = pick_threads(pl.dev.L, 32);
-    auto kern = r2r_x_kernel<T, 32, S, D>;
-    allow_lds(kern, lds);
-    hipLaunchKernelGGL(kern, dim3((nlines + 1) / 2), dim3(nt), lds, st, N, kind, dir, n_in, n_out, nlines, s, d,
-                       pl.dev);
+  for (int i = 0; i < b.n; ++i) {
+    if (b.nlines[i] <= 0 || b.n_out[i] <= 0) continue;
+    if (pl.dev.L <= 256 * 16) {
+      const int nt = pick_threads(pl.dev.L, 16);
+      auto kern = r2r_x_kernel<T, 16, S, D>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((b.nlines[i] + 1) / 2), dim3(nt), lds, st, N, b.kind[i], dir, b.n_in[i],
+                         b.n_out[i], b.nlines[i], b.src[i], b.dst[i], pl.dev);
+    } else {
+      const int nt = pick_threads(pl.dev.L, 32);
+      auto kern = r2r_x_kernel<T, 32, S, D>;
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3((b.nlines[i] + 1) / 2), dim3(nt), lds, st, N, b.kind[i], dir, b.n_in[i],
+                         b.n_out[i], b.nlines[i], b.src[i], b.dst[i], pl.dev);
+    }
+    HIP_OK(hipGetLastError());
   }
-  HIP_OK(hipGetLastError());
 }
 
+// y passes of every item (item i: nlines[i] columns of n_in[i] valid rows).
 template <typename T, class S, class D>
-void launch_y(const LinePlan<T>& pl, int TX, int N, int kind, int dir, int n_in, int n_out, int ncols, const S& s,
-              const D& d, hipStream_t st) {
-  if (ncols <= 0 || n_out <= 0) return;
+void launch_y(const LinePlan<T>& pl, int TX, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+  if (batch_max_lines(b) <= 0) return;
   if (pl.V2) {
-    launch_reg<T, true>(pl, TX / 2, N, kind, dir, n_in, n_out, ncols, s, d, st);
+    launch_reg<T, true>(pl, TX / 2, N, dir, b, st);
     return;
   }
   const size_t lds = (size_t)(TX / 2) * pl.dev.L * sizeof(C2<T>);
@@ -527,25 +542,101 @@ void launch_y(const LinePlan<T>& pl, int TX, int N, int kind, int dir, int n_in,
   const int nt = pick_threads((long)(TX / 2) * pl.dev.L, 16);
   auto kern = r2r_y_kernel<T, 16, S, D>;
   allow_lds(kern, lds);
-  hipLaunchKernelGGL(kern, dim3((ncols + TX - 1) / TX), dim3(nt), lds, st, N, kind, dir, n_in, n_out, ncols, TX, s,
-                     d, pl.dev);
-  HIP_OK(hipGetLastError());
+  for (int i = 0; i < b.n; ++i) {
+    if (b.nlines[i] <= 0 || b.n_out[i] <= 0) continue;
+    hipLaunchKernelGGL(kern, dim3((b.nlines[i] + TX - 1) / TX), dim3(nt), lds, st, N, b.kind[i], dir, b.n_in[i],
+                       b.n_out[i], b.nlines[i], TX, b.src[i], b.dst[i], pl.dev);
+    HIP_OK(hipGetLastError());
+  }
 }
 
-// 2-D r2r: src (valid extents src.nx, src.ny) -> dst with output extents
-// nout1 x nout2.  tmp must hold nout1 * src.ny values.
+// Independent 2-D transforms on one grid in one direction (shared functor
+// types): item i maps src[i] (valid extents src.nx x src.ny) to dst[i] with
+// output extents nout1[i] x nout2[i], through the intermediate tmp[i]
+// (nout1[i] * min(src.ny, N2) values).
+template <typename T, class S, class D>
+struct Batch2 {
+  int n = 0;
+  int kx[kMaxBatch] = {}, ky[kMaxBatch] = {}, nout1[kMaxBatch] = {}, nout2[kMaxBatch] = {};
+  S src[kMaxBatch];
+  D dst[kMaxBatch];
+  T* tmp[kMaxBatch] = {};
+  void add(int kx_, int ky_, const S& s, int o1, int o2, const D& d, T* t) {
+    if (n >= kMaxBatch) throw std::runtime_error("transform batch full");
+    kx[n] = kx_;
+    ky[n] = ky_;
+    src[n] = s;
+    nout1[n] = o1;
+    nout2[n] = o2;
+    dst[n] = d;
+    tmp[n] = t;
+    ++n;
+  }
+};
+
+// serial_y: the y passes run one launch per item, in order (destinations that
+// accumulate into shared arrays).
+template <typename T, class S, class D>
+void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream_t st, bool serial_y = false) {
+  if (b.n == 0) return;
+  if (g.N2 == 1) {
+    LineBatch<S, D> x;
+    x.n = b.n;
+    for (int i = 0; i < b.n; ++i) {
+      x.kind[i] = b.kx[i];
+      x.n_in[i] = b.src[i].nx;
+      x.n_out[i] = b.nout1[i];
+      x.nlines[i] = 1;
+      x.src[i] = b.src[i];
+      x.dst[i] = b.dst[i];
+    }
+    launch_x<T>(g.px, g.N1, dir, x, st);
+    return;
+  }
+  LineBatch<S, Dst<T>> x;
+  LineBatch<Src<T, T>, D> y;
+  x.n = y.n = b.n;
+  for (int i = 0; i < b.n; ++i) {
+    const int nl = b.src[i].ny < g.N2 ? b.src[i].ny : g.N2;
+    x.kind[i] = b.kx[i];
+    x.n_in[i] = b.src[i].nx;
+    x.n_out[i] = b.nout1[i];
+    x.nlines[i] = nl;
+    x.src[i] = b.src[i];
+    x.dst[i] = Dst<T>{b.tmp[i], b.nout1[i], 0};
+    y.kind[i] = b.ky[i];
+    y.n_in[i] = nl;
+    y.n_out[i] = b.nout2[i];
+    y.nlines[i] = b.nout1[i];
+    y.src[i] = src_plain<T, T>(b.tmp[i], b.nout1[i], b.nout1[i], nl);
+    y.dst[i] = b.dst[i];
+  }
+  launch_x<T>(g.px, g.N1, dir, x, st);
+  if (!serial_y) {
+    launch_y<T>(g.py, g.TX, g.N2, dir, y, st);
+    return;
+  }
+  for (int i = 0; i < b.n; ++i) {
+    LineBatch<Src<T, T>, D> one;
+    one.n = 1;
+    one.kind[0] = y.kind[i];
+    one.n_in[0] = y.n_in[i];
+    one.n_out[0] = y.n_out[i];
+    one.nlines[0] = y.nlines[i];
+    one.src[0] = y.src[i];
+    one.dst[0] = y.dst[i];
+    launch_y<T>(g.py, g.TX, g.N2, dir, one, st);
+  }
+}
+
+// Single 2-D transform: src (valid extents src.nx, src.ny) -> dst with output
+// extents nout1 x nout2.  tmp must hold nout1 * src.ny values.
 template <typename T, class S, class D>
 void r2r2d(const Grid2<T>& g, int kx, int ky, int dir, const S& src, int nout1, int nout2, const D& dst, T* tmp,
            hipStream_t st) {
-  if (g.N2 == 1) {
-    launch_x<T>(g.px, g.N1, kx, dir, src.nx, nout1, 1, src, dst, st);
-    return;
-  }
-  const int nl = src.ny < g.N2 ? src.ny : g.N2;
-  Dst<T> dt{tmp, nout1, 0};
-  launch_x<T>(g.px, g.N1, kx, dir, src.nx, nout1, nl, src, dt, st);
-  Src<T, T> s2 = src_plain<T, T>(tmp, nout1, nout1, nl);
-  launch_y<T>(g.py, g.TX, g.N2, ky, dir, nl, nout2, nout1, s2, dst, st);
+  Batch2<T, S, D> b;
+  b.add(kx, ky, src, nout1, nout2, dst, tmp);
+  r2r2d_batch<T>(g, dir, b, st);
 }
 
 }  // namespace hosnwt
