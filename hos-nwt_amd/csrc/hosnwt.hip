@@ -1161,8 +1161,8 @@ struct Engine : EngineBase {
   }
   // independent extended-grid transforms in one launch per pass
   template <class S, class D>
-  void bigB(int dir, const Batch2<T, S, D>& b, bool serial_y = false) {
-    r2r2d_batch<T>(gb, dir, b, c->st, serial_y);
+  void bigB(int dir, const Batch2<T, S, D>& b, bool serial = false, bool yfirst = false) {
+    r2r2d_batch<T>(gb, dir, b, c->st, serial, yfirst);
     c->stats.n_big += b.n;
   }
   // base-grid transforms use tmpB (large enough) as their intermediate
@@ -1230,8 +1230,9 @@ struct Engine : EngineBase {
     }
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
     for (int iH = 1; iH <= M; ++iH) {
-      // the x passes of every order j in one launch; the y passes accumulate
-      // into phim / phizm(iH) and run in the reference's order j = iH..1
+      // the y passes of every order j in one launch; the x passes, whose
+      // contiguous lines carry the read-modify-write epilogue, accumulate into
+      // phim / phizm(iH) in the reference's order j = iH..1
       Batch2<T, Src<T, T>, DstHos<T>> b;
       for (int j = iH; j >= 1; --j) {
         const int j2 = iH - j + 1;
@@ -1246,7 +1247,7 @@ struct Engine : EngineBase {
                 N1b, N2b, d, tmpX[b.n]);
         }
       }
-      bigB(INV, b, true);
+      bigB(INV, b, true, true);
       if (iH < M) {
         // aHOS(iH+1) = S2F_big(phimext), truncated to n1 x n2 modes when iH+1 = M
         if (iH + 1 == M)

@@ -574,10 +574,39 @@ struct Batch2 {
   }
 };
 
-// serial_y: the y passes run one launch per item, in order (destinations that
-// accumulate into shared arrays).
+// serial: the second pass runs one launch per item, in order (destinations
+// that accumulate into shared arrays).  yfirst: the y pass runs first and the
+// x pass, whose lines are contiguous, writes the destination (the better
+// access pattern for read-modify-write epilogues); tmp[i] then holds
+// min(src.nx, N1) x nout2[i] values.
+template <typename T, class D>
+void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>, D>& b, hipStream_t st, bool serial) {
+  auto go = [&](const LineBatch<Src<T, T>, D>& bb) {
+    if (y)
+      launch_y<T>(g.py, g.TX, g.N2, dir, bb, st);
+    else
+      launch_x<T>(g.px, g.N1, dir, bb, st);
+  };
+  if (!serial) {
+    go(b);
+    return;
+  }
+  for (int i = 0; i < b.n; ++i) {
+    LineBatch<Src<T, T>, D> one;
+    one.n = 1;
+    one.kind[0] = b.kind[i];
+    one.n_in[0] = b.n_in[i];
+    one.n_out[0] = b.n_out[i];
+    one.nlines[0] = b.nlines[i];
+    one.src[0] = b.src[i];
+    one.dst[0] = b.dst[i];
+    go(one);
+  }
+}
+
 template <typename T, class S, class D>
-void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream_t st, bool serial_y = false) {
+void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream_t st, bool serial = false,
+                 bool yfirst = false) {
   if (b.n == 0) return;
   if (g.N2 == 1) {
     LineBatch<S, D> x;
@@ -590,42 +619,61 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
       x.src[i] = b.src[i];
       x.dst[i] = b.dst[i];
     }
-    launch_x<T>(g.px, g.N1, dir, x, st);
+    if (serial) {
+      for (int i = 0; i < b.n; ++i) {
+        LineBatch<S, D> one;
+        one.n = 1;
+        one.kind[0] = x.kind[i];
+        one.n_in[0] = x.n_in[i];
+        one.n_out[0] = x.n_out[i];
+        one.nlines[0] = 1;
+        one.src[0] = x.src[i];
+        one.dst[0] = x.dst[i];
+        launch_x<T>(g.px, g.N1, dir, one, st);
+      }
+    } else {
+      launch_x<T>(g.px, g.N1, dir, x, st);
+    }
     return;
   }
-  LineBatch<S, Dst<T>> x;
-  LineBatch<Src<T, T>, D> y;
-  x.n = y.n = b.n;
+  LineBatch<S, Dst<T>> p1;
+  LineBatch<Src<T, T>, D> p2;
+  p1.n = p2.n = b.n;
   for (int i = 0; i < b.n; ++i) {
-    const int nl = b.src[i].ny < g.N2 ? b.src[i].ny : g.N2;
-    x.kind[i] = b.kx[i];
-    x.n_in[i] = b.src[i].nx;
-    x.n_out[i] = b.nout1[i];
-    x.nlines[i] = nl;
-    x.src[i] = b.src[i];
-    x.dst[i] = Dst<T>{b.tmp[i], b.nout1[i], 0};
-    y.kind[i] = b.ky[i];
-    y.n_in[i] = nl;
-    y.n_out[i] = b.nout2[i];
-    y.nlines[i] = b.nout1[i];
-    y.src[i] = src_plain<T, T>(b.tmp[i], b.nout1[i], b.nout1[i], nl);
-    y.dst[i] = b.dst[i];
+    if (!yfirst) {
+      const int nl = b.src[i].ny < g.N2 ? b.src[i].ny : g.N2;
+      p1.kind[i] = b.kx[i];
+      p1.n_in[i] = b.src[i].nx;
+      p1.n_out[i] = b.nout1[i];
+      p1.nlines[i] = nl;
+      p1.dst[i] = Dst<T>{b.tmp[i], b.nout1[i], 0};
+      p2.kind[i] = b.ky[i];
+      p2.n_in[i] = nl;
+      p2.n_out[i] = b.nout2[i];
+      p2.nlines[i] = b.nout1[i];
+      p2.src[i] = src_plain<T, T>(b.tmp[i], b.nout1[i], b.nout1[i], nl);
+    } else {
+      const int nc = b.src[i].nx < g.N1 ? b.src[i].nx : g.N1;
+      p1.kind[i] = b.ky[i];
+      p1.n_in[i] = b.src[i].ny;
+      p1.n_out[i] = b.nout2[i];
+      p1.nlines[i] = nc;
+      p1.dst[i] = Dst<T>{b.tmp[i], nc, 0};
+      p2.kind[i] = b.kx[i];
+      p2.n_in[i] = nc;
+      p2.n_out[i] = b.nout1[i];
+      p2.nlines[i] = b.nout2[i];
+      p2.src[i] = src_plain<T, T>(b.tmp[i], nc, nc, b.nout2[i]);
+    }
+    p1.src[i] = b.src[i];
+    p2.dst[i] = b.dst[i];
   }
-  launch_x<T>(g.px, g.N1, dir, x, st);
-  if (!serial_y) {
-    launch_y<T>(g.py, g.TX, g.N2, dir, y, st);
-    return;
-  }
-  for (int i = 0; i < b.n; ++i) {
-    LineBatch<Src<T, T>, D> one;
-    one.n = 1;
-    one.kind[0] = y.kind[i];
-    one.n_in[0] = y.n_in[i];
-    one.n_out[0] = y.n_out[i];
-    one.nlines[0] = y.nlines[i];
-    one.src[0] = y.src[i];
-    one.dst[0] = y.dst[i];
-    launch_y<T>(g.py, g.TX, g.N2, dir, one, st);
+  if (!yfirst) {
+    launch_x<T>(g.px, g.N1, dir, p1, st);
+    launch_second<T, D>(true, g, dir, p2, st, serial);
+  } else {
+    launch_y<T>(g.py, g.TX, g.N2, dir, p1, st);
+    launch_second<T, D>(false, g, dir, p2, st, serial);
   }
 }
 
