@@ -1041,6 +1041,31 @@ struct Engine : EngineBase {
   int K1 = 0, K2 = 0;
   T *XA, *YA, *XB, *YB, *PA, *PB, *RA, *RB, *T1, *T2;
   rocblas_handle blas = nullptr;
+  // The wavemaker chain (paddle motion, paddle-grid products, wavemaker sums,
+  // first-order and gradient transforms) depends only on the base-grid eta, so
+  // it runs on a second stream beside the HOS recursion, with its own scratch;
+  // helpers use the current stream / handle / scratch (cs, cb, ctmp, ctmpX).
+  hipStream_t sw = nullptr;
+  rocblas_handle bw = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  T* tmpBw = nullptr;
+  std::vector<T*> tmpXw;
+  hipStream_t cs = nullptr;
+  rocblas_handle cb = nullptr;
+  T* ctmp = nullptr;
+  std::vector<T*>* ctmpX = nullptr;
+  void use_main() {
+    cs = c->st;
+    cb = blas;
+    ctmp = tmpB;
+    ctmpX = &tmpX;
+  }
+  void use_side() {
+    cs = sw;
+    cb = bw;
+    ctmp = tmpBw;
+    ctmpX = &tmpXw;
+  }
   // base grid as dense products (lengths without a Bluestein-free FFT plan)
   bool dense = false;
   DenseGrid<T> dT;
@@ -1097,6 +1122,14 @@ struct Engine : EngineBase {
     Wy = upload(h_wy);
     blas_ok(rocblas_create_handle(&blas), "create_handle");
     blas_ok(rocblas_set_stream(blas, c->st), "set_stream");
+    HIP_OK(hipStreamCreateWithFlags(&sw, hipStreamNonBlocking));
+    blas_ok(rocblas_create_handle(&bw), "create_handle");
+    blas_ok(rocblas_set_stream(bw, sw), "set_stream");
+    HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    tmpBw = ar.get<T>(nb);
+    for (int i = 0; i < 3; ++i) tmpXw.push_back(ar.get<T>(nb));
+    use_main();
     dense = dense_base_grid(c->n1, c->n2);
     if (dense) {
       build_dense<T>(dT, ar, c->n1, c->n2);
@@ -1138,6 +1171,11 @@ struct Engine : EngineBase {
   }
   ~Engine() override {
     if (blas) (void)rocblas_destroy_handle(blas);
+    if (bw) (void)rocblas_destroy_handle(bw);
+    if (sw) (void)hipStreamSynchronize(sw);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (sw) (void)hipStreamDestroy(sw);
   }
 
   T* upload_t(const std::vector<double>& v) {
@@ -1156,33 +1194,33 @@ struct Engine : EngineBase {
   // ---- transform helpers ------------------------------------------------
   template <class S, class D>
   void big(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
-    r2r2d<T>(gb, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
+    r2r2d<T>(gb, kx, ky, dir, s, o1, o2, d, ctmp, cs);
     c->stats.n_big++;
   }
   // independent extended-grid transforms in one launch per pass
   template <class S, class D>
   void bigB(int dir, const Batch2<T, S, D>& b, bool serial = false, bool yfirst = false) {
-    r2r2d_batch<T>(gb, dir, b, c->st, serial, yfirst);
+    r2r2d_batch<T>(gb, dir, b, cs, serial, yfirst);
     c->stats.n_big += b.n;
   }
   // base-grid transforms use tmpB (large enough) as their intermediate
   template <class S, class D>
   void base(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
     if (dense)
-      dense2d<T>(dT, blas, c->st, kx, ky, dir, s, o1, o2, d);
+      dense2d<T>(dT, cb, cs, kx, ky, dir, s, o1, o2, d);
     else
-      r2r2d<T>(gsT, kx, ky, dir, s, o1, o2, d, tmpB, c->st);
+      r2r2d<T>(gsT, kx, ky, dir, s, o1, o2, d, ctmp, cs);
     c->stats.n_base++;
   }
   template <class S, class D>
   void base64(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
     if (dense) {
       if constexpr (sizeof(T) == sizeof(double))
-        dense2d<double>(dT, blas, c->st, kx, ky, dir, s, o1, o2, d);
+        dense2d<double>(dT, cb, cs, kx, ky, dir, s, o1, o2, d);
       else
-        dense2d<double>(d64, blas, c->st, kx, ky, dir, s, o1, o2, d);
+        dense2d<double>(d64, cb, cs, kx, ky, dir, s, o1, o2, d);
     } else {
-      r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, c->st);
+      r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, cs);
     }
     c->stats.n_base++;
   }
@@ -1204,6 +1242,18 @@ struct Engine : EngineBase {
                        C.volmode(), C.last_alpha, C.last_pos_top, C.p.xlen, C.p.d_hinge);
     // --- physical eta on the base grid ---
     base(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), n1, n2, Dst<T>{eta_phys, n1, 0});
+    // --- wavemaker chain on the side stream: it needs only the base-grid eta ---
+    HIP_OK(hipEventRecord(ev_fork, st));
+    HIP_OK(hipStreamWaitEvent(sw, ev_fork, 0));
+    {
+      struct Side {
+        Engine* e;
+        ~Side() { e->use_main(); }
+      } side{this};
+      use_side();
+      wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr);
+    }
+    HIP_OK(hipEventRecord(ev_join, sw));
     // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
@@ -1261,8 +1311,8 @@ struct Engine : EngineBase {
     for (int j = 1; j <= M; ++j) zz.p[j] = phizm[j];
     hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, X);
     big(KCOS, KCOS, FWD, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0});
-    // --- wavemaker (solveHOS_SWEET_1 + phiadd_SL) ---
-    wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr);
+    // --- join the wavemaker chain (solveHOS_SWEET_1 + phiadd_SL) ---
+    HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     // --- free-surface products, filtered to the base modes ---
     if (M >= 2) {
       if (n2 == 1) {
@@ -1306,9 +1356,9 @@ struct Engine : EngineBase {
       b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0}, tmpX[3]);
     }
     bigB(FWD, b);
-    gemm(blas, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
-    if (n2 > 1) gemm(blas, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
-    hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, C.st, n1, n2, (const T*)RA,
+    gemm(cb, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
+    if (n2 > 1) gemm(cb, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
+    hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, (const T*)RA,
                        (const T*)(n2 > 1 ? RB : nullptr), D1, D2);
   }
 
@@ -1322,7 +1372,7 @@ struct Engine : EngineBase {
   void wavemaker(double t, const double* a1, const double* e1, const double* aad, double* ka1, double* ke1,
                  double* kad1, double* kadr) {
     hos_ctx& C = *c;
-    hipStream_t st = C.st;
+    hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a, N1b = C.N1b, N2b = C.N2b;
     const int ldb = N1b;
     if (!C.wmk) {
@@ -1366,9 +1416,9 @@ struct Engine : EngineBase {
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
-      b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, tmpX[0]);
-      if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, tmpX[1]);
-      b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, tmpX[2]);
+      b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
+      if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, (*ctmpX)[1]);
+      b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
       bigB(INV, b);
       if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
     }
@@ -1380,25 +1430,25 @@ struct Engine : EngineBase {
     hos_ctx& C = *c;
     const int n2 = C.n2, n3a = C.n3a;
     const double one = 1.0, zero = 0.0;
-    blas_ok(rocblas_dgemm(blas, rocblas_operation_none, rocblas_operation_none, n3a, nfields * n2, n3a, &one, Wz, n3a,
+    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, nfields * n2, n3a, &one, Wz, n3a,
                           f1, n3a, &zero, fz1, n3a),
             "dgemm paddle z");
     const double* zin = fz1;
     if (n2 > 1) {
-      blas_ok(rocblas_dgemm_strided_batched(blas, rocblas_operation_none, rocblas_operation_none, n3a, n2, n2, &one,
+      blas_ok(rocblas_dgemm_strided_batched(cb, rocblas_operation_none, rocblas_operation_none, n3a, n2, n2, &one,
                                             fz1, n3a, (rocblas_stride)na, Wy, n2, 0, &zero, fz2, n3a,
                                             (rocblas_stride)na, nfields),
               "dgemm paddle y");
       zin = fz2;
     }
-    hipLaunchKernelGGL(k_add_scale, dim3(nblk(na * nfields)), dim3(256), 0, C.st, n2, n3a, nfields, C.p.xlen, C.piy,
+    hipLaunchKernelGGL(k_add_scale, dim3(nblk(na * nfields)), dim3(256), 0, cs, n2, n3a, nfields, C.p.xlen, C.piy,
                        zin, out1, out2);
     HIP_OK(hipGetLastError());
   }
 
   void phiadd(const double* a, const double* da, const T* eta) {
     hos_ctx& C = *c;
-    hipStream_t st = C.st;
+    hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2, n3a = C.n3a;
     if (n2 == 1) {
       hipLaunchKernelGGL((k_phiadd2d<T>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx, a, da, eta,
