@@ -29,6 +29,8 @@
 
 #include <rocblas/rocblas.h>
 
+#include <limits>
+
 #include "../../include/hosnwt.h"
 #include "r2r.h"
 
@@ -262,8 +264,10 @@ __global__ void k_dphin(size_t n, const T* __restrict__ fq, const double* __rest
 // (resol_HOS.f90:623-644, variablechange.f90:37-80); k_G2(1,1) = 0
 // (runge_kutta.f90:150,199,205).
 template <typename T>
+// filt: optional mode weights of the free-surface filters (resol_HOS.f90:613-621).
 __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, const T* D1, const T* P,
-                            const T* phis_hat, const T* phizh, const T* D2, const T* Zh, double* dG1, double* dG2) {
+                            const T* phis_hat, const T* phizh, const T* D2, const T* Zh, const double* filt,
+                            double* dG1, double* dG2) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -271,8 +275,9 @@ __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, c
   const double k = wave_k(i1, i2, pix, piy);
   const double w = wave_omega(k);
   const double kth1 = k * tanh(k);
-  const double nl1 = (D1 ? (double)D1[i] : 0.0) + (double)P[i] - (double)phis_hat[i] * kth1 + (double)phizh[i];
-  const double nl2 = ((D2 ? (double)D2[i] : 0.0) + (double)Zh[i]) * w;
+  const double f = filt ? filt[i] : 1.0;
+  const double nl1 = ((D1 ? (double)D1[i] : 0.0) + (double)P[i] - (double)phis_hat[i] * kth1 + (double)phizh[i]) * f;
+  const double nl2 = ((D2 ? (double)D2[i] : 0.0) + (double)Zh[i]) * f * w;
   const double c = cos(w * dt), s = sin(w * dt);
   dG1[i] = nl1 * c - nl2 * s;
   dG2[i] = (i == 0) ? 0.0 : nl2 * c + nl1 * s;
@@ -354,8 +359,8 @@ __global__ void k_paddle_rhs(int n2, int n3, int n3a, double delz, const double*
 // (fourier_FFTW.f90:432-447, resol_wmkr.f90:160-171): the z REDFT01 and y
 // REDFT00 are dense products with fixed matrices (rocBLAS); this kernel applies
 // the normalisation s_2_f_add and the division.
-__global__ void k_add_scale(int n2, int n3a, int nfields, double xlen, double piy, const double* in, double* out1,
-                            double* out2) {
+__global__ void k_add_scale(int n2, int n3a, int nfields, double xlen, double piy, const double* wz,
+                            const double* in, double* out1, double* out2) {
   const size_t na = (size_t)n3a * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= na * nfields) return;
@@ -368,7 +373,8 @@ __global__ void k_add_scale(int n2, int n3a, int nfields, double xlen, double pi
   const double div = ka * tanh(ka * xlen);
   double ys2f = 1.0;
   if (n2 > 1) ys2f = ((m == 0 || m == n2 - 1) ? 1.0 : 2.0) / (2.0 * (n2 - 1));
-  (w ? out2 : out1)[r] = in[i] * (ys2f / (double)n3a) / div;
+  const double f = wz ? wz[k] : 1.0;  // paddle z-mode filter (resol_wmkr.f90:158-160)
+  (w ? out2 : out1)[r] = in[i] * (ys2f / (double)n3a) * f / div;
 }
 
 // csh_add_x and k_add_sh_add_x / k (HOS-NWT.f90:364-405) for one (x, k).
@@ -648,6 +654,11 @@ __global__ void k_step_begin(int n1, int n2, double pix, double piy, const doubl
   G2[i] = phis_hat[i] * w;
 }
 
+__global__ void k_mulw(size_t n, const double* __restrict__ w, double* __restrict__ x) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= w[i];
+}
+
 __global__ void k_sub(size_t n, double* a, const double* b) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] -= b[i];
@@ -862,6 +873,73 @@ inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, co
 }
 
 // ---------------------------------------------------------------------------
+// spectral filters (filtering.f90:334-525)
+// ---------------------------------------------------------------------------
+// Weight of modes i = 1..n along one direction for coefficient c, as
+// filtering_base applies it: c >= 0 keeps the first INT(n c) modes (all when
+// c >= 0.9999); -1 zeroes the last mode (the integer quotient i/n); -2 sinc;
+// -3 raised cosine; -4 the degree-7 smoothstep of the raised cosine; -5 an
+// erfc profile centred at 3n/4 (its 0/0 at the centre is taken at the limit
+// 1/2); other negatives the exponential cut-off of i > INT(n |c|).
+std::vector<double> filter_weights(int n, double c) {
+  const int nf = (c >= 0.9999) ? n : std::abs((int)(n * c));
+  const double tiny = std::numeric_limits<double>::epsilon();
+  const double tp = 2.0 * M_PI;
+  std::vector<double> w(n, 1.0);
+  for (int i = 1; i <= n; ++i) {
+    double v = 1.0;
+    if (c >= 0.0) {
+      v = (i > nf) ? 0.0 : 1.0;
+    } else if (std::fabs(c + 1.0) < tiny) {
+      v = 1.0 - (double)(i / n);
+    } else if (std::fabs(c + 2.0) < tiny) {
+      const double a = tp * i / n;
+      v = std::sin(a) / a;
+    } else if (std::fabs(c + 3.0) < tiny) {
+      v = (1.0 + std::cos(tp * i / n)) / 2.0;
+    } else if (std::fabs(c + 4.0) < tiny) {
+      const double s0 = (1.0 + std::cos(tp * i / n)) / 2.0;
+      v = std::pow(s0, 4) * (35.0 - 84.0 * s0 + 70.0 * s0 * s0 - 20.0 * s0 * s0 * s0);
+    } else if (std::fabs(c + 5.0) < tiny) {
+      const double s0 = ((double)i - 0.75 * n) / (n + 1.0);
+      const double q = (4.0 / 3.0 * s0) * (4.0 / 3.0 * s0);
+      const double r = (q > 0.0) ? -std::log(1.0 - q) / q : 1.0;
+      v = 0.5 * std::erfc(2.0 * std::sqrt(10.0) * s0 * std::sqrt(r));
+    } else if (i > nf) {
+      const double d = tp * (double)(i / n) - tp * (double)(nf / n);
+      v = std::exp(-2.0 * d * d * d * d);
+    }
+    w[i - 1] = v;
+  }
+  return w;
+}
+
+// A zero coefficient (a zero-initialised hos_params) means off, like 1.
+inline bool filter_active(double c) { return c < 1.0 && c != 0.0; }
+
+// 2-D weights of filtering_x (coefficient cx) then filtering_y (cy) on an
+// n1 x n2 modal array; the exponential y cut-off touches only the last x mode
+// (filtering.f90:513-517).
+std::vector<double> filter_weights_2d(int n1, int n2, double cx, double cy) {
+  std::vector<double> w((size_t)n1 * n2, 1.0);
+  if (filter_active(cx)) {
+    const std::vector<double> wx = filter_weights(n1, cx);
+    for (int i2 = 0; i2 < n2; ++i2)
+      for (int i1 = 0; i1 < n1; ++i1) w[i1 + (size_t)n1 * i2] *= wx[i1];
+  }
+  if (n2 > 1 && filter_active(cy)) {
+    const double tiny = std::numeric_limits<double>::epsilon();
+    bool expo = cy < 0.0;
+    for (double t : {1.0, 2.0, 3.0, 4.0, 5.0})
+      if (std::fabs(cy + t) < tiny) expo = false;
+    const std::vector<double> wy = filter_weights(n2, cy);
+    for (int i2 = 0; i2 < n2; ++i2)
+      for (int i1 = expo ? n1 - 1 : 0; i1 < n1; ++i1) w[i1 + (size_t)n1 * i2] *= wy[i2];
+  }
+  return w;
+}
+
+// ---------------------------------------------------------------------------
 // base-grid transforms as dense products
 // ---------------------------------------------------------------------------
 // At the reference sizes the base-grid lengths 2(n-1) carry large primes
@@ -995,6 +1073,8 @@ struct hos_ctx {
   double *kG1[5], *kG2[5], *kad1[5], *kadr[5], *ka1[5], *ke1[5];
   double *partial, *pmax_e, *pmax_x, *red6;
   double *phys_tmp, *nu_d;
+  double* filt_fs = nullptr;  // free-surface mode filter weights (n1*n2), null when off
+  double* filt_z = nullptr;   // paddle z-mode filter weights (n3_add), null when off
   double *phi_add_last;
   hos_stats stats{};
   bool have_trial = false;
@@ -1339,7 +1419,7 @@ struct Engine : EngineBase {
     base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
     hipLaunchKernelGGL(k_expon_out<T>, dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                        (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
-                       (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, kG1, kG2);
+                       (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)C.filt_fs, kG1, kG2);
     HIP_OK(hipMemcpyAsync(C.phi_add_last, phi_add, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
 
@@ -1442,6 +1522,7 @@ struct Engine : EngineBase {
       zin = fz2;
     }
     hipLaunchKernelGGL(k_add_scale, dim3(nblk(na * nfields)), dim3(256), 0, cs, n2, n3a, nfields, C.p.xlen, C.piy,
+                       (const double*)C.filt_z,
                        zin, out1, out2);
     HIP_OK(hipGetLastError());
   }
@@ -1726,6 +1807,16 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
       HIP_OK(hipMemcpy(c->nu_d, p.nu, ns * sizeof(double), hipMemcpyHostToDevice));
     }
     c->p.nu = nullptr;
+    if (filter_active(p.coeffilt[0]) || (p.n2 > 1 && filter_active(p.coeffilt[1]))) {
+      const std::vector<double> w = filter_weights_2d(p.n1, p.n2, p.coeffilt[0], p.coeffilt[1]);
+      c->filt_fs = ar.get<double>(ns);
+      HIP_OK(hipMemcpy(c->filt_fs, w.data(), ns * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (filter_active(p.coeffilt[2])) {
+      const std::vector<double> w = filter_weights(c->n3a, p.coeffilt[2]);
+      c->filt_z = ar.get<double>(c->n3a);
+      HIP_OK(hipMemcpy(c->filt_z, w.data(), c->n3a * sizeof(double), hipMemcpyHostToDevice));
+    }
     c->freq.assign(p.freq, p.freq + p.n_freq);
     c->pre0.resize(p.n_freq);
     c->pim0.resize(p.n_freq);
@@ -1941,6 +2032,25 @@ int hos_get_wmk(hos_ctx* c, double* alpha, double* pos_top) {
 
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy) {
   return guard([&] { c->eng->slope_max(*sx, *sy); });
+}
+
+int hos_check_slope(hos_ctx* c, double threshold, double coeff, int* fx, int* fy) {
+  return guard([&] {
+    double sx = 0.0, sy = 0.0;
+    c->eng->slope_max(sx, sy);
+    *fx = sx > threshold ? 1 : 0;
+    *fy = (sy > threshold && c->n2 != 1) ? 1 : 0;
+    if (!*fx && !*fy) return;
+    // filtering_x then filtering_y of the committed eta and phis, i.e. their
+    // modes times the filter weights
+    const size_t ns = (size_t)c->n1 * c->n2;
+    const std::vector<double> w = filter_weights_2d(c->n1, c->n2, *fx ? coeff : 1.0, *fy ? coeff : 1.0);
+    HIP_OK(hipMemcpyAsync(c->phys_tmp, w.data(), ns * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_mulw, dim3(nblk(ns)), dim3(256), 0, c->st, ns, (const double*)c->phys_tmp, c->eta_hat);
+    hipLaunchKernelGGL(k_mulw, dim3(nblk(ns)), dim3(256), 0, c->st, ns, (const double*)c->phys_tmp, c->phis_hat);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->st));
+  });
 }
 
 int hos_time_big_transform(hos_ctx* c, int reps, double* out4) {

@@ -33,7 +33,7 @@ REAL(RP), ALLOCATABLE, TARGET :: nu(:,:), eta(:,:), phis(:,:), deta(:,:), phi_ad
 REAL(RP), ALLOCATABLE, TARGET :: freq(:), pre(:), pim(:)
 TYPE(hos_params) :: p
 TYPE(C_PTR) :: ctx
-INTEGER(C_INT) :: n_acc, n_rej
+INTEGER(C_INT) :: n_acc, n_rej, fx, fy
 INTEGER :: i1, i2, n_rk_tot, n_er_tot, unit_in
 !
 n1 = 513; n2 = 1; n3 = 33; mHOS = 3; prec = 0
@@ -178,6 +178,12 @@ DO WHILE (time_cur < T_stop + delt_out)
    h_rk = delt
    CALL hos_check(hos_advance(ctx, time_cur, time_next, h_rk, toler, dt_stab, delt_out, n_acc, n_rej), 'hos_advance')
    delt = h_rk
+   ! check_slope (initial_condition.f90:687-716)
+   CALL hos_check(hos_check_slope(ctx, 8.0_rp, 0.7_rp, fx, fy), 'hos_check_slope')
+   IF (fx /= 0 .OR. fy /= 0) THEN
+      WRITE(*,'(A)') '******************** Slope of FS elevation too high ******************** '
+      WRITE(*,'(A)') 'Trying to filter eta and phis...'
+   END IF
 END DO
 CLOSE(2)
 CALL hos_check(hos_destroy(ctx), 'hos_destroy')

@@ -15,6 +15,7 @@ TYPE, BIND(C) :: hos_params
    TYPE(C_PTR)    :: nu
    INTEGER(C_INT) :: n_freq
    TYPE(C_PTR)    :: freq, pos_ini_re, pos_ini_im
+   REAL(C_DOUBLE) :: coeffilt(3) = 1.0_C_DOUBLE
 END TYPE hos_params
 !
 INTERFACE
@@ -83,6 +84,12 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: sx, sy
    END FUNCTION hos_get_slope_max
+   INTEGER(C_INT) FUNCTION hos_check_slope(ctx, threshold, coeff, fx, fy) BIND(C, NAME='hos_check_slope')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: threshold, coeff
+      INTEGER(C_INT), INTENT(OUT) :: fx, fy
+   END FUNCTION hos_check_slope
 END INTERFACE
 !
 CONTAINS

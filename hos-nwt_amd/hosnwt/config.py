@@ -120,6 +120,8 @@ class TankConfig:
     toler: float = 1e-7
     f_out: float = 1.0
     precision: int = 0
+    # spectral filters (common_vars.f90:43): x, y free-surface modes, z paddle modes
+    coeffilt: tuple = (1.0, 1.0, 1.0)
     # icase = 3: components in SI (freq Hz, ampli m, angle deg, phase rad)
     components: dict | None = None
 

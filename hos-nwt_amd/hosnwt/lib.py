@@ -26,7 +26,7 @@ class HosParams(C.Structure):
         ("xlen", C.c_double), ("ylen", C.c_double), ("d_hinge", C.c_double),
         ("Tramp", C.c_double), ("T_stop", C.c_double), ("T_mono", C.c_double),
         ("nu", _dp), ("n_freq", C.c_int), ("freq", _dp),
-        ("pos_ini_re", _dp), ("pos_ini_im", _dp),
+        ("pos_ini_re", _dp), ("pos_ini_im", _dp), ("coeffilt", C.c_double * 3),
     ]
 
 
@@ -41,7 +41,7 @@ EXPORTS = [
     "hos_create", "hos_destroy", "hos_last_error", "hos_initialize", "hos_set_state",
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
     "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
-    "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform",
+    "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_check_slope",
 ]
 
 _lib = None
@@ -70,6 +70,7 @@ def load() -> C.CDLL:
     lib.hos_rhs.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 11
     lib.hos_get_physical.argtypes = [vp] + [_dp] * 4
     lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
+    lib.hos_check_slope.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.hos_get_wmk.argtypes = [vp, _dp, _dp]
     lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
     lib.hos_synchronize.argtypes = [vp]

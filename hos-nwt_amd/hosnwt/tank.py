@@ -45,6 +45,8 @@ class Tank:
         p.freq = ptr(fr) if len(fr) else None
         p.pos_ini_re = ptr(pr) if len(fr) else None
         p.pos_ini_im = ptr(pi) if len(fr) else None
+        for i in range(3):
+            p.coeffilt[i] = float(cfg.coeffilt[i])
         h = C.c_void_p()
         check(self.lib.hos_create(C.byref(p), C.byref(h)))
         self.h = h
@@ -123,6 +125,12 @@ class Tank:
         sx, sy = C.c_double(), C.c_double()
         check(self.lib.hos_get_slope_max(self.h, C.byref(sx), C.byref(sy)))
         return sx.value, sy.value
+
+    def check_slope(self, threshold: float = 8.0, coeff: float = 0.7):
+        """check_slope (initial_condition.f90:687-716): filter eta, phis on steep slopes."""
+        fx, fy = C.c_int(), C.c_int()
+        check(self.lib.hos_check_slope(self.h, threshold, coeff, C.byref(fx), C.byref(fy)))
+        return bool(fx.value), bool(fy.value)
 
     def stats(self):
         s = HosStats()

@@ -28,6 +28,8 @@
  *                       (main/HOS-NWT.f90:579, IO/vol_energy.f90:69)
  *   hos_get_slope_max : max |eta_x|, |eta_y| read by check_slope
  *                       (main/initial_condition.f90:694-696)
+ *   hos_check_slope   : check_slope with its filtering of eta and phis
+ *                       (main/initial_condition.f90:687-716)
  *   hos_get_wmk       : alpha and pos1st(n3,1) read by the volume output
  *                       (IO/output.f90:251-256)
  *   hos_space_2_fourier / hos_fourier_2_space (+ _big):
@@ -63,6 +65,12 @@ typedef struct hos_params {
   const double* freq; /* n_freq frequencies (wavemaking.f90:136) */
   const double* pos_ini_re; /* pos_ini(i2,f): n2*n_freq, i2 fastest */
   const double* pos_ini_im;
+  /* Spectral filters (common_vars.f90:43, filtering.f90): coefficient of the
+   * x and y free-surface modes and of the paddle z modes; 1 = off (the
+   * reference default), >= 0 keeps the first INT(n*c) modes, -1..-5 select
+   * the shaped filters of filtering_base, other negatives the exponential
+   * cut-off.  0 (a zero-initialised struct) also means off. */
+  double coeffilt[3];
 } hos_params;
 
 /* Kernel statistics of the last call (transform counts, bytes). */
@@ -109,6 +117,11 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
 
 int hos_get_physical(hos_ctx* c, double* eta, double* phis, double* deta, double* phi_add);
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy);
+/* check_slope (initial_condition.f90:687-716): when max |eta_x| (|eta_y|) of
+ * the last right-hand side exceeds `threshold` (8 in the reference), filter
+ * the committed eta and phis along x (y) with coefficient `coeff` (0.7).
+ * *fx / *fy report which filters ran. */
+int hos_check_slope(hos_ctx* c, double threshold, double coeff, int* fx, int* fy);
 /* Ramp value alpha and paddle-top displacement pos1st(n3,1) of the last
  * wavemaker evaluation, the module globals output.f90:251-256 subtracts
  * from the volume (resol_wmkr.f90:1268, wavemaking.f90:277). */

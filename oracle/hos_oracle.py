@@ -223,6 +223,8 @@ class Params:
     xabsf: float = 0.8
     coeffabsf: float = 1.0
     i_wmk: int = 1
+    # spectral filters (common_vars.f90:43): x, y free-surface modes, z paddle modes
+    coeffilt: tuple = (1.0, 1.0, 1.0)
     # wavemaker spectral content: pos_ini (n2 x n_freq complex) and freq (n_freq)
     freq: np.ndarray = field(default_factory=lambda: np.zeros(0))
     pos_ini: np.ndarray = field(default_factory=lambda: np.zeros((1, 0), complex))
@@ -515,6 +517,75 @@ def phiadd_sl_bruteforce(a_add, da_add, eta, tb: Tables):
 # ---------------------------------------------------------------------------
 # solver state
 # ---------------------------------------------------------------------------
+# ---------------------------------------------------------------------------
+# spectral filters (filtering.f90)
+# ---------------------------------------------------------------------------
+def filter_nfilt(n: int, coeff: float) -> int:
+    """Number of kept modes (filtering.f90:53-55): ABS(INT(n*coeff)), n when coeff >= 0.9999."""
+    return n if coeff >= 0.9999 else abs(int(n * coeff))
+
+
+def filter_weights(n: int, coeff: float, first_dir: bool = True) -> np.ndarray:
+    """Weights of modes i = 1..n along one direction (filtering_base, filtering.f90:334-525).
+
+    Returned as an array over 0-based mode index i-1.  The reference's integer
+    quotients REAL(i/n) (types -1 and the exponential cut-off) are kept, and so
+    is the exponential y branch that only touches the last x mode, which
+    ``first_dir=False`` cannot express per direction: callers apply
+    :func:`filter_modes` for that case.
+    """
+    nf = filter_nfilt(n, coeff)
+    i = np.arange(1, n + 1)
+    w = np.ones(n)
+    tiny = TINY
+    if coeff >= 0.0:
+        w[i > nf] = 0.0
+    elif abs(coeff + 1.0) < tiny:
+        w = 1.0 - (i // n).astype(float)
+    elif abs(coeff + 2.0) < tiny:
+        a = TWOPI * i / n
+        w = np.sin(a) / a
+    elif abs(coeff + 3.0) < tiny:
+        w = (1.0 + np.cos(TWOPI * i / n)) / 2.0
+    elif abs(coeff + 4.0) < tiny:
+        s0 = (1.0 + np.cos(TWOPI * i / n)) / 2.0
+        w = s0 ** 4 * (35.0 - 84.0 * s0 + 70.0 * s0 ** 2 - 20.0 * s0 ** 3)
+    elif abs(coeff + 5.0) < tiny:
+        from scipy.special import erfc
+        pp = 10
+        s0 = (i - 0.75 * n) / (n + 1.0)
+        q = (4.0 / 3.0 * s0) ** 2
+        # -log(1-q)/q -> 1 as q -> 0: the mode i = 3n/4 (s0 = 0) takes the limit
+        # value 1/2 where the literal expression would be 0/0
+        r = np.ones(n)
+        nz = q > 0.0
+        r[nz] = -np.log(1.0 - q[nz]) / q[nz]
+        w = 0.5 * erfc(2.0 * math.sqrt(pp) * s0 * np.sqrt(r))
+    else:
+        w = np.ones(n)
+        sel = i > nf
+        w[sel] = np.exp(-2.0 * (TWOPI * (i[sel] // n) - TWOPI * (nf // n)) ** 4)
+    return w
+
+
+def filter_modes(a: np.ndarray, coeff: float, axis: int) -> np.ndarray:
+    """filtering_base on a modal (n1, n2) array along x (axis 0) or y (axis 1)."""
+    n1, n2 = a.shape
+    out = a.copy()
+    if axis == 0:
+        out *= filter_weights(n1, coeff)[:, None]
+        return out
+    nf = filter_nfilt(n2, coeff)
+    if coeff < 0.0 and not any(abs(coeff + c) < TINY for c in (1.0, 2.0, 3.0, 4.0, 5.0)):
+        # exponential cut-off along y touches only the last x mode (filtering.f90:513-517)
+        j = np.arange(1, n2 + 1)
+        sel = j > nf
+        out[n1 - 1, sel] *= np.exp(-2.0 * (TWOPI * (j[sel] // n2) - TWOPI * (nf // n2)) ** 4)
+        return out
+    out *= filter_weights(n2, coeff)[None, :]
+    return out
+
+
 @dataclass
 class State:
     """Driver state copied into runge4_adapt2 each attempt (HOS-NWT.f90:506-515)."""
@@ -712,8 +783,13 @@ class Oracle:
         dr[:n3] = -alt * self.dpos1stdt - altt * self.pos1st
         dr = match_surface_wmk(dr, n3, tb.delz)
         d1 = match_surface_wmk(d1, n3, tb.delz)
-        d1 = self.FA.space_2_fourier(d1) / tb.k_add_thk_add
-        dr = self.FA.space_2_fourier(dr) / tb.k_add_thk_add
+        d1 = self.FA.space_2_fourier(d1)
+        dr = self.FA.space_2_fourier(dr)
+        if p.coeffilt[2] < 1.0:
+            d1 = filter_modes(d1, p.coeffilt[2], 0)
+            dr = filter_modes(dr, p.coeffilt[2], 0)
+        d1 = d1 / tb.k_add_thk_add
+        dr = dr / tb.k_add_thk_add
         # phiadd_1st_SWEET at z=0: only phi_add_1, phit_add_1 and phiz_add_1 reach the RK path
         if self.wmk_direct:
             sl = phiadd_sl_bruteforce(a_add_1, d1, None, tb)
@@ -797,6 +873,12 @@ class Oracle:
             dphin = detark + h["phiz"] + phiz_add - gradeta_gradphi
             detark = dphin - h["W1"]
             dphisrk = 0.5 * dphisrk - phit_add - tb.nu * dphin - gradphis_gradphi
+        if p.coeffilt[0] < 1.0:
+            detark = self.filtering(detark, p.coeffilt[0], 0)
+            dphisrk = self.filtering(dphisrk, p.coeffilt[0], 0)
+        if p.n2 != 1 and p.coeffilt[1] < 1.0:
+            detark = self.filtering(detark, p.coeffilt[1], 1)
+            dphisrk = self.filtering(dphisrk, p.coeffilt[1], 1)
         NL1 = self.s2f(detark)
         NL2 = self.s2f(dphisrk)
         dphisrk_1 = NL2[0, 0]
@@ -814,6 +896,31 @@ class Oracle:
         if p.igeom == 1:
             return x + nl11 * x
         return 0.0
+
+    # ----- filtering_x / filtering_y (filtering.f90:37-134) -----------------
+    def filtering(self, f, coeff, axis):
+        """Filter a physical base-grid field: S2F, filtering_base, F2S."""
+        return self.f2s(filter_modes(self.s2f(f), coeff, axis))
+
+    # ----- check_slope (initial_condition.f90:687-716) ------------------------
+    def check_slope(self, st: State, etax_ext, etay_ext, threshold=8.0, coeff=0.7):
+        """Filter eta and phis when a slope of the extended-grid eta exceeds the threshold.
+
+        Returns (state, filtered_x, filtered_y); etax_ext / etay_ext are the
+        extended-grid slopes of the last right-hand side.
+        """
+        sx = float(np.max(np.abs(etax_ext))) if etax_ext is not None else 0.0
+        sy = float(np.max(np.abs(etay_ext))) if etay_ext is not None else 0.0
+        st = st.copy()
+        fx = sx > threshold
+        fy = sy > threshold and self.p.n2 != 1
+        if fx:
+            st.phis = self.filtering(st.phis, coeff, 0)
+            st.eta = self.filtering(st.eta, coeff, 0)
+        if fy:
+            st.phis = self.filtering(st.phis, coeff, 1)
+            st.eta = self.filtering(st.eta, coeff, 1)
+        return st, fx, fy
 
     # ----- runge4_adapt2 (runge_kutta.f90:81-360) ----------------------------
     A = np.array([[0, 0, 0, 0, 0], [0.5, 0, 0, 0, 0], [0, 0.5, 0, 0, 0], [0, 0, 1.0, 0, 0],

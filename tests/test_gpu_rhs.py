@@ -1,7 +1,8 @@
 """One solveHOS2 evaluation on the GPU (FP64) vs the oracle: relative L2 <= 1e-10.
 
 Covers the 2-D piston tank (M=3, M=5), the 3-D tank with a directional
-snake wavemaker (M=5), a hinged flap, and sloshing without wavemaker (M=5).
+snake wavemaker (M=5), a hinged flap, sloshing without wavemaker (M=5), and
+the spectral filters of filtering.f90 on the free-surface and paddle modes.
 Each case calls the RHS twice so that the mean-volume fix, which reads the
 wavemaker state of the previous evaluation, is exercised with nonzero values.
 """
@@ -22,6 +23,9 @@ CASES = {
     "3d_snake_M5": dict(n1=24, n2=12, M=5, n3=9, xlen=8.0, ylen=4.0),
     "3d_slosh_M5": dict(n1=20, n2=10, M=5, n3=9, xlen=4.0, ylen=2.0, icase=1),
     "2d_M1": dict(n1=32, n2=1, M=1, n3=9),
+    # spectral filters (filtering.f90): shaped x filter, cut-off y filter, paddle z filter
+    "3d_filtered_M5": dict(n1=24, n2=12, M=5, n3=9, xlen=8.0, ylen=4.0, coeffilt=(-3.0, 0.7, 0.8)),
+    "2d_filtered_M3": dict(n1=40, n2=1, M=3, n3=9, coeffilt=(-4.0, 1.0, -2.0)),
 }
 
 
@@ -55,5 +59,39 @@ def test_rhs_parity(name):
             print(name, call, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
             for k, v in errs.items():
                 assert v < TOL, (name, call, k, v)
+    finally:
+        tank.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n2", [1, 10])
+def test_check_slope_filters_committed_state(n2):
+    """hos_check_slope with a threshold below the slopes filters eta and phis exactly
+    like the oracle's check_slope (filtering_x, then filtering_y, coefficient 0.7)."""
+    kw = dict(n1=32, n2=n2, M=3, n3=9, xlen=6.0, ylen=3.0, icase=1)
+    s = make_setup(**kw)
+    orc = ho.Oracle(oracle_params(s))
+    tank = Tank(s, precision=0)
+    try:
+        rng = np.random.default_rng(3)
+        eta = 0.01 * rng.standard_normal((32, n2))
+        phis = 0.01 * rng.standard_normal((32, n2))
+        z = np.zeros((32, n2))
+        za = np.zeros((2 * (9 - 1), n2))
+        tank.set_state(eta, phis, za, za, z, z)
+        tank.runge4_adapt2(0.0, 0.01)  # any right-hand side fills the slope arrays
+        sx, sy = tank.slope_max()
+        assert sx > 0.0
+        fx, fy = tank.check_slope(threshold=1e9)
+        assert not fx and not fy
+        fx, fy = tank.check_slope(threshold=0.0)
+        assert fx and fy == (n2 > 1)
+        st = tank.get_state()
+        e_ref = orc.filtering(eta, 0.7, 0)
+        p_ref = orc.filtering(phis, 0.7, 0)
+        if n2 > 1:
+            e_ref = orc.filtering(e_ref, 0.7, 1)
+            p_ref = orc.filtering(p_ref, 0.7, 1)
+        assert rel(st["eta"], e_ref) < TOL and rel(st["phis"], p_ref) < TOL
     finally:
         tank.close()

@@ -292,3 +292,74 @@ def test_rk_tableau_on_linear_rhs():
     etark, _ = o.gtof(G1n, G2n, h)
     etark[0, 0] = 0.0  # no wavemaker: the mean-volume mode is reset to 0
     np.testing.assert_allclose(new.eta, o.f2s(etark), atol=1e-15, rtol=0)
+
+
+def test_filter_weights_properties():
+    """filtering_base weights, pinned by identities rather than by their formulas.
+
+    * c >= 0 keeps exactly the first INT(n c) modes (a projection);
+    * -3 is cos^2(pi i / n);
+    * -4 is the degree-7 smoothstep p(s) = I_s(4, 4): p(s) + p(1 - s) = 1 with
+      s = cos^2(pi i / n), and p = 1/2 at s = 1/2;
+    * -2 is sinc(2 pi i / n), vanishing at i = n/2;
+    * -5 (erfc profile) decreases from 1 to 0 and passes 1/2 at i = 3n/4;
+    * -1 zeroes only the last mode (the reference's integer quotient i/n).
+    """
+    n = 16
+    i = np.arange(1, n + 1)
+    w = ho.filter_weights(n, 0.6)
+    assert np.array_equal(w, (i <= int(16 * 0.6)).astype(float))
+    np.testing.assert_allclose(ho.filter_weights(n, -3), np.cos(np.pi * i / n) ** 2, atol=1e-15)
+    s = np.cos(np.pi * i / n) ** 2
+    w4 = ho.filter_weights(n, -4)
+    # 1 - s(i) = sin^2(pi i/n) = s(n/2 - i)
+    ii = np.arange(1, n // 2)
+    np.testing.assert_allclose(w4[ii - 1] + w4[n // 2 - ii - 1], 1.0, atol=1e-14)
+    assert abs(w4[n // 4 - 1] - 0.5) < 1e-14 and abs(s[n // 4 - 1] - 0.5) < 1e-14
+    w2 = ho.filter_weights(n, -2)
+    assert abs(w2[n // 2 - 1]) < 1e-15 and abs(w2[0] - np.sinc(2.0 / n)) < 1e-15
+    w5 = ho.filter_weights(n, -5)
+    assert np.all(np.diff(w5) < 0) and abs(w5[3 * n // 4 - 1] - 0.5) < 1e-15 and w5[0] > 0.999
+    w1 = ho.filter_weights(n, -1)
+    assert np.array_equal(w1, np.r_[np.ones(n - 1), 0.0])
+
+
+def test_filtering_acts_on_modes_and_is_placed_before_expon():
+    """filtering_x of a single cosine mode scales it by the mode weight; in solveHOS2
+    the filter multiplies dG1, dG2 (and the frozen mean term) by the 2-D weight."""
+    p = ho.Params(n1=24, n2=8, mHOS=3, xlen=4.0, ylen=2.0, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    x = np.arange(24) * 4.0 / 23
+    for k in (3, 20):
+        f = np.repeat(np.cos(k * np.pi * x / 4.0)[:, None], 8, axis=1)
+        g = o.filtering(f, 0.5, 0)
+        np.testing.assert_allclose(g, f * (1.0 if k < 12 else 0.0), atol=1e-13)
+    rng = np.random.default_rng(5)
+    kx = np.arange(24) * np.pi / 4.0
+    ky = np.arange(8) * np.pi / 2.0
+    dec = np.exp(-0.3 * np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2))
+    G1 = 0.01 * rng.standard_normal((24, 8)) * dec
+    G2 = 0.01 * rng.standard_normal((24, 8)) * dec
+    z = np.zeros((24, 8))
+    za = np.zeros((p.n3_add, 8))
+    ref = o.solve_hos2(z, z, G2, G1, za, 0.02)
+    pf = ho.Params(n1=24, n2=8, mHOS=3, xlen=4.0, ylen=2.0, icase=1, iabsnb=0, coeffilt=(-3.0, 0.5, 1.0))
+    of = ho.Oracle(pf)
+    got = of.solve_hos2(z, z, G2, G1, za, 0.02)
+    W = ho.filter_weights(24, -3.0)[:, None] * ho.filter_weights(8, 0.5)[None, :]
+    np.testing.assert_allclose(got[0], W * ref[0], atol=1e-16)
+    np.testing.assert_allclose(got[1], W * ref[1], atol=1e-16)
+
+
+def test_check_slope_filters_only_steep_states():
+    p = ho.Params(n1=32, n2=1, mHOS=3, xlen=4.0, icase=1, iabsnb=0)
+    o = ho.Oracle(p)
+    x = np.arange(32) * 4.0 / 31
+    eta = (0.01 * np.cos(np.pi * x / 4.0) + 0.001 * np.cos(30 * np.pi * x / 4.0))[:, None]
+    st = o.initialize(eta0=eta, phis0=0.5 * eta)
+    same, fx, fy = o.check_slope(st, np.array([1.0]), None)
+    assert not fx and not fy and np.array_equal(same.eta, st.eta)
+    filt, fx, _ = o.check_slope(st, np.array([9.0]), None)
+    assert fx
+    np.testing.assert_allclose(filt.eta[:, 0], 0.01 * np.cos(np.pi * x / 4.0), atol=1e-14)
+    np.testing.assert_allclose(filt.phis, 0.5 * filt.eta, atol=1e-14)
