@@ -38,6 +38,25 @@ struct Plan2 {
 
 __host__ __device__ __forceinline__ int padx(int p) { return p + (p >> 3); }
 
+// LDS position of element p in the exchange after a pass of radix R and
+// stride Ns.  The writing pass puts lane j at (j/Ns) R Ns + j%Ns + r Ns, the
+// reading pass takes aligned runs of consecutive p, and one LDS cycle serves
+// 256 bytes: a run of W = 32 doubles or 64 floats.
+//  x lines (one line per lane run):
+//    Ns = 1        p + p/W         stride-R writes and the runs both spread
+//    1 < Ns < W    p + Ns p/(R Ns) the Ns-long write runs land Ns banks apart
+//    Ns >= W       p               writes are runs already
+//  y tiles (lane = position-major, line-minor; element p of line cl at
+//  xpos(p) * ncl + cl): p + p/8 spreads both patterns over a run of W/ncl.
+template <typename T, bool YP, int R, int Ns>
+__device__ __forceinline__ int xpos(int p) {
+  constexpr int W = 256 / (int)sizeof(T);
+  if constexpr (YP) return p + (p >> 3);
+  else if constexpr (Ns == 1) return p + p / W;
+  else if constexpr (Ns < W) return p + Ns * (p / (R * Ns));
+  else return p;
+}
+
 template <int A, int... B>
 struct hosfft2_first {
   static constexpr int value = A;
@@ -87,28 +106,29 @@ __device__ __forceinline__ void pass_compute(C2<T>* v, int tl, int ntl, int Ns, 
 
 // Write this pass's outputs (component c of each value) to LDS at their
 // Stockham positions (j - k) R + k + r Ns.
-template <typename T, int V, int R>
-__device__ __forceinline__ void pass_write(const C2<T>* v, T* line, int tl, int ntl, int Ns, int comp) {
+template <typename T, int V, int R, int Ns, bool YP>
+__device__ __forceinline__ void pass_write(const C2<T>* v, T* line, int es, int tl, int ntl, int comp) {
 #pragma unroll
   for (int b = 0; b < V / R; ++b) {
     const int j = tl + b * ntl;
     const int k = j % Ns;
     const int o = (j - k) * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) line[padx(o + r * Ns)] = comp ? v[b * R + r].y : v[b * R + r].x;
+    for (int r = 0; r < R; ++r)
+      line[(YP ? es : 1) * xpos<T, YP, R, Ns>(o + r * Ns)] = comp ? v[b * R + r].y : v[b * R + r].x;
   }
 }
 
 // Read the next pass's inputs (radix R2): element j + r L/R2 for j = tl + b ntl.
-template <typename T, int V, int R2>
-__device__ __forceinline__ void pass_read(C2<T>* v, const T* line, int tl, int ntl, int L, int comp) {
+template <typename T, int V, int R2, int RW, int NsW, bool YP>
+__device__ __forceinline__ void pass_read(C2<T>* v, const T* line, int es, int tl, int ntl, int L, int comp) {
   const int stride = L / R2;
 #pragma unroll
   for (int b = 0; b < V / R2; ++b) {
     const int j = tl + b * ntl;
 #pragma unroll
     for (int r = 0; r < R2; ++r) {
-      const T x = line[padx(j + r * stride)];
+      const T x = line[(YP ? es : 1) * xpos<T, YP, RW, NsW>(j + r * stride)];
       if (comp)
         v[b * R2 + r].y = x;
       else
@@ -121,21 +141,21 @@ __device__ __forceinline__ void pass_read(C2<T>* v, const T* line, int tl, int n
 // first-pass layout, on exit the last-pass layout (value b*R_last + r at
 // position tl + b ntl + r L/R_last).  `line` is this line's LDS region; every
 // thread of the workgroup must call.
-template <typename T, int V, int L, int Ns, int R, int... Rest>
-__device__ __forceinline__ void run_passes(C2<T>* v, T* line, int tl, const C2<T>* t0, const C2<T>* t1) {
+template <typename T, int V, bool YP, int L, int Ns, int R, int... Rest>
+__device__ __forceinline__ void run_passes(C2<T>* v, T* line, int es, int tl, const C2<T>* t0, const C2<T>* t1) {
   constexpr int ntl = L / V;
   pass_compute<T, V, R>(v, tl, ntl, Ns, L, t0, t1);
   if constexpr (sizeof...(Rest) > 0) {
     constexpr int R2 = hosfft2_first<Rest...>::value;
-    pass_write<T, V, R>(v, line, tl, ntl, Ns, 0);
+    pass_write<T, V, R, Ns, YP>(v, line, es, tl, ntl, 0);
     __syncthreads();
-    pass_read<T, V, R2>(v, line, tl, ntl, L, 0);
+    pass_read<T, V, R2, R, Ns, YP>(v, line, es, tl, ntl, L, 0);
     __syncthreads();
-    pass_write<T, V, R>(v, line, tl, ntl, Ns, 1);
+    pass_write<T, V, R, Ns, YP>(v, line, es, tl, ntl, 1);
     __syncthreads();
-    pass_read<T, V, R2>(v, line, tl, ntl, L, 1);
+    pass_read<T, V, R2, R, Ns, YP>(v, line, es, tl, ntl, L, 1);
     __syncthreads();
-    run_passes<T, V, L, Ns * R, Rest...>(v, line, tl, t0, t1);
+    run_passes<T, V, YP, L, Ns * R, Rest...>(v, line, es, tl, t0, t1);
   }
 }
 
@@ -295,7 +315,9 @@ __global__ void __launch_bounds__(256) r2r_seq_kernel(int N, int dir, int ncl, L
   if (2 * grp * ncl >= nlines) return;  // uniform over the workgroup
   const int la = 2 * (grp * ncl + cl), lb = la + 1;
   const bool ha = la < nlines, hb = lb < nlines;
-  T* line = lds + (size_t)cl * pl.lp;
+  // exchange buffer of this lane's line: x lines are contiguous regions of lp
+  // values; the lines of a y tile are interleaved element by element
+  T* line = YP ? lds + cl : lds + (size_t)cl * pl.lp;
   const T* mult = nullptr;
   if (src.mul != 0) {
     // multipliers of the two real lines of every complex line: line[q*N + e]
@@ -306,14 +328,14 @@ __global__ void __launch_bounds__(256) r2r_seq_kernel(int N, int dir, int ncl, L
       lds[(size_t)c2 * pl.lp + h * N + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
     }
     __syncthreads();
-    mult = line;
+    mult = lds + (size_t)cl * pl.lp;
   }
   C2<T> v[V];
   load_line<T, V, R0, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb, mult);
   for (int m = tid; m < 64; m += blockDim.x) t0[m] = pl.tw0[m];
   for (int m = tid; m < (L + 63) / 64; m += blockDim.x) t1[m] = pl.tw1[m];
   __syncthreads();
-  run_passes<T, V, L, 1, Rs...>(v, line, tl, t0, t1);
+  run_passes<T, V, YP, L, 1, Rs...>(v, line, ncl, tl, t0, t1);
   store_line<T, V, Rl, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb);
 }
 

@@ -419,16 +419,6 @@ struct Grid2 {
       TX = sizeof(T) == 8 ? 8 : 16;
       while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 64 * 1024 || (TX / 2) * py.dev2.ntl > 256))
         TX /= 2;
-      // The lines of a y tile are interleaved across lanes (line index
-      // fastest), so lanes tl*ncl + cl address cl*lp + p(tl).  One LDS cycle
-      // serves 32 doubles or 64 floats; offsetting the line stride by the tl
-      // span of those lanes puts the ncl lines in disjoint banks.
-      const int ncl = TX / 2;
-      if (ncl > 1) {
-        const int period = sizeof(T) == 8 ? 32 : 64;
-        const int want = period / ncl;
-        while (py.dev2.lp % period != want) ++py.dev2.lp;
-      }
     }
   }
   void release() {
