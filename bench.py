@@ -106,9 +106,53 @@ def roofline(tank, precision):
     """Live HIP-event timing of the dominant kernel family (extended-grid r2r passes)."""
     r = tank.time_big_transform(reps=20)
     achieved = r["bytes"] / (r["ms"] * 1e-3) / 1e9
+    traffic = None
+    t = traffic_table()
+    st = tank.stats()
+    if t is not None and precision == 0 and list(t["grid"]) == [st["big_n1"], st["big_n2"]]:
+        traffic = t["traffic_bytes"][r["pass"]]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": None, "kernel": r["kernel"],
+            "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": r["kernel"],
             "bytes_per_launch": r["bytes"], "avg_ms": round(r["ms"], 5)}
+
+
+def traffic_table():
+    """Scaled FETCH_SIZE + WRITE_SIZE per launch of the extended-grid passes, from the committed
+    rocprofv3 summary (profiles/extended_pass_1024x256_M5_traffic.md); None when absent."""
+    path = os.path.join(ROOT, "profiles", "extended_pass_1024x256_M5_traffic.md")
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return None
+
+
+def pass_probe(reps):
+    """Extended-grid pass timing (the roofline launches) plus a 64 MiB device-to-device copy whose
+    blit kernel calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    from hosnwt.tank import Tank
+    s = make_setup(0)
+    tank = Tank(s)
+    tank.time_big_transform(reps=2)
+    r = tank.time_big_transform(reps=reps)
+    print(json.dumps({"pass_ms": {r["kernel"]: r["ms"], "other": r["other"]["ms"]}}), flush=True)
+    tank.close()
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 64 * 1024 * 1024
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)):
+        raise RuntimeError("hipMemset failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print("copied 64 MiB device-to-device 3 times", flush=True)
 
 
 def cpu_baseline(wl=WORKLOAD):
@@ -145,7 +189,12 @@ def main():
     ap.add_argument("--spin-up", type=int, default=0, help="untimed accepted/rejected attempts before warmup")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--pass-probe", type=int, default=0, metavar="REPS",
+                    help="only time the roofline passes REPS times and run the counter-calibration copy")
     args = ap.parse_args()
+    if args.pass_probe:
+        pass_probe(args.pass_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(ensembles and a single field both fit one MI355X).", file=sys.stderr)

@@ -142,10 +142,10 @@ class Tank:
         out = np.zeros(4)
         check(self.lib.hos_time_big_transform(self.h, reps, ptr(out)))
         if out[2] > out[0]:
-            return dict(kernel="r2r_y_kernel (extended grid, inverse cos-cos)", ms=out[2], bytes=out[3],
-                        other=dict(ms=out[0], bytes=out[1]))
-        return dict(kernel="r2r_x_kernel (extended grid, inverse cos-cos)", ms=out[0], bytes=out[1],
-                    other=dict(ms=out[2], bytes=out[3]))
+            return dict(kernel="r2r_seq_kernel y pass (extended grid, inverse cos)", ms=out[2],
+                        bytes=out[3], other=dict(ms=out[0], bytes=out[1]), **{"pass": "y"})
+        return dict(kernel="r2r_seq_kernel x pass (extended grid, inverse cos)", ms=out[0], bytes=out[1],
+                    other=dict(ms=out[2], bytes=out[3]), **{"pass": "x"})
 
     def synchronize(self):
         check(self.lib.hos_synchronize(self.h))
