@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -416,8 +417,12 @@ struct Grid2 {
            ((size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 || (TX / 2) * py.dev.L > 256 * hosfft::kValsPerThread))
       TX /= 2;
     if (N2 > 1 && py.V2) {
-      TX = sizeof(T) == 8 ? 8 : 16;
-      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 64 * 1024 || (TX / 2) * py.dev2.ntl > 256))
+      // y tile width: 16 columns (128-byte rows in FP64, 64 in FP32), measured
+      // fastest of 4..32 on the 3073 x 769 grid; HOSNWT_Y_TILE overrides it
+      int want = 16;
+      if (const char* e = std::getenv("HOSNWT_Y_TILE")) want = std::max(2, std::atoi(e));
+      TX = want;
+      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 128 * 1024 || (TX / 2) * py.dev2.ntl > 512))
         TX /= 2;
     }
   }
