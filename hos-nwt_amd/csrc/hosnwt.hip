@@ -719,7 +719,14 @@ class Arena {
   size_t bytes_ = 0;
 };
 
+// Extended grid size N' >= N: the smallest N' whose line length 2(N'-1) has a
+// compiled register-engine sequence, when that costs at most a third more
+// points; otherwise the smallest N' with a 2,3,5-smooth 2(N'-1).  Any N' >= N
+// leaves the retained modes unchanged (dealiasing factor (M+1)/2).
 int pad_smooth(int N) {
+  const long need = 2L * (N - 1);
+  for (int L : kSeqLengths)
+    if (L >= need && 3L * L <= 4L * need) return L / 2 + 1;
   int n = N;
   while (!smooth235(2L * (n - 1))) ++n;
   return n;

@@ -255,17 +255,20 @@ __global__ void __launch_bounds__(256) r2r_y_kernel(int N, int kind, int dir, in
 // Extended-grid line lengths with a compiled radix sequence (values per thread V).
 inline bool seq_supported(int L) {
   switch (L) {
-    case 6144: case 3072: case 1536: case 768: case 4096: case 2048: case 1024: case 512: return true;
+    case 12288: case 8192: case 6144: case 3072: case 1536: case 768: case 4096: case 2048: case 1024: case 512:
+      return true;
     default: return false;
   }
 }
 inline int seq_values(int L) {
   switch (L) {
-    case 6144: case 3072: case 1536: case 768: return 24;
-    case 4096: case 2048: return 16;
+    case 12288: case 6144: case 3072: case 1536: case 768: return 24;
+    case 8192: case 4096: case 2048: return 16;
     default: return 8;
   }
 }
+// Compiled lengths in increasing order (extended grids are padded to one).
+constexpr int kSeqLengths[] = {512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 12288};
 
 
 inline bool smooth235(long L) {
@@ -480,6 +483,8 @@ void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<
 template <typename T, bool YP, class S, class D>
 void launch_reg(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   switch (pl.dev2.L) {
+    case 12288: launch_seq<T, 24, YP, S, D, 8, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
+    case 8192: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8, 2>(pl, ncl, N, dir, b, st); break;
     case 6144: launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
     case 3072: launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, dir, b, st); break;
     case 1536: launch_seq<T, 24, YP, S, D, 8, 8, 8, 3>(pl, ncl, N, dir, b, st); break;
