@@ -468,6 +468,9 @@ inline int batch_max_lines(const LineBatch<S, D>& b) {
 
 template <typename T, int V, bool YP, class S, class D, int... Rs>
 void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+  static_assert(hosfft2::hosfft2_prod<Rs...>::value % V == 0, "V must divide the line length");
+  if (hosfft2::hosfft2_prod<Rs...>::value != pl.dev2.L)
+    throw std::runtime_error("radix sequence does not multiply to the line length " + std::to_string(pl.dev2.L));
   const size_t lds = (size_t)ncl * pl.dev2.lp * sizeof(T) + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
   const int nt = ncl * pl.dev2.ntl;
   const int nl = batch_max_lines(b);
@@ -483,7 +486,7 @@ void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<
 template <typename T, bool YP, class S, class D>
 void launch_reg(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   switch (pl.dev2.L) {
-    case 12288: launch_seq<T, 24, YP, S, D, 8, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
+    case 12288: launch_seq<T, 24, YP, S, D, 8, 8, 8, 8, 3>(pl, ncl, N, dir, b, st); break;
     case 8192: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8, 2>(pl, ncl, N, dir, b, st); break;
     case 6144: launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
     case 3072: launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, dir, b, st); break;

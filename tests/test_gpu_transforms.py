@@ -1,7 +1,8 @@
 """GPU r2r transforms vs the oracle's (FFTW-definition) transforms.
 
 FP64: relative L2 <= 1e-13; FP32: relative L2 <= 2e-6.  Sizes cover smooth
-lengths (Stockham radix 2/3/4/5/8) and non-smooth ones (Bluestein):
+lengths (Stockham radix 2/3/4/5/8), every compiled register-engine length
+(512 ... 12288) and non-smooth ones (Bluestein):
 2(n-1) = 16, 32, 510 (2.3.5.17), 2046 (2.3.11.31), 6144, 1536, 4094 (2.23.89).
 """
 import numpy as np
@@ -11,8 +12,8 @@ from helpers import rel
 from hosnwt import lib
 from oracle import hos_oracle as ho
 
-SIZES_1D = [5, 9, 17, 33, 129, 256, 257, 513, 1024, 2048, 3073]
-SIZES_2D = [(9, 5), (33, 17), (256, 64), (1024, 256), (3073, 769)]
+SIZES_1D = [5, 9, 17, 33, 129, 256, 257, 385, 513, 769, 1024, 1025, 1537, 2048, 2049, 3073, 4097, 6145]
+SIZES_2D = [(9, 5), (33, 17), (256, 64), (1024, 256), (3073, 769), (2049, 513)]
 
 
 @pytest.mark.gpu
