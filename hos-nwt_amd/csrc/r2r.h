@@ -420,9 +420,11 @@ struct Grid2 {
            ((size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 || (TX / 2) * py.dev.L > 256 * hosfft::kValsPerThread))
       TX /= 2;
     if (N2 > 1 && py.V2) {
-      // y tile width: 16 columns (128-byte rows in FP64, 64 in FP32), measured
-      // fastest of 4..32 on the 3073 x 769 grid; HOSNWT_Y_TILE overrides it
-      int want = 16;
+      // y tile width: 8 columns.  A lone 3073 x 769 pass is 3-13 % faster with
+      // 16 (longer row segments), but the batched launches of a right-hand side
+      // run 2-7 % faster with 8, whose smaller LDS footprint keeps two tiles per
+      // CU; HOSNWT_Y_TILE overrides it
+      int want = 8;
       if (const char* e = std::getenv("HOSNWT_Y_TILE")) want = std::max(2, std::atoi(e));
       TX = want;
       while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 128 * 1024 || (TX / 2) * py.dev2.ntl > 512))
