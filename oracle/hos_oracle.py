@@ -228,6 +228,8 @@ class Params:
     # wavemaker spectral content: pos_ini (n2 x n_freq complex) and freq (n_freq)
     freq: np.ndarray = field(default_factory=lambda: np.zeros(0))
     pos_ini: np.ndarray = field(default_factory=lambda: np.zeros((1, 0), complex))
+    # y derivative of pos_ini (wavemaking.f90:147-149), needed for i_wmk >= 2; None: zero
+    dposdy_ini: np.ndarray | None = None
 
     def __post_init__(self):
         if self.p1 is None:
@@ -241,6 +243,8 @@ class Params:
         if self.n2 == 1:
             self.ylen = 1.0
         self.n3_add = L_ADD * (self.n3 - 1)
+        if self.dposdy_ini is None:
+            self.dposdy_ini = np.zeros_like(np.asarray(self.pos_ini, complex))
         self.has_wmk = self.icase in (2, 3, 31, 32, 33, 4, 41, 42) and self.igeom in (1, 2)
 
 
@@ -285,8 +289,18 @@ class Tables:
                 sel = self.cx >= x0
                 x3 = (self.cx[sel] - x0) / x2
                 self.nu[sel, :] = (p.coeffabsf * x3 * x3)[:, None]
-        # wavemaker vertical shape (wavemaking.f90:184-215)
+        # beach x derivative as the reference builds it, d nu / d x3 = 2 c x3 without the
+        # 1/(xlen - x0) of the chain rule (initial_condition.f90:768); no side beach: dnuy = 0
+        self.dnux = np.zeros((n1, n2))
+        self.dnuy = np.zeros((n1, n2))
+        if p.iabsnb != 0 and abs(p.xabsf - 1.0) > TINY:
+            x0 = p.xlen * p.xabsf
+            sel = self.cx >= x0
+            x3 = (self.cx[sel] - x0) / (p.xlen - x0)
+            self.dnux[sel, :] = (p.coeffabsf * 2.0 * x3)[:, None]
+        # wavemaker vertical shape and its derivative (wavemaking.f90:184-250)
         self.gz = np.array([gzed(z, p.igeom, p.d_hinge) for z in self.cz])
+        self.dgz = np.array([dgzeddz(z, p.igeom, p.d_hinge) for z in self.cz])
 
     @staticmethod
     def _kth(k: np.ndarray, M: int) -> np.ndarray:
@@ -331,6 +345,19 @@ def gzed(z: float, igeom: int, d_hinge: float) -> float:
 # ---------------------------------------------------------------------------
 # wavemaker motion per stage (resol_wmkr.f90:1268-1397, wavemaking.f90:259-304)
 # ---------------------------------------------------------------------------
+def dgzeddz(z: float, igeom: int, d_hinge: float) -> float:
+    """Vertical derivative of the paddle shape (wavemaking.f90:222-250)."""
+    delta = min(d_hinge, (1.0 - d_hinge) / 10.0)
+    if igeom == 1:
+        return 0.0
+    if z >= d_hinge - 1.0 + delta:
+        return 1.0 / (1.0 - d_hinge)
+    if z <= d_hinge - 1.0 - delta:
+        return 0.0
+    u = (z - (d_hinge - 1.0)) / delta
+    return 1.0 / (1.0 - d_hinge) * (0.5 + 15.0 / 16.0 * (u - 2.0 / 3.0 * u ** 3 + u ** 5 / 5.0))
+
+
 def ramp(t: float, p: Params):
     """alpha, alpha_t, alpha_tt of the time ramp (resol_wmkr.f90:1268-1397)."""
     Tr, Ts = p.Tramp, p.T_stop
@@ -411,6 +438,19 @@ def pos_1st(t: float, p: Params, tb: Tables):
     return (gz * s0[None, :]).real, (gz * s1[None, :]).real, (gz * s2[None, :]).real
 
 
+def dpos_1st(t: float, p: Params, tb: Tables):
+    """dX/dy, d2X/dydt, dX/dz, d2X/dzdt on the paddle, shape (n3, n2) (wavemaking.f90:306-406)."""
+    w = TWOPI * p.freq
+    e = np.exp(1j * w * t)
+    by = p.dposdy_ini * e[None, :]
+    bz = p.pos_ini * e[None, :]
+    sy0, sy1 = by.sum(axis=1), (by * (1j * w)[None, :]).sum(axis=1)
+    sz0, sz1 = bz.sum(axis=1), (bz * (1j * w)[None, :]).sum(axis=1)
+    gz, dgz = tb.gz[:, None], tb.dgz[:, None]
+    return ((gz * sy0[None, :]).real, (gz * sy1[None, :]).real,
+            (dgz * sz0[None, :]).real, (dgz * sz1[None, :]).real)
+
+
 def match_surface_coeffs(f, df, d2f):
     """Coefficients c1..c4 of the C2 continuation polynomial.
 
@@ -488,6 +528,21 @@ def phiadd_sl(a_add, da_add, eta, tb: Tables, need=("phi", "phix", "phiy", "phiz
             out["phiy"] -= c * fourier_2_space_y_axis(a * csh * tb.ky[None, :], "sin", 1)
         if "phiz" in need:
             out["phiz"] -= s * fourier_2_space_y_axis(a * csh * kxa, "cos", 1)
+        # fields of phiadd_1st_SWEET needed by i_wmk >= 2 (resol_wmkr.f90:500-524)
+        if "phizz" in need:
+            out["phizz"] -= c * fourier_2_space_y_axis(a * csh * kxa * kxa, "cos", 1)
+        if "phixt" in need:
+            out["phixt"] -= c * fourier_2_space_y_axis(da * ksh, "cos", 1)
+        if "phiyt" in need:
+            out["phiyt"] -= c * fourier_2_space_y_axis(da * csh * tb.ky[None, :], "sin", 1)
+        if "phizt" in need:
+            out["phizt"] -= s * fourier_2_space_y_axis(da * csh * kxa, "cos", 1)
+        if "phixx" in need:
+            out["phixx"] += c * fourier_2_space_y_axis(a * csh * tb.k_add[i3][None, :] ** 2, "cos", 1)
+        if "phixz" in need:
+            out["phixz"] += s * fourier_2_space_y_axis(a * ksh * kxa, "cos", 1)
+        if "phiyz" in need:
+            out["phiyz"] += s * fourier_2_space_y_axis(a * csh * tb.ky[None, :], "sin", 1)
     return out
 
 
@@ -596,10 +651,22 @@ class State:
     a_add_ramp: np.ndarray
     a1st: np.ndarray
     eta1st: np.ndarray
+    # second-order wavemaker state (i_wmk >= 2): paddle modes, modal and physical FS parts
+    a2nd_add: np.ndarray | None = None
+    a2nd: np.ndarray | None = None
+    eta2nd: np.ndarray | None = None
+
+    def __post_init__(self):
+        if self.a2nd_add is None:
+            self.a2nd_add = np.zeros_like(self.a1st_add)
+        if self.a2nd is None:
+            self.a2nd = np.zeros_like(self.a1st)
+        if self.eta2nd is None:
+            self.eta2nd = np.zeros_like(self.eta1st)
 
     def copy(self):
         return State(*(x.copy() for x in (self.eta, self.phis, self.a1st_add, self.a_add_ramp,
-                                          self.a1st, self.eta1st)))
+                                          self.a1st, self.eta1st, self.a2nd_add, self.a2nd, self.eta2nd)))
 
 
 class Oracle:
@@ -771,6 +838,9 @@ class Oracle:
                     geta2phiz2=geta2phiz2, phiz2=phiz2)
 
     # ----- wavemaker first order (resol_wmkr.f90:109-210) ----------------
+    SECOND_FIELDS = ("phi", "phit", "phix", "phiy", "phiz", "phizz", "phixt", "phiyt", "phizt", "phixx",
+                     "phixz", "phiyz")
+
     def solve_sweet_1(self, a1st, eta1st, a_add_1):
         p, tb = self.p, self.tb
         self.alpha, self.alphat, self.alphatt = ramp(self.time, p)
@@ -790,21 +860,116 @@ class Oracle:
             dr = filter_modes(dr, p.coeffilt[2], 0)
         d1 = d1 / tb.k_add_thk_add
         dr = dr / tb.k_add_thk_add
-        # phiadd_1st_SWEET at z=0: only phi_add_1, phit_add_1 and phiz_add_1 reach the RK path
-        if self.wmk_direct:
+        second = p.i_wmk >= 2
+        # phiadd_1st_SWEET at z=0 (resol_wmkr.f90:446-574); the RK path needs phit and phiz,
+        # the second-order wavemaker also its derivatives
+        if self.wmk_direct and not second:
             sl = phiadd_sl_bruteforce(a_add_1, d1, None, tb)
         else:
-            sl = phiadd_sl(a_add_1, d1, None, tb, need=("phit", "phiz"))
+            sl = phiadd_sl(a_add_1, d1, None, tb, need=self.SECOND_FIELDS if second else ("phit", "phiz"))
         # phidphiz (resol_wmkr.f90:583-636)
         phiz1st = self.f2s(a1st * tb.kth[:, :, 0]) + sl["phiz"]
         da1 = -eta1st - sl["phit"] - p.iabsnb * tb.nu * phiz1st
         deta1 = phiz1st
         da1 = self.s2f(da1)
+        if second:
+            self.first, da1 = self.d1st_for_2nd(a1st, da1, sl, a_add_1, d1, phiz1st)
         return da1, deta1, d1, dr
 
-    # ----- solveHOS2 (resol_HOS.f90:270-656) ------------------------------
-    def solve_hos2(self, a1st, eta1st, G2, G1, a_add_1, delta_t):
+    # ----- first-order fields for the second-order wavemaker (resol_wmkr.f90:645-793) ----
+    def d1st_for_2nd(self, a1, da1, sl, a_add_1, d1, phiz1st):
+        """Returns (fields, da1): first-order potential derivatives on the paddle (n3 x n2)
+        and on the free surface (n1 x n2), and da1 after the optional x/y filters."""
         p, tb = self.p, self.tb
+        n3 = p.n3
+        a1 = a1.copy()
+        if p.coeffilt[0] < 1.0:
+            a1 = filter_modes(a1, p.coeffilt[0], 0)
+            da1 = filter_modes(da1, p.coeffilt[0], 0)
+        if p.n2 != 1 and p.coeffilt[1] < 1.0:
+            a1 = filter_modes(a1, p.coeffilt[1], 1)
+            da1 = filter_modes(da1, p.coeffilt[1], 1)
+        FA = self.FA
+        ka2 = tb.k_add ** 2
+        kyb = tb.ky[None, :]
+        kxa = tb.kx_add[:, None]
+        # additional-potential derivatives on the paddle (phiadd_1st_SWEET :535-566)
+        add = dict(
+            axx=FA.fourier_2_space(a_add_1 * ka2, "cos", "cos")[:n3],
+            axxt=FA.fourier_2_space(d1 * ka2, "cos", "cos")[:n3],
+            ay=FA.fourier_2_space(-a_add_1 * kyb, "cos", "sin")[:n3],
+            ayt=FA.fourier_2_space(-d1 * kyb, "cos", "sin")[:n3],
+            az=FA.fourier_2_space(-a_add_1 * kxa, "sin", "cos")[:n3],
+            azt=FA.fourier_2_space(-d1 * kxa, "sin", "cos")[:n3],
+        )
+        # free-surface modes on the paddle x = 0 (:686-709; tables HOS-NWT.f90:342-355)
+        k = tb.k
+        kx2 = (tb.kx ** 2)[:, None]
+        sums = {key: np.zeros((n3, p.n2)) for key in ("axx", "axxt", "ay", "ayt", "az", "azt")}
+        for i3 in range(n3):
+            cxa = i3 * tb.delz  # cx_add (HOS-NWT.f90:242)
+            ch = np.cosh(k * cxa) / np.cosh(k)
+            sh = k * np.sinh(k * cxa) / np.cosh(k)
+            sums["axx"][i3] = -(a1 * ch * kx2).sum(axis=0)
+            sums["axxt"][i3] = -(da1 * ch * kx2).sum(axis=0)
+            sums["ay"][i3] = -(a1 * ch * kyb).sum(axis=0)
+            sums["ayt"][i3] = -(da1 * ch * kyb).sum(axis=0)
+            sums["az"][i3] = (a1 * sh).sum(axis=0)
+            sums["azt"][i3] = (da1 * sh).sum(axis=0)
+        f = {}
+        for key, cy in (("axx", "cos"), ("axxt", "cos"), ("ay", "sin"), ("ayt", "sin"), ("az", "cos"),
+                        ("azt", "cos")):
+            f[key] = (fourier_2_space_y_axis(sums[key], cy, 1) if p.n2 != 1 else sums[key]) + add[key]
+        # free-surface first-order fields (:711-777)
+        kx = tb.kx[:, None]
+        kth1 = tb.kth[:, :, 0]
+        f["phix1st"] = self.f2s(-a1 * kx, "sin", "cos") + sl["phix"]
+        f["phiy1st"] = self.f2s(-a1 * kyb, "cos", "sin") + sl["phiy"]
+        f["phizt1st"] = self.f2s(da1 * kth1) + sl["phizt"]
+        f["phixx1st"] = self.f2s(-a1 * kx * kx) + sl["phixx"]
+        f["phizz1st"] = self.f2s(a1 * k * k) + sl["phizz"]
+        phixz1st = self.f2s(-a1 * kx * kth1, "sin", "cos") + sl["phixz"]
+        phiyz1st = self.f2s(-a1 * kyb * kth1, "cos", "sin") + sl["phiyz"]
+        f["etax1st"] = (self.f2s(da1 * kx, "sin", "cos") - sl["phixt"]
+                        - p.iabsnb * (tb.dnux * phiz1st + tb.nu * phixz1st))
+        f["etay1st"] = (self.f2s(da1 * kyb, "cos", "sin") - sl["phiyt"]
+                        - p.iabsnb * (tb.dnuy * phiz1st + tb.nu * phiyz1st))
+        f["phiz1st"] = phiz1st
+        return f, da1
+
+    # ----- wavemaker second order (resol_wmkr.f90:219-329) -----------------
+    def solve_sweet_2(self, a2nd, eta2nd, eta1st, a2nd_add):
+        """Returns (da2nd modal, deta2nd physical, da2nd_add modal, phiadd_2nd fields)."""
+        p, tb = self.p, self.tb
+        f = self.first
+        n3, n3a = p.n3, p.n3_add
+        al, alt = self.alpha, self.alphat
+        dy, dyt, dz, dzt = dpos_1st(self.time, p, tb)
+        pos, post = self.pos1st, self.dpos1stdt
+        noramp = -pos * f["axx"] + dy * f["ay"] + dz * f["az"]
+        d2 = np.zeros((n3a, p.n2))
+        d2[:n3] = (-al * (-pos * f["axxt"] - post * f["axx"] + dy * f["ayt"] + dyt * f["ay"]
+                          + dz * f["azt"] + dzt * f["az"]) - alt * noramp)
+        d2 = match_surface_wmk(d2, n3, tb.delz)
+        d2 = self.FA.space_2_fourier(d2)
+        if p.coeffilt[2] < 1.0:
+            d2 = filter_modes(d2, p.coeffilt[2], 0)
+        d2 = d2 / tb.k_add_thk_add
+        # phiadd_2nd (resol_wmkr.f90:802-864): the z = 0 sums of phiadd_SL
+        sl2 = phiadd_sl(a2nd_add, d2, None, tb, need=("phi", "phix", "phiy", "phit", "phiz"))
+        # phidphiz for the second order, then the free-surface conditions (:304-318)
+        phiz2nd = self.f2s(a2nd * tb.kth[:, :, 0]) + sl2["phiz"]
+        deta2 = phiz2nd - f["phix1st"] * f["etax1st"] - f["phiy1st"] * f["etay1st"] + eta1st * f["phizz1st"]
+        da2 = (-eta2nd - sl2["phit"] - 0.5 * (f["phix1st"] ** 2 + f["phiy1st"] ** 2 + f["phiz1st"] ** 2)
+               - eta1st * f["phizt1st"] - p.iabsnb * tb.nu * deta2)
+        return self.s2f(da2), deta2, d2, sl2
+
+    # ----- solveHOS2 (resol_HOS.f90:270-656) ------------------------------
+    def solve_hos2(self, a1st, eta1st, G2, G1, a_add_1, delta_t, a2nd=None, eta2nd=None, a_add_2=None):
+        """One RHS.  Returns (dG1, dG2, da1st, deta1st, da_add_1, da_add_ramp); with i_wmk >= 2
+        the second-order derivatives (da2nd, deta2nd, da_add_2) are left in ``self.second``."""
+        p, tb = self.p, self.tb
+        self.second = None
         self.n_rhs += 1
         etark, phisrk = self.gtof(G1, G2, delta_t)
         om = tb.omega.copy()
@@ -831,10 +996,20 @@ class Oracle:
         n1, n2, n3a = p.n1, p.n2, p.n3_add
         if p.has_wmk:
             da1, deta1, d1, dr = self.solve_sweet_1(a1st, eta1st, a_add_1)
+            a_tot, d_tot = a_add_1, d1
+            if p.i_wmk >= 2:
+                z1 = np.zeros((n1, n2))
+                a2 = z1 if a2nd is None else a2nd
+                e2 = z1 if eta2nd is None else eta2nd
+                aa2 = np.zeros((n3a, n2)) if a_add_2 is None else a_add_2
+                da2, deta2, d2, _ = self.solve_sweet_2(a2, e2, eta1st, aa2)
+                self.second = (da2, deta2, d2)
+                # total additional potential (resol_HOS.f90:498-502)
+                a_tot, d_tot = a_add_1 + aa2, d1 + d2
             if self.wmk_direct:
-                sl = phiadd_sl_bruteforce(a_add_1, d1, etark, tb)
+                sl = phiadd_sl_bruteforce(a_tot, d_tot, etark, tb)
             else:
-                sl = phiadd_sl(a_add_1, d1, etark, tb)
+                sl = phiadd_sl(a_tot, d_tot, etark, tb)
             phi_add, phix_add, phiy_add = sl["phi"], sl["phix"], sl["phiy"]
             phiz_add, phit_add = sl["phiz"], sl["phit"]
         else:
@@ -939,9 +1114,19 @@ class Oracle:
         G2 = G2 * tb.omega
         s = 5
         kG1, kG2, ka1, ke1, kad1, kadr = ([None] * s for _ in range(6))
+        # second-order wavemaker state (zero unless i_wmk >= 2)
+        ka2, ke2, kad2 = ([None] * s for _ in range(3))
+
+        def second_out():
+            if self.second is None:
+                return np.zeros_like(st.a2nd), np.zeros_like(st.eta2nd), np.zeros_like(st.a2nd_add)
+            return self.second
+
         self.time = t_o
-        out = self.solve_hos2(st.a1st, st.eta1st, G2, G1, st.a1st_add, h * self.C[0])
+        out = self.solve_hos2(st.a1st, st.eta1st, G2, G1, st.a1st_add, h * self.C[0], st.a2nd, st.eta2nd,
+                              st.a2nd_add)
         kG1[0], kG2[0], ka1[0], ke1[0], kad1[0], kadr[0] = out
+        ka2[0], ke2[0], kad2[0] = second_out()
         kG2[0][0, 0] = 0.0
         for jl in range(1, s):
             def comb(arr_list, base):
@@ -954,9 +1139,13 @@ class Oracle:
             aad = comb(kad1, st.a1st_add)
             a1m = comb(ka1, st.a1st)
             e1m = comb(ke1, st.eta1st)
+            a2m = comb(ka2, st.a2nd)
+            e2m = comb(ke2, st.eta2nd)
+            aad2 = comb(kad2, st.a2nd_add)
             self.time = t_o + h * self.C[jl]
-            out = self.solve_hos2(a1m, e1m, G2m, G1m, aad, h * self.C[jl])
+            out = self.solve_hos2(a1m, e1m, G2m, G1m, aad, h * self.C[jl], a2m, e2m, aad2)
             kG1[jl], kG2[jl], ka1[jl], ke1[jl], kad1[jl], kadr[jl] = out
+            ka2[jl], ke2[jl], kad2[jl] = second_out()
             kG2[jl][0, 0] = 0.0
 
         def bsum(arr_list, w):
@@ -968,18 +1157,24 @@ class Oracle:
         dad1, dadr = bsum(kad1, self.B), bsum(kadr, self.B)
         da1, de1 = bsum(ka1, self.B), bsum(ke1, self.B)
         er1, er2, era = bsum(kG1, self.E), bsum(kG2, self.E), bsum(kad1, self.E)
+        dad2, da2, de2 = bsum(kad2, self.B), bsum(ka2, self.B), bsum(ke2, self.B)
+        era2 = bsum(kad2, self.E)
         G1 = G1 + h * dG1m
         G2 = G2 + h * dG2m
         a_add_1 = st.a1st_add + h * dad1
         a_add_r = st.a_add_ramp + h * dadr
         a1st = st.a1st + h * da1
         eta1st = st.eta1st + h * de1
+        a_add_2 = st.a2nd_add + h * dad2
+        a2nd = st.a2nd + h * da2
+        eta2nd = st.eta2nd + h * de2
         er1 = np.abs((er1 - dG1m) * h) / (np.max(np.abs(G1)) + TINY)
         er2 = np.abs((er2 - dG2m) * h) / (np.max(np.abs(G2)) + TINY)
         era = np.abs((era - dad1) * h) / (np.max(np.abs(a_add_1)) + TINY)
-        # a_add_2 and a_add_3 are identically zero for a first-order wavemaker: their error
-        # terms are 0/(0+tiny) = 0 (runge_kutta.f90:273-279)
-        err = max(er1.max(), er2.max(), era.max())
+        era2 = np.abs((era2 - dad2) * h) / (np.max(np.abs(a_add_2)) + TINY)
+        # a_add_3 is identically zero below a third-order wavemaker, and so is a_add_2 below
+        # a second-order one: their error terms are 0/(0+tiny) = 0 (runge_kutta.f90:273-279)
+        err = max(er1.max(), er2.max(), era.max(), era2.max())
         etark, phisrk = self.gtof(G1, G2, h)
         deta, _ = self.gtof(kG1[s - 1], kG2[s - 1], h)
         om = tb.omega.copy()
@@ -992,7 +1187,7 @@ class Oracle:
         etark = etark.copy()
         etark[0, 0] = self._volume_mode(modesFS[0, 0])
         self.modesFS = etark.copy()
-        new = State(self.f2s(etark), self.f2s(ph), a_add_1, a_add_r, a1st, eta1st)
+        new = State(self.f2s(etark), self.f2s(ph), a_add_1, a_add_r, a1st, eta1st, a_add_2, a2nd, eta2nd)
         deta = self.f2s(d)
         return new, err, deta
 

@@ -268,7 +268,8 @@ def test_rk_tableau_on_linear_rhs():
     lam1 = -0.8 + 1.3 * rng.standard_normal((n1, n2))
     lam2 = 0.4 * rng.standard_normal((n1, n2))
 
-    def linear_rhs(a1st, eta1st, G2, G1, a_add, delta_t):
+    def linear_rhs(a1st, eta1st, G2, G1, a_add, delta_t, *second_order_state):
+        o.second = None
         za = np.zeros_like(a_add)
         return lam1 * G1, lam2 * G2, np.zeros_like(a1st), np.zeros_like(eta1st), za, za.copy()
 
@@ -363,3 +364,59 @@ def test_check_slope_filters_only_steep_states():
     assert fx
     np.testing.assert_allclose(filt.eta[:, 0], 0.01 * np.cos(np.pi * x / 4.0), atol=1e-14)
     np.testing.assert_allclose(filt.phis, 0.5 * filt.eta, atol=1e-14)
+
+
+def test_second_order_paddle_fields_are_consistent():
+    """d1st_for_2nd pinned by identities between independent code paths.
+
+    * at the paddle top (z = 0) the free-surface-mode sums equal the x = 0 column of the
+      base-grid transforms: phi_xx = F2S(-a kx^2), phi_y = F2S(-a ky) (cos-sin),
+      phi_z = F2S(a k tanh k);
+    * at the bottom (z = -1) the vertical derivative vanishes (no flux);
+    * phi_z of the sums is the z derivative of the potential sum a cosh(k(z+1))/cosh k
+      (checked by centred differences in z);
+    * the additional-potential part on the paddle is F2S_add of the modal coefficients
+      times k_add^2, -ky, -kx_add (cos/sin kinds), i.e. its own z derivative again.
+    """
+    p = ho.Params(n1=20, n2=9, mHOS=3, xlen=5.0, ylen=3.0, n3=9, icase=2, i_wmk=2, iabsnb=0)
+    o = ho.Oracle(p)
+    tb = o.tb
+    rng = np.random.default_rng(8)
+    dec = np.exp(-0.4 * tb.k)
+    a1 = 0.01 * rng.standard_normal((20, 9)) * dec
+    da1 = 0.01 * rng.standard_normal((20, 9)) * dec
+    zadd = np.zeros((p.n3_add, 9))
+    sl = ho.phiadd_sl(zadd, zadd, None, tb, need=o.SECOND_FIELDS)
+    f, _ = o.d1st_for_2nd(a1, da1, sl, zadd, zadd, np.zeros((20, 9)))
+    top = p.n3 - 1
+    np.testing.assert_allclose(f["axx"][top], o.f2s(-a1 * (tb.kx ** 2)[:, None])[0], atol=1e-14)
+    np.testing.assert_allclose(f["ay"][top], o.f2s(-a1 * tb.ky[None, :], "cos", "sin")[0], atol=1e-14)
+    np.testing.assert_allclose(f["az"][top], o.f2s(a1 * tb.kth[:, :, 0])[0], atol=1e-14)
+    np.testing.assert_allclose(f["az"][0], 0.0, atol=1e-15)
+    # potential sum and its z derivative by centred differences
+    def phi_at(z):
+        ch = np.cosh(tb.k * (z + 1.0)) / np.cosh(tb.k)
+        return ho.fourier_2_space_y_axis((a1 * ch).sum(axis=0)[None, :], "cos", 1)[0]
+    for i3 in (2, 5):
+        z = tb.cz[i3]
+        dz = 1e-5
+        fd = (phi_at(z + dz) - phi_at(z - dz)) / (2 * dz)
+        np.testing.assert_allclose(f["az"][i3], fd, atol=1e-9)
+    # additional-potential part: with FS modes zero, axx / az follow F2S_add of the coefficients
+    aadd = 0.01 * rng.standard_normal((p.n3_add, 9)) * np.exp(-0.3 * np.arange(p.n3_add))[:, None]
+    g, _ = o.d1st_for_2nd(np.zeros((20, 9)), np.zeros((20, 9)), sl, aadd, zadd, np.zeros((20, 9)))
+    phi_add = o.FA.fourier_2_space(aadd, "cos", "cos")[:p.n3]
+    # z derivative of phi_add on the paddle by centred differences of the cos(kx_add (z+1)) series
+    def phiadd_at(z):
+        c = np.cos(tb.kx_add[:, None] * (z + 1.0)) * aadd
+        return ho.fourier_2_space_y_axis(c.sum(axis=0)[None, :], "cos", 1)[0]
+    # the inverse sine transform along z is FFTW's RODFT10, Y_k = 2 sum_j X_j sin(pi (j+1/2)(k+1)/n):
+    # entry k holds node k+1, so the reference's paddle z derivative at index i3 is the one at
+    # node i3+1 (resol_wmkr.f90:548-566 keeps that index), whereas the cosine fields are at node i3
+    for i3 in (1, 6):
+        z = tb.cz[i3 + 1]
+        fd = (phiadd_at(z + 1e-5) - phiadd_at(z - 1e-5)) / 2e-5
+        np.testing.assert_allclose(g["az"][i3], fd, atol=1e-8)
+    np.testing.assert_allclose(g["axx"], o.FA.fourier_2_space(aadd * tb.k_add ** 2, "cos", "cos")[:p.n3],
+                               atol=1e-15)
+    assert np.abs(phi_add).max() > 0
