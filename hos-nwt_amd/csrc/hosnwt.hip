@@ -561,6 +561,368 @@ __global__ void k_phiadd2d(int n1, int n3a, double xlen, double dx, const double
   o_phiz1[i1] = phiz1;
 }
 
+// ---------------------------------------------------------------------------
+// second-order wavemaker (i_wmk = 2) kernels
+// ---------------------------------------------------------------------------
+// Wavemaker sums at z = 0 (phiadd_1st_SWEET, resol_wmkr.f90:485-532, and
+// phiadd_2nd, :836-857), same workgroup layout as k_phiadd3d.  With
+// S_f(i1,i3,i2) = sum_j2 A_f(i1,j2,i3) cos|sin(ky(j2) y(i2)) and c0, s0 =
+// cos, sin(kx_add(i3)):
+//   SET 0 (11 fields, from a, da): phit = c0 S(da csh), phiz = -s0 kx S(a csh),
+//     phix = -c0 S(a k sh), phiy = -c0 S_sin(a ky csh), phizz = -c0 kx^2 S(a csh),
+//     phixt = -c0 S(da k sh), phiyt = -c0 S_sin(da ky csh), phizt = -s0 kx S(da csh),
+//     phixx = c0 S(a k^2 csh), phixz = s0 kx S(a k sh), phiyz = s0 S_sin(a ky csh)
+//   SET 1 (2 fields): phit, phiz as above.
+// A 2-D tank (n2 = 1) has the single mode j2 = 0.
+template <int SET>
+struct Z0Set;
+template <>
+struct Z0Set<0> {
+  static constexpr int NB = 7, NF = 11;
+};
+template <>
+struct Z0Set<1> {
+  static constexpr int NB = 2, NF = 2;
+};
+
+template <int SET>
+__global__ void __launch_bounds__(256) k_phiadd_z0(int n1, int n2, int n3a, double xlen, double piy, double dx,
+                                                   const double* __restrict__ a, const double* __restrict__ da,
+                                                   double* part) {
+  constexpr int NB = Z0Set<SET>::NB, NF = Z0Set<SET>::NF;
+  __shared__ double A[NB][kPhiCh];
+  const int i1 = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
+  const double x = i1 * dx;
+  const int i3a = ch * kPhiI3, i3b = min(n3a, i3a + kPhiI3);
+  if (x * kx_add_of(i3a) > kCut) return;  // the reduction skips this chunk too
+  const size_t ns = (size_t)n1 * n2;
+  double out[NF][kPhiMaxPer];
+#pragma unroll
+  for (int q = 0; q < kPhiMaxPer; ++q)
+#pragma unroll
+    for (int f = 0; f < NF; ++f) out[f][q] = 0.0;
+  const double th = (n2 > 1) ? 1.0 / (double)(n2 - 1) : 0.0;
+  for (int i3 = i3a; i3 < i3b; ++i3) {
+    const double kxa = kx_add_of(i3);
+    if (x * kxa > kCut) break;
+    int jmax = n2;
+    if (x > 0.0 && n2 > 1) {
+      const double r = kCut / x;
+      jmax = min(n2, (int)floor(sqrt(r * r - kxa * kxa) / piy) + 1);
+    }
+    double S[NB][kPhiMaxPer];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int q = 0; q < kPhiMaxPer; ++q) S[b][q] = 0.0;
+    for (int j0 = 0; j0 < jmax; j0 += kPhiCh) {
+      const int jn = min(kPhiCh, jmax - j0);
+      for (int jj = tid; jj < jn; jj += nt) {
+        const int j2 = j0 + jj;
+        const double ky = j2 * piy;
+        const double k = sqrt(kxa * kxa + ky * ky);
+        double csh, sh;
+        csh_sh(k, x, xlen, csh, sh);
+        const double av = a[i3 + (size_t)n3a * j2], dv = da[i3 + (size_t)n3a * j2];
+        A[0][jj] = av * csh;
+        A[1][jj] = dv * csh;
+        if constexpr (SET == 0) {
+          A[2][jj] = av * sh * k;
+          A[3][jj] = av * csh * ky;
+          A[4][jj] = dv * sh * k;
+          A[5][jj] = dv * csh * ky;
+          A[6][jj] = av * csh * k * k;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < kPhiMaxPer; ++q) {
+        const int i2 = tid + q * nt;
+        if (i2 >= n2) continue;
+        const double ang = (n2 > 1) ? (double)(((long)j0 * i2) % (2 * (n2 - 1))) * th : 0.0;
+        double c = cospi(ang), sn = sinpi(ang);
+        const double cr = cospi(i2 * th), sr = sinpi(i2 * th);
+        double acc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[b] = S[b][q];
+        for (int jj = 0; jj < jn; ++jj) {
+          acc[0] += A[0][jj] * c;
+          acc[1] += A[1][jj] * c;
+          if constexpr (SET == 0) {
+            acc[2] += A[2][jj] * c;
+            acc[3] += A[3][jj] * sn;
+            acc[4] += A[4][jj] * c;
+            acc[5] += A[5][jj] * sn;
+            acc[6] += A[6][jj] * c;
+          }
+          const double cn = c * cr - sn * sr;
+          sn = sn * cr + c * sr;
+          c = cn;
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) S[b][q] = acc[b];
+      }
+      __syncthreads();
+    }
+    double s0, c0;
+    sincos(kxa, &s0, &c0);
+#pragma unroll
+    for (int q = 0; q < kPhiMaxPer; ++q) {
+      out[0][q] += c0 * S[1][q];
+      out[1][q] += -s0 * kxa * S[0][q];
+      if constexpr (SET == 0) {
+        out[2][q] += -c0 * S[2][q];
+        out[3][q] += -c0 * S[3][q];
+        out[4][q] += -c0 * kxa * kxa * S[0][q];
+        out[5][q] += -c0 * S[4][q];
+        out[6][q] += -c0 * S[5][q];
+        out[7][q] += -s0 * kxa * S[1][q];
+        out[8][q] += c0 * S[6][q];
+        out[9][q] += s0 * kxa * S[2][q];
+        out[10][q] += s0 * S[3][q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kPhiMaxPer; ++q) {
+    const int i2 = tid + q * nt;
+    if (i2 >= n2) continue;
+    const size_t o = i1 + (size_t)n1 * i2;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) part[((size_t)ch * NF + f) * ns + o] = out[f][q];
+  }
+}
+
+struct Ptr11 {
+  double* p[11];
+};
+
+// Sum the active chunks of each x-node in order (nf fields).
+__global__ void k_phiadd_reduce_n(int n1, int n2, int nch, int nf, double dx, const double* __restrict__ part,
+                                  Ptr11 outp) {
+  const size_t ns = (size_t)n1 * n2;
+  const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (o >= ns) return;
+  const double x = (double)(o % n1) * dx;
+  double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int ch = 0; ch < nch; ++ch) {
+    if (x * kx_add_of(ch * kPhiI3) > kCut) break;
+    const double* pc = part + (size_t)ch * nf * ns;
+    for (int f = 0; f < nf; ++f) acc[f] += pc[f * ns + o];
+  }
+  for (int f = 0; f < nf; ++f)
+    if (outp.p[f]) outp.p[f][o] = acc[f];
+}
+
+// y and z derivatives of the paddle displacement and their time derivatives
+// (dposdy_1st, dposdz_1st: wavemaking.f90:306-406): sums over harmonics of
+// dposdy_ini(i2,f) e^{iwt} g(z), pos_ini(i2,f) e^{iwt} g'(z), times 1 and i w.
+__global__ void k_dpos(int n2, int n3, int nf, double t, const double* __restrict__ freq,
+                       const double* __restrict__ pre, const double* __restrict__ pim,
+                       const double* __restrict__ dyre, const double* __restrict__ dyim,
+                       const double* __restrict__ gz, const double* __restrict__ dgz, double* Xy, double* Xyt,
+                       double* Xz, double* Xzt) {
+  const int i2 = blockIdx.x;
+  double y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+  for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+    const double w = 2.0 * M_PI * freq[f];
+    double sn, cs;
+    sincos(w * t, &sn, &cs);
+    const size_t o = i2 + (size_t)n2 * f;
+    const double yr = dyre ? dyre[o] : 0.0, yi = dyim ? dyim[o] : 0.0;
+    const double ry = yr * cs - yi * sn, iy = yr * sn + yi * cs;
+    const double rz = pre[o] * cs - pim[o] * sn, iz = pre[o] * sn + pim[o] * cs;
+    y0 += ry;
+    y1 += -w * iy;
+    z0 += rz;
+    z1 += -w * iz;
+  }
+  __shared__ double tot[4];
+  y0 = block_sum(y0);
+  if (threadIdx.x == 0) tot[0] = y0;
+  y1 = block_sum(y1);
+  if (threadIdx.x == 0) tot[1] = y1;
+  z0 = block_sum(z0);
+  if (threadIdx.x == 0) tot[2] = z0;
+  z1 = block_sum(z1);
+  if (threadIdx.x == 0) tot[3] = z1;
+  __syncthreads();
+  for (int i3 = threadIdx.x; i3 < n3; i3 += blockDim.x) {
+    const size_t o = i3 + (size_t)n3 * i2;
+    Xy[o] = gz[i3] * tot[0];
+    Xyt[o] = gz[i3] * tot[1];
+    Xz[o] = dgz[i3] * tot[2];
+    Xzt[o] = dgz[i3] * tot[3];
+  }
+}
+
+// Modal coefficients of the first-order additional-potential derivatives on the
+// paddle (phiadd_1st_SWEET :535-547), stacked [xx, xxt, y, yt, z, zt] (n3a x n2
+// each): a k_add^2, da k_add^2, -a ky, -da ky, -a kx_add, -da kx_add.
+__global__ void k_paddle_prep6(int n2, int n3a, double piy, const double* __restrict__ a,
+                               const double* __restrict__ da, double* out) {
+  const size_t na = (size_t)n3a * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const int i3 = (int)(i % n3a), j2 = (int)(i / n3a);
+  const double kxa = kx_add_of(i3), ky = j2 * piy;
+  const double k2 = kxa * kxa + ky * ky;
+  out[i] = a[i] * k2;
+  out[na + i] = da[i] * k2;
+  out[2 * na + i] = -a[i] * ky;
+  out[3 * na + i] = -da[i] * ky;
+  out[4 * na + i] = -a[i] * kxa;
+  out[5 * na + i] = -da[i] * kxa;
+}
+
+// First-order free-surface modes evaluated on the paddle x = 0 at the nodes
+// z(i3) (d1st_for_2nd, resol_wmkr.f90:686-700, tables HOS-NWT.f90:342-355),
+// added to the stacked z-physical / y-modal fields: one workgroup per (i3, j2),
+// block sums over i1 in a fixed order.  cosh(k z')/cosh k and k sinh(k z')/cosh k
+// (z' = z + 1) are evaluated in overflow-free exponential form.
+__global__ void k_fs_on_paddle(int n1, int n2, int n3, int n3a, double delz, double pix, double piy,
+                               const double* __restrict__ a1, const double* __restrict__ da1, double* stacked) {
+  const int i3 = blockIdx.x, j2 = blockIdx.y;
+  const size_t na = (size_t)n3a * n2;
+  const double zp = i3 * delz;  // cx_add (HOS-NWT.f90:242)
+  const double ky = j2 * piy;
+  double sxx = 0, sxxt = 0, sy = 0, syt = 0, sz = 0, szt = 0;
+  for (int i1 = threadIdx.x; i1 < n1; i1 += blockDim.x) {
+    const double kx = i1 * pix;
+    const double k = sqrt(kx * kx + ky * ky);
+    const double e = exp(k * (zp - 1.0)) / (1.0 + exp(-2.0 * k));
+    const double e2 = exp(-2.0 * k * zp);
+    const double ch = e * (1.0 + e2), sh = k * e * (1.0 - e2);
+    const size_t o = i1 + (size_t)n1 * j2;
+    const double av = a1[o], dv = da1[o];
+    sxx -= av * ch * kx * kx;
+    sxxt -= dv * ch * kx * kx;
+    sy -= av * ch * ky;
+    syt -= dv * ch * ky;
+    sz += av * sh;
+    szt += dv * sh;
+  }
+  const double v[6] = {sxx, sxxt, sy, syt, sz, szt};
+  __shared__ double tot[6];
+  for (int f = 0; f < 6; ++f) {
+    const double r = block_sum(v[f]);
+    if (threadIdx.x == 0) tot[f] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) stacked[threadIdx.x * na + i3 + (size_t)n3a * j2] += tot[threadIdx.x];
+}
+
+// Modal inputs of the first-order free-surface fields (d1st_for_2nd :711-721):
+// [phix (sin-cos), phiy (cos-sin), etax (sin-cos), etay (cos-sin), phizt, phizz
+// (cos-cos), phixz (sin-cos), phiyz (cos-sin)].
+__global__ void k_fs_modes8(int n1, int n2, double pix, double piy, const double* __restrict__ a1,
+                            const double* __restrict__ da1, double* out) {
+  const size_t ns = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const int i1 = (int)(i % n1), i2 = (int)(i / n1);
+  const double kx = i1 * pix, ky = i2 * piy;
+  const double k = sqrt(kx * kx + ky * ky);
+  const double kth1 = k * tanh(k);
+  const double av = a1[i], dv = da1[i];
+  out[i] = -av * kx;
+  out[ns + i] = -av * ky;
+  out[2 * ns + i] = dv * kx;
+  out[3 * ns + i] = dv * ky;
+  out[4 * ns + i] = dv * kth1;
+  out[5 * ns + i] = av * k * k;
+  out[6 * ns + i] = -av * kx * kth1;
+  out[7 * ns + i] = -av * ky * kth1;
+}
+
+// First-order free-surface fields (d1st_for_2nd :739-777) from the transformed
+// modes T (stacked as in k_fs_modes8) and the z = 0 wavemaker fields; written in
+// place into T: [phix1st, phiy1st, etax1st, etay1st, phizt1st, phizz1st].
+struct Z0Fields {
+  const double *phix, *phiy, *phizz, *phixt, *phiyt, *phizt, *phixz, *phiyz;
+};
+__global__ void k_fs_fields(size_t ns, double* T, Z0Fields z, const double* __restrict__ phiz1st,
+                            const double* __restrict__ nu, const double* __restrict__ nux,
+                            const double* __restrict__ nuy, double iabs) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const double phix = T[i] + z.phix[i];
+  const double phiy = T[ns + i] + z.phiy[i];
+  const double phizt = T[4 * ns + i] + z.phizt[i];
+  const double phizz = T[5 * ns + i] + z.phizz[i];
+  const double phixz = T[6 * ns + i] + z.phixz[i];
+  const double phiyz = T[7 * ns + i] + z.phiyz[i];
+  const double nv = nu ? nu[i] : 0.0, nx = nux ? nux[i] : 0.0, ny = nuy ? nuy[i] : 0.0;
+  const double etax = T[2 * ns + i] - z.phixt[i] - iabs * (nx * phiz1st[i] + nv * phixz);
+  const double etay = T[3 * ns + i] - z.phiyt[i] - iabs * (ny * phiz1st[i] + nv * phiyz);
+  T[i] = phix;
+  T[ns + i] = phiy;
+  T[2 * ns + i] = etax;
+  T[3 * ns + i] = etay;
+  T[4 * ns + i] = phizt;
+  T[5 * ns + i] = phizz;
+}
+
+// C2 continuation of a paddle column above the paddle (match_surface_wmk,
+// resol_wmkr.f90:1655-1741): rows n3 .. n3a-1 from the last four rows.
+__device__ __forceinline__ void continue_column(double* col, int n3, int n3a, double delz, const Cont& cm) {
+  const double fm3 = col[n3 - 4], fm2 = col[n3 - 3], fm1 = col[n3 - 2], f0 = col[n3 - 1];
+  const double df = (fm2 - 4.0 * fm1 + 3.0 * f0) / (2.0 * delz);
+  const double d2f = (-fm3 + 4.0 * fm2 - 5.0 * fm1 + 2.0 * f0) / (delz * delz);
+  double c[4];
+  for (int r = 0; r < 4; ++r) c[r] = cm.m[0][r] * f0 + cm.m[1][r] * df + cm.m[2][r] * d2f;
+  for (int i3 = n3; i3 < n3a; ++i3) {
+    const double xx = (double)i3 / (double)(n3 - 1) - (double)L_ADD;
+    const double xx2 = xx * xx;
+    col[i3] = (((c[0] * xx2 + c[1]) * xx2 + c[2]) * xx2 + c[3]) * xx * exp(-6.0 * (xx + 1.0));
+  }
+}
+
+// Second-order paddle condition (solveHOS_SWEET_2, resol_wmkr.f90:264-277, ifree = 0):
+//   d2 = -alpha (-X P_xxt - X_t P_xx + X_y P_yt + X_yt P_y + X_z P_zt + X_zt P_z)
+//        - alpha' (-X P_xx + X_y P_y + X_z P_z)
+// on i3 < n3, then the continuation.  P = the stacked paddle fields (n3a x n2,
+// rows < n3 used), X.. (n3 x n2).
+__global__ void k_paddle_bc2(int n2, int n3, int n3a, double delz, double al, double alt, const double* P,
+                             const double* X, const double* Xt, const double* Xy, const double* Xyt,
+                             const double* Xz, const double* Xzt, Cont cm, double* d2) {
+  const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i2 >= n2) return;
+  const size_t na = (size_t)n3a * n2;
+  double* col = d2 + (size_t)n3a * i2;
+  for (int i3 = 0; i3 < n3; ++i3) {
+    const size_t q = i3 + (size_t)n3a * i2, o = i3 + (size_t)n3 * i2;
+    const double pxx = P[q], pxxt = P[na + q], py = P[2 * na + q], pyt = P[3 * na + q], pz = P[4 * na + q],
+                 pzt = P[5 * na + q];
+    const double noramp = -X[o] * pxx + Xy[o] * py + Xz[o] * pz;
+    col[i3] = -al * (-X[o] * pxxt - Xt[o] * pxx + Xy[o] * pyt + Xyt[o] * py + Xz[o] * pzt + Xzt[o] * pz) -
+              alt * noramp;
+  }
+  continue_column(col, n3, n3a, delz, cm);
+}
+
+// Second-order free-surface conditions (solveHOS_SWEET_2 :302-318):
+//   deta2 = phiz2nd - phix1st etax1st - phiy1st etay1st + eta1st phizz1st
+//   tmp   = -eta2nd - phit_add_2 - (phix1st^2 + phiy1st^2 + phiz1st^2)/2
+//           - eta1st phizt1st - iabsnb nu deta2
+// with phiz2nd = F2S(a2nd kth1) + phiz_add_2; F = [phix1st, phiy1st, etax1st,
+// etay1st, phizt1st, phizz1st] (stacked n1 x n2).
+__global__ void k_second_fs(size_t ns, const double* __restrict__ w2, const double* __restrict__ phiz2,
+                            const double* __restrict__ phit2, const double* __restrict__ F,
+                            const double* __restrict__ phiz1st, const double* __restrict__ eta1,
+                            const double* __restrict__ eta2, const double* __restrict__ nu, double iabs,
+                            double* deta2, double* tmp) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const double phix = F[i], phiy = F[ns + i], etax = F[2 * ns + i], etay = F[3 * ns + i], phizt = F[4 * ns + i],
+               phizz = F[5 * ns + i];
+  const double d = (w2[i] + phiz2[i]) - phix * etax - phiy * etay + eta1[i] * phizz;
+  deta2[i] = d;
+  const double pz = phiz1st[i];
+  tmp[i] = -eta2[i] - phit2[i] - 0.5 * (phix * phix + phiy * phiy + pz * pz) - eta1[i] * phizt -
+           iabs * (nu ? nu[i] : 0.0) * d;
+}
+
 // First-order free surface RHS (resol_wmkr.f90:186-193 via phidphiz :583-636):
 // deta1 = F2S(a1 kth1) + phiz_add_1 ; tmp = -eta1 - phit_add_1 - iabsnb nu deta1.
 __global__ void k_first_order(size_t n, const double* w1, const double* phiz1, const double* phit1,
@@ -652,6 +1014,15 @@ __global__ void k_step_begin(int n1, int n2, double pix, double piy, const doubl
   const double w = wave_omega(wave_k(i1, i2, pix, piy));
   G1[i] = eta_hat[i];
   G2[i] = phis_hat[i] * w;
+}
+
+// a = a1 + a2, d = d1 + d2 (total additional-potential modes, resol_HOS.f90:498-499)
+__global__ void k_sum2(size_t n, const double* a1, const double* a2, const double* d1, const double* d2, double* a,
+                       double* d) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  a[i] = a1[i] + a2[i];
+  d[i] = d1[i] + d2[i];
 }
 
 __global__ void k_mulw(size_t n, const double* __restrict__ w, double* __restrict__ x) {
@@ -808,6 +1179,16 @@ double gzed(double z, int igeom, double d) {
   const double u = (z - (d - 1.0)) / delta;
   return delta / (2.0 * (1.0 - d)) *
          (5.0 / 16.0 + u + 15.0 / 16.0 * (u * u - std::pow(u, 4) / 3.0 + std::pow(u, 6) / 15.0));
+}
+
+// vertical derivative of the paddle shape (wavemaking.f90:222-250)
+double dgzeddz(double z, int igeom, double d) {
+  const double delta = std::min(d, (1.0 - d) / 10.0);
+  if (igeom == 1) return 0.0;
+  if (z >= d - 1.0 + delta) return 1.0 / (1.0 - d);
+  if (z <= d - 1.0 - delta) return 0.0;
+  const double u = (z - (d - 1.0)) / delta;
+  return 1.0 / (1.0 - d) * (0.5 + 15.0 / 16.0 * (u - 2.0 / 3.0 * u * u * u + u * u * u * u * u / 5.0));
 }
 
 // Inverse of the 4x4 system P(-1)=f, P'(-1)=df, P''(-1)=d2f, P(-2/3)=0 for
@@ -1050,9 +1431,14 @@ void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx,
 // ===========================================================================
 struct EngineBase {
   virtual ~EngineBase() = default;
+  // S2 carries the second-order wavemaker stage (null below i_wmk = 2)
+  struct S2 {
+    const double *a2, *e2, *aad2;  // a2nd (modal), eta2nd (physical), a2nd_add (modal)
+    double *ka2, *ke2, *kad2;      // their derivatives
+  };
   virtual void rhs(double t, double dt, const double* G1, const double* G2, const double* a1, const double* e1,
                    const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
-                   double* kadr) = 0;
+                   double* kadr, const S2* s2) = 0;
   virtual void slope_max(double& sx, double& sy) = 0;
   // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
   // and one y pass (full inverse cos-cos transform), timed with HIP events
@@ -1078,6 +1464,13 @@ struct hos_ctx {
   double *eta_hat_t, *phis_hat_t, *a1st_add_t, *a_add_ramp_t, *a1st_t, *eta1st_t, *deta_hat, *deta_hat_t;
   double *G1, *G2, *G1m, *G2m, *aadm, *a1m, *e1m;
   double *kG1[5], *kG2[5], *kad1[5], *kadr[5], *ka1[5], *ke1[5];
+  // second-order wavemaker state (i_wmk = 2; null otherwise): paddle modes, modal and
+  // physical free-surface parts, trial copies, stage derivatives and stage values
+  bool wmk2 = false;
+  double *a2nd_add = nullptr, *a2nd = nullptr, *eta2nd = nullptr;
+  double *a2nd_add_t = nullptr, *a2nd_t = nullptr, *eta2nd_t = nullptr;
+  double *ka2[5] = {}, *ke2[5] = {}, *kad2[5] = {};
+  double *a2m = nullptr, *e2m = nullptr, *aad2m = nullptr;
   double *partial, *pmax_e, *pmax_x, *red6;
   double *phys_tmp, *nu_d;
   double* filt_fs = nullptr;  // free-surface mode filter weights (n1*n2), null when off
@@ -1153,6 +1546,15 @@ struct Engine : EngineBase {
     ctmp = tmpBw;
     ctmpX = &tmpXw;
   }
+  // second-order wavemaker (i_wmk = 2): paddle derivatives, stacked paddle
+  // fields, inverse paddle-grid matrices, z = 0 sums and first-order fields
+  double *dyre = nullptr, *dyim = nullptr, *dgz = nullptr, *nux = nullptr, *nuy = nullptr;
+  double *Xdy = nullptr, *Xdyt = nullptr, *Xdz = nullptr, *Xdzt = nullptr;
+  double *stk6 = nullptr, *stkz = nullptr, *Pw = nullptr;
+  double *Cz = nullptr, *Sz = nullptr, *Cy = nullptr, *Sy = nullptr;
+  double *part2 = nullptr, *zf[11] = {};
+  double *modes8 = nullptr, *T8 = nullptr, *a1f = nullptr;
+  double *phit2 = nullptr, *phiz2 = nullptr, *w2 = nullptr, *tmp2 = nullptr, *atot = nullptr, *dtot = nullptr;
   // base grid as dense products (lengths without a Bluestein-free FFT plan)
   bool dense = false;
   DenseGrid<T> dT;
@@ -1234,6 +1636,47 @@ struct Engine : EngineBase {
     pim = upload(hi);
     cont = continuation_matrix();
     if (c->n2 > 1 && c->n2 > kPhiMaxPer * 256) throw std::runtime_error("n2 too large for the wavemaker kernel");
+    if (c->wmk2) {
+      const int n1 = c->n1;
+      std::vector<double> hdgz(n3);
+      for (int i3 = 0; i3 < n3; ++i3) hdgz[i3] = dgzeddz(i3 * c->delz - 1.0, c->p.igeom, c->p.d_hinge);
+      dgz = upload(hdgz);
+      if (c->p.dposdy_ini_re && c->p.dposdy_ini_im && nf) {
+        dyre = upload(std::vector<double>(c->p.dposdy_ini_re, c->p.dposdy_ini_re + (size_t)n2 * nf));
+        dyim = upload(std::vector<double>(c->p.dposdy_ini_im, c->p.dposdy_ini_im + (size_t)n2 * nf));
+      }
+      if (c->p.nu_x) nux = upload(std::vector<double>(c->p.nu_x, c->p.nu_x + ns));
+      if (c->p.nu_y) nuy = upload(std::vector<double>(c->p.nu_y, c->p.nu_y + ns));
+      Xdy = ar.get<double>((size_t)n3 * n2); Xdyt = ar.get<double>((size_t)n3 * n2);
+      Xdz = ar.get<double>((size_t)n3 * n2); Xdzt = ar.get<double>((size_t)n3 * n2);
+      stk6 = ar.get<double>(6 * na); stkz = ar.get<double>(6 * na); Pw = ar.get<double>(6 * na);
+      // inverse paddle-grid transforms (fourier_FFTW.f90:451-492 with f_2_s_add):
+      // z: REDFT10 / RODFT10 (node k, mode m), y: REDFT00 / RODFT00
+      std::vector<double> hcz((size_t)n3a * n3a), hsz((size_t)n3a * n3a), hcy((size_t)n2 * n2, 0.0),
+          hsy((size_t)n2 * n2, 0.0);
+      for (int m = 0; m < n3a; ++m)
+        for (int k = 0; k < n3a; ++k) {
+          hcz[k + (size_t)n3a * m] = std::cos(M_PI * (2 * m + 1) * k / (2.0 * n3a));
+          hsz[k + (size_t)n3a * m] = std::sin(M_PI * (2 * m + 1) * (k + 1) / (2.0 * n3a));
+        }
+      if (n2 > 1)
+        for (int m = 0; m < n2; ++m)
+          for (int k = 0; k < n2; ++k) {
+            const double ang = M_PI * (double)(((long)k * m) % (2 * (n2 - 1))) / (double)(n2 - 1);
+            hcy[k + (size_t)n2 * m] = std::cos(ang);
+            const bool interior = k > 0 && k < n2 - 1 && m > 0 && m < n2 - 1;
+            hsy[k + (size_t)n2 * m] = interior ? std::sin(ang) : 0.0;
+          }
+      Cz = upload(hcz); Sz = upload(hsz); Cy = upload(hcy); Sy = upload(hsy);
+      part2 = ar.get<double>((size_t)nch * 11 * ns);
+      zf[0] = phit1;
+      zf[1] = phiz1;
+      for (int f = 2; f < 11; ++f) zf[f] = ar.get<double>(ns);
+      modes8 = ar.get<double>(8 * ns); T8 = ar.get<double>(8 * ns); a1f = ar.get<double>(ns);
+      phit2 = ar.get<double>(ns); phiz2 = ar.get<double>(ns); w2 = ar.get<double>(ns); tmp2 = ar.get<double>(ns);
+      atot = ar.get<double>(na); dtot = ar.get<double>(na);
+      (void)n1;
+    }
     // reference extended grid (common_vars.f90:38-39) for the exact mixed-parity filter
     const int n1 = c->n1;
     const int Nd1 = ((c->p.p1 + 1) * n1) / 2, Nd2 = ((c->p.p2 + 1) * n2) / 2;
@@ -1314,7 +1757,7 @@ struct Engine : EngineBase {
 
   void rhs(double t, double dt, const double* G1, const double* G2, const double* a1, const double* e1,
            const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
-           double* kadr) override {
+           double* kadr, const S2* s2) override {
     hos_ctx& C = *c;
     hipStream_t st = C.st;
     const int n1 = C.n1, n2 = C.n2, M = C.M, N1b = C.N1b, N2b = C.N2b;
@@ -1338,7 +1781,7 @@ struct Engine : EngineBase {
         ~Side() { e->use_main(); }
       } side{this};
       use_side();
-      wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr);
+      wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, s2);
     }
     HIP_OK(hipEventRecord(ev_join, sw));
     // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
@@ -1457,7 +1900,7 @@ struct Engine : EngineBase {
   T* tmpB2() { return X; }
 
   void wavemaker(double t, const double* a1, const double* e1, const double* aad, double* ka1, double* ke1,
-                 double* kad1, double* kadr) {
+                 double* kad1, double* kadr, const S2* s2) {
     hos_ctx& C = *c;
     hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a, N1b = C.N1b, N2b = C.N2b;
@@ -1476,6 +1919,10 @@ struct Engine : EngineBase {
       if (ke1) HIP_OK(hipMemsetAsync(ke1, 0, ns * sizeof(double), st));
       if (kad1) HIP_OK(hipMemsetAsync(kad1, 0, na * sizeof(double), st));
       if (kadr) HIP_OK(hipMemsetAsync(kadr, 0, na * sizeof(double), st));
+      return;
+    }
+    if (s2) {
+      second_order_wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, *s2);
       return;
     }
     double al, alt, altt;
@@ -1498,6 +1945,135 @@ struct Engine : EngineBase {
                        (double)C.p.iabsnb, ke1, tmp1);
     base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp1, n1, n1, n2), n1, n2, Dst<double>{ka1, n1, 0});
     // grad phi_add onto the extended grid (resol_HOS.f90:534-544)
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
+    {
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
+      if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, (*ctmpX)[1]);
+      b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
+      bigB(INV, b);
+      if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+    }
+  }
+
+  // z = 0 wavemaker sums of the SET 0 (11 fields) or SET 1 (2 fields) kind.
+  template <int SET>
+  void phiadd_z0(const double* a, const double* da, Ptr11 outp) {
+    hos_ctx& C = *c;
+    int nt = 64;
+    while (nt < 256 && nt < C.n2) nt *= 2;
+    hipLaunchKernelGGL((k_phiadd_z0<SET>), dim3(C.n1, nch), dim3(nt), 0, cs, C.n1, C.n2, C.n3a, C.p.xlen, C.piy,
+                       C.dx, a, da, part2);
+    hipLaunchKernelGGL(k_phiadd_reduce_n, dim3(nblk(ns)), dim3(256), 0, cs, C.n1, C.n2, nch, Z0Set<SET>::NF, C.dx,
+                       (const double*)part2, outp);
+    HIP_OK(hipGetLastError());
+  }
+
+  // First- and second-order wavemaker (i_wmk = 2): solveHOS_SWEET_1 with
+  // d1st_for_2nd (resol_wmkr.f90:109-210, :645-793), solveHOS_SWEET_2
+  // (:219-329), then the additional potential of a_add_1 + a_add_2 on the free
+  // surface (resol_HOS.f90:498-502).
+  void second_order_wavemaker(double t, const double* a1, const double* e1, const double* aad, double* ka1,
+                              double* ke1, double* kad1, double* kadr, const S2& q) {
+    hos_ctx& C = *c;
+    hipStream_t st = cs;
+    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a, N1b = C.N1b, N2b = C.N2b;
+    const int ldb = N1b;
+    double al, alt, altt;
+    ramp_eval(C.p, t, al, alt, altt);
+    C.last_alpha = al;
+    C.last_pos_top = C.pos_top(t);
+    // --- SWEET 1: paddle condition and modes
+    hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
+                       (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt);
+    hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
+                       (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2);
+    paddle_modes(2, kad1, kadr);
+    // phiadd_1st_SWEET at z = 0: the 11 fields (phit, phiz first)
+    Ptr11 z1{};
+    for (int f = 0; f < 11; ++f) z1.p[f] = zf[f];
+    z1.p[8] = nullptr;  // phixx_add_1 feeds only the third-order wavemaker
+    phiadd_z0<0>(aad, kad1, z1);
+    // phidphiz and the first-order free-surface conditions
+    base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
+           Dst<double>{w1, n1, 0});
+    hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
+                       (const double*)phiz1, (const double*)phit1, e1, (const double*)C.nu_d,
+                       (double)C.p.iabsnb, ke1, tmp1);
+    base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp1, n1, n1, n2), n1, n2, Dst<double>{ka1, n1, 0});
+    // --- d1st_for_2nd: optional x/y filters of a1st and da1st (resol_wmkr.f90:673-681)
+    const double* a1u = a1;
+    if (C.filt_fs) {
+      HIP_OK(hipMemcpyAsync(a1f, a1, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_mulw, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)C.filt_fs, a1f);
+      hipLaunchKernelGGL(k_mulw, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)C.filt_fs, ka1);
+      a1u = a1f;
+    }
+    // first-order potential derivatives on the paddle: additional-potential part
+    // (inverse paddle-grid transforms as dense products) plus the free-surface modes
+    const double one = 1.0, zero = 0.0;
+    hipLaunchKernelGGL(k_paddle_prep6, dim3(nblk(na)), dim3(256), 0, st, n2, n3a, C.piy, aad, (const double*)kad1,
+                       stk6);
+    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 4 * n2, n3a, &one, Cz, n3a, stk6,
+                          n3a, &zero, stkz, n3a),
+            "dgemm paddle Cz");
+    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 2 * n2, n3a, &one, Sz, n3a,
+                          stk6 + 4 * na, n3a, &zero, stkz + 4 * na, n3a),
+            "dgemm paddle Sz");
+    hipLaunchKernelGGL(k_fs_on_paddle, dim3(n3, n2), dim3(256), 0, st, n1, n2, n3, n3a, C.delz, C.pix, C.piy, a1u,
+                       (const double*)ka1, stkz);
+    if (n2 > 1) {
+      const double* My[3] = {Cy, Sy, Cy};
+      for (int g = 0; g < 3; ++g)
+        blas_ok(rocblas_dgemm_strided_batched(cb, rocblas_operation_none, rocblas_operation_transpose, n3a, n2, n2,
+                                              &one, stkz + 2 * g * na, n3a, (rocblas_stride)na, My[g], n2, 0, &zero,
+                                              Pw + 2 * g * na, n3a, (rocblas_stride)na, 2),
+                "dgemm paddle y");
+    } else {
+      HIP_OK(hipMemcpyAsync(Pw, stkz, 6 * na * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    // first-order free-surface fields
+    hipLaunchKernelGGL(k_fs_modes8, dim3(nblk(ns)), dim3(256), 0, st, n1, n2, C.pix, C.piy, a1u, (const double*)ka1,
+                       modes8);
+    const int kxs[8] = {KSIN, KCOS, KSIN, KCOS, KCOS, KCOS, KSIN, KCOS};
+    const int kys[8] = {KCOS, KSIN, KCOS, KSIN, KCOS, KCOS, KCOS, KSIN};
+    for (int f = 0; f < 8; ++f) {
+      if (n2 == 1 && kys[f] == KSIN) {  // ky = 0 in a 2-D tank
+        HIP_OK(hipMemsetAsync(T8 + f * ns, 0, ns * sizeof(double), st));
+        continue;
+      }
+      base64(kxs[f], kys[f], INV, src_plain<double, double>(modes8 + f * ns, n1, n1, n2), n1, n2,
+             Dst<double>{T8 + f * ns, n1, 0});
+    }
+    Z0Fields zz{zf[2], zf[3], zf[4], zf[5], zf[6], zf[7], zf[9], zf[10]};
+    hipLaunchKernelGGL(k_fs_fields, dim3(nblk(ns)), dim3(256), 0, st, ns, T8, zz, (const double*)ke1,
+                       (const double*)C.nu_d, (const double*)nux, (const double*)nuy, (double)C.p.iabsnb);
+    // --- SWEET 2: paddle condition, modes, z = 0 potential, free-surface conditions
+    hipLaunchKernelGGL(k_dpos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
+                       (const double*)pre, (const double*)pim, (const double*)dyre, (const double*)dyim,
+                       (const double*)gz, (const double*)dgz, Xdy, Xdyt, Xdz, Xdzt);
+    hipLaunchKernelGGL(k_paddle_bc2, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, al, alt,
+                       (const double*)Pw, (const double*)Xp, (const double*)Xpt, (const double*)Xdy,
+                       (const double*)Xdyt, (const double*)Xdz, (const double*)Xdzt, cont, f1);
+    paddle_modes(1, q.kad2, (double*)nullptr);
+    Ptr11 z2{};
+    z2.p[0] = phit2;
+    z2.p[1] = phiz2;
+    phiadd_z0<1>(q.aad2, q.kad2, z2);
+    base64(KCOS, KCOS, INV, src_mul<double, double>(q.a2, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
+           Dst<double>{w2, n1, 0});
+    hipLaunchKernelGGL(k_second_fs, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w2, (const double*)phiz2,
+                       (const double*)phit2, (const double*)T8, (const double*)ke1, e1, q.e2, (const double*)C.nu_d,
+                       (double)C.p.iabsnb, q.ke2, tmp2);
+    base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp2, n1, n1, n2), n1, n2, Dst<double>{q.ka2, n1, 0});
+    // --- additional potential of the total paddle modes on the free surface
+    hipLaunchKernelGGL(k_sum2, dim3(nblk(na)), dim3(256), 0, st, na, aad, q.aad2, kad1, (const double*)q.kad2, atot,
+                       dtot);
+    phiadd(atot, dtot, eta_phys);
+    // phit1 / phiz1 were overwritten by the surface sums of the totals; the first-order
+    // conditions above have used the first-order values already.
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
@@ -1628,7 +2204,8 @@ void check_params(const hos_params& p) {
   const int p2ok = (p.n2 == 1) ? 1 : p.mHOS;
   if (p.p1 != p.mHOS || (p.p2 != p2ok && !(p.n2 == 1 && p.p2 == p.mHOS)))
     throw std::runtime_error("supported dealiasing: p1 = mHOS, p2 = mHOS (3-D) or 1 (2-D)");
-  if (p.i_wmk != 1 && (p.icase != 1)) throw std::runtime_error("supported wavemaker order: i_wmk = 1");
+  if (p.i_wmk != 1 && p.i_wmk != 2 && (p.icase != 1))
+    throw std::runtime_error("supported wavemaker order: i_wmk = 1 or 2");
   if (p.precision != 0 && p.precision != 1) throw std::runtime_error("precision must be 0 or 1");
   if (p.xlen <= 0.0 || (p.n2 > 1 && p.ylen <= 0.0)) throw std::runtime_error("tank size must be positive");
   if (p.n_freq < 0 || (p.n_freq > 0 && (!p.freq || !p.pos_ini_re || !p.pos_ini_im)))
@@ -1645,8 +2222,12 @@ double stage_time(double t, double h, double cs) {
 }
 
 void rhs_eval(hos_ctx* c, double t, double dt, const double* G1, const double* G2, const double* a1,
-              const double* e1, const double* aad, int s) {
-  c->eng->rhs(t, dt, G1, G2, a1, e1, aad, c->kG1[s], c->kG2[s], c->ka1[s], c->ke1[s], c->kad1[s], c->kadr[s]);
+              const double* e1, const double* aad, int s, const double* a2 = nullptr, const double* e2 = nullptr,
+              const double* aad2 = nullptr) {
+  if (c->wmk2 && !(a2 && e2 && aad2)) throw std::runtime_error("i_wmk = 2 needs the second-order state");
+  EngineBase::S2 s2{a2, e2, aad2, c->ka2[s], c->ke2[s], c->kad2[s]};
+  c->eng->rhs(t, dt, G1, G2, a1, e1, aad, c->kG1[s], c->kG2[s], c->ka1[s], c->ke1[s], c->kad1[s], c->kadr[s],
+              c->wmk2 ? &s2 : nullptr);
 }
 
 void rk_attempt(hos_ctx* c, double t, double h, double* err) {
@@ -1654,7 +2235,7 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   hipStream_t st = c->st;
   hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
                      (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
-  rhs_eval(c, t, h * c->C[0], c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0);
+  rhs_eval(c, t, h * c->C[0], c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add);
   for (int jl = 1; jl < 5; ++jl) {
     auto comb = [&](double* out, const double* base, double* const* karr, size_t n) {
       KSet k{};
@@ -1670,7 +2251,13 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
     comb(c->aadm, c->a1st_add, c->kad1, na);
     comb(c->a1m, c->a1st, c->ka1, ns);
     comb(c->e1m, c->eta1st, c->ke1, ns);
-    rhs_eval(c, stage_time(t, h, c->C[jl]), h * c->C[jl], c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, jl);
+    if (c->wmk2) {
+      comb(c->a2m, c->a2nd, c->ka2, ns);
+      comb(c->e2m, c->eta2nd, c->ke2, ns);
+      comb(c->aad2m, c->a2nd_add, c->kad2, na);
+    }
+    rhs_eval(c, stage_time(t, h, c->C[jl]), h * c->C[jl], c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, jl, c->a2m,
+             c->e2m, c->aad2m);
   }
   // final combination and embedded error (runge_kutta.f90:225-279)
   auto kset = [&](double* const* karr, const double* w) {
@@ -1703,6 +2290,17 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
                      kset(c->ka1, c->B));
   hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->eta1st_t, (const double*)c->eta1st, h,
                      kset(c->ke1, c->B));
+  if (c->wmk2) {
+    // second-order wavemaker modes and their error term (runge_kutta.f90:249-256, :275)
+    hipLaunchKernelGGL(k_final_err, dim3(nba), dim3(256), 0, st, na, c->a2nd_add_t, (const double*)c->a2nd_add, h,
+                       kset(c->kad2, c->B), kset(c->kad2, c->E), c->pmax_e, c->pmax_x);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_e, c->red6 + 6);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_x, c->red6 + 7);
+    hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->a2nd_t, (const double*)c->a2nd, h,
+                       kset(c->ka2, c->B));
+    hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->eta2nd_t, (const double*)c->eta2nd, h,
+                       kset(c->ke2, c->B));
+  }
   // GtoF back, deta, volume fix with the last stage's wavemaker globals
   hipLaunchKernelGGL(k_step_end, dim3(nbs), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy, h, (const double*)Gn1,
                      (const double*)Gn2, (const double*)c->kG1[4], (const double*)c->kG2[4],
@@ -1711,13 +2309,15 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   hipLaunchKernelGGL(k_volfix<double>, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->partial,
                      (double*)nullptr, c->eta_hat_t, c->volmode(), c->last_alpha, c->last_pos_top, c->p.xlen,
                      c->p.d_hinge);
-  double r[6];
-  HIP_OK(hipMemcpyAsync(r, c->red6, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  double r[8];
+  const int nr = c->wmk2 ? 8 : 6;
+  HIP_OK(hipMemcpyAsync(r, c->red6, nr * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
   const double tiny = 2.220446049250313e-16;
   double e = r[0] / (r[1] + tiny);
   e = std::max(e, r[2] / (r[3] + tiny));
   e = std::max(e, r[4] / (r[5] + tiny));
+  if (c->wmk2) e = std::max(e, r[6] / (r[7] + tiny));
   *err = e;
   c->have_trial = true;
 }
@@ -1731,6 +2331,11 @@ void commit(hos_ctx* c) {
   std::swap(c->a1st, c->a1st_t);
   std::swap(c->eta1st, c->eta1st_t);
   std::swap(c->deta_hat, c->deta_hat_t);
+  if (c->wmk2) {
+    std::swap(c->a2nd_add, c->a2nd_add_t);
+    std::swap(c->a2nd, c->a2nd_t);
+    std::swap(c->eta2nd, c->eta2nd_t);
+  }
   c->have_trial = false;
 }
 
@@ -1801,6 +2406,15 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
       c->kad1[s] = ar.get<double>(na); c->kadr[s] = ar.get<double>(na);
       c->ka1[s] = ar.get<double>(ns); c->ke1[s] = ar.get<double>(ns);
     }
+    c->wmk2 = c->wmk && p.i_wmk == 2;
+    if (c->wmk2) {
+      c->a2nd_add = ar.get<double>(na); c->a2nd = ar.get<double>(ns); c->eta2nd = ar.get<double>(ns);
+      c->a2nd_add_t = ar.get<double>(na); c->a2nd_t = ar.get<double>(ns); c->eta2nd_t = ar.get<double>(ns);
+      c->a2m = ar.get<double>(ns); c->e2m = ar.get<double>(ns); c->aad2m = ar.get<double>(na);
+      for (int s = 0; s < 5; ++s) {
+        c->ka2[s] = ar.get<double>(ns); c->ke2[s] = ar.get<double>(ns); c->kad2[s] = ar.get<double>(na);
+      }
+    }
     const size_t nbk = nblk(std::max(ns, na)) + 1;
     c->partial = ar.get<double>(nbk);
     c->pmax_e = ar.get<double>(nbk);
@@ -1838,6 +2452,7 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
       c->eng = std::make_unique<Engine<float>>(c.get());
     c->p.freq = nullptr;
     c->p.pos_ini_re = c->p.pos_ini_im = nullptr;
+    c->p.dposdy_ini_re = c->p.dposdy_ini_im = c->p.nu_x = c->p.nu_y = nullptr;
     HIP_OK(hipDeviceSynchronize());
     *out = c.release();
   });
@@ -1864,6 +2479,11 @@ int hos_initialize(hos_ctx* c) {
     HIP_OK(hipMemsetAsync(c->a1st_add, 0, na * sizeof(double), st));
     HIP_OK(hipMemsetAsync(c->a_add_ramp, 0, na * sizeof(double), st));
     HIP_OK(hipMemsetAsync(c->deta_hat, 0, ns * sizeof(double), st));
+    if (c->wmk2) {  // a2nd, eta2nd, a2nd_add start at zero (initial_condition.f90:370-373)
+      HIP_OK(hipMemsetAsync(c->a2nd, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(c->eta2nd, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(c->a2nd_add, 0, na * sizeof(double), st));
+    }
     c->endramp = 0;
     if (!c->wmk) {
       c->last_alpha = 0.0;
@@ -1890,7 +2510,7 @@ int hos_initialize(hos_ctx* c) {
     // one RHS at t = 0 sets phi_add (phiadd_SL with da1st_add) and the wavemaker globals
     hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
                        (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
-    rhs_eval(c, 0.0, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0);
+    rhs_eval(c, 0.0, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add);
     HIP_OK(hipStreamSynchronize(st));
   });
 }
@@ -1935,6 +2555,29 @@ int hos_get_state(hos_ctx* c, double* eta, double* phis, double* a1st_add, doubl
     if (a_add_ramp) HIP_OK(hipMemcpy(a_add_ramp, c->a_add_ramp, na * sizeof(double), hipMemcpyDeviceToHost));
     if (a1st) HIP_OK(hipMemcpy(a1st, c->a1st, ns * sizeof(double), hipMemcpyDeviceToHost));
     if (eta1st) HIP_OK(hipMemcpy(eta1st, c->eta1st, ns * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int hos_set_state2(hos_ctx* c, const double* a2nd_add, const double* a2nd, const double* eta2nd) {
+  return guard([&] {
+    if (!c->wmk2) throw std::runtime_error("no second-order wavemaker state (i_wmk = 2 only)");
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    if (a2nd_add) HIP_OK(hipMemcpyAsync(c->a2nd_add, a2nd_add, na * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (a2nd) HIP_OK(hipMemcpyAsync(c->a2nd, a2nd, ns * sizeof(double), hipMemcpyHostToDevice, c->st));
+    if (eta2nd) HIP_OK(hipMemcpyAsync(c->eta2nd, eta2nd, ns * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    c->have_trial = false;
+  });
+}
+
+int hos_get_state2(hos_ctx* c, double* a2nd_add, double* a2nd, double* eta2nd) {
+  return guard([&] {
+    if (!c->wmk2) throw std::runtime_error("no second-order wavemaker state (i_wmk = 2 only)");
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    HIP_OK(hipStreamSynchronize(c->st));
+    if (a2nd_add) HIP_OK(hipMemcpy(a2nd_add, c->a2nd_add, na * sizeof(double), hipMemcpyDeviceToHost));
+    if (a2nd) HIP_OK(hipMemcpy(a2nd, c->a2nd, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (eta2nd) HIP_OK(hipMemcpy(eta2nd, c->eta2nd, ns * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 
@@ -2019,13 +2662,50 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
     HIP_OK(hipMemcpyAsync(c->a1m, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(c->e1m, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(c->aadm, a_add_1, na * sizeof(double), hipMemcpyHostToDevice, st));
-    rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0);
+    if (c->wmk2) {  // the second-order state taken as zero
+      HIP_OK(hipMemsetAsync(c->a2m, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(c->e2m, 0, ns * sizeof(double), st));
+      HIP_OK(hipMemsetAsync(c->aad2m, 0, na * sizeof(double), st));
+      rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0, c->a2m, c->e2m, c->aad2m);
+    } else {
+      rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0);
+    }
     HIP_OK(hipMemcpyAsync(dG1, c->kG1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_OK(hipMemcpyAsync(dG2, c->kG2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
     if (da1st) HIP_OK(hipMemcpyAsync(da1st, c->ka1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
     if (deta1st) HIP_OK(hipMemcpyAsync(deta1st, c->ke1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
     if (da_add_1) HIP_OK(hipMemcpyAsync(da_add_1, c->kad1[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
     if (da_add_ramp) HIP_OK(hipMemcpyAsync(da_add_ramp, c->kadr[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+  });
+}
+
+int hos_rhs2(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
+             const double* eta1st, const double* a_add_1, const double* a2nd, const double* eta2nd,
+             const double* a_add_2, double* dG1, double* dG2, double* da1st, double* deta1st, double* da_add_1,
+             double* da_add_ramp, double* da2nd, double* deta2nd, double* da_add_2) {
+  return guard([&] {
+    if (!c->wmk2) throw std::runtime_error("hos_rhs2 needs the second-order wavemaker (i_wmk = 2)");
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    hipStream_t st = c->st;
+    HIP_OK(hipMemcpyAsync(c->G1m, G1, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->G2m, G2, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->a1m, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->e1m, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->aadm, a_add_1, na * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->a2m, a2nd, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->e2m, eta2nd, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(c->aad2m, a_add_2, na * sizeof(double), hipMemcpyHostToDevice, st));
+    rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0, c->a2m, c->e2m, c->aad2m);
+    HIP_OK(hipMemcpyAsync(dG1, c->kG1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(dG2, c->kG2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da1st) HIP_OK(hipMemcpyAsync(da1st, c->ka1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (deta1st) HIP_OK(hipMemcpyAsync(deta1st, c->ke1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da_add_1) HIP_OK(hipMemcpyAsync(da_add_1, c->kad1[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da_add_ramp) HIP_OK(hipMemcpyAsync(da_add_ramp, c->kadr[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da2nd) HIP_OK(hipMemcpyAsync(da2nd, c->ka2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (deta2nd) HIP_OK(hipMemcpyAsync(deta2nd, c->ke2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (da_add_2) HIP_OK(hipMemcpyAsync(da_add_2, c->kad2[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
   });
 }

@@ -16,6 +16,7 @@ TYPE, BIND(C) :: hos_params
    INTEGER(C_INT) :: n_freq
    TYPE(C_PTR)    :: freq, pos_ini_re, pos_ini_im
    REAL(C_DOUBLE) :: coeffilt(3) = 1.0_C_DOUBLE
+   TYPE(C_PTR)    :: dposdy_ini_re = C_NULL_PTR, dposdy_ini_im = C_NULL_PTR, nu_x = C_NULL_PTR, nu_y = C_NULL_PTR
 END TYPE hos_params
 !
 INTERFACE

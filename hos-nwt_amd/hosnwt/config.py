@@ -144,6 +144,10 @@ class Setup:
     dt_stab: float
     delt_out: float
     waves: WaveComponents | None
+    # y derivative of pos_ini (n2, n_freq) complex, read with i_wmk = 2
+    dposdy_ini: np.ndarray | None = None
+    # x derivative of the beach coefficient as the reference builds it (n1, n2)
+    nu_x: np.ndarray | None = None
 
 
 def gzed(z: float, igeom: int, d: float) -> float:
@@ -190,30 +194,38 @@ def build(cfg: TankConfig) -> Setup:
     dely = ylen / (n2 - 1) if n2 != 1 else 0.0
     cy = np.arange(n2) * dely
     if waves is not None:
-        freq, pos_ini = wmk_ini(waves, cy)
+        freq, pos_ini, dposdy_ini = wmk_ini(waves, cy)
     else:
         freq, pos_ini = np.zeros(0), np.zeros((n2, 0), complex)
+        dposdy_ini = np.zeros((n2, 0), complex)
     nu = absorb(cfg, xlen)
+    nu_x = absorb_dx(cfg, xlen)
     stab = dt_stab(n1, n2, cfg.n3, xlen, ylen, cfg.icase)
     return Setup(cfg=cfg, xlen=xlen, ylen=ylen, Tscale=Tscale, Tramp=Tramp, T_stop=T_stop, T_mono=T_mono,
                  f_out=f_out, freq=freq, pos_ini=pos_ini, nu=nu, dt_stab=stab, delt_out=1.0 / f_out,
-                 waves=waves)
+                 waves=waves, dposdy_ini=dposdy_ini, nu_x=nu_x)
 
 
 def wmk_ini(w: WaveComponents, cy: np.ndarray):
-    """Per-frequency paddle amplitudes pos_ini(y, f) (wavemaking.f90:104-155), full-width paddle."""
+    """Per-frequency paddle amplitudes pos_ini(y, f) and their y derivatives dposdy_ini(y, f)
+    (wavemaking.f90:104-155), full-width paddle (no lateral ramp)."""
     harms = sorted(set(int(x) for x in w.harmo))
     freq = []
-    cols = []
+    cols, dcols = [], []
     for hm in harms:
         sel = np.nonzero(w.harmo == hm)[0]
         freq.append(w.freq[sel[0]])
         col = np.zeros(len(cy), complex)
+        dcol = np.zeros(len(cy), complex)
         for i in sel:
-            col += (w.TF[i] * math.cos(w.angle[i]) * w.ampli[i]
-                    * np.exp(1j * (-w.k[i] * math.sin(w.angle[i]) * cy + w.phase[i])))
+            ky = w.k[i] * math.sin(w.angle[i])
+            term = w.TF[i] * math.cos(w.angle[i]) * w.ampli[i] * np.exp(1j * (-ky * cy + w.phase[i]))
+            col += term
+            dcol += -1j * ky * term
         cols.append(col)
-    return np.array(freq), (np.array(cols).T if cols else np.zeros((len(cy), 0), complex))
+        dcols.append(dcol)
+    empty = np.zeros((len(cy), 0), complex)
+    return (np.array(freq), (np.array(cols).T if cols else empty), (np.array(dcols).T if dcols else empty))
 
 
 def absorb(cfg: TankConfig, xlen: float) -> np.ndarray:
@@ -229,6 +241,22 @@ def absorb(cfg: TankConfig, xlen: float) -> np.ndarray:
         x3 = (cx[sel] - x0) / x2
         nu[sel, :] = (cfg.coeffabsf * x3 * x3)[:, None]
     return nu
+
+
+def absorb_dx(cfg: TankConfig, xlen: float) -> np.ndarray:
+    """x derivative of the front beach as initial_condition.f90:768 builds it: 2 c x3, without
+    the 1/(xlen - x0) factor of the chain rule (read by the second-order wavemaker)."""
+    nux = np.zeros((cfg.n1, cfg.n2))
+    if cfg.iabsnb == 0:
+        return nux
+    x0 = xlen * cfg.xabsf
+    if abs(cfg.xabsf - 1.0) > np.finfo(float).eps:
+        x2 = xlen - x0
+        cx = np.arange(cfg.n1) * xlen / (cfg.n1 - 1)
+        sel = cx >= x0
+        x3 = (cx[sel] - x0) / x2
+        nux[sel, :] = (cfg.coeffabsf * 2.0 * x3)[:, None]
+    return nux
 
 
 def dt_stab(n1, n2, n3, xlen, ylen, icase) -> float:

@@ -27,6 +27,7 @@ class HosParams(C.Structure):
         ("Tramp", C.c_double), ("T_stop", C.c_double), ("T_mono", C.c_double),
         ("nu", _dp), ("n_freq", C.c_int), ("freq", _dp),
         ("pos_ini_re", _dp), ("pos_ini_im", _dp), ("coeffilt", C.c_double * 3),
+        ("dposdy_ini_re", _dp), ("dposdy_ini_im", _dp), ("nu_x", _dp), ("nu_y", _dp),
     ]
 
 
@@ -42,6 +43,7 @@ EXPORTS = [
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
     "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
     "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_check_slope",
+    "hos_set_state2", "hos_get_state2", "hos_rhs2",
 ]
 
 _lib = None
@@ -68,6 +70,9 @@ def load() -> C.CDLL:
     lib.hos_advance.argtypes = [vp, _dp, C.c_double, _dp, C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.hos_rhs.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 11
+    lib.hos_rhs2.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 17
+    lib.hos_set_state2.argtypes = [vp] + [_dp] * 3
+    lib.hos_get_state2.argtypes = [vp] + [_dp] * 3
     lib.hos_get_physical.argtypes = [vp] + [_dp] * 4
     lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
     lib.hos_check_slope.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]

@@ -32,6 +32,10 @@ class Tank:
         pi = fortran(setup.pos_ini.imag)
         fr = np.ascontiguousarray(setup.freq, dtype=np.float64)
         self._keep += [nu, pr, pi, fr]
+        dy = setup.dposdy_ini if setup.dposdy_ini is not None else np.zeros_like(setup.pos_ini)
+        dyr, dyi = fortran(dy.real), fortran(dy.imag)
+        nux = fortran(setup.nu_x) if setup.nu_x is not None else None
+        self._keep += [dyr, dyi, nux]
         p = HosParams()
         p.n1, p.n2, p.n3, p.mHOS = cfg.n1, cfg.n2, cfg.n3, cfg.mHOS
         p.p1 = cfg.mHOS
@@ -47,6 +51,10 @@ class Tank:
         p.pos_ini_im = ptr(pi) if len(fr) else None
         for i in range(3):
             p.coeffilt[i] = float(cfg.coeffilt[i])
+        p.dposdy_ini_re = ptr(dyr) if len(fr) else None
+        p.dposdy_ini_im = ptr(dyi) if len(fr) else None
+        p.nu_x = ptr(nux) if nux is not None else None
+        p.nu_y = None  # no side beach
         h = C.c_void_p()
         check(self.lib.hos_create(C.byref(p), C.byref(h)))
         self.h = h
@@ -83,6 +91,17 @@ class Tank:
         check(self.lib.hos_get_state(self.h, *[ptr(a) for a in out]))
         return dict(zip(("eta", "phis", "a1st_add", "a_add_ramp", "a1st", "eta1st"), out))
 
+    def set_state2(self, a2nd_add=None, a2nd=None, eta2nd=None):
+        """Second-order wavemaker state (i_wmk = 2): paddle modes, modal a2nd, physical eta2nd."""
+        arrs = [None if a is None else fortran(a) for a in (a2nd_add, a2nd, eta2nd)]
+        check(self.lib.hos_set_state2(self.h, *[ptr(a) for a in arrs]))
+
+    def get_state2(self):
+        n1, n2, na = self.n1, self.n2, self.n3a
+        out = [np.zeros((na, n2), order="F"), np.zeros((n1, n2), order="F"), np.zeros((n1, n2), order="F")]
+        check(self.lib.hos_get_state2(self.h, *[ptr(a) for a in out]))
+        return dict(zip(("a2nd_add", "a2nd", "eta2nd"), out))
+
     def physical(self):
         n1, n2 = self.n1, self.n2
         out = [np.zeros((n1, n2), order="F") for _ in range(4)]
@@ -108,6 +127,16 @@ class Tank:
         outs = [np.zeros((n1, n2), order="F") for _ in range(4)] + [np.zeros((na, n2), order="F") for _ in range(2)]
         check(self.lib.hos_rhs(self.h, t, delta_t, *[ptr(a) for a in ins], *[ptr(a) for a in outs]))
         return dict(zip(("dG1", "dG2", "da1st", "deta1st", "da_add_1", "da_add_ramp"), outs))
+
+    def solve_hos2_second(self, t, delta_t, G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2):
+        """One RHS with the second-order wavemaker state (i_wmk = 2)."""
+        n1, n2, na = self.n1, self.n2, self.n3a
+        ins = [fortran(x) for x in (G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2)]
+        outs = ([np.zeros((n1, n2), order="F") for _ in range(4)] + [np.zeros((na, n2), order="F") for _ in range(2)]
+                + [np.zeros((n1, n2), order="F") for _ in range(2)] + [np.zeros((na, n2), order="F")])
+        check(self.lib.hos_rhs2(self.h, t, delta_t, *[ptr(a) for a in ins], *[ptr(a) for a in outs]))
+        return dict(zip(("dG1", "dG2", "da1st", "deta1st", "da_add_1", "da_add_ramp", "da2nd", "deta2nd",
+                         "da_add_2"), outs))
 
     def advance(self, t_next: float, h_rk: float):
         """Adaptive loop to t_next (HOS-NWT.f90:498-571); returns the next step proposal."""

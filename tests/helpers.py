@@ -7,7 +7,7 @@ from oracle import hos_oracle as ho
 
 def make_setup(n1=32, n2=1, M=3, n3=9, xlen=20.0, ylen=10.0, icase=2, igeom=1, T=1.5, kA=0.08,
                Tramp=4.5, iramp=1, d_hinge=0.0, xabsf=0.8, precision=0, components=None, T_stop=45.0,
-               toler=1e-7, f_out=None, coeffilt=(1.0, 1.0, 1.0)):
+               toler=1e-7, f_out=None, coeffilt=(1.0, 1.0, 1.0), i_wmk=1):
     """Nondimensional setup; default is the 2-D regular-wave tank (monochromatic piston)."""
     nu_mono = 1.0 / T
     k = hc.wave_number(nu_mono / np.sqrt(9.81))  # h = 1, so f* = f sqrt(h/g)
@@ -15,7 +15,7 @@ def make_setup(n1=32, n2=1, M=3, n3=9, xlen=20.0, ylen=10.0, icase=2, igeom=1, T
     cfg = hc.TankConfig(n1=n1, n2=n2, mHOS=M, xlen=xlen, ylen=ylen, n3=n3, icase=icase, amp_mono=amp,
                         nu_mono=nu_mono, igeom=igeom, d_hinge=d_hinge, iramp=iramp, Tramp=Tramp, xabsf=xabsf,
                         T_stop=T_stop, toler=toler, f_out=(f_out if f_out else 1.0 / T), precision=precision,
-                        components=components, coeffilt=tuple(coeffilt))
+                        components=components, coeffilt=tuple(coeffilt), i_wmk=i_wmk)
     return hc.build(cfg)
 
 
@@ -24,7 +24,8 @@ def oracle_params(s):
     return ho.Params(n1=c.n1, n2=c.n2, mHOS=c.mHOS, xlen=s.xlen, ylen=s.ylen, n3=c.n3, icase=c.icase,
                      igeom=c.igeom, d_hinge=c.d_hinge / c.h, iramp=c.iramp, Tramp=s.Tramp, T_stop=s.T_stop,
                      T_mono=s.T_mono, iabsnb=c.iabsnb, xabsf=c.xabsf, coeffabsf=c.coeffabsf, freq=s.freq,
-                     pos_ini=s.pos_ini, coeffilt=tuple(c.coeffilt))
+                     pos_ini=s.pos_ini, coeffilt=tuple(c.coeffilt), i_wmk=c.i_wmk,
+                     dposdy_ini=s.dposdy_ini)
 
 
 def random_state(s, seed, amp=0.02):
