@@ -1563,6 +1563,7 @@ struct Engine : EngineBase {
   explicit Engine(hos_ctx* ctx) : c(ctx) {
     Arena& ar = c->ar;
     gb.build(c->N1b, c->N2b);
+    gb.check_lengths();
     gsT.build(c->n1, c->n2);
     nb = (size_t)c->N1b * c->N2b;
     ns = (size_t)c->n1 * c->n2;
@@ -1620,6 +1621,10 @@ struct Engine : EngineBase {
     for (int i = 0; i < 3; ++i) tmpXw.push_back(ar.get<T>(nb));
     use_main();
     dense = dense_base_grid(c->n1, c->n2);
+    if (!dense) {
+      gsT.check_lengths();
+      c->gs64.check_lengths();
+    }
     if (dense) {
       build_dense<T>(dT, ar, c->n1, c->n2);
       if (sizeof(T) != sizeof(double)) build_dense<double>(d64, ar, c->n1, c->n2);

@@ -431,6 +431,17 @@ struct Grid2 {
         TX /= 2;
     }
   }
+  // Throws when a line is longer than both engines hold: the register engine
+  // (compiled lengths) and one LDS workgroup of the Stockham/Bluestein engine.
+  void check_lengths() const {
+    auto fits = [](const LinePlan<T>& pl, int vpt) {
+      return pl.V2 || ((size_t)pl.dev.L * sizeof(C2<T>) <= 160 * 1024 && (long)pl.dev.L <= 256L * vpt);
+    };
+    if (!fits(px, 32) || (N2 > 1 && !fits(py, 16)))
+      throw std::runtime_error("grid " + std::to_string(N1) + " x " + std::to_string(N2) +
+                               ": line transform longer than the supported lengths (compiled register lengths up "
+                               "to 12288 points, or one LDS workgroup)");
+  }
   void release() {
     px.release();
     py.release();

@@ -28,9 +28,9 @@ CHARACTER(LEN=100) :: file_name, pro_file
 REAL(RP) :: Tscale, T_mono, w_mono, dt_stab, delt, delt_out, time_cur, time_next, h_rk
 REAL(RP) :: k_w, freq1, delx, dely, x0, x2, x3, c8, d1, d2, d3, voltot, enepot, enekin, alpha, pos_top
 REAL(RP) :: t_adim
-COMPLEX(RP) :: tf
+COMPLEX(RP) :: tf, ctmp
 REAL(RP), ALLOCATABLE, TARGET :: nu(:,:), eta(:,:), phis(:,:), deta(:,:), phi_add(:,:), zero2(:,:), zadd(:,:)
-REAL(RP), ALLOCATABLE, TARGET :: freq(:), pre(:), pim(:)
+REAL(RP), ALLOCATABLE, TARGET :: freq(:), pre(:), pim(:), dyre(:), dyim(:), nux(:,:)
 TYPE(hos_params) :: p
 TYPE(C_PTR) :: ctx
 INTEGER(C_INT) :: n_acc, n_rej, fx, fy
@@ -68,7 +68,7 @@ IF (icase == 2 .AND. ibat /= 2) ERROR STOP 'only the front wave (ibat = 2) is bu
 islosh = islosh + 1
 IF (n2 == 1) theta_mono = 0.0_rp
 IF (iramp == 0) Tramp = 0.0_rp
-i_wmk = 1
+IF (i_wmk /= 1 .AND. i_wmk /= 2) ERROR STOP 'i_wmk = 1 or 2 is built here'
 !
 ! nondimensionalisation (HOS-NWT.f90:99-119)
 Tscale = 1.0_rp / SQRT(G / h)
@@ -107,18 +107,28 @@ IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
    END DO
 END IF
 !
-ALLOCATE(freq(1), pre(n2), pim(n2))
-freq = 0.0_rp; pre = 0.0_rp; pim = 0.0_rp
+ALLOCATE(freq(1), pre(n2), pim(n2), dyre(n2), dyim(n2), nux(n1,n2))
+freq = 0.0_rp; pre = 0.0_rp; pim = 0.0_rp; dyre = 0.0_rp; dyim = 0.0_rp; nux = 0.0_rp
+! beach x derivative as initial_condition.f90:768 builds it (2 c x3), read with i_wmk = 2
+IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
+   DO i1 = 1, n1
+      IF ((i1 - 1) * delx >= x0) nux(i1,:) = coeffabsf * 2.0_rp * ((i1 - 1) * delx - x0) / x2
+   END DO
+END IF
 IF (icase == 2) THEN
    ! front(): one harmonic at nu_mono (wave_def.f90:54-80), wave_init and wmk_ini
    freq1 = nu_mono
    k_w = wave_number(freq1)
    tf = transfer(igeom, k_w, d_hinge)
    DO i2 = 1, n2
-      pre(i2) = REAL(tf * COS(theta_mono * PI / 180.0_rp) * amp_mono * &
-                EXP(CMPLX(0.0_rp, -k_w * SIN(theta_mono * PI / 180.0_rp) * (i2 - 1) * dely + ph_mono, RP)))
-      pim(i2) = AIMAG(tf * COS(theta_mono * PI / 180.0_rp) * amp_mono * &
-                EXP(CMPLX(0.0_rp, -k_w * SIN(theta_mono * PI / 180.0_rp) * (i2 - 1) * dely + ph_mono, RP)))
+      ctmp = tf * COS(theta_mono * PI / 180.0_rp) * amp_mono * &
+             EXP(CMPLX(0.0_rp, -k_w * SIN(theta_mono * PI / 180.0_rp) * (i2 - 1) * dely + ph_mono, RP))
+      pre(i2) = REAL(ctmp)
+      pim(i2) = AIMAG(ctmp)
+      ! y derivative of the paddle amplitude (wavemaking.f90:147-149, no lateral ramp)
+      ctmp = CMPLX(0.0_rp, -k_w * SIN(theta_mono * PI / 180.0_rp), RP) * ctmp
+      dyre(i2) = REAL(ctmp)
+      dyim(i2) = AIMAG(ctmp)
    END DO
    freq(1) = freq1
 END IF
@@ -132,6 +142,7 @@ p%nu = C_LOC(nu)
 p%n_freq = 0
 IF (icase == 2) p%n_freq = 1
 p%freq = C_LOC(freq); p%pos_ini_re = C_LOC(pre); p%pos_ini_im = C_LOC(pim)
+p%dposdy_ini_re = C_LOC(dyre); p%dposdy_ini_im = C_LOC(dyim); p%nu_x = C_LOC(nux)
 CALL hos_check(hos_create(p, ctx), 'hos_create')
 IF (icase == 1) THEN
    DO i1 = 1, n1

@@ -48,6 +48,35 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: eta(*), phis(*), a1st_add(*), a_add_ramp(*), a1st(*), eta1st(*)
    END FUNCTION hos_get_state
+   ! second-order wavemaker state (i_wmk = 2): a2nd_add (m3_add,m2), a2nd (modal), eta2nd
+   INTEGER(C_INT) FUNCTION hos_set_state2(ctx, a2nd_add, a2nd, eta2nd) BIND(C, NAME='hos_set_state2')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: a2nd_add(*), a2nd(*), eta2nd(*)
+   END FUNCTION hos_set_state2
+   INTEGER(C_INT) FUNCTION hos_get_state2(ctx, a2nd_add, a2nd, eta2nd) BIND(C, NAME='hos_get_state2')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: a2nd_add(*), a2nd(*), eta2nd(*)
+   END FUNCTION hos_get_state2
+   ! one solveHOS2 evaluation (resol_HOS.f90:270-656)
+   INTEGER(C_INT) FUNCTION hos_rhs(ctx, t, delta_t, G1, G2, a1st, eta1st, a_add_1, dG1, dG2, da1st, deta1st, &
+        da_add_1, da_add_ramp) BIND(C, NAME='hos_rhs')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: t, delta_t
+      REAL(C_DOUBLE), INTENT(IN)  :: G1(*), G2(*), a1st(*), eta1st(*), a_add_1(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: dG1(*), dG2(*), da1st(*), deta1st(*), da_add_1(*), da_add_ramp(*)
+   END FUNCTION hos_rhs
+   INTEGER(C_INT) FUNCTION hos_rhs2(ctx, t, delta_t, G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2, &
+        dG1, dG2, da1st, deta1st, da_add_1, da_add_ramp, da2nd, deta2nd, da_add_2) BIND(C, NAME='hos_rhs2')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: t, delta_t
+      REAL(C_DOUBLE), INTENT(IN)  :: G1(*), G2(*), a1st(*), eta1st(*), a_add_1(*), a2nd(*), eta2nd(*), a_add_2(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: dG1(*), dG2(*), da1st(*), deta1st(*), da_add_1(*), da_add_ramp(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: da2nd(*), deta2nd(*), da_add_2(*)
+   END FUNCTION hos_rhs2
    INTEGER(C_INT) FUNCTION hos_rk_step(ctx, t, h, err) BIND(C, NAME='hos_rk_step')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx

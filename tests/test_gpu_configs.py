@@ -75,3 +75,13 @@ def test_rhs_parity_at_config_size(name):
             assert v < TOL, (name, k, v)
     finally:
         tank.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n1,M", [(4096, 3), (2048, 6)])
+def test_lines_beyond_the_engines_are_refused_at_create(n1, M):
+    """cfg4 members whose extended lines exceed both line engines (16382 and 14334 points)
+    fail at hos_create with a message, before any device work."""
+    s = make_setup(n1=n1, n2=1, M=M, n3=33, xlen=60.0, Tramp=3.0)
+    with pytest.raises(RuntimeError, match="line transform longer"):
+        Tank(s, precision=0)

@@ -43,7 +43,7 @@ Peak period (s)              :: Tp               :: 1.d0
 Shape factor (Jonswap)       :: gamma            :: 3.3d0
 Seed number for random numb. :: iseed            :: 513
 -------------------- Wavemaker definition ---------------------
-Nonlinear order of wavemaker :: i_wmk            :: 1
+Nonlinear order of wavemaker :: i_wmk            :: {i_wmk}
 Type (1: piston, 2: hinged)  :: igeom            :: 1
 Rotation axis distance       :: d_hinge          :: 0.d0
 Time ramp                    :: iramp            :: 1
@@ -57,7 +57,7 @@ Use of probes                :: iprobes          :: 0
 Filename of probe positions  :: pro_file         :: prob.inp
 ---------------------- Time-integration -----------------------
 Duration of the simulation   :: T_stop           :: {tstop!r}
-Time tolerance: RK 4(5)      :: toler            :: 1.e-7
+Time tolerance: RK 4(5)      :: toler            :: {toler!r}
 Output frequency             :: f_out            :: {fout!r}
 --------------------------- Output ----------------------------
 Output: 1-dim. ; 0-nondim.   :: idim             :: 0
@@ -69,13 +69,14 @@ Swense output 1='yes',0='no' :: i_sw             :: 0
 
 
 @pytest.mark.gpu
-def test_fortran_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("i_wmk,toler,tstop", [(1, 1e-7, 3.0), (2, 1e-6, 1.5)])
+def test_fortran_driver_matches_oracle(tmp_path, i_wmk, toler, tstop):
     assert os.path.exists(EXE), "build the Fortran driver with hos-nwt_amd/build.sh"
-    T, tstop, n1, M = 1.5, 3.0, 64, 3
-    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T)
+    T, n1, M = 1.5, 64, 3
+    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T, toler=toler, i_wmk=i_wmk)
     amp = s.cfg.amp_mono
     # values written with Python repr are read back exactly by ES25.16
-    text = INPUT.format(amp=amp, nu=1.0 / T, tstop=tstop, fout=2.0 / T)
+    text = INPUT.format(amp=amp, nu=1.0 / T, tstop=tstop, fout=2.0 / T, i_wmk=i_wmk, toler=toler)
     (tmp_path / "input_HOS-NWT.dat").write_text(text)
     r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])

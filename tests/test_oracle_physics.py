@@ -420,3 +420,133 @@ def test_second_order_paddle_fields_are_consistent():
     np.testing.assert_allclose(g["axx"], o.FA.fourier_2_space(aadd * tb.k_add ** 2, "cos", "cos")[:p.n3],
                                atol=1e-15)
     assert np.abs(phi_add).max() > 0
+
+
+@pytest.mark.parametrize("n2", [1, 6])
+def test_second_order_z0_fields_are_derivatives(n2):
+    """The extra z = 0 wavemaker fields of phiadd_1st_SWEET (resol_wmkr.f90:500-524) are
+    derivatives of the brute-force-pinned fields: z derivatives by centred differences of
+    phiadd_sl at z = +-d, time derivatives as the same sums of da, and (2-D) phi_xx = -phi_zz
+    from Laplace's equation."""
+    p = tiny_params(n1=9, n2=n2)
+    tb = ho.Tables(p)
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((p.n3_add, n2)) * np.exp(-0.3 * np.arange(p.n3_add))[:, None]
+    da = rng.standard_normal((p.n3_add, n2)) * np.exp(-0.3 * np.arange(p.n3_add))[:, None]
+    z0 = ho.phiadd_sl(a, da, None, tb, need=ho.Oracle.SECOND_FIELDS)
+    d = 1e-5
+    up = ho.phiadd_sl(a, da, np.full((p.n1, n2), d), tb)
+    dn = ho.phiadd_sl(a, da, np.full((p.n1, n2), -d), tb)
+    scale = max(np.abs(z0[k]).max() for k in z0)
+    for fld, base in (("phizz", "phiz"), ("phixz", "phix")):
+        np.testing.assert_allclose(z0[fld], (up[base] - dn[base]) / (2 * d), atol=1e-7 * scale, err_msg=fld)
+    # the reference's phi_yz sum has no kx_add factor (resol_wmkr.f90:522-523): it is the z
+    # derivative of phi_y for the coefficients a / kx_add
+    ak = a / tb.kx_add[:, None]
+    upk = ho.phiadd_sl(ak, da, np.full((p.n1, n2), d), tb, need=("phiy",))
+    dnk = ho.phiadd_sl(ak, da, np.full((p.n1, n2), -d), tb, need=("phiy",))
+    np.testing.assert_allclose(z0["phiyz"], (upk["phiy"] - dnk["phiy"]) / (2 * d), atol=1e-7 * scale)
+    zt_up = ho.phiadd_sl(da, da, np.full((p.n1, n2), d), tb, need=("phiz",))
+    zt_dn = ho.phiadd_sl(da, da, np.full((p.n1, n2), -d), tb, need=("phiz",))
+    np.testing.assert_allclose(z0["phizt"], ho.phiadd_sl(da, da, None, tb, need=("phiz",))["phiz"], atol=1e-14)
+    np.testing.assert_allclose(z0["phizt"], (zt_up["phiz"] + zt_dn["phiz"]) / 2, atol=1e-8 * scale)
+    dd = ho.phiadd_sl(da, da, None, tb)
+    np.testing.assert_allclose(z0["phixt"], dd["phix"], atol=1e-14)
+    np.testing.assert_allclose(z0["phiyt"], dd["phiy"], atol=1e-14)
+    if n2 == 1:
+        np.testing.assert_allclose(z0["phixx"], -z0["phizz"], atol=1e-12 * scale)
+    assert np.abs(z0["phizz"]).max() > 0 and np.abs(z0["phixz"]).max() > 0
+
+
+@pytest.mark.parametrize("n2", [1, 7])
+def test_second_order_free_surface_conditions_closed_form(n2):
+    """solveHOS_SWEET_2's free-surface conditions (resol_wmkr.f90:302-318) on a single
+    first-order mode phi1 = a cosh(k(z+1))/cosh k cos(kx x) cos(ky y), eta1 = e cos cos, with a
+    still paddle (no additional potential): the Taylor-expanded second-order forcing
+      deta2 = -phi1_x eta1_x - phi1_y eta1_y + eta1 phi1_zz,
+      da2   = S2F(-(phi1_x^2 + phi1_y^2 + phi1_z^2)/2 - eta1 phi1_zt),  phi1_zt = -e k tanh k cc
+    (da1 = -eta1 without beach) in closed form at the nodes."""
+    n1, m, nm = 16, 3, (2 if n2 > 1 else 0)
+    p = ho.Params(n1=n1, n2=n2, mHOS=3, xlen=6.0, ylen=4.0, n3=9, icase=2, i_wmk=2, iabsnb=0,
+                  pos_ini=np.zeros((n2, 0), complex), freq=np.zeros(0))
+    o = ho.Oracle(p)
+    tb = o.tb
+    a, e = 0.013, 0.021
+    a1 = np.zeros((n1, n2))
+    a1[m, nm] = a
+    kx, ky = tb.kx[m], tb.ky[nm]
+    k = math.hypot(kx, ky)
+    X, Y = np.meshgrid(tb.cx, tb.cy if n2 > 1 else np.zeros(1), indexing="ij")
+    cx, sx = np.cos(kx * X), np.sin(kx * X)
+    cy, sy = np.cos(ky * Y), np.sin(ky * Y)
+    e1 = e * cx * cy
+    z1, za = np.zeros((n1, n2)), np.zeros((p.n3_add, n2))
+    o.time = 0.3
+    o.solve_hos2(a1, e1, z1, z1, za, 0.01, a2nd=z1, eta2nd=z1, a_add_2=za)
+    da2, deta2, d2 = o.second
+    th = math.tanh(k)
+    px, py, pz = -a * kx * sx * cy, -a * ky * cx * sy, a * k * th * cx * cy
+    ex, ey = -e * kx * sx * cy, -e * ky * cx * sy
+    pzz, pzt = a * k * k * cx * cy, -e * k * th * cx * cy
+    want_deta = -px * ex - py * ey + e1 * pzz
+    want_tmp = -0.5 * (px ** 2 + py ** 2 + pz ** 2) - e1 * pzt
+    np.testing.assert_allclose(d2, 0.0, atol=1e-18)
+    np.testing.assert_allclose(deta2, want_deta, atol=1e-15)
+    np.testing.assert_allclose(o.f2s(da2), want_tmp, atol=1e-15)
+    assert np.abs(want_deta).max() > 1e-5 and np.abs(want_tmp).max() > 1e-5
+
+
+@pytest.mark.parametrize("n2", [1, 6])
+def test_second_order_paddle_condition_is_a_time_derivative(n2):
+    """solveHOS_SWEET_2's paddle condition (resol_wmkr.f90:264-277) is
+    d2 = -d/dt[alpha (-X phi_xx + X_y phi_y + X_z phi_z)] on the paddle, the time derivative
+    taken along the first-order solution (a1 + t da1, a_add_1 + t da_add_1, X(t)).  Checked by
+    centred differences in t of that bracket, evaluated by solveHOS_SWEET_1 / d1st_for_2nd at
+    shifted times and states, then continued, transformed and scaled as the reference does."""
+    n1 = 14
+    rng = np.random.default_rng(31)
+    nf = 2
+    pos = (0.02 * rng.standard_normal((n2, nf)) + 0.02j * rng.standard_normal((n2, nf)))
+    dpos = (0.05 * rng.standard_normal((n2, nf)) + 0.05j * rng.standard_normal((n2, nf))) if n2 > 1 else \
+        np.zeros((1, nf), complex)
+    p = ho.Params(n1=n1, n2=n2, mHOS=3, xlen=5.0, ylen=3.0, n3=9, icase=3, i_wmk=2, igeom=2, d_hinge=0.3,
+                  iramp=2, Tramp=4.0, iabsnb=0, freq=np.array([0.31, 0.47]), pos_ini=pos, dposdy_ini=dpos)
+    o = ho.Oracle(p)
+    tb = o.tb
+    dec = np.exp(-0.4 * tb.k)
+    a1 = 0.01 * rng.standard_normal((n1, n2)) * dec
+    e1 = 0.01 * rng.standard_normal((n1, n2)) * dec
+    aad = 0.003 * rng.standard_normal((p.n3_add, n2)) * np.exp(-0.3 * np.arange(p.n3_add))[:, None]
+    z1, za = np.zeros((n1, n2)), np.zeros((p.n3_add, n2))
+    t0 = 1.3
+    o.time = t0
+    o.solve_hos2(a1, e1, z1, z1, aad, 0.01, a2nd=z1, eta2nd=z1, a_add_2=za)
+    d2 = o.second[2]
+    o.time = t0
+    da1, _, d1, _ = o.solve_sweet_1(a1, e1, aad)
+
+    def bracket(dt):
+        o.time = t0 + dt
+        o.solve_sweet_1(a1 + dt * da1, e1, aad + dt * d1)
+        f = o.first
+        dy, _, dz, _ = ho.dpos_1st(o.time, p, tb)
+        return o.alpha * (-o.pos1st * f["axx"] + dy * f["ay"] + dz * f["az"])
+
+    h = 1e-4
+    fd = np.zeros((p.n3_add, n2))
+    fd[:p.n3] = -(bracket(h) - bracket(-h)) / (2 * h)
+    fd = ho.match_surface_wmk(fd, p.n3, tb.delz)
+    fd = o.FA.space_2_fourier(fd) / tb.k_add_thk_add
+    scale = np.abs(d2).max()
+    assert scale > 1e-6
+    np.testing.assert_allclose(d2, fd, atol=1e-6 * scale)
+
+
+@pytest.mark.parametrize("igeom,d", [(1, 0.0), (2, 0.0), (2, 0.3), (2, 0.55)])
+def test_dgzeddz_is_the_derivative_of_gzed(igeom, d):
+    """dgzeddz (wavemaking.f90:219-250) against centred differences of gzed, including the
+    smoothed hinge region."""
+    h = 1e-6
+    for z in np.linspace(-0.999, -0.001, 37):
+        fd = (ho.gzed(z + h, igeom, d) - ho.gzed(z - h, igeom, d)) / (2 * h)
+        assert abs(ho.dgzeddz(z, igeom, d) - fd) < 1e-6, (z, igeom, d)
