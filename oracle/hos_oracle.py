@@ -246,6 +246,10 @@ class Params:
         if self.dposdy_ini is None:
             self.dposdy_ini = np.zeros_like(np.asarray(self.pos_ini, complex))
         self.has_wmk = self.icase in (2, 3, 31, 32, 33, 4, 41, 42) and self.igeom in (1, 2)
+        if self.i_wmk == 3 and min(self.coeffilt) < 1.0:
+            # wmk_3rdorder filters its INTENT(IN) stage inputs in place (resol_wmkr.f90:1019-1030):
+            # what that does to the caller's arrays is left to the compiler, so it is not restated
+            raise ValueError("i_wmk = 3 with coeffilt < 1 is not supported")
 
 
 class Tables:
@@ -655,6 +659,8 @@ class State:
     a2nd_add: np.ndarray | None = None
     a2nd: np.ndarray | None = None
     eta2nd: np.ndarray | None = None
+    # third-order wavemaker paddle modes (i_wmk = 3)
+    a3rd_add: np.ndarray | None = None
 
     def __post_init__(self):
         if self.a2nd_add is None:
@@ -663,10 +669,13 @@ class State:
             self.a2nd = np.zeros_like(self.a1st)
         if self.eta2nd is None:
             self.eta2nd = np.zeros_like(self.eta1st)
+        if self.a3rd_add is None:
+            self.a3rd_add = np.zeros_like(self.a1st_add)
 
     def copy(self):
         return State(*(x.copy() for x in (self.eta, self.phis, self.a1st_add, self.a_add_ramp,
-                                          self.a1st, self.eta1st, self.a2nd_add, self.a2nd, self.eta2nd)))
+                                          self.a1st, self.eta1st, self.a2nd_add, self.a2nd, self.eta2nd,
+                                          self.a3rd_add)))
 
 
 class Oracle:
@@ -678,6 +687,8 @@ class Oracle:
         self.F = Fourier(p.n1, p.n2)
         self.FB = Fourier(p.Nd1, p.Nd2)
         self.FA = FourierAdd(p.n3_add, p.n2)
+        self.second = None  # (da2nd, deta2nd, da_add_2) of the last solve_hos2, i_wmk >= 2
+        self.third = None  # da_add_3 of the last solve_hos2, i_wmk = 3
         self.wmk_direct = wmk_direct
         # module globals written inside the step (resol_wmkr.f90:1268-1397, wavemaking.f90:277-295)
         self.alpha = self.alphat = self.alphatt = 0.0
@@ -881,7 +892,6 @@ class Oracle:
         """Returns (fields, da1): first-order potential derivatives on the paddle (n3 x n2)
         and on the free surface (n1 x n2), and da1 after the optional x/y filters."""
         p, tb = self.p, self.tb
-        n3 = p.n3
         a1 = a1.copy()
         if p.coeffilt[0] < 1.0:
             a1 = filter_modes(a1, p.coeffilt[0], 0)
@@ -889,37 +899,9 @@ class Oracle:
         if p.n2 != 1 and p.coeffilt[1] < 1.0:
             a1 = filter_modes(a1, p.coeffilt[1], 1)
             da1 = filter_modes(da1, p.coeffilt[1], 1)
-        FA = self.FA
-        ka2 = tb.k_add ** 2
+        f = self.paddle_fields(a1, da1, a_add_1, d1)
         kyb = tb.ky[None, :]
-        kxa = tb.kx_add[:, None]
-        # additional-potential derivatives on the paddle (phiadd_1st_SWEET :535-566)
-        add = dict(
-            axx=FA.fourier_2_space(a_add_1 * ka2, "cos", "cos")[:n3],
-            axxt=FA.fourier_2_space(d1 * ka2, "cos", "cos")[:n3],
-            ay=FA.fourier_2_space(-a_add_1 * kyb, "cos", "sin")[:n3],
-            ayt=FA.fourier_2_space(-d1 * kyb, "cos", "sin")[:n3],
-            az=FA.fourier_2_space(-a_add_1 * kxa, "sin", "cos")[:n3],
-            azt=FA.fourier_2_space(-d1 * kxa, "sin", "cos")[:n3],
-        )
-        # free-surface modes on the paddle x = 0 (:686-709; tables HOS-NWT.f90:342-355)
         k = tb.k
-        kx2 = (tb.kx ** 2)[:, None]
-        sums = {key: np.zeros((n3, p.n2)) for key in ("axx", "axxt", "ay", "ayt", "az", "azt")}
-        for i3 in range(n3):
-            cxa = i3 * tb.delz  # cx_add (HOS-NWT.f90:242)
-            ch = np.cosh(k * cxa) / np.cosh(k)
-            sh = k * np.sinh(k * cxa) / np.cosh(k)
-            sums["axx"][i3] = -(a1 * ch * kx2).sum(axis=0)
-            sums["axxt"][i3] = -(da1 * ch * kx2).sum(axis=0)
-            sums["ay"][i3] = -(a1 * ch * kyb).sum(axis=0)
-            sums["ayt"][i3] = -(da1 * ch * kyb).sum(axis=0)
-            sums["az"][i3] = (a1 * sh).sum(axis=0)
-            sums["azt"][i3] = (da1 * sh).sum(axis=0)
-        f = {}
-        for key, cy in (("axx", "cos"), ("axxt", "cos"), ("ay", "sin"), ("ayt", "sin"), ("az", "cos"),
-                        ("azt", "cos")):
-            f[key] = (fourier_2_space_y_axis(sums[key], cy, 1) if p.n2 != 1 else sums[key]) + add[key]
         # free-surface first-order fields (:711-777)
         kx = tb.kx[:, None]
         kth1 = tb.kth[:, :, 0]
@@ -936,6 +918,92 @@ class Oracle:
                         - p.iabsnb * (tb.dnuy * phiz1st + tb.nu * phiyz1st))
         f["phiz1st"] = phiz1st
         return f, da1
+
+    def paddle_fields(self, a_fs, da_fs, a_add, d_add):
+        """phi_xx, phi_y, phi_z and their time derivatives on the paddle (n3 x n2) of one order
+        of the potential: the additional part (phiadd_1st_SWEET :535-566, wmk_3rdorder
+        :982-1012) plus the free-surface modes at x = 0 (d1st_for_2nd :686-709, wmk_3rdorder
+        :1045-1073; tables HOS-NWT.f90:342-355), y-transformed.  Keys axx, axxt, ay, ayt, az, azt."""
+        p, tb = self.p, self.tb
+        n3 = p.n3
+        FA = self.FA
+        ka2 = tb.k_add ** 2
+        kyb = tb.ky[None, :]
+        kxa = tb.kx_add[:, None]
+        add = dict(
+            axx=FA.fourier_2_space(a_add * ka2, "cos", "cos")[:n3],
+            axxt=FA.fourier_2_space(d_add * ka2, "cos", "cos")[:n3],
+            ay=FA.fourier_2_space(-a_add * kyb, "cos", "sin")[:n3],
+            ayt=FA.fourier_2_space(-d_add * kyb, "cos", "sin")[:n3],
+            az=FA.fourier_2_space(-a_add * kxa, "sin", "cos")[:n3],
+            azt=FA.fourier_2_space(-d_add * kxa, "sin", "cos")[:n3],
+        )
+        k = tb.k
+        kx2 = (tb.kx ** 2)[:, None]
+        sums = {key: np.zeros((n3, p.n2)) for key in ("axx", "axxt", "ay", "ayt", "az", "azt")}
+        for i3 in range(n3):
+            cxa = i3 * tb.delz  # cx_add (HOS-NWT.f90:242)
+            ch = np.cosh(k * cxa) / np.cosh(k)
+            sh = k * np.sinh(k * cxa) / np.cosh(k)
+            sums["axx"][i3] = -(a_fs * ch * kx2).sum(axis=0)
+            sums["axxt"][i3] = -(da_fs * ch * kx2).sum(axis=0)
+            sums["ay"][i3] = -(a_fs * ch * kyb).sum(axis=0)
+            sums["ayt"][i3] = -(da_fs * ch * kyb).sum(axis=0)
+            sums["az"][i3] = (a_fs * sh).sum(axis=0)
+            sums["azt"][i3] = (da_fs * sh).sum(axis=0)
+        f = {}
+        for key, cy in (("axx", "cos"), ("axxt", "cos"), ("ay", "sin"), ("ayt", "sin"), ("az", "cos"),
+                        ("azt", "cos")):
+            f[key] = (fourier_2_space_y_axis(sums[key], cy, 1) if p.n2 != 1 else sums[key]) + add[key]
+        return f
+
+    # ----- wavemaker third order (resol_wmkr.f90:338-437, :942-1115) --------------
+    def wmk_3rdorder(self, a2nd, da2nd, a_add_1, d1, a_add_2, d2):
+        """First-order third derivatives (additional part only: the free-surface part is a sine
+        series in x, zero on the paddle) and second-order paddle fields (wmk_3rdorder
+        :982-1098).  Keys xxx, xxxt, xy, xyt, xz, xzt and the paddle_fields keys of order 2."""
+        p, tb = self.p, self.tb
+        n3 = p.n3
+        FA = self.FA
+        kt = tb.k_add_thk_add
+        k2 = tb.k_add ** 2
+        kyb = tb.ky[None, :]
+        kxa = tb.kx_add[:, None]
+        first = dict(
+            xxx=FA.fourier_2_space(-a_add_1 * k2 * kt, "cos", "cos")[:n3],
+            xxxt=FA.fourier_2_space(-d1 * k2 * kt, "cos", "cos")[:n3],
+            xy=FA.fourier_2_space(a_add_1 * kyb * kt, "cos", "sin")[:n3],
+            xyt=FA.fourier_2_space(d1 * kyb * kt, "cos", "sin")[:n3],
+            xz=FA.fourier_2_space(a_add_1 * kxa * kt, "sin", "cos")[:n3],
+            xzt=FA.fourier_2_space(d1 * kxa * kt, "sin", "cos")[:n3],
+        )
+        second = self.paddle_fields(a2nd, da2nd, a_add_2, d2)
+        return first, second
+
+    def solve_sweet_3(self, a2nd, da2nd, a_add_1, d1, a_add_2, d2):
+        """Third-order paddle condition and modes (solveHOS_SWEET_3, resol_wmkr.f90:338-437).
+        Returns da3rd_add (n3_add x n2 modal).  phiadd_3rd (:872-933) is not formed: the
+        right-hand side reads none of its fields."""
+        p, tb = self.p, self.tb
+        n3, n3a = p.n3, p.n3_add
+        f1, f2 = self.wmk_3rdorder(a2nd, da2nd, a_add_1, d1, a_add_2, d2)
+        al, alt = self.alpha, self.alphat
+        X, Xt = self.pos1st, self.dpos1stdt
+        Xy, Xyt, Xz, Xzt = dpos_1st(self.time, p, tb)
+        t_al = -al * (-Xt * f2["axx"] - X * f2["axxt"] + Xyt * f2["ay"] + Xy * f2["ayt"]
+                      + Xzt * f2["az"] + Xz * f2["azt"])
+        t_alt = -alt * (-X * f2["axx"] + Xy * f2["ay"] + Xz * f2["az"])
+        t_al2 = -al ** 2 * (-(X ** 2) / 2.0 * f1["xxxt"] - Xt * X * f1["xxx"] + Xt * Xy * f1["xy"]
+                            + X * Xy * f1["xyt"] + X * Xyt * f1["xy"] + Xt * Xz * f1["xz"]
+                            + X * Xz * f1["xzt"] + X * Xzt * f1["xz"])
+        t_alalt = -al * alt * (-(X ** 2) * f1["xxx"] + 2.0 * X * Xy * f1["xy"] + 2.0 * X * Xz * f1["xz"])
+        d3 = np.zeros((n3a, p.n2))
+        d3[:n3] = t_al + t_alt + t_al2 + t_alalt
+        d3 = match_surface_wmk(d3, n3, tb.delz)
+        d3 = self.FA.space_2_fourier(d3)
+        if p.coeffilt[2] < 1.0:
+            d3 = filter_modes(d3, p.coeffilt[2], 0)
+        return d3 / tb.k_add_thk_add
 
     # ----- wavemaker second order (resol_wmkr.f90:219-329) -----------------
     def solve_sweet_2(self, a2nd, eta2nd, eta1st, a2nd_add):
@@ -965,11 +1033,14 @@ class Oracle:
         return self.s2f(da2), deta2, d2, sl2
 
     # ----- solveHOS2 (resol_HOS.f90:270-656) ------------------------------
-    def solve_hos2(self, a1st, eta1st, G2, G1, a_add_1, delta_t, a2nd=None, eta2nd=None, a_add_2=None):
+    def solve_hos2(self, a1st, eta1st, G2, G1, a_add_1, delta_t, a2nd=None, eta2nd=None, a_add_2=None,
+                   a_add_3=None):
         """One RHS.  Returns (dG1, dG2, da1st, deta1st, da_add_1, da_add_ramp); with i_wmk >= 2
-        the second-order derivatives (da2nd, deta2nd, da_add_2) are left in ``self.second``."""
+        the second-order derivatives (da2nd, deta2nd, da_add_2) are left in ``self.second``,
+        and with i_wmk = 3 da_add_3 in ``self.third``."""
         p, tb = self.p, self.tb
         self.second = None
+        self.third = None
         self.n_rhs += 1
         etark, phisrk = self.gtof(G1, G2, delta_t)
         om = tb.omega.copy()
@@ -1006,6 +1077,11 @@ class Oracle:
                 self.second = (da2, deta2, d2)
                 # total additional potential (resol_HOS.f90:498-502)
                 a_tot, d_tot = a_add_1 + aa2, d1 + d2
+                if p.i_wmk >= 3:
+                    aa3 = np.zeros((n3a, n2)) if a_add_3 is None else a_add_3
+                    d3 = self.solve_sweet_3(a2, da2, a_add_1, d1, aa2, d2)
+                    self.third = d3
+                    a_tot, d_tot = a_tot + aa3, d_tot + d3
             if self.wmk_direct:
                 sl = phiadd_sl_bruteforce(a_tot, d_tot, etark, tb)
             else:
@@ -1114,19 +1190,23 @@ class Oracle:
         G2 = G2 * tb.omega
         s = 5
         kG1, kG2, ka1, ke1, kad1, kadr = ([None] * s for _ in range(6))
-        # second-order wavemaker state (zero unless i_wmk >= 2)
-        ka2, ke2, kad2 = ([None] * s for _ in range(3))
+        # second- and third-order wavemaker state (zero below i_wmk = 2, 3)
+        ka2, ke2, kad2, kad3 = ([None] * s for _ in range(4))
 
         def second_out():
             if self.second is None:
                 return np.zeros_like(st.a2nd), np.zeros_like(st.eta2nd), np.zeros_like(st.a2nd_add)
             return self.second
 
+        def third_out():
+            return np.zeros_like(st.a3rd_add) if self.third is None else self.third
+
         self.time = t_o
         out = self.solve_hos2(st.a1st, st.eta1st, G2, G1, st.a1st_add, h * self.C[0], st.a2nd, st.eta2nd,
-                              st.a2nd_add)
+                              st.a2nd_add, st.a3rd_add)
         kG1[0], kG2[0], ka1[0], ke1[0], kad1[0], kadr[0] = out
         ka2[0], ke2[0], kad2[0] = second_out()
+        kad3[0] = third_out()
         kG2[0][0, 0] = 0.0
         for jl in range(1, s):
             def comb(arr_list, base):
@@ -1142,10 +1222,12 @@ class Oracle:
             a2m = comb(ka2, st.a2nd)
             e2m = comb(ke2, st.eta2nd)
             aad2 = comb(kad2, st.a2nd_add)
+            aad3 = comb(kad3, st.a3rd_add)
             self.time = t_o + h * self.C[jl]
-            out = self.solve_hos2(a1m, e1m, G2m, G1m, aad, h * self.C[jl], a2m, e2m, aad2)
+            out = self.solve_hos2(a1m, e1m, G2m, G1m, aad, h * self.C[jl], a2m, e2m, aad2, aad3)
             kG1[jl], kG2[jl], ka1[jl], ke1[jl], kad1[jl], kadr[jl] = out
             ka2[jl], ke2[jl], kad2[jl] = second_out()
+            kad3[jl] = third_out()
             kG2[jl][0, 0] = 0.0
 
         def bsum(arr_list, w):
@@ -1159,6 +1241,7 @@ class Oracle:
         er1, er2, era = bsum(kG1, self.E), bsum(kG2, self.E), bsum(kad1, self.E)
         dad2, da2, de2 = bsum(kad2, self.B), bsum(ka2, self.B), bsum(ke2, self.B)
         era2 = bsum(kad2, self.E)
+        dad3, era3 = bsum(kad3, self.B), bsum(kad3, self.E)
         G1 = G1 + h * dG1m
         G2 = G2 + h * dG2m
         a_add_1 = st.a1st_add + h * dad1
@@ -1168,13 +1251,15 @@ class Oracle:
         a_add_2 = st.a2nd_add + h * dad2
         a2nd = st.a2nd + h * da2
         eta2nd = st.eta2nd + h * de2
+        a_add_3 = st.a3rd_add + h * dad3
         er1 = np.abs((er1 - dG1m) * h) / (np.max(np.abs(G1)) + TINY)
         er2 = np.abs((er2 - dG2m) * h) / (np.max(np.abs(G2)) + TINY)
         era = np.abs((era - dad1) * h) / (np.max(np.abs(a_add_1)) + TINY)
         era2 = np.abs((era2 - dad2) * h) / (np.max(np.abs(a_add_2)) + TINY)
+        era3 = np.abs((era3 - dad3) * h) / (np.max(np.abs(a_add_3)) + TINY)
         # a_add_3 is identically zero below a third-order wavemaker, and so is a_add_2 below
         # a second-order one: their error terms are 0/(0+tiny) = 0 (runge_kutta.f90:273-279)
-        err = max(er1.max(), er2.max(), era.max(), era2.max())
+        err = max(er1.max(), er2.max(), era.max(), era2.max(), era3.max())
         etark, phisrk = self.gtof(G1, G2, h)
         deta, _ = self.gtof(kG1[s - 1], kG2[s - 1], h)
         om = tb.omega.copy()
@@ -1187,7 +1272,8 @@ class Oracle:
         etark = etark.copy()
         etark[0, 0] = self._volume_mode(modesFS[0, 0])
         self.modesFS = etark.copy()
-        new = State(self.f2s(etark), self.f2s(ph), a_add_1, a_add_r, a1st, eta1st, a_add_2, a2nd, eta2nd)
+        new = State(self.f2s(etark), self.f2s(ph), a_add_1, a_add_r, a1st, eta1st, a_add_2, a2nd, eta2nd,
+                    a_add_3)
         deta = self.f2s(d)
         return new, err, deta
 

@@ -550,3 +550,99 @@ def test_dgzeddz_is_the_derivative_of_gzed(igeom, d):
     for z in np.linspace(-0.999, -0.001, 37):
         fd = (ho.gzed(z + h, igeom, d) - ho.gzed(z - h, igeom, d)) / (2 * h)
         assert abs(ho.dgzeddz(z, igeom, d) - fd) < 1e-6, (z, igeom, d)
+
+
+def _wmk3_params(n2, **kw):
+    rng = np.random.default_rng(41)
+    nf = 2
+    pos = 0.02 * rng.standard_normal((n2, nf)) + 0.02j * rng.standard_normal((n2, nf))
+    dpos = (0.05 * rng.standard_normal((n2, nf)) + 0.05j * rng.standard_normal((n2, nf))) if n2 > 1 else \
+        np.zeros((1, nf), complex)
+    base = dict(n1=14, n2=n2, mHOS=3, xlen=5.0, ylen=3.0, n3=9, icase=3, i_wmk=3, igeom=2, d_hinge=0.3,
+                iramp=2, Tramp=4.0, iabsnb=0, freq=np.array([0.31, 0.47]), pos_ini=pos, dposdy_ini=dpos)
+    base.update(kw)
+    return ho.Params(**base)
+
+
+@pytest.mark.parametrize("n2", [1, 6])
+def test_third_order_first_order_paddle_derivatives_by_direct_sums(n2):
+    """wmk_3rdorder's first-order third derivatives on the paddle (resol_wmkr.f90:982-1012)
+    against direct sums of the additional potential
+    phi = sum a cos(kx_add (z+1)) cosh(k (x - xlen))/cosh(k xlen) cos(ky y)
+    differentiated analytically at x = 0 and evaluated at the physical nodes (no transforms);
+    the z-sine field xz is read at node i3+1 (the DST-II index shift)."""
+    p = _wmk3_params(n2)
+    o = ho.Oracle(p)
+    tb = o.tb
+    rng = np.random.default_rng(43)
+    aad = rng.standard_normal((p.n3_add, n2)) * np.exp(-0.5 * np.arange(p.n3_add))[:, None]
+    z1, za = np.zeros((p.n1, n2)), np.zeros((p.n3_add, n2))
+    f1, _ = o.wmk_3rdorder(z1, z1, aad, za, za, za)
+    kt = np.tanh(tb.k_add * p.xlen)  # cosh'(.)/cosh at x = 0 is -k tanh(k xlen)
+    for i3 in range(p.n3 - 1):
+        z = tb.cz[i3]
+        zs = tb.cz[i3 + 1]
+        for i2 in range(n2):
+            y = tb.cy[i2]
+            xxx = xy = xz = 0.0
+            for m in range(p.n3_add):
+                for j in range(n2):
+                    k, kxa, ky = tb.k_add[m, j], tb.kx_add[m], tb.ky[j]
+                    xxx += aad[m, j] * math.cos(kxa * (z + 1.0)) * (-k ** 3 * kt[m, j]) * math.cos(ky * y)
+                    xy += aad[m, j] * math.cos(kxa * (z + 1.0)) * (-k * kt[m, j]) * (-ky * math.sin(ky * y))
+                    xz += aad[m, j] * (-kxa * math.sin(kxa * (zs + 1.0))) * (-k * kt[m, j]) * math.cos(ky * y)
+            assert abs(f1["xxx"][i3, i2] - xxx) < 1e-9 * max(1.0, abs(xxx)), (i3, i2)
+            assert abs(f1["xy"][i3, i2] - xy) < 1e-9 * max(1.0, abs(xy)), (i3, i2)
+            assert abs(f1["xz"][i3, i2] - xz) < 1e-9 * max(1.0, abs(xz)), (i3, i2)
+
+
+@pytest.mark.parametrize("n2", [1, 6])
+def test_third_order_paddle_condition_is_a_time_derivative(n2):
+    """solveHOS_SWEET_3's paddle condition (resol_wmkr.f90:366-396) is
+    d3 = -d/dt[alpha (-X phi2_xx + X_y phi2_y + X_z phi2_z)
+               + alpha^2 (-X^2/2 phi1_xxx + X X_y phi1_xy + X X_z phi1_xz)]
+    along the solution (a2nd + t da2nd, a_add_2 + t da_add_2, a_add_1 + t da_add_1, X(t)),
+    checked by centred differences in t through wmk_3rdorder at shifted states."""
+    p = _wmk3_params(n2)
+    o = ho.Oracle(p)
+    tb = o.tb
+    rng = np.random.default_rng(44)
+    dec = np.exp(-0.4 * tb.k)
+    a1 = 0.01 * rng.standard_normal((p.n1, n2)) * dec
+    e1 = 0.01 * rng.standard_normal((p.n1, n2)) * dec
+    a2 = 0.004 * rng.standard_normal((p.n1, n2)) * dec
+    e2 = 0.004 * rng.standard_normal((p.n1, n2)) * dec
+    w = np.exp(-0.3 * np.arange(p.n3_add))[:, None]
+    aad = 0.003 * rng.standard_normal((p.n3_add, n2)) * w
+    aad2 = 0.002 * rng.standard_normal((p.n3_add, n2)) * w
+    aad3 = 0.001 * rng.standard_normal((p.n3_add, n2)) * w
+    z1 = np.zeros((p.n1, n2))
+    t0 = 1.3
+    o.time = t0
+    r = o.solve_hos2(a1, e1, z1, z1, aad, 0.01, a2nd=a2, eta2nd=e2, a_add_2=aad2, a_add_3=aad3)
+    d1 = r[4]
+    da2, _, d2 = o.second
+    d3 = o.third
+
+    def bracket(dt):
+        t = t0 + dt
+        al, _, _ = ho.ramp(t, p)
+        X, _, _ = ho.pos_1st(t, p, tb)
+        Xy, _, Xz, _ = ho.dpos_1st(t, p, tb)
+        f1, f2 = o.wmk_3rdorder(a2 + dt * da2, da2, aad + dt * d1, d1, aad2 + dt * d2, d2)
+        return (al * (-X * f2["axx"] + Xy * f2["ay"] + Xz * f2["az"])
+                + al ** 2 * (-(X ** 2) / 2.0 * f1["xxx"] + X * Xy * f1["xy"] + X * Xz * f1["xz"]))
+
+    h = 1e-4
+    fd = np.zeros((p.n3_add, n2))
+    fd[:p.n3] = -(bracket(h) - bracket(-h)) / (2 * h)
+    fd = ho.match_surface_wmk(fd, p.n3, tb.delz)
+    fd = o.FA.space_2_fourier(fd) / tb.k_add_thk_add
+    scale = np.abs(d3).max()
+    assert scale > 1e-7
+    np.testing.assert_allclose(d3, fd, atol=1e-6 * scale)
+
+
+def test_third_order_rejects_filters():
+    with pytest.raises(ValueError):
+        _wmk3_params(1, coeffilt=(0.8, 1.0, 1.0))
