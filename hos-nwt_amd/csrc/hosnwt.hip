@@ -775,6 +775,30 @@ __global__ void k_paddle_prep6(int n2, int n3a, double piy, const double* __rest
   out[5 * na + i] = -da[i] * kxa;
 }
 
+// Third-order wavemaker (wmk_3rdorder, resol_wmkr.f90:982-990): modal
+// coefficients of the first-order additional potential's third derivatives on
+// the paddle, stacked [xxx, xxxt, xy, xyt, xz, xzt] in the kinds of
+// k_paddle_prep6: -a k^2 kt, -da k^2 kt, a ky kt, da ky kt, a kx_add kt,
+// da kx_add kt with kt = k tanh(k xlen).  The free-surface part is a sine series
+// in x and vanishes on the paddle.
+__global__ void k_paddle_prep6_x3(int n2, int n3a, double piy, double xlen, const double* __restrict__ a,
+                                  const double* __restrict__ da, double* out) {
+  const size_t na = (size_t)n3a * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= na) return;
+  const int i3 = (int)(i % n3a), j2 = (int)(i / n3a);
+  const double kxa = kx_add_of(i3), ky = j2 * piy;
+  const double k = sqrt(kxa * kxa + ky * ky);
+  const double kt = k * tanh(k * xlen);
+  const double k2 = k * k;
+  out[i] = -a[i] * k2 * kt;
+  out[na + i] = -da[i] * k2 * kt;
+  out[2 * na + i] = a[i] * ky * kt;
+  out[3 * na + i] = da[i] * ky * kt;
+  out[4 * na + i] = a[i] * kxa * kt;
+  out[5 * na + i] = da[i] * kxa * kt;
+}
+
 // First-order free-surface modes evaluated on the paddle x = 0 at the nodes
 // z(i3) (d1st_for_2nd, resol_wmkr.f90:686-700, tables HOS-NWT.f90:342-355),
 // added to the stacked z-physical / y-modal fields: one workgroup per (i3, j2),
@@ -901,6 +925,39 @@ __global__ void k_paddle_bc2(int n2, int n3, int n3a, double delz, double al, do
   continue_column(col, n3, n3a, delz, cm);
 }
 
+// Third-order paddle condition (solveHOS_SWEET_3, resol_wmkr.f90:366-393), summed
+// in the reference's order alpha + alphat + alpha^2 + alpha alphat:
+//   -alpha (-X_t P2xx - X P2xxt + X_yt P2y + X_y P2yt + X_zt P2z + X_z P2zt)
+//   - alpha' (-X P2xx + X_y P2y + X_z P2z)
+//   - alpha^2 (-X^2/2 P1xxxt - X_t X P1xxx + X_t X_y P1xy + X X_y P1xyt + X X_yt P1xy
+//              + X_t X_z P1xz + X X_z P1xzt + X X_zt P1xz)
+//   - alpha alpha' (-X^2 P1xxx + 2 X X_y P1xy + 2 X X_z P1xz)
+// P1 = first-order third derivatives [xxx, xxxt, xy, xyt, xz, xzt], P2 = second-order
+// fields [xx, xxt, y, yt, z, zt] (n3a x n2 each, rows < n3 used), then the continuation.
+__global__ void k_paddle_bc3(int n2, int n3, int n3a, double delz, double al, double alt, const double* P1,
+                             const double* P2, const double* X, const double* Xt, const double* Xy,
+                             const double* Xyt, const double* Xz, const double* Xzt, Cont cm, double* d3) {
+  const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i2 >= n2) return;
+  const size_t na = (size_t)n3a * n2;
+  double* col = d3 + (size_t)n3a * i2;
+  for (int i3 = 0; i3 < n3; ++i3) {
+    const size_t q = i3 + (size_t)n3a * i2, o = i3 + (size_t)n3 * i2;
+    const double x = X[o], xt = Xt[o], xy = Xy[o], xyt = Xyt[o], xz = Xz[o], xzt = Xzt[o];
+    const double xxx = P1[q], xxxt = P1[na + q], pxy = P1[2 * na + q], pxyt = P1[3 * na + q],
+                 pxz = P1[4 * na + q], pxzt = P1[5 * na + q];
+    const double xx = P2[q], xxt = P2[na + q], py = P2[2 * na + q], pyt = P2[3 * na + q], pz = P2[4 * na + q],
+                 pzt = P2[5 * na + q];
+    const double t_al = -al * (-xt * xx - x * xxt + xyt * py + xy * pyt + xzt * pz + xz * pzt);
+    const double t_alt = -alt * (-x * xx + xy * py + xz * pz);
+    const double t_al2 = -(al * al) * (-(x * x) / 2.0 * xxxt - xt * x * xxx + xt * xy * pxy + x * xy * pxyt +
+                                       x * xyt * pxy + xt * xz * pxz + x * xz * pxzt + x * xzt * pxz);
+    const double t_alalt = -al * alt * (-(x * x) * xxx + 2.0 * x * xy * pxy + 2.0 * x * xz * pxz);
+    col[i3] = t_al + t_alt + t_al2 + t_alalt;
+  }
+  continue_column(col, n3, n3a, delz, cm);
+}
+
 // Second-order free-surface conditions (solveHOS_SWEET_2 :302-318):
 //   deta2 = phiz2nd - phix1st etax1st - phiy1st etay1st + eta1st phizz1st
 //   tmp   = -eta2nd - phit_add_2 - (phix1st^2 + phiy1st^2 + phiz1st^2)/2
@@ -1016,13 +1073,17 @@ __global__ void k_step_begin(int n1, int n2, double pix, double piy, const doubl
   G2[i] = phis_hat[i] * w;
 }
 
-// a = a1 + a2, d = d1 + d2 (total additional-potential modes, resol_HOS.f90:498-499)
-__global__ void k_sum2(size_t n, const double* a1, const double* a2, const double* d1, const double* d2, double* a,
-                       double* d) {
+// a = a1 + a2 (+ a3), d = d1 + d2 (+ d3): total additional-potential modes
+// (resol_HOS.f90:498-499); a3, d3 null below a third-order wavemaker
+__global__ void k_sum_orders3(size_t n, const double* a1, const double* a2, const double* a3, const double* d1,
+                              const double* d2, const double* d3, double* a, double* d) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  a[i] = a1[i] + a2[i];
-  d[i] = d1[i] + d2[i];
+  double av = a1[i] + a2[i], dv = d1[i] + d2[i];
+  if (a3) av = av + a3[i];
+  if (d3) dv = dv + d3[i];
+  a[i] = av;
+  d[i] = dv;
 }
 
 __global__ void k_mulw(size_t n, const double* __restrict__ w, double* __restrict__ x) {
@@ -1435,6 +1496,8 @@ struct EngineBase {
   struct S2 {
     const double *a2, *e2, *aad2;  // a2nd (modal), eta2nd (physical), a2nd_add (modal)
     double *ka2, *ke2, *kad2;      // their derivatives
+    const double* aad3;            // a3rd_add (modal), null below i_wmk = 3
+    double* kad3;                  // its derivative
   };
   virtual void rhs(double t, double dt, const double* G1, const double* G2, const double* a1, const double* e1,
                    const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
@@ -1471,6 +1534,9 @@ struct hos_ctx {
   double *a2nd_add_t = nullptr, *a2nd_t = nullptr, *eta2nd_t = nullptr;
   double *ka2[5] = {}, *ke2[5] = {}, *kad2[5] = {};
   double *a2m = nullptr, *e2m = nullptr, *aad2m = nullptr;
+  // third-order wavemaker paddle modes (i_wmk = 3; null otherwise)
+  bool wmk3 = false;
+  double *a3rd_add = nullptr, *a3rd_add_t = nullptr, *aad3m = nullptr, *kad3[5] = {};
   double *partial, *pmax_e, *pmax_x, *red6;
   double *phys_tmp, *nu_d;
   double* filt_fs = nullptr;  // free-surface mode filter weights (n1*n2), null when off
@@ -1550,7 +1616,7 @@ struct Engine : EngineBase {
   // fields, inverse paddle-grid matrices, z = 0 sums and first-order fields
   double *dyre = nullptr, *dyim = nullptr, *dgz = nullptr, *nux = nullptr, *nuy = nullptr;
   double *Xdy = nullptr, *Xdyt = nullptr, *Xdz = nullptr, *Xdzt = nullptr;
-  double *stk6 = nullptr, *stkz = nullptr, *Pw = nullptr;
+  double *stk6 = nullptr, *stkz = nullptr, *Pw = nullptr, *Pw3 = nullptr;
   double *Cz = nullptr, *Sz = nullptr, *Cy = nullptr, *Sy = nullptr;
   double *part2 = nullptr, *zf[11] = {};
   double *modes8 = nullptr, *T8 = nullptr, *a1f = nullptr;
@@ -1680,6 +1746,7 @@ struct Engine : EngineBase {
       modes8 = ar.get<double>(8 * ns); T8 = ar.get<double>(8 * ns); a1f = ar.get<double>(ns);
       phit2 = ar.get<double>(ns); phiz2 = ar.get<double>(ns); w2 = ar.get<double>(ns); tmp2 = ar.get<double>(ns);
       atot = ar.get<double>(na); dtot = ar.get<double>(na);
+      if (c->wmk3) Pw3 = ar.get<double>(6 * na);
       (void)n1;
     }
     // reference extended grid (common_vars.f90:38-39) for the exact mixed-parity filter
@@ -1963,6 +2030,54 @@ struct Engine : EngineBase {
     }
   }
 
+  // Stacked paddle-grid modes stk6 [xx, xxt, y, yt, z, zt] (kinds cos-cos x2,
+  // cos-sin x2, sin-cos x2) to their values at the paddle nodes (IDCT_FFTW_add,
+  // fourier_FFTW.f90:451-492) in out; when a_fs is given, the free-surface
+  // modes a_fs, da_fs evaluated at x = 0 are added before the y transform.
+  void paddle_fields(const double* a_fs, const double* da_fs, double* out) {
+    hos_ctx& C = *c;
+    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a;
+    const double one = 1.0, zero = 0.0;
+    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 4 * n2, n3a, &one, Cz, n3a, stk6,
+                          n3a, &zero, stkz, n3a),
+            "dgemm paddle Cz");
+    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 2 * n2, n3a, &one, Sz, n3a,
+                          stk6 + 4 * na, n3a, &zero, stkz + 4 * na, n3a),
+            "dgemm paddle Sz");
+    if (a_fs)
+      hipLaunchKernelGGL(k_fs_on_paddle, dim3(n3, n2), dim3(256), 0, cs, n1, n2, n3, n3a, C.delz, C.pix, C.piy,
+                         a_fs, da_fs, stkz);
+    if (n2 > 1) {
+      const double* My[3] = {Cy, Sy, Cy};
+      for (int g = 0; g < 3; ++g)
+        blas_ok(rocblas_dgemm_strided_batched(cb, rocblas_operation_none, rocblas_operation_transpose, n3a, n2, n2,
+                                              &one, stkz + 2 * g * na, n3a, (rocblas_stride)na, My[g], n2, 0, &zero,
+                                              out + 2 * g * na, n3a, (rocblas_stride)na, 2),
+                "dgemm paddle y");
+    } else {
+      HIP_OK(hipMemcpyAsync(out, stkz, 6 * na * sizeof(double), hipMemcpyDeviceToDevice, cs));
+    }
+    HIP_OK(hipGetLastError());
+  }
+
+  // Third-order paddle condition and modes (solveHOS_SWEET_3 with wmk_3rdorder,
+  // resol_wmkr.f90:338-437, :942-1115); the paddle motion X.., its derivatives
+  // and the second-order derivatives q.ka2, q.kad2 of this stage are in place.
+  void third_order_paddle(double al, double alt, const double* aad, const double* kad1, const S2& q) {
+    hos_ctx& C = *c;
+    const int n2 = C.n2, n3 = C.n3, n3a = C.n3a;
+    hipLaunchKernelGGL(k_paddle_prep6_x3, dim3(nblk(na)), dim3(256), 0, cs, n2, n3a, C.piy, C.p.xlen, aad, kad1,
+                       stk6);
+    paddle_fields(nullptr, nullptr, Pw3);
+    hipLaunchKernelGGL(k_paddle_prep6, dim3(nblk(na)), dim3(256), 0, cs, n2, n3a, C.piy, q.aad2,
+                       (const double*)q.kad2, stk6);
+    paddle_fields(q.a2, q.ka2, Pw);
+    hipLaunchKernelGGL(k_paddle_bc3, dim3(nblk(n2, 64)), dim3(64), 0, cs, n2, n3, n3a, C.delz, al, alt,
+                       (const double*)Pw3, (const double*)Pw, (const double*)Xp, (const double*)Xpt,
+                       (const double*)Xdy, (const double*)Xdyt, (const double*)Xdz, (const double*)Xdzt, cont, f1);
+    paddle_modes(1, q.kad3, (double*)nullptr);
+  }
+
   // z = 0 wavemaker sums of the SET 0 (11 fields) or SET 1 (2 fields) kind.
   template <int SET>
   void phiadd_z0(const double* a, const double* da, Ptr11 outp) {
@@ -2018,27 +2133,9 @@ struct Engine : EngineBase {
     }
     // first-order potential derivatives on the paddle: additional-potential part
     // (inverse paddle-grid transforms as dense products) plus the free-surface modes
-    const double one = 1.0, zero = 0.0;
     hipLaunchKernelGGL(k_paddle_prep6, dim3(nblk(na)), dim3(256), 0, st, n2, n3a, C.piy, aad, (const double*)kad1,
                        stk6);
-    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 4 * n2, n3a, &one, Cz, n3a, stk6,
-                          n3a, &zero, stkz, n3a),
-            "dgemm paddle Cz");
-    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 2 * n2, n3a, &one, Sz, n3a,
-                          stk6 + 4 * na, n3a, &zero, stkz + 4 * na, n3a),
-            "dgemm paddle Sz");
-    hipLaunchKernelGGL(k_fs_on_paddle, dim3(n3, n2), dim3(256), 0, st, n1, n2, n3, n3a, C.delz, C.pix, C.piy, a1u,
-                       (const double*)ka1, stkz);
-    if (n2 > 1) {
-      const double* My[3] = {Cy, Sy, Cy};
-      for (int g = 0; g < 3; ++g)
-        blas_ok(rocblas_dgemm_strided_batched(cb, rocblas_operation_none, rocblas_operation_transpose, n3a, n2, n2,
-                                              &one, stkz + 2 * g * na, n3a, (rocblas_stride)na, My[g], n2, 0, &zero,
-                                              Pw + 2 * g * na, n3a, (rocblas_stride)na, 2),
-                "dgemm paddle y");
-    } else {
-      HIP_OK(hipMemcpyAsync(Pw, stkz, 6 * na * sizeof(double), hipMemcpyDeviceToDevice, st));
-    }
+    paddle_fields(a1u, ka1, Pw);
     // first-order free-surface fields
     hipLaunchKernelGGL(k_fs_modes8, dim3(nblk(ns)), dim3(256), 0, st, n1, n2, C.pix, C.piy, a1u, (const double*)ka1,
                        modes8);
@@ -2073,9 +2170,11 @@ struct Engine : EngineBase {
                        (const double*)phit2, (const double*)T8, (const double*)ke1, e1, q.e2, (const double*)C.nu_d,
                        (double)C.p.iabsnb, q.ke2, tmp2);
     base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp2, n1, n1, n2), n1, n2, Dst<double>{q.ka2, n1, 0});
+    // --- SWEET 3: third-order paddle modes
+    if (q.aad3) third_order_paddle(al, alt, aad, kad1, q);
     // --- additional potential of the total paddle modes on the free surface
-    hipLaunchKernelGGL(k_sum2, dim3(nblk(na)), dim3(256), 0, st, na, aad, q.aad2, kad1, (const double*)q.kad2, atot,
-                       dtot);
+    hipLaunchKernelGGL(k_sum_orders3, dim3(nblk(na)), dim3(256), 0, st, na, aad, q.aad2, q.aad3, kad1,
+                       (const double*)q.kad2, (const double*)q.kad3, atot, dtot);
     phiadd(atot, dtot, eta_phys);
     // phit1 / phiz1 were overwritten by the surface sums of the totals; the first-order
     // conditions above have used the first-order values already.
@@ -2209,8 +2308,13 @@ void check_params(const hos_params& p) {
   const int p2ok = (p.n2 == 1) ? 1 : p.mHOS;
   if (p.p1 != p.mHOS || (p.p2 != p2ok && !(p.n2 == 1 && p.p2 == p.mHOS)))
     throw std::runtime_error("supported dealiasing: p1 = mHOS, p2 = mHOS (3-D) or 1 (2-D)");
-  if (p.i_wmk != 1 && p.i_wmk != 2 && (p.icase != 1))
-    throw std::runtime_error("supported wavemaker order: i_wmk = 1 or 2");
+  if (p.i_wmk != 1 && p.i_wmk != 2 && p.i_wmk != 3 && (p.icase != 1))
+    throw std::runtime_error("supported wavemaker order: i_wmk = 1, 2 or 3");
+  // wmk_3rdorder filters its INTENT(IN) stage inputs in place (resol_wmkr.f90:1019-1030),
+  // which the Fortran standard leaves undefined for the caller's arrays
+  if (p.i_wmk == 3 && p.icase != 1 &&
+      (filter_active(p.coeffilt[0]) || (p.n2 > 1 && filter_active(p.coeffilt[1])) || filter_active(p.coeffilt[2])))
+    throw std::runtime_error("i_wmk = 3 is supported without filters (coeffilt = 1)");
   if (p.precision != 0 && p.precision != 1) throw std::runtime_error("precision must be 0 or 1");
   if (p.xlen <= 0.0 || (p.n2 > 1 && p.ylen <= 0.0)) throw std::runtime_error("tank size must be positive");
   if (p.n_freq < 0 || (p.n_freq > 0 && (!p.freq || !p.pos_ini_re || !p.pos_ini_im)))
@@ -2228,9 +2332,11 @@ double stage_time(double t, double h, double cs) {
 
 void rhs_eval(hos_ctx* c, double t, double dt, const double* G1, const double* G2, const double* a1,
               const double* e1, const double* aad, int s, const double* a2 = nullptr, const double* e2 = nullptr,
-              const double* aad2 = nullptr) {
-  if (c->wmk2 && !(a2 && e2 && aad2)) throw std::runtime_error("i_wmk = 2 needs the second-order state");
-  EngineBase::S2 s2{a2, e2, aad2, c->ka2[s], c->ke2[s], c->kad2[s]};
+              const double* aad2 = nullptr, const double* aad3 = nullptr) {
+  if (c->wmk2 && !(a2 && e2 && aad2)) throw std::runtime_error("i_wmk >= 2 needs the second-order state");
+  if (c->wmk3 && !aad3) throw std::runtime_error("i_wmk = 3 needs the third-order paddle modes");
+  EngineBase::S2 s2{a2, e2, aad2, c->ka2[s], c->ke2[s], c->kad2[s], c->wmk3 ? aad3 : nullptr,
+                    c->wmk3 ? c->kad3[s] : nullptr};
   c->eng->rhs(t, dt, G1, G2, a1, e1, aad, c->kG1[s], c->kG2[s], c->ka1[s], c->ke1[s], c->kad1[s], c->kadr[s],
               c->wmk2 ? &s2 : nullptr);
 }
@@ -2240,7 +2346,8 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   hipStream_t st = c->st;
   hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
                      (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
-  rhs_eval(c, t, h * c->C[0], c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add);
+  rhs_eval(c, t, h * c->C[0], c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add,
+           c->a3rd_add);
   for (int jl = 1; jl < 5; ++jl) {
     auto comb = [&](double* out, const double* base, double* const* karr, size_t n) {
       KSet k{};
@@ -2261,8 +2368,9 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
       comb(c->e2m, c->eta2nd, c->ke2, ns);
       comb(c->aad2m, c->a2nd_add, c->kad2, na);
     }
+    if (c->wmk3) comb(c->aad3m, c->a3rd_add, c->kad3, na);
     rhs_eval(c, stage_time(t, h, c->C[jl]), h * c->C[jl], c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, jl, c->a2m,
-             c->e2m, c->aad2m);
+             c->e2m, c->aad2m, c->aad3m);
   }
   // final combination and embedded error (runge_kutta.f90:225-279)
   auto kset = [&](double* const* karr, const double* w) {
@@ -2306,6 +2414,13 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
     hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->eta2nd_t, (const double*)c->eta2nd, h,
                        kset(c->ke2, c->B));
   }
+  if (c->wmk3) {
+    // third-order paddle modes and their error term (runge_kutta.f90:276)
+    hipLaunchKernelGGL(k_final_err, dim3(nba), dim3(256), 0, st, na, c->a3rd_add_t, (const double*)c->a3rd_add, h,
+                       kset(c->kad3, c->B), kset(c->kad3, c->E), c->pmax_e, c->pmax_x);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_e, c->red6 + 8);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_x, c->red6 + 9);
+  }
   // GtoF back, deta, volume fix with the last stage's wavemaker globals
   hipLaunchKernelGGL(k_step_end, dim3(nbs), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy, h, (const double*)Gn1,
                      (const double*)Gn2, (const double*)c->kG1[4], (const double*)c->kG2[4],
@@ -2314,8 +2429,8 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   hipLaunchKernelGGL(k_volfix<double>, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->partial,
                      (double*)nullptr, c->eta_hat_t, c->volmode(), c->last_alpha, c->last_pos_top, c->p.xlen,
                      c->p.d_hinge);
-  double r[8];
-  const int nr = c->wmk2 ? 8 : 6;
+  double r[10];
+  const int nr = c->wmk3 ? 10 : (c->wmk2 ? 8 : 6);
   HIP_OK(hipMemcpyAsync(r, c->red6, nr * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
   const double tiny = 2.220446049250313e-16;
@@ -2323,6 +2438,7 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   e = std::max(e, r[2] / (r[3] + tiny));
   e = std::max(e, r[4] / (r[5] + tiny));
   if (c->wmk2) e = std::max(e, r[6] / (r[7] + tiny));
+  if (c->wmk3) e = std::max(e, r[8] / (r[9] + tiny));
   *err = e;
   c->have_trial = true;
 }
@@ -2341,6 +2457,7 @@ void commit(hos_ctx* c) {
     std::swap(c->a2nd, c->a2nd_t);
     std::swap(c->eta2nd, c->eta2nd_t);
   }
+  if (c->wmk3) std::swap(c->a3rd_add, c->a3rd_add_t);
   c->have_trial = false;
 }
 
@@ -2360,6 +2477,45 @@ int guard(F&& f) {
 // ===========================================================================
 // C-ABI
 // ===========================================================================
+namespace {
+// hos_rhs2 / hos_rhs3: one solveHOS2 with the higher-order wavemaker state
+void rhs_higher(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
+                const double* eta1st, const double* a_add_1, const double* a2nd, const double* eta2nd,
+                const double* a_add_2, const double* a_add_3, double* dG1, double* dG2, double* da1st,
+                double* deta1st, double* da_add_1, double* da_add_ramp, double* da2nd, double* deta2nd,
+                double* da_add_2, double* da_add_3) {
+  const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+  hipStream_t st = c->st;
+  HIP_OK(hipMemcpyAsync(c->G1m, G1, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->G2m, G2, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->a1m, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->e1m, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->aadm, a_add_1, na * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->a2m, a2nd, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->e2m, eta2nd, ns * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->aad2m, a_add_2, na * sizeof(double), hipMemcpyHostToDevice, st));
+  if (c->wmk3) {
+    if (a_add_3)
+      HIP_OK(hipMemcpyAsync(c->aad3m, a_add_3, na * sizeof(double), hipMemcpyHostToDevice, st));
+    else
+      HIP_OK(hipMemsetAsync(c->aad3m, 0, na * sizeof(double), st));
+  }
+  rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0, c->a2m, c->e2m, c->aad2m, c->aad3m);
+  HIP_OK(hipMemcpyAsync(dG1, c->kG1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(dG2, c->kG2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (da1st) HIP_OK(hipMemcpyAsync(da1st, c->ka1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (deta1st) HIP_OK(hipMemcpyAsync(deta1st, c->ke1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (da_add_1) HIP_OK(hipMemcpyAsync(da_add_1, c->kad1[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (da_add_ramp) HIP_OK(hipMemcpyAsync(da_add_ramp, c->kadr[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (da2nd) HIP_OK(hipMemcpyAsync(da2nd, c->ka2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (deta2nd) HIP_OK(hipMemcpyAsync(deta2nd, c->ke2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (da_add_2) HIP_OK(hipMemcpyAsync(da_add_2, c->kad2[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (da_add_3 && c->wmk3)
+    HIP_OK(hipMemcpyAsync(da_add_3, c->kad3[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+}
+}  // namespace
+
 extern "C" {
 
 const char* hos_last_error(void) { return g_last_error.c_str(); }
@@ -2411,7 +2567,8 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
       c->kad1[s] = ar.get<double>(na); c->kadr[s] = ar.get<double>(na);
       c->ka1[s] = ar.get<double>(ns); c->ke1[s] = ar.get<double>(ns);
     }
-    c->wmk2 = c->wmk && p.i_wmk == 2;
+    c->wmk2 = c->wmk && p.i_wmk >= 2;
+    c->wmk3 = c->wmk && p.i_wmk == 3;
     if (c->wmk2) {
       c->a2nd_add = ar.get<double>(na); c->a2nd = ar.get<double>(ns); c->eta2nd = ar.get<double>(ns);
       c->a2nd_add_t = ar.get<double>(na); c->a2nd_t = ar.get<double>(ns); c->eta2nd_t = ar.get<double>(ns);
@@ -2420,11 +2577,15 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
         c->ka2[s] = ar.get<double>(ns); c->ke2[s] = ar.get<double>(ns); c->kad2[s] = ar.get<double>(na);
       }
     }
+    if (c->wmk3) {
+      c->a3rd_add = ar.get<double>(na); c->a3rd_add_t = ar.get<double>(na); c->aad3m = ar.get<double>(na);
+      for (int s = 0; s < 5; ++s) c->kad3[s] = ar.get<double>(na);
+    }
     const size_t nbk = nblk(std::max(ns, na)) + 1;
     c->partial = ar.get<double>(nbk);
     c->pmax_e = ar.get<double>(nbk);
     c->pmax_x = ar.get<double>(nbk);
-    c->red6 = ar.get<double>(8);
+    c->red6 = ar.get<double>(10);
     c->phys_tmp = ar.get<double>(ns);
     c->phi_add_last = ar.get<double>(ns);
     c->nu_d = nullptr;
@@ -2489,6 +2650,7 @@ int hos_initialize(hos_ctx* c) {
       HIP_OK(hipMemsetAsync(c->eta2nd, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(c->a2nd_add, 0, na * sizeof(double), st));
     }
+    if (c->wmk3) HIP_OK(hipMemsetAsync(c->a3rd_add, 0, na * sizeof(double), st));  // :376
     c->endramp = 0;
     if (!c->wmk) {
       c->last_alpha = 0.0;
@@ -2515,7 +2677,8 @@ int hos_initialize(hos_ctx* c) {
     // one RHS at t = 0 sets phi_add (phiadd_SL with da1st_add) and the wavemaker globals
     hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
                        (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
-    rhs_eval(c, 0.0, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add);
+    rhs_eval(c, 0.0, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add,
+             c->a3rd_add);
     HIP_OK(hipStreamSynchronize(st));
   });
 }
@@ -2565,7 +2728,7 @@ int hos_get_state(hos_ctx* c, double* eta, double* phis, double* a1st_add, doubl
 
 int hos_set_state2(hos_ctx* c, const double* a2nd_add, const double* a2nd, const double* eta2nd) {
   return guard([&] {
-    if (!c->wmk2) throw std::runtime_error("no second-order wavemaker state (i_wmk = 2 only)");
+    if (!c->wmk2) throw std::runtime_error("no second-order wavemaker state (i_wmk = 2, 3 only)");
     const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
     if (a2nd_add) HIP_OK(hipMemcpyAsync(c->a2nd_add, a2nd_add, na * sizeof(double), hipMemcpyHostToDevice, c->st));
     if (a2nd) HIP_OK(hipMemcpyAsync(c->a2nd, a2nd, ns * sizeof(double), hipMemcpyHostToDevice, c->st));
@@ -2577,7 +2740,7 @@ int hos_set_state2(hos_ctx* c, const double* a2nd_add, const double* a2nd, const
 
 int hos_get_state2(hos_ctx* c, double* a2nd_add, double* a2nd, double* eta2nd) {
   return guard([&] {
-    if (!c->wmk2) throw std::runtime_error("no second-order wavemaker state (i_wmk = 2 only)");
+    if (!c->wmk2) throw std::runtime_error("no second-order wavemaker state (i_wmk = 2, 3 only)");
     const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
     HIP_OK(hipStreamSynchronize(c->st));
     if (a2nd_add) HIP_OK(hipMemcpy(a2nd_add, c->a2nd_add, na * sizeof(double), hipMemcpyDeviceToHost));
@@ -2667,11 +2830,12 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
     HIP_OK(hipMemcpyAsync(c->a1m, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(c->e1m, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(c->aadm, a_add_1, na * sizeof(double), hipMemcpyHostToDevice, st));
-    if (c->wmk2) {  // the second-order state taken as zero
+    if (c->wmk2) {  // the higher-order state taken as zero
       HIP_OK(hipMemsetAsync(c->a2m, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(c->e2m, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(c->aad2m, 0, na * sizeof(double), st));
-      rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0, c->a2m, c->e2m, c->aad2m);
+      if (c->wmk3) HIP_OK(hipMemsetAsync(c->aad3m, 0, na * sizeof(double), st));
+      rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0, c->a2m, c->e2m, c->aad2m, c->aad3m);
     } else {
       rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0);
     }
@@ -2690,28 +2854,40 @@ int hos_rhs2(hos_ctx* c, double t, double delta_t, const double* G1, const doubl
              const double* a_add_2, double* dG1, double* dG2, double* da1st, double* deta1st, double* da_add_1,
              double* da_add_ramp, double* da2nd, double* deta2nd, double* da_add_2) {
   return guard([&] {
-    if (!c->wmk2) throw std::runtime_error("hos_rhs2 needs the second-order wavemaker (i_wmk = 2)");
-    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
-    hipStream_t st = c->st;
-    HIP_OK(hipMemcpyAsync(c->G1m, G1, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->G2m, G2, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->a1m, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->e1m, eta1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->aadm, a_add_1, na * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->a2m, a2nd, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->e2m, eta2nd, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(c->aad2m, a_add_2, na * sizeof(double), hipMemcpyHostToDevice, st));
-    rhs_eval(c, t, delta_t, c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, 0, c->a2m, c->e2m, c->aad2m);
-    HIP_OK(hipMemcpyAsync(dG1, c->kG1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipMemcpyAsync(dG2, c->kG2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (da1st) HIP_OK(hipMemcpyAsync(da1st, c->ka1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (deta1st) HIP_OK(hipMemcpyAsync(deta1st, c->ke1[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (da_add_1) HIP_OK(hipMemcpyAsync(da_add_1, c->kad1[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (da_add_ramp) HIP_OK(hipMemcpyAsync(da_add_ramp, c->kadr[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (da2nd) HIP_OK(hipMemcpyAsync(da2nd, c->ka2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (deta2nd) HIP_OK(hipMemcpyAsync(deta2nd, c->ke2[0], ns * sizeof(double), hipMemcpyDeviceToHost, st));
-    if (da_add_2) HIP_OK(hipMemcpyAsync(da_add_2, c->kad2[0], na * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
+    if (!c->wmk2) throw std::runtime_error("hos_rhs2 needs the second-order wavemaker (i_wmk >= 2)");
+    rhs_higher(c, t, delta_t, G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2, nullptr, dG1, dG2, da1st,
+               deta1st, da_add_1, da_add_ramp, da2nd, deta2nd, da_add_2, nullptr);
+  });
+}
+
+int hos_rhs3(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
+             const double* eta1st, const double* a_add_1, const double* a2nd, const double* eta2nd,
+             const double* a_add_2, const double* a_add_3, double* dG1, double* dG2, double* da1st,
+             double* deta1st, double* da_add_1, double* da_add_ramp, double* da2nd, double* deta2nd,
+             double* da_add_2, double* da_add_3) {
+  return guard([&] {
+    if (!c->wmk3) throw std::runtime_error("hos_rhs3 needs the third-order wavemaker (i_wmk = 3)");
+    rhs_higher(c, t, delta_t, G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2, a_add_3, dG1, dG2, da1st,
+               deta1st, da_add_1, da_add_ramp, da2nd, deta2nd, da_add_2, da_add_3);
+  });
+}
+
+int hos_set_state3(hos_ctx* c, const double* a3rd_add) {
+  return guard([&] {
+    if (!c->wmk3) throw std::runtime_error("no third-order wavemaker state (i_wmk = 3 only)");
+    const size_t na = (size_t)c->n3a * c->n2;
+    HIP_OK(hipMemcpyAsync(c->a3rd_add, a3rd_add, na * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    c->have_trial = false;
+  });
+}
+
+int hos_get_state3(hos_ctx* c, double* a3rd_add) {
+  return guard([&] {
+    if (!c->wmk3) throw std::runtime_error("no third-order wavemaker state (i_wmk = 3 only)");
+    const size_t na = (size_t)c->n3a * c->n2;
+    HIP_OK(hipStreamSynchronize(c->st));
+    HIP_OK(hipMemcpy(a3rd_add, c->a3rd_add, na * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 

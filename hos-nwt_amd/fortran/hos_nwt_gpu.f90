@@ -68,7 +68,7 @@ IF (icase == 2 .AND. ibat /= 2) ERROR STOP 'only the front wave (ibat = 2) is bu
 islosh = islosh + 1
 IF (n2 == 1) theta_mono = 0.0_rp
 IF (iramp == 0) Tramp = 0.0_rp
-IF (i_wmk /= 1 .AND. i_wmk /= 2) ERROR STOP 'i_wmk = 1 or 2 is built here'
+IF (i_wmk < 1 .OR. i_wmk > 3) ERROR STOP 'i_wmk = 1, 2 or 3 is built here'
 !
 ! nondimensionalisation (HOS-NWT.f90:99-119)
 Tscale = 1.0_rp / SQRT(G / h)

@@ -59,6 +59,17 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: a2nd_add(*), a2nd(*), eta2nd(*)
    END FUNCTION hos_get_state2
+   ! third-order wavemaker paddle modes (i_wmk = 3): a3rd_add (m3_add,m2)
+   INTEGER(C_INT) FUNCTION hos_set_state3(ctx, a3rd_add) BIND(C, NAME='hos_set_state3')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: a3rd_add(*)
+   END FUNCTION hos_set_state3
+   INTEGER(C_INT) FUNCTION hos_get_state3(ctx, a3rd_add) BIND(C, NAME='hos_get_state3')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: a3rd_add(*)
+   END FUNCTION hos_get_state3
    ! one solveHOS2 evaluation (resol_HOS.f90:270-656)
    INTEGER(C_INT) FUNCTION hos_rhs(ctx, t, delta_t, G1, G2, a1st, eta1st, a_add_1, dG1, dG2, da1st, deta1st, &
         da_add_1, da_add_ramp) BIND(C, NAME='hos_rhs')
@@ -77,6 +88,17 @@ INTERFACE
       REAL(C_DOUBLE), INTENT(OUT) :: dG1(*), dG2(*), da1st(*), deta1st(*), da_add_1(*), da_add_ramp(*)
       REAL(C_DOUBLE), INTENT(OUT) :: da2nd(*), deta2nd(*), da_add_2(*)
    END FUNCTION hos_rhs2
+   INTEGER(C_INT) FUNCTION hos_rhs3(ctx, t, delta_t, G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2, &
+        a_add_3, dG1, dG2, da1st, deta1st, da_add_1, da_add_ramp, da2nd, deta2nd, da_add_2, da_add_3) &
+        BIND(C, NAME='hos_rhs3')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: t, delta_t
+      REAL(C_DOUBLE), INTENT(IN)  :: G1(*), G2(*), a1st(*), eta1st(*), a_add_1(*), a2nd(*), eta2nd(*), a_add_2(*)
+      REAL(C_DOUBLE), INTENT(IN)  :: a_add_3(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: dG1(*), dG2(*), da1st(*), deta1st(*), da_add_1(*), da_add_ramp(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: da2nd(*), deta2nd(*), da_add_2(*), da_add_3(*)
+   END FUNCTION hos_rhs3
    INTEGER(C_INT) FUNCTION hos_rk_step(ctx, t, h, err) BIND(C, NAME='hos_rk_step')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx

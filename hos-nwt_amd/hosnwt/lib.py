@@ -43,7 +43,7 @@ EXPORTS = [
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
     "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
     "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_check_slope",
-    "hos_set_state2", "hos_get_state2", "hos_rhs2",
+    "hos_set_state2", "hos_get_state2", "hos_rhs2", "hos_set_state3", "hos_get_state3", "hos_rhs3",
 ]
 
 _lib = None
@@ -73,6 +73,9 @@ def load() -> C.CDLL:
     lib.hos_rhs2.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 17
     lib.hos_set_state2.argtypes = [vp] + [_dp] * 3
     lib.hos_get_state2.argtypes = [vp] + [_dp] * 3
+    lib.hos_rhs3.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 19
+    lib.hos_set_state3.argtypes = [vp, _dp]
+    lib.hos_get_state3.argtypes = [vp, _dp]
     lib.hos_get_physical.argtypes = [vp] + [_dp] * 4
     lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
     lib.hos_check_slope.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]

@@ -102,6 +102,15 @@ class Tank:
         check(self.lib.hos_get_state2(self.h, *[ptr(a) for a in out]))
         return dict(zip(("a2nd_add", "a2nd", "eta2nd"), out))
 
+    def set_state3(self, a3rd_add):
+        """Third-order wavemaker paddle modes (i_wmk = 3)."""
+        check(self.lib.hos_set_state3(self.h, ptr(fortran(a3rd_add))))
+
+    def get_state3(self):
+        out = np.zeros((self.n3a, self.n2), order="F")
+        check(self.lib.hos_get_state3(self.h, ptr(out)))
+        return out
+
     def physical(self):
         n1, n2 = self.n1, self.n2
         out = [np.zeros((n1, n2), order="F") for _ in range(4)]
@@ -137,6 +146,16 @@ class Tank:
         check(self.lib.hos_rhs2(self.h, t, delta_t, *[ptr(a) for a in ins], *[ptr(a) for a in outs]))
         return dict(zip(("dG1", "dG2", "da1st", "deta1st", "da_add_1", "da_add_ramp", "da2nd", "deta2nd",
                          "da_add_2"), outs))
+
+    def solve_hos2_third(self, t, delta_t, G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2, a_add_3):
+        """One RHS with the second- and third-order wavemaker state (i_wmk = 3)."""
+        n1, n2, na = self.n1, self.n2, self.n3a
+        ins = [fortran(x) for x in (G1, G2, a1st, eta1st, a_add_1, a2nd, eta2nd, a_add_2, a_add_3)]
+        outs = ([np.zeros((n1, n2), order="F") for _ in range(4)] + [np.zeros((na, n2), order="F") for _ in range(2)]
+                + [np.zeros((n1, n2), order="F") for _ in range(2)] + [np.zeros((na, n2), order="F") for _ in range(2)])
+        check(self.lib.hos_rhs3(self.h, t, delta_t, *[ptr(a) for a in ins], *[ptr(a) for a in outs]))
+        return dict(zip(("dG1", "dG2", "da1st", "deta1st", "da_add_1", "da_add_ramp", "da2nd", "deta2nd",
+                         "da_add_2", "da_add_3"), outs))
 
     def advance(self, t_next: float, h_rk: float):
         """Adaptive loop to t_next (HOS-NWT.f90:498-571); returns the next step proposal."""

@@ -53,7 +53,7 @@ typedef struct hos_params {
   int p1, p2;         /* dealiasing parameters; supported: p1 = M, p2 = M (3-D) or 1 */
   int icase;          /* 1 sloshing (no wavemaker), 2/3/31/32/33/4/41 wavemaker */
   int igeom;          /* 1 piston, 2 hinged flap */
-  int i_wmk;          /* wavemaker nonlinear order; supported: 1, 2 */
+  int i_wmk;          /* wavemaker nonlinear order; supported: 1, 2, 3 (3 without filters) */
   int iramp;          /* time ramp type (resol_wmkr.f90:1268-1397) */
   int iabsnb;         /* numerical beach on/off */
   int precision;      /* 0: FP64 everywhere; 1: FP32 transforms and products */
@@ -71,7 +71,7 @@ typedef struct hos_params {
    * the shaped filters of filtering_base, other negatives the exponential
    * cut-off.  0 (a zero-initialised struct) also means off. */
   double coeffilt[3];
-  /* Second-order wavemaker (i_wmk = 2) inputs, may be NULL (zero):
+  /* Second- and third-order wavemaker (i_wmk >= 2) inputs, may be NULL (zero):
    * dposdy_ini(i2,f), n2*n_freq (wavemaking.f90:147-149), and the beach
    * derivatives d nu/dx, d nu/dy, n1*n2, as initial_condition.f90:768-787
    * builds them. */
@@ -115,11 +115,16 @@ int hos_end_ramp(hos_ctx* c);
 int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
                 double delt_out, int* n_acc, int* n_rej);
 
-/* Second-order wavemaker state (i_wmk = 2): a2nd_add (n3_add*n2, modal),
+/* Second-order wavemaker state (i_wmk >= 2): a2nd_add (n3_add*n2, modal),
  * a2nd (n1*n2, modal), eta2nd (n1*n2, physical); the driver's arrays of
  * HOS-NWT.f90:508-516.  Error if the context has no second-order wavemaker. */
 int hos_set_state2(hos_ctx* c, const double* a2nd_add, const double* a2nd, const double* eta2nd);
 int hos_get_state2(hos_ctx* c, double* a2nd_add, double* a2nd, double* eta2nd);
+
+/* Third-order wavemaker paddle modes a3rd_add (n3_add*n2, modal; i_wmk = 3),
+ * the driver's array of HOS-NWT.f90:508-516.  Error below i_wmk = 3. */
+int hos_set_state3(hos_ctx* c, const double* a3rd_add);
+int hos_get_state3(hos_ctx* c, double* a3rd_add);
 
 /* One RHS evaluation (solveHOS2) on modal inputs G1, G2 (n1*n2) and the
  * stage values a1st (modal), eta1st (physical), a_add_1 (n3_add*n2) at
@@ -129,13 +134,22 @@ int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double
             const double* eta1st, const double* a_add_1, double* dG1, double* dG2, double* da1st,
             double* deta1st, double* da_add_1, double* da_add_ramp);
 
-/* hos_rhs with the second-order wavemaker (i_wmk = 2): also takes the stage's
+/* hos_rhs with the second-order wavemaker (i_wmk >= 2): also takes the stage's
  * a2nd, eta2nd, a_add_2 and returns da2nd, deta2nd, da_add_2
  * (solveHOS2 with solveHOS_SWEET_2, resol_HOS.f90:435-460). */
 int hos_rhs2(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
              const double* eta1st, const double* a_add_1, const double* a2nd, const double* eta2nd,
              const double* a_add_2, double* dG1, double* dG2, double* da1st, double* deta1st, double* da_add_1,
              double* da_add_ramp, double* da2nd, double* deta2nd, double* da_add_2);
+
+/* hos_rhs2 with the third-order wavemaker (i_wmk = 3): also takes the stage's
+ * a_add_3 and returns da_add_3 (solveHOS2 with solveHOS_SWEET_3,
+ * resol_HOS.f90:406-434; resol_wmkr.f90:338-437). */
+int hos_rhs3(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
+             const double* eta1st, const double* a_add_1, const double* a2nd, const double* eta2nd,
+             const double* a_add_2, const double* a_add_3, double* dG1, double* dG2, double* da1st,
+             double* deta1st, double* da_add_1, double* da_add_ramp, double* da2nd, double* deta2nd,
+             double* da_add_2, double* da_add_3);
 
 int hos_get_physical(hos_ctx* c, double* eta, double* phis, double* deta, double* phi_add);
 int hos_get_slope_max(hos_ctx* c, double* sx, double* sy);

@@ -26,8 +26,8 @@ Choice of computed case      :: icase            :: 2
 Number of the natural mode   :: islosh           :: 3
 Amplitude (m) of the mode    :: aslosh           :: 0.02d0
 --------------- Monochromatic case (icase = 2) ----------------
-Amplitude (m) of wave train  :: amp_mono         :: {amp!r}
-Frequency (Hz) of wave train :: nu_mono          :: {nu!r}
+Amplitude (m) of wave train  :: amp_mono         :: {amp}
+Frequency (Hz) of wave train :: nu_mono          :: {nu}
 Angle (deg) from x-axis      :: theta_mono       :: 0.d0
 Phasis (rad) of wave train   :: ph_mono          :: 0.d0
 Directional wmk type         :: ibat             :: 2
@@ -56,9 +56,9 @@ Absorption strength front    :: coeffabsf        :: 1.d0
 Use of probes                :: iprobes          :: 0
 Filename of probe positions  :: pro_file         :: prob.inp
 ---------------------- Time-integration -----------------------
-Duration of the simulation   :: T_stop           :: {tstop!r}
-Time tolerance: RK 4(5)      :: toler            :: {toler!r}
-Output frequency             :: f_out            :: {fout!r}
+Duration of the simulation   :: T_stop           :: {tstop}
+Time tolerance: RK 4(5)      :: toler            :: {toler}
+Output frequency             :: f_out            :: {fout}
 --------------------------- Output ----------------------------
 Output: 1-dim. ; 0-nondim.   :: idim             :: 0
 free surface plot            :: i3d              :: 0
@@ -75,8 +75,13 @@ def test_fortran_driver_matches_oracle(tmp_path, i_wmk, toler, tstop):
     T, n1, M = 1.5, 64, 3
     s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T, toler=toler, i_wmk=i_wmk)
     amp = s.cfg.amp_mono
-    # values written with Python repr are read back exactly by ES25.16
-    text = INPUT.format(amp=amp, nu=1.0 / T, tstop=tstop, fout=2.0 / T, i_wmk=i_wmk, toler=toler)
+    # 17 significant digits with an explicit decimal point are read back exactly by ES25.16
+    # (a field without a decimal point would be scaled by the descriptor's 16 implied digits)
+    def num(x):
+        return f"{x:.16e}"
+
+    text = INPUT.format(amp=num(amp), nu=num(1.0 / T), tstop=num(tstop), fout=num(2.0 / T), i_wmk=i_wmk,
+                        toler=num(toler))
     (tmp_path / "input_HOS-NWT.dat").write_text(text)
     r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
