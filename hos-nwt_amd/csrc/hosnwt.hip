@@ -1503,6 +1503,8 @@ struct EngineBase {
                    const double* aad, double* kG1, double* kG2, double* ka1, double* ke1, double* kad1,
                    double* kadr, const S2* s2) = 0;
   virtual void slope_max(double& sx, double& sy) = 0;
+  // FP64 base-grid cos-cos transform on the context stream (host-facing state I/O)
+  virtual void base_cc64(int dir, const double* in, double* out) = 0;
   // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
   // and one y pass (full inverse cos-cos transform), timed with HIP events
   virtual void time_big(int reps, double* out4) = 0;
@@ -1600,17 +1602,21 @@ struct Engine : EngineBase {
   rocblas_handle cb = nullptr;
   T* ctmp = nullptr;
   std::vector<T*>* ctmpX = nullptr;
+  // HBM workspaces of long extended x lines, one per stream (null when unused)
+  T *gwsM = nullptr, *gwsS = nullptr, *cgws = nullptr;
   void use_main() {
     cs = c->st;
     cb = blas;
     ctmp = tmpB;
     ctmpX = &tmpX;
+    cgws = gwsM;
   }
   void use_side() {
     cs = sw;
     cb = bw;
     ctmp = tmpBw;
     ctmpX = &tmpXw;
+    cgws = gwsS;
   }
   // second-order wavemaker (i_wmk = 2): paddle derivatives, stacked paddle
   // fields, inverse paddle-grid matrices, z = 0 sums and first-order fields
@@ -1629,7 +1635,7 @@ struct Engine : EngineBase {
   explicit Engine(hos_ctx* ctx) : c(ctx) {
     Arena& ar = c->ar;
     gb.build(c->N1b, c->N2b);
-    gb.check_lengths();
+    gb.check_lengths(true);
     gsT.build(c->n1, c->n2);
     nb = (size_t)c->N1b * c->N2b;
     ns = (size_t)c->n1 * c->n2;
@@ -1685,11 +1691,15 @@ struct Engine : EngineBase {
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     tmpBw = ar.get<T>(nb);
     for (int i = 0; i < 3; ++i) tmpXw.push_back(ar.get<T>(nb));
+    if (const size_t gw = gb.gline_ws_elems()) {
+      gwsM = ar.get<T>(gw);
+      gwsS = ar.get<T>(gw);
+    }
     use_main();
     dense = dense_base_grid(c->n1, c->n2);
     if (!dense) {
-      gsT.check_lengths();
-      c->gs64.check_lengths();
+      gsT.check_lengths(false);
+      c->gs64.check_lengths(false);
     }
     if (dense) {
       build_dense<T>(dT, ar, c->n1, c->n2);
@@ -1796,13 +1806,13 @@ struct Engine : EngineBase {
   // ---- transform helpers ------------------------------------------------
   template <class S, class D>
   void big(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
-    r2r2d<T>(gb, kx, ky, dir, s, o1, o2, d, ctmp, cs);
+    r2r2d<T>(gb, kx, ky, dir, s, o1, o2, d, ctmp, cs, cgws);
     c->stats.n_big++;
   }
   // independent extended-grid transforms in one launch per pass
   template <class S, class D>
   void bigB(int dir, const Batch2<T, S, D>& b, bool serial = false, bool yfirst = false) {
-    r2r2d_batch<T>(gb, dir, b, cs, serial, yfirst);
+    r2r2d_batch<T>(gb, dir, b, cs, serial, yfirst, cgws);
     c->stats.n_big += b.n;
   }
   // base-grid transforms use tmpB (large enough) as their intermediate
@@ -1825,6 +1835,12 @@ struct Engine : EngineBase {
       r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, cs);
     }
     c->stats.n_base++;
+  }
+
+  void base_cc64(int dir, const double* in, double* out) override {
+    use_main();
+    base64(KCOS, KCOS, dir, src_plain<double, double>(in, c->n1, c->n1, c->n2), c->n1, c->n2,
+           Dst<double>{out, c->n1, 0});
   }
 
   void rhs(double t, double dt, const double* G1, const double* G2, const double* a1, const double* e1,
@@ -2250,7 +2266,7 @@ struct Engine : EngineBase {
     bx.nlines[0] = N2b;
     bx.src[0] = sx;
     bx.dst[0] = dx;
-    for (int r = 0; r < reps; ++r) launch_x<T>(gb.px, N1b, INV, bx, C.st);
+    for (int r = 0; r < reps; ++r) launch_x<T>(gb.px, N1b, INV, bx, C.st, gwsM);
     HIP_OK(hipEventRecord(e1, C.st));
     if (N2b > 1) {
       Src<T, T> sy = src_plain<T>(tmpB, N1b, N1b, N2b);
@@ -2689,13 +2705,12 @@ int hos_set_state(hos_ctx* c, const double* eta, const double* phis, const doubl
     const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
     hipStream_t st = c->st;
     // eta, phis physical -> modal (runge_kutta.f90:128-134)
-    HIP_OK(hipMemcpyAsync(c->phys_tmp, eta, ns * sizeof(double), hipMemcpyHostToDevice, st));
-    r2r2d<double>(c->gs64, KCOS, KCOS, FWD, src_plain<double, double>(c->phys_tmp, c->n1, c->n1, c->n2), c->n1,
-                  c->n2, Dst<double>{c->eta_hat, c->n1, 0}, c->G1m, st);
-    HIP_OK(hipMemcpyAsync(c->phys_tmp, phis, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    // staged through G1m: the non-dense base transform uses phys_tmp as its intermediate
+    HIP_OK(hipMemcpyAsync(c->G1m, eta, ns * sizeof(double), hipMemcpyHostToDevice, st));
+    c->eng->base_cc64(FWD, c->G1m, c->eta_hat);
+    HIP_OK(hipMemcpyAsync(c->G1m, phis, ns * sizeof(double), hipMemcpyHostToDevice, st));
     HIP_OK(hipStreamSynchronize(st));
-    r2r2d<double>(c->gs64, KCOS, KCOS, FWD, src_plain<double, double>(c->phys_tmp, c->n1, c->n1, c->n2), c->n1,
-                  c->n2, Dst<double>{c->phis_hat, c->n1, 0}, c->G1m, st);
+    c->eng->base_cc64(FWD, c->G1m, c->phis_hat);
     if (a1st_add) HIP_OK(hipMemcpyAsync(c->a1st_add, a1st_add, na * sizeof(double), hipMemcpyHostToDevice, st));
     if (a_add_ramp) HIP_OK(hipMemcpyAsync(c->a_add_ramp, a_add_ramp, na * sizeof(double), hipMemcpyHostToDevice, st));
     if (a1st) HIP_OK(hipMemcpyAsync(c->a1st, a1st, ns * sizeof(double), hipMemcpyHostToDevice, st));
@@ -2707,9 +2722,8 @@ int hos_set_state(hos_ctx* c, const double* eta, const double* phis, const doubl
 
 static void to_phys(hos_ctx* c, const double* modal, double* host) {
   const size_t ns = (size_t)c->n1 * c->n2;
-  r2r2d<double>(c->gs64, KCOS, KCOS, INV, src_plain<double, double>(modal, c->n1, c->n1, c->n2), c->n1, c->n2,
-                Dst<double>{c->phys_tmp, c->n1, 0}, c->G1m, c->st);
-  HIP_OK(hipMemcpyAsync(host, c->phys_tmp, ns * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  c->eng->base_cc64(INV, modal, c->G1m);
+  HIP_OK(hipMemcpyAsync(host, c->G1m, ns * sizeof(double), hipMemcpyDeviceToHost, c->st));
   HIP_OK(hipStreamSynchronize(c->st));
 }
 
@@ -2949,18 +2963,24 @@ static int standalone(int n1, int n2, int kx, int ky, int prec, int dir, const d
     if (prec == 0) {
       Grid2<double> g;
       g.build(n1, n2);
+      g.check_lengths(true);
       double* tmp = ar.get<double>(n);
+      double* gws = g.gline_ws_elems() ? ar.get<double>(g.gline_ws_elems()) : nullptr;
+      HIP_OK(hipDeviceSynchronize());  // the arena's zero fills run on the null stream
       r2r2d<double>(g, kx, ky, dir, src_plain<double, double>(din, n1, n1, n2), n1, n2, Dst<double>{dout, n1, 0},
-                    tmp, st);
+                    tmp, st, gws);
       HIP_OK(hipStreamSynchronize(st));
       g.release();
     } else {
       Grid2<float> g;
       g.build(n1, n2);
+      g.check_lengths(true);
       float* tmp = ar.get<float>(n);
       float* fo = ar.get<float>(n);
+      float* gws = g.gline_ws_elems() ? ar.get<float>(g.gline_ws_elems()) : nullptr;
+      HIP_OK(hipDeviceSynchronize());  // the arena's zero fills run on the null stream
       r2r2d<float>(g, kx, ky, dir, src_plain<float, double>(din, n1, n1, n2), n1, n2, Dst<float>{fo, n1, 0}, tmp,
-                   st);
+                   st, gws);
       hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n)), dim3(256), 0, st, n, (const float*)fo, dout);
       HIP_OK(hipStreamSynchronize(st));
       g.release();

@@ -298,12 +298,100 @@ inline void host_fft_pow2(std::vector<std::complex<double>>& a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Long x lines (beyond the register engine's compiled lengths and one LDS
+// workgroup): the same packed-pair DCT-I/DST-I, with the complex FFT done as
+// Stockham radix passes between two HBM buffers, one launch per pass, each
+// butterfly on its own thread, so one line spreads over the whole chip.
+// ws holds 2 * L * npairs complex values; pair p uses [p*L, (p+1)*L).
+// ---------------------------------------------------------------------------
+template <typename T, class S>
+__global__ void __launch_bounds__(256) k_gline_load(int N, int kind, int dir, int n_in, int nlines, int L, S src,
+                                                    C2<T>* __restrict__ ws) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (j >= N) return;
+  const int la = 2 * pair, lb = la + 1;
+  T a = (T)0, b = (T)0;
+  if (j < n_in) {
+    a = src.get(j, la);
+    if (lb < nlines) b = src.get(j, lb);
+  }
+  if (dir == INV) {
+    const T f = f2s_fac<T>(j, N);
+    a *= f;
+    b *= f;
+  }
+  C2<T> z = hosfft::mk<T>(a, b);
+  if (kind == KSIN && (j == 0 || j == N - 1)) z = hosfft::mk<T>((T)0, (T)0);
+  C2<T>* line = ws + (size_t)pair * L;
+  line[j] = z;
+  if (j >= 1 && j <= N - 2) line[L - j] = (kind == KCOS) ? z : hosfft::mk<T>(-z.x, -z.y);
+}
+
+// One Stockham pass (radix R, current sub-transform length Ns): butterfly j reads
+// j + r*(L/R), twiddles input r by W_{Ns R}^{r (j%Ns)} and writes
+// (j/Ns)*Ns*R + (j%Ns) + r*Ns (the LDS engine's stockham_pass, through HBM).
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_gline_pass(int L, int Ns, const C2<T>* __restrict__ in,
+                                                    C2<T>* __restrict__ out, const C2<T>* __restrict__ tw) {
+  const int nb = L / R;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const size_t off = (size_t)blockIdx.y * L;
+  C2<T> v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = in[off + j + r * nb];
+  const int k = j % Ns;
+  if (Ns > 1) {
+    const C2<T> w1 = tw[(L / (Ns * R)) * k];
+    C2<T> w = w1;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      v[r] = hosfft::cmul<T>(v[r], w);
+      if (r + 1 < R) w = hosfft::cmul<T>(w, w1);
+    }
+  }
+  hosfft::Bfly<T, R>::run(v);
+  C2<T>* dst = out + off + (size_t)(j - k) * R + k;
+#pragma unroll
+  for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
+}
+
+template <typename T, class D>
+__global__ void __launch_bounds__(256) k_gline_store(int N, int kind, int dir, int n_out, int nlines, int L,
+                                                     const C2<T>* __restrict__ ws, D dst) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int pair = blockIdx.y;
+  if (k >= n_out) return;
+  const int la = 2 * pair, lb = la + 1;
+  const C2<T> Z = ws[(size_t)pair * L + k];
+  T A, B;
+  if (kind == KCOS) {
+    A = Z.x;
+    B = Z.y;
+  } else {
+    A = -Z.y;
+    B = Z.x;
+    if (k == 0 || k == N - 1) A = B = (T)0;
+  }
+  if (dir == FWD) {
+    const T inv = (T)(1.0 / (double)L);
+    const T f = (k == 0 || k == N - 1) ? inv : (T)2 * inv;
+    A *= f;
+    B *= f;
+  }
+  dst.put(k, la, A);
+  if (lb < nlines) dst.put(k, lb, B);
+}
+
 // Device tables for one logical line length (complex FFT length Lorig).
 template <typename T>
 struct LinePlan {
   FftPlan<T> dev{};
   hosfft2::Plan2<T> dev2{};
   int V2 = 0;  // values per thread of the register-resident engine (0: not usable)
+  int glob = 0;  // 1: line too long for both in-workgroup engines, multi-launch HBM passes
   std::vector<void*> allocs;
   // Register-resident plan: lengths 2^a 3^b with radices 8/4/2/3 dividing V.
   void build2(int L) {
@@ -414,6 +502,7 @@ struct Grid2 {
     N2 = n2;
     px.build(2 * (N1 - 1));
     if (N2 > 1) py.build(2 * (N2 - 1));
+    px.glob = (!px.V2 && !lds_fits(px, 32) && !px.dev.bluestein) ? 1 : 0;
     TX = sizeof(T) == 8 ? 8 : 16;
     // keep the y tile within the LDS budget
     while (TX > 2 && N2 > 1 &&
@@ -431,17 +520,20 @@ struct Grid2 {
         TX /= 2;
     }
   }
-  // Throws when a line is longer than both engines hold: the register engine
-  // (compiled lengths) and one LDS workgroup of the Stockham/Bluestein engine.
-  void check_lengths() const {
-    auto fits = [](const LinePlan<T>& pl, int vpt) {
-      return pl.V2 || ((size_t)pl.dev.L * sizeof(C2<T>) <= 160 * 1024 && (long)pl.dev.L <= 256L * vpt);
-    };
-    if (!fits(px, 32) || (N2 > 1 && !fits(py, 16)))
-      throw std::runtime_error("grid " + std::to_string(N1) + " x " + std::to_string(N2) +
-                               ": line transform longer than the supported lengths (compiled register lengths up "
-                               "to 12288 points, or one LDS workgroup)");
+  static bool lds_fits(const LinePlan<T>& pl, int vpt) {
+    return (size_t)pl.dev.L * sizeof(C2<T>) <= 160 * 1024 && (long)pl.dev.L <= 256L * vpt;
   }
+  // Throws when a line is longer than the engines hold: the register engine
+  // (compiled lengths), one LDS workgroup of the Stockham/Bluestein engine, or
+  // (x lines of 2,3,5-smooth length) the multi-launch HBM passes.
+  // allow_glob: the caller passes the HBM workspace to every transform on this grid.
+  void check_lengths(bool allow_glob) const {
+    if (!(px.V2 || lds_fits(px, 32) || (allow_glob && px.glob)) || (N2 > 1 && !(py.V2 || lds_fits(py, 16))))
+      throw std::runtime_error("grid " + std::to_string(N1) + " x " + std::to_string(N2) +
+                               ": line transform longer than the supported lengths");
+  }
+  // T values of the HBM workspace of long x lines (0 when not needed)
+  size_t gline_ws_elems() const { return px.glob ? (size_t)4 * px.dev.L * ((N2 + 1) / 2) : 0; }
   void release() {
     px.release();
     py.release();
@@ -515,9 +607,42 @@ void launch_reg(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<
 
 // x passes of every item of a batch (lines of item i: nlines[i] x-lines).
 template <typename T, class S, class D>
-void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st, T* gws = nullptr) {
   const int nl = batch_max_lines(b);
   if (nl <= 0) return;
+  if (pl.glob) {
+    if (!gws) throw std::runtime_error("long x lines need the HBM workspace");
+    const int L = pl.dev.L;
+    if (L != 2 * (N - 1)) throw std::runtime_error("long x line plan does not match N");
+    C2<T>* w0 = reinterpret_cast<C2<T>*>(gws);
+    for (int i = 0; i < b.n; ++i) {
+      if (b.nlines[i] <= 0 || b.n_out[i] <= 0) continue;
+      const int npairs = (b.nlines[i] + 1) / 2;
+      C2<T>* in = w0;
+      C2<T>* out = w0 + (size_t)L * npairs;
+      hipLaunchKernelGGL((k_gline_load<T, S>), dim3((N + 255) / 256, npairs), dim3(256), 0, st, N, b.kind[i], dir,
+                         b.n_in[i], b.nlines[i], L, b.src[i], in);
+      int Ns = 1;
+      for (int p = 0; p < pl.dev.npass; ++p) {
+        const int R = pl.dev.radix[p];
+        const dim3 g((L / R + 255) / 256, npairs);
+        switch (R) {
+          case 8: hipLaunchKernelGGL((k_gline_pass<T, 8>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
+          case 4: hipLaunchKernelGGL((k_gline_pass<T, 4>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
+          case 2: hipLaunchKernelGGL((k_gline_pass<T, 2>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
+          case 3: hipLaunchKernelGGL((k_gline_pass<T, 3>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
+          case 5: hipLaunchKernelGGL((k_gline_pass<T, 5>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
+          default: throw std::runtime_error("long x line: unsupported radix");
+        }
+        Ns *= R;
+        std::swap(in, out);
+      }
+      hipLaunchKernelGGL((k_gline_store<T, D>), dim3((b.n_out[i] + 255) / 256, npairs), dim3(256), 0, st, N,
+                         b.kind[i], dir, b.n_out[i], b.nlines[i], L, (const C2<T>*)in, b.dst[i]);
+      HIP_OK(hipGetLastError());
+    }
+    return;
+  }
   if (pl.V2) {
     int ncl = 1;
     while (2 * ncl * pl.dev2.ntl <= 256 && 2 * ncl <= (nl + 1) / 2) ncl *= 2;
@@ -596,12 +721,13 @@ struct Batch2 {
 // access pattern for read-modify-write epilogues); tmp[i] then holds
 // min(src.nx, N1) x nout2[i] values.
 template <typename T, class D>
-void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>, D>& b, hipStream_t st, bool serial) {
+void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>, D>& b, hipStream_t st, bool serial,
+                   T* gws = nullptr) {
   auto go = [&](const LineBatch<Src<T, T>, D>& bb) {
     if (y)
       launch_y<T>(g.py, g.TX, g.N2, dir, bb, st);
     else
-      launch_x<T>(g.px, g.N1, dir, bb, st);
+      launch_x<T>(g.px, g.N1, dir, bb, st, gws);
   };
   if (!serial) {
     go(b);
@@ -622,7 +748,7 @@ void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>
 
 template <typename T, class S, class D>
 void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream_t st, bool serial = false,
-                 bool yfirst = false) {
+                 bool yfirst = false, T* gws = nullptr) {
   if (b.n == 0) return;
   if (g.N2 == 1) {
     LineBatch<S, D> x;
@@ -645,10 +771,10 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
         one.nlines[0] = 1;
         one.src[0] = x.src[i];
         one.dst[0] = x.dst[i];
-        launch_x<T>(g.px, g.N1, dir, one, st);
+        launch_x<T>(g.px, g.N1, dir, one, st, gws);
       }
     } else {
-      launch_x<T>(g.px, g.N1, dir, x, st);
+      launch_x<T>(g.px, g.N1, dir, x, st, gws);
     }
     return;
   }
@@ -685,11 +811,11 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
     p2.dst[i] = b.dst[i];
   }
   if (!yfirst) {
-    launch_x<T>(g.px, g.N1, dir, p1, st);
-    launch_second<T, D>(true, g, dir, p2, st, serial);
+    launch_x<T>(g.px, g.N1, dir, p1, st, gws);
+    launch_second<T, D>(true, g, dir, p2, st, serial, gws);
   } else {
     launch_y<T>(g.py, g.TX, g.N2, dir, p1, st);
-    launch_second<T, D>(false, g, dir, p2, st, serial);
+    launch_second<T, D>(false, g, dir, p2, st, serial, gws);
   }
 }
 
@@ -697,10 +823,10 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
 // extents nout1 x nout2.  tmp must hold nout1 * src.ny values.
 template <typename T, class S, class D>
 void r2r2d(const Grid2<T>& g, int kx, int ky, int dir, const S& src, int nout1, int nout2, const D& dst, T* tmp,
-           hipStream_t st) {
+           hipStream_t st, T* gws = nullptr) {
   Batch2<T, S, D> b;
   b.add(kx, ky, src, nout1, nout2, dst, tmp);
-  r2r2d_batch<T>(g, dir, b, st);
+  r2r2d_batch<T>(g, dir, b, st, false, false, gws);
 }
 
 }  // namespace hosnwt

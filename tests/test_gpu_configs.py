@@ -7,7 +7,8 @@
 * 3-D directional basin: 40 x 10, 1024 x 256, M=5, JONSWAP 256 frequencies x 32
   directions (cos^2s, s = 10), beach on the last 15 %; extended grid 3073 x 769.
 * 3-D member of the learned-order ensemble: 512 x 128 at M_max = 7.
-* 2-D ensemble members: (512, M=3), (1024, M=4), (2048, M=3).
+* 2-D ensemble members: (512, M=3), (1024, M=4), (2048, M=3), and the members whose extended
+  lines exceed the in-workgroup engines: (2048, M=6), (4096, M=3), (4096, M=6).
 
 The random modal state has a decaying spectrum so that the products stay well
 conditioned; the wavemaker runs at a mid-ramp time.
@@ -36,6 +37,13 @@ CASES = {
                               spec=dict(Hs=0.07, Tp=1.8, n_freq=64, n_dir=1, seed=12)),
     "member_2d_2048_M3": dict(n1=2048, n2=1, M=3, n3=33, xlen=60.0, Tramp=3.0,
                               spec=dict(Hs=0.09, Tp=1.3, n_freq=64, n_dir=1, seed=13)),
+    # extended lines beyond the in-workgroup engines: 14400, 16384 and 28800 points
+    "member_2d_2048_M6": dict(n1=2048, n2=1, M=6, n3=33, xlen=60.0, Tramp=3.0,
+                              spec=dict(Hs=0.06, Tp=1.6, n_freq=64, n_dir=1, seed=14)),
+    "member_2d_4096_M3": dict(n1=4096, n2=1, M=3, n3=33, xlen=80.0, Tramp=3.0,
+                              spec=dict(Hs=0.05, Tp=1.9, n_freq=64, n_dir=1, seed=15)),
+    "member_2d_4096_M6": dict(n1=4096, n2=1, M=6, n3=33, xlen=80.0, Tramp=3.0,
+                              spec=dict(Hs=0.08, Tp=1.4, n_freq=64, n_dir=1, seed=16)),
 }
 
 
@@ -75,13 +83,3 @@ def test_rhs_parity_at_config_size(name):
             assert v < TOL, (name, k, v)
     finally:
         tank.close()
-
-
-@pytest.mark.gpu
-@pytest.mark.parametrize("n1,M", [(4096, 3), (2048, 6)])
-def test_lines_beyond_the_engines_are_refused_at_create(n1, M):
-    """cfg4 members whose extended lines exceed both line engines (16382 and 14334 points)
-    fail at hos_create with a message, before any device work."""
-    s = make_setup(n1=n1, n2=1, M=M, n3=33, xlen=60.0, Tramp=3.0)
-    with pytest.raises(RuntimeError, match="line transform longer"):
-        Tank(s, precision=0)

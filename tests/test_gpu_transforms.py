@@ -3,7 +3,9 @@
 FP64: relative L2 <= 1e-13; FP32: relative L2 <= 2e-6.  Sizes cover smooth
 lengths (Stockham radix 2/3/4/5/8), every compiled register-engine length
 (512 ... 12288) and non-smooth ones (Bluestein):
-2(n-1) = 16, 32, 510 (2.3.5.17), 2046 (2.3.11.31), 6144, 1536, 4094 (2.23.89).
+2(n-1) = 16, 32, 510 (2.3.5.17), 2046 (2.3.11.31), 6144, 1536, 4094 (2.23.89),
+and x lines longer than both in-workgroup engines (multi-launch HBM passes):
+14400 (2^6 3^2 5^2), 16384, 20480, 28800.
 """
 import numpy as np
 import pytest
@@ -12,8 +14,9 @@ from helpers import rel
 from hosnwt import lib
 from oracle import hos_oracle as ho
 
-SIZES_1D = [5, 9, 17, 33, 129, 256, 257, 385, 513, 769, 1024, 1025, 1537, 2048, 2049, 3073, 4097, 6145]
-SIZES_2D = [(9, 5), (33, 17), (256, 64), (1024, 256), (3073, 769), (2049, 513)]
+SIZES_1D = [5, 9, 17, 33, 129, 256, 257, 385, 513, 769, 1024, 1025, 1537, 2048, 2049, 3073, 4097, 6145,
+            7201, 8193, 10241, 14401]
+SIZES_2D = [(9, 5), (33, 17), (256, 64), (1024, 256), (3073, 769), (2049, 513), (8193, 5)]
 
 
 @pytest.mark.gpu
