@@ -210,7 +210,9 @@ template <typename T, int V, int R, bool YP, class D>
 __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl, int ntl, int L, int N, int kind,
                                            int dir, int n_out, int la, int lb, bool ha, bool hb) {
   const T inv = (T)(1.0 / (double)L);
-  constexpr int CH = 8;  // outputs whose loads are issued together
+  // outputs whose loads are issued together: 8, or 4 for FP64 epilogues that
+  // read several fields per output (fewer spills at two waves per SIMD)
+  constexpr int CH = (sizeof(T) == 8 && sizeof(typename D::Pre) > 2 * sizeof(T)) ? 4 : 8;
 #pragma unroll
   for (int i0 = 0; i0 < V; i0 += CH) {
     typename D::Pre qa[CH], qb[CH];
@@ -268,6 +270,9 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
 // column tiles are visited in an XCD-aware order so that neighbouring tiles
 // share an L2.  Spectral multipliers (kth, -kx, -ky) are evaluated once per
 // element in a rolled loop into LDS before the loads.
+// FP64 instances are held to two waves per SIMD (at most 256 registers): the
+// HOS epilogue then spills a few dwords, which costs less than running at one
+// wave per SIMD.
 // Independent line transforms sharing one launch (blockIdx.y = item): same
 // length, direction and functor types; kind, extents, source and destination
 // per item.  A workgroup whose lines lie beyond its item's extent exits at
@@ -285,7 +290,8 @@ struct LineBatch {
 };
 
 template <typename T, int V, bool YP, class S, class D, int... Rs>
-__global__ void __launch_bounds__((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256) r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt,
+__global__ void __launch_bounds__((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256,
+                                  sizeof(T) == 8 ? 2 : 1) r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt,
                                                                  Plan2<T> pl) {
   constexpr int L = hosfft2_prod<Rs...>::value;
   const int item = blockIdx.y;
