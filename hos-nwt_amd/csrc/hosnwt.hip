@@ -980,6 +980,29 @@ __global__ void k_second_fs(size_t ns, const double* __restrict__ w2, const doub
            iabs * (nu ? nu[i] : 0.0) * d;
 }
 
+// Free-surface elevation at probe positions (output.f90:266-276 with
+// mat_coscos, initial_condition.f90:502-510): eta(x, y) = sum_{i1,i2}
+// eta_hat(i1,i2) cos(kx(i1) x) cos(ky(i2) y); one workgroup per probe, the
+// cosines staged in LDS (n1 + n2 doubles), a fixed-order block reduction.
+__global__ void __launch_bounds__(256) k_probes(int n1, int n2, double pix, double piy, const double* __restrict__ xp,
+                                                const double* __restrict__ yp, const double* __restrict__ eta_hat,
+                                                double* out) {
+  extern __shared__ double cxy[];
+  const int j = blockIdx.x;
+  const double x = xp[j], y = yp[j];
+  for (int i = threadIdx.x; i < n1; i += blockDim.x) cxy[i] = cos((i * pix) * x);
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) cxy[n1 + i] = cos((i * piy) * y);
+  __syncthreads();
+  double acc = 0.0;
+  const size_t ns = (size_t)n1 * n2;
+  for (size_t o = threadIdx.x; o < ns; o += blockDim.x) {
+    const int i1 = (int)(o % n1), i2 = (int)(o / n1);
+    acc += eta_hat[o] * cxy[i1] * cxy[n1 + i2];
+  }
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) out[j] = acc;
+}
+
 // First-order free surface RHS (resol_wmkr.f90:186-193 via phidphiz :583-636):
 // deta1 = F2S(a1 kth1) + phiz_add_1 ; tmp = -eta1 - phit_add_1 - iabsnb nu deta1.
 __global__ void k_first_order(size_t n, const double* w1, const double* phiz1, const double* phit1,
@@ -1505,6 +1528,9 @@ struct EngineBase {
   virtual void slope_max(double& sx, double& sy) = 0;
   // FP64 base-grid cos-cos transform on the context stream (host-facing state I/O)
   virtual void base_cc64(int dir, const double* in, double* out) = 0;
+  // paddle displacement and velocity at the top node (i3 = n3) for every y node,
+  // from the last wavemaker evaluation (host arrays of n2)
+  virtual void paddle_top(double* x, double* xt) = 0;
   // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
   // and one y pass (full inverse cos-cos transform), timed with HIP events
   virtual void time_big(int reps, double* out4) = 0;
@@ -1523,7 +1549,7 @@ struct hos_ctx {
   std::vector<double> freq, pre0, pim0;
   double gz_top = 1.0;
   // wavemaker globals of the last evaluation (alpha, pos1st(n3,1))
-  double last_alpha = 0.0, last_pos_top = 0.0;
+  double last_alpha = 0.0, last_pos_top = 0.0, last_alphat = 0.0;
   // device state
   double *eta_hat, *phis_hat, *a1st_add, *a_add_ramp, *a1st, *eta1st;
   double *eta_hat_t, *phis_hat_t, *a1st_add_t, *a_add_ramp_t, *a1st_t, *eta1st_t, *deta_hat, *deta_hat_t;
@@ -1837,6 +1863,15 @@ struct Engine : EngineBase {
     c->stats.n_base++;
   }
 
+  void paddle_top(double* x, double* xt) override {
+    const int n2 = c->n2, n3 = c->n3;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy2D(x, sizeof(double), Xp + (n3 - 1), (size_t)n3 * sizeof(double), sizeof(double), n2,
+                       hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy2D(xt, sizeof(double), Xpt + (n3 - 1), (size_t)n3 * sizeof(double), sizeof(double), n2,
+                       hipMemcpyDeviceToHost));
+  }
+
   void base_cc64(int dir, const double* in, double* out) override {
     use_main();
     base64(KCOS, KCOS, dir, src_plain<double, double>(in, c->n1, c->n1, c->n2), c->n1, c->n2,
@@ -2016,6 +2051,7 @@ struct Engine : EngineBase {
     double al, alt, altt;
     ramp_eval(C.p, t, al, alt, altt);
     C.last_alpha = al;
+    C.last_alphat = alt;
     C.last_pos_top = C.pos_top(t);
     hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
                        (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt);
@@ -2120,6 +2156,7 @@ struct Engine : EngineBase {
     double al, alt, altt;
     ramp_eval(C.p, t, al, alt, altt);
     C.last_alpha = al;
+    C.last_alphat = alt;
     C.last_pos_top = C.pos_top(t);
     // --- SWEET 1: paddle condition and modes
     hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
@@ -2670,6 +2707,7 @@ int hos_initialize(hos_ctx* c) {
     c->endramp = 0;
     if (!c->wmk) {
       c->last_alpha = 0.0;
+      c->last_alphat = 0.0;
       c->last_pos_top = 0.0;
       HIP_OK(hipStreamSynchronize(st));
       return;
@@ -2909,6 +2947,60 @@ int hos_get_wmk(hos_ctx* c, double* alpha, double* pos_top) {
   return guard([&] {
     *alpha = c->last_alpha;
     *pos_top = c->last_pos_top;
+  });
+}
+
+int hos_get_wmk_motion(hos_ctx* c, double* alpha, double* alphat, double* x_top, double* v_top) {
+  return guard([&] {
+    *alpha = c->last_alpha;
+    *alphat = c->last_alphat;
+    const int n2 = c->n2;
+    if (!c->wmk) {
+      for (int i = 0; i < n2; ++i) x_top[i] = v_top[i] = 0.0;
+      return;
+    }
+    std::vector<double> X(n2), Xt(n2);
+    c->eng->paddle_top(X.data(), Xt.data());
+    for (int i = 0; i < n2; ++i) {
+      x_top[i] = c->last_alpha * X[i];
+      v_top[i] = c->last_alphat * X[i] + c->last_alpha * Xt[i];
+    }
+  });
+}
+
+int hos_get_modes(hos_ctx* c, double* eta_modes, double* phis_modes) {
+  return guard([&] {
+    const size_t ns = (size_t)c->n1 * c->n2;
+    HIP_OK(hipStreamSynchronize(c->st));
+    if (eta_modes) HIP_OK(hipMemcpy(eta_modes, c->eta_hat, ns * sizeof(double), hipMemcpyDeviceToHost));
+    if (phis_modes) HIP_OK(hipMemcpy(phis_modes, c->phis_hat, ns * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int hos_probes(hos_ctx* c, int np, const double* xp, const double* yp, double* eta_p) {
+  return guard([&] {
+    if (np < 0) throw std::runtime_error("negative probe count");
+    if (np == 0) return;
+    const size_t lds = (size_t)(c->n1 + c->n2) * sizeof(double);
+    if (lds > 160 * 1024) throw std::runtime_error("grid too large for the probe kernel");
+    std::vector<double> y0(np, 0.0);
+    double *d = nullptr;
+    HIP_OK(hipMalloc(&d, 3 * (size_t)np * sizeof(double)));
+    struct Free {
+      double* p;
+      ~Free() { (void)hipFree(p); }
+    } fr{d};
+    HIP_OK(hipMemcpy(d, xp, np * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(d + np, (c->n2 > 1 && yp) ? yp : y0.data(), np * sizeof(double), hipMemcpyHostToDevice));
+    HIP_OK(hipStreamSynchronize(c->st));
+    if (lds > 64 * 1024)
+      HIP_OK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_probes),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_probes, dim3(np), dim3(256), lds, c->st, c->n1, c->n2, c->pix, c->piy, (const double*)d,
+                       (const double*)(d + np), (const double*)c->eta_hat, d + 2 * np);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(eta_p, d + 2 * np, np * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
   });
 }
 

@@ -6,8 +6,11 @@
 ! piston or hinged flap) or a sloshing mode (icase = 1), the front beach, the
 ! RK4 stability bound (main/HOS-NWT.f90:132-196), and runs the time loop of
 ! main/HOS-NWT.f90:446-581 with the inner adaptive loop on the GPU
-! (hos_advance).  Writes Results/vol_energy.dat (IO/output.f90:60-70,
-! :251-262).  The reference's sizes are compile-time parameters
+! (hos_advance).  Writes the reference's output files (IO/output.f90:37-482)
+! with its header echo (IO/input.f90:318-395): vol_energy.dat, and as the
+! input flags ask 3d.dat (i3d), HOS_modes.dat (imodes), wmk_motion.dat (iwmk)
+! and probes.dat (iprobes = 1, positions read from pro_file); SWENSE output and
+! pressure probes are not built.  The reference's sizes are compile-time parameters
 ! (common_vars.f90:31-35); here they are runtime arguments:
 !     hos_nwt_gpu [n1 n2 n3 mHOS [precision]]      (default 513 1 33 3 0)
 PROGRAM hos_nwt_gpu
@@ -27,8 +30,10 @@ CHARACTER(LEN=100) :: file_name, pro_file
 ! derived
 REAL(RP) :: Tscale, T_mono, w_mono, dt_stab, delt, delt_out, time_cur, time_next, h_rk
 REAL(RP) :: k_w, freq1, delx, dely, x0, x2, x3, c8, d1, d2, d3, voltot, enepot, enekin, alpha, pos_top
-REAL(RP) :: t_adim
+REAL(RP) :: t_adim, x_adim, alphat
 COMPLEX(RP) :: tf, ctmp
+INTEGER :: nprobes
+REAL(RP), ALLOCATABLE :: xprobe(:), yprobe(:), eta_probe(:), etam(:,:), phism(:,:), xtop(:), vtop(:)
 REAL(RP), ALLOCATABLE, TARGET :: nu(:,:), eta(:,:), phis(:,:), deta(:,:), phi_add(:,:), zero2(:,:), zadd(:,:)
 REAL(RP), ALLOCATABLE, TARGET :: freq(:), pre(:), pim(:), dyre(:), dyim(:), nux(:,:)
 TYPE(hos_params) :: p
@@ -65,10 +70,15 @@ CALL blank(); CALL rd_i(idim); CALL rd_i(i3d); CALL rd_i(imodes); CALL rd_i(iwmk
 CLOSE(unit_in)
 IF (icase /= 1 .AND. icase /= 2) ERROR STOP 'this driver builds icase = 1 or 2 (files: use the Python host)'
 IF (icase == 2 .AND. ibat /= 2) ERROR STOP 'only the front wave (ibat = 2) is built here'
+! adjustments made while reading (IO/input.f90:163-172)
 islosh = islosh + 1
 IF (n2 == 1) theta_mono = 0.0_rp
+theta_mono = theta_mono * PI / 180.0_rp
 IF (iramp == 0) Tramp = 0.0_rp
 IF (i_wmk < 1 .OR. i_wmk > 3) ERROR STOP 'i_wmk = 1, 2 or 3 is built here'
+IF (iprobes == 2) ERROR STOP 'pressure probes (iprobes = 2) are not built here'
+! output files, before the nondimensionalisation as in main/HOS-NWT.f90:92
+CALL init_output()
 !
 ! nondimensionalisation (HOS-NWT.f90:99-119)
 Tscale = 1.0_rp / SQRT(G / h)
@@ -121,12 +131,12 @@ IF (icase == 2) THEN
    k_w = wave_number(freq1)
    tf = transfer(igeom, k_w, d_hinge)
    DO i2 = 1, n2
-      ctmp = tf * COS(theta_mono * PI / 180.0_rp) * amp_mono * &
-             EXP(CMPLX(0.0_rp, -k_w * SIN(theta_mono * PI / 180.0_rp) * (i2 - 1) * dely + ph_mono, RP))
+      ctmp = tf * COS(theta_mono) * amp_mono * &
+             EXP(CMPLX(0.0_rp, -k_w * SIN(theta_mono) * (i2 - 1) * dely + ph_mono, RP))
       pre(i2) = REAL(ctmp)
       pim(i2) = AIMAG(ctmp)
       ! y derivative of the paddle amplitude (wavemaking.f90:147-149, no lateral ramp)
-      ctmp = CMPLX(0.0_rp, -k_w * SIN(theta_mono * PI / 180.0_rp), RP) * ctmp
+      ctmp = CMPLX(0.0_rp, -k_w * SIN(theta_mono), RP) * ctmp
       dyre(i2) = REAL(ctmp)
       dyim(i2) = AIMAG(ctmp)
    END DO
@@ -154,15 +164,12 @@ ELSE
 END IF
 !
 t_adim = 1.0_rp
-IF (idim == 1) t_adim = Tscale
-CALL EXECUTE_COMMAND_LINE('mkdir -p Results')
-OPEN(2, FILE='Results/vol_energy.dat', STATUS='UNKNOWN')
-WRITE(2,'(A)') 'TITLE=" volume and energy evolution "'
-IF (idim == 0) THEN
-   WRITE(2,'(A)') 'VARIABLES="time*sqrt(g/h)", "voltot", "enepot", "enekin", "enetot"'
-ELSE
-   WRITE(2,'(A)') 'VARIABLES="time", "voltot", "enepot", "enekin", "enetot"'
+x_adim = 1.0_rp
+IF (idim == 1) THEN
+   t_adim = Tscale
+   x_adim = h
 END IF
+ALLOCATE(etam(n1,n2), phism(n1,n2), xtop(n2), vtop(n2))
 !
 time_cur = 0.0_rp
 n_rk_tot = 0; n_er_tot = 0; n_acc = 0; n_rej = 0
@@ -181,6 +188,7 @@ DO WHILE (time_cur < T_stop + delt_out)
    ELSE
       WRITE(2,'(5(ES12.5,X))') time_cur * t_adim, voltot, enepot, enekin, enepot + enekin
    END IF
+   CALL output_time_step()
    WRITE(*,'("    volume = ",ES12.4," t= ",ES11.4," timestep:",ES11.4,2(X,I5))') voltot, time_cur * t_adim, &
         delt * t_adim, n_rk_tot, n_er_tot
    IF (ABS(time_cur - T_stop) <= TINY) EXIT
@@ -197,6 +205,10 @@ DO WHILE (time_cur < T_stop + delt_out)
    END IF
 END DO
 CLOSE(2)
+IF (i3d == 1) CLOSE(1)
+IF (imodes == 1) CLOSE(5)
+IF (iwmk /= 0 .AND. igeom /= 3) CLOSE(8)
+IF (iprobes == 1) CLOSE(9)
 CALL hos_check(hos_destroy(ctx), 'hos_destroy')
 !
 CONTAINS
@@ -219,6 +231,264 @@ SUBROUTINE rd_c(v)
    CHARACTER(LEN=N_TOT) :: d
    READ(unit_in, '(A52,A)') d, v
 END SUBROUTINE rd_c
+!
+! header echo of the input (IO/input.f90:318-395, formats :457-470)
+SUBROUTINE write_input(u)
+   INTEGER, INTENT(IN) :: u
+   CALL wb(u, '----------------------- Tank dimensions -----------------------')
+   CALL wr(u, xlen, 'xlen', 'Length(m) of the wavetank')
+   CALL wr(u, ylen, 'ylen', 'Beam(m) of the wavetank')
+   CALL wr(u, h, 'h', 'Depth(m) of the wavetank')
+   CALL wb(u, '------------------------ Select case --------------------------')
+   CALL wi(u, icase, 'icase', 'Choice of computed case')
+   CALL wb(u, '------------------ Sloshing case (icase = 1) ------------------')
+   CALL wi(u, islosh, 'islosh', 'Number of the natural mode')
+   CALL wr(u, aslosh, 'aslosh', 'Amplitude (m) of the mode')
+   CALL wb(u, '--------------- Monochromatic case (icase = 2) ----------------')
+   CALL wr(u, amp_mono, 'amp_mono', 'Amplitude (m) of wave train')
+   CALL wr(u, nu_mono, 'nu_mono', 'Frequency (Hz) of wave train')
+   CALL wr(u, theta_mono, 'theta_mono', 'Angle (deg) from x-axis')
+   CALL wr(u, ph_mono, 'ph_mono', 'Phasis (rad) of wave train')
+   CALL wi(u, ibat, 'ibat', 'Directional wmk type')
+   CALL wr(u, xd_mono, 'xd_mono', 'Wave target distance (ibat=3)')
+   CALL wb(u, '-------------------- File case (icase = 3) --------------------')
+   CALL wc(u, file_name, 'file_name', 'Name of frequency file')
+   CALL wi(u, i_cut, 'i_cut', 'Frequency cut-off')
+   CALL wr(u, nuc_low, 'nuc_low', 'Low cut_off frequency (Hz)')
+   CALL wr(u, nuc_high, 'nuc_high', 'High cut_off frequency (Hz) ')
+   CALL wb(u, '------------------ Irregular wave (icase=4) -------------------')
+   CALL wr(u, Hs, 'Hs', 'Significant wave height (m)')
+   CALL wr(u, Tp, 'Tp', 'Peak period (s)')
+   CALL wr(u, gamma, 'gamma', 'Shape factor (Jonswap)')
+   CALL wi(u, iseed, 'iseed', 'Seed number for random numb. ')
+   CALL wb(u, '-------------------- Wavemaker definition ---------------------')
+   CALL wi(u, i_wmk, 'i_wmk', 'Nonlinear order of wavemaker')
+   CALL wi(u, igeom, 'igeom', 'Type (1: piston, 2: hinged)')
+   CALL wr(u, d_hinge, 'd_hinge', 'Rotation axis distance')
+   CALL wi(u, iramp, 'iramp', 'Time ramp')
+   CALL wr(u, Tramp, 'Tramp', 'Time ramp duration')
+   CALL wb(u, '----------------------- Numerical beach -----------------------')
+   CALL wi(u, iabsnb, 'iabsnb', 'Absorption numerical beach')
+   CALL wr(u, xabsf, 'xabsf', 'Beginning front num. beach')
+   CALL wr(u, coeffabsf, 'coeffabsf', 'Absorption strength front')
+   CALL wb(u, '------------- Elevation/Velocity-Pressure probes --------------')
+   CALL wi(u, iprobes, 'iprobes', 'Use of probes')
+   CALL wc(u, pro_file, 'pro_file', 'Filename of probe positions')
+   CALL wb(u, '---------------------- Time-integration -----------------------')
+   CALL wr(u, T_stop, 'T_stop', 'Duration of the simulation')
+   CALL wr(u, toler, 'toler', 'Time tolerance: RK 4(5)')
+   CALL wr(u, f_out, 'f_out', 'Output frequency')
+   CALL wb(u, '--------------------------- Output ----------------------------')
+   CALL wi(u, idim, 'idim', 'Output: 1-dim. ; 0-nondim.')
+   CALL wi(u, i3d, 'i3d', 'Free surface plot')
+   CALL wi(u, imodes, 'imodes', 'Modes plot')
+   CALL wi(u, iwmk, 'iwmk', 'Wavemaker motion plot')
+   CALL wi(u, i_sw, 'i_sw', 'Swense output 1=''yes'',0=''no''')
+   CALL wb(u, '--------------------- Numerical parameters ---------------------')
+   CALL wi(u, n1, 'n1', 'Number of nodes/modes in x')
+   CALL wi(u, n2, 'n2', 'Number of nodes/modes in y')
+   CALL wi(u, n3, 'n3', 'Number of nodes/modes on wmk')
+   CALL wi(u, mHOS, 'mHOS', 'HOS order')
+   CALL wi(u, p%p1, 'p1', 'Dealiasing in x-direction')
+   CALL wi(u, p%p2, 'p2', 'Dealiasing in y-direction')
+   CALL wr(u, p%coeffilt(1), 'coeffilt(1)', 'Filtering ratio in x-modes')
+   CALL wr(u, p%coeffilt(2), 'coeffilt(2)', 'Filtering ratio in y-modes')
+   CALL wr(u, p%coeffilt(3), 'coeffilt(3)', 'Filtering ratio in z-modes')
+END SUBROUTINE write_input
+SUBROUTINE wb(u, text)
+   INTEGER, INTENT(IN) :: u
+   CHARACTER(LEN=*), INTENT(IN) :: text
+   WRITE(u,'("#",A)') text
+END SUBROUTINE wb
+SUBROUTINE wi(u, v, var, text)
+   INTEGER, INTENT(IN) :: u, v
+   CHARACTER(LEN=*), INTENT(IN) :: var, text
+   WRITE(u,'("#",A33,":: ",A13,":: ",I5)') text, var, v
+END SUBROUTINE wi
+SUBROUTINE wr(u, v, var, text)
+   INTEGER, INTENT(IN) :: u
+   REAL(RP), INTENT(IN) :: v
+   CHARACTER(LEN=*), INTENT(IN) :: var, text
+   WRITE(u,'("#",A33,":: ",A13,":: ",ES25.16)') text, var, v
+END SUBROUTINE wr
+SUBROUTINE wc(u, v, var, text)
+   INTEGER, INTENT(IN) :: u
+   CHARACTER(LEN=*), INTENT(IN) :: v, var, text
+   WRITE(u,'("#",A33,":: ",A13,":: ",A)') text, var, v
+END SUBROUTINE wc
+!
+! init_output (IO/output.f90:37-214): headers, and the probe positions
+SUBROUTINE init_output()
+   INTEGER :: j, ios
+   REAL(RP) :: xr, yr
+   CHARACTER(LEN=12) :: num
+   CHARACTER(LEN=4000) :: line
+   p%p1 = mHOS
+   p%p2 = 1
+   IF (n2 /= 1) p%p2 = mHOS
+   CALL EXECUTE_COMMAND_LINE('mkdir -p Results')
+   IF (i3d == 1) THEN
+      OPEN(1, FILE='Results/3d.dat', STATUS='UNKNOWN')
+      CALL write_input(1)
+      WRITE(1,*) 'TITLE=" 3D free surface elevation "'
+      IF (idim == 0) THEN
+         WRITE(1,'(A)') 'VARIABLES="x/h","y/h","eta/h","phis*sqrt(h/g)/h**2"'
+      ELSE
+         WRITE(1,'(A)') 'VARIABLES="x","y","eta","phis"'
+      END IF
+   END IF
+   OPEN(2, FILE='Results/vol_energy.dat', STATUS='UNKNOWN')
+   CALL write_input(2)
+   WRITE(2,'(A)') 'TITLE=" volume and energy evolution "'
+   IF (idim == 0) THEN
+      WRITE(2,'(A)') 'VARIABLES="time*sqrt(g/h)", "voltot", "enepot", "enekin", "enetot"'
+   ELSE
+      WRITE(2,'(A)') 'VARIABLES="time", "voltot", "enepot", "enekin", "enetot"'
+   END IF
+   IF (imodes == 1) THEN
+      OPEN(5, FILE='Results/HOS_modes.dat', STATUS='UNKNOWN')
+      CALL write_input(5)
+      WRITE(5,'(A)') 'TITLE = " 3D spectral potential modes"'
+      IF (idim == 0) THEN
+         WRITE(5,'(15A)') 'VARIABLES = "kx*h", "ky*h","a_eta/h","a_phis*sqrt(h/g)/h**2","LOG10-a-eta","LOG10-a-phis"'
+      ELSE
+         WRITE(5,'(15A)') 'VARIABLES = "kx", "ky","a_eta","a_phis","LOG10-a-eta","LOG10-a-phis"'
+      END IF
+   END IF
+   IF (iwmk /= 0 .AND. igeom /= 3) THEN
+      OPEN(8, FILE='Results/wmk_motion.dat', STATUS='UNKNOWN')
+      CALL write_input(8)
+      IF (n2 == 1) THEN
+         WRITE(8,*) 'TITLE="Wavemaker motion against time"'
+         IF (idim == 0) THEN
+            WRITE(8,'(a)') 'VARIABLES="time*sqrt(g/h)", "X-top/h", "Vx-top/sqrt(g*h)"'
+         ELSE
+            WRITE(8,'(a)') 'VARIABLES="time", "X-top", "Vx-top"'
+         END IF
+      ELSE
+         WRITE(8,*) 'TITLE="Wavemaker motion against time and transverse direction"'
+         IF (idim == 0) THEN
+            WRITE(8,'(a)') 'VARIABLES="y/h", "X-top/h", "Vx-top/sqrt(g*h)"'
+         ELSE
+            WRITE(8,'(a)') 'VARIABLES="y", "X-top", "Vx-top"'
+         END IF
+      END IF
+   END IF
+   nprobes = 0
+   IF (iprobes == 1) THEN
+      OPEN(55, FILE=TRIM(pro_file), STATUS='OLD')
+      DO
+         READ(55, *, IOSTAT=ios)
+         IF (ios /= 0) EXIT
+         nprobes = nprobes + 1
+      END DO
+      REWIND(55)
+      IF (nprobes > 60) ERROR STOP 'at most 60 wave probes (output format of probes.dat)'
+      ALLOCATE(xprobe(nprobes), yprobe(nprobes), eta_probe(nprobes))
+      DO j = 1, nprobes
+         IF (n2 == 1) THEN
+            READ(55, *) xr
+            yr = 0.0_rp
+         ELSE
+            READ(55, *) xr, yr
+         END IF
+         xprobe(j) = xr / h
+         yprobe(j) = yr / h
+      END DO
+      CLOSE(55)
+      OPEN(9, FILE='Results/probes.dat', STATUS='UNKNOWN')
+      CALL write_input(9)
+      WRITE(9,'(A)') 'TITLE="Probes elevation records versus time"'
+      IF (idim == 0) THEN
+         line = 'VARIABLES = "time*sqrt(g/h)" '
+      ELSE
+         line = 'VARIABLES = "time" '
+      END IF
+      DO j = 1, nprobes
+         WRITE(num,'(I0)') j
+         IF (idim == 0) THEN
+            line = TRIM(line) // ' "p' // TRIM(num) // '/h"'
+         ELSE
+            line = TRIM(line) // ' "p' // TRIM(num) // '"'
+         END IF
+      END DO
+      WRITE(9,'(A)') TRIM(line) // ' '
+   ELSE
+      ALLOCATE(xprobe(1), yprobe(1), eta_probe(1))
+   END IF
+END SUBROUTINE init_output
+!
+! output_time_step (IO/output.f90:232-482) for the files opened by init_output
+! (vol_energy.dat is written in the main loop); tecplot = 11 zone headers
+SUBROUTINE output_time_step()
+   REAL(RP) :: eta_target, xx, yy
+   INTEGER :: j
+   IF (iprobes == 1) THEN
+      CALL hos_check(hos_probes(ctx, INT(nprobes, C_INT), xprobe, yprobe, eta_probe), 'hos_probes')
+      WRITE(9,'(61d13.5)') time_cur * t_adim, (eta_probe(j) * x_adim, j = 1, nprobes)
+   END IF
+   IF (i3d == 1) THEN
+      IF (ABS(time_cur) < TINY) THEN
+         WRITE(1,'(A,F6.2,A,I4,A,I4)') 'ZONE SOLUTIONTIME = ', time_cur * t_adim, ', I=', n1, ', J=', n2
+         DO i2 = 1, n2
+            DO i1 = 1, n1
+               WRITE(1,'(2(F10.5,X),2(ES12.5,X))') (i1 - 1) * delx * x_adim, (i2 - 1) * dely * x_adim, &
+                    eta(i1,i2) * x_adim, phis(i1,i2) * x_adim**2 / t_adim
+            END DO
+         END DO
+      ELSE
+         WRITE(1,'(A,F6.2,A,I4,A,I4)') 'ZONE SOLUTIONTIME = ', time_cur * t_adim, ', D=(1,2), I=', n1, ', J=', n2
+         DO i2 = 1, n2
+            DO i1 = 1, n1
+               eta_target = 0.0_rp
+               IF (icase == 2) THEN
+                  xx = (i1 - 1) * delx
+                  yy = (i2 - 1) * dely
+                  eta_target = amp_mono * COS(w_mono * time_cur - k_w * (xx * COS(theta_mono) + yy * SIN(theta_mono)) &
+                       + ph_mono)
+               END IF
+               WRITE(1,'(2(ES12.5,X))') eta(i1,i2) * x_adim, (eta(i1,i2) - eta_target) * x_adim
+            END DO
+         END DO
+      END IF
+   END IF
+   IF (iwmk == 1 .AND. igeom /= 3) THEN
+      CALL hos_check(hos_get_wmk_motion(ctx, alpha, alphat, xtop, vtop), 'hos_get_wmk_motion')
+      IF (n2 == 1) THEN
+         WRITE(8,'(3d13.5)') time_cur * t_adim, xtop(1) * x_adim, vtop(1) * x_adim / t_adim
+      ELSE
+         WRITE(8,'(A,F8.2,A,I3)') 'ZONE SOLUTIONTIME = ', time_cur * t_adim, ', I=', n2
+         DO i2 = 1, n2
+            WRITE(8,'(3d13.5)') (i2 - 1) * dely * x_adim, xtop(i2) * x_adim, vtop(i2) * x_adim / t_adim
+         END DO
+      END IF
+   END IF
+   IF (imodes == 1) THEN
+      CALL hos_check(hos_get_modes(ctx, etam, phism), 'hos_get_modes')
+      etam = etam * x_adim
+      phism = phism * x_adim**2 / t_adim
+      IF (ABS(time_cur) < TINY) THEN
+         WRITE(5,'(A,F6.2,A,I4,A,I4)') 'ZONE SOLUTIONTIME = ', time_cur * t_adim, ', I=', n1, ', J=', n2
+         DO i2 = 1, n2
+            DO i1 = 1, n1
+               xx = (i1 - 1) * PI / xlen
+               yy = 0.0_rp
+               IF (n2 /= 1) yy = (i2 - 1) * PI / ylen
+               WRITE(5,'(6(ES12.5,X))') xx / x_adim, yy / x_adim, etam(i1,i2), phism(i1,i2), &
+                    LOG10(MAX(TINY, ABS(etam(i1,i2)))), LOG10(MAX(TINY, ABS(phism(i1,i2))))
+            END DO
+         END DO
+      ELSE
+         WRITE(5,'(A,F6.2,A,I4,A,I4)') 'ZONE SOLUTIONTIME = ', time_cur * t_adim, ', D=(1,2), I=', n1, ', J=', n2
+         DO i2 = 1, n2
+            DO i1 = 1, n1
+               WRITE(5,'(4(ES12.5,X))') etam(i1,i2), phism(i1,i2), LOG10(MAX(TINY, ABS(etam(i1,i2)))), &
+                    LOG10(MAX(TINY, ABS(phism(i1,i2))))
+            END DO
+         END DO
+      END IF
+   END IF
+END SUBROUTINE output_time_step
 !
 ! k with (2 pi f)^2 = k tanh k by bisection (linear_wave.f90:137-151, :427-466)
 REAL(RP) FUNCTION wave_number(f)

@@ -131,6 +131,23 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: alpha, pos_top
    END FUNCTION hos_get_wmk
+   INTEGER(C_INT) FUNCTION hos_get_wmk_motion(ctx, alpha, alphat, x_top, v_top) BIND(C, NAME='hos_get_wmk_motion')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: alpha, alphat, x_top(*), v_top(*)
+   END FUNCTION hos_get_wmk_motion
+   INTEGER(C_INT) FUNCTION hos_get_modes(ctx, eta_modes, phis_modes) BIND(C, NAME='hos_get_modes')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: eta_modes(*), phis_modes(*)
+   END FUNCTION hos_get_modes
+   INTEGER(C_INT) FUNCTION hos_probes(ctx, np, xp, yp, eta_p) BIND(C, NAME='hos_probes')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: np
+      REAL(C_DOUBLE), INTENT(IN)  :: xp(*), yp(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: eta_p(*)
+   END FUNCTION hos_probes
    INTEGER(C_INT) FUNCTION hos_get_slope_max(ctx, sx, sy) BIND(C, NAME='hos_get_slope_max')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx

@@ -44,6 +44,7 @@ EXPORTS = [
     "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
     "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_check_slope",
     "hos_set_state2", "hos_get_state2", "hos_rhs2", "hos_set_state3", "hos_get_state3", "hos_rhs3",
+    "hos_get_wmk_motion", "hos_get_modes", "hos_probes",
 ]
 
 _lib = None
@@ -80,6 +81,9 @@ def load() -> C.CDLL:
     lib.hos_get_slope_max.argtypes = [vp, _dp, _dp]
     lib.hos_check_slope.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.hos_get_wmk.argtypes = [vp, _dp, _dp]
+    lib.hos_get_wmk_motion.argtypes = [vp, _dp, _dp, _dp, _dp]
+    lib.hos_get_modes.argtypes = [vp, _dp, _dp]
+    lib.hos_probes.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
     lib.hos_synchronize.argtypes = [vp]
     lib.hos_time_big_transform.argtypes = [vp, C.c_int, _dp]

@@ -169,6 +169,28 @@ class Tank:
         self.n_er_tot += nr.value
         return h.value, na.value, nr.value
 
+    def modes(self):
+        """Modal eta and phis of the committed state (HOS_modes.dat, output.f90:380-383)."""
+        out = [np.zeros((self.n1, self.n2), order="F") for _ in range(2)]
+        check(self.lib.hos_get_modes(self.h, *[ptr(a) for a in out]))
+        return dict(zip(("eta", "phis"), out))
+
+    def probes(self, xp, yp=None):
+        """Elevation at wave probes (nondimensional positions; output.f90:266-276)."""
+        xp = np.ascontiguousarray(xp, dtype=np.float64)
+        yp = None if yp is None else np.ascontiguousarray(yp, dtype=np.float64)
+        out = np.zeros(len(xp))
+        check(self.lib.hos_probes(self.h, len(xp), ptr(xp), ptr(yp), ptr(out)))
+        return out
+
+    def wmk_motion(self):
+        """alpha, alpha', paddle-top displacement and velocity per y node (wmk_motion.dat)."""
+        a, at = C.c_double(), C.c_double()
+        x = np.zeros(self.n2)
+        v = np.zeros(self.n2)
+        check(self.lib.hos_get_wmk_motion(self.h, C.byref(a), C.byref(at), ptr(x), ptr(v)))
+        return a.value, at.value, x, v
+
     def slope_max(self):
         sx, sy = C.c_double(), C.c_double()
         check(self.lib.hos_get_slope_max(self.h, C.byref(sx), C.byref(sy)))

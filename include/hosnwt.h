@@ -162,6 +162,19 @@ int hos_check_slope(hos_ctx* c, double threshold, double coeff, int* fx, int* fy
  * wavemaker evaluation, the module globals output.f90:251-256 subtracts
  * from the volume (resol_wmkr.f90:1268, wavemaking.f90:277). */
 int hos_get_wmk(hos_ctx* c, double* alpha, double* pos_top);
+/* Wavemaker motion of the last wavemaker evaluation, as written to
+ * wmk_motion.dat (output.f90:361-376): alpha, alpha', and for every y node
+ * x_top = alpha pos1st(n3,i2), v_top = alpha' pos1st(n3,i2) + alpha
+ * dpos1stdt(n3,i2) (n2 values each, nondimensional). */
+int hos_get_wmk_motion(hos_ctx* c, double* alpha, double* alphat, double* x_top, double* v_top);
+/* Modal free-surface elevation and potential of the committed state
+ * (n1*n2 each): the space_2_Fourier(eta), space_2_Fourier(phis) of
+ * HOS_modes.dat (output.f90:380-383).  Either pointer may be NULL. */
+int hos_get_modes(hos_ctx* c, double* eta_modes, double* phis_modes);
+/* Free-surface elevation at np wave probes (xp, yp nondimensional; yp may be
+ * NULL in a 2-D tank) of the committed state: the mat_coscos sums of
+ * output.f90:266-276 (initial_condition.f90:502-510). */
+int hos_probes(hos_ctx* c, int np, const double* xp, const double* yp, double* eta_p);
 int hos_get_stats(hos_ctx* c, hos_stats* s);
 /* Measurement helper (no reference counterpart): average duration in ms and
  * algorithmic HBM bytes of one extended-grid x pass and one y pass of a full

@@ -1,8 +1,11 @@
 """The Fortran driver (iso_c_binding into libhosnwt) reproduces the oracle's
-volume/energy trace for a reference-format input file.
+output files for a reference-format input file.
 
-Results/vol_energy.dat carries 5 significant digits (ES12.5, output.f90:262),
-so the comparison is at 2e-4 relative to the largest energy value.
+Results/vol_energy.dat, probes.dat, wmk_motion.dat, 3d.dat and HOS_modes.dat
+carry 5 or 6 significant digits (output.f90:201, :109, :801, :101-102, :501-502),
+so the values are compared at 2e-4 of each column's scale; the files' layout
+(header echo of the input, TITLE / VARIABLES lines, tecplot zone lines) is
+checked line by line.
 """
 import os
 import subprocess
@@ -53,7 +56,7 @@ Absorption numerical beach   :: iabsnb           :: 1
 Beginning front num. beach   :: xabsf            :: 0.8d0
 Absorption strength front    :: coeffabsf        :: 1.d0
 ------------- Elevation/Velocity-Pressure probes --------------
-Use of probes                :: iprobes          :: 0
+Use of probes                :: iprobes          :: {iprobes}
 Filename of probe positions  :: pro_file         :: prob.inp
 ---------------------- Time-integration -----------------------
 Duration of the simulation   :: T_stop           :: {tstop}
@@ -61,9 +64,9 @@ Time tolerance: RK 4(5)      :: toler            :: {toler}
 Output frequency             :: f_out            :: {fout}
 --------------------------- Output ----------------------------
 Output: 1-dim. ; 0-nondim.   :: idim             :: 0
-free surface plot            :: i3d              :: 0
-modes plot                   :: imodes           :: 0
-wavemaker motion plot        :: iwmk             :: 0
+free surface plot            :: i3d              :: {flag}
+modes plot                   :: imodes           :: {flag}
+wavemaker motion plot        :: iwmk             :: {flag}
 Swense output 1='yes',0='no' :: i_sw             :: 0
 """
 
@@ -80,23 +83,27 @@ def test_fortran_driver_matches_oracle(tmp_path, i_wmk, toler, tstop):
     def num(x):
         return f"{x:.16e}"
 
+    outputs = i_wmk == 1
     text = INPUT.format(amp=num(amp), nu=num(1.0 / T), tstop=num(tstop), fout=num(2.0 / T), i_wmk=i_wmk,
-                        toler=num(toler))
+                        toler=num(toler), iprobes=1 if outputs else 0, flag=1 if outputs else 0)
     (tmp_path / "input_HOS-NWT.dat").write_text(text)
+    xprobes = [2.0, 5.5, 10.25]
+    (tmp_path / "prob.inp").write_text("".join(f"{x}\n" for x in xprobes))
     r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
-    rows = [list(map(float, ln.split())) for ln in (tmp_path / "Results" / "vol_energy.dat").read_text().split("\n")[2:]
-            if ln.strip()]
-    got = np.array(rows)
+    got = np.array(data_rows(tmp_path / "Results" / "vol_energy.dat"))
     orc = ho.Oracle(oracle_params(s))
     st = orc.initialize()
     deta = np.zeros_like(st.eta)
     t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
     ref = []
+    snaps = []
     while t < s.T_stop + s.delt_out:
         ep, ek, et = orc.energy(st.eta, st.phis + orc.phi_add, deta)
         ref.append((t, ep, ek, et))
+        snaps.append(dict(t=t, eta=st.eta.copy(), phis=st.phis.copy(), alpha=orc.alpha, alphat=orc.alphat,
+                          x=orc.pos1st[-1, 0], xt=orc.dpos1stdt[-1, 0]))
         if abs(t - s.T_stop) <= 2.2e-16:
             break
         tn = min(t + s.delt_out, s.T_stop + s.delt_out)
@@ -107,3 +114,71 @@ def test_fortran_driver_matches_oracle(tmp_path, i_wmk, toler, tstop):
     np.testing.assert_allclose(got[:, 0], ref[:, 0], rtol=1e-4, atol=1e-4)
     scale = np.abs(ref[:, 3]).max()
     np.testing.assert_allclose(got[:, 2:5], ref[:, 1:4], rtol=0, atol=2e-4 * scale)
+    if outputs:
+        check_output_files(tmp_path / "Results", s, orc, snaps, xprobes)
+
+
+def data_rows(path):
+    """Numeric rows of a Tecplot file (skips the '#' echo, TITLE, VARIABLES and ZONE lines)."""
+    rows = []
+    for ln in path.read_text().split("\n"):
+        t = ln.strip()
+        if not t or t.startswith("#") or t.startswith("TITLE") or t.startswith("VARIABLES") or t.startswith("ZONE"):
+            continue
+        rows.append([float(v.replace("D", "E")) for v in t.split()])
+    return rows
+
+
+def close(got, ref, what):
+    got, ref = np.asarray(got, float), np.asarray(ref, float)
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-4 * max(np.abs(ref).max(), 1e-12), err_msg=what)
+
+
+def check_output_files(res, s, orc, snaps, xprobes):
+    n1 = s.cfg.n1
+    tb = orc.tb
+    # header echo (IO/input.f90:318-395): 59 '#' lines before TITLE, same in every file
+    for name in ("vol_energy.dat", "probes.dat", "wmk_motion.dat", "3d.dat", "HOS_modes.dat"):
+        lines = (res / name).read_text().split("\n")
+        head = [ln for ln in lines if ln.startswith("#")]
+        assert len(head) == 59, (name, len(head))
+        assert head[0] == "#----------------------- Tank dimensions -----------------------"
+        # '("#",A33,":: ",A13,":: ",ES25.16)': character data shorter than the field is right-justified
+        assert head[1] == "#" + "Length(m) of the wavetank".rjust(33) + ":: " + "xlen".rjust(13) + ":: " + \
+            "2.0000000000000000E+01".rjust(25)
+        assert head[-1].split("::")[1].strip() == "coeffilt(3)"
+        assert "TITLE" in lines[59]
+        assert lines[60].startswith("VARIABLES")
+    # probes.dat: time, eta at the probes = sum_k eta_hat cos(kx x_p) (output.f90:266-276)
+    pr = np.array(data_rows(res / "probes.dat"))
+    want = np.array([[d["t"]] + [float(np.dot(orc.s2f(d["eta"])[:, 0], np.cos(tb.kx * x))) for x in xprobes]
+                     for d in snaps])
+    close(pr[:, 0], want[:, 0], "probes time")
+    close(pr[:, 1:], want[:, 1:], "probes eta")
+    # wmk_motion.dat (2-D): time, alpha X_top, alpha' X_top + alpha X_t,top (output.f90:361-366)
+    wm = np.array(data_rows(res / "wmk_motion.dat"))
+    close(wm[:, 1], [d["alpha"] * d["x"] for d in snaps], "wmk X-top")
+    close(wm[:, 2], [d["alphat"] * d["x"] + d["alpha"] * d["xt"] for d in snaps], "wmk Vx-top")
+    # 3d.dat: first zone x, y, eta, phis; later zones eta, eta - eta_target (output.f90:291-338)
+    z = data_rows(res / "3d.dat")
+    first = np.array(z[:n1])
+    close(first[:, 0], tb.cx, "3d x")
+    close(first[:, 2], snaps[0]["eta"][:, 0], "3d eta t=0")
+    close(first[:, 3], snaps[0]["phis"][:, 0], "3d phis t=0")
+    for i, d in enumerate(snaps[1:], start=1):
+        blk = np.array(z[n1 * i:n1 * (i + 1)])
+        close(blk[:, 0], d["eta"][:, 0], f"3d eta zone {i}")
+    zones = [ln for ln in (res / "3d.dat").read_text().split("\n") if ln.startswith("ZONE")]
+    assert len(zones) == len(snaps)
+    assert zones[0].startswith("ZONE SOLUTIONTIME =   0.00, I=  64, J=   1")
+    assert "D=(1,2)" in zones[1]
+    # HOS_modes.dat: first zone kx, ky, eta_hat, phis_hat, log10 |.|; later zones without kx, ky
+    m = data_rows(res / "HOS_modes.dat")
+    first = np.array(m[:n1])
+    close(first[:, 0], tb.kx, "modes kx")
+    for i, d in enumerate(snaps):
+        blk = np.array(m[n1 * i:n1 * (i + 1)])
+        c = 2 if i == 0 else 0
+        close(blk[:, c], orc.s2f(d["eta"])[:, 0], f"modes eta zone {i}")
+        close(blk[:, c + 1], orc.s2f(d["phis"])[:, 0], f"modes phis zone {i}")
