@@ -203,6 +203,11 @@ def main():
     r64 = time_path(0, args.steps, args.warmup, args.spin_up)
     print(f"[bench] FP64 {r64['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     roof = roofline(r64["tank"], 0)
+    # one live context at a time, the deployment shape: with the FP64 context still alive the
+    # mixed path measured 114 instead of 135 steps/s (its two streams then share the process's
+    # four hardware queues with the idle context's streams)
+    r64["tank"].close()
+    r64["tank"] = None
     r32 = None
     if not args.fp64_only:
         print("[bench] mixed path", file=sys.stderr, flush=True)
@@ -219,7 +224,7 @@ def main():
         print("[bench] CPU baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
     for r in (r64, r32):
-        if r is not None:
+        if r is not None and r["tank"] is not None:
             r["tank"].close()
     line = {
         "metric": "RK4(5) steps/s (3D 1024x256, M=5) mixed/learned vs FP64 GPU at eta err <=1e-5",

@@ -291,7 +291,8 @@ struct LineBatch {
 
 template <typename T, int V, bool YP, class S, class D, int... Rs>
 __global__ void __launch_bounds__((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256,
-                                  sizeof(T) == 8 ? 2 : 1) r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt,
+                                  sizeof(T) == 8 ? 2 : (!YP ? 1 : (sizeof(typename D::Pre) > 2 * sizeof(T) ? 3 : 4)))
+    r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt,
                                                                  Plan2<T> pl) {
   constexpr int L = hosfft2_prod<Rs...>::value;
   const int item = blockIdx.y;
