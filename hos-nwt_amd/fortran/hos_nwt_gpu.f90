@@ -32,7 +32,14 @@ REAL(RP) :: Tscale, T_mono, w_mono, dt_stab, delt, delt_out, time_cur, time_next
 REAL(RP) :: k_w, freq1, delx, dely, x0, x2, x3, c8, d1, d2, d3, voltot, enepot, enekin, alpha, pos_top
 REAL(RP) :: t_adim, x_adim, alphat
 COMPLEX(RP) :: tf, ctmp
-INTEGER :: nprobes
+INTEGER :: nprobes, nh, nc, rnum_f
+! icase = 3 wave components (file2wave, wave/wave_def.f90:181-278): harmonic, frequency,
+! amplitude, angle (rad), phase, wave number, transfer function; paddle of the file
+INTEGER, ALLOCATABLE :: c_harmo(:)
+REAL(RP), ALLOCATABLE :: c_f(:), c_a(:), c_ang(:), c_ph(:), c_k(:)
+COMPLEX(RP), ALLOCATABLE :: c_tf(:)
+REAL(RP) :: clock_f, depth_f, hinge_f
+CHARACTER(LEN=16) :: typ_wmk_f
 REAL(RP), ALLOCATABLE :: xprobe(:), yprobe(:), eta_probe(:), etam(:,:), phism(:,:), xtop(:), vtop(:)
 REAL(RP), ALLOCATABLE, TARGET :: nu(:,:), eta(:,:), phis(:,:), deta(:,:), phi_add(:,:), zero2(:,:), zadd(:,:)
 REAL(RP), ALLOCATABLE, TARGET :: freq(:), pre(:), pim(:), dyre(:), dyim(:), nux(:,:)
@@ -68,7 +75,7 @@ CALL blank(); CALL rd_i(iprobes); CALL rd_c(pro_file)
 CALL blank(); CALL rd_r(T_stop); CALL rd_r(toler); CALL rd_r(f_out)
 CALL blank(); CALL rd_i(idim); CALL rd_i(i3d); CALL rd_i(imodes); CALL rd_i(iwmk); CALL rd_i(i_sw)
 CLOSE(unit_in)
-IF (icase /= 1 .AND. icase /= 2) ERROR STOP 'this driver builds icase = 1 or 2 (files: use the Python host)'
+IF (icase < 1 .OR. icase > 3) ERROR STOP 'this driver builds icase = 1, 2 or 3 (icase 4: use the Python host)'
 IF (icase == 2 .AND. ibat /= 2) ERROR STOP 'only the front wave (ibat = 2) is built here'
 ! adjustments made while reading (IO/input.f90:163-172)
 islosh = islosh + 1
@@ -117,7 +124,9 @@ IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
    END DO
 END IF
 !
-ALLOCATE(freq(1), pre(n2), pim(n2), dyre(n2), dyim(n2), nux(n1,n2))
+nh = 1
+IF (icase == 3) CALL file2wave()
+ALLOCATE(freq(nh), pre(n2*nh), pim(n2*nh), dyre(n2*nh), dyim(n2*nh), nux(n1,n2))
 freq = 0.0_rp; pre = 0.0_rp; pim = 0.0_rp; dyre = 0.0_rp; dyim = 0.0_rp; nux = 0.0_rp
 ! beach x derivative as initial_condition.f90:768 builds it (2 c x3), read with i_wmk = 2
 IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
@@ -141,6 +150,8 @@ IF (icase == 2) THEN
       dyim(i2) = AIMAG(ctmp)
    END DO
    freq(1) = freq1
+ELSE IF (icase == 3) THEN
+   CALL wmk_ini()
 END IF
 !
 p%n1 = n1; p%n2 = n2; p%n3 = n3; p%mHOS = mHOS; p%p1 = mHOS
@@ -151,6 +162,7 @@ p%xlen = xlen; p%ylen = ylen; p%d_hinge = d_hinge; p%Tramp = Tramp; p%T_stop = T
 p%nu = C_LOC(nu)
 p%n_freq = 0
 IF (icase == 2) p%n_freq = 1
+IF (icase == 3) p%n_freq = nh
 p%freq = C_LOC(freq); p%pos_ini_re = C_LOC(pre); p%pos_ini_im = C_LOC(pim)
 p%dposdy_ini_re = C_LOC(dyre); p%dposdy_ini_im = C_LOC(dyim); p%nu_x = C_LOC(nux)
 CALL hos_check(hos_create(p, ctx), 'hos_create')
@@ -179,10 +191,10 @@ DO WHILE (time_cur < T_stop + delt_out)
    CALL hos_check(hos_get_physical(ctx, eta, phis, deta, phi_add), 'hos_get_physical')
    CALL hos_check(hos_get_wmk(ctx, alpha, pos_top), 'hos_get_wmk')
    CALL volume_energy()
-   IF (igeom == 2 .AND. icase == 2) THEN
+   IF (igeom == 2 .AND. icase /= 1) THEN
       WRITE(2,'(5(ES12.5,X))') time_cur * t_adim, voltot - 0.5_rp * alpha * (1 - d_hinge) * pos_top / xlen &
            - eta(1,1) * alpha * pos_top / xlen, enepot, enekin, enepot + enekin
-   ELSE IF (icase == 2) THEN
+   ELSE IF (icase /= 1) THEN
       WRITE(2,'(5(ES12.5,X))') time_cur * t_adim, voltot - alpha * pos_top / xlen - eta(1,1) * alpha * pos_top / xlen, &
            enepot, enekin, enepot + enekin
    ELSE
@@ -489,6 +501,103 @@ SUBROUTINE output_time_step()
       END IF
    END IF
 END SUBROUTINE output_time_step
+!
+! file2wave (wave/wave_def.f90:181-278, with wave_init :282-329): <file_name>.cfg and
+! .dat, components between the cut-off frequencies, nondimensionalised by the file's depth
+SUBROUTINE file2wave()
+   INTEGER :: u, ios, nf, j, n_start, n_stop, ig
+   REAL(RP) :: ramp_f, lo, hi, ly_f, tsc
+   CHARACTER(LEN=16) :: typ_ramp_f
+   INTEGER, ALLOCATABLE :: hh(:)
+   REAL(RP), ALLOCATABLE :: ff(:), aa(:), an(:), ph(:)
+   u = 999
+   OPEN(u, FILE=TRIM(file_name)//'.cfg', STATUS='OLD')
+   READ(u,'(I3)') rnum_f
+   READ(u,*) clock_f
+   READ(u,*) depth_f
+   READ(u,*) ramp_f
+   READ(u,*) typ_ramp_f
+   READ(u,*) lo
+   READ(u,*) hi
+   READ(u,*) ly_f
+   READ(u,*) typ_wmk_f
+   READ(u,*) hinge_f
+   CLOSE(u)
+   tsc = SQRT(depth_f / G)
+   clock_f = clock_f * tsc
+   lo = lo * tsc
+   hi = hi * tsc
+   hinge_f = hinge_f / depth_f
+   OPEN(u, FILE=TRIM(file_name)//'.dat', STATUS='OLD')
+   nf = 0
+   DO
+      READ(u, *, IOSTAT=ios)
+      IF (ios /= 0) EXIT
+      nf = nf + 1
+   END DO
+   REWIND(u)
+   ALLOCATE(hh(nf), ff(nf), aa(nf), an(nf), ph(nf))
+   n_start = 1
+   n_stop = 0
+   DO j = 1, nf
+      READ(u,*) hh(j), ff(j), aa(j), an(j), ph(j)
+      IF (ff(j) <= hi / tsc) n_stop = j
+      IF (ff(j) < lo / tsc) n_start = n_start + 1
+   END DO
+   CLOSE(u)
+   nc = n_stop - n_start + 1
+   IF (nc <= 0) ERROR STOP 'no wave component between the cut-off frequencies'
+   ALLOCATE(c_harmo(nc), c_f(nc), c_a(nc), c_ang(nc), c_ph(nc), c_k(nc), c_tf(nc))
+   c_harmo = hh(n_start:n_stop)
+   c_f = ff(n_start:n_stop) * tsc
+   c_a = aa(n_start:n_stop) / depth_f
+   c_ang = an(n_start:n_stop) * PI / 180.0_rp
+   c_ph = ph(n_start:n_stop)
+   ig = 1
+   IF (TRIM(typ_wmk_f) == 'hinged') ig = 2
+   DO j = 1, nc
+      c_k(j) = wave_number(c_f(j))
+      c_tf(j) = CMPLX(1.0_rp, 0.0_rp, RP)
+      IF (ABS(c_k(j)) > TINY) c_tf(j) = transfer(ig, c_k(j), hinge_f)
+      IF (c_f(j) < 1.0E-8_rp) THEN
+         c_k(j) = 0.0_rp
+         c_tf(j) = CMPLX(1.0_rp, 0.0_rp, RP)
+      END IF
+   END DO
+   ! number of distinct harmonics up to 4 n_max (wavemaking.f90:104-118)
+   nh = 0
+   DO j = 0, 4 * (2**rnum_f / MAX(1, FLOOR(clock_f * SQRT(G / depth_f))))
+      IF (ANY(c_harmo == j)) nh = nh + 1
+   END DO
+   IF (nh == 0) ERROR STOP 'no wave harmonic below the maximum frequency'
+END SUBROUTINE file2wave
+!
+! wmk_ini (wavemaking.f90:119-155): per harmonic, the frequency of its first component and
+! pos_ini(y) = sum TF cos(angle) a exp(i(-k sin(angle) y + phase)), and its y derivative
+SUBROUTINE wmk_ini()
+   INTEGER :: hm, j, n, jf
+   COMPLEX(RP) :: term
+   n = 0
+   DO hm = 0, 4 * (2**rnum_f / MAX(1, FLOOR(clock_f * SQRT(G / depth_f))))
+      IF (.NOT. ANY(c_harmo == hm)) CYCLE
+      n = n + 1
+      jf = 0
+      DO j = 1, nc
+         IF (c_harmo(j) /= hm) CYCLE
+         IF (jf == 0) jf = j
+         DO i2 = 1, n2
+            term = c_tf(j) * COS(c_ang(j)) * c_a(j) * &
+                   EXP(CMPLX(0.0_rp, -c_k(j) * SIN(c_ang(j)) * (i2 - 1) * dely + c_ph(j), RP))
+            pre(i2 + n2 * (n - 1)) = pre(i2 + n2 * (n - 1)) + REAL(term)
+            pim(i2 + n2 * (n - 1)) = pim(i2 + n2 * (n - 1)) + AIMAG(term)
+            term = CMPLX(0.0_rp, -c_k(j) * SIN(c_ang(j)), RP) * term
+            dyre(i2 + n2 * (n - 1)) = dyre(i2 + n2 * (n - 1)) + REAL(term)
+            dyim(i2 + n2 * (n - 1)) = dyim(i2 + n2 * (n - 1)) + AIMAG(term)
+         END DO
+      END DO
+      freq(n) = c_f(jf)
+   END DO
+END SUBROUTINE wmk_ini
 !
 ! k with (2 pi f)^2 = k tanh k by bisection (linear_wave.f90:137-151, :427-466)
 REAL(RP) FUNCTION wave_number(f)

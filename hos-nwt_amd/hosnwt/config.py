@@ -194,7 +194,10 @@ def build(cfg: TankConfig) -> Setup:
     dely = ylen / (n2 - 1) if n2 != 1 else 0.0
     cy = np.arange(n2) * dely
     if waves is not None:
-        freq, pos_ini, dposdy_ini = wmk_ini(waves, cy)
+        hmax = None
+        if cfg.icase == 3:
+            hmax = 4 * (2 ** int(cfg.components["rnum"]) // max(1, math.floor(float(cfg.components["clock"]))))
+        freq, pos_ini, dposdy_ini = wmk_ini(waves, cy, hmax)
     else:
         freq, pos_ini = np.zeros(0), np.zeros((n2, 0), complex)
         dposdy_ini = np.zeros((n2, 0), complex)
@@ -206,10 +209,12 @@ def build(cfg: TankConfig) -> Setup:
                  waves=waves, dposdy_ini=dposdy_ini, nu_x=nu_x)
 
 
-def wmk_ini(w: WaveComponents, cy: np.ndarray):
+def wmk_ini(w: WaveComponents, cy: np.ndarray, hmax: int | None = None):
     """Per-frequency paddle amplitudes pos_ini(y, f) and their y derivatives dposdy_ini(y, f)
-    (wavemaking.f90:104-155), full-width paddle (no lateral ramp)."""
-    harms = sorted(set(int(x) for x in w.harmo))
+    (wavemaking.f90:104-155), full-width paddle (no lateral ramp).  Harmonics above ``hmax``
+    (4 n_max, n_max = 2^rnum / floor(clock in s): the 4 Hz limit of wavemaking.f90:104-118)
+    are left out."""
+    harms = sorted(set(int(x) for x in w.harmo if hmax is None or 0 <= int(x) <= hmax))
     freq = []
     cols, dcols = [], []
     for hm in harms:

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from helpers import make_setup, oracle_params
+from hosnwt import config as hc
 from oracle import hos_oracle as ho
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -24,7 +25,7 @@ Length(m) of the wavetank    :: xlen             :: 20.d0
 Beam(m) of the wavetank      :: ylen             :: 1.d0
 Depth(m) of the wavetank     :: h                :: 1.d0
 ------------------------ Select case --------------------------
-Choice of computed case      :: icase            :: 2
+Choice of computed case      :: icase            :: {icase}
 ------------------ Sloshing case (icase = 1) ------------------
 Number of the natural mode   :: islosh           :: 3
 Amplitude (m) of the mode    :: aslosh           :: 0.02d0
@@ -36,7 +37,7 @@ Phasis (rad) of wave train   :: ph_mono          :: 0.d0
 Directional wmk type         :: ibat             :: 2
 Wave target distance (ibat=3):: xd_mono          :: 18.d0
 -------------------- File case (icase = 3) --------------------
-Name of frequency file       :: file_name        :: none
+Name of frequency file       :: file_name        :: {file_name}
 Frequency cut-off            :: i_cut            :: 0
 Low cut_off frequency (Hz)   :: nuc_low          :: 0.3d0
 High cut_off frequency (Hz)  :: nuc_high         :: 3.d0
@@ -47,8 +48,8 @@ Shape factor (Jonswap)       :: gamma            :: 3.3d0
 Seed number for random numb. :: iseed            :: 513
 -------------------- Wavemaker definition ---------------------
 Nonlinear order of wavemaker :: i_wmk            :: {i_wmk}
-Type (1: piston, 2: hinged)  :: igeom            :: 1
-Rotation axis distance       :: d_hinge          :: 0.d0
+Type (1: piston, 2: hinged)  :: igeom            :: {igeom}
+Rotation axis distance       :: d_hinge          :: {d_hinge}
 Time ramp                    :: iramp            :: 1
 Time ramp duration (s)       :: Tramp            :: 4.5d0
 ----------------------- Numerical beach -----------------------
@@ -85,7 +86,8 @@ def test_fortran_driver_matches_oracle(tmp_path, i_wmk, toler, tstop):
 
     outputs = i_wmk == 1
     text = INPUT.format(amp=num(amp), nu=num(1.0 / T), tstop=num(tstop), fout=num(2.0 / T), i_wmk=i_wmk,
-                        toler=num(toler), iprobes=1 if outputs else 0, flag=1 if outputs else 0)
+                        toler=num(toler), iprobes=1 if outputs else 0, flag=1 if outputs else 0, icase=2,
+                        file_name="none", igeom=1, d_hinge=num(0.0))
     (tmp_path / "input_HOS-NWT.dat").write_text(text)
     xprobes = [2.0, 5.5, 10.25]
     (tmp_path / "prob.inp").write_text("".join(f"{x}\n" for x in xprobes))
@@ -182,3 +184,52 @@ def check_output_files(res, s, orc, snaps, xprobes):
         c = 2 if i == 0 else 0
         close(blk[:, c], orc.s2f(d["eta"])[:, 0], f"modes eta zone {i}")
         close(blk[:, c + 1], orc.s2f(d["phis"])[:, 0], f"modes phis zone {i}")
+
+
+@pytest.mark.gpu
+def test_fortran_driver_reads_wave_files(tmp_path):
+    """icase = 3: the driver reads <file_name>.cfg / .dat (file2wave, wave_def.f90:181-278) for a
+    flap and runs to the oracle's volume / energy trace built from the same components."""
+    assert os.path.exists(EXE), "build the Fortran driver with hos-nwt_amd/build.sh"
+    n1, M, tstop, T = 64, 3, 2.0, 1.5
+    comps = hc.jonswap_components(0.05, 1.5, 3.3, n_freq=6, seed=21, rnum=6, clock=8.0)
+    d_hinge = 0.3
+    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T, igeom=2,
+                   d_hinge=d_hinge, components=comps, icase=3)
+
+    def num(x):
+        return f"{x:.16e}"
+
+    # the reference's wave files: .cfg (rnum as I3, then one value per line), .dat components
+    cfg = [f"{comps['rnum']:3d}", num(comps["clock"]), num(1.0), num(0.0), "lin", num(0.0), num(100.0), num(1.0),
+           "hinged", num(d_hinge)]
+    (tmp_path / "waves.cfg").write_text("\n".join(cfg) + "\n")
+    rows = [f"{h:d} {num(f)} {num(a)} {num(an)} {num(p)}" for h, f, a, an, p in
+            zip(comps["harmo"], comps["freq"], comps["ampli"], comps["angle"], comps["phase"])]
+    (tmp_path / "waves.dat").write_text("\n".join(rows) + "\n")
+    text = INPUT.format(amp=num(0.0), nu=num(1.0 / T), tstop=num(tstop), fout=num(2.0 / T), i_wmk=1,
+                        toler=num(1e-7), iprobes=0, flag=0, icase=3, file_name="waves", igeom=2,
+                        d_hinge=num(d_hinge))
+    (tmp_path / "input_HOS-NWT.dat").write_text(text)
+    r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    got = np.array(data_rows(tmp_path / "Results" / "vol_energy.dat"))
+    orc = ho.Oracle(oracle_params(s))
+    st = orc.initialize()
+    deta = np.zeros_like(st.eta)
+    t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+    ref = []
+    while t < s.T_stop + s.delt_out:
+        ep, ek, et = orc.energy(st.eta, st.phis + orc.phi_add, deta)
+        ref.append((t, ep, ek, et))
+        if abs(t - s.T_stop) <= 2.2e-16:
+            break
+        tn = min(t + s.delt_out, s.T_stop + s.delt_out)
+        st, t, h, _, _, d = ho.advance(orc, st, t, tn, h, s.cfg.toler, s.dt_stab, s.delt_out)
+        deta = d if d is not None else deta
+    ref = np.array(ref)
+    assert got.shape[0] == ref.shape[0]
+    scale = np.abs(ref[:, 3]).max()
+    assert scale > 0
+    np.testing.assert_allclose(got[:, 2:5], ref[:, 1:4], rtol=0, atol=2e-4 * scale)
