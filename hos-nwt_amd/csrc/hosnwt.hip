@@ -160,6 +160,30 @@ __global__ void k_powers(size_t n, int M, PtrSet<T> e, const T* __restrict__ ex,
   g2[i] = x * x + y * y;
 }
 
+// One level iH of the HOS recursion from its inverse transforms v_j (item q of
+// lv holds j = iH - q): the accumulation the x-pass epilogue DstHos performs
+// transform by transform, in the same order j = iH..1 and with the same
+// expressions, phim = 0 - sum v_j eta^j/j!, phizm(iH) = 0 + sum v_j eta^(j-1)/(j-1)!
+// (resol_HOS.f90:736-772).  Lets 2-D tanks, whose transforms are single lines,
+// run the level's transforms in one launch.
+template <typename T>
+__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, PtrSet<T> ep, T* phim, T* phizm) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool do_phim = iH < M;
+  T pm = (T)0, pz = (T)0;
+  for (int q = 0; q < iH; ++q) {
+    const int j = iH - q;
+    const T v = lv.p[q][i];
+    const T e = do_phim ? ep.p[j][i] : (T)0;
+    const T e1 = (j >= 2) ? ep.p[j - 1][i] : (T)1;
+    pm = pm - v * e;
+    pz = pz + v * e1;
+  }
+  if (do_phim) phim[i] = pm;
+  phizm[i] = pz;
+}
+
 template <typename T>
 __global__ void k_sum_orders(size_t n, int M, PtrSet<T> z, T* out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1343,6 +1367,16 @@ inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, co
   const float one = 1.0f, zero = 0.0f;
   blas_ok(rocblas_sgemm(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, Cm, ldc), "sgemm");
 }
+// y(m) = AT^T x for AT stored k x m (leading dimension k): each output a
+// contiguous dot product
+inline void gemv_t(rocblas_handle h, int m, int k, const double* AT, const double* x, double* y) {
+  const double one = 1.0, zero = 0.0;
+  blas_ok(rocblas_dgemv(h, rocblas_operation_transpose, k, m, &one, AT, k, x, 1, &zero, y, 1), "dgemv");
+}
+inline void gemv_t(rocblas_handle h, int m, int k, const float* AT, const float* x, float* y) {
+  const float one = 1.0f, zero = 0.0f;
+  blas_ok(rocblas_sgemv(h, rocblas_operation_transpose, k, m, &one, AT, k, x, 1, &zero, y, 1), "sgemv");
+}
 
 // ---------------------------------------------------------------------------
 // spectral filters (filtering.f90:334-525)
@@ -1483,7 +1517,15 @@ void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
   g.n2 = n2;
   for (int k = 0; k < 2; ++k)
     for (int d = 0; d < 2; ++d) {
-      g.mx[k][d] = upload_matrix<U>(ar, r2r_matrix(n1, k, d == 0 ? FWD : INV));
+      std::vector<double> m = r2r_matrix(n1, k, d == 0 ? FWD : INV);
+      if (n2 == 1) {
+        // 2-D tank: a single column, done as a transposed GEMV; stored transposed
+        std::vector<double> t(m.size());
+        for (int i = 0; i < n1; ++i)
+          for (int j = 0; j < n1; ++j) t[j + (size_t)n1 * i] = m[i + (size_t)n1 * j];
+        m.swap(t);
+      }
+      g.mx[k][d] = upload_matrix<U>(ar, m);
       if (n2 > 1) g.my[k][d] = upload_matrix<U>(ar, r2r_matrix(n2, k, d == 0 ? FWD : INV));
     }
   g.stage = ar.get<U>((size_t)n1 * n2);
@@ -1501,7 +1543,7 @@ void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx,
   hipLaunchKernelGGL((k_dense_in<U, S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n1, n2, src, g.stage);
   HIP_OK(hipGetLastError());
   if (n2 == 1) {
-    gemm(blas, rocblas_operation_none, nout1, 1, n1, g.mx[kx][di], n1, g.stage, n1, d.p, d.ld);
+    gemv_t(blas, nout1, n1, g.mx[kx][di], g.stage, d.p);
     return;
   }
   gemm(blas, rocblas_operation_none, nout1, n2, n1, g.mx[kx][di], n1, g.stage, n1, g.mid, n1);
@@ -1603,6 +1645,7 @@ struct Engine : EngineBase {
   T *eta_hat, *phis_hat, *eta_phys, *P, *D1, *D2, *Q, *Zp, *Zh, *phixh, *phiyh, *phizh, *tmpS;
   // big (T)
   std::vector<T*> etapm, phizm, aHOS;
+  std::vector<T*> lvl;  // 2-D: outputs of one HOS level's transforms
   std::vector<T*> tmpX;  // x-pass intermediates of batched transforms
   T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
   // wavemaker / first order (double)
@@ -1681,6 +1724,8 @@ struct Engine : EngineBase {
       if (j >= 2) aHOS[j] = ar.get<T>(nb);
     }
     for (int i = 0; i < kMaxBatch && i < std::max(M, 5); ++i) tmpX.push_back(ar.get<T>(nb));
+    if (c->n2 == 1)
+      for (int i = 0; i < M; ++i) lvl.push_back(ar.get<T>(nb));
     etax = ar.get<T>(nb); etay = ar.get<T>(nb); phisx = ar.get<T>(nb); phisy = ar.get<T>(nb);
     g2 = ar.get<T>(nb); phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
     phiza = ar.get<T>(nb); X = ar.get<T>(nb); Y = ar.get<T>(nb); tmpB = ar.get<T>(nb);
@@ -1932,7 +1977,34 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL(k_powers<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, 2, ep_one(), etax, etay, g2);
     }
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
-    for (int iH = 1; iH <= M; ++iH) {
+    for (int iH = 1; iH <= M && n2 == 1; ++iH) {
+      // 2-D: the level's single-line transforms in one launch, then the
+      // accumulation in the reference's order (k_hos_level)
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      PtrSet<T> lv{};
+      for (int j = iH; j >= 1; --j) {
+        const int j2 = iH - j + 1;
+        lv.p[b.n] = lvl[b.n];
+        if (j2 == 1) {
+          b.add(KCOS, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, j, pix, piy), N1b, N2b,
+                Dst<T>{lvl[b.n], ldb, 0}, tmpX[b.n]);
+        } else {
+          const bool trunc = (j2 == M);
+          b.add(KCOS, KCOS,
+                src_mul<T>(aHOS[j2], trunc ? n1 : ldb, trunc ? n1 : N1b, trunc ? n2 : N2b, MUL_KTH, j, pix, piy),
+                N1b, N2b, Dst<T>{lvl[b.n], ldb, 0}, tmpX[b.n]);
+        }
+      }
+      bigB(INV, b);
+      hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, ep, phim, phizm[iH]);
+      if (iH < M) {
+        if (iH + 1 == M)
+          big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
+        else
+          big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), N1b, N2b, Dst<T>{aHOS[iH + 1], ldb, 0});
+      }
+    }
+    for (int iH = 1; iH <= M && n2 > 1; ++iH) {
       // the y passes of every order j in one launch; the x passes, whose
       // contiguous lines carry the read-modify-write epilogue, accumulate into
       // phim / phizm(iH) in the reference's order j = iH..1
