@@ -181,6 +181,65 @@ def cpu_baseline(wl=WORKLOAD):
                       f"in reference mode: {dt:.2f} s"}
 
 
+def ensemble_bench(t_next, n_members=64):
+    """The batched-ensemble workload: n_members independent 2-D irregular realisations, n1 drawn
+    from {512, 1024, 2048, 4096} and M from {3, 4, 5, 6}, JONSWAP Hs ~ U[0.03, 0.10],
+    Tp ~ U[1.2, 2.0], gamma 3.3, 512 components with seeded phases, length 60, ramp 2 Tp, mixed
+    precision, RK4(5) tolerance 1e-7; every member advanced from rest to t_next (nondimensional),
+    once member after member (hos_advance) and once together (hos_advance_ensemble)."""
+    from hosnwt.tank import Tank, advance_ensemble
+    rng = np.random.default_rng(2024)
+    members = []
+    for _ in range(n_members):
+        n1 = int(rng.choice([512, 1024, 2048, 4096]))
+        M = int(rng.choice([3, 4, 5, 6]))
+        Hs, Tp = float(rng.uniform(0.03, 0.10)), float(rng.uniform(1.2, 2.0))
+        comps = hc.jonswap_components(Hs, Tp, 3.3, n_freq=512, seed=int(rng.integers(1 << 30)), rnum=12,
+                                      clock=32.0)
+        cfg = hc.TankConfig(n1=n1, n2=1, mHOS=M, xlen=60.0, n3=33, icase=3, igeom=1, iramp=1, Tramp=2.0 * Tp,
+                            xabsf=0.8, T_stop=100 * Tp, toler=1e-7, f_out=1.0 / Tp, precision=1, components=comps)
+        members.append(hc.build(cfg))
+    h0 = [min(0.95 * s.dt_stab, s.delt_out) for s in members]
+
+    def make():
+        t0 = time.perf_counter()
+        tanks = [Tank(s) for s in members]
+        for tk in tanks:
+            tk.initialize()
+        for tk in tanks:
+            tk.synchronize()
+        print(f"[ensemble] {len(tanks)} contexts created in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+        return tanks
+
+    tanks = make()
+    t0 = time.perf_counter()
+    seq = []
+    for tk, h in zip(tanks, h0):
+        seq.append(tk.advance(t_next, h))
+    dt_seq = time.perf_counter() - t0
+    seq_att = sum(na + nr for _, na, nr in seq)
+    print(f"[ensemble] member after member: {seq_att} attempts in {dt_seq:.2f} s", file=sys.stderr, flush=True)
+    for tk in tanks:
+        tk.close()
+    tanks = make()
+    t0 = time.perf_counter()
+    _, nas, nrs = advance_ensemble(tanks, t_next, h0)
+    dt_ens = time.perf_counter() - t0
+    ens_att = sum(nas) + sum(nrs)
+    print(f"[ensemble] together: {ens_att} attempts in {dt_ens:.2f} s", file=sys.stderr, flush=True)
+    for tk in tanks:
+        tk.close()
+    assert ens_att == seq_att, (ens_att, seq_att)
+    return {"metric": f"ragged 2-D ensemble RK4(5) steps/s ({n_members} members, n1 512-4096, M 3-6)",
+            "value": round(ens_att / dt_ens, 2), "unit": "attempts/s summed over members", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "fp32 transforms, fp64 RK", "data": "synthetic (seeded JONSWAP)",
+            "t_next": t_next, "attempts": ens_att, "seconds": round(dt_ens, 3),
+            "sequential_value": round(seq_att / dt_seq, 2), "sequential_seconds": round(dt_seq, 3),
+            "ensemble_vs_sequential": round(dt_seq / dt_ens, 3),
+            "members": {f"n1={n1}": sum(s.cfg.n1 == n1 for s in members) for n1 in (512, 1024, 2048, 4096)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,9 +250,14 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--pass-probe", type=int, default=0, metavar="REPS",
                     help="only time the roofline passes REPS times and run the counter-calibration copy")
+    ap.add_argument("--ensemble", type=float, default=0.0, metavar="T",
+                    help="only run the ragged 2-D ensemble to nondimensional time T (sequential vs together)")
     args = ap.parse_args()
     if args.pass_probe:
         pass_probe(args.pass_probe)
+        return
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(args.ensemble)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "

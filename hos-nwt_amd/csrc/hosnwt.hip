@@ -101,7 +101,8 @@ inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / 
 // also partial sums of eta_hat for the mean-volume fix (resol_HOS.f90:353-364).
 template <typename T>
 __global__ void k_gtof_in(int n1, int n2, double pix, double piy, double dt, const double* __restrict__ G1,
-                          const double* __restrict__ G2, T* eta_hat, T* phis_hat, double* partial) {
+                          const double* __restrict__ G2, T* eta_hat, T* phis_hat, double* partial,
+                          double* phis64) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   double s = 0.0;
@@ -112,7 +113,9 @@ __global__ void k_gtof_in(int n1, int n2, double pix, double piy, double dt, con
     const double F1 = G1[i] * c - G2[i] * sn;
     const double F2 = G2[i] * c + G1[i] * sn;
     eta_hat[i] = (T)F1;
-    phis_hat[i] = (T)((i == 0) ? F2 : F2 / w);
+    const double ph = (i == 0) ? F2 : F2 / w;
+    phis_hat[i] = (T)ph;
+    if (phis64) phis64[i] = ph;
     s = F1;
   }
   s = block_sum(s);
@@ -184,13 +187,20 @@ __global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, PtrSet<T> ep,
   phizm[i] = pz;
 }
 
+// out = sum_{j0 <= j <= M} z_j (zero when j0 > M)
 template <typename T>
-__global__ void k_sum_orders(size_t n, int M, PtrSet<T> z, T* out) {
+__global__ void k_sum_orders(size_t n, int M, int j0, PtrSet<T> z, T* out) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  T s = z.p[1][i];
-  for (int j = 2; j <= M; ++j) s += z.p[j][i];
+  T s = (j0 <= M) ? z.p[j0][i] : (T)0;
+  for (int j = j0 + 1; j <= M; ++j) s += z.p[j][i];
   out[i] = s;
+}
+
+template <typename T>
+__global__ void k_narrow(size_t n, const double* __restrict__ in, T* __restrict__ out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)in[i];
 }
 
 // Free-surface products on the extended grid (resol_HOS.f90:376-392, :534-555 and
@@ -274,23 +284,27 @@ __global__ void k_add3(size_t n, const T* a, const T* b, T* out) {
   if (i < n) out[i] = a[i] + b[i];
 }
 
-// dphin (physical) = F2S(D1+P) + phiz_add ; Z = -phit_add - nu dphin
-template <typename T>
-__global__ void k_dphin(size_t n, const T* __restrict__ fq, const double* __restrict__ phiz_add,
-                        const double* __restrict__ phit_add, const double* __restrict__ nu, T* Z) {
+// dphin (physical) = F2S(D1+P) + phiz_add ; Z = -phit_add - nu dphin.  With
+// wlin (physical first-order vertical velocity, FP64), P holds only orders >= 2
+// and dphin = F2S(D1+P) + wlin + phiz_add.
+template <typename T, typename TO>
+__global__ void k_dphin(size_t n, const T* __restrict__ fq, const double* __restrict__ wlin,
+                        const double* __restrict__ phiz_add, const double* __restrict__ phit_add,
+                        const double* __restrict__ nu, TO* Z) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double dphin = (double)fq[i] + phiz_add[i];
-  Z[i] = (T)(-phit_add[i] - (nu ? nu[i] : 0.0) * dphin);
+  const double dphin = wlin ? (double)fq[i] + wlin[i] + phiz_add[i] : (double)fq[i] + phiz_add[i];
+  Z[i] = (TO)(-phit_add[i] - (nu ? nu[i] : 0.0) * dphin);
 }
 
 // NL1 = D1 + P - phis_hat kth1 + S2F(phiz_add) ; NL2 = D2 + S2F(Z) ; expon
 // (resol_HOS.f90:623-644, variablechange.f90:37-80); k_G2(1,1) = 0
-// (runge_kutta.f90:150,199,205).
-template <typename T>
+// (runge_kutta.f90:150,199,205).  phis_hat null: P holds only orders >= 2, whose
+// first-order part P(1) = kth1 phis_hat cancels the subtracted term exactly.
 // filt: optional mode weights of the free-surface filters (resol_HOS.f90:613-621).
+template <typename T, typename TL>
 __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, const T* D1, const T* P,
-                            const T* phis_hat, const T* phizh, const T* D2, const T* Zh, const double* filt,
+                            const T* phis_hat, const TL* phizh, const T* D2, const TL* Zh, const double* filt,
                             double* dG1, double* dG2) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -300,7 +314,8 @@ __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, c
   const double w = wave_omega(k);
   const double kth1 = k * tanh(k);
   const double f = filt ? filt[i] : 1.0;
-  const double nl1 = ((D1 ? (double)D1[i] : 0.0) + (double)P[i] - (double)phis_hat[i] * kth1 + (double)phizh[i]) * f;
+  const double lin = phis_hat ? (double)phis_hat[i] * kth1 : 0.0;
+  const double nl1 = ((D1 ? (double)D1[i] : 0.0) + (double)P[i] - lin + (double)phizh[i]) * f;
   const double nl2 = ((D2 ? (double)D2[i] : 0.0) + (double)Zh[i]) * f * w;
   const double c = cos(w * dt), s = sin(w * dt);
   dG1[i] = nl1 * c - nl2 * s;
@@ -1465,17 +1480,25 @@ std::vector<double> r2r_matrix(int N, int kind, int dir) {
     return m;
   }
   const long P = 2L * (N - 1);
-  for (int c = 0; c < N; ++c)
-    for (int r = 0; r < N; ++r) {
-      const double ang = M_PI * (double)(((long)r * c) % P) / (double)(N - 1);
-      const bool er = (r == 0 || r == N - 1), ec = (c == 0 || c == N - 1);
+  // the angle depends on r c mod P only: one trigonometric evaluation per residue
+  std::vector<double> tab(P);
+  for (long k = 0; k < P; ++k) {
+    const double ang = M_PI * (double)k / (double)(N - 1);
+    tab[k] = (kind == KCOS) ? std::cos(ang) : std::sin(ang);
+  }
+  for (int c = 0; c < N; ++c) {
+    const bool ec = (c == 0 || c == N - 1);
+    long k = 0;  // r c mod P, stepped by c
+    for (int r = 0; r < N; ++r, k = (k + c) % P) {
+      const bool er = (r == 0 || r == N - 1);
       double v;
       if (kind == KCOS)
-        v = (dir == FWD) ? (er ? 1.0 : 2.0) / (double)P * (ec ? 1.0 : 2.0) * std::cos(ang) : std::cos(ang);
+        v = (dir == FWD) ? (er ? 1.0 : 2.0) / (double)P * (ec ? 1.0 : 2.0) * tab[k] : tab[k];
       else
-        v = (er || ec) ? 0.0 : ((dir == FWD) ? 2.0 / (double)(N - 1) : 1.0) * std::sin(ang);
+        v = (er || ec) ? 0.0 : ((dir == FWD) ? 2.0 / (double)(N - 1) : 1.0) * tab[k];
       m[r + (size_t)N * c] = v;
     }
+  }
   return m;
 }
 
@@ -1608,6 +1631,10 @@ struct hos_ctx {
   bool wmk3 = false;
   double *a3rd_add = nullptr, *a3rd_add_t = nullptr, *aad3m = nullptr, *kad3[5] = {};
   double *partial, *pmax_e, *pmax_x, *red6;
+  double* red_host = nullptr;  // pinned copy of red6 (the attempt's error maxima)
+  ~hos_ctx() {
+    if (red_host) (void)hipHostFree(red_host);
+  }
   double *phys_tmp, *nu_d;
   double* filt_fs = nullptr;  // free-surface mode filter weights (n1*n2), null when off
   double* filt_z = nullptr;   // paddle z-mode filter weights (n3_add), null when off
@@ -1653,6 +1680,14 @@ struct Engine : EngineBase {
   int nch = 1;
   double *phi_add, *phix_add, *phiy_add, *phiz_add, *phit_add, *phit1, *phiz1, *w1, *tmp1;
   T *phix_addT, *phiy_addT, *phiz_addT;
+  // Mixed path: the first-order terms of the free-surface conditions (the linear
+  // vertical velocity kth1 phis_hat and the paddle forcing phiz_add, phit_add)
+  // stay in FP64, so the FP32 rounding enters only through the products of
+  // order >= 2.  From rest the fields are first-order, and an FP32 linear part
+  // would set an h-independent floor (~2.5e-7 relative) under the embedded
+  // error estimate that a 1e-7 tolerance never passes.
+  static constexpr bool kSplit = sizeof(T) != sizeof(double);
+  double *phis64 = nullptr, *wlin = nullptr, *Z64 = nullptr, *Zh64 = nullptr, *phizh64 = nullptr;
   Cont cont;
   // mixed-parity products: spectra, conversion matrices, results
   int K1 = 0, K2 = 0;
@@ -1715,6 +1750,10 @@ struct Engine : EngineBase {
     Zp = ar.get<T>(ns); Zh = ar.get<T>(ns); phixh = ar.get<T>(ns); phiyh = ar.get<T>(ns);
     phizh = ar.get<T>(ns); tmpS = ar.get<T>(ns);
     phix_addT = ar.get<T>(ns); phiy_addT = ar.get<T>(ns); phiz_addT = ar.get<T>(ns);
+    if (kSplit) {
+      phis64 = ar.get<double>(ns); wlin = ar.get<double>(ns); Z64 = ar.get<double>(ns);
+      Zh64 = ar.get<double>(ns); phizh64 = ar.get<double>(ns);
+    }
     etapm.assign(M + 1, nullptr);
     phizm.assign(M + 1, nullptr);
     aHOS.assign(M + 1, nullptr);
@@ -1935,7 +1974,7 @@ struct Engine : EngineBase {
     // --- GtoF, /omega, volume fix (uses the previous wavemaker evaluation) ---
     const unsigned nbs = nblk(ns);
     hipLaunchKernelGGL(k_gtof_in<T>, dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt, G1, G2, eta_hat,
-                       phis_hat, C.partial);
+                       phis_hat, C.partial, phis64);
     hipLaunchKernelGGL(k_volfix<T>, dim3(1), dim3(256), 0, st, (int)nbs, C.partial, eta_hat, (double*)nullptr,
                        C.volmode(), C.last_alpha, C.last_pos_top, C.p.xlen, C.p.d_hinge);
     // --- physical eta on the base grid ---
@@ -2031,10 +2070,10 @@ struct Engine : EngineBase {
           big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), N1b, N2b, Dst<T>{aHOS[iH + 1], ldb, 0});
       }
     }
-    // phiz = filter_ext(sum_j phizm(j)) -> modal P
+    // phiz = filter_ext(sum_j phizm(j)) -> modal P (orders >= 2 on the mixed path)
     PtrSet<T> zz{};
     for (int j = 1; j <= M; ++j) zz.p[j] = phizm[j];
-    hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, X);
+    hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, kSplit ? 2 : 1, zz, X);
     big(KCOS, KCOS, FWD, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0});
     // --- join the wavemaker chain (solveHOS_SWEET_1 + phiadd_SL) ---
     HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
@@ -2059,12 +2098,26 @@ struct Engine : EngineBase {
     }
     // dphin and the final base-grid RHS
     base(KCOS, KCOS, INV, src_plain<T>(Q, n1, n1, n2), n1, n2, Dst<T>{tmpS, n1, 0});
-    hipLaunchKernelGGL(k_dphin<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)phiz_add,
-                       (const double*)phit_add, (const double*)C.nu_d, Zp);
-    base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
-    hipLaunchKernelGGL(k_expon_out<T>, dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
-                       (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
-                       (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)C.filt_fs, kG1, kG2);
+    if constexpr (kSplit) {
+      // first-order vertical velocity in FP64; Z and its modes in FP64
+      base64(KCOS, KCOS, INV, src_mul<double, double>(phis64, n1, n1, n2, MUL_KTH, 1, pix, piy), n1, n2,
+             Dst<double>{wlin, n1, 0});
+      hipLaunchKernelGGL((k_dphin<T, double>), dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS,
+                         (const double*)wlin, (const double*)phiz_add, (const double*)phit_add,
+                         (const double*)C.nu_d, Z64);
+      base64(KCOS, KCOS, FWD, src_plain<double, double>(Z64, n1, n1, n2), n1, n2, Dst<double>{Zh64, n1, 0});
+      hipLaunchKernelGGL((k_expon_out<T, double>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
+                         (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)nullptr,
+                         (const double*)phizh64, (const T*)(M >= 2 ? D2 : nullptr), (const double*)Zh64,
+                         (const double*)C.filt_fs, kG1, kG2);
+    } else {
+      hipLaunchKernelGGL((k_dphin<T, T>), dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)nullptr,
+                         (const double*)phiz_add, (const double*)phit_add, (const double*)C.nu_d, Zp);
+      base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
+      hipLaunchKernelGGL((k_expon_out<T, T>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
+                         (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
+                         (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)C.filt_fs, kG1, kG2);
+    }
     HIP_OK(hipMemcpyAsync(C.phi_add_last, phi_add, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
 
@@ -2107,6 +2160,7 @@ struct Engine : EngineBase {
       HIP_OK(hipMemsetAsync(phiz_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phit_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phizh, 0, ns * sizeof(T), st));
+      if (phizh64) HIP_OK(hipMemsetAsync(phizh64, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phixa, 0, nb * sizeof(T), st));
       HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
       HIP_OK(hipMemsetAsync(phiza, 0, nb * sizeof(T), st));
@@ -2143,7 +2197,7 @@ struct Engine : EngineBase {
     // grad phi_add onto the extended grid (resol_HOS.f90:534-544)
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
-    base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
+    phiz_modes();
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
@@ -2151,6 +2205,18 @@ struct Engine : EngineBase {
       b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
       bigB(INV, b);
       if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+    }
+  }
+
+  // modes of phiz_add in T (the extended-grid gradient products); on the mixed
+  // path from the FP64 modes, which the free-surface condition uses directly
+  void phiz_modes() {
+    const int n1 = c->n1, n2 = c->n2;
+    if constexpr (kSplit) {
+      base64(KCOS, KCOS, FWD, src_plain<double, double>(phiz_add, n1, n1, n2), n1, n2, Dst<double>{phizh64, n1, 0});
+      hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, cs, ns, (const double*)phizh64, phizh);
+    } else {
+      base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
     }
   }
 
@@ -2305,7 +2371,7 @@ struct Engine : EngineBase {
     // conditions above have used the first-order values already.
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
-    base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
+    phiz_modes();
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
@@ -2466,7 +2532,9 @@ void rhs_eval(hos_ctx* c, double t, double dt, const double* G1, const double* G
               c->wmk2 ? &s2 : nullptr);
 }
 
-void rk_attempt(hos_ctx* c, double t, double h, double* err) {
+// Enqueues one runge4_adapt2 attempt and the readback of its error maxima on
+// the context's stream, without waiting (rk_finish completes it).
+void rk_enqueue(hos_ctx* c, double t, double h) {
   const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
   hipStream_t st = c->st;
   hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
@@ -2554,10 +2622,15 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   hipLaunchKernelGGL(k_volfix<double>, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->partial,
                      (double*)nullptr, c->eta_hat_t, c->volmode(), c->last_alpha, c->last_pos_top, c->p.xlen,
                      c->p.d_hinge);
-  double r[10];
   const int nr = c->wmk3 ? 10 : (c->wmk2 ? 8 : 6);
-  HIP_OK(hipMemcpyAsync(r, c->red6, nr * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIP_OK(hipStreamSynchronize(st));
+  HIP_OK(hipMemcpyAsync(c->red_host, c->red6, nr * sizeof(double), hipMemcpyDeviceToHost, st));
+  c->have_trial = false;
+}
+
+// Waits for the enqueued attempt and forms the embedded error (runge_kutta.f90:272-279).
+void rk_finish(hos_ctx* c, double* err) {
+  HIP_OK(hipStreamSynchronize(c->st));
+  const double* r = c->red_host;
   const double tiny = 2.220446049250313e-16;
   double e = r[0] / (r[1] + tiny);
   e = std::max(e, r[2] / (r[3] + tiny));
@@ -2566,6 +2639,44 @@ void rk_attempt(hos_ctx* c, double t, double h, double* err) {
   if (c->wmk3) e = std::max(e, r[8] / (r[9] + tiny));
   *err = e;
   c->have_trial = true;
+}
+
+void rk_attempt(hos_ctx* c, double t, double h, double* err) {
+  rk_enqueue(c, t, h);
+  rk_finish(c, err);
+}
+
+void commit(hos_ctx* c);
+
+// The accept / reject controller of the inner loop (HOS-NWT.f90:518-563) for one
+// finished attempt of length hl from t; updates t, h and the counters.
+struct Advance {
+  double t, h;
+  int na, nr;
+};
+void control(hos_ctx* c, Advance& a, double hl, double err, double toler, double dt_stab, double delt_out) {
+  const double tiny = 2.220446049250313e-16;
+  const hos_params& p = c->p;
+  double corr;
+  if (err > toler && a.h > 0.001 * 0.95 * dt_stab) {
+    corr = 0.9 * std::pow(err / toler, -(1.0 / 4.0));
+    ++a.nr;
+    if (std::fabs(p.Tramp - (a.t + hl)) <= tiny && p.Tramp > a.t) c->endramp = 0;
+    c->have_trial = false;
+  } else {
+    commit(c);
+    if (p.iramp == 1 && p.Tramp - (a.t + hl) <= tiny && p.Tramp > a.t && c->endramp != 1) {
+      const size_t n = (size_t)c->n3a * c->n2;
+      hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, c->st, n, c->a1st_add, (const double*)c->a_add_ramp);
+      c->endramp = 1;
+    }
+    a.t = a.t + hl;
+    corr = (std::fabs(err) < tiny) ? 2.0 : 0.9 * std::pow(err / toler, -(1.0 / 4.0));
+    ++a.na;
+  }
+  a.h = a.h * corr;
+  a.h = std::min({a.h, 0.95 * dt_stab, delt_out});
+  a.h = std::max({a.h, 0.0001 * 0.95 * dt_stab, 0.0001 * delt_out});
 }
 
 void commit(hos_ctx* c) {
@@ -2711,6 +2822,7 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     c->pmax_e = ar.get<double>(nbk);
     c->pmax_x = ar.get<double>(nbk);
     c->red6 = ar.get<double>(10);
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&c->red_host), 16 * sizeof(double), hipHostMallocDefault));
     c->phys_tmp = ar.get<double>(ns);
     c->phi_add_last = ar.get<double>(ns);
     c->nu_d = nullptr;
@@ -2905,41 +3017,62 @@ int hos_end_ramp(hos_ctx* c) {
 int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
                 double delt_out, int* n_acc, int* n_rej) {
   return guard([&] {
-    const double tiny = 2.220446049250313e-16;
-    double t = *t_cur, h = *h_rk;
-    int na = 0, nr = 0;
-    const hos_params& p = c->p;
-    while (t < t_next) {
-      double hl = h;
-      if (t_next - t < h) hl = t_next - t;
+    Advance a{*t_cur, *h_rk, 0, 0};
+    while (a.t < t_next) {
+      double hl = a.h;
+      if (t_next - a.t < a.h) hl = t_next - a.t;
       double err = 0.0;
-      rk_attempt(c, t, hl, &err);
-      double corr;
-      if (err > toler && h > 0.001 * 0.95 * dt_stab) {
-        corr = 0.9 * std::pow(err / toler, -(1.0 / 4.0));
-        ++nr;
-        if (std::fabs(p.Tramp - (t + hl)) <= tiny && p.Tramp > t) c->endramp = 0;
-        c->have_trial = false;
-      } else {
-        commit(c);
-        if (p.iramp == 1 && p.Tramp - (t + hl) <= tiny && p.Tramp > t && c->endramp != 1) {
-          const size_t n = (size_t)c->n3a * c->n2;
-          hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, c->st, n, c->a1st_add, (const double*)c->a_add_ramp);
-          c->endramp = 1;
-        }
-        t = t + hl;
-        corr = (std::fabs(err) < tiny) ? 2.0 : 0.9 * std::pow(err / toler, -(1.0 / 4.0));
-        ++na;
-      }
-      h = h * corr;
-      h = std::min({h, 0.95 * dt_stab, delt_out});
-      h = std::max({h, 0.0001 * 0.95 * dt_stab, 0.0001 * delt_out});
+      rk_attempt(c, a.t, hl, &err);
+      control(c, a, hl, err, toler, dt_stab, delt_out);
     }
     HIP_OK(hipStreamSynchronize(c->st));
-    *t_cur = t;
-    *h_rk = h;
-    if (n_acc) *n_acc = na;
-    if (n_rej) *n_rej = nr;
+    *t_cur = a.t;
+    *h_rk = a.h;
+    if (n_acc) *n_acc = a.na;
+    if (n_rej) *n_rej = a.nr;
+  });
+}
+
+int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
+                         const double* dt_stab, const double* delt_out, int* n_acc, int* n_rej) {
+  return guard([&] {
+    if (n < 0) throw std::runtime_error("negative ensemble size");
+    std::vector<Advance> a(n);
+    std::vector<double> hl(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+      if (!ctx[i]) throw std::runtime_error("null context in the ensemble");
+      for (int j = 0; j < i; ++j)
+        if (ctx[j] == ctx[i]) throw std::runtime_error("a context appears twice in the ensemble");
+      a[i] = Advance{t_cur[i], h_rk[i], 0, 0};
+    }
+    auto launch = [&](int i) {
+      hl[i] = (t_next - a[i].t < a[i].h) ? t_next - a[i].t : a[i].h;
+      rk_enqueue(ctx[i], a[i].t, hl[i]);
+    };
+    // every member's attempt is in flight on its own streams before the first wait; a member's
+    // next attempt is enqueued as soon as its decision is taken, while the others still run
+    for (int i = 0; i < n; ++i)
+      if (a[i].t < t_next) launch(i);
+    for (bool any = true; any;) {
+      any = false;
+      for (int i = 0; i < n; ++i) {
+        if (!(a[i].t < t_next)) continue;
+        double err = 0.0;
+        rk_finish(ctx[i], &err);
+        control(ctx[i], a[i], hl[i], err, toler, dt_stab[i], delt_out[i]);
+        if (a[i].t < t_next) {
+          launch(i);
+          any = true;
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      HIP_OK(hipStreamSynchronize(ctx[i]->st));
+      t_cur[i] = a[i].t;
+      h_rk[i] = a[i].h;
+      if (n_acc) n_acc[i] = a[i].na;
+      if (n_rej) n_rej[i] = a[i].nr;
+    }
   });
 }
 

@@ -44,7 +44,7 @@ EXPORTS = [
     "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
     "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_check_slope",
     "hos_set_state2", "hos_get_state2", "hos_rhs2", "hos_set_state3", "hos_get_state3", "hos_rhs3",
-    "hos_get_wmk_motion", "hos_get_modes", "hos_probes",
+    "hos_get_wmk_motion", "hos_get_modes", "hos_probes", "hos_advance_ensemble",
 ]
 
 _lib = None
@@ -70,6 +70,8 @@ def load() -> C.CDLL:
     lib.hos_end_ramp.argtypes = [vp]
     lib.hos_advance.argtypes = [vp, _dp, C.c_double, _dp, C.c_double, C.c_double, C.c_double,
                                 C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.hos_advance_ensemble.argtypes = [C.c_int, C.POINTER(vp), _dp, C.c_double, _dp, C.c_double, _dp, _dp,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.hos_rhs.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 11
     lib.hos_rhs2.argtypes = [vp, C.c_double, C.c_double] + [_dp] * 17
     lib.hos_set_state2.argtypes = [vp] + [_dp] * 3

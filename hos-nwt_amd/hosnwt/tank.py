@@ -270,3 +270,26 @@ class Tank:
             t_next = min(self.time + delt_out, T_stop + delt_out)
             delt, _, _ = self.advance(t_next, delt)
         return records
+
+
+def advance_ensemble(tanks, t_next: float, h_rk):
+    """hos_advance_ensemble: every tank to t_next, the members' attempts overlapping on the GPU.
+    Returns (h proposals, accepted counts, rejected counts); each tank's time is updated."""
+    n = len(tanks)
+    lib = tanks[0].lib
+    ctx = (C.c_void_p * n)(*[t.h.value for t in tanks])
+    t = np.array([tk.time for tk in tanks], dtype=np.float64)
+    h = np.array(h_rk, dtype=np.float64)
+    stab = np.array([tk.s.dt_stab for tk in tanks], dtype=np.float64)
+    dout = np.array([tk.s.delt_out for tk in tanks], dtype=np.float64)
+    na = (C.c_int * n)()
+    nr = (C.c_int * n)()
+    toler = min(tk.cfg.toler for tk in tanks)
+    if any(tk.cfg.toler != toler for tk in tanks):
+        raise ValueError("the ensemble shares one RK tolerance")
+    check(lib.hos_advance_ensemble(n, ctx, ptr(t), t_next, ptr(h), toler, ptr(stab), ptr(dout), na, nr))
+    for i, tk in enumerate(tanks):
+        tk.time = float(t[i])
+        tk.n_rk_tot += na[i]
+        tk.n_er_tot += nr[i]
+    return h, list(na), list(nr)

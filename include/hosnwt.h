@@ -114,6 +114,14 @@ int hos_end_ramp(hos_ctx* c);
  * Counts accepted/rejected attempts. */
 int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
                 double delt_out, int* n_acc, int* n_rej);
+/* hos_advance for n independent contexts (an ensemble of realisations of
+ * different sizes and orders) to a common t_next: every member's next attempt
+ * is enqueued on its own streams before the host waits for any of them, so
+ * the members' kernels overlap on the GPU.  Arrays of n: t_cur, h_rk (in/out),
+ * dt_stab, delt_out, n_acc, n_rej (may be NULL).  Each member takes exactly
+ * the decisions hos_advance would take for it alone (HOS-NWT.f90:498-571). */
+int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
+                         const double* dt_stab, const double* delt_out, int* n_acc, int* n_rej);
 
 /* Second-order wavemaker state (i_wmk >= 2): a2nd_add (n3_add*n2, modal),
  * a2nd (n1*n2, modal), eta2nd (n1*n2, physical); the driver's arrays of

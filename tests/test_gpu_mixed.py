@@ -1,0 +1,127 @@
+"""Mixed path (FP32 transforms and products, FP64 RK state, error estimate and
+step control) against the FP64 oracle and the FP64 path.
+
+* RHS at O(1e-2) amplitudes: relative L2 <= 1e-5 against the oracle (2-D piston,
+  2-D flap JONSWAP, 3-D directional).
+* The first-order terms of the free-surface conditions (the linear vertical
+  velocity and the paddle forcing) are computed in FP64 on the mixed path, so
+  its rounding error scales with the products of order >= 2: near rest (state
+  amplitude 1e-5, 5 % into the ramp) the RHS must be within 1e-7 of the oracle
+  and at least 20x closer than at amplitude 1e-2 (with an FP32 linear part the
+  relative error is amplitude-independent, 1e-6 to 1e-5 at these sizes).
+* The adaptive loop from rest at tolerance 1e-7: the mixed context, with its
+  own error estimates and decisions, must reach the target time in about the
+  attempts the FP64 context takes, with eta and phis within 1e-5 relative L2.
+  (An FP32 linear part gives an h-independent error ratio ~2.5e-7 near rest,
+  so every attempt is rejected until h reaches its floor.)
+"""
+import numpy as np
+import pytest
+
+from helpers import make_setup, oracle_params, random_state, rel
+from hosnwt import config as hc
+from hosnwt.tank import Tank
+from oracle import hos_oracle as ho
+
+CASES = {
+    "2d_piston_M3": dict(n1=128, n2=1, M=3, n3=17, xlen=20.0),
+    "2d_flap_jonswap_M6": dict(n1=256, n2=1, M=6, n3=17, xlen=30.0, igeom=2, d_hinge=0.3, Tramp=3.2,
+                               spec=dict(Hs=0.08, Tp=1.6, n_freq=32, n_dir=1)),
+    "3d_directional_M5": dict(n1=48, n2=24, M=5, n3=9, xlen=10.0, ylen=5.0, Tramp=2.8,
+                              spec=dict(Hs=0.06, Tp=1.4, n_freq=8, n_dir=4)),
+}
+
+
+def setup_for(name, **extra):
+    kw = dict(CASES[name], **extra)
+    spec = kw.pop("spec", None)
+    if spec is not None:
+        kw["components"] = hc.jonswap_components(spec["Hs"], spec["Tp"], 3.3, n_freq=spec["n_freq"], seed=21,
+                                                 rnum=8, clock=16.0, n_dir=spec["n_dir"], spread_s=10)
+        kw["icase"] = 3
+        kw["f_out"] = 1.0 / spec["Tp"]
+    return make_setup(**kw)
+
+
+def rhs_errors(s, orc, tank, amp, frac):
+    """One solveHOS2 on both; the oracle and the context see the same sequence of
+    calls (the volume fix uses the previous call's wavemaker values)."""
+    t = frac * s.Tramp
+    G1, G2, a1, e1, aad = random_state(s, 3, amp=amp)
+    orc.time = t
+    r = orc.solve_hos2(a1, e1, G2, G1, aad, 0.01)
+    g = tank.solve_hos2(t, 0.01, G1, G2, a1, e1, aad)
+    ref_dG2 = r[1].copy()
+    ref_dG2[0, 0] = 0.0
+    return {"dG1": rel(g["dG1"], r[0]), "dG2": rel(g["dG2"], ref_dG2), "da1st": rel(g["da1st"], r[2]),
+            "deta1st": rel(g["deta1st"], r[3]), "da_add_1": rel(g["da_add_1"], r[4])}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_rhs_error_scales_with_the_nonlinear_part(name):
+    s = setup_for(name)
+    orc = ho.Oracle(oracle_params(s))
+    tank = Tank(s, precision=1)
+    try:
+        big = None
+        for amp, frac, tol in ((0.01, 0.6, 1e-5), (1e-5, 0.05, 1e-7)):
+            errs = rhs_errors(s, orc, tank, amp, frac)
+            print(name, f"amp={amp:g}", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+            for k, v in errs.items():
+                assert v < tol, (name, amp, k, v)
+            if big is None:
+                big = errs
+            else:
+                assert errs["dG1"] < big["dG1"] / 20 and errs["dG2"] < big["dG2"] / 20
+    finally:
+        tank.close()
+
+
+def run_from_rest(s, precision, t_next, max_attempts):
+    """The reference controller (HOS-NWT.f90:518-563) with a cap on the attempts."""
+    tank = Tank(s, precision=precision)
+    tank.initialize()
+    t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+    toler, stab = s.cfg.toler, s.dt_stab
+    n = 0
+    errs = []
+    while t < t_next and n < max_attempts:
+        hl = min(h, t_next - t)
+        err = tank.runge4_adapt2(t, hl)
+        errs.append(err)
+        n += 1
+        if err > toler and h > 0.001 * 0.95 * stab:
+            corr = 0.9 * (err / toler) ** -0.25
+        else:
+            tank.commit()
+            if s.cfg.iramp == 1 and s.Tramp - (t + hl) <= 2.2e-16 and s.Tramp > t:
+                tank.end_ramp()
+            t += hl
+            corr = 2.0 if err < 2.2e-16 else 0.9 * (err / toler) ** -0.25
+        h = max(min(h * corr, 0.95 * stab, s.delt_out), 1e-4 * 0.95 * stab, 1e-4 * s.delt_out)
+    return tank, t, n, errs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_adaptive_run_from_rest(name):
+    s = setup_for(name, toler=1e-7)
+    t_next = 0.5 * s.Tramp
+    tank64, t64, n64, e64 = run_from_rest(s, 0, t_next, 2000)
+    try:
+        assert t64 == pytest.approx(t_next, abs=1e-12)
+        tank32, t32, n32, e32 = run_from_rest(s, 1, t_next, 3 * n64 + 20)
+        try:
+            print(name, f"attempts FP64 {n64} mixed {n32}; first errors FP64 {e64[:4]} mixed {e32[:4]}", flush=True)
+            assert t32 == pytest.approx(t_next, abs=1e-12), (t32, n32)
+            assert n32 <= n64 + max(3, n64 // 10)
+            g64, g32 = tank64.physical(), tank32.physical()
+            r_eta, r_phis = rel(g32["eta"], g64["eta"]), rel(g32["phis"], g64["phis"])
+            print(name, f"eta rel {r_eta:.2e} phis rel {r_phis:.2e}", flush=True)
+            assert np.abs(g64["eta"]).max() > 0
+            assert r_eta < 1e-5 and r_phis < 1e-5
+        finally:
+            tank32.close()
+    finally:
+        tank64.close()
