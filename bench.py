@@ -44,61 +44,59 @@ def make_setup(precision: int, wl=WORKLOAD) -> hc.Setup:
     return hc.build(cfg)
 
 
-def run_steps(tank, s, n, t0, h0, log=None, replay=None):
-    """n attempts with the reference accept/reject rule (HOS-NWT.f90:518-563).
+def run_steps(tank, s, n, t0, h0, t_stop=None):
+    """At most n attempts with the reference accept/reject rule (HOS-NWT.f90:518-563).
 
-    ``replay``: list of (h, accept) decisions to repeat instead (same physical times).
+    ``t_stop``: stop there (the last step shortened to land on it, as the inner
+    loop does at an output time).
     """
     t, h = t0, h0
     toler, stab = s.cfg.toler, s.dt_stab
     n_acc = n_rej = 0
-    for i in range(n):
-        if replay is not None:
-            h, acc = replay[i]
-            tank.runge4_adapt2(t, h)
-            if acc:
-                tank.commit()
-                t += h
-                n_acc += 1
-            else:
-                n_rej += 1
-            continue
-        err = tank.runge4_adapt2(t, h)
+    for _ in range(n):
+        if t_stop is not None and t >= t_stop:
+            break
+        hl = h if t_stop is None else min(h, t_stop - t)
+        err = tank.runge4_adapt2(t, hl)
         acc = not (err > toler and h > 0.001 * 0.95 * stab)
-        if log is not None:
-            log.append((h, acc))
         if not acc:
             corr = 0.9 * (err / toler) ** -0.25
             n_rej += 1
         else:
             tank.commit()
-            t += h
+            t += hl
             corr = 2.0 if err < 2.2e-16 else 0.9 * (err / toler) ** -0.25
             n_acc += 1
         h = max(min(h * corr, 0.95 * stab, s.delt_out), 1e-4 * 0.95 * stab, 1e-4 * s.delt_out)
     return t, h, n_acc, n_rej
 
 
-def time_path(precision, steps, warmup, spin_up, replay=None):
+def time_path(precision, steps, warmup, spin_up, follow=None):
+    """``follow`` = (t_warm, t_end) of an earlier run: run freely (own controller) to t_warm
+    untimed, then time the attempts that reach t_end."""
     from hosnwt.tank import Tank
     s = make_setup(precision)
     tank = Tank(s)
     tank.initialize()
     h0 = min(0.95 * s.dt_stab, s.delt_out)
     n0 = spin_up + warmup
-    log = [] if replay is None else None
-    t, h, _, _ = run_steps(tank, s, n0, 0.0, h0, log=log, replay=None if replay is None else replay[:n0])
+    cap = 10 * (n0 + steps) + 100
+    t, h, _, _ = run_steps(tank, s, cap if follow else n0, 0.0, h0,
+                           t_stop=follow[0] if follow else None)
+    t_warm = t
     tank.synchronize()
     st0 = tank.stats()
     t_start = time.perf_counter()
-    t, h, na, nr = run_steps(tank, s, steps, t, h, log=log, replay=None if replay is None else replay[n0:])
+    t, h, na, nr = run_steps(tank, s, cap if follow else steps, t, h, t_stop=follow[1] if follow else None)
     tank.synchronize()
     dt = time.perf_counter() - t_start
+    steps = na + nr
     st1 = tank.stats()
     eta = tank.get_state()["eta"]
-    out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, n_acc=na, n_rej=nr, t_end=t, eta=eta,
+    out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, n_acc=na, n_rej=nr, t_warm=t_warm, t_end=t,
+               eta=eta,
                big_per_step=(st1["n_big"] - st0["n_big"]) / steps, base_per_step=(st1["n_base"] - st0["n_base"]) / steps,
-               big_grid=(st1["big_n1"], st1["big_n2"]), tank=tank, setup=s, log=log)
+               big_grid=(st1["big_n1"], st1["big_n2"]), tank=tank, setup=s)
     return out
 
 
@@ -275,7 +273,8 @@ def main():
     r32 = None
     if not args.fp64_only:
         print("[bench] mixed path", file=sys.stderr, flush=True)
-        r32 = time_path(1, args.steps, args.warmup, args.spin_up, replay=r64["log"])
+        # free-running with its own error estimates and decisions, to the FP64 run's times
+        r32 = time_path(1, args.steps, args.warmup, args.spin_up, follow=(r64["t_warm"], r64["t_end"]))
         print(f"[bench] mixed {r32['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     eta_err = None
     if r32 is not None:
@@ -311,6 +310,8 @@ def main():
         "mixed_vs_fp64": round(r32["steps_per_s"] / r64["steps_per_s"], 3) if r32 else None,
         "mixed_eta_rel_err": eta_err,
         "accepted_rejected": [r64["n_acc"], r64["n_rej"]],
+        "mixed_accepted_rejected": [r32["n_acc"], r32["n_rej"]] if r32 else None,
+        "simulated_time": [r64["t_warm"], r64["t_end"]],
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -22,9 +22,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -284,28 +288,40 @@ __global__ void k_add3(size_t n, const T* a, const T* b, T* out) {
   if (i < n) out[i] = a[i] + b[i];
 }
 
-// dphin (physical) = F2S(D1+P) + phiz_add ; Z = -phit_add - nu dphin.  With
-// wlin (physical first-order vertical velocity, FP64), P holds only orders >= 2
-// and dphin = F2S(D1+P) + wlin + phiz_add.
-template <typename T, typename TO>
-__global__ void k_dphin(size_t n, const T* __restrict__ fq, const double* __restrict__ wlin,
-                        const double* __restrict__ phiz_add, const double* __restrict__ phit_add,
-                        const double* __restrict__ nu, TO* Z) {
+// dphin (physical) = F2S(D1+P) + phiz_add ; Z = -phit_add - nu dphin
+template <typename T>
+__global__ void k_dphin(size_t n, const T* __restrict__ fq, const double* __restrict__ phiz_add,
+                        const double* __restrict__ phit_add, const double* __restrict__ nu, T* Z) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double dphin = wlin ? (double)fq[i] + wlin[i] + phiz_add[i] : (double)fq[i] + phiz_add[i];
-  Z[i] = (TO)(-phit_add[i] - (nu ? nu[i] : 0.0) * dphin);
+  const double dphin = (double)fq[i] + phiz_add[i];
+  Z[i] = (T)(-phit_add[i] - (nu ? nu[i] : 0.0) * dphin);
+}
+
+// Mixed path, the same Z in two parts: the first-order part
+// -phit_add - nu (phiz_add + wlin), wlin = F2S(kth1 phis_hat) (FP64), and the
+// beach term of the orders >= 2, -nu F2S(D1 + P) (type T, P without order 1).
+template <typename T>
+__global__ void k_beach(size_t n, const T* __restrict__ fq, const double* __restrict__ nu, T* Z) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) Z[i] = (T)(-(nu ? nu[i] : 0.0) * (double)fq[i]);
+}
+__global__ void k_forcing(size_t n, const double* __restrict__ phiz_add, const double* __restrict__ phit_add,
+                          const double* __restrict__ wlin, const double* __restrict__ nu, double* Z) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) Z[i] = -phit_add[i] - (nu ? nu[i] : 0.0) * (phiz_add[i] + wlin[i]);
 }
 
 // NL1 = D1 + P - phis_hat kth1 + S2F(phiz_add) ; NL2 = D2 + S2F(Z) ; expon
 // (resol_HOS.f90:623-644, variablechange.f90:37-80); k_G2(1,1) = 0
 // (runge_kutta.f90:150,199,205).  phis_hat null: P holds only orders >= 2, whose
-// first-order part P(1) = kth1 phis_hat cancels the subtracted term exactly.
+// first-order part P(1) = kth1 phis_hat cancels the subtracted term exactly;
+// Zh2: optional second part of S2F(Z) (FP64).
 // filt: optional mode weights of the free-surface filters (resol_HOS.f90:613-621).
 template <typename T, typename TL>
 __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, const T* D1, const T* P,
-                            const T* phis_hat, const TL* phizh, const T* D2, const TL* Zh, const double* filt,
-                            double* dG1, double* dG2) {
+                            const T* phis_hat, const TL* phizh, const T* D2, const T* Zh, const double* Zh2,
+                            const double* filt, double* dG1, double* dG2) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -316,7 +332,9 @@ __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, c
   const double f = filt ? filt[i] : 1.0;
   const double lin = phis_hat ? (double)phis_hat[i] * kth1 : 0.0;
   const double nl1 = ((D1 ? (double)D1[i] : 0.0) + (double)P[i] - lin + (double)phizh[i]) * f;
-  const double nl2 = ((D2 ? (double)D2[i] : 0.0) + (double)Zh[i]) * f * w;
+  double z = (double)Zh[i];
+  if (Zh2) z += Zh2[i];
+  const double nl2 = ((D2 ? (double)D2[i] : 0.0) + z) * f * w;
   const double c = cos(w * dt), s = sin(w * dt);
   dG1[i] = nl1 * c - nl2 * s;
   dG2[i] = (i == 0) ? 0.0 : nl2 * c + nl1 * s;
@@ -1355,10 +1373,14 @@ Cont continuation_matrix() {
 // column-major (ld = n).
 std::vector<double> sin_to_cos_matrix(int N, int n, int K) {
   const int M = N - 1;
-  auto C = [&](long q) -> double {
-    if (q % 2 == 0) return 0.0;
-    return 1.0 / std::tan(M_PI * (double)q / (2.0 * M));
-  };
+  // C(q) for q = s +- k in (-n, K + n): one evaluation per argument
+  const long q0 = -(long)n, nq = (long)K + 2L * n;
+  std::vector<double> Ct(nq);
+  for (long j = 0; j < nq; ++j) {
+    const long q = q0 + j;
+    Ct[j] = (q % 2 == 0) ? 0.0 : 1.0 / std::tan(M_PI * (double)q / (2.0 * M));
+  }
+  auto C = [&](long q) { return Ct[q - q0]; };
   std::vector<double> t((size_t)n * K);
   for (int s = 0; s < K; ++s)
     for (int k = 0; k < n; ++k) {
@@ -1502,6 +1524,25 @@ std::vector<double> r2r_matrix(int N, int kind, int dir) {
   return m;
 }
 
+// r2r_matrix on the device from the host table of the P = 2(N-1) residues:
+// stored element e of the N x N matrix is row r, column c (transposed: e = c + N r).
+template <typename U>
+__global__ void k_r2r_matrix(int N, int kind, int dir, int transposed, const double* __restrict__ tab, U* out) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * N) return;
+  const long P = 2L * (N - 1);
+  const int a = (int)(e % N), b = (int)(e / N);
+  const int r = transposed ? b : a, c = transposed ? a : b;
+  const double t = tab[((long)r * c) % P];
+  const bool er = (r == 0 || r == N - 1), ec = (c == 0 || c == N - 1);
+  double v;
+  if (kind == KCOS)
+    v = (dir == FWD) ? (er ? 1.0 : 2.0) / (double)P * (ec ? 1.0 : 2.0) * t : t;
+  else
+    v = (er || ec) ? 0.0 : ((dir == FWD) ? 2.0 / (double)(N - 1) : 1.0) * t;
+  out[e] = (U)v;
+}
+
 // Dense path applies when a base-grid direction has no FFT plan without
 // Bluestein and the matrices stay small (N <= 4096).
 inline bool dense_base_grid(int n1, int n2) {
@@ -1534,22 +1575,40 @@ U* upload_matrix(Arena& ar, const std::vector<double>& m) {
   return d;
 }
 
+// The N x N matrix of r2r_matrix built on the device (N >= 2); synchronous.
+template <typename U>
+U* device_matrix(Arena& ar, int N, int kind, int dir, bool transposed) {
+  const long P = 2L * (N - 1);
+  std::vector<double> tab(P);
+  for (long k = 0; k < P; ++k) {
+    const double ang = M_PI * (double)k / (double)(N - 1);
+    tab[k] = (kind == KCOS) ? std::cos(ang) : std::sin(ang);
+  }
+  double* dt = ar.get<double>(P);
+  HIP_OK(hipMemcpy(dt, tab.data(), P * sizeof(double), hipMemcpyHostToDevice));
+  U* m = ar.get<U>((size_t)N * N);
+  hipLaunchKernelGGL(k_r2r_matrix<U>, dim3(nblk((size_t)N * N)), dim3(256), 0, 0, N, kind, dir, transposed ? 1 : 0,
+                     (const double*)dt, m);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipDeviceSynchronize());
+  return m;
+}
+
+template <typename U>
+U* dense_matrix(Arena& ar, int N, int kind, int dir, bool transposed) {
+  if (N == 1) return upload_matrix<U>(ar, r2r_matrix(1, kind, dir));
+  return device_matrix<U>(ar, N, kind, dir, transposed);
+}
+
 template <typename U>
 void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
   g.n1 = n1;
   g.n2 = n2;
   for (int k = 0; k < 2; ++k)
     for (int d = 0; d < 2; ++d) {
-      std::vector<double> m = r2r_matrix(n1, k, d == 0 ? FWD : INV);
-      if (n2 == 1) {
-        // 2-D tank: a single column, done as a transposed GEMV; stored transposed
-        std::vector<double> t(m.size());
-        for (int i = 0; i < n1; ++i)
-          for (int j = 0; j < n1; ++j) t[j + (size_t)n1 * i] = m[i + (size_t)n1 * j];
-        m.swap(t);
-      }
-      g.mx[k][d] = upload_matrix<U>(ar, m);
-      if (n2 > 1) g.my[k][d] = upload_matrix<U>(ar, r2r_matrix(n2, k, d == 0 ? FWD : INV));
+      // 2-D tank: a single column, done as a transposed GEMV; stored transposed
+      g.mx[k][d] = dense_matrix<U>(ar, n1, k, d == 0 ? FWD : INV, n2 == 1);
+      if (n2 > 1) g.my[k][d] = dense_matrix<U>(ar, n2, k, d == 0 ? FWD : INV, false);
     }
   g.stage = ar.get<U>((size_t)n1 * n2);
   g.mid = ar.get<U>((size_t)n1 * n2);
@@ -1603,6 +1662,7 @@ struct EngineBase {
 
 struct hos_ctx {
   hos_params p{};
+  int device = 0;  // the HIP device the context was created on
   int n1, n2, n3, n3a, M, N1b, N2b;
   double pix, piy, dx, dy, delz;
   bool wmk;
@@ -1680,14 +1740,17 @@ struct Engine : EngineBase {
   int nch = 1;
   double *phi_add, *phix_add, *phiy_add, *phiz_add, *phit_add, *phit1, *phiz1, *w1, *tmp1;
   T *phix_addT, *phiy_addT, *phiz_addT;
-  // Mixed path: the first-order terms of the free-surface conditions (the linear
-  // vertical velocity kth1 phis_hat and the paddle forcing phiz_add, phit_add)
-  // stay in FP64, so the FP32 rounding enters only through the products of
-  // order >= 2.  From rest the fields are first-order, and an FP32 linear part
-  // would set an h-independent floor (~2.5e-7 relative) under the embedded
-  // error estimate that a 1e-7 tolerance never passes.
+  // Mixed path: the first-order terms of the free-surface conditions stay out of
+  // FP32.  The linear vertical velocity kth1 phis_hat cancels exactly against the
+  // term the change of variables subtracts, so P keeps orders >= 2 only; the
+  // paddle forcing (phiz_add, phit_add) and the beach's first-order term
+  // nu F2S(kth1 phis_hat) are transformed in FP64 on the wavemaker stream.  FP32
+  // rounding then enters through the products of order >= 2 only.  From rest the
+  // fields are first-order, and an FP32 linear part sets an h-independent floor
+  // (~2.5e-7 relative) under the embedded error estimate, which a 1e-7 tolerance
+  // never passes.
   static constexpr bool kSplit = sizeof(T) != sizeof(double);
-  double *phis64 = nullptr, *wlin = nullptr, *Z64 = nullptr, *Zh64 = nullptr, *phizh64 = nullptr;
+  double *phis64 = nullptr, *wlin = nullptr, *Zf64 = nullptr, *Zh64 = nullptr, *phizh64 = nullptr;
   Cont cont;
   // mixed-parity products: spectra, conversion matrices, results
   int K1 = 0, K2 = 0;
@@ -1751,8 +1814,8 @@ struct Engine : EngineBase {
     phizh = ar.get<T>(ns); tmpS = ar.get<T>(ns);
     phix_addT = ar.get<T>(ns); phiy_addT = ar.get<T>(ns); phiz_addT = ar.get<T>(ns);
     if (kSplit) {
-      phis64 = ar.get<double>(ns); wlin = ar.get<double>(ns); Z64 = ar.get<double>(ns);
-      Zh64 = ar.get<double>(ns); phizh64 = ar.get<double>(ns);
+      phis64 = ar.get<double>(ns); wlin = ar.get<double>(ns);
+      Zf64 = ar.get<double>(ns); Zh64 = ar.get<double>(ns); phizh64 = ar.get<double>(ns);
     }
     etapm.assign(M + 1, nullptr);
     phizm.assign(M + 1, nullptr);
@@ -2099,24 +2162,22 @@ struct Engine : EngineBase {
     // dphin and the final base-grid RHS
     base(KCOS, KCOS, INV, src_plain<T>(Q, n1, n1, n2), n1, n2, Dst<T>{tmpS, n1, 0});
     if constexpr (kSplit) {
-      // first-order vertical velocity in FP64; Z and its modes in FP64
-      base64(KCOS, KCOS, INV, src_mul<double, double>(phis64, n1, n1, n2, MUL_KTH, 1, pix, piy), n1, n2,
-             Dst<double>{wlin, n1, 0});
-      hipLaunchKernelGGL((k_dphin<T, double>), dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS,
-                         (const double*)wlin, (const double*)phiz_add, (const double*)phit_add,
-                         (const double*)C.nu_d, Z64);
-      base64(KCOS, KCOS, FWD, src_plain<double, double>(Z64, n1, n1, n2), n1, n2, Dst<double>{Zh64, n1, 0});
+      // beach term of the orders >= 2 in T; the first-order modes Zh64 come from the
+      // wavemaker stream
+      hipLaunchKernelGGL(k_beach<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)C.nu_d, Zp);
+      base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
       hipLaunchKernelGGL((k_expon_out<T, double>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)nullptr,
-                         (const double*)phizh64, (const T*)(M >= 2 ? D2 : nullptr), (const double*)Zh64,
-                         (const double*)C.filt_fs, kG1, kG2);
+                         (const double*)phizh64, (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh,
+                         (const double*)Zh64, (const double*)C.filt_fs, kG1, kG2);
     } else {
-      hipLaunchKernelGGL((k_dphin<T, T>), dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)nullptr,
-                         (const double*)phiz_add, (const double*)phit_add, (const double*)C.nu_d, Zp);
+      hipLaunchKernelGGL(k_dphin<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)phiz_add,
+                         (const double*)phit_add, (const double*)C.nu_d, Zp);
       base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
       hipLaunchKernelGGL((k_expon_out<T, T>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
-                         (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)C.filt_fs, kG1, kG2);
+                         (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)nullptr,
+                         (const double*)C.filt_fs, kG1, kG2);
     }
     HIP_OK(hipMemcpyAsync(C.phi_add_last, phi_add, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
@@ -2160,7 +2221,7 @@ struct Engine : EngineBase {
       HIP_OK(hipMemsetAsync(phiz_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phit_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phizh, 0, ns * sizeof(T), st));
-      if (phizh64) HIP_OK(hipMemsetAsync(phizh64, 0, ns * sizeof(double), st));
+      if constexpr (kSplit) phiz_modes();  // the beach's first-order term
       HIP_OK(hipMemsetAsync(phixa, 0, nb * sizeof(T), st));
       HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
       HIP_OK(hipMemsetAsync(phiza, 0, nb * sizeof(T), st));
@@ -2209,12 +2270,18 @@ struct Engine : EngineBase {
   }
 
   // modes of phiz_add in T (the extended-grid gradient products); on the mixed
-  // path from the FP64 modes, which the free-surface condition uses directly
+  // path from the FP64 modes, which the free-surface condition uses directly, and
+  // the FP64 modes of the first-order part -phit_add - nu (phiz_add + F2S(kth1 phis))
   void phiz_modes() {
     const int n1 = c->n1, n2 = c->n2;
     if constexpr (kSplit) {
       base64(KCOS, KCOS, FWD, src_plain<double, double>(phiz_add, n1, n1, n2), n1, n2, Dst<double>{phizh64, n1, 0});
       hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, cs, ns, (const double*)phizh64, phizh);
+      base64(KCOS, KCOS, INV, src_mul<double, double>(phis64, n1, n1, n2, MUL_KTH, 1, c->pix, c->piy), n1, n2,
+             Dst<double>{wlin, n1, 0});
+      hipLaunchKernelGGL(k_forcing, dim3(nblk(ns)), dim3(256), 0, cs, ns, (const double*)phiz_add,
+                         (const double*)phit_add, (const double*)wlin, (const double*)c->nu_d, Zf64);
+      base64(KCOS, KCOS, FWD, src_plain<double, double>(Zf64, n1, n1, n2), n1, n2, Dst<double>{Zh64, n1, 0});
     } else {
       base(KCOS, KCOS, FWD, src_plain<T, double>(phiz_add, n1, n1, n2), n1, n2, Dst<T>{phizh, n1, 0});
     }
@@ -2764,6 +2831,7 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     HIP_OK(hipGetDeviceCount(&ndev));
     if (ndev < 1) throw std::runtime_error("no GPU device");
     auto c = std::make_unique<hos_ctx>();
+    HIP_OK(hipGetDevice(&c->device));
     c->p = *pin;
     const hos_params& p = c->p;
     c->n1 = p.n1;
@@ -2882,6 +2950,10 @@ int hos_initialize(hos_ctx* c) {
     HIP_OK(hipMemsetAsync(c->a1st_add, 0, na * sizeof(double), st));
     HIP_OK(hipMemsetAsync(c->a_add_ramp, 0, na * sizeof(double), st));
     HIP_OK(hipMemsetAsync(c->deta_hat, 0, ns * sizeof(double), st));
+    // the free surface starts at rest (initial_condition.f90:329-348)
+    HIP_OK(hipMemsetAsync(c->eta_hat, 0, ns * sizeof(double), st));
+    HIP_OK(hipMemsetAsync(c->phis_hat, 0, ns * sizeof(double), st));
+    c->have_trial = false;
     if (c->wmk2) {  // a2nd, eta2nd, a2nd_add start at zero (initial_condition.f90:370-373)
       HIP_OK(hipMemsetAsync(c->a2nd, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(c->eta2nd, 0, ns * sizeof(double), st));
@@ -3045,29 +3117,70 @@ int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_nex
         if (ctx[j] == ctx[i]) throw std::runtime_error("a context appears twice in the ensemble");
       a[i] = Advance{t_cur[i], h_rk[i], 0, 0};
     }
-    auto launch = [&](int i) {
-      hl[i] = (t_next - a[i].t < a[i].h) ? t_next - a[i].t : a[i].h;
-      rk_enqueue(ctx[i], a[i].t, hl[i]);
-    };
-    // every member's attempt is in flight on its own streams before the first wait; a member's
-    // next attempt is enqueued as soon as its decision is taken, while the others still run
-    for (int i = 0; i < n; ++i)
-      if (a[i].t < t_next) launch(i);
-    for (bool any = true; any;) {
-      any = false;
-      for (int i = 0; i < n; ++i) {
-        if (!(a[i].t < t_next)) continue;
-        double err = 0.0;
-        rk_finish(ctx[i], &err);
-        control(ctx[i], a[i], hl[i], err, toler, dt_stab[i], delt_out[i]);
-        if (a[i].t < t_next) {
-          launch(i);
-          any = true;
-        }
-      }
-    }
+    // Host threads: an attempt is a few hundred small launches, so a ragged
+    // ensemble of small tanks is bound by the launch rate of one host thread.
+    // Members are dealt to the threads longest first (cost ~ extended points x
+    // M / dt_stab), each thread keeping all of its members' attempts in flight.
+    int nth = 8;
+    if (const char* e = std::getenv("HOSNWT_ENSEMBLE_THREADS")) nth = std::max(1, std::atoi(e));
+    nth = std::max(1, std::min(nth, n));
+    std::vector<int> order(n);
+    std::vector<double> cost(n);
     for (int i = 0; i < n; ++i) {
-      HIP_OK(hipStreamSynchronize(ctx[i]->st));
+      order[i] = i;
+      cost[i] = (double)ctx[i]->N1b * ctx[i]->N2b * ctx[i]->M * std::max(0.0, t_next - a[i].t) / dt_stab[i];
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<std::vector<int>> part(nth);
+    std::vector<double> load(nth, 0.0);
+    for (int i : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      part[k].push_back(i);
+      load[k] += cost[i];
+    }
+    std::mutex em;
+    std::string first_error;
+    std::atomic<bool> failed{false};
+    auto run = [&](const std::vector<int>& mine) {
+      try {
+        if (!mine.empty()) HIP_OK(hipSetDevice(ctx[mine[0]]->device));
+        auto launch = [&](int i) {
+          hl[i] = (t_next - a[i].t < a[i].h) ? t_next - a[i].t : a[i].h;
+          rk_enqueue(ctx[i], a[i].t, hl[i]);
+        };
+        // every member's attempt is in flight on its own streams before the first
+        // wait; a member's next attempt is enqueued as soon as its decision is taken
+        for (int i : mine)
+          if (a[i].t < t_next) launch(i);
+        for (bool any = true; any && !failed.load();) {
+          any = false;
+          for (int i : mine) {
+            if (!(a[i].t < t_next)) continue;
+            double err = 0.0;
+            rk_finish(ctx[i], &err);
+            control(ctx[i], a[i], hl[i], err, toler, dt_stab[i], delt_out[i]);
+            if (a[i].t < t_next) {
+              launch(i);
+              any = true;
+            }
+          }
+        }
+        for (int i : mine) HIP_OK(hipStreamSynchronize(ctx[i]->st));
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(em);
+        if (first_error.empty()) first_error = e.what();
+        failed = true;
+      }
+    };
+    if (nth == 1) {
+      run(part[0]);
+    } else {
+      std::vector<std::thread> th;
+      for (int k = 0; k < nth; ++k) th.emplace_back(run, std::cref(part[k]));
+      for (auto& t : th) t.join();
+    }
+    if (!first_error.empty()) throw std::runtime_error(first_error);
+    for (int i = 0; i < n; ++i) {
       t_cur[i] = a[i].t;
       h_rk[i] = a[i].h;
       if (n_acc) n_acc[i] = a[i].na;

@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -554,8 +555,11 @@ constexpr size_t kLdsMax = 160 * 1024;
 template <typename K>
 inline void allow_lds(K kernel, size_t bytes) {
   static std::map<const void*, size_t> granted;
+  static std::mutex mu;  // contexts may be driven from several host threads
+  if (bytes <= 64 * 1024) return;
+  std::lock_guard<std::mutex> lk(mu);
   const void* key = reinterpret_cast<const void*>(kernel);
-  if (bytes > 64 * 1024 && bytes > granted[key]) {
+  if (bytes > granted[key]) {
     HIP_OK(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
     granted[key] = bytes;
   }

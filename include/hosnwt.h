@@ -94,8 +94,10 @@ int hos_create(const hos_params* p, hos_ctx** out);
 int hos_destroy(hos_ctx* c);
 const char* hos_last_error(void);
 
-/* Initial wavemaker modes and potential from rest at t = 0
- * (initial_condition.f90:367-466). */
+/* Rest state (eta = phis = 0, no wavemaker modes beyond the initial ones)
+ * and the initial wavemaker modes and potential at t = 0
+ * (initial_condition.f90:329-348, :367-466); resets a used context too.
+ * Sloshing (icase 1) sets its initial eta with hos_set_state instead. */
 int hos_initialize(hos_ctx* c);
 
 /* state: eta, phis, a1st, eta1st: n1*n2; a1st_add, a_add_ramp: n3_add*n2

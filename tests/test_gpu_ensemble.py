@@ -5,7 +5,9 @@ Every member must take exactly the decisions and reach exactly the state that
 hos_advance reaches for it alone (same kernels in the same order on its own
 streams), here checked to relative L2 1e-14 on eta and phis with equal
 accepted / rejected counts and step proposals; FP64 and mixed members mixed in
-one ensemble.
+one ensemble, driven from one host thread and from several
+(HOSNWT_ENSEMBLE_THREADS).  hos_initialize on a used context restarts from rest:
+the same trajectory as a fresh context.
 """
 import numpy as np
 import pytest
@@ -32,7 +34,9 @@ def setups():
 
 
 @pytest.mark.gpu
-def test_ensemble_matches_members_run_alone():
+@pytest.mark.parametrize("threads", [1, 3])
+def test_ensemble_matches_members_run_alone(threads, monkeypatch):
+    monkeypatch.setenv("HOSNWT_ENSEMBLE_THREADS", str(threads))
     ss = setups()
     t_next = min(s.delt_out for s, _ in ss) * 0.6
     h0 = [min(0.95 * s.dt_stab, s.delt_out) for s, _ in ss]
@@ -70,5 +74,25 @@ def test_ensemble_rejects_a_repeated_context():
         tk.initialize()
         with pytest.raises(RuntimeError, match="twice"):
             advance_ensemble([tk, tk], 0.1, [0.01, 0.01])
+    finally:
+        tk.close()
+
+
+@pytest.mark.gpu
+def test_reinitialize_restarts_from_rest():
+    s, prec = setups()[1]
+    tk = Tank(s, precision=prec)
+    try:
+        runs = []
+        for _ in range(2):
+            tk.initialize()
+            h, na, nr = tk.advance(0.3 * s.delt_out, min(0.95 * s.dt_stab, s.delt_out))
+            st = tk.get_state()
+            runs.append((h, na, nr, st["eta"], st["phis"], st["a1st_add"]))
+        print("attempts", runs[0][1:3], runs[1][1:3], flush=True)
+        assert runs[0][:3] == runs[1][:3]
+        for a, b in zip(runs[0][3:], runs[1][3:]):
+            assert np.array_equal(a, b)
+        assert np.abs(runs[0][3]).max() > 0
     finally:
         tk.close()

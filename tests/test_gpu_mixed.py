@@ -2,9 +2,10 @@
 step control) against the FP64 oracle and the FP64 path.
 
 * RHS at O(1e-2) amplitudes: relative L2 <= 1e-5 against the oracle (2-D piston,
-  2-D flap JONSWAP, 3-D directional).
+  2-D flap JONSWAP, 3-D directional, 2-D sloshing without a wavemaker).
 * The first-order terms of the free-surface conditions (the linear vertical
-  velocity and the paddle forcing) are computed in FP64 on the mixed path, so
+  velocity, the paddle forcing and the beach's first-order damping) are
+  computed in FP64 on the mixed path, so
   its rounding error scales with the products of order >= 2: near rest (state
   amplitude 1e-5, 5 % into the ramp) the RHS must be within 1e-7 of the oracle
   and at least 20x closer than at amplitude 1e-2 (with an FP32 linear part the
@@ -30,10 +31,12 @@ CASES = {
     "3d_directional_M5": dict(n1=48, n2=24, M=5, n3=9, xlen=10.0, ylen=5.0, Tramp=2.8,
                               spec=dict(Hs=0.06, Tp=1.4, n_freq=8, n_dir=4)),
 }
+# without a wavemaker (sloshing, icase 1): the beach's first-order term alone
+RHS_CASES = dict(CASES, sloshing_2d_M4=dict(n1=64, n2=1, M=4, n3=9, xlen=10.0, icase=1))
 
 
 def setup_for(name, **extra):
-    kw = dict(CASES[name], **extra)
+    kw = dict(RHS_CASES[name], **extra)
     spec = kw.pop("spec", None)
     if spec is not None:
         kw["components"] = hc.jonswap_components(spec["Hs"], spec["Tp"], 3.3, n_freq=spec["n_freq"], seed=21,
@@ -58,7 +61,7 @@ def rhs_errors(s, orc, tank, amp, frac):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("name", list(RHS_CASES))
 def test_mixed_rhs_error_scales_with_the_nonlinear_part(name):
     s = setup_for(name)
     orc = ho.Oracle(oracle_params(s))
