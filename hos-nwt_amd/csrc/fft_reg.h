@@ -272,7 +272,8 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
 // element in a rolled loop into LDS before the loads.
 // FP64 instances are held to two waves per SIMD (at most 256 registers): the
 // HOS epilogue then spills a few dwords, which costs less than running at one
-// wave per SIMD.
+// wave per SIMD.  FP32 bounds (x: free, y: 4 waves, 3 with the HOS epilogue)
+// measured best among x 1-3 / y 3-4 / y-HOS 2-3 on the 3-D mixed step.
 // Independent line transforms sharing one launch (blockIdx.y = item): same
 // length, direction and functor types; kind, extents, source and destination
 // per item.  A workgroup whose lines lie beyond its item's extent exits at

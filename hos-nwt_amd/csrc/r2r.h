@@ -513,7 +513,7 @@ struct Grid2 {
       // y tile width: 8 columns.  A lone 3073 x 769 pass is 3-13 % faster with
       // 16 (longer row segments), but the batched launches of a right-hand side
       // run 2-7 % faster with 8, whose smaller LDS footprint keeps two tiles per
-      // CU; HOSNWT_Y_TILE overrides it
+      // CU (FP32: 8 and 16 equal on the full step); HOSNWT_Y_TILE overrides it
       int want = 8;
       if (const char* e = std::getenv("HOSNWT_Y_TILE")) want = std::max(2, std::atoi(e));
       TX = want;

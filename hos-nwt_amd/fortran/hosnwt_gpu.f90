@@ -121,6 +121,16 @@ INTERFACE
       REAL(C_DOUBLE), VALUE :: t_next, toler, dt_stab, delt_out
       INTEGER(C_INT), INTENT(OUT) :: n_acc, n_rej
    END FUNCTION hos_advance
+   INTEGER(C_INT) FUNCTION hos_advance_ensemble(n, ctx, t_cur, t_next, h_rk, toler, dt_stab, delt_out, &
+        n_acc, n_rej) BIND(C, NAME='hos_advance_ensemble')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      INTEGER(C_INT), VALUE :: n
+      TYPE(C_PTR), INTENT(IN) :: ctx(*)
+      REAL(C_DOUBLE), INTENT(INOUT) :: t_cur(*), h_rk(*)
+      REAL(C_DOUBLE), VALUE :: t_next, toler
+      REAL(C_DOUBLE), INTENT(IN) :: dt_stab(*), delt_out(*)
+      INTEGER(C_INT), INTENT(OUT) :: n_acc(*), n_rej(*)
+   END FUNCTION hos_advance_ensemble
    INTEGER(C_INT) FUNCTION hos_get_physical(ctx, eta, phis, deta, phi_add) BIND(C, NAME='hos_get_physical')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx

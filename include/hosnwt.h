@@ -121,7 +121,10 @@ int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double t
  * is enqueued on its own streams before the host waits for any of them, so
  * the members' kernels overlap on the GPU.  Arrays of n: t_cur, h_rk (in/out),
  * dt_stab, delt_out, n_acc, n_rej (may be NULL).  Each member takes exactly
- * the decisions hos_advance would take for it alone (HOS-NWT.f90:498-571). */
+ * the decisions hos_advance would take for it alone (HOS-NWT.f90:498-571).
+ * The members are driven from HOSNWT_ENSEMBLE_THREADS host threads (default
+ * 4); no other thread may use them during the call.  A context may appear
+ * only once. */
 int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
                          const double* dt_stab, const double* delt_out, int* n_acc, int* n_rej);
 
