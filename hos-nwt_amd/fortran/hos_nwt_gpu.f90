@@ -2,8 +2,10 @@
 !
 ! Reads the reference's input_HOS-NWT.dat (same 52-column description + value
 ! format, IO/input.f90:62-133), nondimensionalises (main/HOS-NWT.f90:99-119),
-! builds the paddle motion of a monochromatic wave (icase = 2, front wave,
-! piston or hinged flap) or a sloshing mode (icase = 1), the front beach, the
+! builds the paddle motion of a monochromatic wave (icase = 2, front wave or
+! Dalrymple's directional wave, piston or hinged flap), of wave files
+! (icase = 3 .cfg/.dat, icase = 31/32 Ocean .txt) or a sloshing mode
+! (icase = 1), the front beach, the
 ! RK4 stability bound (main/HOS-NWT.f90:132-196), and runs the time loop of
 ! main/HOS-NWT.f90:446-581 with the inner adaptive loop on the GPU
 ! (hos_advance).  Writes the reference's output files (IO/output.f90:37-482)
@@ -38,7 +40,7 @@ INTEGER :: nprobes, nh, nc, rnum_f
 INTEGER, ALLOCATABLE :: c_harmo(:)
 REAL(RP), ALLOCATABLE :: c_f(:), c_a(:), c_ang(:), c_ph(:), c_k(:)
 COMPLEX(RP), ALLOCATABLE :: c_tf(:)
-REAL(RP) :: clock_f, depth_f, hinge_f
+REAL(RP) :: clock_f, depth_f, hinge_f, ly_f
 CHARACTER(LEN=16) :: typ_wmk_f
 REAL(RP), ALLOCATABLE :: xprobe(:), yprobe(:), eta_probe(:), etam(:,:), phism(:,:), xtop(:), vtop(:)
 REAL(RP), ALLOCATABLE, TARGET :: nu(:,:), eta(:,:), phis(:,:), deta(:,:), phi_add(:,:), zero2(:,:), zadd(:,:)
@@ -75,8 +77,9 @@ CALL blank(); CALL rd_i(iprobes); CALL rd_c(pro_file)
 CALL blank(); CALL rd_r(T_stop); CALL rd_r(toler); CALL rd_r(f_out)
 CALL blank(); CALL rd_i(idim); CALL rd_i(i3d); CALL rd_i(imodes); CALL rd_i(iwmk); CALL rd_i(i_sw)
 CLOSE(unit_in)
-IF (icase < 1 .OR. icase > 3) ERROR STOP 'this driver builds icase = 1, 2 or 3 (icase 4: use the Python host)'
-IF (icase == 2 .AND. ibat /= 2) ERROR STOP 'only the front wave (ibat = 2) is built here'
+IF (.NOT. ANY(icase == [1, 2, 3, 31, 32])) &
+   ERROR STOP 'this driver builds icase = 1, 2, 3, 31 and 32 (icase 4: use the Python host)'
+IF (ibat /= 2 .AND. ibat /= 3) ERROR STOP 'ibat = 2 (front) or 3 (Dalrymple)'
 ! adjustments made while reading (IO/input.f90:163-172)
 islosh = islosh + 1
 IF (n2 == 1) theta_mono = 0.0_rp
@@ -125,7 +128,22 @@ IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
 END IF
 !
 nh = 1
-IF (icase == 3) CALL file2wave()
+! Ocean .txt files become <file_name>.cfg/.dat first (initial_condition.f90:180-181)
+IF (icase == 31 .OR. icase == 32) CALL ocean_txt2cfg_dat()
+IF (icase == 3 .OR. icase == 31 .OR. icase == 32) THEN
+   CALL file2wave()
+   ! Dalrymple's method per component in a 3-D basin (initial_condition.f90:192-199)
+   IF (ibat == 3 .AND. n2 /= 1) CALL dalrymple_expand(xd_mono)
+ELSE IF (icase == 2 .AND. ibat == 3) THEN
+   ! one harmonic on the clock 2 nu, rnum 1, Ly = ylen (initial_condition.f90:147-166)
+   rnum_f = 1; clock_f = 2.0_rp * nu_mono; depth_f = h; ly_f = ylen; hinge_f = d_hinge
+   typ_wmk_f = 'piston'
+   IF (igeom == 2) typ_wmk_f = 'hinged'
+   nc = 1
+   ALLOCATE(c_harmo(1), c_f(1), c_a(1), c_ang(1), c_ph(1), c_k(1), c_tf(1))
+   c_harmo = 1; c_f = nu_mono; c_a = amp_mono; c_ang = theta_mono; c_ph = ph_mono
+   CALL dalrymple_expand(xd_mono)
+END IF
 ALLOCATE(freq(nh), pre(n2*nh), pim(n2*nh), dyre(n2*nh), dyim(n2*nh), nux(n1,n2))
 freq = 0.0_rp; pre = 0.0_rp; pim = 0.0_rp; dyre = 0.0_rp; dyim = 0.0_rp; nux = 0.0_rp
 ! beach x derivative as initial_condition.f90:768 builds it (2 c x3), read with i_wmk = 2
@@ -134,7 +152,7 @@ IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
       IF ((i1 - 1) * delx >= x0) nux(i1,:) = coeffabsf * 2.0_rp * ((i1 - 1) * delx - x0) / x2
    END DO
 END IF
-IF (icase == 2) THEN
+IF (icase == 2 .AND. ibat == 2) THEN
    ! front(): one harmonic at nu_mono (wave_def.f90:54-80), wave_init and wmk_ini
    freq1 = nu_mono
    k_w = wave_number(freq1)
@@ -150,7 +168,7 @@ IF (icase == 2) THEN
       dyim(i2) = AIMAG(ctmp)
    END DO
    freq(1) = freq1
-ELSE IF (icase == 3) THEN
+ELSE IF (icase == 3 .OR. icase == 31 .OR. icase == 32 .OR. (icase == 2 .AND. ibat == 3)) THEN
    CALL wmk_ini()
 END IF
 !
@@ -162,7 +180,7 @@ p%xlen = xlen; p%ylen = ylen; p%d_hinge = d_hinge; p%Tramp = Tramp; p%T_stop = T
 p%nu = C_LOC(nu)
 p%n_freq = 0
 IF (icase == 2) p%n_freq = 1
-IF (icase == 3) p%n_freq = nh
+IF (icase == 3 .OR. icase == 31 .OR. icase == 32) p%n_freq = nh
 p%freq = C_LOC(freq); p%pos_ini_re = C_LOC(pre); p%pos_ini_im = C_LOC(pim)
 p%dposdy_ini_re = C_LOC(dyre); p%dposdy_ini_im = C_LOC(dyim); p%nu_x = C_LOC(nux)
 CALL hos_check(hos_create(p, ctx), 'hos_create')
@@ -528,6 +546,7 @@ SUBROUTINE file2wave()
    lo = lo * tsc
    hi = hi * tsc
    hinge_f = hinge_f / depth_f
+   ly_f = ly_f / depth_f
    OPEN(u, FILE=TRIM(file_name)//'.dat', STATUS='OLD')
    nf = 0
    DO
@@ -571,6 +590,197 @@ SUBROUTINE file2wave()
    END DO
    IF (nh == 0) ERROR STOP 'no wave harmonic below the maximum frequency'
 END SUBROUTINE file2wave
+!
+! Ocean .txt (ocean_txt2cfg_dat, wave/read_ocean_txt.f90:33-228): the run's rnumber, ramp,
+! optional rnumber compression and its list of fronts (harmonic, frequency, amplitude,
+! angle and phase in rad) become <file_name>.cfg/.dat in the formats file2wave reads.
+! Frequencies are harmonic * 32 Hz / 2^rnum; in the ECN basin (icase 31) the phases are
+! moved to the basin's centre line, k(f, 5 m) * 29.74 m / 2 * sin(angle).
+SUBROUTINE ocean_txt2cfg_dat()
+   INTEGER :: u, ios, nl, j, rnum, ramp, rcomp, nfr, first
+   LOGICAL :: comp
+   REAL(RP) :: clk, fb, lo, hi, depth, kd
+   CHARACTER(LEN=200) :: ln
+   INTEGER, ALLOCATABLE :: hh(:)
+   REAL(RP), ALLOCATABLE :: ff(:), aa(:), an(:), ph(:)
+   u = 998
+   OPEN(u, FILE=TRIM(file_name)//'.txt', STATUS='OLD')
+   nl = 0; comp = .FALSE.; rnum = -1; ramp = 0; rcomp = 1; nfr = -1; first = 0
+   DO
+      READ(u, '(A)', IOSTAT=ios) ln
+      IF (ios /= 0) EXIT
+      nl = nl + 1
+      IF (INDEX(ln, 'rnumber  =') /= 0) rnum = int_ending_at(ln, 18)
+      IF (INDEX(ln, 'ramp     =') /= 0) ramp = int_ending_at(ln, 18)
+      IF (INDEX(ln, 'rnumber compression') /= 0) THEN
+         comp = .TRUE.
+         rcomp = int_ending_at(ln, 62)
+      END IF
+      IF (INDEX(ln, 'Wave has ') /= 0) THEN
+         nfr = int_starting_at(ln, 10)
+         first = nl + 2
+      END IF
+   END DO
+   IF (rnum < 0 .OR. nfr < 0) ERROR STOP 'Ocean .txt file: no rnumber or no "Wave has" line'
+   REWIND(u)
+   DO j = 1, first - 1
+      READ(u, '(A)')
+   END DO
+   ALLOCATE(hh(nfr), ff(nfr), aa(nfr), an(nfr), ph(nfr))
+   DO j = 1, nfr
+      READ(u, *) hh(j), ff(j), aa(j), an(j), ph(j)
+   END DO
+   CLOSE(u)
+   IF (comp) hh = hh * rcomp
+   clk = 32.0_rp
+   fb = clk / 2.0_rp**rnum
+   ff = hh * fb
+   depth = 5.0_rp
+   IF (icase == 32) depth = 2.88_rp
+   IF (icase == 31) THEN
+      DO j = 1, nfr
+         kd = wave_number(ff(j) * SQRT(depth / G)) / depth
+         ph(j) = ph(j) + kd * 29.74_rp / 2.0_rp * SIN(an(j))
+      END DO
+   END IF
+   an = 180.0_rp * an / PI
+   lo = 0.0_rp; hi = 100.0_rp
+   IF (i_cut == 1) THEN
+      lo = nuc_low; hi = nuc_high
+   END IF
+   OPEN(u, FILE=TRIM(file_name)//'.cfg', STATUS='REPLACE')
+   WRITE(u,'(I2)') rnum
+   WRITE(u,'(F5.2)') clk
+   WRITE(u,'(F5.2)') depth
+   WRITE(u,'(I4)') ramp
+   WRITE(u,'(A)') 'lin'
+   WRITE(u,'(F5.2)') lo
+   WRITE(u,'(F6.2)') hi
+   IF (icase == 31) THEN
+      WRITE(u,'(F5.2)') 30.0_rp
+      WRITE(u,'(A)') 'hinged'
+      WRITE(u,'(F6.3)') 2.147_rp
+   ELSE
+      WRITE(u,'(F5.2)') 5.0_rp
+      WRITE(u,'(A)') 'hinged'
+      WRITE(u,'(F6.3)') 0.0_rp
+   END IF
+   CLOSE(u)
+   WHERE (aa < 1.0e-98_rp) aa = 0.0_rp
+   OPEN(u, FILE=TRIM(file_name)//'.dat', STATUS='REPLACE')
+   DO j = 1, nfr
+      IF (ff(j) >= lo .AND. ff(j) <= hi) WRITE(u,'(I5,4(X,G20.7))') hh(j), ff(j), aa(j), an(j), ph(j)
+   END DO
+   CLOSE(u)
+END SUBROUTINE ocean_txt2cfg_dat
+!
+! the integer whose last digit is at column c (read leftwards to a blank)
+INTEGER FUNCTION int_ending_at(ln, c)
+   CHARACTER(LEN=*), INTENT(IN) :: ln
+   INTEGER, INTENT(IN) :: c
+   INTEGER :: b
+   b = c
+   DO WHILE (b > 1 .AND. ln(b:b) /= ' ')
+      b = b - 1
+   END DO
+   READ(ln(b:c), *) int_ending_at
+END FUNCTION int_ending_at
+!
+! the integer whose first digit is at column c (read rightwards to a blank)
+INTEGER FUNCTION int_starting_at(ln, c)
+   CHARACTER(LEN=*), INTENT(IN) :: ln
+   INTEGER, INTENT(IN) :: c
+   INTEGER :: e
+   e = c
+   DO WHILE (e < LEN(ln) .AND. ln(e:e) /= ' ')
+      e = e + 1
+   END DO
+   READ(ln(c:e), *) int_starting_at
+END FUNCTION int_starting_at
+!
+! Dalrymple's directional wavemaker (dalrymple, wave/wave_def.f90:111-177): each component
+! (harmonic m, amplitude a, angle theta, phase phi) becomes 2(N+1) components of frequency
+! f = m clock / 2^rnum, N = floor(k Ly / pi), at the angles +-asin(mu_n / k), mu_n = n pi / Ly,
+! with complex amplitude a e^{i phi} I_n e^{(i sqrt(k^2 - mu_n^2) - i k cos theta) x_d} / 2
+! (I_n the projection of e^{i k sin(theta) y} on cos(mu_n y)), so that the wave focuses at
+! the distance x_d from the wavemaker.  The phase is taken with the reference's phase()
+! (ref_phase).  Then wave_init: k and the transfer function.
+SUBROUTINE dalrymple_expand(xd)
+   REAL(RP), INTENT(IN) :: xd
+   INTEGER :: j, n, nn, m, ntot, ig
+   REAL(RP) :: fr, kk, beta, mu
+   COMPLEX(RP) :: tmp, In, kn, a
+   INTEGER, ALLOCATABLE :: hh(:)
+   REAL(RP), ALLOCATABLE :: ff(:), aa(:), an(:), ph(:)
+   COMPLEX(RP), PARAMETER :: CI = (0.0_rp, 1.0_rp)
+   ntot = 0
+   DO j = 1, nc
+      fr = c_harmo(j) * clock_f / 2.0_rp**rnum_f
+      ntot = ntot + 2 * (FLOOR(wave_number(fr) * ly_f / PI) + 1)
+   END DO
+   ALLOCATE(hh(ntot), ff(ntot), aa(ntot), an(ntot), ph(ntot))
+   m = 0
+   DO j = 1, nc
+      fr = c_harmo(j) * clock_f / 2.0_rp**rnum_f
+      kk = wave_number(fr)
+      nn = FLOOR(kk * ly_f / PI)
+      beta = kk * SIN(c_ang(j))
+      tmp = -EXP(-CI * beta * ly_f)
+      DO n = 0, nn
+         mu = n * PI / ly_f
+         tmp = -tmp
+         IF (ABS(c_ang(j)) < TINY) THEN
+            In = 0.0_rp
+            IF (n == 0) In = 1.0_rp
+         ELSE IF (ABS(mu - ABS(beta)) < TINY) THEN
+            In = 1.0_rp
+         ELSE
+            In = 2.0_rp * CI * beta * (tmp - 1.0_rp) / (beta**2 - mu**2) / (MERGE(2.0_rp, 1.0_rp, n == 0) * ly_f)
+         END IF
+         kn = CI * SQRT(kk**2 - mu**2)
+         a = c_a(j) * EXP(CI * c_ph(j)) * In * EXP((kn - CI * kk * COS(c_ang(j))) * xd)
+         hh(m + n + 1) = c_harmo(j); hh(m + nn + n + 2) = c_harmo(j)
+         ff(m + n + 1) = fr; ff(m + nn + n + 2) = fr
+         aa(m + n + 1) = ABS(a / 2.0_rp); aa(m + nn + n + 2) = ABS(a / 2.0_rp)
+         an(m + n + 1) = ASIN(mu / kk); an(m + nn + n + 2) = -ASIN(mu / kk)
+         ph(m + n + 1) = ref_phase(a); ph(m + nn + n + 2) = ref_phase(a)
+      END DO
+      m = m + 2 * (nn + 1)
+   END DO
+   DEALLOCATE(c_harmo, c_f, c_a, c_ang, c_ph, c_k, c_tf)
+   nc = ntot
+   ALLOCATE(c_harmo(nc), c_f(nc), c_a(nc), c_ang(nc), c_ph(nc), c_k(nc), c_tf(nc))
+   c_harmo = hh; c_f = ff; c_a = aa; c_ang = an; c_ph = ph
+   ig = 1
+   IF (TRIM(typ_wmk_f) == 'hinged') ig = 2
+   DO j = 1, nc
+      c_k(j) = wave_number(c_f(j))
+      c_tf(j) = CMPLX(1.0_rp, 0.0_rp, RP)
+      IF (ABS(c_k(j)) > TINY) c_tf(j) = transfer(ig, c_k(j), hinge_f)
+      IF (c_f(j) < 1.0E-8_rp) THEN
+         c_k(j) = 0.0_rp
+         c_tf(j) = CMPLX(1.0_rp, 0.0_rp, RP)
+      END IF
+   END DO
+END SUBROUTINE dalrymple_expand
+!
+! phase() of wave/wave_def.f90:491-524 as written: 0 whenever |z| > tiny, the argument in
+! [0, 2 pi) only below (0 here for z = 0, where the reference divides 0 by 0)
+REAL(RP) FUNCTION ref_phase(z)
+   COMPLEX(RP), INTENT(IN) :: z
+   COMPLEX(RP) :: zt
+   REAL(RP) :: pt
+   ref_phase = 0.0_rp
+   IF (ABS(z) > TINY .OR. ABS(z) == 0.0_rp) RETURN
+   zt = z / ABS(z)
+   pt = ASIN(AIMAG(zt))
+   IF (REAL(zt) >= 0.0_rp) THEN
+      ref_phase = pt
+      IF (AIMAG(zt) < 0.0_rp) ref_phase = TWOPI + pt
+   ELSE
+      ref_phase = PI - pt
+   END IF
+END FUNCTION ref_phase
 !
 ! wmk_ini (wavemaking.f90:119-155): per harmonic, the frequency of its first component and
 ! pos_ini(y) = sum TF cos(angle) a exp(i(-k sin(angle) y + phase)), and its y derivative

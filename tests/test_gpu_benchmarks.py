@@ -11,6 +11,15 @@ digits (D13.5):
   flap hinged 1 m above the bed of a 5 m tank, no ramp, 512 modes, M 3,
   toler 1e-4, 50 s (3386 RK attempts).  FP64: every sample within one unit of
   the record's last digit.  Mixed path: within 1e-3 of each probe's range.
+* Irreg_3D: icase 31, an Ocean .txt file of 2614 fronts (read_ocean_txt.f90:
+  rnumber 13, phases moved to the basin centre line), i_wmk 2, flap, 3-D
+  46.4 x 29.74 m basin, 256 x 32 modes, n3 16, M 3, the first 40 s of the
+  record (the current input's T_stop; the wavemaker ramps down only after it):
+  every sample of the 6 probes within one unit of the last digit.
+* Regular_3D: icase 2 with Dalrymple's directional wavemaker (ibat 3, 30
+  degrees, focus 25 m; wave_def.f90:111-177, its phase() returning 0 for any
+  non-negligible amplitude), i_wmk 1, flap, 256 x 32 modes, 40 s: every sample
+  within one unit of the last digit (3d.dat output switched off).
 * Sloshing: icase 1, cos(3 pi x / L) of amplitude 0.2 m in a 50 m tank, M 1,
   129 modes, 20 s: every sample within one unit of the last digit.  The
   record comes from an older input format (its header) whose mode 4 is the
@@ -100,3 +109,23 @@ def test_sloshing_probes_match_the_reference_record(tmp_path):
     # the antinode probes carry 5 digits; the node probe (x = 25 m) records rounding noise
     compare(got[:, [0, 1, 3]], ref[:, [0, 1, 3]], 1.0 * last_digit(ref[:, [1, 3]]) + 1e-12)
     assert np.abs(got[:, 2]).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_irreg_3d_ocean_file_probes_match_the_reference_record(tmp_path):
+    ref = rows(os.path.join(GOLD, "Irreg_3D", "probes.dat"))
+    got = run_case(tmp_path, "Irreg_3D", (256, 32, 16, 3))
+    ref = ref[: got.shape[0]]
+    assert abs(got[-1, 0] - 40.0) < 1e-9
+    compare(got, ref, 1.0 * last_digit(ref[:, 1:]) + 1e-12)
+    # the .cfg written from the Ocean file (read_ocean_txt.f90:173-185)
+    cfg = (tmp_path / "CAT1__22.cfg").read_text().split("\n")
+    assert [c.strip() for c in cfg[:10]] == ["13", "32.00", "5.00", "32", "lin", "0.00", "100.00", "30.00",
+                                             "hinged", "2.147"]
+
+
+@pytest.mark.gpu
+def test_regular_3d_dalrymple_probes_match_the_reference_record(tmp_path):
+    ref = rows(os.path.join(GOLD, "Regular_3D", "probes.dat"))
+    got = run_case(tmp_path, "Regular_3D", (256, 32, 16, 3), edits={"i3d": 0})
+    compare(got, ref, 1.0 * last_digit(ref[:, 1:]) + 1e-12)
