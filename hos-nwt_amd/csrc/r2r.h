@@ -304,14 +304,30 @@ inline void host_fft_pow2(std::vector<std::complex<double>>& a) {
 // workgroup): the same packed-pair DCT-I/DST-I, with the complex FFT done as
 // Stockham radix passes between two HBM buffers, one launch per pass, each
 // butterfly on its own thread, so one line spreads over the whole chip.
-// ws holds 2 * L * npairs complex values; pair p uses [p*L, (p+1)*L).
+// The line pairs of several items of a batch share each launch: blockIdx.y is
+// the pair across the group, first[i] the first pair of item i.
+// ws holds 2 * L * (pairs of the group) complex values; pair g uses [g*L, (g+1)*L).
 // ---------------------------------------------------------------------------
-template <typename T, class S>
-__global__ void __launch_bounds__(256) k_gline_load(int N, int kind, int dir, int n_in, int nlines, int L, S src,
+struct GPairs {
+  int n;
+  int first[hosfft2::kMaxBatch + 1];
+  __device__ __forceinline__ int item(int g) const {
+    int it = 0;
+    while (it + 1 < n && g >= first[it + 1]) ++it;
+    return it;
+  }
+};
+
+template <typename T, class S, class D>
+__global__ void __launch_bounds__(256) k_gline_load(int N, int dir, int L, hosfft2::LineBatch<S, D> bt, GPairs gp,
                                                     C2<T>* __restrict__ ws) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int pair = blockIdx.y;
+  const int g = blockIdx.y;
+  const int it = gp.item(g);
+  const int pair = g - gp.first[it];
   if (j >= N) return;
+  const int kind = bt.kind[it], n_in = bt.n_in[it], nlines = bt.nlines[it];
+  const S& src = bt.src[it];
   const int la = 2 * pair, lb = la + 1;
   T a = (T)0, b = (T)0;
   if (j < n_in) {
@@ -325,7 +341,7 @@ __global__ void __launch_bounds__(256) k_gline_load(int N, int kind, int dir, in
   }
   C2<T> z = hosfft::mk<T>(a, b);
   if (kind == KSIN && (j == 0 || j == N - 1)) z = hosfft::mk<T>((T)0, (T)0);
-  C2<T>* line = ws + (size_t)pair * L;
+  C2<T>* line = ws + (size_t)g * L;
   line[j] = z;
   if (j >= 1 && j <= N - 2) line[L - j] = (kind == KCOS) ? z : hosfft::mk<T>(-z.x, -z.y);
 }
@@ -359,14 +375,18 @@ __global__ void __launch_bounds__(256) k_gline_pass(int L, int Ns, const C2<T>* 
   for (int r = 0; r < R; ++r) dst[r * Ns] = v[r];
 }
 
-template <typename T, class D>
-__global__ void __launch_bounds__(256) k_gline_store(int N, int kind, int dir, int n_out, int nlines, int L,
-                                                     const C2<T>* __restrict__ ws, D dst) {
+template <typename T, class S, class D>
+__global__ void __launch_bounds__(256) k_gline_store(int N, int dir, int L, const C2<T>* __restrict__ ws,
+                                                     hosfft2::LineBatch<S, D> bt, GPairs gp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int pair = blockIdx.y;
+  const int g = blockIdx.y;
+  const int it = gp.item(g);
+  const int pair = g - gp.first[it];
+  const int kind = bt.kind[it], n_out = bt.n_out[it], nlines = bt.nlines[it];
+  const D& dst = bt.dst[it];
   if (k >= n_out) return;
   const int la = 2 * pair, lb = la + 1;
-  const C2<T> Z = ws[(size_t)pair * L + k];
+  const C2<T> Z = ws[(size_t)g * L + k];
   T A, B;
   if (kind == KCOS) {
     A = Z.x;
@@ -534,7 +554,10 @@ struct Grid2 {
                                ": line transform longer than the supported lengths");
   }
   // T values of the HBM workspace of long x lines (0 when not needed)
-  size_t gline_ws_elems() const { return px.glob ? (size_t)4 * px.dev.L * ((N2 + 1) / 2) : 0; }
+  // line pairs the HBM workspace holds: one item's lines, and in a 2-D tank (one
+  // line per item) every item of a batch, so that a batch takes one set of launches
+  int gline_ws_pairs() const { return px.glob ? std::max((N2 + 1) / 2, N2 == 1 ? hosfft2::kMaxBatch : 1) : 0; }
+  size_t gline_ws_elems() const { return (size_t)4 * px.dev.L * gline_ws_pairs(); }
   void release() {
     px.release();
     py.release();
@@ -611,7 +634,8 @@ void launch_reg(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<
 
 // x passes of every item of a batch (lines of item i: nlines[i] x-lines).
 template <typename T, class S, class D>
-void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st, T* gws = nullptr) {
+void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st, T* gws = nullptr,
+              int ws_pairs = 0) {
   const int nl = batch_max_lines(b);
   if (nl <= 0) return;
   if (pl.glob) {
@@ -619,17 +643,39 @@ void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, h
     const int L = pl.dev.L;
     if (L != 2 * (N - 1)) throw std::runtime_error("long x line plan does not match N");
     C2<T>* w0 = reinterpret_cast<C2<T>*>(gws);
-    for (int i = 0; i < b.n; ++i) {
-      if (b.nlines[i] <= 0 || b.n_out[i] <= 0) continue;
-      const int npairs = (b.nlines[i] + 1) / 2;
+    int i = 0;
+    while (i < b.n) {
+      // a group of consecutive items whose line pairs fit the workspace
+      LineBatch<S, D> sub;
+      GPairs gp{};
+      int tot = 0, max_out = 0;
+      for (; i < b.n; ++i) {
+        if (b.nlines[i] <= 0 || b.n_out[i] <= 0) continue;
+        const int np = (b.nlines[i] + 1) / 2;
+        if (np > ws_pairs) throw std::runtime_error("long x lines: workspace too small for one item");
+        if (sub.n > 0 && tot + np > ws_pairs) break;
+        const int k = sub.n++;
+        sub.kind[k] = b.kind[i];
+        sub.n_in[k] = b.n_in[i];
+        sub.n_out[k] = b.n_out[i];
+        sub.nlines[k] = b.nlines[i];
+        sub.src[k] = b.src[i];
+        sub.dst[k] = b.dst[i];
+        gp.first[k] = tot;
+        tot += np;
+        max_out = std::max(max_out, b.n_out[i]);
+      }
+      if (sub.n == 0) break;
+      gp.n = sub.n;
+      gp.first[sub.n] = tot;
       C2<T>* in = w0;
-      C2<T>* out = w0 + (size_t)L * npairs;
-      hipLaunchKernelGGL((k_gline_load<T, S>), dim3((N + 255) / 256, npairs), dim3(256), 0, st, N, b.kind[i], dir,
-                         b.n_in[i], b.nlines[i], L, b.src[i], in);
+      C2<T>* out = w0 + (size_t)L * tot;
+      hipLaunchKernelGGL((k_gline_load<T, S, D>), dim3((N + 255) / 256, tot), dim3(256), 0, st, N, dir, L, sub, gp,
+                         in);
       int Ns = 1;
       for (int p = 0; p < pl.dev.npass; ++p) {
         const int R = pl.dev.radix[p];
-        const dim3 g((L / R + 255) / 256, npairs);
+        const dim3 g((L / R + 255) / 256, tot);
         switch (R) {
           case 8: hipLaunchKernelGGL((k_gline_pass<T, 8>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
           case 4: hipLaunchKernelGGL((k_gline_pass<T, 4>), g, dim3(256), 0, st, L, Ns, in, out, pl.dev.tw); break;
@@ -641,8 +687,8 @@ void launch_x(const LinePlan<T>& pl, int N, int dir, const LineBatch<S, D>& b, h
         Ns *= R;
         std::swap(in, out);
       }
-      hipLaunchKernelGGL((k_gline_store<T, D>), dim3((b.n_out[i] + 255) / 256, npairs), dim3(256), 0, st, N,
-                         b.kind[i], dir, b.n_out[i], b.nlines[i], L, (const C2<T>*)in, b.dst[i]);
+      hipLaunchKernelGGL((k_gline_store<T, S, D>), dim3((max_out + 255) / 256, tot), dim3(256), 0, st, N, dir, L,
+                         (const C2<T>*)in, sub, gp);
       HIP_OK(hipGetLastError());
     }
     return;
@@ -731,7 +777,7 @@ void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>
     if (y)
       launch_y<T>(g.py, g.TX, g.N2, dir, bb, st);
     else
-      launch_x<T>(g.px, g.N1, dir, bb, st, gws);
+      launch_x<T>(g.px, g.N1, dir, bb, st, gws, g.gline_ws_pairs());
   };
   if (!serial) {
     go(b);
@@ -775,10 +821,10 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
         one.nlines[0] = 1;
         one.src[0] = x.src[i];
         one.dst[0] = x.dst[i];
-        launch_x<T>(g.px, g.N1, dir, one, st, gws);
+        launch_x<T>(g.px, g.N1, dir, one, st, gws, g.gline_ws_pairs());
       }
     } else {
-      launch_x<T>(g.px, g.N1, dir, x, st, gws);
+      launch_x<T>(g.px, g.N1, dir, x, st, gws, g.gline_ws_pairs());
     }
     return;
   }
@@ -815,7 +861,7 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
     p2.dst[i] = b.dst[i];
   }
   if (!yfirst) {
-    launch_x<T>(g.px, g.N1, dir, p1, st, gws);
+    launch_x<T>(g.px, g.N1, dir, p1, st, gws, g.gline_ws_pairs());
     launch_second<T, D>(true, g, dir, p2, st, serial, gws);
   } else {
     launch_y<T>(g.py, g.TX, g.N2, dir, p1, st);
