@@ -2221,7 +2221,7 @@ struct Engine : EngineBase {
       HIP_OK(hipMemsetAsync(phiz_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phit_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phizh, 0, ns * sizeof(T), st));
-      if constexpr (kSplit) phiz_modes();  // the beach's first-order term
+      if constexpr (kSplit) surface_forcing_modes();  // the beach's first-order term
       HIP_OK(hipMemsetAsync(phixa, 0, nb * sizeof(T), st));
       HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
       HIP_OK(hipMemsetAsync(phiza, 0, nb * sizeof(T), st));
@@ -2258,7 +2258,7 @@ struct Engine : EngineBase {
     // grad phi_add onto the extended grid (resol_HOS.f90:534-544)
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
-    phiz_modes();
+    surface_forcing_modes();
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
@@ -2272,7 +2272,7 @@ struct Engine : EngineBase {
   // modes of phiz_add in T (the extended-grid gradient products); on the mixed
   // path from the FP64 modes, which the free-surface condition uses directly, and
   // the FP64 modes of the first-order part -phit_add - nu (phiz_add + F2S(kth1 phis))
-  void phiz_modes() {
+  void surface_forcing_modes() {
     const int n1 = c->n1, n2 = c->n2;
     if constexpr (kSplit) {
       base64(KCOS, KCOS, FWD, src_plain<double, double>(phiz_add, n1, n1, n2), n1, n2, Dst<double>{phizh64, n1, 0});
@@ -2438,7 +2438,7 @@ struct Engine : EngineBase {
     // conditions above have used the first-order values already.
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
-    phiz_modes();
+    surface_forcing_modes();
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
