@@ -210,8 +210,8 @@ template <typename T, int V, int R, bool YP, class D>
 __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl, int ntl, int L, int N, int kind,
                                            int dir, int n_out, int la, int lb, bool ha, bool hb) {
   const T inv = (T)(1.0 / (double)L);
-  // outputs whose loads are issued together: 8, or 4 for FP64 epilogues that
-  // read several fields per output (fewer spills at two waves per SIMD)
+  // outputs whose loads are issued together: 8, or 4 for FP64 destinations
+  // that read back several values per output (fewer spills at two waves per SIMD)
   constexpr int CH = (sizeof(T) == 8 && sizeof(typename D::Pre) > 2 * sizeof(T)) ? 4 : 8;
 #pragma unroll
   for (int i0 = 0; i0 < V; i0 += CH) {
@@ -270,10 +270,10 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
 // column tiles are visited in an XCD-aware order so that neighbouring tiles
 // share an L2.  Spectral multipliers (kth, -kx, -ky) are evaluated once per
 // element in a rolled loop into LDS before the loads.
-// FP64 instances are held to two waves per SIMD (at most 256 registers): the
-// HOS epilogue then spills a few dwords, which costs less than running at one
-// wave per SIMD.  FP32 bounds (x: free, y: 4 waves, 3 with the HOS epilogue)
-// measured best among x 1-3 / y 3-4 / y-HOS 2-3 on the 3-D mixed step.
+// FP64 instances are held to two waves per SIMD (at most 256 registers).
+// FP32 bounds (x: free, y: 4 waves; 3 for destinations that read back more
+// than one value per output) measured best among x 1-3 and y 3-4 waves on the
+// 3-D mixed step.
 // Independent line transforms sharing one launch (blockIdx.y = item): same
 // length, direction and functor types; kind, extents, source and destination
 // per item.  A workgroup whose lines lie beyond its item's extent exits at

@@ -117,36 +117,6 @@ struct Dst {
   __device__ __forceinline__ void put(int ix, int iy, T v) const { put(ix, iy, v, pre(ix, iy, true)); }
 };
 
-// HOS recursion epilogue (resol_HOS.f90:737-769): with phizi = v,
-//   phimext -= phizi * eta^j/j!   and   phizm(iHOS) += phizi * eta^(j-1)/(j-1)!
-// (`first` overwrites instead of accumulating; ejm1 == nullptr means factor 1).
-template <typename T>
-struct DstHos {
-  T* __restrict__ phim;
-  T* __restrict__ phizm;
-  const T* __restrict__ ej;
-  const T* __restrict__ ejm1;
-  int ld, first, do_phim;
-  struct Pre {
-    T e, e1, pm, pz;
-  };
-  __device__ __forceinline__ Pre pre(int ix, int iy, bool ok) const {
-    const size_t i = ok ? ix + (size_t)ld * iy : 0;
-    Pre q;
-    q.e = do_phim ? ej[i] : (T)0;
-    q.e1 = ejm1 ? ejm1[i] : (T)1;
-    q.pm = (do_phim && !first) ? phim[i] : (T)0;
-    q.pz = first ? (T)0 : phizm[i];
-    return q;
-  }
-  __device__ __forceinline__ void put(int ix, int iy, T v, const Pre& q) const {
-    const size_t i = ix + (size_t)ld * iy;
-    if (do_phim) phim[i] = q.pm - v * q.e;
-    phizm[i] = q.pz + v * q.e1;
-  }
-  __device__ __forceinline__ void put(int ix, int iy, T v) const { put(ix, iy, v, pre(ix, iy, true)); }
-};
-
 // ---------------------------------------------------------------------------
 // line kernels
 // ---------------------------------------------------------------------------
@@ -765,40 +735,21 @@ struct Batch2 {
   }
 };
 
-// serial: the second pass runs one launch per item, in order (destinations
-// that accumulate into shared arrays).  yfirst: the y pass runs first and the
-// x pass, whose lines are contiguous, writes the destination (the better
-// access pattern for read-modify-write epilogues); tmp[i] then holds
-// min(src.nx, N1) x nout2[i] values.
+// yfirst: the y pass runs first (on the src.nx columns that hold data) and the
+// x pass, whose lines are contiguous, writes the destination; tmp[i] then
+// holds min(src.nx, N1) x nout2[i] values.
 template <typename T, class D>
-void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>, D>& b, hipStream_t st, bool serial,
+void launch_second(bool y, const Grid2<T>& g, int dir, const LineBatch<Src<T, T>, D>& b, hipStream_t st,
                    T* gws = nullptr) {
-  auto go = [&](const LineBatch<Src<T, T>, D>& bb) {
-    if (y)
-      launch_y<T>(g.py, g.TX, g.N2, dir, bb, st);
-    else
-      launch_x<T>(g.px, g.N1, dir, bb, st, gws, g.gline_ws_pairs());
-  };
-  if (!serial) {
-    go(b);
-    return;
-  }
-  for (int i = 0; i < b.n; ++i) {
-    LineBatch<Src<T, T>, D> one;
-    one.n = 1;
-    one.kind[0] = b.kind[i];
-    one.n_in[0] = b.n_in[i];
-    one.n_out[0] = b.n_out[i];
-    one.nlines[0] = b.nlines[i];
-    one.src[0] = b.src[i];
-    one.dst[0] = b.dst[i];
-    go(one);
-  }
+  if (y)
+    launch_y<T>(g.py, g.TX, g.N2, dir, b, st);
+  else
+    launch_x<T>(g.px, g.N1, dir, b, st, gws, g.gline_ws_pairs());
 }
 
 template <typename T, class S, class D>
-void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream_t st, bool serial = false,
-                 bool yfirst = false, T* gws = nullptr) {
+void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream_t st, bool yfirst = false,
+                 T* gws = nullptr) {
   if (b.n == 0) return;
   if (g.N2 == 1) {
     LineBatch<S, D> x;
@@ -811,21 +762,7 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
       x.src[i] = b.src[i];
       x.dst[i] = b.dst[i];
     }
-    if (serial) {
-      for (int i = 0; i < b.n; ++i) {
-        LineBatch<S, D> one;
-        one.n = 1;
-        one.kind[0] = x.kind[i];
-        one.n_in[0] = x.n_in[i];
-        one.n_out[0] = x.n_out[i];
-        one.nlines[0] = 1;
-        one.src[0] = x.src[i];
-        one.dst[0] = x.dst[i];
-        launch_x<T>(g.px, g.N1, dir, one, st, gws, g.gline_ws_pairs());
-      }
-    } else {
-      launch_x<T>(g.px, g.N1, dir, x, st, gws, g.gline_ws_pairs());
-    }
+    launch_x<T>(g.px, g.N1, dir, x, st, gws, g.gline_ws_pairs());
     return;
   }
   LineBatch<S, Dst<T>> p1;
@@ -862,10 +799,10 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
   }
   if (!yfirst) {
     launch_x<T>(g.px, g.N1, dir, p1, st, gws, g.gline_ws_pairs());
-    launch_second<T, D>(true, g, dir, p2, st, serial, gws);
+    launch_second<T, D>(true, g, dir, p2, st, gws);
   } else {
     launch_y<T>(g.py, g.TX, g.N2, dir, p1, st);
-    launch_second<T, D>(false, g, dir, p2, st, serial, gws);
+    launch_second<T, D>(false, g, dir, p2, st, gws);
   }
 }
 
@@ -876,7 +813,7 @@ void r2r2d(const Grid2<T>& g, int kx, int ky, int dir, const S& src, int nout1, 
            hipStream_t st, T* gws = nullptr) {
   Batch2<T, S, D> b;
   b.add(kx, ky, src, nout1, nout2, dst, tmp);
-  r2r2d_batch<T>(g, dir, b, st, false, false, gws);
+  r2r2d_batch<T>(g, dir, b, st, false, gws);
 }
 
 }  // namespace hosnwt
