@@ -1825,7 +1825,7 @@ struct Engine : EngineBase {
       phizm[j] = ar.get<T>(nb);
       if (j >= 2) aHOS[j] = ar.get<T>(nb);
     }
-    for (int i = 0; i < kMaxBatch && i < std::max(M, 5); ++i) tmpX.push_back(ar.get<T>(nb));
+    for (int i = 0; i < kMaxBatch && i < std::max(M, 7); ++i) tmpX.push_back(ar.get<T>(nb));
     for (int i = 0; i < M; ++i) lvl.push_back(ar.get<T>(nb));
     etax = ar.get<T>(nb); etay = ar.get<T>(nb); phisx = ar.get<T>(nb); phisy = ar.get<T>(nb);
     g2 = ar.get<T>(nb); phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
@@ -2068,6 +2068,9 @@ struct Engine : EngineBase {
         b.add(KCOS, KSIN, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
               Dst<T>{phisy, ldb, 0}, tmpX[4]);
       }
+      // the first HOS level's one transform (kth1 phis) rides along
+      b.add(KCOS, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, 1, pix, piy), N1b, N2b, Dst<T>{lvl[0], ldb, 0},
+            tmpX[b.n]);
       bigB(INV, b);
     }
     PtrSet<T> ep{};
@@ -2084,7 +2087,7 @@ struct Engine : EngineBase {
       // order (k_hos_level)
       Batch2<T, Src<T, T>, Dst<T>> b;
       PtrSet<T> lv{};
-      for (int j = iH; j >= 1; --j) {
+      for (int j = iH; j >= 1 && iH > 1; --j) {
         const int j2 = iH - j + 1;
         lv.p[b.n] = lvl[b.n];
         if (j2 == 1) {
@@ -2097,7 +2100,10 @@ struct Engine : EngineBase {
                 N1b, N2b, Dst<T>{lvl[b.n], ldb, 0}, tmpX[b.n]);
         }
       }
-      bigB(INV, b, n2 > 1);
+      if (iH == 1)
+        lv.p[0] = lvl[0];  // transformed with the slopes
+      else
+        bigB(INV, b, n2 > 1);
       hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, ep, phim, phizm[iH]);
       if (iH < M) {
         if (iH + 1 == M)
@@ -2109,25 +2115,31 @@ struct Engine : EngineBase {
     // phiz = filter_ext(sum_j phizm(j)) -> modal P (orders >= 2 on the mixed path)
     PtrSet<T> zz{};
     for (int j = 1; j <= M; ++j) zz.p[j] = phizm[j];
-    hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, kSplit ? 2 : 1, zz, X);
-    big(KCOS, KCOS, FWD, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0});
+    // (the order sum goes to lvl[0], free after the last level)
+    T* zsum = lvl[0];
+    hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, kSplit ? 2 : 1, zz, zsum);
     // --- join the wavemaker chain (solveHOS_SWEET_1 + phiadd_SL) ---
     HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
-    // --- free-surface products, filtered to the base modes ---
+    // --- free-surface products, filtered to the base modes; P, the products and
+    // the mixed-parity spectra in one batch ---
+    {
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      b.add(KCOS, KCOS, src_plain<T>(zsum, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0}, tmpX[0]);
+      if (M >= 2) {
+        if (n2 == 1) {
+          HIP_OK(hipMemsetAsync(etay, 0, nb * sizeof(T), st));
+          HIP_OK(hipMemsetAsync(phisy, 0, nb * sizeof(T), st));
+        }
+        hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
+                           phixa, phiya, phiza, X, Y, XA, YA, XB, YB);
+        b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[1]);
+        b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[2]);
+        if (C.wmk) mixed_parity_spectra(b);
+      }
+      bigB(FWD, b);
+    }
     if (M >= 2) {
-      if (n2 == 1) {
-        HIP_OK(hipMemsetAsync(etay, 0, nb * sizeof(T), st));
-        HIP_OK(hipMemsetAsync(phisy, 0, nb * sizeof(T), st));
-      }
-      hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
-                         phixa, phiya, phiza, X, Y, XA, YA, XB, YB);
-      {
-        Batch2<T, Src<T, T>, Dst<T>> b;
-        b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[0]);
-        b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[1]);
-        bigB(FWD, b);
-      }
-      if (C.wmk) mixed_parity();
+      if (C.wmk) mixed_parity_finish();
       hipLaunchKernelGGL(k_add3<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)D1, (const T*)P, Q);
     } else {
       HIP_OK(hipMemcpyAsync(Q, P, ns * sizeof(T), hipMemcpyDeviceToDevice, st));
@@ -2156,18 +2168,21 @@ struct Engine : EngineBase {
   }
 
   // filter_ext of the mixed-parity products on the reference grid: exact sine
-  // spectra on the padded grid, then the reference grid's sine-to-cosine map.
-  void mixed_parity() {
+  // spectra on the padded grid (added to the caller's batch), then the
+  // reference grid's sine-to-cosine map (mixed_parity_finish).
+  void mixed_parity_spectra(Batch2<T, Src<T, T>, Dst<T>>& b) {
     hos_ctx& C = *c;
     const int n1 = C.n1, n2 = C.n2, N1b = C.N1b, N2b = C.N2b, ldb = N1b;
-    Batch2<T, Src<T, T>, Dst<T>> b;
-    b.add(KSIN, KCOS, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0}, tmpX[0]);
-    b.add(KSIN, KCOS, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0}, tmpX[1]);
+    b.add(KSIN, KCOS, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0}, tmpX[b.n]);
+    b.add(KSIN, KCOS, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0}, tmpX[b.n]);
     if (n2 > 1) {
-      b.add(KCOS, KSIN, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, Dst<T>{PB, 2 * n1, 0}, tmpX[2]);
-      b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0}, tmpX[3]);
+      b.add(KCOS, KSIN, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, Dst<T>{PB, 2 * n1, 0}, tmpX[b.n]);
+      b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0}, tmpX[b.n]);
     }
-    bigB(FWD, b);
+  }
+  void mixed_parity_finish() {
+    hos_ctx& C = *c;
+    const int n1 = C.n1, n2 = C.n2;
     gemm(cb, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
     if (n2 > 1) gemm(cb, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
     hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, (const T*)RA,
