@@ -1414,6 +1414,19 @@ inline void gemv_t(rocblas_handle h, int m, int k, const float* AT, const float*
   const float one = 1.0f, zero = 0.0f;
   blas_ok(rocblas_sgemv(h, rocblas_operation_transpose, k, m, &one, AT, k, x, 1, &zero, y, 1), "sgemv");
 }
+// C_b (m x n) = A_b (m x n) B^T (n x n) for b < batch, A_b and C_b stride s apart
+inline void gemm_yb(rocblas_handle h, int m, int n, const double* A, long s, const double* B, double* Cm, int batch) {
+  const double one = 1.0, zero = 0.0;
+  blas_ok(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m, n, n, &one, A, m, s,
+                                        B, n, 0, &zero, Cm, m, s, batch),
+          "dgemm batched");
+}
+inline void gemm_yb(rocblas_handle h, int m, int n, const float* A, long s, const float* B, float* Cm, int batch) {
+  const float one = 1.0f, zero = 0.0f;
+  blas_ok(rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m, n, n, &one, A, m, s,
+                                        B, n, 0, &zero, Cm, m, s, batch),
+          "sgemm batched");
+}
 
 // ---------------------------------------------------------------------------
 // spectral filters (filtering.f90:334-525)
@@ -1798,6 +1811,17 @@ struct Engine : EngineBase {
   bool dense = false;
   DenseGrid<T> dT;
   DenseGrid<double> d64;
+  // 3-D dense base grid: the wavemaker chain's base transforms of one kind and
+  // direction go through one x product for all fields and one strided batch of
+  // y products (kMulti fields at most); staging, intermediate and output scratch
+  static constexpr int kMulti = 4;
+  double* multi = nullptr;
+  DenseGrid<double>& grid64() {
+    if constexpr (sizeof(T) == sizeof(double))
+      return dT;
+    else
+      return d64;
+  }
 
   explicit Engine(hos_ctx* ctx) : c(ctx) {
     Arena& ar = c->ar;
@@ -1876,6 +1900,7 @@ struct Engine : EngineBase {
     if (dense) {
       build_dense<T>(dT, ar, c->n1, c->n2);
       if (sizeof(T) != sizeof(double)) build_dense<double>(d64, ar, c->n1, c->n2);
+      if (c->n2 > 1) multi = ar.get<double>((size_t)3 * kMulti * ns);
     }
     const int nf = c->p.n_freq;
     std::vector<double> hf(c->p.freq, c->p.freq + nf);
@@ -2016,6 +2041,24 @@ struct Engine : EngineBase {
                        hipMemcpyDeviceToHost));
     HIP_OK(hipMemcpy2D(xt, sizeof(double), Xpt + (n3 - 1), (size_t)n3 * sizeof(double), sizeof(double), n2,
                        hipMemcpyDeviceToHost));
+  }
+
+  // out[f] = the cos-cos base transform of src[f], f < k (dense 3-D grid, stream cs)
+  template <typename U>
+  void dense_multi(DenseGrid<U>& g, int dir, int k, const Src<U, double>* src, U* const* out) {
+    const int n1 = c->n1, n2 = c->n2, di = (dir == FWD) ? 0 : 1;
+    if (k > kMulti || !multi) throw std::runtime_error("dense_multi: no scratch for this batch");
+    U* stage = reinterpret_cast<U*>(multi);
+    U* mid = stage + (size_t)kMulti * ns;
+    U* res = mid + (size_t)kMulti * ns;
+    for (int f = 0; f < k; ++f)
+      hipLaunchKernelGGL((k_dense_in<U, Src<U, double>>), dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, src[f],
+                         stage + (size_t)f * ns);
+    gemm(cb, rocblas_operation_none, n1, k * n2, n1, g.mx[KCOS][di], n1, stage, n1, mid, n1);
+    gemm_yb(cb, n1, n2, mid, (long)ns, g.my[KCOS][di], res, k);
+    for (int f = 0; f < k; ++f)
+      HIP_OK(hipMemcpyAsync(out[f], res + (size_t)f * ns, ns * sizeof(U), hipMemcpyDeviceToDevice, cs));
+    c->stats.n_base += k;
   }
 
   void base_cc64(int dir, const double* in, double* out) override {
@@ -2236,7 +2279,27 @@ struct Engine : EngineBase {
     paddle_modes(2, kad1, kadr);
     // additional potential at z = eta and its first-order part at z = 0
     phiadd(aad, kad1, eta_phys);
-    // first-order free-surface RHS (phidphiz)
+    if (multi) {
+      first_order_batched(e1, a1, ka1, ke1);
+    } else {
+      first_order_single(e1, a1, ka1, ke1);
+    }
+    {
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
+      if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, (*ctmpX)[1]);
+      b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
+      bigB(INV, b);
+      if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+    }
+  }
+
+  // First-order free-surface RHS (phidphiz) and the modes of grad phi_add, one
+  // base transform at a time.
+  void first_order_single(const double* e1, const double* a1, double* ka1, double* ke1) {
+    hos_ctx& C = *c;
+    hipStream_t st = cs;
+    const int n1 = C.n1, n2 = C.n2;
     base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
            Dst<double>{w1, n1, 0});
     hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
@@ -2247,13 +2310,46 @@ struct Engine : EngineBase {
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
     base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
     surface_forcing_modes();
+  }
+
+  // The same on a dense 3-D base grid, the transforms of one direction and
+  // precision batched (dense_multi): inverse {kth1 a1st, kth1 phis (mixed)},
+  // forward FP64 {phidphiz, phiz_add, forcing (mixed)} or {phidphiz, grad phi_add}
+  // (FP64), forward FP32 {phix_add, phiy_add} (mixed).
+  void first_order_batched(const double* e1, const double* a1, double* ka1, double* ke1) {
+    hos_ctx& C = *c;
+    hipStream_t st = cs;
+    const int n1 = C.n1, n2 = C.n2;
     {
-      Batch2<T, Src<T, T>, Dst<T>> b;
-      b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
-      if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, (*ctmpX)[1]);
-      b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
-      bigB(INV, b);
-      if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+      // (the FP64 path transforms the first field only)
+      Src<double, double> in[2] = {src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy),
+                                   src_mul<double, double>(kSplit ? phis64 : a1, n1, n1, n2, MUL_KTH, 1, C.pix,
+                                                           C.piy)};
+      double* out[2] = {w1, wlin};
+      dense_multi<double>(grid64(), INV, kSplit ? 2 : 1, in, out);
+    }
+    hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
+                       (const double*)phiz1, (const double*)phit1, e1, (const double*)C.nu_d,
+                       (double)C.p.iabsnb, ke1, tmp1);
+    if constexpr (kSplit) {
+      hipLaunchKernelGGL(k_forcing, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phiz_add,
+                         (const double*)phit_add, (const double*)wlin, (const double*)C.nu_d, Zf64);
+      Src<double, double> in[3] = {src_plain<double, double>(tmp1, n1, n1, n2),
+                                   src_plain<double, double>(phiz_add, n1, n1, n2),
+                                   src_plain<double, double>(Zf64, n1, n1, n2)};
+      double* out[3] = {ka1, phizh64, Zh64};
+      dense_multi<double>(grid64(), FWD, 3, in, out);
+      hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phizh64, phizh);
+      Src<T, double> inT[2] = {src_plain<T, double>(phix_add, n1, n1, n2), src_plain<T, double>(phiy_add, n1, n1, n2)};
+      T* outT[2] = {phixh, phiyh};
+      dense_multi<T>(dT, FWD, 2, inT, outT);
+    } else {
+      Src<double, double> in[4] = {src_plain<double, double>(tmp1, n1, n1, n2),
+                                   src_plain<double, double>(phix_add, n1, n1, n2),
+                                   src_plain<double, double>(phiy_add, n1, n1, n2),
+                                   src_plain<double, double>(phiz_add, n1, n1, n2)};
+      double* out[4] = {ka1, (double*)phixh, (double*)phiyh, (double*)phizh};
+      dense_multi<double>(grid64(), FWD, 4, in, out);
     }
   }
 
