@@ -29,6 +29,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include <rocblas/rocblas.h>
@@ -1635,13 +1636,23 @@ void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx,
   if (d.mode != 0) throw std::runtime_error("dense base transform: plain stores only");
   const int n1 = g.n1, n2 = g.n2, di = (dir == FWD) ? 0 : 1;
   const size_t n = (size_t)n1 * n2;
-  hipLaunchKernelGGL((k_dense_in<U, S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n1, n2, src, g.stage);
-  HIP_OK(hipGetLastError());
+  // a plain source of the product's type and layout is read in place
+  const U* in = g.stage;
+  bool staged = true;
+  if constexpr (std::is_same<S, Src<U, U>>::value)
+    if (src.mul == MUL_NONE && src.ld == n1 && src.nx == n1 && src.ny == n2 && (const void*)src.p != d.p) {
+      in = src.p;
+      staged = false;
+    }
+  if (staged) {
+    hipLaunchKernelGGL((k_dense_in<U, S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n1, n2, src, g.stage);
+    HIP_OK(hipGetLastError());
+  }
   if (n2 == 1) {
-    gemv_t(blas, nout1, n1, g.mx[kx][di], g.stage, d.p);
+    gemv_t(blas, nout1, n1, g.mx[kx][di], in, d.p);
     return;
   }
-  gemm(blas, rocblas_operation_none, nout1, n2, n1, g.mx[kx][di], n1, g.stage, n1, g.mid, n1);
+  gemm(blas, rocblas_operation_none, nout1, n2, n1, g.mx[kx][di], n1, in, n1, g.mid, n1);
   gemm(blas, rocblas_operation_transpose, nout1, nout2, n2, g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
 }
 
