@@ -3276,13 +3276,14 @@ int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_nex
       for (int k = 0; k < nth; ++k) th.emplace_back(run, std::cref(part[k]));
       for (auto& t : th) t.join();
     }
-    if (!first_error.empty()) throw std::runtime_error(first_error);
+    // the members' committed times, proposals and counts, also when one failed
     for (int i = 0; i < n; ++i) {
       t_cur[i] = a[i].t;
       h_rk[i] = a[i].h;
       if (n_acc) n_acc[i] = a[i].na;
       if (n_rej) n_rej[i] = a[i].nr;
     }
+    if (!first_error.empty()) throw std::runtime_error(first_error);
   });
 }
 

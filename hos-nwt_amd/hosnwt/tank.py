@@ -276,6 +276,10 @@ def advance_ensemble(tanks, t_next: float, h_rk):
     """hos_advance_ensemble: every tank to t_next, the members' attempts overlapping on the GPU.
     Returns (h proposals, accepted counts, rejected counts); each tank's time is updated."""
     n = len(tanks)
+    if n == 0:
+        return np.zeros(0), [], []
+    if len(h_rk) != n:
+        raise ValueError("one step proposal per tank")
     lib = tanks[0].lib
     ctx = (C.c_void_p * n)(*[t.h.value for t in tanks])
     t = np.array([tk.time for tk in tanks], dtype=np.float64)
@@ -287,9 +291,11 @@ def advance_ensemble(tanks, t_next: float, h_rk):
     toler = min(tk.cfg.toler for tk in tanks)
     if any(tk.cfg.toler != toler for tk in tanks):
         raise ValueError("the ensemble shares one RK tolerance")
-    check(lib.hos_advance_ensemble(n, ctx, ptr(t), t_next, ptr(h), toler, ptr(stab), ptr(dout), na, nr))
+    rc = lib.hos_advance_ensemble(n, ctx, ptr(t), t_next, ptr(h), toler, ptr(stab), ptr(dout), na, nr)
+    # the library reports every member's committed progress, also on an error
     for i, tk in enumerate(tanks):
         tk.time = float(t[i])
         tk.n_rk_tot += na[i]
         tk.n_er_tot += nr[i]
+    check(rc)
     return h, list(na), list(nr)

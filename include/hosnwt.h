@@ -124,7 +124,8 @@ int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double t
  * the decisions hos_advance would take for it alone (HOS-NWT.f90:498-571).
  * The members are driven from HOSNWT_ENSEMBLE_THREADS host threads (default
  * 4); no other thread may use them during the call.  A context may appear
- * only once. */
+ * only once.  On an error the arrays still hold every member's last committed
+ * time, step proposal and counts. */
 int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
                          const double* dt_stab, const double* delt_out, int* n_acc, int* n_rej);
 
