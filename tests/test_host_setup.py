@@ -75,3 +75,23 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert names <= set(lib.EXPORTS) | {"hos_last_error"}
+
+
+def test_ensemble_wrapper_argument_checks():
+    """advance_ensemble: an empty ensemble is a no-op; proposals must match the tanks;
+    the members share one tolerance (checked before the library is called)."""
+    from types import SimpleNamespace
+
+    from hosnwt.tank import advance_ensemble
+
+    h, na, nr = advance_ensemble([], 1.0, [])
+    assert len(h) == 0 and na == [] and nr == []
+
+    def stub(toler):
+        return SimpleNamespace(lib=None, h=SimpleNamespace(value=1), time=0.0,
+                               s=SimpleNamespace(dt_stab=0.1, delt_out=1.0), cfg=SimpleNamespace(toler=toler))
+
+    with pytest.raises(ValueError, match="proposal"):
+        advance_ensemble([stub(1e-7), stub(1e-7)], 1.0, [0.01])
+    with pytest.raises(ValueError, match="tolerance"):
+        advance_ensemble([stub(1e-7), stub(1e-6)], 1.0, [0.01, 0.01])
