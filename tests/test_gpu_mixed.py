@@ -128,3 +128,31 @@ def test_mixed_adaptive_run_from_rest(name):
             tank32.close()
     finally:
         tank64.close()
+
+
+@pytest.mark.gpu
+def test_mixed_directional_sea_at_output_times():
+    """The directional-sea workload (1024 x 256, M = 5, 256 x 32 components), both paths
+    free-running from rest through the ramp to three output times (Tp/2, Tp, 3Tp/2):
+    eta within 1e-5 relative L2 of the FP64 path at each."""
+    import bench
+
+    s64, s32 = bench.make_setup(0), bench.make_setup(1)
+    outs = [k * s64.delt_out / 2.0 for k in (1, 2, 3)]
+    tanks = [Tank(s64), Tank(s32)]
+    try:
+        h = [min(0.95 * s64.dt_stab, s64.delt_out)] * 2
+        for tk in tanks:
+            tk.initialize()
+        for t_out in outs:
+            for i, tk in enumerate(tanks):
+                h[i], _, _ = tk.advance(t_out, h[i])
+            e64, e32 = tanks[0].physical()["eta"], tanks[1].physical()["eta"]
+            r = rel(e32, e64)
+            print(f"t = {t_out:.3f}: attempts FP64 {tanks[0].n_rk_tot} + {tanks[0].n_er_tot} rejected, mixed "
+                  f"{tanks[1].n_rk_tot} + {tanks[1].n_er_tot}; eta max {np.abs(e64).max():.3e}, rel L2 {r:.2e}",
+                  flush=True)
+            assert np.abs(e64).max() > 0 and r < 1e-5
+    finally:
+        for tk in tanks:
+            tk.close()
