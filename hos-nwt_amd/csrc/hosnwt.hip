@@ -224,11 +224,15 @@ template <typename T>
 __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, const T* __restrict__ py,
                        const T* __restrict__ ex, const T* __restrict__ ey, const T* __restrict__ g2,
                        const T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ az, T* X, T* Y,
-                       T* XA, T* YA, T* XB, T* YB) {
+                       T* XA, T* YA, T* XB, T* YB, int j0, T* zsum) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   T z[10];
   for (int j = 1; j <= M; ++j) z[j] = zz.p[j][i];
+  // sum_{j0 <= j <= M} z_j as k_sum_orders forms it
+  T zs = (j0 <= M) ? z[j0] : (T)0;
+  for (int j = j0 + 1; j <= M; ++j) zs += z[j];
+  zsum[i] = zs;
   const T g = g2[i];
   T sgz = (T)0;
   for (int j = 1; j <= M - 3; ++j) sgz += z[j];
@@ -2170,8 +2174,11 @@ struct Engine : EngineBase {
     PtrSet<T> zz{};
     for (int j = 1; j <= M; ++j) zz.p[j] = phizm[j];
     // (the order sum goes to lvl[0], free after the last level)
+    // (with M >= 2 k_fsbc forms the sum beside the products, which read the same fields)
     T* zsum = lvl[0];
-    hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, kSplit ? 2 : 1, zz, zsum);
+    const int j0 = kSplit ? 2 : 1;
+    if (M < 2)
+      hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, j0, zz, zsum);
     // --- join the wavemaker chain (solveHOS_SWEET_1 + phiadd_SL) ---
     HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     // --- free-surface products, filtered to the base modes; P, the products and
@@ -2185,7 +2192,7 @@ struct Engine : EngineBase {
           HIP_OK(hipMemsetAsync(phisy, 0, nb * sizeof(T), st));
         }
         hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
-                           phixa, phiya, phiza, X, Y, XA, YA, XB, YB);
+                           phixa, phiya, phiza, X, Y, XA, YA, XB, YB, j0, zsum);
         b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[1]);
         b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[2]);
         if (C.wmk) mixed_parity_spectra(b);
