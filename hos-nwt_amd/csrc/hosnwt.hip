@@ -2099,18 +2099,11 @@ struct Engine : EngineBase {
                        C.volmode(), C.last_alpha, C.last_pos_top, C.p.xlen, C.p.d_hinge);
     // --- physical eta on the base grid ---
     base(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), n1, n2, Dst<T>{eta_phys, n1, 0});
-    // --- wavemaker chain on the side stream: it needs only the base-grid eta ---
+    // --- the wavemaker chain runs on the side stream from here: it needs only the
+    // base-grid eta (and the stage inputs); it is enqueued after the main stream's
+    // work up to the join, so that the main stream does not wait for the host
+    // while the host enqueues the chain ---
     HIP_OK(hipEventRecord(ev_fork, st));
-    HIP_OK(hipStreamWaitEvent(sw, ev_fork, 0));
-    {
-      struct Side {
-        Engine* e;
-        ~Side() { e->use_main(); }
-      } side{this};
-      use_side();
-      wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, s2);
-    }
-    HIP_OK(hipEventRecord(ev_join, sw));
     // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
@@ -2179,7 +2172,17 @@ struct Engine : EngineBase {
     const int j0 = kSplit ? 2 : 1;
     if (M < 2)
       hipLaunchKernelGGL(k_sum_orders<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, j0, zz, zsum);
-    // --- join the wavemaker chain (solveHOS_SWEET_1 + phiadd_SL) ---
+    // --- the wavemaker chain (solveHOS_SWEET_1 + phiadd_SL), then the join ---
+    HIP_OK(hipStreamWaitEvent(sw, ev_fork, 0));
+    {
+      struct Side {
+        Engine* e;
+        ~Side() { e->use_main(); }
+      } side{this};
+      use_side();
+      wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, s2);
+    }
+    HIP_OK(hipEventRecord(ev_join, sw));
     HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     // --- free-surface products, filtered to the base modes; P, the products and
     // the mixed-parity spectra in one batch ---
