@@ -1091,6 +1091,25 @@ __global__ void k_axpyk(size_t n, double* out, const double* base, double h, KSe
   for (int s = 0; s < ks.nk; ++s) acc = acc + ks.k[s][i] * ks.c[s];
   out[i] = base[i] + h * acc;
 }
+// The stage combinations of every state field in one launch (blockIdx.y = field),
+// each with k_axpyk's arithmetic.
+constexpr int kMaxAxpy = 10;
+struct AxpyBatch {
+  int n;
+  double* out[kMaxAxpy];
+  const double* base[kMaxAxpy];
+  size_t len[kMaxAxpy];
+  KSet ks[kMaxAxpy];
+};
+__global__ void k_axpyk_batch(double h, AxpyBatch b) {
+  const int f = blockIdx.y;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= b.len[f]) return;
+  const KSet& ks = b.ks[f];
+  double acc = 0.0;
+  for (int s = 0; s < ks.nk; ++s) acc = acc + ks.k[s][i] * ks.c[s];
+  b.out[f][i] = b.base[f][i] + h * acc;
+}
 // X_new = X + h sum b k ; err = |(sum e k - sum b k) h| ; block maxima of err and |X_new|
 __global__ void k_final_err(size_t n, double* out, const double* base, double h, KSet kb, KSet ke,
                             double* pmax_err, double* pmax_x) {
@@ -2714,14 +2733,18 @@ void rk_enqueue(hos_ctx* c, double t, double h) {
   rhs_eval(c, t, h * c->C[0], c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add,
            c->a3rd_add);
   for (int jl = 1; jl < 5; ++jl) {
+    AxpyBatch ab{};
     auto comb = [&](double* out, const double* base, double* const* karr, size_t n) {
-      KSet k{};
+      KSet& k = ab.ks[ab.n];
       k.nk = jl;
       for (int s = 0; s < jl; ++s) {
         k.k[s] = karr[s];
         k.c[s] = c->A[jl][s];
       }
-      hipLaunchKernelGGL(k_axpyk, dim3(nblk(n)), dim3(256), 0, st, n, out, base, h, k);
+      ab.out[ab.n] = out;
+      ab.base[ab.n] = base;
+      ab.len[ab.n] = n;
+      ++ab.n;
     };
     comb(c->G1m, c->G1, c->kG1, ns);
     comb(c->G2m, c->G2, c->kG2, ns);
@@ -2734,6 +2757,7 @@ void rk_enqueue(hos_ctx* c, double t, double h) {
       comb(c->aad2m, c->a2nd_add, c->kad2, na);
     }
     if (c->wmk3) comb(c->aad3m, c->a3rd_add, c->kad3, na);
+    hipLaunchKernelGGL(k_axpyk_batch, dim3(nblk(std::max(ns, na)), ab.n), dim3(256), 0, st, h, ab);
     rhs_eval(c, stage_time(t, h, c->C[jl]), h * c->C[jl], c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, jl, c->a2m,
              c->e2m, c->aad2m, c->aad3m);
   }
