@@ -1137,6 +1137,56 @@ __global__ void k_reduce_max(int n, const double* p, double* out) {
   m = block_max(m);
   if (threadIdx.x == 0) *out = m;
 }
+// k_final_err over every error-carrying field in one launch (blockIdx.y = field f);
+// field f's block maxima go to pmax_err/pmax_x + f * stride.
+constexpr int kMaxFinal = 5;
+struct FinalBatch {
+  int n;
+  size_t stride;
+  double* out[kMaxFinal];
+  const double* base[kMaxFinal];
+  size_t len[kMaxFinal];
+  KSet kb[kMaxFinal], ke[kMaxFinal];
+};
+__global__ void k_final_err_batch(double h, FinalBatch b, double* pmax_err, double* pmax_x) {
+  const int f = blockIdx.y;
+  const size_t n = b.len[f];
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  // blocks past this field's length write nothing: every thread of a block takes the same branch
+  if (blockIdx.x * (size_t)blockDim.x >= n) return;
+  double me = 0.0, mx = 0.0;
+  if (i < n) {
+    const KSet& kb = b.kb[f];
+    const KSet& ke = b.ke[f];
+    double db = 0.0, de = 0.0;
+    for (int s = 0; s < kb.nk; ++s) db = db + kb.k[s][i] * kb.c[s];
+    for (int s = 0; s < ke.nk; ++s) de = de + ke.k[s][i] * ke.c[s];
+    const double xn = b.base[f][i] + h * db;
+    b.out[f][i] = xn;
+    me = fabs((de - db) * h);
+    mx = fabs(xn);
+  }
+  me = block_max(me);
+  mx = block_max(mx);
+  if (threadIdx.x == 0) {
+    pmax_err[f * b.stride + blockIdx.x] = me;
+    pmax_x[f * b.stride + blockIdx.x] = mx;
+  }
+}
+// Block r reduces field r / 2's maxima, of the error for even r and of |X_new| for odd r,
+// into out[r]; nblocks[f] block maxima per field.
+struct FinalCounts {
+  int nb[kMaxFinal];
+};
+__global__ void k_reduce_max_batch(size_t stride, FinalCounts c, const double* pmax_err, const double* pmax_x,
+                                   double* out) {
+  const int r = blockIdx.x, f = r >> 1;
+  const double* p = ((r & 1) ? pmax_x : pmax_err) + f * stride;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < c.nb[f]; i += blockDim.x) m = fmax(m, p[i]);
+  m = block_max(m);
+  if (threadIdx.x == 0) out[r] = m;
+}
 
 // End of runge4_adapt2 (runge_kutta.f90:281-299): GtoF, /omega, phis(1,1)
 // restored, deta = GtoF(k_s) + phis omega^2; partial sums for the volume fix.
@@ -1738,6 +1788,7 @@ struct hos_ctx {
   bool wmk3 = false;
   double *a3rd_add = nullptr, *a3rd_add_t = nullptr, *aad3m = nullptr, *kad3[5] = {};
   double *partial, *pmax_e, *pmax_x, *red6;
+  size_t pmax_stride;
   double* red_host = nullptr;  // pinned copy of red6 (the attempt's error maxima)
   ~hos_ctx() {
     if (red_host) (void)hipHostFree(red_host);
@@ -2774,42 +2825,40 @@ void rk_enqueue(hos_ctx* c, double t, double h) {
   const unsigned nbs = nblk(ns), nba = nblk(na);
   double* Gn1 = c->G1m;  // reuse stage buffers for the new modal G
   double* Gn2 = c->G2m;
-  hipLaunchKernelGGL(k_final_err, dim3(nbs), dim3(256), 0, st, ns, Gn1, (const double*)c->G1, h, kset(c->kG1, c->B),
-                     kset(c->kG1, c->E), c->pmax_e, c->pmax_x);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_e, c->red6 + 0);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_x, c->red6 + 1);
-  hipLaunchKernelGGL(k_final_err, dim3(nbs), dim3(256), 0, st, ns, Gn2, (const double*)c->G2, h, kset(c->kG2, c->B),
-                     kset(c->kG2, c->E), c->pmax_e, c->pmax_x);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_e, c->red6 + 2);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->pmax_x, c->red6 + 3);
-  hipLaunchKernelGGL(k_final_err, dim3(nba), dim3(256), 0, st, na, c->a1st_add_t, (const double*)c->a1st_add, h,
-                     kset(c->kad1, c->B), kset(c->kad1, c->E), c->pmax_e, c->pmax_x);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_e, c->red6 + 4);
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_x, c->red6 + 5);
-  hipLaunchKernelGGL(k_axpyk, dim3(nba), dim3(256), 0, st, na, c->a_add_ramp_t, (const double*)c->a_add_ramp, h,
-                     kset(c->kadr, c->B));
-  hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->a1st_t, (const double*)c->a1st, h,
-                     kset(c->ka1, c->B));
-  hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->eta1st_t, (const double*)c->eta1st, h,
-                     kset(c->ke1, c->B));
+  // runge_kutta.f90:243-279: the new state, the embedded-error maxima and the
+  // wavemaker mode combinations, three launches in all
+  FinalBatch fb{};
+  FinalCounts fc{};
+  fb.stride = c->pmax_stride;
+  auto fin = [&](double* out, const double* base, size_t len, double* const (&k)[5]) {
+    const int f = fb.n++;
+    fb.out[f] = out; fb.base[f] = base; fb.len[f] = len;
+    fb.kb[f] = kset(k, c->B); fb.ke[f] = kset(k, c->E);
+    fc.nb[f] = (int)nblk(len);
+  };
+  fin(Gn1, c->G1, ns, c->kG1);
+  fin(Gn2, c->G2, ns, c->kG2);
+  fin(c->a1st_add_t, c->a1st_add, na, c->kad1);
+  // second- and third-order wavemaker modes and their error terms (runge_kutta.f90:249-256, :275-276)
+  if (c->wmk2) fin(c->a2nd_add_t, c->a2nd_add, na, c->kad2);
+  if (c->wmk3) fin(c->a3rd_add_t, c->a3rd_add, na, c->kad3);
+  hipLaunchKernelGGL(k_final_err_batch, dim3(std::max(nbs, nba), fb.n), dim3(256), 0, st, h, fb, c->pmax_e,
+                     c->pmax_x);
+  hipLaunchKernelGGL(k_reduce_max_batch, dim3(2 * fb.n), dim3(256), 0, st, c->pmax_stride, fc,
+                     (const double*)c->pmax_e, (const double*)c->pmax_x, c->red6);
+  AxpyBatch ab{};
+  auto ax = [&](double* out, const double* base, size_t len, double* const (&k)[5]) {
+    const int f = ab.n++;
+    ab.out[f] = out; ab.base[f] = base; ab.len[f] = len; ab.ks[f] = kset(k, c->B);
+  };
+  ax(c->a_add_ramp_t, c->a_add_ramp, na, c->kadr);
+  ax(c->a1st_t, c->a1st, ns, c->ka1);
+  ax(c->eta1st_t, c->eta1st, ns, c->ke1);
   if (c->wmk2) {
-    // second-order wavemaker modes and their error term (runge_kutta.f90:249-256, :275)
-    hipLaunchKernelGGL(k_final_err, dim3(nba), dim3(256), 0, st, na, c->a2nd_add_t, (const double*)c->a2nd_add, h,
-                       kset(c->kad2, c->B), kset(c->kad2, c->E), c->pmax_e, c->pmax_x);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_e, c->red6 + 6);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_x, c->red6 + 7);
-    hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->a2nd_t, (const double*)c->a2nd, h,
-                       kset(c->ka2, c->B));
-    hipLaunchKernelGGL(k_axpyk, dim3(nbs), dim3(256), 0, st, ns, c->eta2nd_t, (const double*)c->eta2nd, h,
-                       kset(c->ke2, c->B));
+    ax(c->a2nd_t, c->a2nd, ns, c->ka2);
+    ax(c->eta2nd_t, c->eta2nd, ns, c->ke2);
   }
-  if (c->wmk3) {
-    // third-order paddle modes and their error term (runge_kutta.f90:276)
-    hipLaunchKernelGGL(k_final_err, dim3(nba), dim3(256), 0, st, na, c->a3rd_add_t, (const double*)c->a3rd_add, h,
-                       kset(c->kad3, c->B), kset(c->kad3, c->E), c->pmax_e, c->pmax_x);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_e, c->red6 + 8);
-    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, st, (int)nba, (const double*)c->pmax_x, c->red6 + 9);
-  }
+  hipLaunchKernelGGL(k_axpyk_batch, dim3(std::max(nbs, nba), ab.n), dim3(256), 0, st, h, ab);
   // GtoF back, deta, volume fix with the last stage's wavemaker globals
   hipLaunchKernelGGL(k_step_end, dim3(nbs), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy, h, (const double*)Gn1,
                      (const double*)Gn2, (const double*)c->kG1[4], (const double*)c->kG2[4],
@@ -3016,8 +3065,9 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     }
     const size_t nbk = nblk(std::max(ns, na)) + 1;
     c->partial = ar.get<double>(nbk);
-    c->pmax_e = ar.get<double>(nbk);
-    c->pmax_x = ar.get<double>(nbk);
+    c->pmax_stride = nbk;
+    c->pmax_e = ar.get<double>(kMaxFinal * nbk);
+    c->pmax_x = ar.get<double>(kMaxFinal * nbk);
     c->red6 = ar.get<double>(10);
     HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&c->red_host), 16 * sizeof(double), hipHostMallocDefault));
     c->phys_tmp = ar.get<double>(ns);
