@@ -458,7 +458,7 @@ __device__ __forceinline__ void csh_sh(double k, double x, double xlen, double& 
 // :485-532): with B_f(i1,i3,i2) = sum_j2 A_f(i1,j2,i3) cos|sin(ky(j2) y(i2)),
 //   phi  = sum_i3 cos(kx_add (eta+1)) B_phi,  phiz = sum_i3 sin(kx_add (eta+1)) B_z
 // and at z = 0 the same sums with cos(kx_add), sin(kx_add).  Workgroup
-// (i1, i3-chunk), threads over y-nodes; A_f of a y-mode chunk is staged in LDS;
+// (i1, i3-chunk), threads over y-nodes (k_phiadd3d: y-node pairs); A_f of a y-mode chunk is staged in LDS;
 // cos(ky y) advances by rotation.  Modes with k_add x > kCut contribute below
 // 1e-19 and are skipped.  Partial sums per chunk go to `part`
 // [chunk][field][i1 + n1 i2]; k_phiadd_reduce adds the chunks in order.
@@ -468,6 +468,12 @@ constexpr int kPhiI3 = 8;   // wavemaker modes per workgroup
 constexpr int kPhiF = 7;    // phi, phit, phix, phiy, phiz, phit(z=0), phiz(z=0)
 __device__ __forceinline__ double kx_add_of(int i3) { return (2 * i3 + 1) * M_PI / (2.0 * L_ADD); }
 
+// Node pairs: cos(pi j (n2-1-i2)/(n2-1)) = (-1)^j cos(pi j i2/(n2-1)) and
+// sin(pi j (n2-1-i2)/(n2-1)) = -(-1)^j sin(pi j i2/(n2-1)), so one rotation chain
+// over j, summed separately over even and odd j, gives y-node i2 (even + odd) and
+// its mirror n2-1-i2 (even - odd; odd - even for the sine sum): half the work of
+// one chain per node.  Each thread takes kPhi3Per pairs.
+constexpr int kPhi3Per = 2;
 template <typename T>
 __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, double xlen, double piy, double dx,
                                                   const double* __restrict__ a, const double* __restrict__ da,
@@ -478,14 +484,16 @@ __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, doubl
   const int i3a = ch * kPhiI3, i3b = min(n3a, i3a + kPhiI3);
   if (x * kx_add_of(i3a) > kCut) return;  // the reduction skips this chunk too
   const size_t ns = (size_t)n1 * n2;
-  double out[kPhiF][kPhiMaxPer];
-  double ep1[kPhiMaxPer];
+  const int np = (n2 + 1) / 2;  // pairs (i2, n2-1-i2); the middle node of an odd n2 is its own mirror
+  double out[kPhiF][2][kPhi3Per];
+  double ep1[2][kPhi3Per];
 #pragma unroll
-  for (int q = 0; q < kPhiMaxPer; ++q) {
-    const int i2 = tid + q * nt;
-    ep1[q] = (i2 < n2) ? (double)eta[i1 + (size_t)n1 * i2] + 1.0 : 1.0;
+  for (int q = 0; q < kPhi3Per; ++q) {
+    const int pr = tid + q * nt;
+    ep1[0][q] = (pr < np) ? (double)eta[i1 + (size_t)n1 * pr] + 1.0 : 1.0;
+    ep1[1][q] = (pr < np) ? (double)eta[i1 + (size_t)n1 * (n2 - 1 - pr)] + 1.0 : 1.0;
 #pragma unroll
-    for (int f = 0; f < kPhiF; ++f) out[f][q] = 0.0;
+    for (int f = 0; f < kPhiF; ++f) out[f][0][q] = out[f][1][q] = 0.0;
   }
   const double th = 1.0 / (double)(n2 - 1);
   for (int i3 = i3a; i3 < i3b; ++i3) {
@@ -496,12 +504,12 @@ __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, doubl
       const double r = kCut / x;
       jmax = min(n2, (int)floor(sqrt(r * r - kxa * kxa) / piy) + 1);
     }
-    double B[5][kPhiMaxPer];
+    double Be[5][kPhi3Per], Bo[5][kPhi3Per];
 #pragma unroll
     for (int f = 0; f < 5; ++f)
 #pragma unroll
-      for (int q = 0; q < kPhiMaxPer; ++q) B[f][q] = 0.0;
-    for (int j0 = 0; j0 < jmax; j0 += kPhiCh) {
+      for (int q = 0; q < kPhi3Per; ++q) Be[f][q] = Bo[f][q] = 0.0;
+    for (int j0 = 0; j0 < jmax; j0 += kPhiCh) {  // j0 is even: the parity of j2 is that of jj
       const int jn = min(kPhiCh, jmax - j0);
       for (int jj = tid; jj < jn; jj += nt) {
         const int j2 = j0 + jj;
@@ -518,50 +526,80 @@ __global__ void __launch_bounds__(256) k_phiadd3d(int n1, int n2, int n3a, doubl
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kPhiMaxPer; ++q) {
+      for (int q = 0; q < kPhi3Per; ++q) {
         const int i2 = tid + q * nt;
-        if (i2 >= n2) continue;
+        if (i2 >= np) continue;
         // (c, s) = (cos, sin)(pi j2 i2 / (n2-1)), advanced by the rotation (cr, sr)
         const double ang = (double)(((long)j0 * i2) % (2 * (n2 - 1))) * th;
         double c = cospi(ang), sn = sinpi(ang);
         const double cr = cospi(i2 * th), sr = sinpi(i2 * th);
-        double b0 = B[0][q], b1 = B[1][q], b2 = B[2][q], b3 = B[3][q], b4 = B[4][q];
-        for (int jj = 0; jj < jn; ++jj) {
-          b0 += A[0][jj] * c;
-          b1 += A[1][jj] * c;
-          b2 += A[2][jj] * c;
-          b3 += A[3][jj] * sn;
-          b4 += A[4][jj] * c;
-          const double cn = c * cr - sn * sr;
+        double e0 = Be[0][q], e1 = Be[1][q], e2 = Be[2][q], e3 = Be[3][q], e4 = Be[4][q];
+        double o0 = Bo[0][q], o1 = Bo[1][q], o2 = Bo[2][q], o3 = Bo[3][q], o4 = Bo[4][q];
+        int jj = 0;
+        for (; jj + 1 < jn; jj += 2) {
+          e0 += A[0][jj] * c;
+          e1 += A[1][jj] * c;
+          e2 += A[2][jj] * c;
+          e3 += A[3][jj] * sn;
+          e4 += A[4][jj] * c;
+          double cn = c * cr - sn * sr;
+          sn = sn * cr + c * sr;
+          c = cn;
+          o0 += A[0][jj + 1] * c;
+          o1 += A[1][jj + 1] * c;
+          o2 += A[2][jj + 1] * c;
+          o3 += A[3][jj + 1] * sn;
+          o4 += A[4][jj + 1] * c;
+          cn = c * cr - sn * sr;
           sn = sn * cr + c * sr;
           c = cn;
         }
-        B[0][q] = b0; B[1][q] = b1; B[2][q] = b2; B[3][q] = b3; B[4][q] = b4;
+        if (jj < jn) {
+          e0 += A[0][jj] * c;
+          e1 += A[1][jj] * c;
+          e2 += A[2][jj] * c;
+          e3 += A[3][jj] * sn;
+          e4 += A[4][jj] * c;
+        }
+        Be[0][q] = e0; Be[1][q] = e1; Be[2][q] = e2; Be[3][q] = e3; Be[4][q] = e4;
+        Bo[0][q] = o0; Bo[1][q] = o1; Bo[2][q] = o2; Bo[3][q] = o3; Bo[4][q] = o4;
       }
       __syncthreads();
     }
     double s0, c0;
     sincos(kxa, &s0, &c0);
 #pragma unroll
-    for (int q = 0; q < kPhiMaxPer; ++q) {
-      double sn, cs;
-      sincos(kxa * ep1[q], &sn, &cs);
-      out[0][q] += cs * B[0][q];
-      out[1][q] += cs * B[1][q];
-      out[2][q] += cs * B[2][q];
-      out[3][q] += cs * B[3][q];
-      out[4][q] += sn * B[4][q];
-      out[5][q] += c0 * B[1][q];
-      out[6][q] += s0 * B[4][q];
+    for (int q = 0; q < kPhi3Per; ++q) {
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const double sg = side ? -1.0 : 1.0;
+        double B[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) B[f] = Be[f][q] + sg * Bo[f][q];
+        if (side) B[3] = Bo[3][q] - Be[3][q];
+        double sn, cs;
+        sincos(kxa * ep1[side][q], &sn, &cs);
+        out[0][side][q] += cs * B[0];
+        out[1][side][q] += cs * B[1];
+        out[2][side][q] += cs * B[2];
+        out[3][side][q] += cs * B[3];
+        out[4][side][q] += sn * B[4];
+        out[5][side][q] += c0 * B[1];
+        out[6][side][q] += s0 * B[4];
+      }
     }
   }
 #pragma unroll
-  for (int q = 0; q < kPhiMaxPer; ++q) {
+  for (int q = 0; q < kPhi3Per; ++q) {
     const int i2 = tid + q * nt;
-    if (i2 >= n2) continue;
-    const size_t o = i1 + (size_t)n1 * i2;
+    if (i2 >= np) continue;
+    const int m = n2 - 1 - i2;
+    const size_t o = i1 + (size_t)n1 * i2, om = i1 + (size_t)n1 * m;
 #pragma unroll
-    for (int f = 0; f < kPhiF; ++f) part[((size_t)ch * kPhiF + f) * ns + o] = out[f][q];
+    for (int f = 0; f < kPhiF; ++f) {
+      part[((size_t)ch * kPhiF + f) * ns + o] = out[f][0][q];
+      if (m != i2) part[((size_t)ch * kPhiF + f) * ns + om] = out[f][1][q];
+    }
   }
 }
 
@@ -1084,15 +1122,7 @@ struct KSet {
   double c[5];
   int nk;
 };
-__global__ void k_axpyk(size_t n, double* out, const double* base, double h, KSet ks) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double acc = 0.0;
-  for (int s = 0; s < ks.nk; ++s) acc = acc + ks.k[s][i] * ks.c[s];
-  out[i] = base[i] + h * acc;
-}
-// The stage combinations of every state field in one launch (blockIdx.y = field),
-// each with k_axpyk's arithmetic.
+// out = base + h sum_s c_s k_s for every state field in one launch (blockIdx.y = field)
 constexpr int kMaxAxpy = 10;
 struct AxpyBatch {
   int n;
@@ -1101,6 +1131,8 @@ struct AxpyBatch {
   size_t len[kMaxAxpy];
   KSet ks[kMaxAxpy];
 };
+// (at most 9 fields: G1, G2, a1st_add, a1st, eta1st, the three second-order fields, a3rd_add)
+static_assert(kMaxAxpy >= 9, "AxpyBatch holds every RK state field");
 __global__ void k_axpyk_batch(double h, AxpyBatch b) {
   const int f = blockIdx.y;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1110,34 +1142,14 @@ __global__ void k_axpyk_batch(double h, AxpyBatch b) {
   for (int s = 0; s < ks.nk; ++s) acc = acc + ks.k[s][i] * ks.c[s];
   b.out[f][i] = b.base[f][i] + h * acc;
 }
-// X_new = X + h sum b k ; err = |(sum e k - sum b k) h| ; block maxima of err and |X_new|
-__global__ void k_final_err(size_t n, double* out, const double* base, double h, KSet kb, KSet ke,
-                            double* pmax_err, double* pmax_x) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  double me = 0.0, mx = 0.0;
-  if (i < n) {
-    double db = 0.0, de = 0.0;
-    for (int s = 0; s < kb.nk; ++s) db = db + kb.k[s][i] * kb.c[s];
-    for (int s = 0; s < ke.nk; ++s) de = de + ke.k[s][i] * ke.c[s];
-    const double xn = base[i] + h * db;
-    out[i] = xn;
-    me = fabs((de - db) * h);
-    mx = fabs(xn);
-  }
-  me = block_max(me);
-  mx = block_max(mx);
-  if (threadIdx.x == 0) {
-    pmax_err[blockIdx.x] = me;
-    pmax_x[blockIdx.x] = mx;
-  }
-}
 __global__ void k_reduce_max(int n, const double* p, double* out) {
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, p[i]);
   m = block_max(m);
   if (threadIdx.x == 0) *out = m;
 }
-// k_final_err over every error-carrying field in one launch (blockIdx.y = field f);
+// X_new = X + h sum b k ; err = |(sum e k - sum b k) h| ; block maxima of err and
+// |X_new|, for every error-carrying field in one launch (blockIdx.y = field f);
 // field f's block maxima go to pmax_err/pmax_x + f * stride.
 constexpr int kMaxFinal = 5;
 struct FinalBatch {
@@ -1148,6 +1160,8 @@ struct FinalBatch {
   size_t len[kMaxFinal];
   KSet kb[kMaxFinal], ke[kMaxFinal];
 };
+// (G1, G2 and the first-, second- and third-order paddle modes)
+static_assert(kMaxFinal >= 5, "FinalBatch holds every error-carrying field");
 __global__ void k_final_err_batch(double h, FinalBatch b, double* pmax_err, double* pmax_x) {
   const int f = blockIdx.y;
   const size_t n = b.len[f];
@@ -2655,8 +2669,11 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL((k_phiadd2d<T>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx, a, da, eta,
                          phi_add, phit_add, phix_add, phiy_add, phiz_add, phit1, phiz1);
     } else {
+      // one thread per y-node pair (kPhi3Per pairs per thread beyond 256 threads)
+      const int np = (n2 + 1) / 2;
       int nt = 64;
-      while (nt < 256 && nt < n2) nt *= 2;
+      while (nt < 256 && nt < np) nt *= 2;
+      if (nt * kPhi3Per < np) throw std::runtime_error("n2 too large for the wavemaker kernel");
       hipLaunchKernelGGL((k_phiadd3d<T>), dim3(n1, nch), dim3(nt), 0, st, n1, n2, n3a, C.p.xlen, C.piy, C.dx, a, da,
                          eta, part);
       hipLaunchKernelGGL(k_phiadd_reduce, dim3(nblk(ns)), dim3(256), 0, st, n1, n2, nch, C.dx, (const double*)part,
