@@ -685,22 +685,27 @@ struct Z0Set<1> {
   static constexpr int NB = 2, NF = 2;
 };
 
+// The y sums run over mirror-node pairs as in k_phiadd3d (even + odd for node i2,
+// even - odd for n2-1-i2; odd - even for the sine sums).
 template <int SET>
 __global__ void __launch_bounds__(256) k_phiadd_z0(int n1, int n2, int n3a, double xlen, double piy, double dx,
                                                    const double* __restrict__ a, const double* __restrict__ da,
                                                    double* part) {
   constexpr int NB = Z0Set<SET>::NB, NF = Z0Set<SET>::NF;
+  // the S sums taken with sin(ky y) (the y derivatives)
+  constexpr bool kSine[7] = {false, false, false, true, false, true, false};
   __shared__ double A[NB][kPhiCh];
   const int i1 = blockIdx.x, ch = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
   const double x = i1 * dx;
   const int i3a = ch * kPhiI3, i3b = min(n3a, i3a + kPhiI3);
   if (x * kx_add_of(i3a) > kCut) return;  // the reduction skips this chunk too
   const size_t ns = (size_t)n1 * n2;
-  double out[NF][kPhiMaxPer];
+  const int np = (n2 + 1) / 2;  // pairs (i2, n2-1-i2); the middle node of an odd n2 is its own mirror
+  double out[NF][2][kPhi3Per];
 #pragma unroll
-  for (int q = 0; q < kPhiMaxPer; ++q)
+  for (int q = 0; q < kPhi3Per; ++q)
 #pragma unroll
-    for (int f = 0; f < NF; ++f) out[f][q] = 0.0;
+    for (int f = 0; f < NF; ++f) out[f][0][q] = out[f][1][q] = 0.0;
   const double th = (n2 > 1) ? 1.0 / (double)(n2 - 1) : 0.0;
   for (int i3 = i3a; i3 < i3b; ++i3) {
     const double kxa = kx_add_of(i3);
@@ -710,12 +715,12 @@ __global__ void __launch_bounds__(256) k_phiadd_z0(int n1, int n2, int n3a, doub
       const double r = kCut / x;
       jmax = min(n2, (int)floor(sqrt(r * r - kxa * kxa) / piy) + 1);
     }
-    double S[NB][kPhiMaxPer];
+    double Se[NB][kPhi3Per], So[NB][kPhi3Per];
 #pragma unroll
     for (int b = 0; b < NB; ++b)
 #pragma unroll
-      for (int q = 0; q < kPhiMaxPer; ++q) S[b][q] = 0.0;
-    for (int j0 = 0; j0 < jmax; j0 += kPhiCh) {
+      for (int q = 0; q < kPhi3Per; ++q) Se[b][q] = So[b][q] = 0.0;
+    for (int j0 = 0; j0 < jmax; j0 += kPhiCh) {  // j0 is even: the parity of j2 is that of jj
       const int jn = min(kPhiCh, jmax - j0);
       for (int jj = tid; jj < jn; jj += nt) {
         const int j2 = j0 + jj;
@@ -736,60 +741,80 @@ __global__ void __launch_bounds__(256) k_phiadd_z0(int n1, int n2, int n3a, doub
       }
       __syncthreads();
 #pragma unroll
-      for (int q = 0; q < kPhiMaxPer; ++q) {
+      for (int q = 0; q < kPhi3Per; ++q) {
         const int i2 = tid + q * nt;
-        if (i2 >= n2) continue;
+        if (i2 >= np) continue;
         const double ang = (n2 > 1) ? (double)(((long)j0 * i2) % (2 * (n2 - 1))) * th : 0.0;
         double c = cospi(ang), sn = sinpi(ang);
         const double cr = cospi(i2 * th), sr = sinpi(i2 * th);
-        double acc[NB];
+        double e[NB], o[NB];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) acc[b] = S[b][q];
-        for (int jj = 0; jj < jn; ++jj) {
-          acc[0] += A[0][jj] * c;
-          acc[1] += A[1][jj] * c;
-          if constexpr (SET == 0) {
-            acc[2] += A[2][jj] * c;
-            acc[3] += A[3][jj] * sn;
-            acc[4] += A[4][jj] * c;
-            acc[5] += A[5][jj] * sn;
-            acc[6] += A[6][jj] * c;
-          }
-          const double cn = c * cr - sn * sr;
+        for (int b = 0; b < NB; ++b) {
+          e[b] = Se[b][q];
+          o[b] = So[b][q];
+        }
+        int jj = 0;
+        for (; jj + 1 < jn; jj += 2) {
+#pragma unroll
+          for (int b = 0; b < NB; ++b) e[b] += A[b][jj] * (kSine[b] ? sn : c);
+          double cn = c * cr - sn * sr;
+          sn = sn * cr + c * sr;
+          c = cn;
+#pragma unroll
+          for (int b = 0; b < NB; ++b) o[b] += A[b][jj + 1] * (kSine[b] ? sn : c);
+          cn = c * cr - sn * sr;
           sn = sn * cr + c * sr;
           c = cn;
         }
+        if (jj < jn) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) S[b][q] = acc[b];
+          for (int b = 0; b < NB; ++b) e[b] += A[b][jj] * (kSine[b] ? sn : c);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          Se[b][q] = e[b];
+          So[b][q] = o[b];
+        }
       }
       __syncthreads();
     }
     double s0, c0;
     sincos(kxa, &s0, &c0);
 #pragma unroll
-    for (int q = 0; q < kPhiMaxPer; ++q) {
-      out[0][q] += c0 * S[1][q];
-      out[1][q] += -s0 * kxa * S[0][q];
-      if constexpr (SET == 0) {
-        out[2][q] += -c0 * S[2][q];
-        out[3][q] += -c0 * S[3][q];
-        out[4][q] += -c0 * kxa * kxa * S[0][q];
-        out[5][q] += -c0 * S[4][q];
-        out[6][q] += -c0 * S[5][q];
-        out[7][q] += -s0 * kxa * S[1][q];
-        out[8][q] += c0 * S[6][q];
-        out[9][q] += s0 * kxa * S[2][q];
-        out[10][q] += s0 * S[3][q];
+    for (int q = 0; q < kPhi3Per; ++q) {
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        double S[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          S[b] = side == 0 ? Se[b][q] + So[b][q] : (kSine[b] ? So[b][q] - Se[b][q] : Se[b][q] - So[b][q]);
+        out[0][side][q] += c0 * S[1];
+        out[1][side][q] += -s0 * kxa * S[0];
+        if constexpr (SET == 0) {
+          out[2][side][q] += -c0 * S[2];
+          out[3][side][q] += -c0 * S[3];
+          out[4][side][q] += -c0 * kxa * kxa * S[0];
+          out[5][side][q] += -c0 * S[4];
+          out[6][side][q] += -c0 * S[5];
+          out[7][side][q] += -s0 * kxa * S[1];
+          out[8][side][q] += c0 * S[6];
+          out[9][side][q] += s0 * kxa * S[2];
+          out[10][side][q] += s0 * S[3];
+        }
       }
     }
   }
 #pragma unroll
-  for (int q = 0; q < kPhiMaxPer; ++q) {
+  for (int q = 0; q < kPhi3Per; ++q) {
     const int i2 = tid + q * nt;
-    if (i2 >= n2) continue;
-    const size_t o = i1 + (size_t)n1 * i2;
+    if (i2 >= np) continue;
+    const int m = n2 - 1 - i2;
+    const size_t o = i1 + (size_t)n1 * i2, om = i1 + (size_t)n1 * m;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) part[((size_t)ch * NF + f) * ns + o] = out[f][q];
+    for (int f = 0; f < NF; ++f) {
+      part[((size_t)ch * NF + f) * ns + o] = out[f][0][q];
+      if (m != i2) part[((size_t)ch * NF + f) * ns + om] = out[f][1][q];
+    }
   }
 }
 
@@ -2528,8 +2553,11 @@ struct Engine : EngineBase {
   template <int SET>
   void phiadd_z0(const double* a, const double* da, Ptr11 outp) {
     hos_ctx& C = *c;
+    // one thread per y-node pair (kPhi3Per pairs per thread beyond 256 threads)
+    const int np = (C.n2 + 1) / 2;
     int nt = 64;
-    while (nt < 256 && nt < C.n2) nt *= 2;
+    while (nt < 256 && nt < np) nt *= 2;
+    if (nt * kPhi3Per < np) throw std::runtime_error("n2 too large for the wavemaker kernel");
     hipLaunchKernelGGL((k_phiadd_z0<SET>), dim3(C.n1, nch), dim3(nt), 0, cs, C.n1, C.n2, C.n3a, C.p.xlen, C.piy,
                        C.dx, a, da, part2);
     hipLaunchKernelGGL(k_phiadd_reduce_n, dim3(nblk(ns)), dim3(256), 0, cs, C.n1, C.n2, nch, Z0Set<SET>::NF, C.dx,

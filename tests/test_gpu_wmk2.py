@@ -26,6 +26,8 @@ CASES = {
     "3d_directional_M5": dict(n1=24, n2=12, M=5, n3=9, xlen=8.0, ylen=4.0, Tramp=1.5),
     "3d_directional_M3_filtered": dict(n1=32, n2=16, M=3, n3=9, xlen=10.0, ylen=5.0, Tramp=1.5,
                                        coeffilt=(0.7, 0.8, 0.9)),
+    # odd n2: the wavemaker y sums pair node i2 with n2-1-i2; the middle node is its own mirror
+    "3d_directional_M3_odd_n2": dict(n1=24, n2=13, M=3, n3=9, xlen=8.0, ylen=4.0, Tramp=1.5),
 }
 
 
