@@ -153,17 +153,11 @@ struct PtrSet {
   T* p[10];
 };
 
+// |grad eta|^2 on the extended grid
 template <typename T>
-__global__ void k_powers(size_t n, int M, PtrSet<T> e, const T* __restrict__ ex, const T* __restrict__ ey,
-                         T* g2) {
+__global__ void k_grad2(size_t n, const T* __restrict__ ex, const T* __restrict__ ey, T* g2) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const T e1 = e.p[1][i];
-  T ej = e1;
-  for (int j = 1; j < M - 1; ++j) {  // eta^M only feeds the dead last-order phimext
-    ej = ej / (T)(j + 1) * e1;
-    e.p[j + 1][i] = ej;
-  }
   const T x = ex[i], y = ey[i];
   g2[i] = x * x + y * y;
 }
@@ -174,19 +168,34 @@ __global__ void k_powers(size_t n, int M, PtrSet<T> e, const T* __restrict__ ex,
 // (resol_HOS.f90:736-772).  The level's transforms then share one launch
 // (measured on 1024 x 256, M = 5: 92 -> 102 FP64 and 130 -> 139 mixed steps/s
 // against accumulating in each transform's x-pass epilogue, one launch each).
+// The powers eta^j/j! (j < M; eta^M only feeds the dead last-order phimext) are
+// formed in registers from eta as (eta^j/j!)/(j+1)*eta, ascending; the loops run
+// over a fixed bound so that every index is known at compile time.
+constexpr int kMaxOrder = 10;
 template <typename T>
-__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, PtrSet<T> ep, T* phim, T* phizm) {
+__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const bool do_phim = iH < M;
+  const int top = min(iH, M - 1);  // highest power read at this level
+  T pw[kMaxOrder];
+#pragma unroll
+  for (int j = 0; j < kMaxOrder; ++j) pw[j] = (T)0;
+  pw[0] = (T)1;
+  if (top >= 1) pw[1] = eta[i];
+#pragma unroll
+  for (int j = 1; j < kMaxOrder - 1; ++j)
+    if (j + 1 <= top) pw[j + 1] = pw[j] / (T)(j + 1) * pw[1];
   T pm = (T)0, pz = (T)0;
-  for (int q = 0; q < iH; ++q) {
-    const int j = iH - q;
-    const T v = lv.p[q][i];
-    const T e = do_phim ? ep.p[j][i] : (T)0;
-    const T e1 = (j >= 2) ? ep.p[j - 1][i] : (T)1;
-    pm = pm - v * e;
-    pz = pz + v * e1;
+#pragma unroll
+  for (int j = kMaxOrder - 1; j >= 1; --j) {  // j = iH..1: item iH - j of lv
+    if (j <= iH) {
+      const T v = lv.p[iH - j][i];
+      const T e = do_phim ? pw[j] : (T)0;
+      const T e1 = (j >= 2) ? pw[j - 1] : (T)1;
+      pm = pm - v * e;
+      pz = pz + v * e1;
+    }
   }
   if (do_phim) phim[i] = pm;
   phizm[i] = pz;
@@ -1868,7 +1877,8 @@ struct Engine : EngineBase {
   // base (T)
   T *eta_hat, *phis_hat, *eta_phys, *P, *D1, *D2, *Q, *Zp, *Zh, *phixh, *phiyh, *phizh, *tmpS;
   // big (T)
-  std::vector<T*> etapm, phizm, aHOS;
+  T* eta_ext = nullptr;  // eta on the extended grid (M >= 2; the HOS levels form its powers)
+  std::vector<T*> phizm, aHOS;
   std::vector<T*> lvl;  // outputs of one HOS level's transforms
   std::vector<T*> tmpX;  // x-pass intermediates of batched transforms
   T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
@@ -1965,11 +1975,10 @@ struct Engine : EngineBase {
       phis64 = ar.get<double>(ns); wlin = ar.get<double>(ns);
       Zf64 = ar.get<double>(ns); Zh64 = ar.get<double>(ns); phizh64 = ar.get<double>(ns);
     }
-    etapm.assign(M + 1, nullptr);
+    if (M >= 2) eta_ext = ar.get<T>(nb);
     phizm.assign(M + 1, nullptr);
     aHOS.assign(M + 1, nullptr);
     for (int j = 1; j <= M; ++j) {
-      if (j < M) etapm[j] = ar.get<T>(nb);
       phizm[j] = ar.get<T>(nb);
       if (j >= 2) aHOS[j] = ar.get<T>(nb);
     }
@@ -2216,7 +2225,7 @@ struct Engine : EngineBase {
     // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
-      b.add(KCOS, KCOS, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{etapm[1] ? etapm[1] : tmpB2(), ldb, 0},
+      b.add(KCOS, KCOS, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{eta_ext ? eta_ext : tmpB2(), ldb, 0},
             tmpX[0]);
       b.add(KSIN, KCOS, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0},
             tmpX[1]);
@@ -2233,13 +2242,7 @@ struct Engine : EngineBase {
             tmpX[b.n]);
       bigB(INV, b);
     }
-    PtrSet<T> ep{};
-    for (int j = 1; j < M; ++j) ep.p[j] = etapm[j];
-    if (M >= 2) {
-      hipLaunchKernelGGL(k_powers<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, ep, etax, etay, g2);
-    } else {
-      hipLaunchKernelGGL(k_powers<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, 2, ep_one(), etax, etay, g2);
-    }
+    hipLaunchKernelGGL(k_grad2<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, (const T*)etax, (const T*)etay, g2);
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
     for (int iH = 1; iH <= M; ++iH) {
       // the level's transforms in one launch (x passes first: 3 % faster in FP64
@@ -2264,7 +2267,7 @@ struct Engine : EngineBase {
         lv.p[0] = lvl[0];  // transformed with the slopes
       else
         bigB(INV, b);
-      hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, ep, phim, phizm[iH]);
+      hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, (const T*)eta_ext, phim, phizm[iH]);
       if (iH < M) {
         if (iH + 1 == M)
           big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
@@ -2362,11 +2365,6 @@ struct Engine : EngineBase {
                        (const T*)(n2 > 1 ? RB : nullptr), D1, D2);
   }
 
-  PtrSet<T> ep_one() {
-    PtrSet<T> e{};
-    e.p[1] = tmpB2();
-    return e;
-  }
   T* tmpB2() { return X; }
 
   void wavemaker(double t, const double* a1, const double* e1, const double* aad, double* ka1, double* ke1,
