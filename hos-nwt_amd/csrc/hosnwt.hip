@@ -2302,10 +2302,8 @@ struct Engine : EngineBase {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(zsum, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0}, tmpX[0]);
       if (M >= 2) {
-        if (n2 == 1) {
-          HIP_OK(hipMemsetAsync(etay, 0, nb * sizeof(T), st));
-          HIP_OK(hipMemsetAsync(phisy, 0, nb * sizeof(T), st));
-        }
+        // (in a 2-D tank etay, phisy and phiya hold zeros from allocation on: the
+        // arena clears every buffer and only the n2 > 1 transforms write them)
         hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
                            phixa, phiya, phiza, X, Y, XA, YA, XB, YB, j0, zsum);
         b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[1]);
@@ -2418,7 +2416,6 @@ struct Engine : EngineBase {
       if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, (*ctmpX)[1]);
       b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
       bigB(INV, b);
-      if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
     }
   }
 
@@ -2660,7 +2657,6 @@ struct Engine : EngineBase {
       if (n2 > 1) b.add(KCOS, KCOS, src_plain<T>(phiyh, n1, n1, n2), N1b, N2b, Dst<T>{phiya, ldb, 0}, (*ctmpX)[1]);
       b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
       bigB(INV, b);
-      if (n2 == 1) HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
     }
   }
 
