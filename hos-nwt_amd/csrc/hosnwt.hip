@@ -153,15 +153,6 @@ struct PtrSet {
   T* p[10];
 };
 
-// |grad eta|^2 on the extended grid
-template <typename T>
-__global__ void k_grad2(size_t n, const T* __restrict__ ex, const T* __restrict__ ey, T* g2) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const T x = ex[i], y = ey[i];
-  g2[i] = x * x + y * y;
-}
-
 // One level iH of the HOS recursion from its inverse transforms v_j (item q of
 // lv holds j = iH - q), accumulated in the reference's order j = iH..1:
 // phim = 0 - sum v_j eta^j/j!, phizm(iH) = 0 + sum v_j eta^(j-1)/(j-1)!
@@ -173,9 +164,14 @@ __global__ void k_grad2(size_t n, const T* __restrict__ ex, const T* __restrict_
 // over a fixed bound so that every index is known at compile time.
 constexpr int kMaxOrder = 10;
 template <typename T>
-__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm) {
+__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm,
+                            const T* __restrict__ ex, const T* __restrict__ ey, T* g2) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (g2) {  // level 1 also forms |grad eta|^2 from the slopes
+    const T x = ex[i], y = ey[i];
+    g2[i] = x * x + y * y;
+  }
   const bool do_phim = iH < M;
   const int top = min(iH, M - 1);  // highest power read at this level
   T pw[kMaxOrder];
@@ -280,9 +276,9 @@ __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, 
 }
 
 // D1 += RA_X + RB_X, D2 += RA_Y + RB_Y (exact reference-grid filter of the
-// mixed-parity products).
+// mixed-parity products), and Q = D1 + P.
 template <typename T>
-__global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* D2) {
+__global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* D2, const T* P, T* Q) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -292,8 +288,10 @@ __global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* 
     a += RB[i1 + (size_t)2 * n1 * i2];
     b += RB[n1 + i1 + (size_t)2 * n1 * i2];
   }
-  D1[i] += a;
+  const T d1 = D1[i] + a;
+  D1[i] = d1;
   D2[i] += b;
+  Q[i] = d1 + P[i];
 }
 
 template <typename T>
@@ -2242,7 +2240,6 @@ struct Engine : EngineBase {
             tmpX[b.n]);
       bigB(INV, b);
     }
-    hipLaunchKernelGGL(k_grad2<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, (const T*)etax, (const T*)etay, g2);
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
     for (int iH = 1; iH <= M; ++iH) {
       // the level's transforms in one launch (x passes first: 3 % faster in FP64
@@ -2267,7 +2264,8 @@ struct Engine : EngineBase {
         lv.p[0] = lvl[0];  // transformed with the slopes
       else
         bigB(INV, b);
-      hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, (const T*)eta_ext, phim, phizm[iH]);
+      hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, (const T*)eta_ext, phim,
+                         phizm[iH], (const T*)etax, (const T*)etay, iH == 1 ? g2 : (T*)nullptr);
       if (iH < M) {
         if (iH + 1 == M)
           big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
@@ -2313,8 +2311,10 @@ struct Engine : EngineBase {
       bigB(FWD, b);
     }
     if (M >= 2) {
-      if (C.wmk) mixed_parity_finish();
-      hipLaunchKernelGGL(k_add3<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)D1, (const T*)P, Q);
+      if (C.wmk)
+        mixed_parity_finish();  // with Q = D1 + P
+      else
+        hipLaunchKernelGGL(k_add3<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)D1, (const T*)P, Q);
     } else {
       HIP_OK(hipMemcpyAsync(Q, P, ns * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
@@ -2354,13 +2354,14 @@ struct Engine : EngineBase {
       b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0}, tmpX[b.n]);
     }
   }
+  // (and Q = D1 + P)
   void mixed_parity_finish() {
     hos_ctx& C = *c;
     const int n1 = C.n1, n2 = C.n2;
     gemm(cb, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
     if (n2 > 1) gemm(cb, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
     hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, (const T*)RA,
-                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2);
+                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q);
   }
 
   T* tmpB2() { return X; }
