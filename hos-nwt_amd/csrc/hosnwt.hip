@@ -1991,7 +1991,10 @@ struct Engine : EngineBase {
     f1 = ar.get<double>(2 * na); f2 = f1 + na; fz1 = ar.get<double>(2 * na); fz2 = ar.get<double>(2 * na);
     nch = (n3a + kPhiI3 - 1) / kPhiI3;
     part = (n2 > 1) ? ar.get<double>((size_t)nch * kPhiF * ns) : nullptr;
-    phi_add = ar.get<double>(ns); phix_add = ar.get<double>(ns); phiy_add = ar.get<double>(ns);
+    // phi_add is read only by hos_get_physical, after the join of the evaluation that
+    // wrote it: the context's last-evaluation field is this buffer itself
+    phi_add = c->phi_add_last;
+    phix_add = ar.get<double>(ns); phiy_add = ar.get<double>(ns);
     phiz_add = ar.get<double>(ns); phit_add = ar.get<double>(ns); phit1 = ar.get<double>(ns);
     phiz1 = ar.get<double>(ns); w1 = ar.get<double>(ns); tmp1 = ar.get<double>(ns);
     // REDFT01 along z and REDFT00 along y as column-major matrices
@@ -2338,7 +2341,6 @@ struct Engine : EngineBase {
                          (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)nullptr,
                          (const double*)C.filt_fs, kG1, kG2);
     }
-    HIP_OK(hipMemcpyAsync(C.phi_add_last, phi_add, ns * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
 
   // filter_ext of the mixed-parity products on the reference grid: exact sine
