@@ -147,7 +147,6 @@ __global__ void k_volfix(int npart, const double* partial, T* eta_hat, double* e
   }
 }
 
-// eta^(j+1)/(j+1)! = eta^j/j! / (j+1) * eta (resol_HOS.f90:974-982); |grad eta|^2.
 template <typename T>
 struct PtrSet {
   T* p[10];
@@ -160,7 +159,8 @@ struct PtrSet {
 // (measured on 1024 x 256, M = 5: 92 -> 102 FP64 and 130 -> 139 mixed steps/s
 // against accumulating in each transform's x-pass epilogue, one launch each).
 // The powers eta^j/j! (j < M; eta^M only feeds the dead last-order phimext) are
-// formed in registers from eta as (eta^j/j!)/(j+1)*eta, ascending; the loops run
+// formed in registers from eta as (eta^j/j!)/(j+1)*eta (resol_HOS.f90:974-982),
+// ascending; the loops run
 // over a fixed bound so that every index is known at compile time.
 constexpr int kMaxOrder = 10;
 template <typename T>
