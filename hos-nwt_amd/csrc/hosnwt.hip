@@ -275,15 +275,57 @@ __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, 
   }
 }
 
-// D1 += RA_X + RB_X, D2 += RA_Y + RB_Y (exact reference-grid filter of the
-// mixed-parity products), and Q = D1 + P.
+// RA = T1 PA in a 2-D tank (n1 x K1 times K1 x 2, column-major): a matrix-vector
+// product with two right-hand sides, split over K in chunks of kc so that the
+// matrix streams through many workgroups.  Chunk s writes its sums to
+// part[s][c][m]; k_add_mixed adds the chunks in order.
+constexpr int kT1Parts = 64;     // chunks at most
+constexpr int kT1MaxChunk = 256;  // k values per chunk at most
 template <typename T>
-__global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* D2, const T* P, T* Q) {
+__global__ void k_t1_partial(int n1, int K1, int kc, const T* __restrict__ T1, const T* __restrict__ PA, T* part) {
+  __shared__ T pa[2][kT1MaxChunk];
+  const int s = blockIdx.y, k0 = s * kc, kn = min(kc, K1 - k0);
+  for (int j = threadIdx.x; j < 2 * kc; j += blockDim.x) {
+    const int cc = j / kc, kk = j % kc;
+    pa[cc][kk] = (kk < kn) ? PA[k0 + kk + (size_t)K1 * cc] : (T)0;
+  }
+  __syncthreads();
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n1) return;
+  const T* col = T1 + m + (size_t)n1 * k0;
+  T a0 = (T)0, a1 = (T)0;
+#pragma unroll 8
+  for (int kk = 0; kk < kn; ++kk) {
+    const T t = col[(size_t)n1 * kk];
+    a0 += t * pa[0][kk];
+    a1 += t * pa[1][kk];
+  }
+  part[(size_t)(2 * s) * n1 + m] = a0;
+  part[(size_t)(2 * s + 1) * n1 + m] = a1;
+}
+
+// D1 += RA_X + RB_X, D2 += RA_Y + RB_Y (exact reference-grid filter of the
+// mixed-parity products), and Q = D1 + P.  With nparts > 0 (2-D) RA comes as the
+// chunk sums of k_t1_partial, added here in chunk order.
+template <typename T>
+__global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* D2, const T* P, T* Q,
+                            int nparts) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int i1 = (int)(i % n1), i2 = (int)(i / n1);
-  T a = RA[i], b = RA[n + i];
+  T a, b;
+  if (nparts > 0) {
+    a = RA[i];
+    b = RA[n1 + i];
+    for (int s = 1; s < nparts; ++s) {
+      a += RA[(size_t)(2 * s) * n1 + i];
+      b += RA[(size_t)(2 * s + 1) * n1 + i];
+    }
+  } else {
+    a = RA[i];
+    b = RA[n + i];
+  }
   if (RB) {
     a += RB[i1 + (size_t)2 * n1 * i2];
     b += RB[n1 + i1 + (size_t)2 * n1 * i2];
@@ -1900,6 +1942,7 @@ struct Engine : EngineBase {
   // mixed-parity products: spectra, conversion matrices, results
   int K1 = 0, K2 = 0;
   T *XA, *YA, *XB, *YB, *PA, *PB, *RA, *RB, *T1, *T2;
+  int t1_chunk = 0, t1_parts = 0;  // 2-D: K chunk of the T1 product and the number of chunks
   rocblas_handle blas = nullptr;
   // The wavemaker chain (paddle motion, paddle-grid products, wavemaker sums,
   // first-order and gradient transforms) depends only on the base-grid eta, so
@@ -2099,7 +2142,17 @@ struct Engine : EngineBase {
       if (M < 3) throw std::runtime_error("the wavemaker coupling needs mHOS >= 3 (or mHOS = 1)");
       K1 = std::min(2 * n1, c->N1b);
       PA = ar.get<T>((size_t)K1 * 2 * n2);
-      RA = ar.get<T>((size_t)n1 * 2 * n2);
+      // 2-D: k_t1_partial in FP64 at every size, and in FP32 from n1 = 2048 up (rocBLAS's
+      // two-column products measured: DGEMM 2.3-2.7x slower at n1 = 4096; SGEMM 4 %
+      // faster at n1 = 1024, equal at 2048, 1.7x slower at 4096)
+      if (n2 == 1 && (sizeof(T) == 8 || n1 >= 2048)) {  // the chunk sums of k_t1_partial
+        t1_chunk = std::max(32, (K1 + kT1Parts - 1) / kT1Parts);
+        t1_parts = (K1 + t1_chunk - 1) / t1_chunk;
+        if (t1_chunk > kT1MaxChunk) throw std::runtime_error("n1 too large for the 2-D mixed-parity product");
+        RA = ar.get<T>((size_t)t1_parts * 2 * n1);
+      } else {
+        RA = ar.get<T>((size_t)n1 * 2 * n2);
+      }
       T1 = upload_t(sin_to_cos_matrix(Nd1, n1, K1));
       if (n2 > 1) {
         K2 = std::min(2 * n2, c->N2b);
@@ -2360,10 +2413,15 @@ struct Engine : EngineBase {
   void mixed_parity_finish() {
     hos_ctx& C = *c;
     const int n1 = C.n1, n2 = C.n2;
-    gemm(cb, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
-    if (n2 > 1) gemm(cb, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
+    if (t1_parts > 0) {
+      hipLaunchKernelGGL(k_t1_partial<T>, dim3(nblk(n1), t1_parts), dim3(256), 0, cs, n1, K1, t1_chunk,
+                         (const T*)T1, (const T*)PA, RA);
+    } else {
+      gemm(cb, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
+      if (n2 > 1) gemm(cb, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
+    }
     hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, (const T*)RA,
-                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q);
+                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q, t1_parts);
   }
 
   T* tmpB2() { return X; }
