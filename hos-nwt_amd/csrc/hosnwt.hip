@@ -275,14 +275,17 @@ __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, 
   }
 }
 
-// RA = T1 PA in a 2-D tank (n1 x K1 times K1 x 2, column-major): a matrix-vector
-// product with two right-hand sides, split over K in chunks of kc so that the
-// matrix streams through many workgroups.  Chunk s writes its sums to
-// part[s][c][m]; k_add_mixed adds the chunks in order.
+// RA = T1 PA in a 2-D tank (n1 x K1 times K1 x 2): a matrix-vector product with two
+// right-hand sides.  T1[k][s] vanishes when s + k is even, so only the other half is
+// stored: T1h[k + n1 j] = T1[k][2j + (k even ? 1 : 0)], j < K1/2.  The product is
+// split over K in chunks of kc (even) so that the matrix streams through many
+// workgroups; chunk s writes its sums to part[s][c][m] and k_add_mixed adds the
+// chunks in order.
 constexpr int kT1Parts = 64;     // chunks at most
 constexpr int kT1MaxChunk = 256;  // k values per chunk at most
 template <typename T>
-__global__ void k_t1_partial(int n1, int K1, int kc, const T* __restrict__ T1, const T* __restrict__ PA, T* part) {
+__global__ void k_t1_partial(int n1, int K1, int kc, const T* __restrict__ T1h, const T* __restrict__ PA,
+                             T* part) {
   __shared__ T pa[2][kT1MaxChunk];
   const int s = blockIdx.y, k0 = s * kc, kn = min(kc, K1 - k0);
   for (int j = threadIdx.x; j < 2 * kc; j += blockDim.x) {
@@ -292,13 +295,14 @@ __global__ void k_t1_partial(int n1, int K1, int kc, const T* __restrict__ T1, c
   __syncthreads();
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n1) return;
-  const T* col = T1 + m + (size_t)n1 * k0;
+  const int par = (m & 1) ? 0 : 1;  // parity of the nonzero columns of row m
+  const T* col = T1h + m + (size_t)n1 * (k0 / 2);
   T a0 = (T)0, a1 = (T)0;
 #pragma unroll 8
-  for (int kk = 0; kk < kn; ++kk) {
-    const T t = col[(size_t)n1 * kk];
-    a0 += t * pa[0][kk];
-    a1 += t * pa[1][kk];
+  for (int jj = 0; jj < kn / 2; ++jj) {
+    const T t = col[(size_t)n1 * jj];
+    a0 += t * pa[0][2 * jj + par];
+    a1 += t * pa[1][2 * jj + par];
   }
   part[(size_t)(2 * s) * n1 + m] = a0;
   part[(size_t)(2 * s + 1) * n1 + m] = a1;
@@ -2146,14 +2150,21 @@ struct Engine : EngineBase {
       // two-column products measured: DGEMM 2.3-2.7x slower at n1 = 4096; SGEMM 4 %
       // faster at n1 = 1024, equal at 2048, 1.7x slower at 4096)
       if (n2 == 1 && (sizeof(T) == 8 || n1 >= 2048)) {  // the chunk sums of k_t1_partial
+        if (K1 % 2) throw std::runtime_error("2-D mixed-parity product: odd K");
         t1_chunk = std::max(32, (K1 + kT1Parts - 1) / kT1Parts);
+        t1_chunk += t1_chunk % 2;  // chunks start at even s
         t1_parts = (K1 + t1_chunk - 1) / t1_chunk;
         if (t1_chunk > kT1MaxChunk) throw std::runtime_error("n1 too large for the 2-D mixed-parity product");
         RA = ar.get<T>((size_t)t1_parts * 2 * n1);
+        const std::vector<double> full = sin_to_cos_matrix(Nd1, n1, K1);
+        std::vector<double> half((size_t)n1 * (K1 / 2));
+        for (int j = 0; j < K1 / 2; ++j)
+          for (int k = 0; k < n1; ++k) half[k + (size_t)n1 * j] = full[k + (size_t)n1 * (2 * j + (k % 2 ? 0 : 1))];
+        T1 = upload_t(half);
       } else {
         RA = ar.get<T>((size_t)n1 * 2 * n2);
+        T1 = upload_t(sin_to_cos_matrix(Nd1, n1, K1));
       }
-      T1 = upload_t(sin_to_cos_matrix(Nd1, n1, K1));
       if (n2 > 1) {
         K2 = std::min(2 * n2, c->N2b);
         XB = ar.get<T>(nb); YB = ar.get<T>(nb);
