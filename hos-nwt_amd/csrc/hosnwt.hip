@@ -2505,7 +2505,8 @@ struct Engine : EngineBase {
     base64(KCOS, KCOS, FWD, src_plain<double, double>(tmp1, n1, n1, n2), n1, n2, Dst<double>{ka1, n1, 0});
     // grad phi_add onto the extended grid (resol_HOS.f90:534-544)
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
-    base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
+    if (n2 > 1)  // (a 2-D tank reads no y gradient)
+      base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
     surface_forcing_modes();
   }
 
@@ -2721,7 +2722,8 @@ struct Engine : EngineBase {
     // phit1 / phiz1 were overwritten by the surface sums of the totals; the first-order
     // conditions above have used the first-order values already.
     base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
-    base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
+    if (n2 > 1)  // (a 2-D tank reads no y gradient)
+      base(KCOS, KCOS, FWD, src_plain<T, double>(phiy_add, n1, n1, n2), n1, n2, Dst<T>{phiyh, n1, 0});
     surface_forcing_modes();
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
