@@ -1821,6 +1821,34 @@ void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx,
   gemm(blas, rocblas_operation_transpose, nout1, nout2, n2, g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
 }
 
+// y_f[m] = sum_k AT[k + n m] x_f[k], f < nv: dense FP64 base transforms of a 2-D
+// tank for several vectors with one read of the matrix (one workgroup per output,
+// the vectors from L2), each sum in a fixed order.
+constexpr int kMaxGemv = 3;
+struct GemvSet {
+  int nv;
+  const double* x[kMaxGemv];
+  double* y[kMaxGemv];
+};
+__global__ void __launch_bounds__(256) k_gemv_multi(int n, const double* __restrict__ AT, GemvSet g) {
+  const int m = blockIdx.x;
+  const double* col = AT + (size_t)n * m;
+  double acc[kMaxGemv] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const double a = col[k];
+#pragma unroll
+    for (int f = 0; f < kMaxGemv; ++f)
+      if (f < g.nv) acc[f] += a * g.x[f][k];
+  }
+#pragma unroll
+  for (int f = 0; f < kMaxGemv; ++f) {
+    if (f < g.nv) {  // the same for every thread
+      const double sum = block_sum(acc[f]);
+      if (threadIdx.x == 0) g.y[f][m] = sum;
+    }
+  }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -2493,10 +2521,63 @@ struct Engine : EngineBase {
 
   // First-order free-surface RHS (phidphiz) and the modes of grad phi_add, one
   // base transform at a time.
+  // Dense FP64 cos-cos base transforms of a 2-D tank for nv <= 3 vectors with one
+  // read of the matrix; sources with a multiplier go through the grid's two staging
+  // vectors.
+  void dense_multi_2d(int dir, int nv, const Src<double, double>* src, double* const* out) {
+    DenseGrid<double>& g = grid64();
+    const int n1 = c->n1, di = (dir == FWD) ? 0 : 1;
+    if (c->n2 != 1 || nv > kMaxGemv) throw std::runtime_error("dense_multi_2d: 2-D, three vectors at most");
+    double* stage[2] = {g.stage, g.mid};
+    int used = 0;
+    GemvSet gs{};
+    gs.nv = nv;
+    for (int f = 0; f < nv; ++f) {
+      const Src<double, double>& sf = src[f];
+      if (sf.mul == MUL_NONE && sf.ld == n1 && sf.nx == n1 && sf.ny == 1 && (const void*)sf.p != (void*)out[f]) {
+        gs.x[f] = sf.p;
+      } else {
+        if (used == 2) throw std::runtime_error("dense_multi_2d: two staged sources at most");
+        double* sp = stage[used++];
+        hipLaunchKernelGGL((k_dense_in<double, Src<double, double>>), dim3(nblk(n1)), dim3(256), 0, cs, n1, 1, sf,
+                           sp);
+        gs.x[f] = sp;
+      }
+      gs.y[f] = out[f];
+    }
+    hipLaunchKernelGGL(k_gemv_multi, dim3(n1), dim3(256), 0, cs, n1, (const double*)g.mx[KCOS][di], gs);
+    HIP_OK(hipGetLastError());
+    c->stats.n_base += nv;
+  }
+
   void first_order_single(const double* e1, const double* a1, double* ka1, double* ke1) {
     hos_ctx& C = *c;
     hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2;
+    if constexpr (kSplit) {
+      if (dense && n2 == 1) {
+        // 2-D dense grid: the FP64 transforms of one direction share one read of
+        // the matrix (inverse {kth1 a1st, kth1 phis}, forward {phidphiz, phiz_add,
+        // forcing}); the same fields and kernels as below and in surface_forcing_modes
+        const Src<double, double> inv[2] = {src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy),
+                                            src_mul<double, double>(phis64, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy)};
+        double* const invo[2] = {w1, wlin};
+        dense_multi_2d(INV, 2, inv, invo);
+        hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
+                           (const double*)phiz1, (const double*)phit1, e1, (const double*)C.nu_d,
+                           (double)C.p.iabsnb, ke1, tmp1);
+        hipLaunchKernelGGL(k_forcing, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phiz_add,
+                           (const double*)phit_add, (const double*)wlin, (const double*)C.nu_d, Zf64);
+        const Src<double, double> fwd[3] = {src_plain<double, double>(tmp1, n1, n1, n2),
+                                            src_plain<double, double>(phiz_add, n1, n1, n2),
+                                            src_plain<double, double>(Zf64, n1, n1, n2)};
+        double* const fwdo[3] = {ka1, phizh64, Zh64};
+        dense_multi_2d(FWD, 3, fwd, fwdo);
+        hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phizh64, phizh);
+        base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
+        return;
+      }
+    }
     base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
            Dst<double>{w1, n1, 0});
     hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
