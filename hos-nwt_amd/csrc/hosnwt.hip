@@ -2577,6 +2577,22 @@ struct Engine : EngineBase {
         base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
         return;
       }
+    } else {
+      if (dense && n2 == 1) {
+        // FP64 path, 2-D dense grid: the forward transforms {phidphiz, phix_add,
+        // phiz_add} share one read of the matrix (surface_forcing_modes's one included)
+        base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
+               Dst<double>{w1, n1, 0});
+        hipLaunchKernelGGL(k_first_order, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)w1,
+                           (const double*)phiz1, (const double*)phit1, e1, (const double*)C.nu_d,
+                           (double)C.p.iabsnb, ke1, tmp1);
+        const Src<double, double> fwd[3] = {src_plain<double, double>(tmp1, n1, n1, n2),
+                                            src_plain<double, double>(phix_add, n1, n1, n2),
+                                            src_plain<double, double>(phiz_add, n1, n1, n2)};
+        double* const fwdo[3] = {ka1, reinterpret_cast<double*>(phixh), reinterpret_cast<double*>(phizh)};
+        dense_multi_2d(FWD, 3, fwd, fwdo);
+        return;
+      }
     }
     base64(KCOS, KCOS, INV, src_mul<double, double>(a1, n1, n1, n2, MUL_KTH, 1, C.pix, C.piy), n1, n2,
            Dst<double>{w1, n1, 0});
