@@ -9,11 +9,13 @@ spreading (s = 10) over 256 frequencies x 32 directions, beach on the last 15 %.
 A step is one runge4_adapt2 attempt (5 solveHOS2 evaluations + combination +
 embedded error), accepted or rejected by the reference controller; the state
 stays resident in HBM for the whole timed region.  Both precisions are timed:
-FP64 everywhere, and FP32 transforms/products with FP64 RK state ("mixed");
-the mixed path repeats the FP64 path's (h, accept) decisions so that both end
-at the same physical time.
+FP64 everywhere, and FP32 transforms/products with FP64 RK state ("mixed").
+Each path first runs untimed from rest through the wavemaker ramp (2 Tp), so
+the timed attempts see a developed sea; the mixed path runs free, with its own
+error estimates and accept/reject decisions, to the FP64 run's two times (end
+of warm-up, end of the timed attempts).
 `value` is the mixed path's steps/s when its eta stays within 1e-5 relative L2
-of the FP64 path after the timed steps, otherwise the FP64 path's.
+of the FP64 path at the end of the timed attempts, otherwise the FP64 path's.
 """
 from __future__ import annotations
 
@@ -64,6 +66,9 @@ def run_steps(tank, s, n, t0, h0, t_stop=None):
             n_rej += 1
         else:
             tank.commit()
+            # end-of-ramp correction of the paddle modes (HOS-NWT.f90:546-550)
+            if s.cfg.iramp == 1 and s.Tramp - (t + hl) <= 2.220446049250313e-16 and s.Tramp > t:
+                tank.end_ramp()
             t += hl
             corr = 2.0 if err < 2.2e-16 else 0.9 * (err / toler) ** -0.25
             n_acc += 1
@@ -72,22 +77,28 @@ def run_steps(tank, s, n, t0, h0, t_stop=None):
 
 
 def time_path(precision, steps, warmup, spin_up, follow=None):
-    """``follow`` = (t_warm, t_end) of an earlier run: run freely (own controller) to t_warm
-    untimed, then time the attempts that reach t_end."""
+    """Untimed: from rest to nondimensional time ``spin_up``, then ``warmup`` attempts; timed:
+    ``steps`` attempts.  ``follow`` = (t_warm, t_end) of an earlier run: run freely (own
+    controller) to t_warm untimed, then time the attempts that reach t_end."""
     from hosnwt.tank import Tank
     s = make_setup(precision)
     tank = Tank(s)
     tank.initialize()
     h0 = min(0.95 * s.dt_stab, s.delt_out)
-    n0 = spin_up + warmup
-    cap = 10 * (n0 + steps) + 100
-    t, h, _, _ = run_steps(tank, s, cap if follow else n0, 0.0, h0,
-                           t_stop=follow[0] if follow else None)
+    cap = 100000
+    if follow:
+        # the same landing times as the run followed: the ramp's end, then its warm-up end
+        t, h, _, _ = run_steps(tank, s, cap, 0.0, h0, t_stop=min(spin_up, follow[0])) if spin_up > 0 else (0.0, h0, 0, 0)
+        t, h, _, _ = run_steps(tank, s, cap, t, h, t_stop=follow[0])
+    else:
+        t, h, _, _ = run_steps(tank, s, cap, 0.0, h0, t_stop=spin_up) if spin_up > 0 else (0.0, h0, 0, 0)
+        t, h, _, _ = run_steps(tank, s, warmup, t, h)
     t_warm = t
     tank.synchronize()
     st0 = tank.stats()
     t_start = time.perf_counter()
     t, h, na, nr = run_steps(tank, s, cap if follow else steps, t, h, t_stop=follow[1] if follow else None)
+    tank.synchronize()
     tank.synchronize()
     dt = time.perf_counter() - t_start
     steps = na + nr
@@ -101,23 +112,25 @@ def time_path(precision, steps, warmup, spin_up, follow=None):
 
 
 def roofline(tank, precision):
-    """Live HIP-event timing of the dominant kernel family (extended-grid r2r passes)."""
+    """Live HIP-event timing of the dominant kernel family (extended-grid r2r passes) in the
+    precision of the headline path, on the context's stream."""
     r = tank.time_big_transform(reps=20)
     achieved = r["bytes"] / (r["ms"] * 1e-3) / 1e9
     traffic = None
-    t = traffic_table()
+    t = traffic_table(precision)
     st = tank.stats()
-    if t is not None and precision == 0 and list(t["grid"]) == [st["big_n1"], st["big_n2"]]:
+    if t is not None and list(t["grid"]) == [st["big_n1"], st["big_n2"]]:
         traffic = t["traffic_bytes"][r["pass"]]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": r["kernel"],
             "bytes_per_launch": r["bytes"], "avg_ms": round(r["ms"], 5)}
 
 
-def traffic_table():
+def traffic_table(precision=0):
     """Scaled FETCH_SIZE + WRITE_SIZE per launch of the extended-grid passes, from the committed
-    rocprofv3 summary (profiles/extended_pass_1024x256_M5_traffic.md); None when absent."""
-    path = os.path.join(ROOT, "profiles", "extended_pass_1024x256_M5_traffic.md")
+    rocprofv3 summary (profiles/extended_pass_1024x256_M5_traffic*.md); None when absent."""
+    name = "extended_pass_1024x256_M5_traffic.md" if precision == 0 else "extended_pass_1024x256_M5_traffic_fp32.md"
+    path = os.path.join(ROOT, "profiles", name)
     try:
         text = open(path).read()
         block = text.split("```json", 1)[1].split("```", 1)[0]
@@ -126,12 +139,12 @@ def traffic_table():
         return None
 
 
-def pass_probe(reps):
+def pass_probe(reps, precision=0):
     """Extended-grid pass timing (the roofline launches) plus a 64 MiB device-to-device copy whose
     blit kernel calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
     import ctypes as C
     from hosnwt.tank import Tank
-    s = make_setup(0)
+    s = make_setup(precision)
     tank = Tank(s)
     tank.time_big_transform(reps=2)
     r = tank.time_big_transform(reps=reps)
@@ -243,7 +256,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--spin-up", type=int, default=0, help="untimed accepted/rejected attempts before warmup")
+    ap.add_argument("--spin-up", type=float, default=None, metavar="T",
+                    help="untimed run from rest to nondimensional time T before the warm-up attempts "
+                         "(default: the end of the wavemaker ramp, 2 Tp)")
+    ap.add_argument("--precision", type=int, default=0, help="precision of --pass-probe (0 FP64, 1 FP32)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--pass-probe", type=int, default=0, metavar="REPS",
@@ -252,7 +268,7 @@ def main():
                     help="only run the ragged 2-D ensemble to nondimensional time T (sequential vs together)")
     args = ap.parse_args()
     if args.pass_probe:
-        pass_probe(args.pass_probe)
+        pass_probe(args.pass_probe, args.precision)
         return
     if args.ensemble:
         print(json.dumps(ensemble_bench(args.ensemble)), flush=True)
@@ -261,10 +277,12 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "
               "(ensembles and a single field both fit one MI355X).", file=sys.stderr)
         sys.exit(2)
-    print(f"[bench] FP64 path: {args.warmup} warmup + {args.steps} timed attempts", file=sys.stderr, flush=True)
-    r64 = time_path(0, args.steps, args.warmup, args.spin_up)
+    spin = make_setup(0).Tramp if args.spin_up is None else args.spin_up
+    print(f"[bench] FP64 path: spin-up to t = {spin:.3f}, {args.warmup} warmup + {args.steps} timed attempts",
+          file=sys.stderr, flush=True)
+    r64 = time_path(0, args.steps, args.warmup, spin)
     print(f"[bench] FP64 {r64['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
-    roof = roofline(r64["tank"], 0)
+    roof64 = roofline(r64["tank"], 0)
     # one live context at a time, the deployment shape: with the FP64 context still alive the
     # mixed path measured 114 instead of 135 steps/s (its two streams then share the process's
     # four hardware queues with the idle context's streams)
@@ -274,7 +292,7 @@ def main():
     if not args.fp64_only:
         print("[bench] mixed path", file=sys.stderr, flush=True)
         # free-running with its own error estimates and decisions, to the FP64 run's times
-        r32 = time_path(1, args.steps, args.warmup, args.spin_up, follow=(r64["t_warm"], r64["t_end"]))
+        r32 = time_path(1, args.steps, args.warmup, spin, follow=(r64["t_warm"], r64["t_end"]))
         print(f"[bench] mixed {r32['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     eta_err = None
     if r32 is not None:
@@ -282,6 +300,9 @@ def main():
         eta_err = float(d / max(np.linalg.norm(r64["eta"]), 1e-300))
     use_mixed = r32 is not None and eta_err is not None and eta_err <= 1e-5
     best = r32 if use_mixed else r64
+    # the roofline object describes the headline path's dominant kernel (the FP32 passes when
+    # the mixed path is the headline)
+    roof = roofline(r32["tank"], 1) if use_mixed else roof64
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
@@ -313,6 +334,7 @@ def main():
         "mixed_accepted_rejected": [r32["n_acc"], r32["n_rej"]] if r32 else None,
         "simulated_time": [r64["t_warm"], r64["t_end"]],
         "roofline": roof,
+        "roofline_fp64": roof64 if use_mixed else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -1953,6 +1953,7 @@ struct Engine : EngineBase {
   std::vector<T*> phizm, aHOS;
   std::vector<T*> lvl;  // outputs of one HOS level's transforms
   std::vector<T*> tmpX;  // x-pass intermediates of batched transforms
+  double* slope_part = nullptr;  // block maxima of |eta_x|, |eta_y| (slope_max)
   T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
   // wavemaker / first order (double)
   double *Xp, *Xpt, *Xptt, *f1, *f2, *fz1, *fz2, *gz, *Wz, *Wy, *freq, *pre, *pim, *part;
@@ -2058,6 +2059,7 @@ struct Engine : EngineBase {
     for (int i = 0; i < kMaxBatch && i < std::max(M, 7); ++i) tmpX.push_back(ar.get<T>(nb));
     for (int i = 0; i < M; ++i) lvl.push_back(ar.get<T>(nb));
     etax = ar.get<T>(nb); etay = ar.get<T>(nb); phisx = ar.get<T>(nb); phisy = ar.get<T>(nb);
+    slope_part = ar.get<double>(2 * (size_t)nblk(nb));
     g2 = ar.get<T>(nb); phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
     phiza = ar.get<T>(nb); X = ar.get<T>(nb); Y = ar.get<T>(nb); tmpB = ar.get<T>(nb);
     // wavemaker
@@ -2177,12 +2179,13 @@ struct Engine : EngineBase {
       // 2-D: k_t1_partial in FP64 at every size, and in FP32 from n1 = 2048 up (rocBLAS's
       // two-column products measured: DGEMM 2.3-2.7x slower at n1 = 4096; SGEMM 4 %
       // faster at n1 = 1024, equal at 2048, 1.7x slower at 4096)
-      if (n2 == 1 && (sizeof(T) == 8 || n1 >= 2048)) {  // the chunk sums of k_t1_partial
+      // (beyond kT1MaxChunk per chunk, i.e. n1 > 8192, the rocBLAS product below)
+      int chunk = std::max(32, (K1 + kT1Parts - 1) / kT1Parts);
+      chunk += chunk % 2;  // chunks start at even s
+      if (n2 == 1 && (sizeof(T) == 8 || n1 >= 2048) && chunk <= kT1MaxChunk) {  // the chunk sums of k_t1_partial
         if (K1 % 2) throw std::runtime_error("2-D mixed-parity product: odd K");
-        t1_chunk = std::max(32, (K1 + kT1Parts - 1) / kT1Parts);
-        t1_chunk += t1_chunk % 2;  // chunks start at even s
+        t1_chunk = chunk;
         t1_parts = (K1 + t1_chunk - 1) / t1_chunk;
-        if (t1_chunk > kT1MaxChunk) throw std::runtime_error("n1 too large for the 2-D mixed-parity product");
         RA = ar.get<T>((size_t)t1_parts * 2 * n1);
         const std::vector<double> full = sin_to_cos_matrix(Nd1, n1, K1);
         std::vector<double> half((size_t)n1 * (K1 / 2));
@@ -2924,8 +2927,9 @@ struct Engine : EngineBase {
   void slope_max(double& sx, double& sy) override {
     hos_ctx& C = *c;
     const unsigned nbk = nblk(nb);
-    double* pa = C.ar.get<double>(nbk);
-    double* pb = C.ar.get<double>(nbk);
+    // (k_absmax2 writes every block's entry of both partial buffers)
+    double* pa = slope_part;
+    double* pb = slope_part + nbk;
     hipLaunchKernelGGL(k_absmax2, dim3(nbk), dim3(256), 0, C.st, nb, (const void*)etax, (const void*)etay,
                        sizeof(T) == 4 ? 1 : 0, pa, pb);
     hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, C.st, (int)nbk, (const double*)pa, C.red6);

@@ -156,3 +156,81 @@ def test_mixed_directional_sea_at_output_times():
     finally:
         for tk in tanks:
             tk.close()
+
+
+
+def _controlled_run(tank, s, t_outs, toler, log=None, replay=None):
+    """The reference inner loop (HOS-NWT.f90:498-571, end-of-ramp correction included) driven
+    from the host: records the (t, h, accept) decisions in ``log`` or replays ``replay``.
+    Returns eta (physical) at each output time."""
+    tiny = 2.220446049250313e-16
+    t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+    stab, k, out = s.dt_stab, 0, []
+    for t_out in t_outs:
+        while t < t_out:
+            if replay is not None:
+                t_r, hl, acc = replay[k]
+                k += 1
+                assert abs(t_r - t) <= 1e-12 * max(1.0, t)
+                err = tank.runge4_adapt2(t, hl)
+            else:
+                hl = min(h, t_out - t)
+                err = tank.runge4_adapt2(t, hl)
+                acc = not (err > toler and h > 0.001 * 0.95 * stab)
+                if log is not None:
+                    log.append((t, hl, acc))
+            if acc:
+                tank.commit()
+                if s.cfg.iramp == 1 and s.Tramp - (t + hl) <= tiny and s.Tramp > t:
+                    tank.end_ramp()
+                t += hl
+                corr = 2.0 if err < tiny else 0.9 * (err / toler) ** -0.25
+            else:
+                corr = 0.9 * (err / toler) ** -0.25
+            h = max(min(h * corr, 0.95 * stab, s.delt_out), 1e-4 * 0.95 * stab, 1e-4 * s.delt_out)
+        out.append(tank.physical()["eta"].copy())
+    return out
+
+
+@pytest.mark.gpu
+def test_mixed_directional_sea_past_ramp():
+    """The directional-sea workload (1024 x 256, M = 5) from rest through the linear ramp
+    (2 Tp) to 4 Tp, output every Tp/2.
+
+    * Precision error: the mixed path replaying the FP64 run's (h, accept) decisions stays
+      within 1e-6 relative L2 of it at every output time (FP32 arithmetic is the only
+      difference).
+    * Free-running: each path with its own controller.  At toler 1e-7 the ramp's end
+      (alpha' jumps to 0 and the paddle modes are corrected, HOS-NWT.f90:546-550) leaves a
+      time-integration error of a few 1e-4 (FP64 at 1e-7 against FP64 at 1e-8), so two
+      free-running paths differ by that error's dependence on the step sequence.  The mixed
+      path must stay within 1e-5 up to the ramp's end and, past it, closer to the FP64 run
+      than the FP64 run is to its own tenfold-tighter integration."""
+    import bench
+
+    s64, s32 = bench.make_setup(0), bench.make_setup(1)
+    tp = s64.delt_out
+    outs = [k * tp / 2.0 for k in range(1, 9)]
+    runs = {}
+    log = []
+    for name, s, tol, kw in (("fp64", s64, 1e-7, dict(log=log)), ("replay", s32, 1e-7, dict(replay=log)),
+                             ("free", s32, 1e-7, {}), ("fp64_tight", s64, 1e-8, {})):
+        tank = Tank(s)
+        try:
+            tank.initialize()
+            runs[name] = _controlled_run(tank, s, outs, tol, **kw)
+        finally:
+            tank.close()
+        print(f"{name}: done", flush=True)
+    for i, t_out in enumerate(outs):
+        ref = runs["fp64"][i]
+        r_rep, r_free = rel(runs["replay"][i], ref), rel(runs["free"][i], ref)
+        r_int = rel(ref, runs["fp64_tight"][i])
+        print(f"t = {t_out / tp:.1f} Tp: mixed replayed {r_rep:.2e}, mixed free {r_free:.2e}, "
+              f"FP64 toler 1e-7 vs 1e-8 {r_int:.2e}", flush=True)
+        assert np.abs(ref).max() > 0
+        assert r_rep < 1e-6
+        if t_out <= s64.Tramp * (1 + 1e-12):
+            assert r_free < 1e-5
+        else:
+            assert r_free < r_int
