@@ -3,9 +3,16 @@
 set -e
 here="$(cd "$(dirname "$0")" && pwd)"
 mkdir -p "$here/lib"
-hipcc --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
-  -Xclang -target-feature -Xclang -packed-fp32-ops \
-  -o "$here/lib/libhosnwt.so" "$here/csrc/hosnwt.hip" -L/opt/rocm/lib -lrocblas
+flags="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Xclang -target-feature -Xclang -packed-fp32-ops"
+# the two translation units compile in parallel
+hipcc $flags -c "$here/csrc/hosnwt.hip" -o "$here/lib/hosnwt.o" &
+p1=$!
+hipcc $flags -c "$here/csrc/mlp.hip" -o "$here/lib/mlp.o" &
+p2=$!
+wait $p1
+wait $p2
+hipcc --offload-arch=gfx950 -shared -fPIC -o "$here/lib/libhosnwt.so" "$here/lib/hosnwt.o" "$here/lib/mlp.o" \
+  -L/opt/rocm/lib -lrocblas
 # Fortran iso_c_binding module and driver (AMD flang)
 if command -v amdflang >/dev/null 2>&1; then
   mkdir -p "$here/lib/mod"

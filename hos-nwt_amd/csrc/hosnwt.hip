@@ -38,6 +38,7 @@
 
 #include "../../include/hosnwt.h"
 #include "r2r.h"
+#include "mlp.h"
 
 using namespace hosnwt;
 
@@ -1368,6 +1369,161 @@ __global__ void k_absmax2(size_t n, const void* a, const void* b, int is_f32, do
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// per-order spectral band energies (features of the learned order selection)
+// ---------------------------------------------------------------------------
+// Record of one right-hand side: rec[b] = sum of eta_hat^2 over the modes of
+// band b; rec[64 j + b] = sum of (k tanh k a_j)^2, a_j the modal potential of
+// order j at z = 0 (order 1: phis_hat; order j >= 2: aHOS(j) of HOSphis_modes,
+// resol_HOS.f90:720-782), i.e. the band energy of the linear part of W^(j);
+// rec[kFeatM] = the order of the evaluation, rec[kFeatEta] = sum of eta_hat^2.
+// Band b holds the modes with b <= 64 |k| / |k|max < b + 1 (the last band
+// closed), |k|max = |k(n1-1, n2-1)|.
+constexpr int kFeatBands = 64;
+constexpr int kFeatOrders = 8;
+constexpr int kFeatM = (kFeatOrders + 1) * kFeatBands;  // 576
+constexpr int kFeatEta = kFeatM + 1;
+constexpr int kFeatRec = kFeatM + 2;
+constexpr int kFeatBlockModes = 512;
+constexpr int kFeatGrid = 256;  // blocks of the band-energy kernel (grid-stride)
+
+template <typename T>
+struct OrderFields {
+  const T* a[kFeatOrders + 1];  // a[j], j = 1..nord
+  int ld[kFeatOrders + 1];
+};
+
+// Deterministic: each block sums its modes band by band in a fixed order, and
+// the blocks' partials are added in block order (k_band_reduce).
+template <typename T>
+__global__ void __launch_bounds__(256) k_band_energy(int n1, int n2, double pix, double piy, int nord,
+                                                     OrderFields<T> f, const T* __restrict__ eta_hat,
+                                                     double* __restrict__ part) {
+  __shared__ int band[kFeatBlockModes];
+  __shared__ double val[kFeatOrders + 1][kFeatBlockModes];
+  const double kx1 = (n1 - 1) * pix, ky1 = (n2 - 1) * piy;
+  const double kmax = sqrt(kx1 * kx1 + ky1 * ky1);
+  const size_t ns = (size_t)n1 * n2;
+  const int nq = (nord + 1) * kFeatBands;
+  double acc[3] = {0.0, 0.0, 0.0};  // q = threadIdx.x + 256 r (nq <= 576)
+  double eta_tot = 0.0;
+  for (size_t base = (size_t)blockIdx.x * kFeatBlockModes; base < ns; base += (size_t)gridDim.x * kFeatBlockModes) {
+    for (int e = threadIdx.x; e < kFeatBlockModes; e += blockDim.x) {
+      const size_t m = base + e;
+      int b = -1;
+      if (m < ns) {
+        const int i1 = (int)(m % n1), i2 = (int)(m / n1);
+        const double k = wave_k(i1, i2, pix, piy);
+        const double kth = k * tanh(k);
+        b = kmax > 0.0 ? (int)(kFeatBands * k / kmax) : 0;
+        b = b < kFeatBands ? b : kFeatBands - 1;
+        const double ev = (double)eta_hat[m];
+        val[0][e] = ev * ev;
+        for (int j = 1; j <= nord; ++j) {
+          const double w = kth * (double)f.a[j][i1 + (size_t)f.ld[j] * i2];
+          val[j][e] = w * w;
+        }
+      }
+      band[e] = b;
+    }
+    __syncthreads();
+    for (int r = 0; r < 3; ++r) {
+      const int q = threadIdx.x + 256 * r;
+      if (q >= nq) break;
+      const int j = q / kFeatBands, b = q % kFeatBands;
+      double sum = 0.0;
+      for (int e = 0; e < kFeatBlockModes; ++e)
+        if (band[e] == b) sum += val[j][e];
+      acc[r] += sum;
+    }
+    if (threadIdx.x == 0)
+      for (int e = 0; e < kFeatBlockModes; ++e)
+        if (band[e] >= 0) eta_tot += val[0][e];
+    __syncthreads();
+  }
+  double* out = part + (size_t)blockIdx.x * kFeatRec;
+  for (int r = 0; r < 3; ++r) {
+    const int q = threadIdx.x + 256 * r;
+    if (q < kFeatM) out[q] = (q < nq) ? acc[r] : 0.0;
+  }
+  if (threadIdx.x == 0) {
+    out[kFeatM] = 0.0;
+    out[kFeatEta] = eta_tot;
+  }
+}
+
+__global__ void k_band_reduce(int nblk, int order, const double* __restrict__ part, double* __restrict__ rec) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= kFeatRec) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[(size_t)b * kFeatRec + q];
+  rec[q] = (q == kFeatM) ? (double)order : s;
+}
+
+
+// MLP input rows of one feature record (row r predicts order r + 2 from orders
+// 1..r+1; rows beyond the record's order, or beyond order 6, are zero):
+//   block 0     s(E_eta(b) / E_eta)
+//   block j     s(E_j(b) / E_eta), j = 1..r+1   (zero above)
+//   block 7     one-hot of r+1 at 448 + r + 1, (log10 E_eta + 10) / 10 at 456
+// with s(x) = (log10(max(x, 1e-30)) + 15) / 5; the target of row r is
+// s(E_{r+2}(b) / E_eta).  hosnwt/learned.py builds the same rows for training.
+constexpr int kRowIn = 512;
+constexpr int kPredOrders = 6;  // predictions for orders 2..7
+__device__ __forceinline__ double feat_scale(double x) { return (log10(fmax(x, 1e-30)) + 15.0) / 5.0; }
+__global__ void k_feat_rows(const double* __restrict__ rec, int m_run, __bf16* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 64 * kRowIn) return;
+  const int r = i / kRowIn, c = i % kRowIn;
+  const int mp = r + 1;  // orders available to this row
+  const int blk = c / kFeatBands, b = c % kFeatBands;
+  const double et = rec[kFeatEta];
+  double v = 0.0;
+  if (mp <= m_run && mp <= kPredOrders) {
+    const double inv = et > 0.0 ? 1.0 / et : 0.0;
+    if (blk == 0) {
+      v = feat_scale(rec[b] * inv);
+    } else if (blk <= kPredOrders) {
+      v = (blk <= mp) ? feat_scale(rec[blk * kFeatBands + b] * inv) : 0.0;
+    } else {
+      if (b == mp) v = 1.0;
+      if (b == 8) v = (log10(fmax(et, 1e-30)) + 10.0) / 10.0;
+    }
+  }
+  x[i] = (__bf16)(float)v;
+}
+
+// sum (a - b)^2 and sum b^2 over n values: block partials, then one block
+__global__ void k_diff_norm(size_t n, const double* __restrict__ a, const double* __restrict__ b,
+                            double* __restrict__ part) {
+  double d = 0.0, s = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double x = a[i] - b[i];
+    d += x * x;
+    s += b[i] * b[i];
+  }
+  d = block_sum(d);
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = d;
+    part[2 * blockIdx.x + 1] = s;
+  }
+}
+__global__ void k_diff_reduce(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  double d = 0.0, s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    d += part[2 * i];
+    s += part[2 * i + 1];
+  }
+  d = block_sum(d);
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    out[0] = d;
+    out[1] = s;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
@@ -1411,6 +1567,18 @@ int pad_smooth(int N) {
 }
 
 // ramp (resol_wmkr.f90:1268-1397)
+// Dealiasing parameters and extended grid of the order-m right-hand side: the
+// context's own p1, p2 at m = mHOS, p1 = p2 = m (3-D) below it, i.e. the grid of a
+// context created with mHOS = m (common_vars.f90:38-39 with p = M).
+void order_grid(const hos_params& p, int m, int& p1, int& p2, int& N1b, int& N2b) {
+  p1 = (m == p.mHOS) ? p.p1 : m;
+  p2 = (m == p.mHOS) ? p.p2 : (p.n2 > 1 ? m : 1);
+  const int Nd1 = ((p1 + 1) * p.n1) / 2;
+  const int Nd2 = p.n2 > 1 ? ((p2 + 1) * p.n2) / 2 : 1;
+  N1b = pad_smooth(std::max(Nd1, 2));
+  N2b = p.n2 > 1 ? pad_smooth(Nd2) : 1;
+}
+
 void ramp_eval(const hos_params& p, double t, double& a, double& at, double& att) {
   const double Tr = p.Tramp, Ts = p.T_stop;
   if (p.iramp == 11) {
@@ -1886,9 +2054,15 @@ struct hos_ctx {
   hipStream_t st = nullptr;
   Arena ar;
   Grid2<double> gs64;
-  std::unique_ptr<EngineBase> eng;
+  // right-hand-side engines by HOS order (built on first use); eng is the active one,
+  // of order M on the extended grid N1b x N2b
+  std::unique_ptr<EngineBase> engines[9];
+  EngineBase* eng = nullptr;
   // host copies for the paddle-top displacement pos1st(n3,1)
   std::vector<double> freq, pre0, pim0;
+  // host copies of the wavemaker inputs (pos_ini, dposdy_ini: n2 x n_freq; d nu/dx,
+  // d nu/dy: n1 x n2; empty when absent), from which every engine is built
+  std::vector<double> h_pos_re, h_pos_im, h_dy_re, h_dy_im, h_nux, h_nuy;
   double gz_top = 1.0;
   // wavemaker globals of the last evaluation (alpha, pos1st(n3,1))
   double last_alpha = 0.0, last_pos_top = 0.0, last_alphat = 0.0;
@@ -1910,9 +2084,36 @@ struct hos_ctx {
   double *partial, *pmax_e, *pmax_x, *red6;
   size_t pmax_stride;
   double* red_host = nullptr;  // pinned copy of red6 (the attempt's error maxima)
+  // learned order selection (hos_advance_learned): current order, floor after a
+  // reversion, accepted attempts since the last judge, the phase of the attempt in
+  // flight (0 at the selected order, 1 its full-order judge), the checkpoint of the
+  // last successful judge, and the pinned copies of the attempt's feature record,
+  // predictions and judged difference norms
+  struct Learn {
+    bool ready = false;
+    int m = 0, floor = 0, since = 0, phase = 0, m_attempt = 0;
+    double hl = 0.0, err_m = 0.0, pred_rate = 0.0;
+    double t_ck = 0.0, h_ck = 0.0, alpha_ck = 0.0, alphat_ck = 0.0, pos_ck = 0.0;
+    int endramp_ck = 0;
+    std::vector<double*> ck;
+    double *judge_eta = nullptr, *dpart = nullptr, *dn = nullptr;
+    double *rec_host = nullptr, *dn_host = nullptr;
+    float* pred_host = nullptr;
+    hosmlp::Workspace ws;
+    hos_learned_stats stats{};
+  } learn;
   ~hos_ctx() {
     if (red_host) (void)hipHostFree(red_host);
+    if (learn.rec_host) (void)hipHostFree(learn.rec_host);
+    if (learn.dn_host) (void)hipHostFree(learn.dn_host);
+    if (learn.pred_host) (void)hipHostFree(learn.pred_host);
+    if (learn.ws.x) hosmlp::workspace_free(learn.ws);
   }
+  // band-energy records of the learned order selection: feat_mode 0 off, 1 the
+  // first right-hand side of each attempt into record 0, 2 every right-hand side
+  // into a ring of feat_cap records (feat_n written since the last take)
+  int feat_mode = 0, feat_cap = 0, feat_n = 0, cur_stage = 0;
+  double *feat = nullptr, *feat_part = nullptr;
   double *phys_tmp, *nu_d;
   double* filt_fs = nullptr;  // free-surface mode filter weights (n1*n2), null when off
   double* filt_z = nullptr;   // paddle z-mode filter weights (n3_add), null when off
@@ -1943,6 +2144,9 @@ struct hos_ctx {
 template <typename T>
 struct Engine : EngineBase {
   hos_ctx* c;
+  // HOS order of this engine and its extended grid (a context holds one engine per
+  // order it runs at; the modal RK state is shared, only the extended grid depends on M)
+  int M, N1b, N2b, p1, p2;
   Grid2<T> gb;   // extended grid
   Grid2<T> gsT;  // base grid in T
   size_t nb, ns, na;
@@ -2031,15 +2235,15 @@ struct Engine : EngineBase {
       return d64;
   }
 
-  explicit Engine(hos_ctx* ctx) : c(ctx) {
+  Engine(hos_ctx* ctx, int order) : c(ctx), M(order) {
     Arena& ar = c->ar;
-    gb.build(c->N1b, c->N2b);
+    order_grid(c->p, order, p1, p2, N1b, N2b);
+    gb.build(N1b, N2b);
     gb.check_lengths(true);
     gsT.build(c->n1, c->n2);
-    nb = (size_t)c->N1b * c->N2b;
+    nb = (size_t)N1b * N2b;
     ns = (size_t)c->n1 * c->n2;
     na = (size_t)c->n3a * c->n2;
-    const int M = c->M;
     eta_hat = ar.get<T>(ns); phis_hat = ar.get<T>(ns); eta_phys = ar.get<T>(ns);
     P = ar.get<T>(ns); D1 = ar.get<T>(ns); D2 = ar.get<T>(ns); Q = ar.get<T>(ns);
     Zp = ar.get<T>(ns); Zh = ar.get<T>(ns); phixh = ar.get<T>(ns); phiyh = ar.get<T>(ns);
@@ -2114,15 +2318,10 @@ struct Engine : EngineBase {
       if (c->n2 > 1) multi = ar.get<double>((size_t)3 * kMulti * ns);
     }
     const int nf = c->p.n_freq;
-    std::vector<double> hf(c->p.freq, c->p.freq + nf);
-    std::vector<double> hr((size_t)n2 * nf), hi((size_t)n2 * nf);
-    if (nf) {
-      std::memcpy(hr.data(), c->p.pos_ini_re, hr.size() * sizeof(double));
-      std::memcpy(hi.data(), c->p.pos_ini_im, hi.size() * sizeof(double));
-    }
-    freq = upload(hf);
-    pre = upload(hr);
-    pim = upload(hi);
+    // (from the context's host copies: engines of other orders are built after hos_create)
+    freq = upload(c->freq);
+    pre = upload(c->h_pos_re);
+    pim = upload(c->h_pos_im);
     cont = continuation_matrix();
     if (c->n2 > 1 && c->n2 > kPhiMaxPer * 256) throw std::runtime_error("n2 too large for the wavemaker kernel");
     if (c->wmk2) {
@@ -2130,12 +2329,12 @@ struct Engine : EngineBase {
       std::vector<double> hdgz(n3);
       for (int i3 = 0; i3 < n3; ++i3) hdgz[i3] = dgzeddz(i3 * c->delz - 1.0, c->p.igeom, c->p.d_hinge);
       dgz = upload(hdgz);
-      if (c->p.dposdy_ini_re && c->p.dposdy_ini_im && nf) {
-        dyre = upload(std::vector<double>(c->p.dposdy_ini_re, c->p.dposdy_ini_re + (size_t)n2 * nf));
-        dyim = upload(std::vector<double>(c->p.dposdy_ini_im, c->p.dposdy_ini_im + (size_t)n2 * nf));
+      if (!c->h_dy_re.empty() && nf) {
+        dyre = upload(c->h_dy_re);
+        dyim = upload(c->h_dy_im);
       }
-      if (c->p.nu_x) nux = upload(std::vector<double>(c->p.nu_x, c->p.nu_x + ns));
-      if (c->p.nu_y) nuy = upload(std::vector<double>(c->p.nu_y, c->p.nu_y + ns));
+      if (!c->h_nux.empty()) nux = upload(c->h_nux);
+      if (!c->h_nuy.empty()) nuy = upload(c->h_nuy);
       Xdy = ar.get<double>((size_t)n3 * n2); Xdyt = ar.get<double>((size_t)n3 * n2);
       Xdz = ar.get<double>((size_t)n3 * n2); Xdzt = ar.get<double>((size_t)n3 * n2);
       stk6 = ar.get<double>(6 * na); stkz = ar.get<double>(6 * na); Pw = ar.get<double>(6 * na);
@@ -2169,12 +2368,12 @@ struct Engine : EngineBase {
     }
     // reference extended grid (common_vars.f90:38-39) for the exact mixed-parity filter
     const int n1 = c->n1;
-    const int Nd1 = ((c->p.p1 + 1) * n1) / 2, Nd2 = ((c->p.p2 + 1) * n2) / 2;
+    const int Nd1 = ((p1 + 1) * n1) / 2, Nd2 = ((p2 + 1) * n2) / 2;
     XA = ar.get<T>(nb); YA = ar.get<T>(nb);
     XB = YB = PB = RB = T2 = nullptr;
     if (M >= 2 && c->wmk) {
       if (M < 3) throw std::runtime_error("the wavemaker coupling needs mHOS >= 3 (or mHOS = 1)");
-      K1 = std::min(2 * n1, c->N1b);
+      K1 = std::min(2 * n1, N1b);
       PA = ar.get<T>((size_t)K1 * 2 * n2);
       // 2-D: k_t1_partial in FP64 at every size, and in FP32 from n1 = 2048 up (rocBLAS's
       // two-column products measured: DGEMM 2.3-2.7x slower at n1 = 4096; SGEMM 4 %
@@ -2197,7 +2396,7 @@ struct Engine : EngineBase {
         T1 = upload_t(sin_to_cos_matrix(Nd1, n1, K1));
       }
       if (n2 > 1) {
-        K2 = std::min(2 * n2, c->N2b);
+        K2 = std::min(2 * n2, N2b);
         XB = ar.get<T>(nb); YB = ar.get<T>(nb);
         PB = ar.get<T>((size_t)2 * n1 * K2);
         RB = ar.get<T>((size_t)2 * n1 * n2);
@@ -2301,7 +2500,7 @@ struct Engine : EngineBase {
            double* kadr, const S2* s2) override {
     hos_ctx& C = *c;
     hipStream_t st = C.st;
-    const int n1 = C.n1, n2 = C.n2, M = C.M, N1b = C.N1b, N2b = C.N2b;
+    const int n1 = C.n1, n2 = C.n2;
     const double pix = C.pix, piy = C.piy;
     const int ldb = N1b;
     C.stats.n_rhs++;
@@ -2371,6 +2570,7 @@ struct Engine : EngineBase {
           big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), N1b, N2b, Dst<T>{aHOS[iH + 1], ldb, 0});
       }
     }
+    if (C.feat_mode == 2 || (C.feat_mode == 1 && C.cur_stage == 0)) record_features();
     // phiz = filter_ext(sum_j phizm(j)) -> modal P (orders >= 2 on the mixed path)
     PtrSet<T> zz{};
     for (int j = 1; j <= M; ++j) zz.p[j] = phizm[j];
@@ -2438,12 +2638,33 @@ struct Engine : EngineBase {
     }
   }
 
+  // band energies of this evaluation's orders 1..M (k_band_energy) into the
+  // context's feature record
+  void record_features() {
+    hos_ctx& C = *c;
+    OrderFields<T> f{};
+    f.a[1] = phis_hat;
+    f.ld[1] = C.n1;
+    for (int j = 2; j <= M && j <= kFeatOrders; ++j) {
+      f.a[j] = aHOS[j];
+      f.ld[j] = (j == M) ? C.n1 : N1b;  // the last order is stored truncated
+    }
+    int slot = 0;
+    if (C.feat_mode == 2) slot = C.feat_n++ % C.feat_cap;
+    const int nord = std::min(M, kFeatOrders);
+    hipLaunchKernelGGL(k_band_energy<T>, dim3(kFeatGrid), dim3(256), 0, C.st, C.n1, C.n2, C.pix, C.piy, nord, f,
+                       (const T*)eta_hat, C.feat_part);
+    hipLaunchKernelGGL(k_band_reduce, dim3((kFeatRec + 255) / 256), dim3(256), 0, C.st, kFeatGrid, M,
+                       (const double*)C.feat_part, C.feat + (size_t)slot * kFeatRec);
+    HIP_OK(hipGetLastError());
+  }
+
   // filter_ext of the mixed-parity products on the reference grid: exact sine
   // spectra on the padded grid (added to the caller's batch), then the
   // reference grid's sine-to-cosine map (mixed_parity_finish).
   void mixed_parity_spectra(Batch2<T, Src<T, T>, Dst<T>>& b) {
     hos_ctx& C = *c;
-    const int n1 = C.n1, n2 = C.n2, N1b = C.N1b, N2b = C.N2b, ldb = N1b;
+    const int n1 = C.n1, n2 = C.n2, ldb = N1b;
     b.add(KSIN, KCOS, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0}, tmpX[b.n]);
     b.add(KSIN, KCOS, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0}, tmpX[b.n]);
     if (n2 > 1) {
@@ -2472,7 +2693,7 @@ struct Engine : EngineBase {
                  double* kad1, double* kadr, const S2* s2) {
     hos_ctx& C = *c;
     hipStream_t st = cs;
-    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a, N1b = C.N1b, N2b = C.N2b;
+    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a;
     const int ldb = N1b;
     if (!C.wmk) {
       HIP_OK(hipMemsetAsync(phi_add, 0, ns * sizeof(double), st));
@@ -2741,7 +2962,7 @@ struct Engine : EngineBase {
                               double* ke1, double* kad1, double* kadr, const S2& q) {
     hos_ctx& C = *c;
     hipStream_t st = cs;
-    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a, N1b = C.N1b, N2b = C.N2b;
+    const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a;
     const int ldb = N1b;
     double al, alt, altt;
     ramp_eval(C.p, t, al, alt, altt);
@@ -2880,7 +3101,6 @@ struct Engine : EngineBase {
 
   void time_big(int reps, double* out4) override {
     hos_ctx& C = *c;
-    const int N1b = C.N1b, N2b = C.N2b;
     hipEvent_t e0, e1, e2;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -2947,6 +3167,26 @@ struct Engine : EngineBase {
 // ===========================================================================
 namespace {
 
+// Makes the order-m engine the active one (built on first use).  Orders below
+// mHOS stand for a context created with mHOS = m; the wavemaker coupling needs
+// m >= 3 (or m = 1).
+void select_order(hos_ctx* c, int m) {
+  if (m < 1 || m > c->p.mHOS) throw std::runtime_error("order outside 1..mHOS");
+  if (c->wmk && m == 2) throw std::runtime_error("the wavemaker coupling needs an order >= 3 (or 1)");
+  if (!c->engines[m]) {
+    if (c->p.precision == 0)
+      c->engines[m] = std::make_unique<Engine<double>>(c, m);
+    else
+      c->engines[m] = std::make_unique<Engine<float>>(c, m);
+  }
+  c->eng = c->engines[m].get();
+  c->M = m;
+  int q1, q2;
+  order_grid(c->p, m, q1, q2, c->N1b, c->N2b);
+  c->stats.big_n1 = c->N1b;
+  c->stats.big_n2 = c->N2b;
+}
+
 void check_params(const hos_params& p) {
   if (p.n1 < 4) throw std::runtime_error("n1 must be >= 4");
   if (p.n2 < 1 || p.n2 == 2 || p.n2 == 3) throw std::runtime_error("n2 must be 1 or >= 4");
@@ -2984,6 +3224,7 @@ void rhs_eval(hos_ctx* c, double t, double dt, const double* G1, const double* G
   if (c->wmk3 && !aad3) throw std::runtime_error("i_wmk = 3 needs the third-order paddle modes");
   EngineBase::S2 s2{a2, e2, aad2, c->ka2[s], c->ke2[s], c->kad2[s], c->wmk3 ? aad3 : nullptr,
                     c->wmk3 ? c->kad3[s] : nullptr};
+  c->cur_stage = s;
   c->eng->rhs(t, dt, G1, G2, a1, e1, aad, c->kG1[s], c->kG2[s], c->ka1[s], c->ke1[s], c->kad1[s], c->kadr[s],
               c->wmk2 ? &s2 : nullptr);
 }
@@ -3170,6 +3411,248 @@ int guard(F&& f) {
 }  // namespace
 
 // ===========================================================================
+// learned truncation order (hos_advance_learned)
+// ===========================================================================
+namespace {
+
+// the committed RK state as commit() swaps it (current pointers, lengths)
+std::vector<std::pair<double*, size_t>> state_fields(hos_ctx* c) {
+  const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+  std::vector<std::pair<double*, size_t>> f = {{c->eta_hat, ns},    {c->phis_hat, ns}, {c->a1st_add, na},
+                                               {c->a_add_ramp, na}, {c->a1st, ns},     {c->eta1st, ns},
+                                               {c->deta_hat, ns}};
+  if (c->wmk2) {
+    f.push_back({c->a2nd_add, na});
+    f.push_back({c->a2nd, ns});
+    f.push_back({c->eta2nd, ns});
+  }
+  if (c->wmk3) f.push_back({c->a3rd_add, na});
+  return f;
+}
+
+void learn_checkpoint(hos_ctx* c, double t, double h) {
+  hos_ctx::Learn& L = c->learn;
+  const auto f = state_fields(c);
+  for (size_t i = 0; i < f.size(); ++i)
+    HIP_OK(hipMemcpyAsync(L.ck[i], f[i].first, f[i].second * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  L.t_ck = t;
+  L.h_ck = h;
+  L.alpha_ck = c->last_alpha;
+  L.alphat_ck = c->last_alphat;
+  L.pos_ck = c->last_pos_top;
+  L.endramp_ck = c->endramp;
+}
+
+void learn_restore(hos_ctx* c, double& t, double& h) {
+  hos_ctx::Learn& L = c->learn;
+  const auto f = state_fields(c);
+  for (size_t i = 0; i < f.size(); ++i)
+    HIP_OK(hipMemcpyAsync(f[i].first, L.ck[i], f[i].second * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  t = L.t_ck;
+  h = L.h_ck;
+  c->last_alpha = L.alpha_ck;
+  c->last_alphat = L.alphat_ck;
+  c->last_pos_top = L.pos_ck;
+  c->endramp = L.endramp_ck;
+  c->have_trial = false;
+}
+
+void learn_prepare(hos_ctx* c, const hos_order_policy& pol) {
+  hos_ctx::Learn& L = c->learn;
+  if (L.ready) return;
+  HIP_OK(hipStreamSynchronize(c->st));
+  const int cur = c->M;
+  for (int m = std::max(pol.m_min, 1); m <= c->p.mHOS; ++m) {
+    if (c->wmk && m == 2) continue;
+    select_order(c, m);  // every engine the selection may use, allocated now
+  }
+  select_order(c, cur);
+  const size_t ns = (size_t)c->n1 * c->n2;
+  for (const auto& f : state_fields(c)) L.ck.push_back(c->ar.get<double>(f.second));
+  L.judge_eta = c->ar.get<double>(ns);
+  L.dpart = c->ar.get<double>(2 * 256);
+  L.dn = c->ar.get<double>(2);
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&L.rec_host), kFeatRec * sizeof(double), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&L.pred_host), 64 * 64 * sizeof(float), hipHostMallocDefault));
+  HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&L.dn_host), 2 * sizeof(double), hipHostMallocDefault));
+  hosmlp::workspace_alloc(L.ws, 64);
+  HIP_OK(hipDeviceSynchronize());  // set-up fills of new engines run on the null stream
+  L.m = 0;                          // the first attempt runs at mHOS
+  L.floor = pol.m_min;
+  L.since = 0;
+  L.ready = true;
+}
+
+// eta drift rate sqrt(E_j / E_eta) of order j in a feature record
+double rate_of(const double* rec, int j) {
+  const double et = rec[kFeatEta];
+  if (!(et > 0.0)) return 0.0;
+  double s = 0.0;
+  for (int b = 0; b < kFeatBands; ++b) s += rec[j * kFeatBands + b];
+  return std::sqrt(s / et);
+}
+
+// the rate of the order after the record's last one: the network's band
+// energies, or the geometric extrapolation r_m^2 / r_{m-1}
+double next_rate(const hos_ctx::Learn& L, bool have_net) {
+  const double* rec = L.rec_host;
+  const int mr = (int)rec[kFeatM];
+  if (have_net && mr >= 1 && mr <= kPredOrders) {
+    const float* y = L.pred_host + (size_t)(mr - 1) * 64;
+    double s = 0.0;
+    for (int b = 0; b < kFeatBands; ++b) s += std::pow(10.0, 5.0 * (double)y[b] - 15.0);
+    return std::sqrt(s);
+  }
+  const double rm = rate_of(rec, mr), rm1 = mr >= 2 ? rate_of(rec, mr - 1) : 0.0;
+  if (rm1 > 0.0) return rm * rm / rm1;
+  return rm > 0.0 ? std::numeric_limits<double>::infinity() : 0.0;
+}
+
+// the lowest order q >= max(m_min, floor) whose first omitted order's rate is
+// below safety * rate_tol (actual rates up to the record's order, the
+// estimate one beyond it; one order up when that is not enough)
+int choose_order(hos_ctx* c, const hos_order_policy& pol, bool have_net, double* rate_used) {
+  const hos_ctx::Learn& L = c->learn;
+  const double* rec = L.rec_host;
+  const int mr = (int)rec[kFeatM], Mx = c->p.mHOS;
+  const double thr = pol.safety * pol.rate_tol;
+  int q = std::max(std::max(pol.m_min, L.floor), 1);
+  double r = 0.0;
+  while (q < Mx) {
+    if (q + 1 <= mr)
+      r = rate_of(rec, q + 1);
+    else if (q + 1 == mr + 1)
+      r = next_rate(L, have_net);
+    else
+      break;
+    if (r <= thr) break;
+    ++q;
+  }
+  if (c->wmk && q == 2) q = 3;
+  q = std::min(q, Mx);
+  *rate_used = (q < Mx) ? r : 0.0;
+  return q;
+}
+
+// one attempt at order m, its first right-hand side's band energies and (with
+// a network) the predictions, all copied to pinned host memory on the stream
+void learn_enqueue(hos_ctx* c, const hos_mlp* net, double t, double hl, int m) {
+  hos_ctx::Learn& L = c->learn;
+  select_order(c, m);
+  L.m_attempt = m;
+  c->feat_mode = 1;
+  rk_enqueue(c, t, hl);
+  hipStream_t st = c->st;
+  if (net) {
+    hipLaunchKernelGGL(k_feat_rows, dim3(64 * kRowIn / 256), dim3(256), 0, st, (const double*)c->feat, m, L.ws.x);
+    hosmlp::forward(net, 64, L.ws, st);
+    HIP_OK(hipMemcpyAsync(L.pred_host, L.ws.y, 64 * 64 * sizeof(float), hipMemcpyDeviceToHost, st));
+  }
+  HIP_OK(hipMemcpyAsync(L.rec_host, c->feat, kFeatRec * sizeof(double), hipMemcpyDeviceToHost, st));
+  L.stats.attempts[m]++;
+}
+
+struct LearnMember {
+  hos_ctx* c;
+  Advance a;
+  double dt_stab, delt_out;
+  bool final_attempt = false;
+  double rate_sel = 0.0;  // the rate estimate behind the order of the attempt in flight
+};
+
+void learn_launch(LearnMember& mb, double t_next, const hos_mlp* net, const hos_order_policy& pol) {
+  hos_ctx* c = mb.c;
+  hos_ctx::Learn& L = c->learn;
+  const double hl = (t_next - mb.a.t < mb.a.h) ? t_next - mb.a.t : mb.a.h;
+  mb.final_attempt = !(mb.a.t + hl < t_next);
+  int m = (L.m == 0) ? c->p.mHOS : L.m;
+  m = std::max(m, std::max(pol.m_min, L.floor));
+  if (c->wmk && m == 2) m = 3;
+  m = std::min(m, c->p.mHOS);
+  L.hl = hl;
+  L.phase = 0;
+  learn_enqueue(c, net, mb.a.t, hl, m);
+}
+
+// Completes the member's attempt in flight; returns true when the member has
+// enqueued another attempt (a judge, or its next attempt before t_next).
+bool learn_finish(LearnMember& mb, double t_next, double toler, const hos_mlp* net, const hos_order_policy& pol) {
+  hos_ctx* c = mb.c;
+  hos_ctx::Learn& L = c->learn;
+  const int Mx = c->p.mHOS;
+  const size_t ns = (size_t)c->n1 * c->n2;
+  double err = 0.0;
+  rk_finish(c, &err);
+  const double hl = L.hl;
+  if (L.phase == 0) {
+    const int m = L.m_attempt;
+    double rate_next = 0.0;
+    const int m_next = choose_order(c, pol, net != nullptr, &rate_next);
+    const bool accepts = !(err > toler && mb.a.h > 0.001 * 0.95 * mb.dt_stab);
+    if (m < Mx && accepts && (L.since + 1 >= pol.judge_every || mb.final_attempt || m_next != m)) {
+      // the judge: the same attempt at mHOS from the same state
+      HIP_OK(hipMemcpyAsync(L.judge_eta, c->eta_hat_t, ns * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+      L.err_m = err;
+      L.phase = 1;
+      L.stats.judges++;
+      const int judged = m;
+      learn_enqueue(c, net, mb.a.t, hl, Mx);
+      L.m_attempt = judged;  // (the order under judgment)
+      return true;
+    }
+    const int na0 = mb.a.na;
+    control(c, mb.a, hl, err, toler, mb.dt_stab, mb.delt_out);
+    if (mb.a.na > na0) {
+      if (m < Mx) {
+        L.since++;
+      } else if (L.since == 0 && (mb.a.na % std::max(pol.judge_every, 1)) == 0) {
+        learn_checkpoint(c, mb.a.t, mb.a.h);  // full-order progress needs no judge
+      }
+    }
+    L.m = m_next;
+    mb.rate_sel = rate_next;
+  } else {
+    const int judged = L.m_attempt;
+    hipLaunchKernelGGL(k_diff_norm, dim3(256), dim3(256), 0, c->st, ns, (const double*)L.judge_eta,
+                       (const double*)c->eta_hat_t, L.dpart);
+    hipLaunchKernelGGL(k_diff_reduce, dim3(1), dim3(256), 0, c->st, 256, (const double*)L.dpart, L.dn);
+    HIP_OK(hipMemcpyAsync(L.dn_host, L.dn, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIP_OK(hipStreamSynchronize(c->st));
+    const double rate = L.dn_host[1] > 0.0 ? std::sqrt(L.dn_host[0] / L.dn_host[1]) / hl : 0.0;
+    if (rate > L.stats.max_judged_rate) {
+      L.stats.max_judged_rate = rate;
+      L.stats.max_pred_rate = mb.rate_sel;
+    }
+    if (rate > pol.rate_tol) {
+      // reversion: back to the last passed judge, above the judged order
+      learn_restore(c, mb.a.t, mb.a.h);
+      L.floor = std::min(Mx, judged + 1);
+      L.since = 0;
+      L.m = L.floor;
+      L.stats.reverts++;
+    } else {
+      const int na0 = mb.a.na;
+      control(c, mb.a, hl, err, toler, mb.dt_stab, mb.delt_out);  // the mHOS attempt's decision
+      if (mb.a.na > na0) {
+        L.since = 0;
+        learn_checkpoint(c, mb.a.t, mb.a.h);
+        L.floor = std::max(pol.m_min, L.floor - 1);
+      }
+      double rate_next = 0.0;
+      L.m = choose_order(c, pol, net != nullptr, &rate_next);  // from the full-order record
+      mb.rate_sel = rate_next;
+    }
+  }
+  if (mb.a.t < t_next) {
+    learn_launch(mb, t_next, net, pol);
+    return true;
+  }
+  return false;
+}
+
+}  // namespace
+
+// ===========================================================================
 // C-ABI
 // ===========================================================================
 namespace {
@@ -3211,6 +3694,10 @@ void rhs_higher(hos_ctx* c, double t, double delta_t, const double* G1, const do
 }
 }  // namespace
 
+namespace hosnwt_err {
+void set(const char* msg) { g_last_error = msg; }
+}  // namespace hosnwt_err
+
 extern "C" {
 
 const char* hos_last_error(void) { return g_last_error.c_str(); }
@@ -3231,10 +3718,10 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     c->n3 = p.n3;
     c->n3a = L_ADD * (p.n3 - 1);
     c->M = p.mHOS;
-    const int Nd1 = ((p.p1 + 1) * p.n1) / 2;
-    const int Nd2 = p.n2 > 1 ? ((p.p2 + 1) * p.n2) / 2 : 1;
-    c->N1b = pad_smooth(std::max(Nd1, 2));
-    c->N2b = p.n2 > 1 ? pad_smooth(Nd2) : 1;
+    {
+      int q1, q2;
+      order_grid(p, p.mHOS, q1, q2, c->N1b, c->N2b);
+    }
     c->pix = M_PI / p.xlen;
     c->piy = p.n2 > 1 ? M_PI / p.ylen : 0.0;
     c->dx = p.xlen / (p.n1 - 1);
@@ -3286,6 +3773,9 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&c->red_host), 16 * sizeof(double), hipHostMallocDefault));
     c->phys_tmp = ar.get<double>(ns);
     c->phi_add_last = ar.get<double>(ns);
+    c->feat_cap = 512;
+    c->feat = ar.get<double>((size_t)c->feat_cap * kFeatRec);
+    c->feat_part = ar.get<double>((size_t)kFeatGrid * kFeatRec);
     c->nu_d = nullptr;
     if (p.nu) {
       c->nu_d = ar.get<double>(ns);
@@ -3310,10 +3800,20 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
       c->pim0[f] = p.pos_ini_im[(size_t)p.n2 * f];
     }
     c->gz_top = gzed(0.0, p.igeom, p.d_hinge);
-    if (p.precision == 0)
-      c->eng = std::make_unique<Engine<double>>(c.get());
-    else
-      c->eng = std::make_unique<Engine<float>>(c.get());
+    {
+      const size_t nfy = (size_t)p.n2 * p.n_freq;
+      if (p.n_freq) {
+        c->h_pos_re.assign(p.pos_ini_re, p.pos_ini_re + nfy);
+        c->h_pos_im.assign(p.pos_ini_im, p.pos_ini_im + nfy);
+        if (p.dposdy_ini_re && p.dposdy_ini_im) {
+          c->h_dy_re.assign(p.dposdy_ini_re, p.dposdy_ini_re + nfy);
+          c->h_dy_im.assign(p.dposdy_ini_im, p.dposdy_ini_im + nfy);
+        }
+      }
+      if (p.nu_x) c->h_nux.assign(p.nu_x, p.nu_x + ns);
+      if (p.nu_y) c->h_nuy.assign(p.nu_y, p.nu_y + ns);
+    }
+    select_order(c.get(), p.mHOS);
     c->p.freq = nullptr;
     c->p.pos_ini_re = c->p.pos_ini_im = nullptr;
     c->p.dposdy_ini_re = c->p.dposdy_ini_im = c->p.nu_x = c->p.nu_y = nullptr;
@@ -3326,7 +3826,8 @@ int hos_destroy(hos_ctx* c) {
   return guard([&] {
     if (!c) return;
     (void)hipStreamSynchronize(c->st);
-    c->eng.reset();
+    for (auto& e : c->engines) e.reset();
+    c->eng = nullptr;
     c->gs64.release();
     hipStream_t st = c->st;
     delete c;
@@ -3362,8 +3863,8 @@ int hos_initialize(hos_ctx* c) {
       return;
     }
     // a1st_add from -alpha X_t - alpha' X (initial_condition.f90:372-392)
-    Engine<double>* e64 = dynamic_cast<Engine<double>*>(c->eng.get());
-    Engine<float>* e32 = dynamic_cast<Engine<float>*>(c->eng.get());
+    Engine<double>* e64 = dynamic_cast<Engine<double>*>(c->eng);
+    Engine<float>* e32 = dynamic_cast<Engine<float>*>(c->eng);
     double al, alt, altt;
     ramp_eval(c->p, 0.0, al, alt, altt);
     auto run = [&](auto* e) {
@@ -3585,6 +4086,120 @@ int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_nex
   });
 }
 
+int hos_learned_last(hos_ctx* c, double* rec, float* pred) {
+  return guard([&] {
+    if (!c) throw std::runtime_error("null context");
+    const hos_ctx::Learn& L = c->learn;
+    if (!L.ready) throw std::runtime_error("no learned-order attempt on this context yet");
+    HIP_OK(hipStreamSynchronize(c->st));
+    if (rec) std::memcpy(rec, L.rec_host, kFeatRec * sizeof(double));
+    if (pred) std::memcpy(pred, L.pred_host, 64 * 64 * sizeof(float));
+  });
+}
+
+int hos_learned_reset(hos_ctx* c) {
+  return guard([&] {
+    if (!c) throw std::runtime_error("null context");
+    hos_ctx::Learn& L = c->learn;
+    L.m = 0;
+    L.floor = 0;
+    L.since = 0;
+    L.phase = 0;
+    L.stats = hos_learned_stats{};
+  });
+}
+
+int hos_advance_learned(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
+                        const double* dt_stab, const double* delt_out, const hos_mlp* net,
+                        const hos_order_policy* policy, int* n_acc, int* n_rej, hos_learned_stats* stats) {
+  return guard([&] {
+    if (n < 0 || !policy) throw std::runtime_error("bad arguments");
+    const hos_order_policy pol = *policy;
+    if (pol.judge_every < 1 || !(pol.rate_tol > 0.0) || !(pol.safety > 0.0))
+      throw std::runtime_error("policy: judge_every >= 1, rate_tol > 0, safety > 0");
+    std::vector<LearnMember> mb(n);
+    for (int i = 0; i < n; ++i) {
+      if (!ctx[i]) throw std::runtime_error("null context in the ensemble");
+      for (int j = 0; j < i; ++j)
+        if (ctx[j] == ctx[i]) throw std::runtime_error("a context appears twice in the ensemble");
+      if (pol.m_min < 1 || pol.m_min > ctx[i]->p.mHOS) throw std::runtime_error("policy: m_min outside 1..mHOS");
+      mb[i] = LearnMember{ctx[i], Advance{t_cur[i], h_rk[i], 0, 0}, dt_stab[i], delt_out[i]};
+    }
+    // set-up (engines of every order, checkpoint buffers) before any thread starts
+    for (int i = 0; i < n; ++i) {
+      hos_ctx* c = ctx[i];
+      const bool first = !c->learn.ready;
+      learn_prepare(c, pol);
+      if (first) c->learn.floor = pol.m_min;
+      learn_checkpoint(c, t_cur[i], h_rk[i]);  // no reversion reaches back beyond this call
+      c->learn.since = 0;
+      HIP_OK(hipStreamSynchronize(c->st));
+    }
+    int nth = 4;
+    if (const char* e = std::getenv("HOSNWT_ENSEMBLE_THREADS")) nth = std::max(1, std::atoi(e));
+    nth = std::max(1, std::min(nth, n));
+    std::vector<int> order(n);
+    std::vector<double> cost(n);
+    for (int i = 0; i < n; ++i) {
+      order[i] = i;
+      cost[i] = (double)ctx[i]->N1b * ctx[i]->N2b * ctx[i]->p.mHOS * std::max(0.0, t_next - t_cur[i]) / dt_stab[i];
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<std::vector<int>> part(nth);
+    std::vector<double> load(nth, 0.0);
+    for (int i : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      part[k].push_back(i);
+      load[k] += cost[i];
+    }
+    std::mutex em;
+    std::string first_error;
+    std::atomic<bool> failed{false};
+    auto run = [&](const std::vector<int>& mine) {
+      try {
+        if (!mine.empty()) HIP_OK(hipSetDevice(ctx[mine[0]]->device));
+        std::vector<char> busy(n, 0);
+        for (int i : mine)
+          if (mb[i].a.t < t_next) {
+            learn_launch(mb[i], t_next, net, pol);
+            busy[i] = 1;
+          }
+        for (bool any = true; any && !failed.load();) {
+          any = false;
+          for (int i : mine) {
+            if (!busy[i]) continue;
+            busy[i] = learn_finish(mb[i], t_next, toler, net, pol) ? 1 : 0;
+            any = any || busy[i];
+          }
+        }
+        for (int i : mine) {
+          ctx[i]->feat_mode = 0;
+          HIP_OK(hipStreamSynchronize(ctx[i]->st));
+        }
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(em);
+        if (first_error.empty()) first_error = e.what();
+        failed = true;
+      }
+    };
+    if (nth == 1) {
+      run(part[0]);
+    } else {
+      std::vector<std::thread> th;
+      for (int k = 0; k < nth; ++k) th.emplace_back(run, std::cref(part[k]));
+      for (auto& t : th) t.join();
+    }
+    for (int i = 0; i < n; ++i) {
+      t_cur[i] = mb[i].a.t;
+      h_rk[i] = mb[i].a.h;
+      if (n_acc) n_acc[i] = mb[i].a.na;
+      if (n_rej) n_rej[i] = mb[i].a.nr;
+      if (stats) stats[i] = ctx[i]->learn.stats;
+    }
+    if (!first_error.empty()) throw std::runtime_error(first_error);
+  });
+}
+
 int hos_rhs(hos_ctx* c, double t, double delta_t, const double* G1, const double* G2, const double* a1st,
             const double* eta1st, const double* a_add_1, double* dG1, double* dG2, double* da1st, double* deta1st,
             double* da_add_1, double* da_add_ramp) {
@@ -3750,6 +4365,51 @@ int hos_time_big_transform(hos_ctx* c, int reps, double* out4) {
 
 int hos_get_stats(hos_ctx* c, hos_stats* s) {
   return guard([&] { *s = c->stats; });
+}
+
+int hos_set_order(hos_ctx* c, int m) {
+  return guard([&] {
+    if (!c) throw std::runtime_error("null context");
+    HIP_OK(hipStreamSynchronize(c->st));
+    select_order(c, m);
+    HIP_OK(hipDeviceSynchronize());  // a new engine's set-up fills run on the null stream
+  });
+}
+
+int hos_get_order(hos_ctx* c, int* m) {
+  return guard([&] {
+    if (!c || !m) throw std::runtime_error("null argument");
+    *m = c->M;
+  });
+}
+
+int hos_record_features(hos_ctx* c, int mode) {
+  return guard([&] {
+    if (!c) throw std::runtime_error("null context");
+    if (mode < 0 || mode > 2) throw std::runtime_error("feature mode must be 0, 1 or 2");
+    HIP_OK(hipStreamSynchronize(c->st));
+    c->feat_mode = mode;
+    c->feat_n = 0;
+  });
+}
+
+int hos_take_features(hos_ctx* c, int max_rec, double* out, int* n_out) {
+  return guard([&] {
+    if (!c || !n_out) throw std::runtime_error("null argument");
+    HIP_OK(hipStreamSynchronize(c->st));
+    const int avail = (c->feat_mode == 2) ? std::min(c->feat_n, c->feat_cap) : 1;
+    const int n = std::max(0, std::min(avail, max_rec));
+    // oldest first: ring slots (feat_n - avail .. feat_n - 1) mod cap
+    const int first = (c->feat_mode == 2) ? c->feat_n - avail : 0;
+    std::vector<double> ring((size_t)c->feat_cap * kFeatRec);
+    HIP_OK(hipMemcpy(ring.data(), c->feat, ring.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int r = 0; r < n; ++r) {
+      const int slot = (c->feat_mode == 2) ? (first + r) % c->feat_cap : 0;
+      std::memcpy(out + (size_t)r * kFeatRec, ring.data() + (size_t)slot * kFeatRec, kFeatRec * sizeof(double));
+    }
+    if (c->feat_mode == 2) c->feat_n = 0;
+    *n_out = n;
+  });
 }
 
 int hos_synchronize(hos_ctx* c) {

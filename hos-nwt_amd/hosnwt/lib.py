@@ -38,6 +38,17 @@ class HosStats(C.Structure):
     ]
 
 
+class HosOrderPolicy(C.Structure):
+    _fields_ = [("m_min", C.c_int), ("judge_every", C.c_int), ("rate_tol", C.c_double), ("safety", C.c_double)]
+
+
+class HosLearnedStats(C.Structure):
+    _fields_ = [("attempts", C.c_longlong * 9), ("judges", C.c_longlong), ("reverts", C.c_longlong),
+                ("max_judged_rate", C.c_double), ("max_pred_rate", C.c_double)]
+
+
+FEAT_REC = 578  # HOS_FEAT_REC
+
 EXPORTS = [
     "hos_create", "hos_destroy", "hos_last_error", "hos_initialize", "hos_set_state",
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
@@ -45,6 +56,10 @@ EXPORTS = [
     "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_check_slope",
     "hos_set_state2", "hos_get_state2", "hos_rhs2", "hos_set_state3", "hos_get_state3", "hos_rhs3",
     "hos_get_wmk_motion", "hos_get_modes", "hos_probes", "hos_advance_ensemble",
+    "hos_set_order", "hos_get_order", "hos_record_features", "hos_take_features",
+    "hos_mlp_info", "hos_mlp_create", "hos_mlp_destroy", "hos_mlp_set_params", "hos_mlp_get_params",
+    "hos_mlp_forward", "hos_mlp_loss_grad", "hos_mlp_set_data", "hos_mlp_train",
+    "hos_advance_learned", "hos_learned_last", "hos_learned_reset",
 ]
 
 _lib = None
@@ -89,6 +104,24 @@ def load() -> C.CDLL:
     lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
     lib.hos_synchronize.argtypes = [vp]
     lib.hos_time_big_transform.argtypes = [vp, C.c_int, _dp]
+    ip, fp = C.POINTER(C.c_int), C.POINTER(C.c_float)
+    lib.hos_set_order.argtypes = [vp, C.c_int]
+    lib.hos_get_order.argtypes = [vp, ip]
+    lib.hos_record_features.argtypes = [vp, C.c_int]
+    lib.hos_take_features.argtypes = [vp, C.c_int, _dp, ip]
+    lib.hos_mlp_info.argtypes = [ip, ip, ip, C.POINTER(C.c_longlong)]
+    lib.hos_mlp_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.hos_mlp_destroy.argtypes = [vp]
+    lib.hos_mlp_set_params.argtypes = [vp, fp]
+    lib.hos_mlp_get_params.argtypes = [vp, fp]
+    lib.hos_mlp_forward.argtypes = [vp, C.c_int, fp, fp]
+    lib.hos_mlp_loss_grad.argtypes = [vp, C.c_int, fp, fp, _dp, fp]
+    lib.hos_mlp_set_data.argtypes = [vp, C.c_int, fp, fp]
+    lib.hos_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp, _dp]
+    lib.hos_advance_learned.argtypes = [C.c_int, C.POINTER(vp), _dp, C.c_double, _dp, C.c_double, _dp, _dp, vp,
+                                        C.POINTER(HosOrderPolicy), ip, ip, C.POINTER(HosLearnedStats)]
+    lib.hos_learned_last.argtypes = [vp, _dp, fp]
+    lib.hos_learned_reset.argtypes = [vp]
     for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
         getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
     for f in EXPORTS:
@@ -108,6 +141,14 @@ def ptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.float64 and (a.flags.f_contiguous or a.flags.c_contiguous)
     return a.ctypes.data_as(_dp)
+
+
+def fptr(a: np.ndarray | None):
+    """float32 pointer of a contiguous float32 array."""
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def fortran(a) -> np.ndarray:

@@ -202,6 +202,36 @@ class Tank:
         check(self.lib.hos_check_slope(self.h, threshold, coeff, C.byref(fx), C.byref(fy)))
         return bool(fx.value), bool(fy.value)
 
+    # ---- truncation order ------------------------------------------------
+    def set_order(self, m: int):
+        """Runs the following attempts at HOS order m <= mHOS (the fixed-M solver at M = m)."""
+        check(self.lib.hos_set_order(self.h, int(m)))
+
+    def order(self) -> int:
+        m = C.c_int()
+        check(self.lib.hos_get_order(self.h, C.byref(m)))
+        return m.value
+
+    def record_features(self, mode: int):
+        """0 off, 1 the first right-hand side of each attempt, 2 every right-hand side."""
+        check(self.lib.hos_record_features(self.h, int(mode)))
+
+    def take_features(self, max_rec: int = 512) -> np.ndarray:
+        """Band-energy records since the last take (oldest first), shape (n, FEAT_REC)."""
+        from .lib import FEAT_REC
+        out = np.zeros((max_rec, FEAT_REC))
+        n = C.c_int()
+        check(self.lib.hos_take_features(self.h, max_rec, ptr(out), C.byref(n)))
+        return out[: n.value].copy()
+
+    def learned_last(self):
+        """(record, predictions) of the member's last learned-order attempt."""
+        from .lib import FEAT_REC, fptr
+        rec = np.zeros(FEAT_REC)
+        pred = np.zeros((64, 64), dtype=np.float32)
+        check(self.lib.hos_learned_last(self.h, ptr(rec), fptr(pred)))
+        return rec, pred
+
     def stats(self):
         s = HosStats()
         check(self.lib.hos_get_stats(self.h, C.byref(s)))

@@ -203,6 +203,97 @@ int hos_synchronize(hos_ctx* c);
 int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
 int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
 
+/* ---------------------------------------------------------------------------
+ * Learned truncation order.  The reference fixes the HOS order M at compile
+ * time (the loop bound of HOSphis_modes, resol_HOS.f90:720-782, with the
+ * dealiasing grid of common_vars.f90:38-39).  A context can also run any order
+ * m <= mHOS on the grid a context created with mHOS = m would use: the modal RK
+ * state is shared, only the extended grid and the recursion depth change, so an
+ * attempt at order m equals the fixed-M solver's attempt at M = m, and mHOS
+ * stays the fallback and the judge.  No reference counterpart beyond that.
+ * ------------------------------------------------------------------------- */
+
+/* Active order (1 <= m <= mHOS; m >= 3 with a wavemaker, as at hos_create).
+ * The first use of an order allocates its engine. */
+int hos_set_order(hos_ctx* c, int m);
+int hos_get_order(hos_ctx* c, int* m);
+
+/* Band-energy records of the right-hand sides: mode 0 off, 1 the first
+ * right-hand side of each attempt, 2 every right-hand side (a ring of 512
+ * records, taken with hos_take_features).  A record has HOS_FEAT_REC doubles:
+ * [b] = sum of eta_hat^2 over the modes of radial band b (64 bands of |k| up
+ * to |k(n1-1, n2-1)|), [64 j + b] = sum of (k tanh k a_j)^2 with a_j the modal
+ * potential of order j (j = 1..8; phis_hat, then aHOS of resol_HOS.f90:720-782),
+ * [576] = the order evaluated, [577] = sum of eta_hat^2. */
+#define HOS_FEAT_REC 578
+int hos_record_features(hos_ctx* c, int mode);
+/* Copies up to max_rec records, oldest first, and empties the ring. */
+int hos_take_features(hos_ctx* c, int max_rec, double* out, int* n_out);
+
+/* The order-prediction network: 512 inputs (band energies of eta and of
+ * orders 1..m, the order), three hidden layers of 256 (ReLU), 64 outputs (the
+ * band energies of order m + 1).  BF16 operands on the gfx950 matrix cores
+ * with FP32 accumulation, FP32 master weights and Adam.  Parameters are FP32,
+ * per layer W (out x in, row-major) then b (out).  Row counts are padded to
+ * multiples of 64 internally; max_rows (a multiple of 64) bounds a batch. */
+typedef struct hos_mlp hos_mlp;
+int hos_mlp_info(int* n_in, int* width, int* n_out, long long* n_params);
+int hos_mlp_create(int max_rows, hos_mlp** out);
+int hos_mlp_destroy(hos_mlp* m);
+int hos_mlp_set_params(hos_mlp* m, const float* params);  /* also resets Adam */
+int hos_mlp_get_params(hos_mlp* m, float* params);
+int hos_mlp_forward(hos_mlp* m, int rows, const float* x, float* y);
+/* Mean squared error of one batch and its gradient (FP32, parameter layout). */
+int hos_mlp_loss_grad(hos_mlp* m, int rows, const float* x, const float* target, double* loss, float* grad);
+/* Training set (n rows of 512 inputs and 64 targets) to the device. */
+int hos_mlp_set_data(hos_mlp* m, int n, const float* x, const float* target);
+/* Adam (lr, beta 0.9 / 0.999) over the device set: `epochs` seeded random
+ * passes in batches of `batch` rows; epoch_loss[e] = the epoch's mean squared
+ * error, *ms = GPU time of all training steps. */
+int hos_mlp_train(hos_mlp* m, int epochs, int batch, float lr, unsigned long long seed, float* epoch_loss,
+                  double* ms);
+
+/* Order selection policy of hos_advance_learned. */
+typedef struct hos_order_policy {
+  int m_min;          /* lowest order chosen (>= 3 with a wavemaker) */
+  int judge_every;    /* K: a full-order judge after every K accepted attempts below mHOS */
+  double rate_tol;    /* allowed eta drift rate |eta_m - eta_M| / |eta_M| / h (per unit time) */
+  double safety;      /* an order is chosen when its first omitted order's rate is below safety * rate_tol */
+} hos_order_policy;
+
+/* Per-member counters of hos_advance_learned (accumulated over calls). */
+typedef struct hos_learned_stats {
+  long long attempts[9];    /* attempts by order, judge attempts included */
+  long long judges;         /* full-order judge attempts */
+  long long reverts;        /* windows rolled back to the checkpoint */
+  double max_judged_rate;   /* largest judged |eta_m - eta_M| / |eta_M| / h */
+  double max_pred_rate;     /* the selection's rate estimate at that judge */
+} hos_learned_stats;
+
+/* hos_advance_ensemble with the truncation order picked per attempt and per
+ * member: from the band energies of the attempt's first right-hand side, the
+ * order is the lowest m (>= max(m_min, floor)) whose first omitted order has an
+ * estimated eta drift rate sqrt(E_{m+1} / E_eta) below safety * rate_tol,
+ * actual energies for the orders evaluated and the network's prediction (net
+ * != NULL) or the geometric extrapolation E_m^2 / E_{m-1} (net == NULL) for the
+ * next one.  After judge_every accepted attempts below mHOS the next attempt
+ * also runs at mHOS from the same state and step (the judge): when its eta
+ * differs from the selected order's by more than rate_tol * h (relative L2 of
+ * the modes), the member returns to the state of its last passed judge and its
+ * order floor rises above the judged order; otherwise the mHOS result is the
+ * attempt's result and the state becomes the new checkpoint.  The accept /
+ * reject controller is that of HOS-NWT.f90:518-563.  stats may be NULL. */
+int hos_advance_learned(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
+                        const double* dt_stab, const double* delt_out, const hos_mlp* net,
+                        const hos_order_policy* policy, int* n_acc, int* n_rej, hos_learned_stats* stats);
+/* Measurement helper: the feature record of the member's last attempt and the
+ * network's predictions made from it (64 rows of 64: row r predicts order
+ * r + 2 from orders 1..r+1).  Either pointer may be NULL. */
+int hos_learned_last(hos_ctx* c, double* rec, float* pred);
+/* Resets a member's learned-selection state (order, floor, checkpoint at the
+ * current committed state, counters). */
+int hos_learned_reset(hos_ctx* c);
+
 #ifdef __cplusplus
 }
 #endif

@@ -845,6 +845,7 @@ class Oracle:
                 temp2 = self.dealias(j + q, 2.0 * phizm[j] * phizm[q])
                 temp1 = temp1 + temp2
         phiz2 = temp1
+        self.last_aHOS = aHOS  # modal potentials of orders 1..M on the extended grid
         return dict(phiz=phiz, W1=W1, phizMm1=phizMm1, geta2phiz=geta2phiz,
                     geta2phiz2=geta2phiz2, phiz2=phiz2)
 
@@ -1051,6 +1052,8 @@ class Oracle:
         NL1 = self.f2s(etark)
         etark = etark.copy()
         etark[0, 0] = self._volume_mode(NL1[0, 0])
+        self.last_eta_modes = etark.copy()  # modal eta of this evaluation (after the volume fix)
+        self.last_phis_modes = ph.copy()    # modal phis / omega-scaled potential of order 1
         etark = self.f2s(etark)
         phisrk = self.f2s(phisrk)
         etapm, etax, etay, gradeta2, phisx, phisy = self.phisxy_etaxy(phisrk, etark)
