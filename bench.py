@@ -166,6 +166,24 @@ def pass_probe(reps, precision=0):
     print("copied 64 MiB device-to-device 3 times", flush=True)
 
 
+def mlp_probe(epochs):
+    """Trains the order-prediction network on 32768 synthetic rows (a fixed random teacher
+    network's outputs) for `epochs` epochs: the matrix-core kernels under the profiler."""
+    from hosnwt import learned as hl
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((32768, 512)).astype(np.float32)
+    teacher = hl.OrderNet(max_rows=1024, seed=11)
+    t = np.concatenate([teacher.forward(x[i:i + 1024]) for i in range(0, len(x), 1024)])
+    teacher.close()
+    net = hl.OrderNet(max_rows=1024, seed=0)
+    for batch in (256, 1024):
+        losses, ms = net.train(x, t, epochs=epochs, batch=batch, lr=1e-3, seed=1)
+        flops = hl.OrderNet.train_flops(len(x), epochs)
+        print(json.dumps({"batch": batch, "epochs": epochs, "ms": round(ms, 2), "tflops": round(flops / ms / 1e9, 3),
+                          "loss": [float(losses[0]), float(losses[-1])]}), flush=True)
+    net.close()
+
+
 def cpu_baseline(wl=WORKLOAD):
     """FP64 oracle (numpy/scipy, one thread) on one solveHOS2 evaluation of the same workload."""
     from oracle import hos_oracle as ho
@@ -264,6 +282,11 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--pass-probe", type=int, default=0, metavar="REPS",
                     help="only time the roofline passes REPS times and run the counter-calibration copy")
+    ap.add_argument("--learned", type=float, default=0.0, metavar="T",
+                    help="only run the learned-order ensemble (16 x 512x128, M_max = 7) to time T against "
+                         "the fixed-order solver")
+    ap.add_argument("--mlp-probe", type=int, default=0, metavar="EPOCHS",
+                    help="only train the order-prediction network on synthetic rows (profiling)")
     ap.add_argument("--ensemble", type=float, default=0.0, metavar="T",
                     help="only run the ragged 2-D ensemble to nondimensional time T (sequential vs together)")
     args = ap.parse_args()
@@ -272,6 +295,16 @@ def main():
         return
     if args.ensemble:
         print(json.dumps(ensemble_bench(args.ensemble)), flush=True)
+        return
+    if args.learned:
+        from hosnwt import learned as hl
+        r = hl.experiment(n_members=16, n_train=4, t_end=args.learned,
+                          log=lambda m: print(m, file=sys.stderr, flush=True))
+        r["metric"] = "learned-order ensemble attempts/s (16 x 3D 512x128, M_max=7) vs fixed order"
+        print(json.dumps(r), flush=True)
+        return
+    if args.mlp_probe:
+        mlp_probe(args.mlp_probe)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported in this build "

@@ -2091,8 +2091,10 @@ struct hos_ctx {
   // predictions and judged difference norms
   struct Learn {
     bool ready = false;
-    int m = 0, floor = 0, since = 0, phase = 0, m_attempt = 0;
+    int m = 0, floor = 0, since = 0, phase = 0, m_attempt = 0, full_run = 0;
     double hl = 0.0, err_m = 0.0, pred_rate = 0.0;
+    // judged / estimated drift rate: the judge's calibration of the selection's estimate
+    double kappa = 1.0;
     double t_ck = 0.0, h_ck = 0.0, alpha_ck = 0.0, alphat_ck = 0.0, pos_ck = 0.0;
     int endramp_ck = 0;
     std::vector<double*> ck;
@@ -3516,6 +3518,10 @@ int choose_order(hos_ctx* c, const hos_order_policy& pol, bool have_net, double*
   const double* rec = L.rec_host;
   const int mr = (int)rec[kFeatM], Mx = c->p.mHOS;
   const double thr = pol.safety * pol.rate_tol;
+  if (!(rec[kFeatEta] > 0.0)) {  // a surface at rest tells nothing about the orders: full order
+    *rate_used = 0.0;
+    return Mx;
+  }
   int q = std::max(std::max(pol.m_min, L.floor), 1);
   double r = 0.0;
   while (q < Mx) {
@@ -3525,7 +3531,7 @@ int choose_order(hos_ctx* c, const hos_order_policy& pol, bool have_net, double*
       r = next_rate(L, have_net);
     else
       break;
-    if (r <= thr) break;
+    if (L.kappa * r <= thr) break;
     ++q;
   }
   if (c->wmk && q == 2) q = 3;
@@ -3605,8 +3611,12 @@ bool learn_finish(LearnMember& mb, double t_next, double toler, const hos_mlp* n
     if (mb.a.na > na0) {
       if (m < Mx) {
         L.since++;
-      } else if (L.since == 0 && (mb.a.na % std::max(pol.judge_every, 1)) == 0) {
-        learn_checkpoint(c, mb.a.t, mb.a.h);  // full-order progress needs no judge
+      } else if (L.since == 0 && ++L.full_run >= std::max(pol.judge_every, 1)) {
+        // judge_every accepted full-order attempts: progress that needs no judge, and the
+        // floor of an earlier reversion relaxes by one order
+        learn_checkpoint(c, mb.a.t, mb.a.h);
+        L.floor = std::max(pol.m_min, L.floor - 1);
+        L.full_run = 0;
       }
     }
     L.m = m_next;
@@ -3619,6 +3629,9 @@ bool learn_finish(LearnMember& mb, double t_next, double toler, const hos_mlp* n
     HIP_OK(hipMemcpyAsync(L.dn_host, L.dn, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
     HIP_OK(hipStreamSynchronize(c->st));
     const double rate = L.dn_host[1] > 0.0 ? std::sqrt(L.dn_host[0] / L.dn_host[1]) / hl : 0.0;
+    // calibration: up at once to the observed ratio, down by a fifth per judge
+    if (mb.rate_sel > 0.0 && std::isfinite(mb.rate_sel))
+      L.kappa = std::min(1e6, std::max(0.1, std::max(rate / mb.rate_sel, 0.8 * L.kappa)));
     if (rate > L.stats.max_judged_rate) {
       L.stats.max_judged_rate = rate;
       L.stats.max_pred_rate = mb.rate_sel;
@@ -4105,6 +4118,8 @@ int hos_learned_reset(hos_ctx* c) {
     L.floor = 0;
     L.since = 0;
     L.phase = 0;
+    L.full_run = 0;
+    L.kappa = 1.0;
     L.stats = hos_learned_stats{};
   });
 }

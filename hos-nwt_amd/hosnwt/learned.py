@@ -243,3 +243,167 @@ def timed(f, *a, **k):
     t0 = time.perf_counter()
     r = f(*a, **k)
     return r, time.perf_counter() - t0
+
+
+# ---------------------------------------------------------------------------
+# the learned-order ensemble workload and its measurement
+# ---------------------------------------------------------------------------
+def ensemble_members(n: int = 16, seed: int = 2025, n1: int = 512, n2: int = 128, M: int = 7,
+                     precision: int = 1, n_freq: int = 128, n_dir: int = 16, toler: float = 1e-7):
+    """3-D directional seas: 40 x 10 basin (h = 1 m), n1 x n2 modes, HOS order M, snake
+    wavemaker (one paddle per y node), JONSWAP Hs ~ U[0.04, 0.09] m, Tp ~ U[1.2, 1.8] s,
+    gamma 3.3, cos^2s spreading (s = 10) over n_freq x n_dir seeded components, ramp 2 Tp,
+    beach on the last 15 %."""
+    from . import config as hc
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        Hs, Tp = float(rng.uniform(0.04, 0.09)), float(rng.uniform(1.2, 1.8))
+        comps = hc.jonswap_components(Hs, Tp, 3.3, n_freq=n_freq, seed=int(rng.integers(1 << 30)), rnum=12,
+                                      clock=32.0, n_dir=n_dir, spread_s=10)
+        cfg = hc.TankConfig(n1=n1, n2=n2, mHOS=M, xlen=40.0, ylen=10.0, n3=33, icase=3, igeom=1, iramp=1,
+                            Tramp=2.0 * Tp, xabsf=0.85, T_stop=50 * Tp, toler=toler, f_out=1.0 / Tp,
+                            precision=precision, components=comps)
+        out.append(hc.build(cfg))
+    return out
+
+
+def _run_fixed(setups, outs, log):
+    """Fixed order (mHOS) through hos_advance_ensemble; eta at every output time, wall time."""
+    from .tank import Tank, advance_ensemble
+    tanks = [Tank(s) for s in setups]
+    try:
+        for tk in tanks:
+            tk.initialize()
+        h = [min(0.95 * s.dt_stab, s.delt_out) for s in setups]
+        etas, wall, att = [], 0.0, 0
+        for t_out in outs:
+            t0 = time.perf_counter()
+            h, na, nr = advance_ensemble(tanks, t_out, h)
+            wall += time.perf_counter() - t0
+            att += sum(na) + sum(nr)
+            etas.append([tk.physical()["eta"] for tk in tanks])
+            log(f"  fixed t={t_out:.2f}: {att} attempts, {wall:.2f} s")
+        return etas, att, wall
+    finally:
+        for tk in tanks:
+            tk.close()
+
+
+def _run_learned(setups, outs, net, pol, log):
+    from .tank import Tank
+    tanks = [Tank(s) for s in setups]
+    try:
+        for tk in tanks:
+            tk.initialize()
+        h = [min(0.95 * s.dt_stab, s.delt_out) for s in setups]
+        etas, wall, att, stats = [], 0.0, 0, None
+        for t_out in outs:
+            t0 = time.perf_counter()
+            h, na, nr, stats = advance_learned(tanks, t_out, h, net, pol)
+            wall += time.perf_counter() - t0
+            att += sum(na) + sum(nr)
+            etas.append([tk.physical()["eta"] for tk in tanks])
+            log(f"  learned t={t_out:.2f}: {att} attempts, {wall:.2f} s")
+        return etas, att, wall, stats
+    finally:
+        for tk in tanks:
+            tk.close()
+
+
+def experiment(n_members: int = 16, n_train: int = 4, t_end: float = 12.0, d_out: float = 2.0,
+               n1: int = 512, n2: int = 128, M: int = 7, epochs: int = 20, eta_tols=(1e-5, 1e-4),
+               judge_every: int = 20, safety: float = 0.5, target_samples: int | None = None,
+               log=print):
+    """The learned-order ensemble measured against the fixed-order solver.
+
+    1. Training data: n_train members (their own seeds) at the fixed order M (FP64), every
+       right-hand side's band energies recorded, to t_end.
+    2. The network is trained on the GPU (Adam, BF16 matrix cores).
+    3. n_members evaluation members to t_end with outputs every d_out: fixed order FP64 (the
+       reference) and mixed precision; then, for each eta tolerance, the learned order (mixed
+       precision) with the network and with the geometric extrapolation instead of it.
+    A tolerance's drift-rate budget is eta_tol / t_end.  Returns a dict of rates (attempts/s,
+    all members), eta errors against the FP64 reference at every output time, and counters."""
+    from .tank import Tank
+    outs = [d_out * k for k in range(1, int(round(t_end / d_out)) + 1)]
+    # 1. training data from the plain solver
+    train = ensemble_members(n_train, seed=99, n1=n1, n2=n2, M=M, precision=0)
+    tanks = [Tank(s) for s in train]
+    t0 = time.perf_counter()
+    try:
+        for tk in tanks:
+            tk.initialize()
+        recs, _ = collect(tanks, t_end, [min(0.95 * s.dt_stab, s.delt_out) for s in train])
+    finally:
+        for tk in tanks:
+            tk.close()
+    x, t = dataset(recs)
+    t_collect = time.perf_counter() - t0
+    log(f"[learned] {len(recs)} records -> {len(x)} samples in {t_collect:.1f} s")
+    if target_samples and len(x) > target_samples:
+        sel = np.random.default_rng(0).choice(len(x), target_samples, replace=False)
+        x, t = x[sel], t[sel]
+    # held-out tenth: the next order's rate, network against geometric extrapolation
+    rng = np.random.default_rng(0)
+    perm = rng.permutation(len(x))
+    n_hold = max(1, len(x) // 10)
+    hold, fit = perm[:n_hold], perm[n_hold:]
+    # 2. training on the GPU
+    net = OrderNet(max_rows=256, seed=0)
+    net.set_params(OrderNet.initial_params(0, out_bias=t[fit].mean(axis=0)))
+    losses, ms = net.train(x[fit], t[fit], epochs=epochs, batch=256, lr=1e-3, seed=1)
+    tflops = OrderNet.train_flops(len(fit), epochs) / (ms * 1e-3) / 1e12
+    pred = np.concatenate([net.forward(x[hold[i:i + 256]]) for i in range(0, n_hold, 256)])
+    true_rate = np.sqrt(unscale(t[hold]).sum(axis=1))
+    net_rate = np.sqrt(unscale(pred).sum(axis=1))
+    geo = []
+    for r in hold:
+        mp = int(np.argmax(x[r, 448:456]))
+        rm = np.sqrt(unscale(x[r, 64 * mp:64 * (mp + 1)]).sum())
+        rm1 = np.sqrt(unscale(x[r, 64 * (mp - 1):64 * mp]).sum()) if mp >= 2 else 0.0
+        geo.append(rm * rm / rm1 if rm1 > 0 else np.inf)
+    geo = np.array(geo)
+    lerr = lambda a: float(np.median(np.abs(np.log10(np.maximum(a, 1e-30) / np.maximum(true_rate, 1e-30)))))
+    log(f"[learned] trained {epochs} epochs on {len(fit)} samples: loss {losses[0]:.3e} -> {losses[-1]:.3e}, "
+        f"{ms:.0f} ms GPU, {tflops:.2f} TFLOP/s; held-out median |log10 rate error| network {lerr(net_rate):.3f}, "
+        f"geometric {lerr(geo):.3f}")
+    # 3. evaluation
+    evals = ensemble_members(n_members, seed=2025, n1=n1, n2=n2, M=M, precision=0)
+    evals32 = ensemble_members(n_members, seed=2025, n1=n1, n2=n2, M=M, precision=1)
+    eta64, att64, w64 = _run_fixed(evals, outs, log)
+    log(f"[learned] fixed M={M} FP64: {att64} attempts in {w64:.2f} s")
+    eta32, att32, w32 = _run_fixed(evals32, outs, log)
+    log(f"[learned] fixed M={M} mixed: {att32} attempts in {w32:.2f} s")
+
+    def errs(etas):
+        return [max(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)) for a, b in zip(e, r))
+                for e, r in zip(etas, eta64)]
+
+    def orders(stats):
+        tot = np.sum([s["attempts"] for s in stats], axis=0)
+        return {int(m): int(tot[m]) for m in range(9) if tot[m]}
+
+    res = dict(
+        members=n_members, grid=[n1, n2], M=M, t_end=t_end, outputs=outs,
+        samples=int(len(fit)), held_out=int(n_hold), records=int(len(recs)), epochs=epochs,
+        train_ms=round(ms, 1), train_tflops=round(tflops, 3), loss_first=float(losses[0]),
+        loss_last=float(losses[-1]), rate_log10_err_network=lerr(net_rate), rate_log10_err_geometric=lerr(geo),
+        fixed64=dict(attempts=att64, seconds=round(w64, 3), rate=round(att64 / w64, 2)),
+        fixed32=dict(attempts=att32, seconds=round(w32, 3), rate=round(att32 / w32, 2), eta_err=errs(eta32)),
+        learned={}, judge_every=judge_every, safety=safety)
+    for tol in eta_tols:
+        pol = policy(m_min=3, judge_every=judge_every, rate_tol=tol / t_end, safety=safety)
+        out = {}
+        for name, use in (("network", net), ("geometric", None)):
+            etaL, attL, wL, stL = _run_learned(evals32, outs, use, pol, log)
+            out[name] = dict(attempts=attL, seconds=round(wL, 3), rate=round(attL / wL, 2), eta_err=errs(etaL),
+                             orders=orders(stL), judges=int(sum(s["judges"] for s in stL)),
+                             reverts=int(sum(s["reverts"] for s in stL)),
+                             gain_vs_fixed64=round(w64 / wL, 3), gain_vs_fixed32=round(w32 / wL, 3))
+            log(f"[learned] eta tol {tol:g}, {name}: {attL} attempts in {wL:.2f} s, orders {out[name]['orders']}, "
+                f"reverts {out[name]['reverts']}, max eta err {max(out[name]['eta_err']):.2e}")
+        out["network_vs_geometric"] = round(out["geometric"]["seconds"] / out["network"]["seconds"], 3)
+        res["learned"][f"{tol:g}"] = out
+    net.close()
+    return res

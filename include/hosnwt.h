@@ -276,11 +276,16 @@ typedef struct hos_learned_stats {
  * estimated eta drift rate sqrt(E_{m+1} / E_eta) below safety * rate_tol,
  * actual energies for the orders evaluated and the network's prediction (net
  * != NULL) or the geometric extrapolation E_m^2 / E_{m-1} (net == NULL) for the
- * next one.  After judge_every accepted attempts below mHOS the next attempt
+ * next one, each scaled by the member's calibration factor (the ratio of the
+ * drift rate its judges measured to the estimate behind the judged order:
+ * raised to each new ratio at once, relaxed by a fifth per judge).  After
+ * judge_every accepted attempts below mHOS, at an interval's last attempt and
+ * before the order changes, the attempt
  * also runs at mHOS from the same state and step (the judge): when its eta
  * differs from the selected order's by more than rate_tol * h (relative L2 of
  * the modes), the member returns to the state of its last passed judge and its
- * order floor rises above the judged order; otherwise the mHOS result is the
+ * order floor rises above the judged order (it relaxes by one order after each
+ * passed judge and after judge_every accepted attempts at mHOS); otherwise the mHOS result is the
  * attempt's result and the state becomes the new checkpoint.  The accept /
  * reject controller is that of HOS-NWT.f90:518-563.  stats may be NULL. */
 int hos_advance_learned(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
