@@ -151,9 +151,10 @@ def _tank_2d(M=6, precision=0):
 
 
 def test_learned_reversion_reaches_the_fixed_order_run():
-    """rate_tol tiny: the selection picks m_min (safety huge), every judge fails, each reversion
-    raises the order floor by one, and at the floor mHOS nothing more is judged: the run ends
-    bit-identical to the fixed-order run from the call's start (its checkpoint)."""
+    """rate_tol tiny: the selection picks the lowest order its floor allows (safety huge), every
+    judge fails and every window below mHOS is rolled back to the checkpoint, which only
+    full-order progress moves: the run ends bit-identical to the fixed-order run, with the same
+    step proposal.  The first failures raise the floor 3 -> 4 -> 5 -> 6 one order at a time."""
     s, ref = _tank_2d()
     _, tk = _tank_2d()
     try:
@@ -164,7 +165,7 @@ def test_learned_reversion_reaches_the_fixed_order_run():
         h, na, nr, st = hl.advance_learned([tk], t1, [h0], None, pol)
         print(st[0], flush=True)
         assert tk.time == ref.time
-        assert st[0]["reverts"] == 3 and st[0]["judges"] == 3
+        assert st[0]["reverts"] == st[0]["judges"] >= 3
         assert st[0]["attempts"][3] > 0 and st[0]["attempts"][4] > 0 and st[0]["attempts"][5] > 0
         a, b = tk.get_state(), ref.get_state()
         for k in a:
