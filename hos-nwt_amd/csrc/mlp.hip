@@ -20,7 +20,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -178,6 +180,40 @@ __global__ void k_to_bf16(size_t n, const float* __restrict__ x, __bf16* __restr
 }
 
 inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---------------------------------------------------------------------------
+// allocation-free inference (hos_mlp_forward_noalloc): the activations of up to
+// kNoAllocRows rows live in one static device array per GPU
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocRows = 256;
+constexpr size_t kScratchX = (size_t)kNoAllocRows * kIn * sizeof(__bf16);      // input rows (bf16)
+constexpr size_t kScratchH = (size_t)kNoAllocRows * kWidth * sizeof(__bf16);   // one hidden layer
+constexpr size_t kScratchY = (size_t)kNoAllocRows * kOut * sizeof(float);      // padded outputs
+constexpr size_t kScratchBytes = kScratchX + 3 * kScratchH + kScratchY;      // 704 KiB
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of `scratch` on the current device (hipGetSymbolAddress once per device;
+// in host code the array's name is not a device address).
+unsigned char* scratch_ptr() {
+  static std::mutex mu;
+  static std::map<int, unsigned char*> per_dev;
+  int dev = 0;
+  MLP_OK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = per_dev.find(dev);
+  if (it != per_dev.end()) return it->second;
+  void* p = nullptr;
+  MLP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  per_dev[dev] = reinterpret_cast<unsigned char*>(p);
+  return per_dev[dev];
+}
+
+// x (rows x kIn fp32) -> padded bf16 rows (zeros up to prows)
+__global__ void k_rows_bf16(int rows, int prows, const float* __restrict__ x, __bf16* __restrict__ xb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= prows * kIn) return;
+  xb[i] = (i < rows * kIn) ? (__bf16)x[i] : (__bf16)0.f;
+}
 
 template <int EPI>
 void gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* Bt, int ldb,
@@ -469,6 +505,40 @@ int hos_mlp_loss_grad(hos_mlp* m, int rows, const float* x, const float* t, doub
     if (grad) MLP_OK(hipMemcpyAsync(grad, m->G, m->nparam * sizeof(float), hipMemcpyDeviceToHost, m->st));
     MLP_OK(hipStreamSynchronize(m->st));
     if (loss) *loss = (double)l / ((double)rows * kOut);
+  });
+}
+
+// No counterpart in the reference: hos_mlp_forward on device arrays, allocating
+// nothing (the activations use the static scratch array of this GPU).
+int hos_mlp_forward_noalloc(const hos_mlp* m, int rows, const float* x_dev, float* y_dev, void* stream) {
+  return guard([&] {
+    if (!m || !x_dev || !y_dev) throw std::runtime_error("null argument");
+    if (rows < 1 || rows > kNoAllocRows)
+      throw std::runtime_error("hos_mlp_forward_noalloc: rows must be in 1.." + std::to_string(kNoAllocRows));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    unsigned char* base = scratch_ptr();
+    const int prows = (rows + kRowTile - 1) / kRowTile * kRowTile;
+    __bf16* xb = reinterpret_cast<__bf16*>(base);
+    __bf16* h[3];
+    for (int l = 0; l < 3; ++l) h[l] = reinterpret_cast<__bf16*>(base + kScratchX + l * kScratchH);
+    float* yp = reinterpret_cast<float*>(base + kScratchX + 3 * kScratchH);
+    hipLaunchKernelGGL(k_rows_bf16, dim3(nblk((size_t)prows * kIn)), dim3(256), 0, st, rows, prows, x_dev, xb);
+    const __bf16* in = xb;
+    for (int l = 0; l < 3; ++l) {
+      EpiArgs e;
+      e.bias = m->P + m->off_b[l];
+      e.out = h[l];
+      e.ldo = kWidth;
+      gemm<EPI_RELU>(st, prows, kWidth, kDims[l], in, kDims[l], m->Wt[l], kDims[l], e);
+      in = h[l];
+    }
+    EpiArgs e;
+    e.bias = m->P + m->off_b[3];
+    e.outf = yp;
+    e.ldf = kOut;
+    gemm<EPI_LINEAR>(st, prows, kOut, kWidth, h[2], kWidth, m->Wt[3], kWidth, e);
+    MLP_OK(hipMemcpyAsync(y_dev, yp, (size_t)rows * kOut * sizeof(float), hipMemcpyDeviceToDevice, st));
+    MLP_OK(hipGetLastError());
   });
 }
 

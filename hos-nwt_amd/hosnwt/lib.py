@@ -59,7 +59,7 @@ EXPORTS = [
     "hos_set_order", "hos_get_order", "hos_record_features", "hos_take_features",
     "hos_mlp_info", "hos_mlp_create", "hos_mlp_destroy", "hos_mlp_set_params", "hos_mlp_get_params",
     "hos_mlp_forward", "hos_mlp_loss_grad", "hos_mlp_set_data", "hos_mlp_train",
-    "hos_advance_learned", "hos_learned_last", "hos_learned_reset",
+    "hos_advance_learned", "hos_learned_last", "hos_learned_reset", "hos_mlp_forward_noalloc",
 ]
 
 _lib = None
@@ -122,6 +122,7 @@ def load() -> C.CDLL:
                                         C.POINTER(HosOrderPolicy), ip, ip, C.POINTER(HosLearnedStats)]
     lib.hos_learned_last.argtypes = [vp, _dp, fp]
     lib.hos_learned_reset.argtypes = [vp]
+    lib.hos_mlp_forward_noalloc.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
     for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
         getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
     for f in EXPORTS:

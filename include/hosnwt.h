@@ -243,6 +243,15 @@ int hos_mlp_destroy(hos_mlp* m);
 int hos_mlp_set_params(hos_mlp* m, const float* params);  /* also resets Adam */
 int hos_mlp_get_params(hos_mlp* m, float* params);
 int hos_mlp_forward(hos_mlp* m, int rows, const float* x, float* y);
+/* No counterpart in the reference: hos_mlp_forward for callers that cannot
+ * allocate (code under HIP graph capture, serving loops).  x_dev (rows x 512)
+ * and y_dev (rows x 64) are FP32 device arrays, stream a hipStream_t (NULL: the
+ * null stream); nothing is allocated: the activations live in one static device
+ * array of 704 KiB per GPU, held until the process ends.  rows <= 256 (larger
+ * counts are an error).  Every call on a GPU shares that GPU's array, so calls
+ * that overlap on one GPU (other streams or threads) race; calls ordered on one
+ * stream do not.  Same results as hos_mlp_forward. */
+int hos_mlp_forward_noalloc(const hos_mlp* m, int rows, const float* x_dev, float* y_dev, void* stream);
 /* Mean squared error of one batch and its gradient (FP32, parameter layout). */
 int hos_mlp_loss_grad(hos_mlp* m, int rows, const float* x, const float* target, double* loss, float* grad);
 /* Training set (n rows of 512 inputs and 64 targets) to the device. */
