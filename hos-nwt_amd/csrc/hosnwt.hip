@@ -103,12 +103,92 @@ inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / 
 // ---------------------------------------------------------------------------
 // spectral-state kernels (double unless noted)
 // ---------------------------------------------------------------------------
+// Ramp alpha(t), alpha', alpha'' (resol_wmkr.f90:1268-1397); false for an unknown ramp type.
+__host__ __device__ inline bool ramp_hd(int iramp, double Tr, double Ts, double T_mono, double t, double& a,
+                                        double& at, double& att) {
+  if (iramp == 11) {
+    const double Tsw = 3.0 / sqrt(5.0 / 9.81);
+    if (t <= Tsw) {
+      a = t / Tsw; at = 1.0 / Tsw; att = 0.0;
+    } else {
+      a = 1.0; at = 0.0; att = 0.0;
+    }
+    return true;
+  }
+  if (iramp == 33) {
+    const double Tsw = -log(0.1) / (10.0 * T_mono);
+    a = 1.0 - exp(-Tsw * t);
+    at = (1.0 - a) * Tsw;
+    att = (1.0 - a) * Tsw * Tsw;
+    return true;
+  }
+  if (t <= Tr) {
+    switch (iramp) {
+      case 0: a = 1.0; at = 0.0; att = 0.0; return true;
+      case 1: a = t / Tr; at = 1.0 / Tr; att = 0.0; return true;
+      case 2: { const double s = t / Tr; a = s * s; at = 2.0 * s / Tr; att = 2.0 / (Tr * Tr); return true; }
+      case 21: {
+        const double T2 = Tr * Tr;
+        if (t <= Tr / 2.0) { a = 2.0 * t * t / T2; at = 4.0 * t / T2; att = 4.0 / T2; }
+        else { const double ts = t / Tr - 1.0; a = 1.0 - 2.0 * ts * ts; at = -4.0 * ts / Tr; att = -4.0 / T2; }
+        return true;
+      }
+      case 4: { const double s = t / Tr, s2 = s * s; a = s2 * s2; at = 4.0 * s2 * s / Tr; att = 12.0 * s2 / (Tr * Tr); return true; }
+      case 41: {
+        const double T2 = Tr * Tr, rap = 8.0 * t / Tr, rap2 = t * t / T2, rap22 = 3.0 * rap2;
+        a = (rap22 - rap + 6.0) * rap2;
+        at = (4.0 * rap22 - 3.0 * rap + 12.0) * t / T2;
+        att = (12.0 * rap22 - 6.0 * rap + 12.0) / T2;
+        return true;
+      }
+      default: return false;
+    }
+  }
+  if (t >= Ts + Tr) { a = at = att = 0.0; return true; }
+  if (t >= Ts) {
+    const double ti = Ts + Tr - t;
+    switch (iramp) {
+      case 1: a = ti / Tr; at = -1.0 / Tr; att = 0.0; return true;
+      case 2: { const double s = ti / Tr; a = s * s; at = -2.0 * s / Tr; att = 2.0 / (Tr * Tr); return true; }
+      case 21: {
+        const double T2 = Tr * Tr;
+        if (ti <= Tr / 2.0) { a = 2.0 * ti * ti / T2; at = -4.0 * ti / T2; att = 4.0 / T2; }
+        else { const double ts = ti / Tr - 1.0; a = 1.0 - 2.0 * ts * ts; at = 4.0 * ts / Tr; att = -4.0 / T2; }
+        return true;
+      }
+      case 4: { const double s = ti / Tr, s2 = s * s; a = s2 * s2; at = -4.0 * s2 * s / Tr; att = 12.0 * s2 / (Tr * Tr); return true; }
+      case 41: {
+        const double T2 = Tr * Tr, rap = 8.0 * ti / Tr, rap2 = ti * ti / T2, rap22 = 3.0 * rap2;
+        a = (rap22 - rap + 6.0) * rap2;
+        at = -(4.0 * rap22 - 3.0 * rap + 12.0) * ti / T2;
+        att = (12.0 * rap22 - 6.0 * rap + 12.0) / T2;
+        return true;
+      }
+      default: a = 1.0; at = 0.0; att = 0.0; return true;
+    }
+  }
+  a = 1.0; at = 0.0; att = 0.0;
+  return true;
+}
+
+// Device step scalars of the graph-captured attempt (hos_advance_graph): the
+// controller's state and, per RK stage s, the stage time, the integrating-factor
+// shift and the ramp and paddle values, so that a captured attempt reads its
+// time-dependent scalars from memory instead of kernel arguments.
+enum DevScal {
+  DS_T = 0, DS_H, DS_HL, DS_TNEXT, DS_TOLER, DS_STAB, DS_DOUT, DS_NACC, DS_NREJ, DS_ACTIVE, DS_ACCEPT,
+  DS_ENDRAMP, DS_ERR, DS_GALPHA, DS_GPOS, DS_GALPHAT, DS_SUB, DS_STAGE0 = 24, DS_COUNT = DS_STAGE0 + 40
+};
+enum StageScal { SS_T = 0, SS_DT, SS_A, SS_AT, SS_ATT, SS_POS, SS_STRIDE = 8 };
+__device__ __forceinline__ double stage_val(const double* dsc, int s, int k) { return dsc[DS_STAGE0 + SS_STRIDE * s + k]; }
+
 // GtoF + division by omega at the start of solveHOS2 (resol_HOS.f90:344-351);
 // also partial sums of eta_hat for the mean-volume fix (resol_HOS.f90:353-364).
 template <typename T>
 __global__ void k_gtof_in(int n1, int n2, double pix, double piy, double dt, const double* __restrict__ G1,
                           const double* __restrict__ G2, T* eta_hat, T* phis_hat, double* partial,
-                          double* phis64) {
+                          double* phis64, const double* dsc, int stage) {
+  if (dsc) dt = stage_val(dsc, stage, SS_DT);
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   double s = 0.0;
@@ -132,7 +212,11 @@ __global__ void k_gtof_in(int n1, int n2, double pix, double piy, double dt, con
 // runge_kutta.f90:306-312): a single block sums the partials.
 template <typename T>
 __global__ void k_volfix(int npart, const double* partial, T* eta_hat, double* eta_hat_d, int mode, double alpha,
-                         double pos_top, double xlen, double d_hinge) {
+                         double pos_top, double xlen, double d_hinge, const double* dsc) {
+  if (dsc) {
+    alpha = dsc[DS_GALPHA];
+    pos_top = dsc[DS_GPOS];
+  }
   double s = 0.0;
   for (int i = threadIdx.x; i < npart; i += blockDim.x) s += partial[i];
   s = block_sum(s);
@@ -380,7 +464,8 @@ __global__ void k_forcing(size_t n, const double* __restrict__ phiz_add, const d
 template <typename T, typename TL>
 __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, const T* D1, const T* P,
                             const T* phis_hat, const TL* phizh, const T* D2, const T* Zh, const double* Zh2,
-                            const double* filt, double* dG1, double* dG2) {
+                            const double* filt, double* dG1, double* dG2, const double* dsc, int stage) {
+  if (dsc) dt = stage_val(dsc, stage, SS_DT);
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -407,7 +492,8 @@ __global__ void k_expon_out(int n1, int n2, double pix, double piy, double dt, c
 // pos arrays are (n3, n2).
 __global__ void k_pos(int n2, int n3, int nf, double t, const double* __restrict__ freq,
                       const double* __restrict__ pre, const double* __restrict__ pim, const double* __restrict__ gz,
-                      double* X, double* Xt, double* Xtt) {
+                      double* X, double* Xt, double* Xtt, const double* dsc = nullptr, int stage = 0) {
+  if (dsc) t = stage_val(dsc, stage, SS_T);
   const int i2 = blockIdx.x;
   double s0 = 0, s1 = 0, s2 = 0;
   for (int f = threadIdx.x; f < nf; f += blockDim.x) {
@@ -442,9 +528,22 @@ __global__ void k_pos(int n2, int n3, int nf, double t, const double* __restrict
 struct Cont {
   double m[3][4];  // c_r = m[0][r] f + m[1][r] df + m[2][r] d2f
 };
+// dsc: the right-hand side's coefficients (alpha'', 2 alpha', alpha; alpha'', alpha', 0) of stage
+// `stage` from the device step scalars
 __global__ void k_paddle_rhs(int n2, int n3, int n3a, double delz, const double* X, const double* Xt,
                              const double* Xtt, double c1x, double c1t, double c1tt, double c2x, double c2t,
-                             double c2tt, Cont cm, double* f1, double* f2) {
+                             double c2tt, Cont cm, double* f1, double* f2, const double* dsc = nullptr,
+                             int stage = 0) {
+  if (dsc) {
+    const double al = stage_val(dsc, stage, SS_A), alt = stage_val(dsc, stage, SS_AT),
+                 altt = stage_val(dsc, stage, SS_ATT);
+    c1x = altt;
+    c1t = 2.0 * alt;
+    c1tt = al;
+    c2x = altt;
+    c2t = alt;
+    c2tt = 0.0;
+  }
   const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
   if (i2 >= n2) return;
   for (int which = 0; which < 2; ++which) {
@@ -1212,7 +1311,117 @@ struct AxpyBatch {
 };
 // (at most 9 fields: G1, G2, a1st_add, a1st, eta1st, the three second-order fields, a3rd_add)
 static_assert(kMaxAxpy >= 9, "AxpyBatch holds every RK state field");
-__global__ void k_axpyk_batch(double h, AxpyBatch b) {
+// Stage scalars of the attempt from the committed time and step proposal in
+// dsc (one block): the step hl = min(h, t_next - t), per stage the time
+// t + hl c_s (two roundings, as stage_time), the shift hl c_s, the ramp and the
+// paddle-top displacement g(0) sum_f Re(pos_ini(1,f) e^{i w t}).
+__global__ void __launch_bounds__(256) k_stage_scalars(double* dsc, int iramp, double Tr, double Ts, double T_mono,
+                                                       int nf, const double* __restrict__ freq,
+                                                       const double* __restrict__ pre0,
+                                                       const double* __restrict__ pim0, double gz_top) {
+  const double cst[5] = {0.0, 0.5, 0.5, 1.0, 1.0};
+  const double t = dsc[DS_T], h = dsc[DS_H], tn = dsc[DS_TNEXT];
+  const double hl = (tn - t < h) ? tn - t : h;
+  for (int st = 0; st < 5; ++st) {
+    const double dt = __dmul_rn(hl, cst[st]);
+    const double ts = __dadd_rn(t, dt);
+    double sum = 0.0;
+    for (int f = threadIdx.x; f < nf; f += blockDim.x) {
+      const double w = 2.0 * M_PI * freq[f];
+      double sn, cs;
+      sincos(w * ts, &sn, &cs);
+      sum += pre0[f] * cs - pim0[f] * sn;
+    }
+    sum = block_sum(sum);
+    if (threadIdx.x == 0) {
+      double a = 1.0, at = 0.0, att = 0.0;
+      ramp_hd(iramp, Tr, Ts, T_mono, ts, a, at, att);
+      double* o = dsc + DS_STAGE0 + SS_STRIDE * st;
+      o[SS_T] = ts;
+      o[SS_DT] = dt;
+      o[SS_A] = a;
+      o[SS_AT] = at;
+      o[SS_ATT] = att;
+      o[SS_POS] = gz_top * sum;
+    }
+  }
+  if (threadIdx.x == 0) {
+    dsc[DS_HL] = hl;
+    dsc[DS_ACTIVE] = (t < tn) ? 1.0 : 0.0;
+  }
+}
+
+// The accept / reject controller of the inner loop (HOS-NWT.f90:518-563) on the
+// device: the embedded error from the attempt's maxima (runge_kutta.f90:272-279),
+// the decision, the end-of-ramp flag, the new time and step proposal.
+__global__ void k_control(double* dsc, const double* __restrict__ red, int npair, int iramp, double Tramp) {
+  const double tiny = 2.220446049250313e-16;
+  if (!(dsc[DS_ACTIVE] > 0.0)) {
+    dsc[DS_ACCEPT] = 0.0;
+    dsc[DS_SUB] = 0.0;
+    return;
+  }
+  double e = 0.0;
+  for (int q = 0; q < npair; ++q) e = fmax(e, red[2 * q] / (red[2 * q + 1] + tiny));
+  const double toler = dsc[DS_TOLER], stab = dsc[DS_STAB], dout = dsc[DS_DOUT];
+  double t = dsc[DS_T], h = dsc[DS_H];
+  const double hl = dsc[DS_HL];
+  double corr;
+  int endramp = (int)dsc[DS_ENDRAMP];
+  if (e > toler && h > 0.001 * 0.95 * stab) {
+    corr = 0.9 * pow(e / toler, -(1.0 / 4.0));
+    dsc[DS_NREJ] += 1.0;
+    if (fabs(Tramp - (t + hl)) <= tiny && Tramp > t) endramp = 0;
+    dsc[DS_ACCEPT] = 0.0;
+    dsc[DS_SUB] = 0.0;
+  } else {
+    const bool sub = iramp == 1 && Tramp - (t + hl) <= tiny && Tramp > t && endramp != 1;
+    if (sub) endramp = 1;
+    dsc[DS_SUB] = sub ? 1.0 : 0.0;
+    dsc[DS_ACCEPT] = 1.0;
+    t = t + hl;
+    corr = (fabs(e) < tiny) ? 2.0 : 0.9 * pow(e / toler, -(1.0 / 4.0));
+    dsc[DS_NACC] += 1.0;
+  }
+  h = h * corr;
+  h = fmin(fmin(h, 0.95 * stab), dout);
+  h = fmax(fmax(h, 0.0001 * 0.95 * stab), 0.0001 * dout);
+  dsc[DS_T] = t;
+  dsc[DS_H] = h;
+  dsc[DS_ERR] = e;
+  dsc[DS_ENDRAMP] = (double)endramp;
+}
+
+// the accept branch's copy-back (HOS-NWT.f90:534-543) of every state field, when
+// the controller accepted (blockIdx.y = field)
+constexpr int kMaxCopy = 12;
+struct CopyBatch {
+  int n;
+  double* dst[kMaxCopy];
+  const double* src[kMaxCopy];
+  size_t len[kMaxCopy];
+};
+__global__ void k_commit_if(const double* dsc, CopyBatch b) {
+  if (!(dsc[DS_ACCEPT] > 0.0)) return;
+  const int f = blockIdx.y;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < b.len[f]) b.dst[f][i] = b.src[f][i];
+}
+// the end-of-ramp correction a1st_add -= a_add_ramp (HOS-NWT.f90:546-550) when flagged
+__global__ void k_sub_if(size_t n, double* a, const double* b, const double* dsc) {
+  if (!(dsc[DS_SUB] > 0.0)) return;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] -= b[i];
+}
+
+// wavemaker globals of the evaluation at stage `stage` (alpha, pos1st(n3,1), alpha')
+__global__ void k_wmk_glob(double* dsc, int stage) {
+  dsc[DS_GALPHA] = stage_val(dsc, stage, SS_A);
+  dsc[DS_GPOS] = stage_val(dsc, stage, SS_POS);
+  dsc[DS_GALPHAT] = stage_val(dsc, stage, SS_AT);
+}
+__global__ void k_axpyk_batch(double h, AxpyBatch b, const double* dsc = nullptr) {
+  if (dsc) h = dsc[DS_HL];
   const int f = blockIdx.y;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= b.len[f]) return;
@@ -1241,7 +1450,9 @@ struct FinalBatch {
 };
 // (G1, G2 and the first-, second- and third-order paddle modes)
 static_assert(kMaxFinal >= 5, "FinalBatch holds every error-carrying field");
-__global__ void k_final_err_batch(double h, FinalBatch b, double* pmax_err, double* pmax_x) {
+__global__ void k_final_err_batch(double h, FinalBatch b, double* pmax_err, double* pmax_x,
+                                  const double* dsc = nullptr) {
+  if (dsc) h = dsc[DS_HL];
   const int f = blockIdx.y;
   const size_t n = b.len[f];
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1285,7 +1496,8 @@ __global__ void k_reduce_max_batch(size_t stride, FinalCounts c, const double* p
 // restored, deta = GtoF(k_s) + phis omega^2; partial sums for the volume fix.
 __global__ void k_step_end(int n1, int n2, double pix, double piy, double h, const double* G1, const double* G2,
                            const double* kG1, const double* kG2, const double* phis_hat_cur, double* eta_hat,
-                           double* phis_hat, double* deta_hat, double* partial) {
+                           double* phis_hat, double* deta_hat, double* partial, const double* dsc = nullptr) {
+  if (dsc) h = dsc[DS_HL];
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   double s = 0.0;
@@ -1580,69 +1792,7 @@ void order_grid(const hos_params& p, int m, int& p1, int& p2, int& N1b, int& N2b
 }
 
 void ramp_eval(const hos_params& p, double t, double& a, double& at, double& att) {
-  const double Tr = p.Tramp, Ts = p.T_stop;
-  if (p.iramp == 11) {
-    const double Tsw = 3.0 / std::sqrt(5.0 / 9.81);
-    if (t <= Tsw) {
-      a = t / Tsw; at = 1.0 / Tsw; att = 0.0;
-    } else {
-      a = 1.0; at = 0.0; att = 0.0;
-    }
-    return;
-  }
-  if (p.iramp == 33) {
-    const double Tsw = -std::log(0.1) / (10.0 * p.T_mono);
-    a = 1.0 - std::exp(-Tsw * t);
-    at = (1.0 - a) * Tsw;
-    att = (1.0 - a) * Tsw * Tsw;
-    return;
-  }
-  if (t <= Tr) {
-    switch (p.iramp) {
-      case 0: a = 1.0; at = 0.0; att = 0.0; return;
-      case 1: a = t / Tr; at = 1.0 / Tr; att = 0.0; return;
-      case 2: { const double s = t / Tr; a = s * s; at = 2.0 * s / Tr; att = 2.0 / (Tr * Tr); return; }
-      case 21: {
-        const double T2 = Tr * Tr;
-        if (t <= Tr / 2.0) { a = 2.0 * t * t / T2; at = 4.0 * t / T2; att = 4.0 / T2; }
-        else { const double ts = t / Tr - 1.0; a = 1.0 - 2.0 * ts * ts; at = -4.0 * ts / Tr; att = -4.0 / T2; }
-        return;
-      }
-      case 4: { const double s = t / Tr, s2 = s * s; a = s2 * s2; at = 4.0 * s2 * s / Tr; att = 12.0 * s2 / (Tr * Tr); return; }
-      case 41: {
-        const double T2 = Tr * Tr, rap = 8.0 * t / Tr, rap2 = t * t / T2, rap22 = 3.0 * rap2;
-        a = (rap22 - rap + 6.0) * rap2;
-        at = (4.0 * rap22 - 3.0 * rap + 12.0) * t / T2;
-        att = (12.0 * rap22 - 6.0 * rap + 12.0) / T2;
-        return;
-      }
-      default: throw std::runtime_error("Unknown ramp");
-    }
-  }
-  if (t >= Ts + Tr) { a = at = att = 0.0; return; }
-  if (t >= Ts) {
-    const double ti = Ts + Tr - t;
-    switch (p.iramp) {
-      case 1: a = ti / Tr; at = -1.0 / Tr; att = 0.0; return;
-      case 2: { const double s = ti / Tr; a = s * s; at = -2.0 * s / Tr; att = 2.0 / (Tr * Tr); return; }
-      case 21: {
-        const double T2 = Tr * Tr;
-        if (ti <= Tr / 2.0) { a = 2.0 * ti * ti / T2; at = -4.0 * ti / T2; att = 4.0 / T2; }
-        else { const double ts = ti / Tr - 1.0; a = 1.0 - 2.0 * ts * ts; at = 4.0 * ts / Tr; att = -4.0 / T2; }
-        return;
-      }
-      case 4: { const double s = ti / Tr, s2 = s * s; a = s2 * s2; at = -4.0 * s2 * s / Tr; att = 12.0 * s2 / (Tr * Tr); return; }
-      case 41: {
-        const double T2 = Tr * Tr, rap = 8.0 * ti / Tr, rap2 = ti * ti / T2, rap22 = 3.0 * rap2;
-        a = (rap22 - rap + 6.0) * rap2;
-        at = -(4.0 * rap22 - 3.0 * rap + 12.0) * ti / T2;
-        att = (12.0 * rap22 - 6.0 * rap + 12.0) / T2;
-        return;
-      }
-      default: a = 1.0; at = 0.0; att = 0.0; return;
-    }
-  }
-  a = 1.0; at = 0.0; att = 0.0;
+  if (!ramp_hd(p.iramp, p.Tramp, p.T_stop, p.T_mono, t, a, at, att)) throw std::runtime_error("Unknown ramp");
 }
 
 // paddle shape g(z) (wavemaking.f90:184-215)
@@ -2105,6 +2255,9 @@ struct hos_ctx {
     hos_learned_stats stats{};
   } learn;
   ~hos_ctx() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (dsc_host) (void)hipHostFree(dsc_host);
     if (red_host) (void)hipHostFree(red_host);
     if (learn.rec_host) (void)hipHostFree(learn.rec_host);
     if (learn.dn_host) (void)hipHostFree(learn.dn_host);
@@ -2115,6 +2268,14 @@ struct hos_ctx {
   // first right-hand side of each attempt into record 0, 2 every right-hand side
   // into a ring of feat_cap records (feat_n written since the last take)
   int feat_mode = 0, feat_cap = 0, feat_n = 0, cur_stage = 0;
+  // graph-captured attempts (hos_advance_graph): the device step scalars (DS_COUNT
+  // doubles), the paddle-top tables for them, dsc_active = dsc while an attempt is
+  // being captured (the kernels then read t, h and the ramp from memory)
+  double *dsc = nullptr, *dsc_active = nullptr, *freq_d = nullptr, *pre0_d = nullptr, *pim0_d = nullptr;
+  double* dsc_host = nullptr;  // pinned copy of dsc
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<const void*> graph_sig;  // state pointers and engine the graph was captured with
   double *feat = nullptr, *feat_part = nullptr;
   double *phys_tmp, *nu_d;
   double* filt_fs = nullptr;  // free-surface mode filter weights (n1*n2), null when off
@@ -2509,9 +2670,9 @@ struct Engine : EngineBase {
     // --- GtoF, /omega, volume fix (uses the previous wavemaker evaluation) ---
     const unsigned nbs = nblk(ns);
     hipLaunchKernelGGL(k_gtof_in<T>, dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt, G1, G2, eta_hat,
-                       phis_hat, C.partial, phis64);
+                       phis_hat, C.partial, phis64, (const double*)C.dsc_active, C.cur_stage);
     hipLaunchKernelGGL(k_volfix<T>, dim3(1), dim3(256), 0, st, (int)nbs, C.partial, eta_hat, (double*)nullptr,
-                       C.volmode(), C.last_alpha, C.last_pos_top, C.p.xlen, C.p.d_hinge);
+                       C.volmode(), C.last_alpha, C.last_pos_top, C.p.xlen, C.p.d_hinge, (const double*)C.dsc_active);
     // --- physical eta on the base grid ---
     base(KCOS, KCOS, INV, src_plain<T>(eta_hat, n1, n1, n2), n1, n2, Dst<T>{eta_phys, n1, 0});
     // --- the wavemaker chain runs on the side stream from here: it needs only the
@@ -2628,7 +2789,7 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL((k_expon_out<T, double>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)nullptr,
                          (const double*)phizh64, (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh,
-                         (const double*)Zh64, (const double*)C.filt_fs, kG1, kG2);
+                         (const double*)Zh64, (const double*)C.filt_fs, kG1, kG2, (const double*)C.dsc_active, C.cur_stage);
     } else {
       hipLaunchKernelGGL(k_dphin<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)phiz_add,
                          (const double*)phit_add, (const double*)C.nu_d, Zp);
@@ -2636,7 +2797,7 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL((k_expon_out<T, T>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
                          (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)nullptr,
-                         (const double*)C.filt_fs, kG1, kG2);
+                         (const double*)C.filt_fs, kG1, kG2, (const double*)C.dsc_active, C.cur_stage);
     }
   }
 
@@ -2718,16 +2879,23 @@ struct Engine : EngineBase {
       second_order_wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, *s2);
       return;
     }
-    double al, alt, altt;
-    ramp_eval(C.p, t, al, alt, altt);
-    C.last_alpha = al;
-    C.last_alphat = alt;
-    C.last_pos_top = C.pos_top(t);
+    double al = 0.0, alt = 0.0, altt = 0.0;
+    const double* dsc = C.dsc_active;
+    if (dsc) {
+      // the wavemaker globals of this evaluation, read by the next volume fix
+      hipLaunchKernelGGL(k_wmk_glob, dim3(1), dim3(1), 0, st, C.dsc_active, C.cur_stage);
+    } else {
+      ramp_eval(C.p, t, al, alt, altt);
+      C.last_alpha = al;
+      C.last_alphat = alt;
+      C.last_pos_top = C.pos_top(t);
+    }
     hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
-                       (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt);
+                       (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt, dsc, C.cur_stage);
     // f1 = -alpha X_tt - 2 alpha' X_t - alpha'' X ; f2 = -alpha' X_t - alpha'' X
     hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
-                       (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2);
+                       (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2,
+                       dsc, C.cur_stage);
     paddle_modes(2, kad1, kadr);
     // additional potential at z = eta and its first-order part at z = 0
     phiadd(aad, kad1, eta_phys);
@@ -3265,7 +3433,8 @@ void rk_enqueue(hos_ctx* c, double t, double h) {
       comb(c->aad2m, c->a2nd_add, c->kad2, na);
     }
     if (c->wmk3) comb(c->aad3m, c->a3rd_add, c->kad3, na);
-    hipLaunchKernelGGL(k_axpyk_batch, dim3(nblk(std::max(ns, na)), ab.n), dim3(256), 0, st, h, ab);
+    hipLaunchKernelGGL(k_axpyk_batch, dim3(nblk(std::max(ns, na)), ab.n), dim3(256), 0, st, h, ab,
+                       (const double*)c->dsc_active);
     rhs_eval(c, stage_time(t, h, c->C[jl]), h * c->C[jl], c->G1m, c->G2m, c->a1m, c->e1m, c->aadm, jl, c->a2m,
              c->e2m, c->aad2m, c->aad3m);
   }
@@ -3300,7 +3469,7 @@ void rk_enqueue(hos_ctx* c, double t, double h) {
   if (c->wmk2) fin(c->a2nd_add_t, c->a2nd_add, na, c->kad2);
   if (c->wmk3) fin(c->a3rd_add_t, c->a3rd_add, na, c->kad3);
   hipLaunchKernelGGL(k_final_err_batch, dim3(std::max(nbs, nba), fb.n), dim3(256), 0, st, h, fb, c->pmax_e,
-                     c->pmax_x);
+                     c->pmax_x, (const double*)c->dsc_active);
   hipLaunchKernelGGL(k_reduce_max_batch, dim3(2 * fb.n), dim3(256), 0, st, c->pmax_stride, fc,
                      (const double*)c->pmax_e, (const double*)c->pmax_x, c->red6);
   AxpyBatch ab{};
@@ -3315,17 +3484,42 @@ void rk_enqueue(hos_ctx* c, double t, double h) {
     ax(c->a2nd_t, c->a2nd, ns, c->ka2);
     ax(c->eta2nd_t, c->eta2nd, ns, c->ke2);
   }
-  hipLaunchKernelGGL(k_axpyk_batch, dim3(std::max(nbs, nba), ab.n), dim3(256), 0, st, h, ab);
+  hipLaunchKernelGGL(k_axpyk_batch, dim3(std::max(nbs, nba), ab.n), dim3(256), 0, st, h, ab,
+                     (const double*)c->dsc_active);
   // GtoF back, deta, volume fix with the last stage's wavemaker globals
   hipLaunchKernelGGL(k_step_end, dim3(nbs), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy, h, (const double*)Gn1,
                      (const double*)Gn2, (const double*)c->kG1[4], (const double*)c->kG2[4],
                      (const double*)c->phis_hat, c->eta_hat_t,
-                     c->phis_hat_t, c->deta_hat_t, c->partial);
+                     c->phis_hat_t, c->deta_hat_t, c->partial, (const double*)c->dsc_active);
   hipLaunchKernelGGL(k_volfix<double>, dim3(1), dim3(256), 0, st, (int)nbs, (const double*)c->partial,
                      (double*)nullptr, c->eta_hat_t, c->volmode(), c->last_alpha, c->last_pos_top, c->p.xlen,
-                     c->p.d_hinge);
+                     c->p.d_hinge, (const double*)c->dsc_active);
   const int nr = c->wmk3 ? 10 : (c->wmk2 ? 8 : 6);
-  HIP_OK(hipMemcpyAsync(c->red_host, c->red6, nr * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (c->dsc_active) {
+    // graph-captured attempt: the controller and the copy-back on the device
+    hipLaunchKernelGGL(k_control, dim3(1), dim3(1), 0, st, c->dsc_active, (const double*)c->red6, nr / 2,
+                       c->p.iramp, c->p.Tramp);
+    CopyBatch cb{};
+    auto cp = [&](double* dst, const double* src, size_t len) {
+      cb.dst[cb.n] = dst;
+      cb.src[cb.n] = src;
+      cb.len[cb.n] = len;
+      ++cb.n;
+    };
+    cp(c->eta_hat, c->eta_hat_t, ns);
+    cp(c->phis_hat, c->phis_hat_t, ns);
+    cp(c->a1st_add, c->a1st_add_t, na);
+    cp(c->a_add_ramp, c->a_add_ramp_t, na);
+    cp(c->a1st, c->a1st_t, ns);
+    cp(c->eta1st, c->eta1st_t, ns);
+    cp(c->deta_hat, c->deta_hat_t, ns);
+    hipLaunchKernelGGL(k_commit_if, dim3(nblk(std::max(ns, na)), cb.n), dim3(256), 0, st,
+                       (const double*)c->dsc_active, cb);
+    hipLaunchKernelGGL(k_sub_if, dim3(nblk(na)), dim3(256), 0, st, na, c->a1st_add, (const double*)c->a_add_ramp,
+                       (const double*)c->dsc_active);
+  } else {
+    HIP_OK(hipMemcpyAsync(c->red_host, c->red6, nr * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
   c->have_trial = false;
 }
 
@@ -3408,6 +3602,118 @@ int guard(F&& f) {
     g_last_error = e.what();
     return 1;
   }
+}
+
+}  // namespace
+
+// ===========================================================================
+// graph-captured attempts with the controller on the device (hos_advance_graph)
+// ===========================================================================
+namespace {
+
+std::vector<const void*> graph_signature(hos_ctx* c) {
+  return {c->eng, c->eta_hat, c->phis_hat, c->a1st_add, c->a_add_ramp, c->a1st, c->eta1st, c->deta_hat,
+          c->eta_hat_t, c->phis_hat_t, c->a1st_add_t, c->a_add_ramp_t, c->a1st_t, c->eta1st_t, c->deta_hat_t};
+}
+
+void graph_release(hos_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_sig.clear();
+}
+
+void graph_prepare(hos_ctx* c) {
+  if (!c->wmk || c->wmk2)
+    throw std::runtime_error("graph-captured attempts need a wavemaker tank with i_wmk = 1");
+  if (c->feat_mode == 2) throw std::runtime_error("graph-captured attempts record features in mode 0 or 1 only");
+  if (!c->dsc) {
+    HIP_OK(hipStreamSynchronize(c->st));
+    c->dsc = c->ar.get<double>(DS_COUNT);
+    const int nf = (int)c->freq.size();
+    c->freq_d = c->ar.get<double>(std::max(nf, 1));
+    c->pre0_d = c->ar.get<double>(std::max(nf, 1));
+    c->pim0_d = c->ar.get<double>(std::max(nf, 1));
+    if (nf) {
+      HIP_OK(hipMemcpy(c->freq_d, c->freq.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(c->pre0_d, c->pre0.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(c->pim0_d, c->pim0.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&c->dsc_host), DS_COUNT * sizeof(double), hipHostMallocDefault));
+    HIP_OK(hipDeviceSynchronize());
+  }
+  if (c->graph_exec && c->graph_sig == graph_signature(c)) return;
+  graph_release(c);
+  // one eager attempt first, so that every library kernel of the attempt is loaded and
+  // initialised outside the capture; its trial result and wavemaker globals are discarded
+  const double la = c->last_alpha, lat = c->last_alphat, lp = c->last_pos_top;
+  double err = 0.0;
+  rk_enqueue(c, 0.0, 1e-3);
+  rk_finish(c, &err);
+  c->last_alpha = la;
+  c->last_alphat = lat;
+  c->last_pos_top = lp;
+  c->have_trial = false;
+  HIP_OK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  c->dsc_active = c->dsc;
+  try {
+    hipLaunchKernelGGL(k_stage_scalars, dim3(1), dim3(256), 0, c->st, c->dsc, c->p.iramp, c->p.Tramp, c->p.T_stop,
+                       c->p.T_mono, (int)c->freq.size(), (const double*)c->freq_d, (const double*)c->pre0_d,
+                       (const double*)c->pim0_d, c->gz_top);
+    rk_enqueue(c, 0.0, 0.0);
+  } catch (...) {
+    c->dsc_active = nullptr;
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  c->dsc_active = nullptr;
+  HIP_OK(hipStreamEndCapture(c->st, &c->graph));
+  HIP_OK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+  c->graph_sig = graph_signature(c);
+}
+
+// the controller's state to the device step scalars
+void graph_upload(hos_ctx* c, double t, double t_next, double h, double toler, double dt_stab, double delt_out) {
+  double* d = c->dsc_host;
+  std::fill(d, d + DS_COUNT, 0.0);
+  d[DS_T] = t;
+  d[DS_H] = h;
+  d[DS_TNEXT] = t_next;
+  d[DS_TOLER] = toler;
+  d[DS_STAB] = dt_stab;
+  d[DS_DOUT] = delt_out;
+  d[DS_ENDRAMP] = c->endramp;
+  d[DS_GALPHA] = c->last_alpha;
+  d[DS_GPOS] = c->last_pos_top;
+  d[DS_GALPHAT] = c->last_alphat;
+  HIP_OK(hipMemcpyAsync(c->dsc, d, DS_COUNT * sizeof(double), hipMemcpyHostToDevice, c->st));
+}
+
+void graph_readback(hos_ctx* c) {
+  HIP_OK(hipMemcpyAsync(c->dsc_host, c->dsc, DS_COUNT * sizeof(double), hipMemcpyDeviceToHost, c->st));
+}
+
+// Attempts that cannot overshoot t_next: each advances at most min(0.95 dt_stab, delt_out).
+int graph_round(double t, double t_next, double dt_stab, double delt_out) {
+  const double hmax = std::min(0.95 * dt_stab, delt_out);
+  const double n = std::floor((t_next - t) / hmax);
+  return (int)std::max(1.0, std::min(n, 4096.0));
+}
+
+void graph_finish(hos_ctx* c, double& t, double& h, int& na, int& nr) {
+  const double* d = c->dsc_host;
+  t = d[DS_T];
+  h = d[DS_H];
+  na = (int)d[DS_NACC];
+  nr = (int)d[DS_NREJ];
+  c->endramp = (int)d[DS_ENDRAMP];
+  c->last_alpha = d[DS_GALPHA];
+  c->last_pos_top = d[DS_GPOS];
+  c->last_alphat = d[DS_GALPHAT];
+  c->have_trial = false;
 }
 
 }  // namespace
@@ -4096,6 +4402,69 @@ int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_nex
       if (n_rej) n_rej[i] = a[i].nr;
     }
     if (!first_error.empty()) throw std::runtime_error(first_error);
+  });
+}
+
+int hos_advance_graph(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
+                      double delt_out, int* n_acc, int* n_rej) {
+  return guard([&] {
+    if (!c || !t_cur || !h_rk) throw std::runtime_error("null argument");
+    graph_prepare(c);
+    graph_upload(c, *t_cur, t_next, *h_rk, toler, dt_stab, delt_out);
+    double t = *t_cur, h = *h_rk;
+    int na = 0, nr = 0;
+    while (t < t_next) {
+      const int k = graph_round(t, t_next, dt_stab, delt_out);
+      for (int r = 0; r < k; ++r) HIP_OK(hipGraphLaunch(c->graph_exec, c->st));
+      graph_readback(c);
+      HIP_OK(hipStreamSynchronize(c->st));
+      graph_finish(c, t, h, na, nr);
+    }
+    *t_cur = t;
+    *h_rk = h;
+    if (n_acc) *n_acc = na;
+    if (n_rej) *n_rej = nr;
+  });
+}
+
+int hos_advance_ensemble_graph(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk,
+                               double toler, const double* dt_stab, const double* delt_out, int* n_acc,
+                               int* n_rej) {
+  return guard([&] {
+    if (n < 0) throw std::runtime_error("negative ensemble size");
+    for (int i = 0; i < n; ++i) {
+      if (!ctx[i]) throw std::runtime_error("null context in the ensemble");
+      for (int j = 0; j < i; ++j)
+        if (ctx[j] == ctx[i]) throw std::runtime_error("a context appears twice in the ensemble");
+    }
+    std::vector<double> t(t_cur, t_cur + n), h(h_rk, h_rk + n);
+    std::vector<int> na(n, 0), nr(n, 0);
+    for (int i = 0; i < n; ++i) {
+      graph_prepare(ctx[i]);
+      graph_upload(ctx[i], t[i], t_next, h[i], toler, dt_stab[i], delt_out[i]);
+    }
+    // rounds: every unfinished member's replays go onto its own stream, then one wait for all
+    for (bool any = true; any;) {
+      any = false;
+      for (int i = 0; i < n; ++i) {
+        if (!(t[i] < t_next)) continue;
+        const int k = graph_round(t[i], t_next, dt_stab[i], delt_out[i]);
+        for (int r = 0; r < k; ++r) HIP_OK(hipGraphLaunch(ctx[i]->graph_exec, ctx[i]->st));
+        graph_readback(ctx[i]);
+        any = true;
+      }
+      for (int i = 0; i < n; ++i) {
+        if (!(t[i] < t_next)) continue;
+        HIP_OK(hipStreamSynchronize(ctx[i]->st));
+        graph_finish(ctx[i], t[i], h[i], na[i], nr[i]);
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      t_cur[i] = t[i];
+      h_rk[i] = h[i];
+      if (n_acc) n_acc[i] = na[i];
+      if (n_rej) n_rej[i] = nr[i];
+    }
   });
 }
 

@@ -60,6 +60,7 @@ EXPORTS = [
     "hos_mlp_info", "hos_mlp_create", "hos_mlp_destroy", "hos_mlp_set_params", "hos_mlp_get_params",
     "hos_mlp_forward", "hos_mlp_loss_grad", "hos_mlp_set_data", "hos_mlp_train",
     "hos_advance_learned", "hos_learned_last", "hos_learned_reset", "hos_mlp_forward_noalloc",
+    "hos_advance_graph", "hos_advance_ensemble_graph",
 ]
 
 _lib = None
@@ -123,6 +124,10 @@ def load() -> C.CDLL:
     lib.hos_learned_last.argtypes = [vp, _dp, fp]
     lib.hos_learned_reset.argtypes = [vp]
     lib.hos_mlp_forward_noalloc.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.hos_advance_graph.argtypes = [vp, _dp, C.c_double, _dp, C.c_double, C.c_double, C.c_double,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.hos_advance_ensemble_graph.argtypes = [C.c_int, C.POINTER(vp), _dp, C.c_double, _dp, C.c_double, _dp, _dp,
+                                               C.POINTER(C.c_int), C.POINTER(C.c_int)]
     for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
         getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
     for f in EXPORTS:

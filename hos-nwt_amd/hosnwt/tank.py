@@ -169,6 +169,19 @@ class Tank:
         self.n_er_tot += nr.value
         return h.value, na.value, nr.value
 
+    def advance_graph(self, t_next: float, h_rk: float):
+        """advance() with the attempt captured in a HIP graph and the controller on the GPU
+        (hos_advance_graph): no host synchronisation per attempt."""
+        t = C.c_double(self.time)
+        h = C.c_double(h_rk)
+        na, nr = C.c_int(), C.c_int()
+        check(self.lib.hos_advance_graph(self.h, C.byref(t), t_next, C.byref(h), self.cfg.toler, self.s.dt_stab,
+                                         self.s.delt_out, C.byref(na), C.byref(nr)))
+        self.time = t.value
+        self.n_rk_tot += na.value
+        self.n_er_tot += nr.value
+        return h.value, na.value, nr.value
+
     def modes(self):
         """Modal eta and phis of the committed state (HOS_modes.dat, output.f90:380-383)."""
         out = [np.zeros((self.n1, self.n2), order="F") for _ in range(2)]
@@ -300,6 +313,27 @@ class Tank:
             t_next = min(self.time + delt_out, T_stop + delt_out)
             delt, _, _ = self.advance(t_next, delt)
         return records
+
+
+def advance_ensemble_graph(tanks, t_next: float, h_rk):
+    """hos_advance_ensemble_graph: every tank to t_next by replays of its captured attempt,
+    the controllers on the GPU, one host wait per round of replays."""
+    n = len(tanks)
+    lib = tanks[0].lib
+    ctx = (C.c_void_p * n)(*[t.h.value for t in tanks])
+    t = np.array([tk.time for tk in tanks], dtype=np.float64)
+    h = np.array(h_rk, dtype=np.float64)
+    stab = np.array([tk.s.dt_stab for tk in tanks], dtype=np.float64)
+    dout = np.array([tk.s.delt_out for tk in tanks], dtype=np.float64)
+    na = (C.c_int * n)()
+    nr = (C.c_int * n)()
+    toler = tanks[0].cfg.toler
+    check(lib.hos_advance_ensemble_graph(n, ctx, ptr(t), t_next, ptr(h), toler, ptr(stab), ptr(dout), na, nr))
+    for i, tk in enumerate(tanks):
+        tk.time = float(t[i])
+        tk.n_rk_tot += na[i]
+        tk.n_er_tot += nr[i]
+    return h, list(na), list(nr)
 
 
 def advance_ensemble(tanks, t_next: float, h_rk):

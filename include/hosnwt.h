@@ -129,6 +129,25 @@ int hos_advance(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double t
 int hos_advance_ensemble(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk, double toler,
                          const double* dt_stab, const double* delt_out, int* n_acc, int* n_rej);
 
+/* hos_advance with the attempt captured once in a HIP graph (the five stages,
+ * the combinations, the error maxima) and the accept / reject controller of
+ * HOS-NWT.f90:518-563 on the device: the stage times, the ramp and the paddle
+ * values are computed on the device from the committed time and step, the
+ * accepted trial is copied into the state, and the host waits once per round
+ * of replays (a round is as many attempts as cannot pass t_next), not once per
+ * attempt.  Same decisions as hos_advance; the wavemaker's time functions are
+ * evaluated by the device's cos/sin, so results agree with hos_advance to
+ * rounding.  Wavemaker tanks with i_wmk = 1; the graph is captured on the first
+ * call and again after the state buffers or the order change. */
+int hos_advance_graph(hos_ctx* c, double* t_cur, double t_next, double* h_rk, double toler, double dt_stab,
+                      double delt_out, int* n_acc, int* n_rej);
+/* hos_advance_graph for an ensemble: every member's replays go onto its own
+ * stream, one host wait per round for all members.  Arrays as in
+ * hos_advance_ensemble. */
+int hos_advance_ensemble_graph(int n, hos_ctx* const* ctx, double* t_cur, double t_next, double* h_rk,
+                               double toler, const double* dt_stab, const double* delt_out, int* n_acc,
+                               int* n_rej);
+
 /* Second-order wavemaker state (i_wmk >= 2): a2nd_add (n3_add*n2, modal),
  * a2nd (n1*n2, modal), eta2nd (n1*n2, physical); the driver's arrays of
  * HOS-NWT.f90:508-516.  Error if the context has no second-order wavemaker. */
