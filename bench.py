@@ -184,30 +184,141 @@ def mlp_probe(epochs):
     net.close()
 
 
-def cpu_baseline(wl=WORKLOAD):
-    """FP64 oracle (numpy/scipy, one thread) on one solveHOS2 evaluation of the same workload."""
+def cpu_baseline(wl=WORKLOAD, rows=8):
+    """Host-CPU reference-mode cost of one runge4_adapt2 attempt of the same workload.
+
+    * FFT / HOS part: the FP64 oracle (numpy/scipy, reference transform sequence, scipy.fft
+      on `cores` threads) runs one full attempt from the initialised state (wavemaker modes of
+      t = 0 active).  The oracle sums the wavemaker potential in the separable form; the time
+      it spends there is measured and replaced by:
+    * the reference's direct sums (resol_wmkr.f90:485-532 and :1149-1175, 12 + 5 fields of
+      n1 n2^2 n3_add iterations with trigonometric calls inside, tables of HOS-NWT.f90:364-405),
+      restated in C (oracle/wmk_direct.c, OpenMP over `cores` threads), timed on `rows` of the
+      n1 x-rows of one right-hand side and scaled to n1 rows x 5 right-hand sides."""
+    import scipy.fft as sfft
     from oracle import hos_oracle as ho
+    from oracle import wmk_direct as wd
+    cores = min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     s = make_setup(0, wl)
     c = s.cfg
     p = ho.Params(n1=c.n1, n2=c.n2, mHOS=c.mHOS, xlen=s.xlen, ylen=s.ylen, n3=c.n3, icase=c.icase, igeom=c.igeom,
                   d_hinge=0.0, iramp=c.iramp, Tramp=s.Tramp, T_stop=s.T_stop, T_mono=s.T_mono, iabsnb=c.iabsnb,
                   xabsf=c.xabsf, coeffabsf=c.coeffabsf, freq=s.freq, pos_ini=s.pos_ini)
     orc = ho.Oracle(p)
-    rng = np.random.default_rng(0)
-    kx = np.arange(c.n1) * math.pi / s.xlen
-    ky = np.arange(c.n2) * math.pi / s.ylen
-    k = np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2)
-    G1 = 1e-3 * rng.standard_normal((c.n1, c.n2)) * np.exp(-0.5 * k)
-    G2 = 1e-3 * rng.standard_normal((c.n1, c.n2)) * np.exp(-0.5 * k)
-    z = np.zeros((c.n1, c.n2))
-    za = np.zeros((2 * (c.n3 - 1), c.n2))
-    orc.time = 1.0
+    st = orc.initialize()
+    h = min(0.95 * s.dt_stab, s.delt_out)
+    sep = [0.0]
+    plain = ho.phiadd_sl
+
+    def timed_phiadd(*a, **k):
+        t0 = time.perf_counter()
+        r = plain(*a, **k)
+        sep[0] += time.perf_counter() - t0
+        return r
+
+    ho.phiadd_sl = timed_phiadd
+    try:
+        with sfft.set_workers(cores):
+            t0 = time.perf_counter()
+            orc.runge4_adapt2(0.0, st, h)
+            t_oracle = time.perf_counter() - t0
+    finally:
+        ho.phiadd_sl = plain
+    # the direct sums on a sample of rows at the initial state's paddle modes and a sea of
+    # 0.01 amplitude (the iteration count, not the values, sets the cost)
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    wd.load()
+    n1, n2, n3a = c.n1, c.n2, 2 * (c.n3 - 1)
+    a = np.asfortranarray(st.a1st_add if hasattr(st, "a1st_add") else np.ones((n3a, n2)))
+    da = np.asfortranarray(np.roll(a, 1, axis=0))
+    eta = 0.01 * np.ones((n1, n2))
+    lo = n1 // 2
     t0 = time.perf_counter()
-    orc.solve_hos2(z, z, G2, G1, za, 0.01)
-    dt = time.perf_counter() - t0
-    return {"value": round(1.0 / (5.0 * dt), 5), "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"one solveHOS2 evaluation (1/5 of an RK4(3) step) of the 1024x256 M=5 workload "
-                      f"in reference mode: {dt:.2f} s"}
+    wd.phiadd_sl(n1, n2, s.xlen, s.ylen, a, da, eta, rows=(lo, lo + rows))
+    t_sl = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    wd.phiadd_1st(n1, n2, s.xlen, s.ylen, a, da, rows=(lo, lo + rows))
+    t_1st = time.perf_counter() - t0
+    direct = 5.0 * (n1 / rows) * (t_sl + t_1st)
+    t_attempt = t_oracle - sep[0] + direct
+    return {"value": round(1.0 / t_attempt, 6), "unit": "steps/s", "cores": cores, "kind": "port",
+            "cpu": model,
+            "sample": (f"one runge4_adapt2 attempt of the 1024x256 M=5 workload from its initial state in reference "
+                       f"mode: oracle FFT/HOS part {t_oracle - sep[0]:.2f} s (scipy.fft on {cores} threads; its "
+                       f"separable wavemaker sums, {sep[0]:.2f} s, left out) + the reference's direct wavemaker sums "
+                       f"(17 fields, C restatement on {cores} threads) timed on {rows} of {n1} x-rows of one "
+                       f"right-hand side ({t_sl + t_1st:.3f} s) and scaled x{n1 / rows:.0f} rows x 5 right-hand "
+                       f"sides = {direct:.1f} s"),
+            "seconds_per_attempt": round(t_attempt, 2), "direct_sums_seconds": round(direct, 2),
+            "separable_sums_seconds": round(sep[0], 3),
+            "direct_over_separable": round(direct / max(sep[0], 1e-9), 1)}
+
+
+def small_tank_probe():
+    """Attempts/s of single small tanks, stream path (hos_advance, a host wait per attempt)
+    against graph-captured attempts (hos_advance_graph): cfg1 (2-D n1 = 256, M = 3, regular
+    waves) and cfg2 (2-D n1 = 2048, M = 5, JONSWAP), FP64 and mixed."""
+    from hosnwt.tank import Tank
+    out = {}
+    cases = {"cfg1 2-D 256 M=3": dict(n1=256, M=3, xlen=20.0, regular=True),
+             "cfg2 2-D 2048 M=5": dict(n1=2048, M=5, xlen=60.0, regular=False)}
+    for name, c in cases.items():
+        for prec in (0, 1):
+            if c["regular"]:
+                k = hc.wave_number(1.0 / 1.5 / np.sqrt(9.81))
+                cfg = hc.TankConfig(n1=c["n1"], n2=1, mHOS=c["M"], xlen=c["xlen"], n3=33, icase=2,
+                                    amp_mono=0.08 / k, nu_mono=1.0 / 1.5, igeom=1, iramp=1, Tramp=4.5, xabsf=0.8,
+                                    T_stop=45.0, toler=1e-7, f_out=1.0 / 1.5, precision=prec)
+            else:
+                comps = hc.jonswap_components(0.08, 1.6, 3.3, n_freq=512, seed=7, rnum=12, clock=32.0)
+                cfg = hc.TankConfig(n1=c["n1"], n2=1, mHOS=c["M"], xlen=c["xlen"], n3=33, icase=3, igeom=2,
+                                    iramp=1, Tramp=3.2, xabsf=0.8, T_stop=160.0, toler=1e-7, f_out=1.0 / 1.6,
+                                    precision=prec, components=comps)
+            s = hc.build(cfg)
+            res = {}
+            for path in ("stream", "graph"):
+                tk = Tank(s)
+                tk.initialize()
+                h = min(0.95 * s.dt_stab, s.delt_out)
+                adv = tk.advance if path == "stream" else tk.advance_graph
+                h, _, _ = adv(2.0 * s.delt_out, h)  # untimed: set-up, capture, spin-up
+                t0 = time.perf_counter()
+                h, na, nr = adv(4.0 * s.delt_out, h)
+                dt = time.perf_counter() - t0
+                res[path] = round((na + nr) / dt, 1)
+                tk.close()
+            res["graph_vs_stream"] = round(res["graph"] / res["stream"], 3)
+            out[f"{name} {'fp64' if prec == 0 else 'mixed'}"] = res
+            print(f"[small] {name} prec {prec}: {res}", file=sys.stderr, flush=True)
+    return out
+
+
+def ensemble_members(n_members=64, precision=1, seed=2024):
+    """The batched-ensemble workload: 2-D irregular realisations, length 60, n1 drawn from
+    {512, 1024, 2048, 4096}, M from {3, 4, 5, 6}, JONSWAP Hs ~ U[0.03, 0.10], Tp ~ U[1.2, 2.0],
+    gamma 3.3, 512 components with seeded phases, piston wavemaker, ramp 2 Tp,
+    RK4(5) tolerance 1e-7."""
+    rng = np.random.default_rng(seed)
+    members = []
+    for _ in range(n_members):
+        n1 = int(rng.choice([512, 1024, 2048, 4096]))
+        M = int(rng.choice([3, 4, 5, 6]))
+        Hs, Tp = float(rng.uniform(0.03, 0.10)), float(rng.uniform(1.2, 2.0))
+        comps = hc.jonswap_components(Hs, Tp, 3.3, n_freq=512, seed=int(rng.integers(1 << 30)), rnum=12,
+                                      clock=32.0)
+        cfg = hc.TankConfig(n1=n1, n2=1, mHOS=M, xlen=60.0, n3=33, icase=3, igeom=1, iramp=1, Tramp=2.0 * Tp,
+                            xabsf=0.8, T_stop=100 * Tp, toler=1e-7, f_out=1.0 / Tp, precision=precision,
+                            components=comps)
+        members.append(hc.build(cfg))
+    return members
 
 
 def ensemble_bench(t_next, n_members=64):
@@ -217,17 +328,7 @@ def ensemble_bench(t_next, n_members=64):
     precision, RK4(5) tolerance 1e-7; every member advanced from rest to t_next (nondimensional),
     once member after member (hos_advance) and once together (hos_advance_ensemble)."""
     from hosnwt.tank import Tank, advance_ensemble
-    rng = np.random.default_rng(2024)
-    members = []
-    for _ in range(n_members):
-        n1 = int(rng.choice([512, 1024, 2048, 4096]))
-        M = int(rng.choice([3, 4, 5, 6]))
-        Hs, Tp = float(rng.uniform(0.03, 0.10)), float(rng.uniform(1.2, 2.0))
-        comps = hc.jonswap_components(Hs, Tp, 3.3, n_freq=512, seed=int(rng.integers(1 << 30)), rnum=12,
-                                      clock=32.0)
-        cfg = hc.TankConfig(n1=n1, n2=1, mHOS=M, xlen=60.0, n3=33, icase=3, igeom=1, iramp=1, Tramp=2.0 * Tp,
-                            xabsf=0.8, T_stop=100 * Tp, toler=1e-7, f_out=1.0 / Tp, precision=1, components=comps)
-        members.append(hc.build(cfg))
+    members = ensemble_members(n_members)
     h0 = [min(0.95 * s.dt_stab, s.delt_out) for s in members]
 
     def make():
@@ -260,12 +361,28 @@ def ensemble_bench(t_next, n_members=64):
     for tk in tanks:
         tk.close()
     assert ens_att == seq_att, (ens_att, seq_att)
+    # graph-captured attempts, controllers on the GPU, one host wait per round (the graphs are
+    # captured in a first, untimed interval of one output period)
+    from hosnwt.tank import advance_ensemble_graph
+    tanks = make()
+    t_warm = min(1.0, 0.5 * t_next)
+    hg, _, _ = advance_ensemble_graph(tanks, t_warm, h0)
+    t0 = time.perf_counter()
+    _, nag, nrg = advance_ensemble_graph(tanks, t_next, hg)
+    dt_graph = time.perf_counter() - t0
+    graph_att = sum(nag) + sum(nrg)
+    print(f"[ensemble] graphs: {graph_att} attempts from t = {t_warm} in {dt_graph:.2f} s", file=sys.stderr,
+          flush=True)
+    for tk in tanks:
+        tk.close()
     return {"metric": f"ragged 2-D ensemble RK4(5) steps/s ({n_members} members, n1 512-4096, M 3-6)",
             "value": round(ens_att / dt_ens, 2), "unit": "attempts/s summed over members", "n_gpus": 1,
             "higher_is_better": True, "dtype": "fp32 transforms, fp64 RK", "data": "synthetic (seeded JONSWAP)",
             "t_next": t_next, "attempts": ens_att, "seconds": round(dt_ens, 3),
             "sequential_value": round(seq_att / dt_seq, 2), "sequential_seconds": round(dt_seq, 3),
             "ensemble_vs_sequential": round(dt_seq / dt_ens, 3),
+            "graph_value": round(graph_att / dt_graph, 2), "graph_attempts": graph_att,
+            "graph_seconds": round(dt_graph, 3), "graph_from_t": t_warm,
             "members": {f"n1={n1}": sum(s.cfg.n1 == n1 for s in members) for n1 in (512, 1024, 2048, 4096)}}
 
 
@@ -285,6 +402,8 @@ def main():
     ap.add_argument("--learned", type=float, default=0.0, metavar="T",
                     help="only run the learned-order ensemble (16 x 512x128, M_max = 7) to time T against "
                          "the fixed-order solver")
+    ap.add_argument("--small-tanks", action="store_true",
+                    help="only time the small 2-D tanks, stream path against graph-captured attempts")
     ap.add_argument("--mlp-probe", type=int, default=0, metavar="EPOCHS",
                     help="only train the order-prediction network on synthetic rows (profiling)")
     ap.add_argument("--ensemble", type=float, default=0.0, metavar="T",
@@ -302,6 +421,9 @@ def main():
                           log=lambda m: print(m, file=sys.stderr, flush=True))
         r["metric"] = "learned-order ensemble attempts/s (16 x 3D 512x128, M_max=7) vs fixed order"
         print(json.dumps(r), flush=True)
+        return
+    if args.small_tanks:
+        print(json.dumps(small_tank_probe()), flush=True)
         return
     if args.mlp_probe:
         mlp_probe(args.mlp_probe)
@@ -338,7 +460,7 @@ def main():
     roof = roofline(r32["tank"], 1) if use_mixed else roof64
     cpu = None
     if not args.no_cpu:
-        print("[bench] CPU baseline (oracle, 1 thread)", file=sys.stderr, flush=True)
+        print("[bench] CPU baseline (reference-mode attempt on the host cores)", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
     for r in (r64, r32):
         if r is not None and r["tank"] is not None:

@@ -98,3 +98,35 @@ def test_feature_rows_format():
     assert mi == [1, 2, 3, 4] and ti is None
     # the rate of a record's order and of a prediction of exactly that order agree
     assert math.isclose(hl.predicted_rate(t[2]), hl.rate_of(rec, 4), rel_tol=1e-5)
+
+
+def test_direct_wavemaker_sums_in_c_match_the_oracle():
+    """oracle/wmk_direct.c (the reference's literal loops of phiadd_SL) against the oracle's
+    brute-force loop and separable form, on small 2-D and 3-D grids; phiadd_1st_SWEET's
+    z = 0 fields against phiadd_SL at eta = 0 where they coincide."""
+    from helpers import make_setup, oracle_params
+    from oracle import hos_oracle as ho
+    from oracle import wmk_direct as wd
+
+    for n1, n2 in ((12, 1), (10, 6)):
+        s = make_setup(n1=n1, n2=n2, M=3, n3=5, ylen=3.0)
+        p = oracle_params(s)
+        tb = ho.Tables(p)
+        rng = np.random.default_rng(n1)
+        a = rng.standard_normal((p.n3_add, n2)) * np.exp(-0.5 * np.arange(p.n3_add))[:, None]
+        da = rng.standard_normal((p.n3_add, n2)) * np.exp(-0.5 * np.arange(p.n3_add))[:, None]
+        eta = 0.05 * rng.standard_normal((n1, n2))
+        ref = ho.phiadd_sl_bruteforce(a, da, eta, tb)
+        got = wd.phiadd_sl(n1, n2, s.xlen, s.ylen, a, da, eta)
+        sep = ho.phiadd_sl(a, da, eta, tb)
+        for k in ref:
+            scale = max(np.abs(ref[k]).max(), 1e-300)
+            assert np.abs(got[k] - ref[k]).max() <= 1e-12 * scale, k
+            assert np.abs(sep[k] - got[k]).max() <= 1e-11 * scale, k
+        # rows computed separately equal the full computation
+        part = wd.phiadd_sl(n1, n2, s.xlen, s.ylen, a, da, eta, rows=(2, 5))
+        assert np.array_equal(part["phi"][2:5], got["phi"][2:5])
+        z0 = wd.phiadd_1st(n1, n2, s.xlen, s.ylen, a, da)
+        at0 = wd.phiadd_sl(n1, n2, s.xlen, s.ylen, a, da, np.zeros((n1, n2)))
+        for f, k in enumerate(("phi", "phit", "phix", "phiy", "phiz")):
+            assert np.allclose(z0[f], at0[k], rtol=1e-13, atol=1e-13 * np.abs(at0[k]).max()), k
