@@ -11,8 +11,7 @@ hipcc $flags -c "$here/csrc/mlp.hip" -o "$here/lib/mlp.o" &
 p2=$!
 wait $p1
 wait $p2
-hipcc --offload-arch=gfx950 -shared -fPIC -o "$here/lib/libhosnwt.so" "$here/lib/hosnwt.o" "$here/lib/mlp.o" \
-  -L/opt/rocm/lib -lrocblas
+hipcc --offload-arch=gfx950 -shared -fPIC -o "$here/lib/libhosnwt.so" "$here/lib/hosnwt.o" "$here/lib/mlp.o"
 # Fortran iso_c_binding module and driver (AMD flang)
 if command -v amdflang >/dev/null 2>&1; then
   mkdir -p "$here/lib/mod"

@@ -31,12 +31,10 @@
 #include <thread>
 #include <type_traits>
 #include <vector>
-
-#include <rocblas/rocblas.h>
-
 #include <limits>
 
 #include "../../include/hosnwt.h"
+#include "gemm.h"
 #include "r2r.h"
 #include "mlp.h"
 
@@ -572,7 +570,7 @@ __global__ void k_paddle_rhs(int n2, int n3, int n3a, double delz, const double*
 
 // space_2_Fourier_add('cos','cos') then / k_add tanh(k_add xlen)
 // (fourier_FFTW.f90:432-447, resol_wmkr.f90:160-171): the z REDFT01 and y
-// REDFT00 are dense products with fixed matrices (rocBLAS); this kernel applies
+// REDFT00 are dense products with fixed matrices (gemm.h); this kernel applies
 // the normalisation s_2_f_add and the division.
 __global__ void k_add_scale(int n2, int n3a, int nfields, double xlen, double piy, const double* wz,
                             const double* in, double* out1, double* out2) {
@@ -1874,42 +1872,35 @@ std::vector<double> sin_to_cos_matrix(int N, int n, int K) {
   return t;
 }
 
-void blas_ok(rocblas_status st, const char* what) {
-  if (st != rocblas_status_success) throw std::runtime_error(std::string("rocBLAS error in ") + what);
+// C (m x n) = A (m x k) B (k x n), C column-major, on the matrix cores (gemm.h);
+// ak / bk: the operand is stored k-contiguous (A[kk + lda i], B[kk + ldb j])
+template <typename U>
+inline void gemm(hipStream_t st, bool ak, bool bk, int m, int n, int k, const U* A, int lda, const U* B, int ldb,
+                 U* Cm, int ldc) {
+  hosmm::mm<U>(st, ak, bk, m, n, k, A, lda, 0, B, ldb, 0, Cm, ldc, 0);
 }
-
-inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const double* A, int lda,
-                 const double* B, int ldb, double* Cm, int ldc) {
-  const double one = 1.0, zero = 0.0;
-  blas_ok(rocblas_dgemm(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, Cm, ldc), "dgemm");
+// C_b (m x n) = A_b (m x n, row-contiguous) B (n x n, k-major) for b < batch,
+// A_b and C_b stride s apart
+template <typename U>
+inline void gemm_yb(hipStream_t st, int m, int n, const U* A, long s, const U* B, U* Cm, int batch) {
+  hosmm::mm<U>(st, false, true, m, n, n, A, m, s, B, n, 0, Cm, m, s, batch);
 }
-inline void gemm(rocblas_handle h, rocblas_operation tb, int m, int n, int k, const float* A, int lda,
-                 const float* B, int ldb, float* Cm, int ldc) {
-  const float one = 1.0f, zero = 0.0f;
-  blas_ok(rocblas_sgemm(h, rocblas_operation_none, tb, m, n, k, &one, A, lda, B, ldb, &zero, Cm, ldc), "sgemm");
+// y(m) = AT^T x for AT stored k x m (leading dimension k): one workgroup per
+// output, each a contiguous dot product summed in a fixed order in FP64
+template <typename U>
+__global__ void __launch_bounds__(256) k_gemv_t(int k, const U* __restrict__ AT, const U* __restrict__ x,
+                                                U* __restrict__ y) {
+  const U* col = AT + (size_t)k * blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) acc += (double)col[i] * (double)x[i];
+  acc = block_sum(acc);
+  if (threadIdx.x == 0) y[blockIdx.x] = (U)acc;
 }
-// y(m) = AT^T x for AT stored k x m (leading dimension k): each output a
-// contiguous dot product
-inline void gemv_t(rocblas_handle h, int m, int k, const double* AT, const double* x, double* y) {
-  const double one = 1.0, zero = 0.0;
-  blas_ok(rocblas_dgemv(h, rocblas_operation_transpose, k, m, &one, AT, k, x, 1, &zero, y, 1), "dgemv");
-}
-inline void gemv_t(rocblas_handle h, int m, int k, const float* AT, const float* x, float* y) {
-  const float one = 1.0f, zero = 0.0f;
-  blas_ok(rocblas_sgemv(h, rocblas_operation_transpose, k, m, &one, AT, k, x, 1, &zero, y, 1), "sgemv");
-}
-// C_b (m x n) = A_b (m x n) B^T (n x n) for b < batch, A_b and C_b stride s apart
-inline void gemm_yb(rocblas_handle h, int m, int n, const double* A, long s, const double* B, double* Cm, int batch) {
-  const double one = 1.0, zero = 0.0;
-  blas_ok(rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m, n, n, &one, A, m, s,
-                                        B, n, 0, &zero, Cm, m, s, batch),
-          "dgemm batched");
-}
-inline void gemm_yb(rocblas_handle h, int m, int n, const float* A, long s, const float* B, float* Cm, int batch) {
-  const float one = 1.0f, zero = 0.0f;
-  blas_ok(rocblas_sgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_transpose, m, n, n, &one, A, m, s,
-                                        B, n, 0, &zero, Cm, m, s, batch),
-          "sgemm batched");
+template <typename U>
+inline void gemv_t(hipStream_t st, int m, int k, const U* AT, const U* x, U* y) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_gemv_t<U>, dim3(m), dim3(256), 0, st, k, AT, x, y);
+  HIP_OK(hipGetLastError());
 }
 
 // ---------------------------------------------------------------------------
@@ -1985,7 +1976,7 @@ std::vector<double> filter_weights_2d(int n1, int n2, double cx, double cy) {
 // At the reference sizes the base-grid lengths 2(n-1) carry large primes
 // (1024: 2046 = 2.3.11.31, 256: 510 = 2.3.5.17), so an FFT needs Bluestein's
 // doubled length; on a grid this small the transform is latency-bound, and the
-// N x N DCT-I/DST-I matrix product (rocBLAS, matrix cores) is both faster and
+// N x N DCT-I/DST-I matrix product (FP64/FP32 MFMA, gemm.h) is both faster and
 // exact to rounding.  Column-major N x N, out_r = sum_c M[r + N c] in_c, with
 // the normalisations of fourier_FFTW.f90:178-229 folded in:
 //   cos forward  a_r = s2f_r (x_0 + (-1)^r x_{N-1} + 2 sum_{0<c<N-1} x_c cos(pi r c/(N-1)))
@@ -2064,6 +2055,14 @@ struct DenseGrid {
   U* mid = nullptr;    // after the x product
 };
 
+// the transpose of a column-major r x c matrix (column-major c x r)
+std::vector<double> transposed(const std::vector<double>& a, int r, int c) {
+  std::vector<double> t(a.size());
+  for (int j = 0; j < c; ++j)
+    for (int i = 0; i < r; ++i) t[j + (size_t)c * i] = a[i + (size_t)r * j];
+  return t;
+}
+
 template <typename U>
 U* upload_matrix(Arena& ar, const std::vector<double>& m) {
   std::vector<U> h(m.begin(), m.end());
@@ -2101,11 +2100,12 @@ template <typename U>
 void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
   g.n1 = n1;
   g.n2 = n2;
+  // stored transposed (k-major): row r of the matrix is contiguous, the layout
+  // of the constant operand of the products and of the 2-D tank's GEMV
   for (int k = 0; k < 2; ++k)
     for (int d = 0; d < 2; ++d) {
-      // 2-D tank: a single column, done as a transposed GEMV; stored transposed
-      g.mx[k][d] = dense_matrix<U>(ar, n1, k, d == 0 ? FWD : INV, n2 == 1);
-      if (n2 > 1) g.my[k][d] = dense_matrix<U>(ar, n2, k, d == 0 ? FWD : INV, false);
+      g.mx[k][d] = dense_matrix<U>(ar, n1, k, d == 0 ? FWD : INV, true);
+      if (n2 > 1) g.my[k][d] = dense_matrix<U>(ar, n2, k, d == 0 ? FWD : INV, true);
     }
   g.stage = ar.get<U>((size_t)n1 * n2);
   g.mid = ar.get<U>((size_t)n1 * n2);
@@ -2114,7 +2114,7 @@ void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
 // out (nout1 x nout2, plain store) = My[ky] (Mx[kx] src)^T restricted to the
 // requested output extents.
 template <typename U, class S>
-void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx, int ky, int dir, const S& src,
+void dense2d(const DenseGrid<U>& g, hipStream_t st, int kx, int ky, int dir, const S& src,
              int nout1, int nout2, const Dst<U>& d) {
   if (d.mode != 0) throw std::runtime_error("dense base transform: plain stores only");
   const int n1 = g.n1, n2 = g.n2, di = (dir == FWD) ? 0 : 1;
@@ -2132,11 +2132,11 @@ void dense2d(const DenseGrid<U>& g, rocblas_handle blas, hipStream_t st, int kx,
     HIP_OK(hipGetLastError());
   }
   if (n2 == 1) {
-    gemv_t(blas, nout1, n1, g.mx[kx][di], in, d.p);
+    gemv_t(st, nout1, n1, g.mx[kx][di], in, d.p);
     return;
   }
-  gemm(blas, rocblas_operation_none, nout1, n2, n1, g.mx[kx][di], n1, in, n1, g.mid, n1);
-  gemm(blas, rocblas_operation_transpose, nout1, nout2, n2, g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
+  gemm(st, true, true, nout1, n2, n1, g.mx[kx][di], n1, in, n1, g.mid, n1);
+  gemm(st, false, true, nout1, nout2, n2, (const U*)g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
 }
 
 // y_f[m] = sum_k AT[k + n m] x_f[k], f < nv: dense FP64 base transforms of a 2-D
@@ -2343,32 +2343,27 @@ struct Engine : EngineBase {
   int K1 = 0, K2 = 0;
   T *XA, *YA, *XB, *YB, *PA, *PB, *RA, *RB, *T1, *T2;
   int t1_chunk = 0, t1_parts = 0;  // 2-D: K chunk of the T1 product and the number of chunks
-  rocblas_handle blas = nullptr;
   // The wavemaker chain (paddle motion, paddle-grid products, wavemaker sums,
   // first-order and gradient transforms) depends only on the base-grid eta, so
   // it runs on a second stream beside the HOS recursion, with its own scratch;
-  // helpers use the current stream / handle / scratch (cs, cb, ctmp, ctmpX).
+  // helpers use the current stream / scratch (cs, ctmp, ctmpX).
   hipStream_t sw = nullptr;
-  rocblas_handle bw = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   T* tmpBw = nullptr;
   std::vector<T*> tmpXw;
   hipStream_t cs = nullptr;
-  rocblas_handle cb = nullptr;
   T* ctmp = nullptr;
   std::vector<T*>* ctmpX = nullptr;
   // HBM workspaces of long extended x lines, one per stream (null when unused)
   T *gwsM = nullptr, *gwsS = nullptr, *cgws = nullptr;
   void use_main() {
     cs = c->st;
-    cb = blas;
     ctmp = tmpB;
     ctmpX = &tmpX;
     cgws = gwsM;
   }
   void use_side() {
     cs = sw;
-    cb = bw;
     ctmp = tmpBw;
     ctmpX = &tmpXw;
     cgws = gwsS;
@@ -2454,13 +2449,9 @@ struct Engine : EngineBase {
                                                                        (double)(n2 - 1))
                    : 1.0;
     gz = upload(h_gz);
-    Wz = upload(h_wz);
+    Wz = upload(transposed(h_wz, n3a, n3a));  // k-major
     Wy = upload(h_wy);
-    blas_ok(rocblas_create_handle(&blas), "create_handle");
-    blas_ok(rocblas_set_stream(blas, c->st), "set_stream");
     HIP_OK(hipStreamCreateWithFlags(&sw, hipStreamNonBlocking));
-    blas_ok(rocblas_create_handle(&bw), "create_handle");
-    blas_ok(rocblas_set_stream(bw, sw), "set_stream");
     HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     tmpBw = ar.get<T>(nb);
@@ -2518,7 +2509,9 @@ struct Engine : EngineBase {
             const bool interior = k > 0 && k < n2 - 1 && m > 0 && m < n2 - 1;
             hsy[k + (size_t)n2 * m] = interior ? std::sin(ang) : 0.0;
           }
-      Cz = upload(hcz); Sz = upload(hsz); Cy = upload(hcy); Sy = upload(hsy);
+      // z matrices k-major; the y matrices are symmetric (functions of k m)
+      Cz = upload(transposed(hcz, n3a, n3a)); Sz = upload(transposed(hsz, n3a, n3a));
+      Cy = upload(hcy); Sy = upload(hsy);
       part2 = ar.get<double>((size_t)nch * 11 * ns);
       zf[0] = phit1;
       zf[1] = phiz1;
@@ -2538,10 +2531,10 @@ struct Engine : EngineBase {
       if (M < 3) throw std::runtime_error("the wavemaker coupling needs mHOS >= 3 (or mHOS = 1)");
       K1 = std::min(2 * n1, N1b);
       PA = ar.get<T>((size_t)K1 * 2 * n2);
-      // 2-D: k_t1_partial in FP64 at every size, and in FP32 from n1 = 2048 up (rocBLAS's
-      // two-column products measured: DGEMM 2.3-2.7x slower at n1 = 4096; SGEMM 4 %
-      // faster at n1 = 1024, equal at 2048, 1.7x slower at 4096)
-      // (beyond kT1MaxChunk per chunk, i.e. n1 > 8192, the rocBLAS product below)
+      // 2-D: k_t1_partial in FP64 at every size, and in FP32 from n1 = 2048 up (a
+      // general two-column product measured: 2.3-2.7x slower in FP64 at n1 = 4096; in
+      // FP32 4 % faster at n1 = 1024, equal at 2048, 1.7x slower at 4096)
+      // (beyond kT1MaxChunk per chunk, i.e. n1 > 8192, the dense product below)
       int chunk = std::max(32, (K1 + kT1Parts - 1) / kT1Parts);
       chunk += chunk % 2;  // chunks start at even s
       if (n2 == 1 && (sizeof(T) == 8 || n1 >= 2048) && chunk <= kT1MaxChunk) {  // the chunk sums of k_t1_partial
@@ -2556,22 +2549,20 @@ struct Engine : EngineBase {
         T1 = upload_t(half);
       } else {
         RA = ar.get<T>((size_t)n1 * 2 * n2);
-        T1 = upload_t(sin_to_cos_matrix(Nd1, n1, K1));
+        T1 = upload_t(transposed(sin_to_cos_matrix(Nd1, n1, K1), n1, K1));  // k-major
       }
       if (n2 > 1) {
         K2 = std::min(2 * n2, N2b);
         XB = ar.get<T>(nb); YB = ar.get<T>(nb);
         PB = ar.get<T>((size_t)2 * n1 * K2);
         RB = ar.get<T>((size_t)2 * n1 * n2);
-        T2 = upload_t(sin_to_cos_matrix(Nd2, n2, K2));
+        T2 = upload_t(transposed(sin_to_cos_matrix(Nd2, n2, K2), n2, K2));  // k-major
       }
     } else {
       PA = RA = T1 = nullptr;
     }
   }
   ~Engine() override {
-    if (blas) (void)rocblas_destroy_handle(blas);
-    if (bw) (void)rocblas_destroy_handle(bw);
     if (sw) (void)hipStreamSynchronize(sw);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
@@ -2607,7 +2598,7 @@ struct Engine : EngineBase {
   template <class S, class D>
   void base(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
     if (dense)
-      dense2d<T>(dT, cb, cs, kx, ky, dir, s, o1, o2, d);
+      dense2d<T>(dT, cs, kx, ky, dir, s, o1, o2, d);
     else
       r2r2d<T>(gsT, kx, ky, dir, s, o1, o2, d, ctmp, cs);
     c->stats.n_base++;
@@ -2616,9 +2607,9 @@ struct Engine : EngineBase {
   void base64(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
     if (dense) {
       if constexpr (sizeof(T) == sizeof(double))
-        dense2d<double>(dT, cb, cs, kx, ky, dir, s, o1, o2, d);
+        dense2d<double>(dT, cs, kx, ky, dir, s, o1, o2, d);
       else
-        dense2d<double>(d64, cb, cs, kx, ky, dir, s, o1, o2, d);
+        dense2d<double>(d64, cs, kx, ky, dir, s, o1, o2, d);
     } else {
       r2r2d<double>(c->gs64, kx, ky, dir, s, o1, o2, d, c->phys_tmp, cs);
     }
@@ -2645,8 +2636,8 @@ struct Engine : EngineBase {
     for (int f = 0; f < k; ++f)
       hipLaunchKernelGGL((k_dense_in<U, Src<U, double>>), dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, src[f],
                          stage + (size_t)f * ns);
-    gemm(cb, rocblas_operation_none, n1, k * n2, n1, g.mx[KCOS][di], n1, stage, n1, mid, n1);
-    gemm_yb(cb, n1, n2, mid, (long)ns, g.my[KCOS][di], res, k);
+    gemm(cs, true, true, n1, k * n2, n1, (const U*)g.mx[KCOS][di], n1, (const U*)stage, n1, mid, n1);
+    gemm_yb(cs, n1, n2, mid, (long)ns, g.my[KCOS][di], res, k);
     for (int f = 0; f < k; ++f)
       HIP_OK(hipMemcpyAsync(out[f], res + (size_t)f * ns, ns * sizeof(U), hipMemcpyDeviceToDevice, cs));
     c->stats.n_base += k;
@@ -2843,8 +2834,8 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL(k_t1_partial<T>, dim3(nblk(n1), t1_parts), dim3(256), 0, cs, n1, K1, t1_chunk,
                          (const T*)T1, (const T*)PA, RA);
     } else {
-      gemm(cb, rocblas_operation_none, n1, 2 * n2, K1, T1, n1, PA, K1, RA, n1);
-      if (n2 > 1) gemm(cb, rocblas_operation_transpose, 2 * n1, n2, K2, PB, 2 * n1, T2, n2, RB, 2 * n1);
+      gemm(cs, true, true, n1, 2 * n2, K1, (const T*)T1, K1, (const T*)PA, K1, RA, n1);
+      if (n2 > 1) gemm(cs, false, true, 2 * n1, n2, K2, (const T*)PB, 2 * n1, (const T*)T2, K2, RB, 2 * n1);
     }
     hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, (const T*)RA,
                        (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q, t1_parts);
@@ -3067,23 +3058,16 @@ struct Engine : EngineBase {
   void paddle_fields(const double* a_fs, const double* da_fs, double* out) {
     hos_ctx& C = *c;
     const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a;
-    const double one = 1.0, zero = 0.0;
-    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 4 * n2, n3a, &one, Cz, n3a, stk6,
-                          n3a, &zero, stkz, n3a),
-            "dgemm paddle Cz");
-    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, 2 * n2, n3a, &one, Sz, n3a,
-                          stk6 + 4 * na, n3a, &zero, stkz + 4 * na, n3a),
-            "dgemm paddle Sz");
+    gemm<double>(cs, true, true, n3a, 4 * n2, n3a, Cz, n3a, stk6, n3a, stkz, n3a);
+    gemm<double>(cs, true, true, n3a, 2 * n2, n3a, Sz, n3a, stk6 + 4 * na, n3a, stkz + 4 * na, n3a);
     if (a_fs)
       hipLaunchKernelGGL(k_fs_on_paddle, dim3(n3, n2), dim3(256), 0, cs, n1, n2, n3, n3a, C.delz, C.pix, C.piy,
                          a_fs, da_fs, stkz);
     if (n2 > 1) {
       const double* My[3] = {Cy, Sy, Cy};
       for (int g = 0; g < 3; ++g)
-        blas_ok(rocblas_dgemm_strided_batched(cb, rocblas_operation_none, rocblas_operation_transpose, n3a, n2, n2,
-                                              &one, stkz + 2 * g * na, n3a, (rocblas_stride)na, My[g], n2, 0, &zero,
-                                              out + 2 * g * na, n3a, (rocblas_stride)na, 2),
-                "dgemm paddle y");
+        hosmm::mm<double>(cs, false, true, n3a, n2, n2, stkz + 2 * g * na, n3a, (long)na, My[g], n2, 0,
+                          out + 2 * g * na, n3a, (long)na, 2);
     } else {
       HIP_OK(hipMemcpyAsync(out, stkz, 6 * na * sizeof(double), hipMemcpyDeviceToDevice, cs));
     }
@@ -3230,16 +3214,10 @@ struct Engine : EngineBase {
   void paddle_modes(int nfields, double* out1, double* out2) {
     hos_ctx& C = *c;
     const int n2 = C.n2, n3a = C.n3a;
-    const double one = 1.0, zero = 0.0;
-    blas_ok(rocblas_dgemm(cb, rocblas_operation_none, rocblas_operation_none, n3a, nfields * n2, n3a, &one, Wz, n3a,
-                          f1, n3a, &zero, fz1, n3a),
-            "dgemm paddle z");
+    gemm<double>(cs, true, true, n3a, nfields * n2, n3a, Wz, n3a, f1, n3a, fz1, n3a);
     const double* zin = fz1;
     if (n2 > 1) {
-      blas_ok(rocblas_dgemm_strided_batched(cb, rocblas_operation_none, rocblas_operation_none, n3a, n2, n2, &one,
-                                            fz1, n3a, (rocblas_stride)na, Wy, n2, 0, &zero, fz2, n3a,
-                                            (rocblas_stride)na, nfields),
-              "dgemm paddle y");
+      hosmm::mm<double>(cs, false, true, n3a, n2, n2, fz1, n3a, (long)na, Wy, n2, 0, fz2, n3a, (long)na, nfields);
       zin = fz2;
     }
     hipLaunchKernelGGL(k_add_scale, dim3(nblk(na * nfields)), dim3(256), 0, cs, n2, n3a, nfields, C.p.xlen, C.piy,
@@ -4846,6 +4824,41 @@ int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, c
 
 int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out) {
   return standalone(n1, n2, kind_x, kind_y, precision, INV, in, out);
+}
+
+int hos_dense_product(int precision, int a_kmajor, int b_kmajor, int b_shared, int m, int n, int k, int batch, int tile,
+                      const double* A, const double* B, double* C) {
+  return guard([&] {
+    if (m < 1 || n < 1 || k < 1 || batch < 1 || !A || !B || !C) throw std::runtime_error("bad dense product shape");
+    if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 or 1");
+    const size_t na = (size_t)m * k, nb = (size_t)k * n, nc = (size_t)m * n;
+    const size_t tb = b_shared ? nb : nb * batch;
+    auto run = [&](auto zero) {
+      using U = decltype(zero);
+      std::vector<U> hA(na * batch), hB(tb), hC(nc * batch);
+      for (size_t i = 0; i < hA.size(); ++i) hA[i] = (U)A[i];
+      for (size_t i = 0; i < hB.size(); ++i) hB[i] = (U)B[i];
+      U *dA = nullptr, *dB = nullptr, *dC = nullptr;
+      HIP_OK(hipMalloc(&dA, hA.size() * sizeof(U)));
+      HIP_OK(hipMalloc(&dB, hB.size() * sizeof(U)));
+      HIP_OK(hipMalloc(&dC, hC.size() * sizeof(U)));
+      std::unique_ptr<void, void (*)(void*)> fa(dA, [](void* q) { (void)hipFree(q); });
+      std::unique_ptr<void, void (*)(void*)> fb(dB, [](void* q) { (void)hipFree(q); });
+      std::unique_ptr<void, void (*)(void*)> fc(dC, [](void* q) { (void)hipFree(q); });
+      HIP_OK(hipMemcpy(dA, hA.data(), hA.size() * sizeof(U), hipMemcpyHostToDevice));
+      HIP_OK(hipMemcpy(dB, hB.data(), hB.size() * sizeof(U), hipMemcpyHostToDevice));
+      hosmm::mm<U>(nullptr, a_kmajor != 0, b_kmajor != 0, m, n, k, dA, a_kmajor ? k : m, (long)na, dB, b_kmajor ? k : n,
+                   b_shared ? 0L : (long)nb, dC,
+                   m, (long)nc, batch, tile);
+      HIP_OK(hipDeviceSynchronize());
+      HIP_OK(hipMemcpy(hC.data(), dC, hC.size() * sizeof(U), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < hC.size(); ++i) C[i] = (double)hC[i];
+    };
+    if (precision == 0)
+      run(0.0);
+    else
+      run(0.0f);
+  });
 }
 
 }  // extern "C"

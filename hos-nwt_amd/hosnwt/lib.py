@@ -60,7 +60,7 @@ EXPORTS = [
     "hos_mlp_info", "hos_mlp_create", "hos_mlp_destroy", "hos_mlp_set_params", "hos_mlp_get_params",
     "hos_mlp_forward", "hos_mlp_loss_grad", "hos_mlp_set_data", "hos_mlp_train",
     "hos_advance_learned", "hos_learned_last", "hos_learned_reset", "hos_mlp_forward_noalloc",
-    "hos_advance_graph", "hos_advance_ensemble_graph",
+    "hos_advance_graph", "hos_advance_ensemble_graph", "hos_dense_product",
 ]
 
 _lib = None
@@ -130,6 +130,7 @@ def load() -> C.CDLL:
                                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
     for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
         getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
+    lib.hos_dense_product.argtypes = [C.c_int] * 9 + [_dp, _dp, _dp]
     for f in EXPORTS:
         if f != "hos_last_error":
             getattr(lib, f).restype = C.c_int
@@ -160,6 +161,34 @@ def fptr(a: np.ndarray | None):
 def fortran(a) -> np.ndarray:
     """Column-major float64 copy (the reference's array layout)."""
     return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+def dense_product(a: np.ndarray, b: np.ndarray, a_kmajor: bool = False, b_kmajor: bool = True,
+                  precision: int = 0, tile: int = 0) -> np.ndarray:
+    """C = A B on the GPU's matrix cores, A: (batch, m, k) or (m, k), B: (k, n),
+    shared by the batch, or (batch, k, n).  a_kmajor / b_kmajor choose the
+    layout the operand is handed to the kernel in (k-contiguous or not)."""
+    lib = load()
+    a3 = np.asarray(a, dtype=np.float64)
+    squeeze = a3.ndim == 2
+    if squeeze:
+        a3 = a3[None]
+    batch, m, k = a3.shape
+    b3 = np.asarray(b, dtype=np.float64)
+    shared = b3.ndim == 2
+    if shared:
+        b3 = b3[None]
+    if b3.shape[1] != k or (not shared and b3.shape[0] != batch):
+        raise ValueError("dense_product: operand shapes do not match")
+    n = b3.shape[2]
+    # C order of (batch, m, k) is k-contiguous: column-major k x m per block
+    ab = np.ascontiguousarray(a3 if a_kmajor else a3.transpose(0, 2, 1))
+    bb = np.ascontiguousarray(b3.transpose(0, 2, 1) if b_kmajor else b3)
+    cb = np.zeros((batch, n, m), dtype=np.float64)
+    check(lib.hos_dense_product(precision, int(a_kmajor), int(b_kmajor), int(shared), m, n, k, batch, tile,
+                                ptr(ab), ptr(bb), ptr(cb)))
+    c = cb.transpose(0, 2, 1)
+    return c[0] if squeeze else c
 
 
 def r2r(grid_in: np.ndarray, kind_x: str, kind_y: str, forward: bool, precision: int = 0) -> np.ndarray:

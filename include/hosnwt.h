@@ -222,6 +222,18 @@ int hos_synchronize(hos_ctx* c);
 int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
 int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
 
+/* Standalone dense product on host arrays, the matrix-core kernel behind the
+ * solver's dense transforms (the matrix forms of fourier_FFTW.f90:432-492 and
+ * :552-653 for non-smooth lengths, and the sine-to-cosine map of the
+ * mixed-parity products): C_b = A_b B_b for b < batch, C_b m x n column-major.
+ * Operands contiguous, one after the other: A_b stored k x m when a_kmajor
+ * (A[kk + k i]), else m x k (A[i + m kk]); B_b stored k x n when b_kmajor
+ * (B[kk + k j]), else n x k (B[j + n kk]); one B for all b when b_shared.
+ * precision 0 FP64, 1 FP32 on the GPU (host data stays double); tile 0
+ * (default), 16 or 32. */
+int hos_dense_product(int precision, int a_kmajor, int b_kmajor, int b_shared, int m, int n, int k, int batch,
+                      int tile, const double* A, const double* B, double* C);
+
 /* ---------------------------------------------------------------------------
  * Learned truncation order.  The reference fixes the HOS order M at compile
  * time (the loop bound of HOSphis_modes, resol_HOS.f90:720-782, with the

@@ -1,7 +1,7 @@
 """solveHOS2 parity at the full sizes of the benchmark configurations (FP64 GPU vs the oracle, 1e-10;
 the mixed path on the 2-D tanks with a wavemaker, 5e-5 relative L2 at amplitude 1e-2: the FP32
 transforms of 6144- to 16384-point lines give 0.8-2.2e-5 on these states, the same to three digits
-with the rocBLAS product and with the split-K kernel for the mixed-parity filter).
+with the matrix-core product and with the split-K kernel for the mixed-parity filter).
 
 * 2-D regular wave tank: n1=256, M=3, piston, monochromatic wave (T=1.5, kA=0.08),
   beach on the last 20 %.
@@ -88,7 +88,7 @@ def test_rhs_parity_at_config_size(name):
         tank.close()
 
 
-# 2-D tanks on both sides of the n1 = 2048 switch between the rocBLAS product and the
+# 2-D tanks on both sides of the n1 = 2048 switch between the matrix-core product and the
 # split-K kernel for the mixed-parity filter on the mixed path
 MIXED_2D = ["member_2d_1024_M4", "irregular_2d_2048_M5", "member_2d_2048_M3", "member_2d_4096_M3"]
 
