@@ -301,6 +301,71 @@ def small_tank_probe():
     return out
 
 
+def config_shapes():
+    """The BASELINE configurations' shapes at full size (cfg2, cfg3, cfg4 members, a cfg5 member):
+    name -> TankConfig keyword arguments (precision set by the caller)."""
+    def js(Hs, Tp, n_freq, n_dir, seed):
+        return hc.jonswap_components(Hs, Tp, 3.3, n_freq=n_freq, seed=seed, rnum=12, clock=32.0, n_dir=n_dir,
+                                     spread_s=10)
+    base = dict(n3=33, icase=3, iramp=1, toler=1e-7, T_stop=100.0, f_out=1.0)
+    return {
+        "cfg2 2-D 2048 M=5": dict(base, n1=2048, n2=1, mHOS=5, xlen=60.0, igeom=2, d_hinge=0.0, Tramp=3.2,
+                                  xabsf=0.8, components=js(0.08, 1.6, 512, 1, 7)),
+        "cfg3 3-D 1024x256 M=5": dict(base, n1=1024, n2=256, mHOS=5, xlen=40.0, ylen=10.0, igeom=1, Tramp=2.8,
+                                      xabsf=0.85, components=js(0.06, 1.4, 256, 32, 2024)),
+        "cfg4 member 2-D 512 M=3": dict(base, n1=512, n2=1, mHOS=3, xlen=30.0, igeom=1, Tramp=3.0, xabsf=0.8,
+                                        components=js(0.05, 1.5, 64, 1, 11)),
+        "cfg4 member 2-D 1024 M=4": dict(base, n1=1024, n2=1, mHOS=4, xlen=40.0, igeom=1, Tramp=3.0, xabsf=0.8,
+                                         components=js(0.07, 1.8, 64, 1, 12)),
+        "cfg4 member 2-D 2048 M=3": dict(base, n1=2048, n2=1, mHOS=3, xlen=60.0, igeom=1, Tramp=3.0, xabsf=0.8,
+                                         components=js(0.09, 1.3, 64, 1, 13)),
+        "cfg5 member 3-D 512x128 M=7": dict(base, n1=512, n2=128, mHOS=7, xlen=20.0, ylen=5.0, igeom=1, Tramp=2.8,
+                                            xabsf=0.85, components=js(0.06, 1.4, 32, 8, 5)),
+    }
+
+
+def tune_report(steps=20):
+    """Schedules the autotuner picks on the configuration shapes, with fixed-h attempts/s
+    (h = 0.5 dt_stab, 3 warm-up attempts) under the tuned and under the default schedule
+    (forced with HOSNWT_FFT_SCHED=0,0,8), one context at a time, FP64 and mixed."""
+    import os
+    from hosnwt.tank import Tank
+    out = {}
+    for name, kw in config_shapes().items():
+        for prec in (0, 1):
+            s = hc.build(hc.TankConfig(precision=prec, **kw))
+            n = steps if kw["n2"] > 1 else 3 * steps
+            res = {}
+            for mode in ("default", "tuned"):
+                if mode == "default":
+                    os.environ["HOSNWT_FFT_SCHED"] = "0,0,8"
+                else:
+                    os.environ.pop("HOSNWT_FFT_SCHED", None)
+                tk = Tank(s)
+                tk.initialize()
+                h = 0.5 * s.dt_stab
+                t = 0.0
+                for _ in range(3):
+                    tk.runge4_adapt2(t, h)
+                    tk.commit()
+                    t += h
+                t0 = time.perf_counter()
+                for _ in range(n):
+                    tk.runge4_adapt2(t, h)
+                    tk.commit()
+                    t += h
+                dt = time.perf_counter() - t0
+                res[mode] = round(n / dt, 1)
+                if mode == "tuned":
+                    res["schedule"] = tk.schedule()
+                tk.close()
+            os.environ.pop("HOSNWT_FFT_SCHED", None)
+            res["tuned_vs_default"] = round(res["tuned"] / res["default"], 3)
+            out[f"{name} {'fp64' if prec == 0 else 'mixed'}"] = res
+            print(f"[tune] {name} prec {prec}: {res}", file=sys.stderr, flush=True)
+    return out
+
+
 def ensemble_members(n_members=64, precision=1, seed=2024):
     """The batched-ensemble workload: 2-D irregular realisations, length 60, n1 drawn from
     {512, 1024, 2048, 4096}, M from {3, 4, 5, 6}, JONSWAP Hs ~ U[0.03, 0.10], Tp ~ U[1.2, 2.0],
@@ -406,6 +471,9 @@ def main():
                     help="only time the small 2-D tanks, stream path against graph-captured attempts")
     ap.add_argument("--mlp-probe", type=int, default=0, metavar="EPOCHS",
                     help="only train the order-prediction network on synthetic rows (profiling)")
+    ap.add_argument("--tune-report", action="store_true",
+                    help="only report the autotuned schedules of the configuration shapes and their "
+                         "attempts/s against the default schedules")
     ap.add_argument("--ensemble", type=float, default=0.0, metavar="T",
                     help="only run the ragged 2-D ensemble to nondimensional time T (sequential vs together)")
     args = ap.parse_args()
@@ -414,6 +482,9 @@ def main():
         return
     if args.ensemble:
         print(json.dumps(ensemble_bench(args.ensemble)), flush=True)
+        return
+    if args.tune_report:
+        print(json.dumps(tune_report(args.steps)), flush=True)
         return
     if args.learned:
         from hosnwt import learned as hl
@@ -438,6 +509,7 @@ def main():
     r64 = time_path(0, args.steps, args.warmup, spin)
     print(f"[bench] FP64 {r64['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
     roof64 = roofline(r64["tank"], 0)
+    sched = {"fp64": r64["tank"].schedule()}
     # one live context at a time, the deployment shape: with the FP64 context still alive the
     # mixed path measured 114 instead of 135 steps/s (its two streams then share the process's
     # four hardware queues with the idle context's streams)
@@ -458,6 +530,8 @@ def main():
     # the roofline object describes the headline path's dominant kernel (the FP32 passes when
     # the mixed path is the headline)
     roof = roofline(r32["tank"], 1) if use_mixed else roof64
+    if r32 is not None:
+        sched["mixed"] = r32["tank"].schedule()
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (reference-mode attempt on the host cores)", file=sys.stderr, flush=True)
@@ -490,6 +564,7 @@ def main():
         "simulated_time": [r64["t_warm"], r64["t_end"]],
         "roofline": roof,
         "roofline_fp64": roof64 if use_mixed else None,
+        "fft_schedule": sched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

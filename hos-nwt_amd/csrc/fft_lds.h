@@ -146,6 +146,24 @@ struct Bfly<T, 5> {
   }
 };
 
+// Radix 6 as the prime-factor (Good-Thomas) split 6 = 2 x 3, which needs no
+// inner twiddles: input n = (3 n1 + 2 n2) mod 6, two 3-point transforms over
+// n2, then 2-point ones over n1; output k = (3 k1 + 4 k2) mod 6.
+template <typename T>
+struct Bfly<T, 6> {
+  __device__ static void run(C2<T>* v) {
+    C2<T> a[3] = {v[0], v[2], v[4]}, b[3] = {v[3], v[5], v[1]};
+    Bfly<T, 3>::run(a);
+    Bfly<T, 3>::run(b);
+    v[0] = cadd<T>(a[0], b[0]);
+    v[3] = csub<T>(a[0], b[0]);
+    v[4] = cadd<T>(a[1], b[1]);
+    v[1] = csub<T>(a[1], b[1]);
+    v[2] = cadd<T>(a[2], b[2]);
+    v[5] = csub<T>(a[2], b[2]);
+  }
+};
+
 // One Stockham pass of radix R on `nlines` lines of length L.
 // Inputs of butterfly j sit at j + r*(L/R); outputs go to
 // (j/Ns)*Ns*R + (j%Ns) + r*Ns, after twiddling input r by W_{Ns R}^{r (j%Ns)}.

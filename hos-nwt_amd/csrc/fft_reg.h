@@ -270,10 +270,12 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
 // column tiles are visited in an XCD-aware order so that neighbouring tiles
 // share an L2.  Spectral multipliers (kth, -kx, -ky) are evaluated once per
 // element in a rolled loop into LDS before the loads.
-// FP64 instances are held to two waves per SIMD (at most 256 registers).
+// Workgroups of up to 1024 threads (lines of more than 512 threads each take
+// a workgroup of their own).  FP64 instances are held to two waves per SIMD
+// (at most 256 registers; a 1024-thread workgroup already needs four).
 // FP32 bounds (x: free, y: 4 waves; 3 for destinations that read back more
 // than one value per output) measured best among x 1-3 and y 3-4 waves on the
-// 3-D mixed step.
+// 3-D mixed step with the first compiled sequences.
 // Independent line transforms sharing one launch (blockIdx.y = item): same
 // length, direction and functor types; kind, extents, source and destination
 // per item.  A workgroup whose lines lie beyond its item's extent exits at
@@ -291,7 +293,9 @@ struct LineBatch {
 };
 
 template <typename T, int V, bool YP, class S, class D, int... Rs>
-__global__ void __launch_bounds__((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256,
+__global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
+                                      ? 1024
+                                      : ((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256),
                                   sizeof(T) == 8 ? 2 : (!YP ? 1 : (sizeof(typename D::Pre) > 2 * sizeof(T) ? 3 : 4)))
     r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt,
                                                                  Plan2<T> pl) {

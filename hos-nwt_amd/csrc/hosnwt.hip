@@ -2172,6 +2172,86 @@ __global__ void __launch_bounds__(256) k_gemv_multi(int n, const double* __restr
 // ===========================================================================
 // engine: the right-hand side in transform type T
 // ===========================================================================
+// ---------------------------------------------------------------------------
+// extended-grid schedule autotuner
+// ---------------------------------------------------------------------------
+// A schedule (GridSched: the compiled radix sequence of the x and of the y
+// lines, the y tile width) is chosen per extended grid and precision by timed
+// trials when the first engine of that grid is built (Engine::tune), and kept
+// in a process-wide table so that every context of one shape runs the same
+// arithmetic (contexts of one shape give bit-identical results).  Modes
+// (HOSNWT_TUNE): 1 (default) trials, 0 the default schedule; HOSNWT_FFT_SCHED
+// ="xs,ys[,tx]" forces one; HOSNWT_TUNE_CACHE names a text file the table is
+// read from at first use and new choices are appended to, so that later
+// processes repeat them without trials.
+struct TuneKey {
+  int n1, n2, bytes, dev;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(n1, n2, bytes, dev) < std::tie(o.n1, o.n2, o.bytes, o.dev);
+  }
+};
+struct TuneRec {
+  GridSched s{};
+  double us_default = 0.0, us_best = 0.0;  // one trial batch, default and chosen schedule
+  int trials = 0;                          // candidates timed (0: from the cache file or forced)
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneRec> g_tune;
+bool g_tune_loaded = false;
+
+// the forced schedule, if any
+bool env_sched(GridSched& g) {
+  if (const char* e = std::getenv("HOSNWT_FFT_SCHED")) {
+    g = GridSched{};
+    if (std::sscanf(e, "%d,%d,%d", &g.xs, &g.ys, &g.tx) < 2) throw std::runtime_error("HOSNWT_FFT_SCHED: xs,ys[,tx]");
+    return true;
+  }
+  return false;
+}
+bool tune_enabled() {
+  const char* e = std::getenv("HOSNWT_TUNE");
+  return !(e && std::atoi(e) == 0);
+}
+// Looks the grid up (loading the cache file once); false when it must be tuned.
+bool tune_lookup(const TuneKey& k, TuneRec& out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tune_loaded) {
+    g_tune_loaded = true;
+    if (const char* f = std::getenv("HOSNWT_TUNE_CACHE")) {
+      if (FILE* fp = std::fopen(f, "r")) {
+        TuneKey q{};
+        TuneRec r{};
+        while (std::fscanf(fp, "%d %d %d %d %d %d %d %lf %lf", &q.n1, &q.n2, &q.bytes, &q.dev, &r.s.xs, &r.s.ys,
+                           &r.s.tx, &r.us_default, &r.us_best) == 9) {
+          r.trials = 0;
+          g_tune[q] = r;
+        }
+        std::fclose(fp);
+      }
+    }
+  }
+  auto it = g_tune.find(k);
+  if (it == g_tune.end()) return false;
+  out = it->second;
+  return true;
+}
+// Records a tuned grid; the first record of a key wins (a concurrent tuning of
+// the same grid adopts it), and is appended to the cache file.
+TuneRec tune_store(const TuneKey& k, const TuneRec& r) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto ins = g_tune.emplace(k, r);
+  if (ins.second) {
+    if (const char* f = std::getenv("HOSNWT_TUNE_CACHE")) {
+      if (FILE* fp = std::fopen(f, "a")) {
+        std::fprintf(fp, "%d %d %d %d %d %d %d %.3f %.3f\n", k.n1, k.n2, k.bytes, k.dev, r.s.xs, r.s.ys, r.s.tx,
+                     r.us_default, r.us_best);
+        std::fclose(fp);
+      }
+    }
+  }
+  return ins.first->second;
+}
+
 struct EngineBase {
   virtual ~EngineBase() = default;
   // S2 carries the second-order wavemaker stage (null below i_wmk = 2)
@@ -2193,6 +2273,10 @@ struct EngineBase {
   // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
   // and one y pass (full inverse cos-cos transform), timed with HIP events
   virtual void time_big(int reps, double* out4) = 0;
+  // the extended grid's schedule: {x line length, x values per thread, x sequence,
+  // y line length, y values per thread, y sequence, y tile width, candidates timed}
+  // and the trial batch's time (us) with the default and the chosen schedule
+  virtual void schedule(int* s8, double* us2) const = 0;
 };
 
 struct hos_ctx {
@@ -2561,6 +2645,99 @@ struct Engine : EngineBase {
     } else {
       PA = RA = T1 = nullptr;
     }
+    tune();
+  }
+
+  // The extended grid's schedule (see tune_lookup): forced, from the table, or
+  // by trials.  A trial is a batch of four independent inverse cos-cos
+  // transforms of the grid (the shape of a right-hand side's batched
+  // launches) on scratch buffers, timed with HIP events on the context stream:
+  // one warm-up and three timed batches per candidate, the best of the three.
+  TuneRec tune_rec{};
+  void tune() {
+    GridSched forced{};
+    if (env_sched(forced)) {
+      rebuild(forced);
+      tune_rec = TuneRec{gb.sched, 0.0, 0.0, 0};
+      return;
+    }
+    const TuneKey key{N1b, N2b, (int)sizeof(T), c->device};
+    TuneRec rec{};
+    if (tune_lookup(key, rec)) {
+      rebuild(rec.s);
+      tune_rec = rec;
+      return;
+    }
+    if (!tune_enabled() || !gb.px.V2) {  // nothing compiled to choose between (HBM or LDS line engines)
+      tune_rec = TuneRec{gb.sched, 0.0, 0.0, 0};
+      return;
+    }
+    std::vector<GridSched> cand;
+    const bool xalt = seq_alt_values(gb.px.dev2.L) > 0;
+    const bool yalt = N2b > 1 && gb.py.V2 && seq_alt_values(gb.py.dev2.L) > 0;
+    const std::vector<int> txs = (N2b > 1 && gb.py.V2) ? std::vector<int>{8, 4, 16} : std::vector<int>{0};
+    for (int xs = 0; xs <= (xalt ? 1 : 0); ++xs)
+      for (int ys = 0; ys <= (yalt ? 1 : 0); ++ys)
+        for (int tx : txs) cand.push_back(GridSched{xs, ys, tx});
+    hipEvent_t ev[5];
+    for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+    double best = 1e30, def = 0.0;
+    GridSched pick = cand[0];
+    std::vector<GridSched> seen;
+    for (const GridSched& g : cand) {
+      rebuild(g);
+      bool dup = false;  // the tile width may clamp two candidates to one schedule
+      for (const GridSched& q : seen) dup = dup || (q == gb.sched);
+      if (dup) continue;
+      seen.push_back(gb.sched);
+      Batch2<T, Src<T, T>, Dst<T>> b;
+      T* out[4] = {tmpX[4], tmpX[5], tmpX[6], Y};
+      for (int i = 0; i < 4; ++i)
+        b.add(KCOS, KCOS, src_plain<T>(X, N1b, N1b, N2b), N1b, N2b, Dst<T>{out[i], N1b, 0}, tmpX[i]);
+      r2r2d_batch<T>(gb, INV, b, c->st, false, gwsM);
+      for (int r = 0; r < 4; ++r) {
+        if (r > 0) r2r2d_batch<T>(gb, INV, b, c->st, false, gwsM);
+        HIP_OK(hipEventRecord(ev[r], c->st));
+      }
+      r2r2d_batch<T>(gb, INV, b, c->st, false, gwsM);
+      HIP_OK(hipEventRecord(ev[4], c->st));
+      HIP_OK(hipEventSynchronize(ev[4]));
+      double t = 1e30;
+      for (int r = 0; r < 4; ++r) {
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, ev[r], ev[r + 1]));
+        t = std::min(t, 1000.0 * ms);
+      }
+      if (seen.size() == 1) def = t;
+      if (t < best) {
+        best = t;
+        pick = gb.sched;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    rec = tune_store(key, TuneRec{pick, def, best, (int)seen.size()});
+    rebuild(rec.s);
+    tune_rec = rec;
+  }
+  void schedule(int* s8, double* us2) const override {
+    s8[0] = gb.px.V2 ? gb.px.dev2.L : gb.px.dev.Lorig;
+    s8[1] = gb.px.V2;
+    s8[2] = gb.sched.xs;
+    s8[3] = N2b > 1 ? (gb.py.V2 ? gb.py.dev2.L : gb.py.dev.Lorig) : 0;
+    s8[4] = N2b > 1 ? gb.py.V2 : 0;
+    s8[5] = gb.sched.ys;
+    s8[6] = N2b > 1 ? gb.TX : 0;
+    s8[7] = tune_rec.trials;
+    us2[0] = tune_rec.us_default;
+    us2[1] = tune_rec.us_best;
+  }
+  // rebuilds the extended grid's plans for schedule g (the stream is idle)
+  void rebuild(const GridSched& g) {
+    if (g == gb.sched) return;
+    HIP_OK(hipStreamSynchronize(c->st));
+    gb.release();
+    gb.build(N1b, N2b, g);
+    gb.check_lengths(true);
   }
   ~Engine() override {
     if (sw) (void)hipStreamSynchronize(sw);
@@ -4824,6 +5001,13 @@ int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, c
 
 int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out) {
   return standalone(n1, n2, kind_x, kind_y, precision, INV, in, out);
+}
+
+int hos_get_schedule(hos_ctx* c, int* s8, double* us2) {
+  return guard([&] {
+    if (!c || !s8 || !us2) throw std::runtime_error("null argument");
+    c->eng->schedule(s8, us2);
+  });
 }
 
 int hos_dense_product(int precision, int a_kmajor, int b_kmajor, int b_shared, int m, int n, int k, int batch, int tile,

@@ -238,6 +238,52 @@ inline int seq_values(int L) {
     default: return 8;
   }
 }
+// Second compiled schedule of a length (schedule 1): half the values per
+// thread, twice the threads per line (up to 1024), radices 4 (and 6 / 3 / 2)
+// where the values per thread are not a multiple of 8.
+inline int seq_alt_values(int L) {
+  switch (L) {
+    case 12288: case 6144: case 3072: case 1536: case 768: return 12;
+    case 8192: case 4096: case 2048: return 8;
+    case 1024: case 512: return 4;
+    default: return 0;
+  }
+}
+// The radix sequences compiled for each length and schedule (launch_reg
+// instantiates exactly these; launch_seq checks the match).
+inline std::vector<int> seq_radices(int L, bool alt) {
+  switch (L) {
+    case 12288: return alt ? std::vector<int>{4, 4, 4, 4, 4, 4, 3} : std::vector<int>{8, 8, 8, 8, 3};
+    case 8192: return {8, 8, 8, 8, 2};
+    case 6144: return alt ? std::vector<int>{4, 4, 4, 4, 4, 6} : std::vector<int>{8, 8, 8, 4, 3};
+    case 3072: return alt ? std::vector<int>{4, 4, 4, 4, 4, 3} : std::vector<int>{8, 8, 8, 2, 3};
+    case 1536: return alt ? std::vector<int>{4, 4, 4, 4, 6} : std::vector<int>{8, 8, 8, 3};
+    case 768: return alt ? std::vector<int>{4, 4, 4, 4, 3} : std::vector<int>{8, 8, 4, 3};
+    case 4096: return {8, 8, 8, 8};
+    case 2048: return {8, 8, 8, 4};
+    case 1024: return alt ? std::vector<int>{4, 4, 4, 4, 4} : std::vector<int>{8, 8, 8, 2};
+    case 512: return alt ? std::vector<int>{4, 4, 4, 4, 2} : std::vector<int>{8, 8, 8};
+    default: return {};
+  }
+}
+// LDS values per line of the register engine: one past the largest padded
+// position any exchange of the sequence uses (hosfft2::xpos for x lines,
+// hosfft2::padx for the interleaved y tiles), rounded up to even.
+inline int seq_lds_stride(int L, const std::vector<int>& rs, int elem_bytes) {
+  const int W = 256 / elem_bytes;
+  int mx = hosfft2::padx(L - 1);
+  int Ns = 1;
+  for (size_t i = 0; i + 1 < rs.size(); ++i) {  // the last pass writes no exchange
+    const int R = rs[i], p = L - 1;
+    int x = p;
+    if (Ns == 1) x = p + p / W;
+    else if (Ns < W) x = p + Ns * (p / (R * Ns));
+    mx = std::max(mx, x);
+    Ns *= R;
+  }
+  const int lp = mx + 1;
+  return lp + (lp & 1);
+}
 // Compiled lengths in increasing order (extended grids are padded to one).
 constexpr int kSeqLengths[] = {512, 768, 1024, 1536, 2048, 3072, 4096, 6144, 8192, 12288};
 
@@ -382,6 +428,7 @@ struct LinePlan {
   FftPlan<T> dev{};
   hosfft2::Plan2<T> dev2{};
   int V2 = 0;  // values per thread of the register-resident engine (0: not usable)
+  int sched = 0;  // register-engine schedule: 0 the first compiled sequence, 1 the second
   int glob = 0;  // 1: line too long for both in-workgroup engines, multi-launch HBM passes
   std::vector<void*> allocs;
   // Register-resident plan: lengths 2^a 3^b with radices 8/4/2/3 dividing V.
@@ -394,14 +441,15 @@ struct LinePlan {
     (void)a;
     (void)b;
     if (!seq_supported(L)) return;
-    const int V = seq_values(L);
-    int np = 0;
+    const bool alt = sched == 1 && seq_alt_values(L) > 0;
+    const int V = alt ? seq_alt_values(L) : seq_values(L);
+    const std::vector<int> rs = seq_radices(L, alt);
+    if (rs.empty() || (int)rs.size() > hosfft2::kMaxPass2) throw std::runtime_error("no radix sequence for this length");
     dev2.L = L;
-    dev2.npass = np;
+    dev2.npass = (int)rs.size();
+    for (size_t i = 0; i < rs.size(); ++i) dev2.radix[i] = rs[i];
     dev2.ntl = L / V;
-    int lp = hosfft2::padx(L - 1) + 1;
-    lp += lp & 1;
-    dev2.lp = lp;
+    dev2.lp = seq_lds_stride(L, rs, (int)sizeof(T));
     std::vector<C2<T>> t0(64), t1((L + 63) / 64);
     for (int m = 0; m < 64; ++m) {
       const double ang = -2.0 * M_PI * (double)m / (double)L;
@@ -482,15 +530,26 @@ struct LinePlan {
   }
 };
 
+// Schedule of one grid's register-engine transforms: the compiled sequence of
+// the x and of the y lines (0 or 1, see seq_alt_values) and the y tile width
+// (columns per workgroup; 0: the default).
+struct GridSched {
+  int xs = 0, ys = 0, tx = 0;
+  bool operator==(const GridSched& o) const { return xs == o.xs && ys == o.ys && tx == o.tx; }
+};
+
 // Transforms on one N1 x N2 grid (N2 == 1: 2-D tank, x lines only).
 template <typename T>
 struct Grid2 {
   int N1 = 0, N2 = 1;
   int TX = 8;
+  GridSched sched{};
   LinePlan<T> px, py;
-  void build(int n1, int n2) {
+  void build(int n1, int n2, GridSched sc = {}) {
     N1 = n1;
     N2 = n2;
+    px.sched = sc.xs;
+    py.sched = sc.ys;
     px.build(2 * (N1 - 1));
     if (N2 > 1) py.build(2 * (N2 - 1));
     px.glob = (!px.V2 && !lds_fits(px, 32) && !px.dev.bluestein) ? 1 : 0;
@@ -500,16 +559,16 @@ struct Grid2 {
            ((size_t)(TX / 2) * py.dev.L * sizeof(C2<T>) > 96 * 1024 || (TX / 2) * py.dev.L > 256 * hosfft::kValsPerThread))
       TX /= 2;
     if (N2 > 1 && py.V2) {
-      // y tile width: 8 columns.  A lone 3073 x 769 pass is 3-13 % faster with
-      // 16 (longer row segments), but the batched launches of a right-hand side
-      // run 2-7 % faster with 8, whose smaller LDS footprint keeps two tiles per
-      // CU (FP32: 8 and 16 equal on the full step); HOSNWT_Y_TILE overrides it
-      int want = 8;
-      if (const char* e = std::getenv("HOSNWT_Y_TILE")) want = std::max(2, std::atoi(e));
-      TX = want;
+      // y tile width: 8 columns by default.  A lone 3073 x 769 pass is 3-13 %
+      // faster with 16 (longer row segments), but the batched launches of a
+      // right-hand side run 2-7 % faster with 8, whose smaller LDS footprint
+      // keeps two tiles per CU (FP32: 8 and 16 equal on the full step)
+      TX = sc.tx > 0 ? sc.tx : 8;
       while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 128 * 1024 || (TX / 2) * py.dev2.ntl > 512))
         TX /= 2;
     }
+    sched = GridSched{px.V2 && px.sched == 1 && seq_alt_values(px.dev2.L) ? 1 : 0,
+                      py.V2 && py.sched == 1 && seq_alt_values(py.dev2.L) ? 1 : 0, N2 > 1 ? TX : 0};
   }
   static bool lds_fits(const LinePlan<T>& pl, int vpt) {
     return (size_t)pl.dev.L * sizeof(C2<T>) <= 160 * 1024 && (long)pl.dev.L <= 256L * vpt;
@@ -571,8 +630,13 @@ inline int batch_max_lines(const LineBatch<S, D>& b) {
 template <typename T, int V, bool YP, class S, class D, int... Rs>
 void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   static_assert(hosfft2::hosfft2_prod<Rs...>::value % V == 0, "V must divide the line length");
-  if (hosfft2::hosfft2_prod<Rs...>::value != pl.dev2.L)
-    throw std::runtime_error("radix sequence does not multiply to the line length " + std::to_string(pl.dev2.L));
+  {
+    constexpr int rs[] = {Rs...};
+    bool same = hosfft2::hosfft2_prod<Rs...>::value == pl.dev2.L && (int)sizeof...(Rs) == pl.dev2.npass && V == pl.V2;
+    for (int i = 0; same && i < (int)sizeof...(Rs); ++i) same = rs[i] == pl.dev2.radix[i];
+    if (!same) throw std::runtime_error("compiled radix sequence does not match the plan of length " +
+                                        std::to_string(pl.dev2.L));
+  }
   const size_t lds = (size_t)ncl * pl.dev2.lp * sizeof(T) + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
   const int nt = ncl * pl.dev2.ntl;
   const int nl = batch_max_lines(b);
@@ -585,19 +649,51 @@ void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<
   HIP_OK(hipGetLastError());
 }
 
+// The compiled schedules: length, values per thread, radix sequence.
 template <typename T, bool YP, class S, class D>
 void launch_reg(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+  const bool alt = pl.V2 == seq_alt_values(pl.dev2.L) && pl.V2 != seq_values(pl.dev2.L);
   switch (pl.dev2.L) {
-    case 12288: launch_seq<T, 24, YP, S, D, 8, 8, 8, 8, 3>(pl, ncl, N, dir, b, st); break;
-    case 8192: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8, 2>(pl, ncl, N, dir, b, st); break;
-    case 6144: launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
-    case 3072: launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, dir, b, st); break;
-    case 1536: launch_seq<T, 24, YP, S, D, 8, 8, 8, 3>(pl, ncl, N, dir, b, st); break;
-    case 768: launch_seq<T, 24, YP, S, D, 8, 8, 4, 3>(pl, ncl, N, dir, b, st); break;
-    case 4096: launch_seq<T, 16, YP, S, D, 8, 8, 8, 8>(pl, ncl, N, dir, b, st); break;
-    case 2048: launch_seq<T, 16, YP, S, D, 8, 8, 8, 4>(pl, ncl, N, dir, b, st); break;
-    case 1024: launch_seq<T, 8, YP, S, D, 8, 8, 8, 2>(pl, ncl, N, dir, b, st); break;
-    case 512: launch_seq<T, 8, YP, S, D, 8, 8, 8>(pl, ncl, N, dir, b, st); break;
+    case 12288:
+      if (alt) launch_seq<T, 12, YP, S, D, 4, 4, 4, 4, 4, 4, 3>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 24, YP, S, D, 8, 8, 8, 8, 3>(pl, ncl, N, dir, b, st);
+      break;
+    case 8192:
+      if (alt) launch_seq<T, 8, YP, S, D, 8, 8, 8, 8, 2>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 16, YP, S, D, 8, 8, 8, 8, 2>(pl, ncl, N, dir, b, st);
+      break;
+    case 6144:
+      if (alt) launch_seq<T, 12, YP, S, D, 4, 4, 4, 4, 4, 6>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 24, YP, S, D, 8, 8, 8, 4, 3>(pl, ncl, N, dir, b, st);
+      break;
+    case 3072:
+      if (alt) launch_seq<T, 12, YP, S, D, 4, 4, 4, 4, 4, 3>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 24, YP, S, D, 8, 8, 8, 2, 3>(pl, ncl, N, dir, b, st);
+      break;
+    case 1536:
+      if (alt) launch_seq<T, 12, YP, S, D, 4, 4, 4, 4, 6>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 24, YP, S, D, 8, 8, 8, 3>(pl, ncl, N, dir, b, st);
+      break;
+    case 768:
+      if (alt) launch_seq<T, 12, YP, S, D, 4, 4, 4, 4, 3>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 24, YP, S, D, 8, 8, 4, 3>(pl, ncl, N, dir, b, st);
+      break;
+    case 4096:
+      if (alt) launch_seq<T, 8, YP, S, D, 8, 8, 8, 8>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 16, YP, S, D, 8, 8, 8, 8>(pl, ncl, N, dir, b, st);
+      break;
+    case 2048:
+      if (alt) launch_seq<T, 8, YP, S, D, 8, 8, 8, 4>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 16, YP, S, D, 8, 8, 8, 4>(pl, ncl, N, dir, b, st);
+      break;
+    case 1024:
+      if (alt) launch_seq<T, 4, YP, S, D, 4, 4, 4, 4, 4>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 8, YP, S, D, 8, 8, 8, 2>(pl, ncl, N, dir, b, st);
+      break;
+    case 512:
+      if (alt) launch_seq<T, 4, YP, S, D, 4, 4, 4, 4, 2>(pl, ncl, N, dir, b, st);
+      else launch_seq<T, 8, YP, S, D, 8, 8, 8>(pl, ncl, N, dir, b, st);
+      break;
     default: throw std::runtime_error("no compiled radix sequence for L=" + std::to_string(pl.dev2.L));
   }
 }

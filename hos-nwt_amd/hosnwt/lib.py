@@ -60,7 +60,7 @@ EXPORTS = [
     "hos_mlp_info", "hos_mlp_create", "hos_mlp_destroy", "hos_mlp_set_params", "hos_mlp_get_params",
     "hos_mlp_forward", "hos_mlp_loss_grad", "hos_mlp_set_data", "hos_mlp_train",
     "hos_advance_learned", "hos_learned_last", "hos_learned_reset", "hos_mlp_forward_noalloc",
-    "hos_advance_graph", "hos_advance_ensemble_graph", "hos_dense_product",
+    "hos_advance_graph", "hos_advance_ensemble_graph", "hos_dense_product", "hos_get_schedule",
 ]
 
 _lib = None
@@ -131,6 +131,7 @@ def load() -> C.CDLL:
     for f in ("hos_space_2_fourier", "hos_fourier_2_space"):
         getattr(lib, f).argtypes = [C.c_int] * 5 + [_dp, _dp]
     lib.hos_dense_product.argtypes = [C.c_int] * 9 + [_dp, _dp, _dp]
+    lib.hos_get_schedule.argtypes = [vp, C.POINTER(C.c_int), _dp]
     for f in EXPORTS:
         if f != "hos_last_error":
             getattr(lib, f).restype = C.c_int

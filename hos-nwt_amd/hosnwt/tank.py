@@ -260,6 +260,16 @@ class Tank:
         return dict(kernel="r2r_seq_kernel x pass (extended grid, inverse cos)", ms=out[0], bytes=out[1],
                     other=dict(ms=out[2], bytes=out[3]), **{"pass": "x"})
 
+    def schedule(self):
+        """The active engine's extended-grid transform schedule (see hos_get_schedule)."""
+        s8 = (C.c_int * 8)()
+        us = np.zeros(2)
+        check(self.lib.hos_get_schedule(self.h, s8, ptr(us)))
+        keys = ("x_length", "x_values", "x_seq", "y_length", "y_values", "y_seq", "y_tile", "trials")
+        out = {k: int(v) for k, v in zip(keys, s8)}
+        out.update(us_default=float(us[0]), us_tuned=float(us[1]))
+        return out
+
     def synchronize(self):
         check(self.lib.hos_synchronize(self.h))
 

@@ -222,6 +222,16 @@ int hos_synchronize(hos_ctx* c);
 int hos_space_2_fourier(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
 int hos_fourier_2_space(int n1, int n2, int kind_x, int kind_y, int precision, const double* in, double* out);
 
+/* Schedule of the active engine's extended-grid transforms (no reference
+ * counterpart as an entry point: the reference plans its transforms with
+ * FFTW_PATIENT in fourier_ini, fourier_FFTW.f90:123-170, which this mirrors
+ * with timed trials of compiled schedules).  s8 = {x line length, x values per thread, x sequence (0/1),
+ * y line length, y values per thread, y sequence, y tile width, candidates
+ * timed (0: forced, default or read from HOSNWT_TUNE_CACHE)}; us2 = the trial
+ * batch (four inverse cos-cos transforms) in microseconds with the default and
+ * with the chosen schedule. */
+int hos_get_schedule(hos_ctx* c, int* s8, double* us2);
+
 /* Standalone dense product on host arrays, the matrix-core kernel behind the
  * solver's dense transforms (the matrix forms of fourier_FFTW.f90:432-492 and
  * :552-653 for non-smooth lengths, and the sine-to-cosine map of the
