@@ -77,8 +77,8 @@ CALL blank(); CALL rd_i(iprobes); CALL rd_c(pro_file)
 CALL blank(); CALL rd_r(T_stop); CALL rd_r(toler); CALL rd_r(f_out)
 CALL blank(); CALL rd_i(idim); CALL rd_i(i3d); CALL rd_i(imodes); CALL rd_i(iwmk); CALL rd_i(i_sw)
 CLOSE(unit_in)
-IF (.NOT. ANY(icase == [1, 2, 3, 31, 32])) &
-   ERROR STOP 'this driver builds icase = 1, 2, 3, 31 and 32 (icase 4: use the Python host)'
+IF (.NOT. ANY(icase == [1, 2, 3, 31, 32, 4, 41])) &
+   ERROR STOP 'this driver builds icase = 1, 2, 3, 31, 32, 4 and 41'
 IF (ibat /= 2 .AND. ibat /= 3) ERROR STOP 'ibat = 2 (front) or 3 (Dalrymple)'
 ! adjustments made while reading (IO/input.f90:163-172)
 islosh = islosh + 1
@@ -130,7 +130,9 @@ END IF
 nh = 1
 ! Ocean .txt files become <file_name>.cfg/.dat first (initial_condition.f90:180-181)
 IF (icase == 31 .OR. icase == 32) CALL ocean_txt2cfg_dat()
-IF (icase == 3 .OR. icase == 31 .OR. icase == 32) THEN
+IF (icase == 4 .OR. icase == 41) THEN
+   CALL spectrum2wave()
+ELSE IF (icase == 3 .OR. icase == 31 .OR. icase == 32) THEN
    CALL file2wave()
    ! Dalrymple's method per component in a 3-D basin (initial_condition.f90:192-199)
    IF (ibat == 3 .AND. n2 /= 1) CALL dalrymple_expand(xd_mono)
@@ -168,7 +170,8 @@ IF (icase == 2 .AND. ibat == 2) THEN
       dyim(i2) = AIMAG(ctmp)
    END DO
    freq(1) = freq1
-ELSE IF (icase == 3 .OR. icase == 31 .OR. icase == 32 .OR. (icase == 2 .AND. ibat == 3)) THEN
+ELSE IF (icase == 3 .OR. icase == 31 .OR. icase == 32 .OR. icase == 4 .OR. icase == 41 .OR. &
+         (icase == 2 .AND. ibat == 3)) THEN
    CALL wmk_ini()
 END IF
 !
@@ -180,7 +183,7 @@ p%xlen = xlen; p%ylen = ylen; p%d_hinge = d_hinge; p%Tramp = Tramp; p%T_stop = T
 p%nu = C_LOC(nu)
 p%n_freq = 0
 IF (icase == 2) p%n_freq = 1
-IF (icase == 3 .OR. icase == 31 .OR. icase == 32) p%n_freq = nh
+IF (icase == 3 .OR. icase == 31 .OR. icase == 32 .OR. icase == 4 .OR. icase == 41) p%n_freq = nh
 p%freq = C_LOC(freq); p%pos_ini_re = C_LOC(pre); p%pos_ini_im = C_LOC(pim)
 p%dposdy_ini_re = C_LOC(dyre); p%dposdy_ini_im = C_LOC(dyim); p%nu_x = C_LOC(nux)
 CALL hos_check(hos_create(p, ctx), 'hos_create')
@@ -590,6 +593,81 @@ SUBROUTINE file2wave()
    END DO
    IF (nh == 0) ERROR STOP 'no wave harmonic below the maximum frequency'
 END SUBROUTINE file2wave
+!
+! Long-crested irregular waves of a JONSWAP (icase 4) or Bretschneider (icase 41) spectrum
+! (initialize_wav_cfg, main/initial_condition.f90:212-308): the components the reference
+! builds in memory instead of reading a .cfg / .dat pair.  Clock 32 Hz, rnum =
+! CEILING(log2(T_stop clock)) with T_stop in seconds; component i = 1 .. NINT(2 2^rnum /
+! clock + 1) has harmonic i - 1 and frequency i f_base (f_base = clock / 2^rnum), as there;
+! S(f) df with sigma = 0.07 below 1/Tp and 0.09 above (single-precision literals, as there)
+! and amplitudes sqrt(2 S df) rescaled so that 4 sqrt(sum S df) = Hs.  Phases: 2 pi u with
+! u = (x - 1) / (2^31 - 2) from the Park-Miller minimal-standard sequence x <- 16807 x
+! mod (2^31 - 1), x_0 = iseed (1 when iseed mod (2^31 - 1) is 0).  The reference draws them
+! with gfortran's SRAND / RAND, which this build does not run; hosnwt/config.py
+! (spectrum_components) draws the same sequence as this driver.
+SUBROUTINE spectrum2wave()
+   INTEGER :: n_pts, j, ig
+   INTEGER(8) :: x
+   REAL(RP) :: f_base, sigma, hs_int, s
+   clock_f = 32.0_rp
+   rnum_f = CEILING(LOG10(T_stop * Tscale * clock_f) / LOG10(2.0_rp))
+   depth_f = h
+   ly_f = ylen * h
+   hinge_f = d_hinge * h
+   typ_wmk_f = 'piston'
+   IF (igeom == 2) typ_wmk_f = 'hinged'
+   n_pts = NINT(2 * 2**rnum_f / clock_f + 1)
+   f_base = clock_f / 2.0_rp**rnum_f
+   nc = n_pts
+   ALLOCATE(c_harmo(nc), c_f(nc), c_a(nc), c_ang(nc), c_ph(nc), c_k(nc), c_tf(nc))
+   x = MODULO(INT(iseed, 8), 2147483647_8)
+   IF (x == 0) x = 1
+   hs_int = 0.0_rp
+   DO j = 1, nc
+      c_harmo(j) = j - 1
+      c_f(j) = j * f_base
+      c_ang(j) = 0.0_rp
+      x = MODULO(16807_8 * x, 2147483647_8)
+      c_ph(j) = TWOPI * REAL(x - 1, RP) / 2147483646.0_rp
+      IF (c_f(j) < 1.0_rp / Tp) THEN
+         sigma = 0.07
+      ELSE
+         sigma = 0.09
+      END IF
+      IF (icase == 4) THEN
+         s = 1.0_rp / (TWOPI**4) * G**2 / c_f(j)**5 * EXP(-5.0_rp / 4.0_rp * (c_f(j) * Tp)**(-4)) &
+             * gamma**(EXP((-(c_f(j) - 1.0_rp / Tp)**2) / (2.0_rp * (sigma / Tp)**2))) * f_base
+      ELSE
+         s = 5.0_rp / 16.0_rp / (Tp**4 * c_f(j)**5) * Hs * EXP(-5.0_rp / 4.0_rp * (c_f(j) * Tp)**(-4)) * f_base
+      END IF
+      hs_int = hs_int + s
+      c_a(j) = SQRT(2.0_rp * s)
+   END DO
+   hs_int = 4.0_rp * SQRT(hs_int)
+   WRITE(*,'(A,ES12.5,A,ES12.5,A,ES12.5)') 'Hs target =', Hs, ' Tp =', Tp, ' Hs_calc =', hs_int
+   c_a = c_a * Hs / hs_int
+   ! nondimensional (initial_condition.f90:288-299)
+   c_f = c_f * Tscale
+   c_a = c_a / h
+   clock_f = clock_f * Tscale
+   ly_f = ly_f / h
+   hinge_f = hinge_f / h
+   ig = 1
+   IF (TRIM(typ_wmk_f) == 'hinged') ig = 2
+   DO j = 1, nc
+      c_k(j) = wave_number(c_f(j))
+      c_tf(j) = CMPLX(1.0_rp, 0.0_rp, RP)
+      IF (ABS(c_k(j)) > TINY) c_tf(j) = transfer(ig, c_k(j), hinge_f)
+      IF (c_f(j) < 1.0E-8_rp) THEN
+         c_k(j) = 0.0_rp
+         c_tf(j) = CMPLX(1.0_rp, 0.0_rp, RP)
+      END IF
+   END DO
+   nh = 0
+   DO j = 0, 4 * (2**rnum_f / MAX(1, FLOOR(clock_f * SQRT(G / depth_f))))
+      IF (ANY(c_harmo == j)) nh = nh + 1
+   END DO
+END SUBROUTINE spectrum2wave
 !
 ! Ocean .txt (ocean_txt2cfg_dat, wave/read_ocean_txt.f90:33-228): the run's rnumber, ramp,
 ! optional rnumber compression and its list of fronts (harmonic, frequency, amplitude,

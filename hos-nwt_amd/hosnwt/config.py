@@ -124,6 +124,11 @@ class TankConfig:
     coeffilt: tuple = (1.0, 1.0, 1.0)
     # icase = 3: components in SI (freq Hz, ampli m, angle deg, phase rad)
     components: dict | None = None
+    # icase = 4 (JONSWAP) / 41 (Bretschneider): spectrum built in memory (input.f90:62-133 order)
+    Hs: float = 0.05
+    Tp: float = 1.0
+    gamma: float = 3.3
+    iseed: int = 1
 
 
 @dataclass
@@ -183,8 +188,9 @@ def build(cfg: TankConfig) -> Setup:
         waves = WaveComponents(harmo=np.array([1]), freq=np.array([nu_mono]),
                                ampli=np.array([cfg.amp_mono / h]), angle=np.array([theta]),
                                phase=np.array([cfg.ph_mono]), rnum=1, clock=2.0 * nu_mono).init(typ, d_hinge)
-    elif cfg.icase == 3:
-        c = cfg.components
+    elif cfg.icase in (3, 4, 41):
+        c = cfg.components if cfg.icase == 3 else spectrum_components(
+            cfg.icase, cfg.Hs, cfg.Tp, cfg.gamma, cfg.iseed, cfg.T_stop)
         waves = WaveComponents(harmo=np.asarray(c["harmo"], int), freq=np.asarray(c["freq"], float) * Tscale,
                                ampli=np.asarray(c["ampli"], float) / h,
                                angle=np.asarray(c["angle"], float) * math.pi / 180.0,
@@ -195,8 +201,9 @@ def build(cfg: TankConfig) -> Setup:
     cy = np.arange(n2) * dely
     if waves is not None:
         hmax = None
-        if cfg.icase == 3:
-            hmax = 4 * (2 ** int(cfg.components["rnum"]) // max(1, math.floor(float(cfg.components["clock"]))))
+        if cfg.icase in (3, 4, 41):
+            # 4 n_max, n_max = 2^rnum / floor(clock sqrt(g / h)) with the nondimensional clock
+            hmax = 4 * (2 ** int(waves.rnum) // max(1, math.floor(waves.clock * math.sqrt(G / h))))
         freq, pos_ini, dposdy_ini = wmk_ini(waves, cy, hmax)
     else:
         freq, pos_ini = np.zeros(0), np.zeros((n2, 0), complex)
@@ -275,6 +282,47 @@ def dt_stab(n1, n2, n3, xlen, ylen, icase) -> float:
     if n2 == 1:
         return min(d1, d3)
     return min(d1, d2, d3)
+
+
+def park_miller(seed: int, n: int) -> np.ndarray:
+    """n uniforms (x - 1) / (2^31 - 2) of the Park-Miller minimal-standard sequence
+    x <- 16807 x mod (2^31 - 1) (Park & Miller, CACM 31, 1988), x_0 = seed (1 when
+    seed mod (2^31 - 1) is 0)."""
+    m = 2147483647
+    x = seed % m or 1
+    out = np.empty(n)
+    for i in range(n):
+        x = (16807 * x) % m
+        out[i] = (x - 1) / (m - 1)
+    return out
+
+
+def spectrum_components(icase: int, Hs: float, Tp: float, gamma: float, iseed: int, T_stop: float,
+                        clock: float = 32.0):
+    """Long-crested irregular components of icase 4 (JONSWAP) / 41 (Bretschneider) as
+    initialize_wav_cfg builds them (initial_condition.f90:212-308), in SI units for
+    ``build``: rnum = ceil(log2(T_stop clock)) with T_stop in seconds, components
+    i = 1 .. round(2 2^rnum / clock + 1) with harmonic i - 1 and frequency i clock / 2^rnum,
+    S(f) df with the single-precision sigma literals 0.07 / 0.09, amplitudes sqrt(2 S df)
+    rescaled to Hs.  Phases 2 pi u from ``park_miller(iseed)``: the reference draws them with
+    gfortran's SRAND / RAND; the Fortran driver draws this same sequence."""
+    if icase not in (4, 41):
+        raise ValueError("icase 4 (JONSWAP) or 41 (Bretschneider)")
+    rnum = math.ceil(math.log10(T_stop * clock) / math.log10(2.0))
+    n_pts = int(np.rint(2 * 2 ** rnum / clock + 1))
+    f_base = clock / 2.0 ** rnum
+    i = np.arange(1, n_pts + 1)
+    f = i * f_base
+    sig = np.where(f < 1.0 / Tp, float(np.float32(0.07)), float(np.float32(0.09)))
+    if icase == 4:
+        S = (1.0 / TWOPI ** 4 * G ** 2 / f ** 5 * np.exp(-5.0 / 4.0 * (f * Tp) ** -4)
+             * gamma ** np.exp(-((f - 1.0 / Tp) ** 2) / (2.0 * (sig / Tp) ** 2)) * f_base)
+    else:
+        S = 5.0 / 16.0 / (Tp ** 4 * f ** 5) * Hs * np.exp(-5.0 / 4.0 * (f * Tp) ** -4) * f_base
+    Hs_calc = 4.0 * math.sqrt(S.sum())
+    amp = np.sqrt(2.0 * S) * Hs / Hs_calc
+    return dict(harmo=i - 1, freq=f, ampli=amp, angle=np.zeros(n_pts), phase=TWOPI * park_miller(iseed, n_pts),
+                rnum=rnum, clock=clock)
 
 
 def jonswap_components(Hs: float, Tp: float, gamma: float, n_freq: int, seed: int, rnum: int = 10,

@@ -233,3 +233,50 @@ def test_fortran_driver_reads_wave_files(tmp_path):
     scale = np.abs(ref[:, 3]).max()
     assert scale > 0
     np.testing.assert_allclose(got[:, 2:5], ref[:, 1:4], rtol=0, atol=2e-4 * scale)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("icase", [4, 41])
+def test_fortran_driver_builds_internal_spectra(tmp_path, icase):
+    """icase = 4 (JONSWAP) / 41 (Bretschneider): the driver builds the components in memory
+    (initialize_wav_cfg, initial_condition.f90:212-308) and runs to the oracle's volume /
+    energy trace built from hosnwt.config.spectrum_components (the same discretisation and
+    the same phase sequence)."""
+    assert os.path.exists(EXE), "build the Fortran driver with hos-nwt_amd/build.sh"
+    n1, M, tstop, T = 64, 3, 2.0, 1.5
+    d_hinge = 0.3
+    comps = hc.spectrum_components(icase, 0.05, 1.0, 3.3, 513, tstop)  # Hs, Tp, gamma, iseed of INPUT
+    print(icase, "components:", len(comps["freq"]), "rnum", comps["rnum"], flush=True)
+    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T, igeom=2,
+                   d_hinge=d_hinge, components=comps, icase=3)
+
+    def num(x):
+        return f"{x:.16e}"
+
+    text = INPUT.format(amp=num(0.0), nu=num(1.0 / T), tstop=num(tstop), fout=num(2.0 / T), i_wmk=1,
+                        toler=num(1e-7), iprobes=0, flag=0, icase=icase, file_name="none", igeom=2,
+                        d_hinge=num(d_hinge))
+    (tmp_path / "input_HOS-NWT.dat").write_text(text)
+    r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "Hs_calc" in r.stdout
+    got = np.array(data_rows(tmp_path / "Results" / "vol_energy.dat"))
+    orc = ho.Oracle(oracle_params(s))
+    st = orc.initialize()
+    deta = np.zeros_like(st.eta)
+    t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+    ref = []
+    while t < s.T_stop + s.delt_out:
+        ep, ek, et = orc.energy(st.eta, st.phis + orc.phi_add, deta)
+        ref.append((t, ep, ek, et))
+        if abs(t - s.T_stop) <= 2.2e-16:
+            break
+        tn = min(t + s.delt_out, s.T_stop + s.delt_out)
+        st, t, h, _, _, d = ho.advance(orc, st, t, tn, h, s.cfg.toler, s.dt_stab, s.delt_out)
+        deta = d if d is not None else deta
+    ref = np.array(ref)
+    assert got.shape[0] == ref.shape[0]
+    scale = np.abs(ref[:, 3]).max()
+    assert scale > 0
+    np.testing.assert_allclose(got[:, 2:5], ref[:, 1:4], rtol=0, atol=2e-4 * scale)
