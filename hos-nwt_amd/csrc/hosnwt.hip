@@ -1735,6 +1735,101 @@ __global__ void k_diff_reduce(int nb, const double* __restrict__ part, double* _
 }
 
 // ---------------------------------------------------------------------------
+// dipole wavemaker (igeom = 3): phi_dipoles (resol_wmkr.f90:1492-1647)
+// ---------------------------------------------------------------------------
+// Components sorted by dipole depth: those of depth z[c] are [start[c], start[c+1]).
+// m holds mu_x re, mu_x im, mu_z re, mu_z im, each nf long; w = 2 pi f; cs the cosine
+// and sine of w t of the evaluation (k_dip_phase).
+constexpr int kDipMaxDepth = 8;
+struct DipSet {
+  int nf = 0, nz = 0, n_side = 0;
+  double x_dip = 0.0, xlen = 0.0, dx = 0.0;
+  int start[kDipMaxDepth + 1] = {};
+  double z[kDipMaxDepth] = {};
+  const double* w = nullptr;
+  const double* m = nullptr;
+  double* cs = nullptr;
+};
+
+// e^{i w t} of every component, as the reference forms EXP(i TWOPI freq time) (:1628)
+__global__ void k_dip_phase(int nf, double t, const double* __restrict__ w, double* __restrict__ cs) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  double s, c;
+  sincos(w[f] * t, &s, &c);
+  cs[f] = c;
+  cs[nf + f] = s;
+}
+
+// One base-grid node per thread at z = eta.  Each image term is linear in the moments:
+// with r2 = 1 / (dx^2 + dz^2), sg the x-moment sign of the image (-1 for odd wall
+// images) and up = +1 (dipole level) or -1 (bottom image, z moment mirrored),
+//   resu    += sg mu_x r2 dx + up mu_z r2 dz
+//   resu_dx += sg mu_x r2 (1 - 2 dx^2 r2) - 2 up mu_z dx dz r2^2
+//   resu_dz += -2 sg mu_x dx dz r2^2 + up mu_z r2 (1 - 2 dz^2 r2)
+// so the six real image sums of a depth are formed once and every component of that
+// depth adds Re(resu e^{i w t}) and the like.  phi_t = alpha' phi + alpha dphi/dt with
+// dphi/dt = i w phi; phiy is not formed by the reference (zero); beta = 1, beta_x = 0.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dipoles(int n1, int ns, DipSet d, double al, double alt,
+                                                 const T* __restrict__ eta, double* __restrict__ phi,
+                                                 double* __restrict__ phit, double* __restrict__ phix,
+                                                 double* __restrict__ phiy, double* __restrict__ phiz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ns) return;
+  const double x = (double)(i % n1) * d.dx;
+  const double e = (double)eta[i];
+  const int nf = d.nf;
+  double P = 0.0, Pt = 0.0, Px = 0.0, Pz = 0.0;
+  for (int c = 0; c < d.nz; ++c) {
+    const double zu = e - d.z[c], zd = e - (-2.0 - d.z[c]);
+    double A1 = 0.0, A2 = 0.0, B1 = 0.0, B2 = 0.0, C1 = 0.0, C2 = 0.0;
+    auto term = [&](double ddx, double sg, double dz, double up) {
+      const double r2 = 1.0 / (ddx * ddx + dz * dz);
+      A1 += sg * r2 * ddx;
+      A2 += up * r2 * dz;
+      B1 += sg * r2 * (1.0 - 2.0 * ddx * ddx * r2);
+      B2 += -2.0 * ddx * r2 * (up * r2 * dz);
+      C1 += -2.0 * dz * r2 * (sg * r2 * ddx);
+      C2 += up * r2 * (1.0 - 2.0 * dz * dz * r2);
+    };
+    term(x - d.x_dip, 1.0, zu, 1.0);
+    term(x - d.x_dip, 1.0, zd, -1.0);
+    for (int j = 1; j <= d.n_side; ++j) {
+      // image abscissae of dipoles_ini (resol_wmkr.f90:1466-1481)
+      const double right = (j % 2 == 0) ? d.x_dip + d.xlen * j : -d.x_dip + d.xlen * (j + 1);
+      const double left = (j % 2 == 0) ? d.x_dip - d.xlen * j : -d.x_dip - d.xlen * (j - 1);
+      const double sg = (j % 2 == 1) ? -1.0 : 1.0;
+      term(x - left, sg, zu, 1.0);
+      term(x - right, sg, zu, 1.0);
+      term(x - right, sg, zd, -1.0);
+      term(x - left, sg, zd, -1.0);
+    }
+    for (int f = d.start[c]; f < d.start[c + 1]; ++f) {
+      const double m1r = d.m[f], m1i = d.m[nf + f], m2r = d.m[2 * nf + f], m2i = d.m[3 * nf + f];
+      const double co = d.cs[f], si = d.cs[nf + f];
+      const double rr = m1r * A1 + m2r * A2, ri = m1i * A1 + m2i * A2;
+      P += rr * co - ri * si;
+      Pt -= d.w[f] * (rr * si + ri * co);
+      const double xr = m1r * B1 + m2r * B2, xi = m1i * B1 + m2i * B2;
+      Px += xr * co - xi * si;
+      const double zr = m1r * C1 + m2r * C2, zi = m1i * C1 + m2i * C2;
+      Pz += zr * co - zi * si;
+    }
+  }
+  const double tp = 2.0 * M_PI;
+  P /= tp;
+  Pt /= tp;
+  Px /= tp;
+  Pz /= tp;
+  phi[i] = P * al;
+  phit[i] = P * alt + Pt * al;
+  phix[i] = Px * al;
+  phiz[i] = Pz * al;
+  phiy[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 struct DevBuf {
@@ -2285,6 +2380,11 @@ struct hos_ctx {
   int n1, n2, n3, n3a, M, N1b, N2b;
   double pix, piy, dx, dy, delz;
   bool wmk;
+  // dipole wavemaker (igeom = 3): the components' moments and depths on the device
+  bool dip = false;
+  DipSet dips{};
+  // an additional potential phi_add enters the free-surface conditions
+  bool add_pot() const { return wmk || dip; }
   hipStream_t st = nullptr;
   Arena ar;
   Grid2<double> gs64;
@@ -2611,7 +2711,7 @@ struct Engine : EngineBase {
     const int Nd1 = ((p1 + 1) * n1) / 2, Nd2 = ((p2 + 1) * n2) / 2;
     XA = ar.get<T>(nb); YA = ar.get<T>(nb);
     XB = YB = PB = RB = T2 = nullptr;
-    if (M >= 2 && c->wmk) {
+    if (M >= 2 && c->add_pot()) {
       if (M < 3) throw std::runtime_error("the wavemaker coupling needs mHOS >= 3 (or mHOS = 1)");
       K1 = std::min(2 * n1, N1b);
       PA = ar.get<T>((size_t)K1 * 2 * n2);
@@ -2935,12 +3035,12 @@ struct Engine : EngineBase {
                            phixa, phiya, phiza, X, Y, XA, YA, XB, YB, j0, zsum);
         b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[1]);
         b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[2]);
-        if (C.wmk) mixed_parity_spectra(b);
+        if (C.add_pot()) mixed_parity_spectra(b);
       }
       bigB(FWD, b);
     }
     if (M >= 2) {
-      if (C.wmk)
+      if (C.add_pot())
         mixed_parity_finish();  // with Q = D1 + P
       else
         hipLaunchKernelGGL(k_add3<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)D1, (const T*)P, Q);
@@ -3026,6 +3126,10 @@ struct Engine : EngineBase {
     hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2, n3 = C.n3, n3a = C.n3a;
     const int ldb = N1b;
+    if (C.dip) {
+      dipole_wavemaker(t, ka1, ke1, kad1, kadr);
+      return;
+    }
     if (!C.wmk) {
       HIP_OK(hipMemsetAsync(phi_add, 0, ns * sizeof(double), st));
       HIP_OK(hipMemsetAsync(phix_add, 0, ns * sizeof(double), st));
@@ -3079,6 +3183,40 @@ struct Engine : EngineBase {
       b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
       bigB(INV, b);
     }
+  }
+
+  // Dipole wavemaker (igeom = 3): the ramp, then phi_dipoles at z = eta
+  // (resol_HOS.f90:494-496).  There is no paddle state: the first-order free-surface
+  // and paddle-mode derivatives stay zero, and phiy_add is zero (the reference never
+  // sets it on this path).  grad phi_add goes to the extended grid as for the paddle.
+  void dipole_wavemaker(double t, double* ka1, double* ke1, double* kad1, double* kadr) {
+    hos_ctx& C = *c;
+    hipStream_t st = cs;
+    const int n1 = C.n1, n2 = C.n2;
+    const int ldb = N1b;
+    if (C.dsc_active) throw std::runtime_error("graph-captured attempts need a paddle wavemaker");
+    double al, alt, altt;
+    ramp_eval(C.p, t, al, alt, altt);
+    C.last_alpha = al;
+    C.last_alphat = alt;
+    C.last_pos_top = 0.0;
+    const DipSet& d = C.dips;
+    hipLaunchKernelGGL(k_dip_phase, dim3(nblk(d.nf)), dim3(256), 0, st, d.nf, t, d.w, d.cs);
+    hipLaunchKernelGGL((k_dipoles<T>), dim3(nblk(ns)), dim3(256), 0, st, n1, (int)ns, d, al, alt,
+                       (const T*)eta_phys, phi_add, phit_add, phix_add, phiy_add, phiz_add);
+    HIP_OK(hipGetLastError());
+    if (ka1) HIP_OK(hipMemsetAsync(ka1, 0, ns * sizeof(double), st));
+    if (ke1) HIP_OK(hipMemsetAsync(ke1, 0, ns * sizeof(double), st));
+    if (kad1) HIP_OK(hipMemsetAsync(kad1, 0, na * sizeof(double), st));
+    if (kadr) HIP_OK(hipMemsetAsync(kadr, 0, na * sizeof(double), st));
+    base(KCOS, KCOS, FWD, src_plain<T, double>(phix_add, n1, n1, n2), n1, n2, Dst<T>{phixh, n1, 0});
+    HIP_OK(hipMemsetAsync(phiyh, 0, ns * sizeof(T), st));
+    HIP_OK(hipMemsetAsync(phiya, 0, nb * sizeof(T), st));
+    surface_forcing_modes();
+    Batch2<T, Src<T, T>, Dst<T>> b;
+    b.add(KCOS, KCOS, src_plain<T>(phixh, n1, n1, n2), N1b, N2b, Dst<T>{phixa, ldb, 0}, (*ctmpX)[0]);
+    b.add(KCOS, KCOS, src_plain<T>(phizh, n1, n1, n2), N1b, N2b, Dst<T>{phiza, ldb, 0}, (*ctmpX)[2]);
+    bigB(INV, b);
   }
 
   // First-order free-surface RHS (phidphiz) and the modes of grad phi_add, one
@@ -3497,7 +3635,7 @@ namespace {
 // m >= 3 (or m = 1).
 void select_order(hos_ctx* c, int m) {
   if (m < 1 || m > c->p.mHOS) throw std::runtime_error("order outside 1..mHOS");
-  if (c->wmk && m == 2) throw std::runtime_error("the wavemaker coupling needs an order >= 3 (or 1)");
+  if (c->add_pot() && m == 2) throw std::runtime_error("the wavemaker coupling needs an order >= 3 (or 1)");
   if (!c->engines[m]) {
     if (c->p.precision == 0)
       c->engines[m] = std::make_unique<Engine<double>>(c, m);
@@ -3510,6 +3648,63 @@ void select_order(hos_ctx* c, int m) {
   order_grid(c->p, m, q1, q2, c->N1b, c->N2b);
   c->stats.big_n1 = c->N1b;
   c->stats.big_n2 = c->N2b;
+}
+
+// Device tables of the dipole wavemaker: the components grouped by dipole depth (the
+// image sums depend on the depth only), each group in the caller's order.
+void dipoles_setup(hos_ctx* c, const hos_params& p) {
+  const int nf = p.n_dip;
+  if (nf <= 0 || !p.dip_freq || !p.dip_mux_re || !p.dip_mux_im || !p.dip_muz_re || !p.dip_muz_im || !p.dip_z)
+    throw std::runtime_error("igeom = 3 needs the dipole components (n_dip > 0 and the dip_* arrays)");
+  if (p.n_side < 0 || p.n_side > 64) throw std::runtime_error("n_side must be in 0..64");
+  std::vector<double> zs;
+  std::vector<int> cls(nf);
+  for (int f = 0; f < nf; ++f) {
+    const double v[5] = {p.dip_freq[f], p.dip_mux_re[f], p.dip_mux_im[f], p.dip_muz_re[f], p.dip_muz_im[f]};
+    for (double x : v)
+      if (!std::isfinite(x))
+        throw std::runtime_error("dipole component " + std::to_string(f + 1) +
+                                 " is not finite (the dipole transfer function of linear_wave.f90:316-327 "
+                                 "divides by k^2 - (k tanh k)^2, zero in FP64 for 18.7 < k <= 20)");
+    if (!(p.dip_z[f] > -1.0 && p.dip_z[f] < 0.0)) throw std::runtime_error("dipole depth must lie in (-1, 0)");
+    size_t q = 0;
+    while (q < zs.size() && zs[q] != p.dip_z[f]) ++q;
+    if (q == zs.size()) {
+      if ((int)zs.size() == kDipMaxDepth) throw std::runtime_error("more than 8 distinct dipole depths");
+      zs.push_back(p.dip_z[f]);
+    }
+    cls[f] = (int)q;
+  }
+  DipSet& d = c->dips;
+  d.nf = nf;
+  d.nz = (int)zs.size();
+  d.n_side = p.n_side;
+  d.x_dip = p.x_dip;
+  d.xlen = p.xlen;
+  d.dx = c->dx;
+  std::vector<double> w, m(4 * (size_t)nf);
+  int o = 0;
+  for (int q = 0; q < d.nz; ++q) {
+    d.z[q] = zs[q];
+    d.start[q] = o;
+    for (int f = 0; f < nf; ++f) {
+      if (cls[f] != q) continue;
+      w.push_back(2.0 * M_PI * p.dip_freq[f]);
+      m[o] = p.dip_mux_re[f];
+      m[nf + o] = p.dip_mux_im[f];
+      m[2 * nf + o] = p.dip_muz_re[f];
+      m[3 * nf + o] = p.dip_muz_im[f];
+      ++o;
+    }
+  }
+  d.start[d.nz] = o;
+  double* wd = c->ar.get<double>(nf);
+  double* md = c->ar.get<double>(4 * (size_t)nf);
+  d.cs = c->ar.get<double>(2 * (size_t)nf);
+  HIP_OK(hipMemcpy(wd, w.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(md, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+  d.w = wd;
+  d.m = md;
 }
 
 void check_params(const hos_params& p) {
@@ -3926,7 +4121,7 @@ void learn_prepare(hos_ctx* c, const hos_order_policy& pol) {
   HIP_OK(hipStreamSynchronize(c->st));
   const int cur = c->M;
   for (int m = std::max(pol.m_min, 1); m <= c->p.mHOS; ++m) {
-    if (c->wmk && m == 2) continue;
+    if (c->add_pot() && m == 2) continue;
     select_order(c, m);  // every engine the selection may use, allocated now
   }
   select_order(c, cur);
@@ -3995,7 +4190,7 @@ int choose_order(hos_ctx* c, const hos_order_policy& pol, bool have_net, double*
     if (L.kappa * r <= thr) break;
     ++q;
   }
-  if (c->wmk && q == 2) q = 3;
+  if (c->add_pot() && q == 2) q = 3;
   q = std::min(q, Mx);
   *rate_used = (q < Mx) ? r : 0.0;
   return q;
@@ -4034,7 +4229,7 @@ void learn_launch(LearnMember& mb, double t_next, const hos_mlp* net, const hos_
   mb.final_attempt = !(mb.a.t + hl < t_next);
   int m = (L.m == 0) ? c->p.mHOS : L.m;
   m = std::max(m, std::max(pol.m_min, L.floor));
-  if (c->wmk && m == 2) m = 3;
+  if (c->add_pot() && m == 2) m = 3;
   m = std::min(m, c->p.mHOS);
   L.hl = hl;
   L.phase = 0;
@@ -4201,9 +4396,10 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     c->dx = p.xlen / (p.n1 - 1);
     c->dy = p.n2 > 1 ? p.ylen / (p.n2 - 1) : 0.0;
     c->delz = 1.0 / (p.n3 - 1);
-    c->wmk = (p.icase == 2 || p.icase == 3 || p.icase == 31 || p.icase == 32 || p.icase == 33 || p.icase == 4 ||
-              p.icase == 41 || p.icase == 42) &&
-             (p.igeom == 1 || p.igeom == 2);
+    const bool wave_case = p.icase == 2 || p.icase == 3 || p.icase == 31 || p.icase == 32 || p.icase == 33 ||
+                           p.icase == 4 || p.icase == 41 || p.icase == 42;
+    c->wmk = wave_case && (p.igeom == 1 || p.igeom == 2);
+    c->dip = wave_case && p.igeom == 3;
     c->stats.big_n1 = c->N1b;
     c->stats.big_n2 = c->N2b;
     HIP_OK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
@@ -4287,10 +4483,12 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
       if (p.nu_x) c->h_nux.assign(p.nu_x, p.nu_x + ns);
       if (p.nu_y) c->h_nuy.assign(p.nu_y, p.nu_y + ns);
     }
+    if (c->dip) dipoles_setup(c.get(), p);
     select_order(c.get(), p.mHOS);
     c->p.freq = nullptr;
     c->p.pos_ini_re = c->p.pos_ini_im = nullptr;
     c->p.dposdy_ini_re = c->p.dposdy_ini_im = c->p.nu_x = c->p.nu_y = nullptr;
+    c->p.dip_freq = c->p.dip_mux_re = c->p.dip_mux_im = c->p.dip_muz_re = c->p.dip_muz_im = c->p.dip_z = nullptr;
     HIP_OK(hipDeviceSynchronize());
     *out = c.release();
   });
@@ -4333,6 +4531,14 @@ int hos_initialize(hos_ctx* c) {
       c->last_alpha = 0.0;
       c->last_alphat = 0.0;
       c->last_pos_top = 0.0;
+      if (c->dip) {
+        // the reference's solveHOS at t = 0 (initial_condition.f90:467-478) sets the
+        // dipole potential of the rest state; one right-hand side at t = 0 does the same
+        hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
+                           (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
+        rhs_eval(c, 0.0, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add,
+                 c->a3rd_add);
+      }
       HIP_OK(hipStreamSynchronize(st));
       return;
     }

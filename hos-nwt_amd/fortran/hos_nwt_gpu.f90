@@ -45,6 +45,12 @@ CHARACTER(LEN=16) :: typ_wmk_f
 REAL(RP), ALLOCATABLE :: xprobe(:), yprobe(:), eta_probe(:), etam(:,:), phism(:,:), xtop(:), vtop(:)
 REAL(RP), ALLOCATABLE, TARGET :: nu(:,:), eta(:,:), phis(:,:), deta(:,:), phi_add(:,:), zero2(:,:), zadd(:,:)
 REAL(RP), ALLOCATABLE, TARGET :: freq(:), pre(:), pim(:), dyre(:), dyim(:), nux(:,:)
+! dipole wavemaker (igeom = 3): per component frequency, mu(:,1), mu(:,2), mu(:,3)
+REAL(RP), ALLOCATABLE, TARGET :: dfr(:), dmxr(:), dmxi(:), dmzr(:), dmzi(:), dzd(:)
+! dipole constants (IO/input.f90:137-139; x_dip / h at HOS-NWT.f90:111)
+REAL(RP), PARAMETER :: z_dipmono = -0.25_rp, x_dip_si = 2.5_rp
+INTEGER, PARAMETER :: n_side = 6
+REAL(RP) :: x_dip
 TYPE(hos_params) :: p
 TYPE(C_PTR) :: ctx
 INTEGER(C_INT) :: n_acc, n_rej, fx, fy
@@ -95,6 +101,7 @@ Tscale = 1.0_rp / SQRT(G / h)
 xlen = xlen / h; ylen = ylen / h; amp_mono = amp_mono / h; d_hinge = d_hinge / h; aslosh = aslosh / h
 Tramp = Tramp / Tscale; T_stop = T_stop / Tscale; f_out = f_out * Tscale; nu_mono = nu_mono * Tscale
 w_mono = nu_mono * TWOPI; T_mono = 1.0_rp / nu_mono
+x_dip = x_dip_si / h
 IF (n2 == 1) ylen = 1.0_rp
 !
 ! RK4 stability bound (HOS-NWT.f90:132-196)
@@ -103,7 +110,7 @@ d1 = c8 * xlen / REAL(n1 - 1, RP)
 d2 = d1 + 1.0_rp
 IF (n2 /= 1) d2 = c8 * ylen / REAL(n2 - 1, RP)
 d3 = c8 * 4.0_rp / REAL(2 * (n3 - 1), RP)
-IF (icase == 1) THEN
+IF (icase == 1 .OR. igeom == 3) THEN
    dt_stab = MIN(d1, d2)
 ELSE
    dt_stab = MIN(d1, d2, d3)
@@ -154,7 +161,17 @@ IF (iabsnb /= 0 .AND. ABS(xabsf - 1.0_rp) > TINY) THEN
       IF ((i1 - 1) * delx >= x0) nux(i1,:) = coeffabsf * 2.0_rp * ((i1 - 1) * delx - x0) / x2
    END DO
 END IF
-IF (icase == 2 .AND. ibat == 2) THEN
+IF (igeom == 3 .AND. icase /= 1) THEN
+   ! dipoles instead of a paddle (initial_condition.f90:168-170, :204-205)
+   IF (icase == 2 .AND. ibat == 2) THEN
+      nc = 1
+      ALLOCATE(c_harmo(1), c_f(1), c_a(1), c_ang(1), c_ph(1), c_k(1), c_tf(1))
+      c_harmo = 1; c_f = nu_mono; c_a = amp_mono; c_ang = theta_mono; c_ph = ph_mono
+      c_k = wave_number(nu_mono)
+   END IF
+   k_w = c_k(1)
+   CALL dipoles_ini()
+ELSE IF (icase == 2 .AND. ibat == 2) THEN
    ! front(): one harmonic at nu_mono (wave_def.f90:54-80), wave_init and wmk_ini
    freq1 = nu_mono
    k_w = wave_number(freq1)
@@ -186,6 +203,13 @@ IF (icase == 2) p%n_freq = 1
 IF (icase == 3 .OR. icase == 31 .OR. icase == 32 .OR. icase == 4 .OR. icase == 41) p%n_freq = nh
 p%freq = C_LOC(freq); p%pos_ini_re = C_LOC(pre); p%pos_ini_im = C_LOC(pim)
 p%dposdy_ini_re = C_LOC(dyre); p%dposdy_ini_im = C_LOC(dyim); p%nu_x = C_LOC(nux)
+IF (igeom == 3 .AND. icase /= 1) THEN
+   p%n_freq = 0
+   p%n_dip = INT(nc, C_INT)
+   p%dip_freq = C_LOC(dfr); p%dip_mux_re = C_LOC(dmxr); p%dip_mux_im = C_LOC(dmxi)
+   p%dip_muz_re = C_LOC(dmzr); p%dip_muz_im = C_LOC(dmzi); p%dip_z = C_LOC(dzd)
+   p%x_dip = x_dip; p%n_side = n_side
+END IF
 CALL hos_check(hos_create(p, ctx), 'hos_create')
 IF (icase == 1) THEN
    DO i1 = 1, n1
@@ -215,7 +239,7 @@ DO WHILE (time_cur < T_stop + delt_out)
    IF (igeom == 2 .AND. icase /= 1) THEN
       WRITE(2,'(5(ES12.5,X))') time_cur * t_adim, voltot - 0.5_rp * alpha * (1 - d_hinge) * pos_top / xlen &
            - eta(1,1) * alpha * pos_top / xlen, enepot, enekin, enepot + enekin
-   ELSE IF (icase /= 1) THEN
+   ELSE IF (icase /= 1 .AND. igeom /= 3) THEN
       WRITE(2,'(5(ES12.5,X))') time_cur * t_adim, voltot - alpha * pos_top / xlen - eta(1,1) * alpha * pos_top / xlen, &
            enepot, enekin, enepot + enekin
    ELSE
@@ -477,6 +501,7 @@ SUBROUTINE output_time_step()
                IF (icase == 2) THEN
                   xx = (i1 - 1) * delx
                   yy = (i2 - 1) * dely
+                  IF (igeom == 3) xx = xx - x_dip  ! (output.f90:330-331)
                   eta_target = amp_mono * COS(w_mono * time_cur - k_w * (xx * COS(theta_mono) + yy * SIN(theta_mono)) &
                        + ph_mono)
                END IF
@@ -886,6 +911,44 @@ SUBROUTINE wmk_ini()
       freq(n) = c_f(jf)
    END DO
 END SUBROUTINE wmk_ini
+!
+! dipoles_ini (main/resol_wmkr.f90:1404-1455): per wave component the dipole depth from its
+! frequency, the dipole transfer function there, mu(:,2) = a TF exp(i(k x_dip + phase)),
+! mu(:,1) = -i mu(:,2) tanh(k (z + 1)); the wavenumbers are the dispersion relation's
+SUBROUTINE dipoles_ini()
+   INTEGER :: j
+   REAL(RP) :: z, k, k2, w2, w4
+   COMPLEX(RP) :: tfd, mz
+   ALLOCATE(dfr(nc), dmxr(nc), dmxi(nc), dmzr(nc), dmzi(nc), dzd(nc))
+   DO j = 1, nc
+      IF (c_f(j) > 1.0_rp) THEN
+         z = -0.35_rp
+      ELSE IF (c_f(j) > 0.6_rp) THEN
+         z = -0.2_rp
+      ELSE IF (c_f(j) > 0.5_rp) THEN
+         z = -0.4_rp
+      ELSE
+         z = -0.6_rp
+      END IF
+      IF (icase == 2) z = z_dipmono
+      k = c_k(j)
+      k2 = k * k
+      w2 = k * TANH(k)
+      ! TF_adim_new, 'dipole' (wave/linear_wave.f90:316-327)
+      IF (k <= 20.0_rp) THEN
+         w4 = w2 * w2
+         tfd = CMPLX((k2 - w4 + w2) / (2.0_rp * SQRT(w2) * (k2 - w4) * COSH(k) * SINH(k * (z + 1.0_rp))), 0.0_rp, RP)
+      ELSE
+         tfd = CMPLX(SQRT(w2) / (2.0_rp * EXP(k * z) * k2), 0.0_rp, RP)
+      END IF
+      mz = c_a(j) * tfd * EXP(CMPLX(0.0_rp, k * x_dip + c_ph(j), RP))
+      dfr(j) = c_f(j)
+      dmzr(j) = REAL(mz); dmzi(j) = AIMAG(mz)
+      mz = CMPLX(0.0_rp, -1.0_rp, RP) * mz * TANH(k * (z + 1.0_rp))
+      dmxr(j) = REAL(mz); dmxi(j) = AIMAG(mz)
+      dzd(j) = z
+   END DO
+END SUBROUTINE dipoles_ini
 !
 ! k with (2 pi f)^2 = k tanh k by bisection (linear_wave.f90:137-151, :427-466)
 REAL(RP) FUNCTION wave_number(f)

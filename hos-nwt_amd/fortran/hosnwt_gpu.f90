@@ -17,6 +17,12 @@ TYPE, BIND(C) :: hos_params
    TYPE(C_PTR)    :: freq, pos_ini_re, pos_ini_im
    REAL(C_DOUBLE) :: coeffilt(3) = 1.0_C_DOUBLE
    TYPE(C_PTR)    :: dposdy_ini_re = C_NULL_PTR, dposdy_ini_im = C_NULL_PTR, nu_x = C_NULL_PTR, nu_y = C_NULL_PTR
+   ! dipole wavemaker (igeom = 3): per component frequency, mu(:,1), mu(:,2), mu(:,3)
+   INTEGER(C_INT) :: n_dip = 0
+   TYPE(C_PTR)    :: dip_freq = C_NULL_PTR, dip_mux_re = C_NULL_PTR, dip_mux_im = C_NULL_PTR
+   TYPE(C_PTR)    :: dip_muz_re = C_NULL_PTR, dip_muz_im = C_NULL_PTR, dip_z = C_NULL_PTR
+   REAL(C_DOUBLE) :: x_dip = 2.5_C_DOUBLE
+   INTEGER(C_INT) :: n_side = 6
 END TYPE hos_params
 !
 INTERFACE

@@ -57,6 +57,18 @@ def transfer_function(typ: str, k: float, hinge: float = 0.0) -> complex:
         if k * (1.0 - d) >= 20:
             return -0.5j
         return -1j * k * (1.0 - d) / (2.0 * k * (1.0 - d) - 2.0 + 2.0 * math.exp(k * (d - 1.0)))
+    if typ == "dipole":
+        d = hinge
+        # d is the dipole depth; IEEE arithmetic as the reference's: k^2 - w^4 is 0 in FP64
+        # once tanh k rounds to 1 (18.7 < k <= 20) and the result is infinite there
+        kk = np.float64(k)
+        k2 = kk * kk
+        w2 = kk * np.tanh(kk)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            if kk <= 20.0:
+                w4 = w2 * w2
+                return complex((k2 - w4 + w2) / (2.0 * np.sqrt(w2) * (k2 - w4) * np.cosh(kk) * np.sinh(kk * (d + 1.0))))
+            return complex(np.sqrt(w2) / (2.0 * np.exp(kk * d) * k2))
     return 1.0 + 0j
 
 
@@ -75,14 +87,16 @@ class WaveComponents:
     clock: float = 1.0
 
     def init(self, typ: str, hinge: float):
-        """wave_init (wave_def.f90:282-329)."""
+        """wave_init (wave_def.f90:282-329).  For dipoles the transfer function is dipoles_ini's
+        (each component's own depth), so TF stays 1 here."""
         n = len(self.freq)
         self.k = np.zeros(n)
         self.TF = np.ones(n, complex)
         for i in range(n):
             kk = wave_number(self.freq[i])
             self.k[i] = kk
-            self.TF[i] = transfer_function(typ, kk, hinge) if abs(kk) > np.finfo(float).eps else 1.0
+            if typ != "dipole":
+                self.TF[i] = transfer_function(typ, kk, hinge) if abs(kk) > np.finfo(float).eps else 1.0
             if self.freq[i] < 1e-8:
                 self.k[i] = 0.0
                 self.TF[i] = 1.0
@@ -153,6 +167,11 @@ class Setup:
     dposdy_ini: np.ndarray | None = None
     # x derivative of the beach coefficient as the reference builds it (n1, n2)
     nu_x: np.ndarray | None = None
+    # dipole wavemaker (igeom = 3): per component frequency, mu(:, 1..4), abscissa, image pairs
+    dip_freq: np.ndarray | None = None
+    dip_mu: np.ndarray | None = None
+    x_dip: float = 2.5
+    n_side: int = 6
 
 
 def gzed(z: float, igeom: int, d: float) -> float:
@@ -179,7 +198,7 @@ def build(cfg: TankConfig) -> Setup:
     f_out = cfg.f_out * Tscale
     nu_mono = cfg.nu_mono * Tscale
     T_mono = 1.0 / nu_mono
-    typ = "piston" if cfg.igeom == 1 else "hinged"
+    typ = {1: "piston", 2: "hinged", 3: "dipole"}[cfg.igeom]
     d_hinge = cfg.d_hinge / h
     waves = None
     if cfg.icase == 2:
@@ -199,7 +218,14 @@ def build(cfg: TankConfig) -> Setup:
     n1, n2 = cfg.n1, cfg.n2
     dely = ylen / (n2 - 1) if n2 != 1 else 0.0
     cy = np.arange(n2) * dely
-    if waves is not None:
+    dip_freq = dip_mu = None
+    x_dip = X_DIP / h
+    if waves is not None and cfg.igeom == 3:
+        dip_freq = np.array(waves.freq, float)
+        dip_mu = dipoles_ini(waves, cfg.icase, x_dip)
+        freq, pos_ini = np.zeros(0), np.zeros((n2, 0), complex)
+        dposdy_ini = np.zeros((n2, 0), complex)
+    elif waves is not None:
         hmax = None
         if cfg.icase in (3, 4, 41):
             # 4 n_max, n_max = 2^rnum / floor(clock sqrt(g / h)) with the nondimensional clock
@@ -210,10 +236,39 @@ def build(cfg: TankConfig) -> Setup:
         dposdy_ini = np.zeros((n2, 0), complex)
     nu = absorb(cfg, xlen)
     nu_x = absorb_dx(cfg, xlen)
-    stab = dt_stab(n1, n2, cfg.n3, xlen, ylen, cfg.icase)
+    stab = dt_stab(n1, n2, cfg.n3, xlen, ylen, cfg.icase, cfg.igeom)
     return Setup(cfg=cfg, xlen=xlen, ylen=ylen, Tscale=Tscale, Tramp=Tramp, T_stop=T_stop, T_mono=T_mono,
                  f_out=f_out, freq=freq, pos_ini=pos_ini, nu=nu, dt_stab=stab, delt_out=1.0 / f_out,
-                 waves=waves, dposdy_ini=dposdy_ini, nu_x=nu_x)
+                 waves=waves, dposdy_ini=dposdy_ini, nu_x=nu_x, dip_freq=dip_freq, dip_mu=dip_mu, x_dip=x_dip,
+                 n_side=N_SIDE)
+
+
+# dipole wavemaker constants (input.f90:137-139): depth of the monochromatic dipole
+# (nondimensional), abscissa (m, divided by h at HOS-NWT.f90:111), wall-image pairs
+Z_DIPMONO = -0.25
+X_DIP = 2.5
+N_SIDE = 6
+
+
+def dipoles_ini(w: WaveComponents, icase: int, x_dip: float) -> np.ndarray:
+    """mu(:, 1..4) of the dipole wavemaker, one row per wave component (resol_wmkr.f90:1404-1455):
+    the depth from the component's nondimensional frequency (icase 2: Z_DIPMONO), the dipole
+    transfer function at that depth, mu_z = a TF e^{i (k x_dip + phase)}, mu_x = -i mu_z
+    tanh(k (z + 1)) and the bottom image's depth -2 - z.  The wavenumbers are the dispersion
+    relation's (the dipole branch of wave_init computes them into its argument and returns its
+    uninitialised local array instead, wave_def.f90:295-323)."""
+    f = np.asarray(w.freq, float)
+    z = np.where(f > 1.0, -0.35, np.where(f > 0.6, -0.2, np.where(f > 0.5, -0.4, -0.6)))
+    if icase == 2:
+        z = np.full_like(f, Z_DIPMONO)
+    mu = np.zeros((len(f), 4), complex)
+    for i in range(len(f)):
+        tf = transfer_function("dipole", float(w.k[i]), float(z[i]))
+        mu[i, 2] = z[i]
+        mu[i, 3] = -2.0 - z[i]
+        mu[i, 1] = w.ampli[i] * tf * np.exp(1j * (w.k[i] * x_dip + w.phase[i]))
+        mu[i, 0] = -1j * mu[i, 1] * math.tanh(w.k[i] * (z[i] + 1.0))
+    return mu
 
 
 def wmk_ini(w: WaveComponents, cy: np.ndarray, hmax: int | None = None):
@@ -271,13 +326,13 @@ def absorb_dx(cfg: TankConfig, xlen: float) -> np.ndarray:
     return nux
 
 
-def dt_stab(n1, n2, n3, xlen, ylen, icase) -> float:
-    """RK4 stability bound (HOS-NWT.f90:132-176)."""
+def dt_stab(n1, n2, n3, xlen, ylen, icase, igeom=1) -> float:
+    """RK4 stability bound (HOS-NWT.f90:132-176); dipoles have no paddle grid (:145-153)."""
     c = math.sqrt(8.0 / math.pi ** 2)
     d1 = c * xlen / (n1 - 1)
     d2 = c * ylen / (n2 - 1) if n2 != 1 else d1 + 1.0
     d3 = c * (2.0 * L_ADD) / (L_ADD * (n3 - 1))
-    if icase == 1:
+    if icase == 1 or igeom == 3:
         return min(d1, d2)
     if n2 == 1:
         return min(d1, d3)

@@ -28,6 +28,8 @@ class HosParams(C.Structure):
         ("nu", _dp), ("n_freq", C.c_int), ("freq", _dp),
         ("pos_ini_re", _dp), ("pos_ini_im", _dp), ("coeffilt", C.c_double * 3),
         ("dposdy_ini_re", _dp), ("dposdy_ini_im", _dp), ("nu_x", _dp), ("nu_y", _dp),
+        ("n_dip", C.c_int), ("dip_freq", _dp), ("dip_mux_re", _dp), ("dip_mux_im", _dp),
+        ("dip_muz_re", _dp), ("dip_muz_im", _dp), ("dip_z", _dp), ("x_dip", C.c_double), ("n_side", C.c_int),
     ]
 
 

@@ -55,6 +55,14 @@ class Tank:
         p.dposdy_ini_im = ptr(dyi) if len(fr) else None
         p.nu_x = ptr(nux) if nux is not None else None
         p.nu_y = None  # no side beach
+        if setup.dip_mu is not None and len(setup.dip_mu):
+            mu = np.asarray(setup.dip_mu, complex)
+            dip = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                   (setup.dip_freq, mu[:, 0].real, mu[:, 0].imag, mu[:, 1].real, mu[:, 1].imag, mu[:, 2].real)]
+            self._keep += dip
+            p.n_dip = len(mu)
+            p.dip_freq, p.dip_mux_re, p.dip_mux_im, p.dip_muz_re, p.dip_muz_im, p.dip_z = (ptr(a) for a in dip)
+            p.x_dip, p.n_side = setup.x_dip, setup.n_side
         h = C.c_void_p()
         check(self.lib.hos_create(C.byref(p), C.byref(h)))
         self.h = h

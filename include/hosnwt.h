@@ -52,7 +52,7 @@ typedef struct hos_params {
   int mHOS;           /* HOS order M (common_vars.f90:35) */
   int p1, p2;         /* dealiasing parameters; supported: p1 = M, p2 = M (3-D) or 1 */
   int icase;          /* 1 sloshing (no wavemaker), 2/3/31/32/33/4/41 wavemaker */
-  int igeom;          /* 1 piston, 2 hinged flap */
+  int igeom;          /* 1 piston, 2 hinged flap, 3 line dipoles (n_dip below) */
   int i_wmk;          /* wavemaker nonlinear order; supported: 1, 2, 3 (3 without filters) */
   int iramp;          /* time ramp type (resol_wmkr.f90:1268-1397) */
   int iabsnb;         /* numerical beach on/off */
@@ -79,6 +79,21 @@ typedef struct hos_params {
   const double* dposdy_ini_im;
   const double* nu_x;
   const double* nu_y;
+  /* Dipole wavemaker (igeom = 3; phi_dipoles, main/resol_wmkr.f90:1492-1647): one entry
+   * per wave component, as dipoles_ini (resol_wmkr.f90:1404-1455) builds them: the
+   * nondimensional frequency, the moments mu(:,1) (x) and mu(:,2) (z), complex, and the
+   * dipole depth mu(:,3) (its bottom image lies at -2 - mu(:,3)); x_dip is the dipole
+   * abscissa and n_side the number of wall-image pairs on each side (input.f90:137-139).
+   * Ignored unless igeom = 3; a zero-initialised tail means no dipoles. */
+  int n_dip;
+  const double* dip_freq;
+  const double* dip_mux_re;
+  const double* dip_mux_im;
+  const double* dip_muz_re;
+  const double* dip_muz_im;
+  const double* dip_z;
+  double x_dip;
+  int n_side;
 } hos_params;
 
 /* Kernel statistics of the last call (transform counts, bytes). */

@@ -26,6 +26,11 @@ Pinning of each function (tests/test_oracle_*.py):
   * runge4_adapt2                  -> y'=lambda*y RK4 polynomial + Kutta-3
                                       embedded difference (sloshing M=1 case)
   * phiadd_sl separable form       -> brute-force quadruple loop on tiny grids
+  * phi_dipoles                    -> Laplace equation, bottom condition and
+                                      derivative consistency by finite
+                                      differences; dipole far field of
+                                      dipole_transfer on the GPU path (linear
+                                      wave amplitude)
   * match_surface_wmk              -> its four defining conditions
   * end to end                     -> linear sloshing closed form, energy and
                                       volume conservation
@@ -230,6 +235,13 @@ class Params:
     pos_ini: np.ndarray = field(default_factory=lambda: np.zeros((1, 0), complex))
     # y derivative of pos_ini (wavemaking.f90:147-149), needed for i_wmk >= 2; None: zero
     dposdy_ini: np.ndarray | None = None
+    # dipole wavemaker (igeom = 3): one row per wave component, mu(:,1..4) of dipoles_ini
+    # (resol_wmkr.f90:1448-1455), the component frequencies, the dipole abscissa and the
+    # number of image pairs on each side (input.f90:137-139, HOS-NWT.f90:111)
+    dip_mu: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), complex))
+    dip_freq: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    x_dip: float = 2.5
+    n_side: int = 6
 
     def __post_init__(self):
         if self.p1 is None:
@@ -246,6 +258,7 @@ class Params:
         if self.dposdy_ini is None:
             self.dposdy_ini = np.zeros_like(np.asarray(self.pos_ini, complex))
         self.has_wmk = self.icase in (2, 3, 31, 32, 33, 4, 41, 42) and self.igeom in (1, 2)
+        self.has_dip = self.icase in (2, 3, 31, 32, 33, 4, 41, 42) and self.igeom == 3
         if self.i_wmk == 3 and min(self.coeffilt) < 1.0:
             # wmk_3rdorder filters its INTENT(IN) stage inputs in place (resol_wmkr.f90:1019-1030):
             # what that does to the caller's arrays is left to the compiler, so it is not restated
@@ -453,6 +466,94 @@ def dpos_1st(t: float, p: Params, tb: Tables):
     gz, dgz = tb.gz[:, None], tb.dgz[:, None]
     return ((gz * sy0[None, :]).real, (gz * sy1[None, :]).real,
             (dgz * sz0[None, :]).real, (dgz * sz1[None, :]).real)
+
+
+def dipole_transfer(k: float, z_dip: float) -> complex:
+    """Transfer function of a dipole at depth z_dip, TF_adim_new 'dipole' (linear_wave.f90:316-327)."""
+    # IEEE arithmetic as the reference's: k^2 - w^4 rounds to 0 once tanh k rounds to 1
+    # (k > 18.7), and the finite-depth branch then gives an infinite transfer function
+    k = np.float64(k)
+    k2 = k * k
+    w2 = k * np.tanh(k)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if k <= 20.0:
+            w4 = w2 * w2
+            return complex((k2 - w4 + w2) / (2.0 * np.sqrt(w2) * (k2 - w4) * np.cosh(k) * np.sinh(k * (z_dip + 1.0))))
+        return complex(np.sqrt(w2) / (2.0 * np.exp(k * z_dip) * k2))
+
+
+def dipoles_ini(freq, k, ampli, phase, icase: int, x_dip: float, z_dipmono: float = -0.25) -> np.ndarray:
+    """mu(:,1..4) of the dipole wavemaker, one row per wave component (resol_wmkr.f90:1404-1455):
+    the depth chosen from the component's nondimensional frequency, its transfer function,
+    mu_z = a TF e^{i(k x_dip + phase)}, mu_x = -i mu_z tanh(k (z_dip + 1)) and the bottom image's
+    depth -2 - z_dip."""
+    freq = np.asarray(freq, float)
+    z = np.where(freq > 1.0, -0.35, np.where(freq > 0.6, -0.2, np.where(freq > 0.5, -0.4, -0.6)))
+    if icase == 2:
+        z = np.full_like(freq, z_dipmono)
+    mu = np.zeros((len(freq), 4), complex)
+    for f in range(len(freq)):
+        tf = dipole_transfer(float(k[f]), float(z[f]))
+        mu[f, 2] = z[f]
+        mu[f, 3] = -2.0 - z[f]
+        mu[f, 1] = ampli[f] * tf * np.exp(1j * (k[f] * x_dip + phase[f]))
+        mu[f, 0] = -1j * mu[f, 1] * math.tanh(k[f] * (z[f] + 1.0))
+    return mu
+
+
+def dipole_images(p: "Params", x: np.ndarray):
+    """delta_x(:, :, -n_side:n_side) of dipoles_ini (resol_wmkr.f90:1461-1481): the main dipole at
+    x_dip and its wall images; returns {j: x - x_j}."""
+    dx = {0: x - p.x_dip}
+    for j in range(1, p.n_side + 1):
+        if j % 2 == 0:
+            right, left = p.x_dip + p.xlen * j, p.x_dip - p.xlen * j
+        else:
+            right, left = -p.x_dip + p.xlen * (j + 1), -p.x_dip - p.xlen * (j - 1)
+        dx[j] = x - right
+        dx[-j] = x - left
+    return dx
+
+
+def phi_dipoles(eta: np.ndarray, t: float, alpha: float, alphat: float, p: "Params", x: np.ndarray):
+    """Potential of the dipole wavemaker and its t, x, z derivatives at z = eta, each component's
+    dipole with its bottom image and n_side pairs of wall images (+ their bottom images), the wall
+    images' x moment alternating in sign (phi_dipoles, resol_wmkr.f90:1492-1647).  beta = 1,
+    beta_x = 0 (dipoles_ini); phiy is not formed."""
+    dx = dipole_images(p, x)
+    phi = np.zeros(eta.shape, complex)
+    dphi_dt = np.zeros(eta.shape, complex)
+    dphi_dx = np.zeros(eta.shape, complex)
+    dphi_dz = np.zeros(eta.shape, complex)
+    for f in range(len(p.dip_freq)):
+        mu1, mu2, mu3, mu4 = p.dip_mu[f]
+        dz_up = eta - mu3.real
+        dz_down = eta - mu4.real
+        resu = np.zeros(eta.shape, complex)
+        resu_dx = np.zeros(eta.shape, complex)
+        resu_dz = np.zeros(eta.shape, complex)
+        terms = [(0, 1.0, dz_up, 1.0), (0, 1.0, dz_down, -1.0)]
+        for j in range(1, p.n_side + 1):
+            sg = -1.0 if j % 2 == 1 else 1.0
+            terms += [(-j, sg, dz_up, 1.0), (j, sg, dz_up, 1.0), (j, sg, dz_down, -1.0), (-j, sg, dz_down, -1.0)]
+        for j, sg, dz, up in terms:
+            d = dx[j]
+            r2 = 1.0 / (d * d + dz * dz)
+            muxor2 = sg * mu1 * r2
+            muzor2 = mu2 * r2
+            tmp = muxor2 * d + up * muzor2 * dz
+            resu = resu + tmp
+            resu_dx = resu_dx + muxor2 - tmp * 2.0 * d * r2
+            resu_dz = resu_dz + up * muzor2 - tmp * 2.0 * dz * r2
+        w = TWOPI * p.dip_freq[f]
+        eiwt = np.exp(1j * w * t)
+        phi = phi + resu * eiwt
+        dphi_dt = dphi_dt + resu * 1j * w * eiwt
+        dphi_dx = dphi_dx + resu_dx * eiwt
+        dphi_dz = dphi_dz + resu_dz * eiwt
+    phi, dphi_dt, dphi_dx, dphi_dz = phi / TWOPI, dphi_dt / TWOPI, dphi_dx / TWOPI, dphi_dz / TWOPI
+    return ((phi * alpha).real, (phi * alphat + dphi_dt * alpha).real, (dphi_dx * alpha).real,
+            (dphi_dz * alpha).real)
 
 
 def match_surface_coeffs(f, df, d2f):
@@ -1091,6 +1192,16 @@ class Oracle:
                 sl = phiadd_sl(a_tot, d_tot, etark, tb)
             phi_add, phix_add, phiy_add = sl["phi"], sl["phix"], sl["phiy"]
             phiz_add, phit_add = sl["phiz"], sl["phit"]
+        elif p.has_dip:
+            # ramp + phi_dipoles at z = eta (resol_HOS.f90:494-496); phiy_add keeps its zero
+            da1 = np.zeros((n1, n2))
+            deta1 = np.zeros((n1, n2))
+            d1 = np.zeros((n3a, n2))
+            dr = np.zeros((n3a, n2))
+            self.alpha, self.alphat, self.alphatt = ramp(self.time, p)
+            phi_add, phit_add, phix_add, phiz_add = phi_dipoles(etark, self.time, self.alpha, self.alphat, p,
+                                                                tb.cx[:, None] + 0.0 * etark)
+            phiy_add = np.zeros((n1, n2))
         else:
             da1 = np.zeros((n1, n2))
             deta1 = np.zeros((n1, n2))
@@ -1300,6 +1411,10 @@ class Oracle:
             sl = phiadd_sl(a, d1, eta, tb, need=("phi",))
             self.phi_add = sl["phi"]
             return State(eta, phis, a, a.copy(), z.copy(), z.copy())
+        if p.has_dip:
+            # solveHOS at t = 0 (initial_condition.f90:467-478): the dipole potential at rest
+            self.alpha, self.alphat, self.alphatt = ramp(0.0, p)
+            self.phi_add = phi_dipoles(eta, 0.0, self.alpha, self.alphat, p, tb.cx[:, None] + 0.0 * eta)[0]
         return State(eta, phis, za, za.copy(), z.copy(), z.copy())
 
     # ----- volume / energy (vol_energy.f90:36-119) ----------------------------
@@ -1341,7 +1456,7 @@ def dt_stab(p: Params) -> float:
     d1 = c * p.xlen / (p.n1 - 1)
     d2 = c * p.ylen / (p.n2 - 1) if p.n2 != 1 else d1 + 1.0
     d3 = c * (2.0 * L_ADD) / (L_ADD * (p.n3 - 1))
-    if p.icase == 1:
+    if p.icase == 1 or p.igeom == 3:
         return min(d1, d2)
     if p.n2 == 1:
         return min(d1, d3)

@@ -25,7 +25,16 @@ def oracle_params(s):
                      igeom=c.igeom, d_hinge=c.d_hinge / c.h, iramp=c.iramp, Tramp=s.Tramp, T_stop=s.T_stop,
                      T_mono=s.T_mono, iabsnb=c.iabsnb, xabsf=c.xabsf, coeffabsf=c.coeffabsf, freq=s.freq,
                      pos_ini=s.pos_ini, coeffilt=tuple(c.coeffilt), i_wmk=c.i_wmk,
-                     dposdy_ini=s.dposdy_ini)
+                     dposdy_ini=s.dposdy_ini, **dipole_params(s))
+
+
+def dipole_params(s):
+    """The oracle's own dipoles_ini (resol_wmkr.f90:1404-1455) from the setup's wave components."""
+    if s.cfg.igeom != 3 or s.waves is None:
+        return {}
+    w = s.waves
+    mu = ho.dipoles_ini(w.freq, w.k, w.ampli, w.phase, s.cfg.icase, s.x_dip)
+    return dict(dip_mu=mu, dip_freq=np.array(w.freq, float), x_dip=s.x_dip, n_side=s.n_side)
 
 
 def random_state(s, seed, amp=0.02):

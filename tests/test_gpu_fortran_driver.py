@@ -120,6 +120,46 @@ def test_fortran_driver_matches_oracle(tmp_path, i_wmk, toler, tstop):
         check_output_files(tmp_path / "Results", s, orc, snaps, xprobes)
 
 
+@pytest.mark.gpu
+def test_fortran_driver_dipole_matches_oracle(tmp_path):
+    """igeom = 3: the driver builds the line dipole of a monochromatic wave (dipoles_ini) and the
+    energy record follows the oracle's dipole tank; the volume column is the plain volume."""
+    assert os.path.exists(EXE), "build the Fortran driver with hos-nwt_amd/build.sh"
+    T, n1, M, tstop, toler = 1.5, 64, 3, 3.0, 1e-7
+    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T, toler=toler, igeom=3)
+
+    def num(x):
+        return f"{x:.16e}"
+
+    text = INPUT.format(amp=num(s.cfg.amp_mono), nu=num(1.0 / T), tstop=num(tstop), fout=num(2.0 / T), i_wmk=1,
+                        toler=num(toler), iprobes=0, flag=0, icase=2, file_name="none", igeom=3, d_hinge=num(0.0))
+    (tmp_path / "input_HOS-NWT.dat").write_text(text)
+    r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    got = np.array(data_rows(tmp_path / "Results" / "vol_energy.dat"))
+    orc = ho.Oracle(oracle_params(s))
+    st = orc.initialize()
+    deta = np.zeros_like(st.eta)
+    t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+    ref = []
+    while t < s.T_stop + s.delt_out:
+        ep, ek, et = orc.energy(st.eta, st.phis + orc.phi_add, deta)
+        ref.append((t, orc.volume(st.eta), ep, ek, et))
+        if abs(t - s.T_stop) <= 2.2e-16:
+            break
+        tn = min(t + s.delt_out, s.T_stop + s.delt_out)
+        st, t, h, _, _, d = ho.advance(orc, st, t, tn, h, s.cfg.toler, s.dt_stab, s.delt_out)
+        deta = d if d is not None else deta
+    ref = np.array(ref)
+    assert got.shape[0] == ref.shape[0]
+    assert np.abs(ref[-1, 4]) > 0.0
+    np.testing.assert_allclose(got[:, 0], ref[:, 0], rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(got[:, 1], ref[:, 1], rtol=0, atol=2e-4 * max(np.abs(ref[:, 1]).max(), 1e-9))
+    scale = np.abs(ref[:, 4]).max()
+    np.testing.assert_allclose(got[:, 2:5], ref[:, 2:5], rtol=0, atol=2e-4 * scale)
+
+
 def data_rows(path):
     """Numeric rows of a Tecplot file (skips the '#' echo, TITLE, VARIABLES and ZONE lines)."""
     rows = []
