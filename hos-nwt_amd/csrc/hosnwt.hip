@@ -1830,6 +1830,57 @@ __global__ void __launch_bounds__(256) k_dipoles(int n1, int ns, DipSet d, doubl
 }
 
 // ---------------------------------------------------------------------------
+// SWENSE output (velocities.f90:40-784) helpers
+// ---------------------------------------------------------------------------
+// out = s A B + t C over n values (B, C may be null: B = 1, C = 0); out may alias A, B or C.
+__global__ void k_ew(size_t n, double s, const double* A, const double* B, double t, const double* C,
+                     double* out) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = s * A[i];
+  if (B) v *= B[i];
+  out[i] = C ? v + t * C[i] : v;
+}
+
+// dst (N1 x N2, zero beyond n1 x n2) = src (n1 x n2): extend (variables.f90:243-266)
+__global__ void k_extend(int n1, int n2, int N1, int N2, const double* __restrict__ src, double* __restrict__ dst) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)N1 * N2) return;
+  const int i1 = (int)(i % N1), i2 = (int)(i / N1);
+  dst[i] = (i1 < n1 && i2 < n2) ? src[i1 + (size_t)n1 * i2] : 0.0;
+}
+
+// Full time derivatives of init_velocity_comp (velocities.f90:586-611) from a right-hand
+// side evaluated with dt = 0 (so GtoF is the identity): deta = dG1 + phis omega^2,
+// dphis = dG2 / omega - eta, and at the mean mode dphis = (the free-surface RHS's mean
+// mode, dphisrk_1) - eta, deta = dG1.
+__global__ void k_swense_dt(int n1, int n2, double pix, double piy, const double* dG1, const double* dG2,
+                            const double* eta, const double* phis, const double* d11, double* deta,
+                            double* dphis) {
+  const size_t n = (size_t)n1 * n2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i == 0) {
+    deta[0] = dG1[0];
+    dphis[0] = d11[0] - eta[0];
+    return;
+  }
+  const double w = wave_omega(wave_k((int)(i % n1), (int)(i / n1), pix, piy));
+  deta[i] = dG1[i] + phis[i] * w * w;
+  dphis[i] = dG2[i] / w - eta[i];
+}
+
+// the mean mode of the free-surface RHS before the omega scaling (resol_HOS.f90:640):
+// (D2 + Z)(1,1) of k_expon_out, times its filter weight
+__global__ void k_mean_mode(const double* D2, const double* Zh, const double* Zh2, const double* filt,
+                            double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double z = Zh[0];
+  if (Zh2) z += Zh2[0];
+  out[0] = ((D2 ? D2[0] : 0.0) + z) * (filt ? filt[0] : 1.0);
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 struct DevBuf {
@@ -2368,6 +2419,10 @@ struct EngineBase {
   // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
   // and one y pass (full inverse cos-cos transform), timed with HIP events
   virtual void time_big(int reps, double* out4) = 0;
+  // SWENSE output (init_velocity_comp + HOSvel2_SWEET + HOSvel_SWEET) after a right-hand
+  // side at the committed state: velocity and phi_t modes at z = 0 (out[0..3], n1 x n2),
+  // the modal d eta / dt (out[4]); device pointers
+  virtual void swense(int meta2, double** out) = 0;
   // the extended grid's schedule: {x line length, x values per thread, x sequence,
   // y line length, y values per thread, y sequence, y tile width, candidates timed}
   // and the trial batch's time (us) with the default and the chosen schedule
@@ -3119,6 +3174,320 @@ struct Engine : EngineBase {
   }
 
   T* tmpB2() { return X; }
+
+  // ---- SWENSE output on the reference's extended grid ----------------------------------
+  // The H2 re-expansions of HOSvel2_SWEET keep content beyond the base modes that depends
+  // on the extended grid (aliasing of the reference's own products), so they run on the
+  // reference's grid Nd = (M+1) n / 2 exactly, in FP64, with dense transform matrices on
+  // the matrix cores (k_mm); every field follows the reference's sequence.
+  struct SwGrid {
+    int N1 = 0, N2 = 1;
+    DenseGrid<double> g;
+    std::vector<double*> ep, ep2, phizm, phizm2, aH[4];
+    double *etax, *etay, *phisx, *phisy, *g2, *pMm1, *pMm2, *p2Mm2, *phiz2, *ge2p2, *t1, *t2, *phizi, *pm;
+    double *vit[4], *pa[4], *pb[4];
+    double *b0, *b1, *b2, *b3, *b4, *d11;  // base-grid work (n1 x n2) and the mean mode
+    double* out[5];
+  };
+  std::unique_ptr<SwGrid> swg;
+
+  void swense(int meta2, double** out_dev) override {
+    if constexpr (sizeof(T) != sizeof(double)) {
+      (void)meta2;
+      (void)out_dev;
+      throw std::runtime_error("the SWENSE output runs in FP64 contexts (precision 0)");
+    } else {
+      hos_ctx& C = *c;
+      use_main();
+      hipStream_t st = cs;
+      const int n1 = C.n1, n2 = C.n2;
+      const size_t nsb = (size_t)n1 * n2;
+      const double pix = C.pix, piy = C.piy;
+      if (!swg) {
+        auto g = std::make_unique<SwGrid>();
+        g->N1 = ((M + 1) * n1) / 2;
+        g->N2 = n2 > 1 ? ((M + 1) * n2) / 2 : 1;
+        if (g->N1 > 16384 || g->N2 > 16384) throw std::runtime_error("SWENSE output: extended grid beyond 16384");
+        build_dense(g->g, C.ar, g->N1, g->N2);
+        const size_t nd = (size_t)g->N1 * g->N2;
+        auto F = [&]() { return C.ar.get<double>(nd); };
+        for (int j = 0; j <= M; ++j) {
+          g->ep.push_back(F());
+          g->ep2.push_back(F());
+          g->phizm.push_back(F());
+          g->phizm2.push_back(F());
+          for (int f = 0; f < 4; ++f) g->aH[f].push_back(F());
+        }
+        g->etax = F(); g->etay = F(); g->phisx = F(); g->phisy = F(); g->g2 = F();
+        g->pMm1 = F(); g->pMm2 = F(); g->p2Mm2 = F(); g->phiz2 = F(); g->ge2p2 = F();
+        g->t1 = F(); g->t2 = F(); g->phizi = F(); g->pm = F();
+        for (int f = 0; f < 4; ++f) {
+          g->vit[f] = F();
+          g->pa[f] = F();
+          g->pb[f] = F();
+        }
+        g->b0 = C.ar.get<double>(nsb); g->b1 = C.ar.get<double>(nsb); g->b2 = C.ar.get<double>(nsb);
+        g->b3 = C.ar.get<double>(nsb); g->b4 = C.ar.get<double>(nsb);
+        g->d11 = C.ar.get<double>(1);
+        for (int f = 0; f < 5; ++f) g->out[f] = C.ar.get<double>(nsb);
+        swg = std::move(g);
+      }
+      SwGrid& G = *swg;
+      const int N1 = G.N1, N2 = G.N2;
+      const size_t nd = (size_t)N1 * N2;
+      const int KX[4] = {KSIN, KCOS, KCOS, KCOS}, KY[4] = {KCOS, KSIN, KCOS, KCOS};
+      auto ew = [&](double* out, double s1, const double* A, const double* B, double t1, const double* Cc,
+                    size_t n) {
+        hipLaunchKernelGGL(k_ew, dim3(nblk(n)), dim3(256), 0, st, n, s1, A, B, t1, Cc, out);
+      };
+      auto zero = [&](double* p, size_t n) { ew(p, 0.0, p, nullptr, 0.0, nullptr, n); };
+      // dense transforms on the reference grid
+      auto T2 = [&](int kx, int ky, int dir, const Src<double, double>& src, int o1, int o2, double* out) {
+        dense2d<double>(G.g, st, kx, ky, dir, src, o1, o2, Dst<double>{out, N1, 0});
+      };
+      auto ext_src = [&](const double* a, int ld, int nx, int ny, int mul = MUL_NONE, int jpow = 0) {
+        return src_mul<double, double>(a, ld, nx, ny, mul, jpow, pix, piy);
+      };
+      // dealias (dealiasing.f90:39-92) in place
+      auto dealias = [&](int order, double* x, int kx, int ky) {
+        const bool ax = M != 1 && (order - 1) % (M - 1) == 0;
+        const int p2 = n2 > 1 ? M : 1;
+        const bool ay = n2 > 1 && p2 != 1 && (order - 1) % (p2 - 1) == 0;
+        if (!ax && !ay) return;
+        const int o1 = ax ? n1 : N1, o2 = ay ? n2 : N2;
+        T2(kx, ky, FWD, ext_src(x, N1, N1, N2), o1, o2, G.pm);
+        T2(kx, ky, INV, ext_src(G.pm, N1, o1, o2), N1, N2, x);
+      };
+      // filter_ext (dealiasing.f90:96-134): base-grid physical field
+      auto filter_ext = [&](const double* x, int kx, int ky, double* out) {
+        T2(kx, ky, FWD, ext_src(x, N1, N1, N2), n1, n2, G.pm);
+        base64(kx, ky, INV, src_plain<double, double>(G.pm, N1, n1, n2), n1, n2, Dst<double>{out, n1, 0});
+      };
+      // --- init_velocity_comp (velocities.f90:501-784): the right-hand side at the
+      // committed state has just run; its time derivatives
+      const double* filtp = C.filt_fs;
+      hipLaunchKernelGGL(k_mean_mode, dim3(1), dim3(64), 0, st, (const double*)(M >= 2 ? D2 : nullptr),
+                         (const double*)Zh, (const double*)nullptr, filtp, G.d11);
+      double* fst = G.out[4];
+      // (init_velocity_comp's eta3, phis3 are the committed state's modes, :579-580)
+      hipLaunchKernelGGL(k_swense_dt, dim3(nblk(nsb)), dim3(256), 0, st, n1, n2, pix, piy,
+                         (const double*)C.kG1[0], (const double*)C.kG2[0], (const double*)C.eta_hat,
+                         (const double*)C.phis_hat, (const double*)G.d11, fst, G.b0);
+      double* dphis3 = G.b1;  // physical d phis / dt
+      base64(KCOS, KCOS, INV, src_plain<double, double>(G.b0, n1, n1, n2), n1, n2, Dst<double>{dphis3, n1, 0});
+      // phisxy_etaxy on the reference grid (resol_HOS.f90:924-1078)
+      T2(KCOS, KCOS, INV, ext_src(eta_hat, n1, n1, n2), N1, N2, G.ep[1]);
+      for (int j = 1; j < M; ++j) {
+        ew(G.ep[j + 1], 1.0 / (j + 1), G.ep[j], G.ep[1], 0.0, nullptr, nd);
+        dealias(j + 1, G.ep[j + 1], KCOS, KCOS);
+      }
+      T2(KSIN, KCOS, INV, ext_src(eta_hat, n1, n1, n2, MUL_NEG_KX), N1, N2, G.etax);
+      T2(KSIN, KCOS, INV, ext_src(phis_hat, n1, n1, n2, MUL_NEG_KX), N1, N2, G.phisx);
+      if (n2 > 1) {
+        T2(KCOS, KSIN, INV, ext_src(eta_hat, n1, n1, n2, MUL_NEG_KY), N1, N2, G.etay);
+        T2(KCOS, KSIN, INV, ext_src(phis_hat, n1, n1, n2, MUL_NEG_KY), N1, N2, G.phisy);
+      } else {
+        zero(G.etay, nd);
+        zero(G.phisy, nd);
+      }
+      ew(G.g2, 1.0, G.etax, G.etax, 0.0, nullptr, nd);
+      ew(G.g2, 1.0, G.etay, G.etay, 1.0, G.g2, nd);
+      dealias(2, G.g2, KCOS, KCOS);
+      // HOSphis_modes on the reference grid (resol_HOS.f90:665-915): phizm by order
+      hipLaunchKernelGGL(k_extend, dim3(nblk(nd)), dim3(256), 0, st, n1, n2, N1, N2, (const double*)phis_hat,
+                         G.aH[0][1]);
+      for (int iH = 1; iH <= M; ++iH) {
+        zero(G.t1, nd);
+        zero(G.phizm[iH], nd);
+        for (int j = iH; j >= 1; --j) {
+          const int jj = (j == 1) ? 1 : j, j2 = (j == 1) ? iH : iH - (j - 1);
+          if (j == 1 && iH > 1) continue;  // the order-1 term below
+          T2(KCOS, KCOS, INV, ext_src(G.aH[0][j2], N1, N1, N2, MUL_KTH, jj), N1, N2, G.phizi);
+          ew(G.t1, -1.0, G.phizi, G.ep[jj], 1.0, G.t1, nd);
+          if (jj > 1)
+            ew(G.phizm[iH], 1.0, G.phizi, G.ep[jj - 1], 1.0, G.phizm[iH], nd);
+          else
+            ew(G.phizm[iH], 1.0, G.phizi, nullptr, 1.0, G.phizm[iH], nd);
+        }
+        if (iH > 1) {
+          T2(KCOS, KCOS, INV, ext_src(G.aH[0][iH], N1, N1, N2, MUL_KTH, 1), N1, N2, G.phizi);
+          ew(G.t1, -1.0, G.phizi, G.ep[1], 1.0, G.t1, nd);
+          ew(G.phizm[iH], 1.0, G.phizi, nullptr, 1.0, G.phizm[iH], nd);
+          dealias(iH, G.phizm[iH], KCOS, KCOS);
+        }
+        if (iH < M) {
+          dealias(iH + 1, G.t1, KCOS, KCOS);
+          T2(KCOS, KCOS, FWD, ext_src(G.t1, N1, N1, N2), N1, N2, G.aH[0][iH + 1]);
+        }
+      }
+      // phizMm2, phizMm1, phiz2Mm2 (resol_HOS.f90:785-823)
+      zero(G.pMm2, nd);
+      for (int j = 1; j <= M - 2; ++j) {
+        ew(G.pMm2, 1.0, G.phizm[j], nullptr, 1.0, G.pMm2, nd);
+        ew(G.phizm2[j], 1.0, G.phizm[j], nullptr, 0.0, nullptr, nd);
+      }
+      if (M >= 2)
+        ew(G.pMm1, 1.0, G.phizm[M - 1], nullptr, 1.0, G.pMm2, nd);
+      else
+        zero(G.pMm1, nd);
+      zero(G.p2Mm2, nd);
+      if (M >= 4) {
+        const int m3d2 = std::max(1, (M - 3) / 2), d2m1 = std::max(1, M / 2 - 1);
+        for (int j = 1; j <= m3d2; ++j) {
+          ew(G.p2Mm2, 2.0, G.phizm2[j], G.phizm2[M - 2 - j], 1.0, G.p2Mm2, nd);
+          ew(G.phizm2[j + 1], 1.0, G.phizm2[j + 1], nullptr, 1.0, G.phizm2[j], nd);
+        }
+        ew(G.p2Mm2, 1.0, G.phizm2[d2m1], G.phizm2[d2m1], 1.0, G.p2Mm2, nd);
+      }
+      // phiz2 and gradeta2 phiz2 with their dealiasing (resol_HOS.f90:851-905)
+      zero(G.phiz2, nd);
+      zero(G.ge2p2, nd);
+      for (int j = 1; j <= (M - 2) / 2; ++j) {
+        ew(G.t2, 1.0, G.phizm[j], G.phizm[j], 0.0, nullptr, nd);
+        dealias(2 * j, G.t2, KCOS, KCOS);
+        ew(G.phiz2, 1.0, G.t2, nullptr, 1.0, G.phiz2, nd);
+        ew(G.t2, 1.0, G.g2, G.t2, 0.0, nullptr, nd);
+        dealias(2 * j + 2, G.t2, KCOS, KCOS);
+        ew(G.ge2p2, 1.0, G.t2, nullptr, 1.0, G.ge2p2, nd);
+      }
+      for (int j = 2; j <= M - 3; ++j)
+        for (int q = 1; q <= std::min(M - 2 - j, j - 1); ++q) {
+          ew(G.t2, 2.0, G.phizm[j], G.phizm[q], 0.0, nullptr, nd);
+          dealias(j + q, G.t2, KCOS, KCOS);
+          ew(G.phiz2, 1.0, G.t2, nullptr, 1.0, G.phiz2, nd);
+          ew(G.t2, 1.0, G.g2, G.t2, 0.0, nullptr, nd);
+          dealias(j + q + 2, G.t2, KCOS, KCOS);
+          ew(G.ge2p2, 1.0, G.t2, nullptr, 1.0, G.ge2p2, nd);
+        }
+      if (M >= 2) {
+        ew(G.t2, 1.0, G.phizm[M / 2], G.phizm[M / 2], 0.0, nullptr, nd);
+        dealias(2 * (M / 2), G.t2, KCOS, KCOS);
+        ew(G.phiz2, 1.0, G.t2, nullptr, 1.0, G.phiz2, nd);
+      }
+      for (int j = 2; j <= M - 1; ++j)
+        for (int q = std::max(0, std::min(M - 2 - j, j - 1)) + 1; q <= std::min(M - j, j - 1); ++q) {
+          ew(G.t2, 2.0, G.phizm[j], G.phizm[q], 0.0, nullptr, nd);
+          dealias(j + q, G.t2, KCOS, KCOS);
+          ew(G.phiz2, 1.0, G.t2, nullptr, 1.0, G.phiz2, nd);
+        }
+      // free-surface velocities and phi_t at z = eta (velocities.f90:656-773); the
+      // etapmext of init_velocity_comp's right-hand side start the H2 levels
+      double* pze = G.aH[1][1];  // extended phiz (the base modal P)
+      double* pxe = G.aH[2][1];  // extended grad phi_add
+      double* pye = G.aH[3][1];
+      double* pza = G.vit[3];
+      T2(KCOS, KCOS, INV, ext_src(P, n1, n1, n2), N1, N2, pze);
+      base64(KCOS, KCOS, FWD, src_plain<double, double>(phix_add, n1, n1, n2), n1, n2, Dst<double>{G.b2, n1, 0});
+      T2(KCOS, KCOS, INV, ext_src(G.b2, n1, n1, n2), N1, N2, pxe);
+      if (n2 > 1) {
+        base64(KCOS, KCOS, FWD, src_plain<double, double>(phiy_add, n1, n1, n2), n1, n2,
+               Dst<double>{G.b2, n1, 0});
+        T2(KCOS, KCOS, INV, ext_src(G.b2, n1, n1, n2), N1, N2, pye);
+      } else {
+        zero(pye, nd);
+      }
+      base64(KCOS, KCOS, FWD, src_plain<double, double>(phiz_add, n1, n1, n2), n1, n2, Dst<double>{G.b2, n1, 0});
+      T2(KCOS, KCOS, INV, ext_src(G.b2, n1, n1, n2), N1, N2, pza);
+      // detaphiz = geta2phiz2 + phiz2 - phisx (etax phizMm2)_d - phisy (etay phizMm2)_d
+      //            - phix_add (etax phizMm1)_d - phiy_add (etay phizMm1)_d + phiz_add phiz
+      double* det = G.vit[2];
+      ew(det, 1.0, G.ge2p2, nullptr, 1.0, G.phiz2, nd);
+      ew(G.t2, 1.0, G.etax, G.pMm2, 0.0, nullptr, nd);
+      dealias(2, G.t2, KSIN, KCOS);
+      ew(det, -1.0, G.phisx, G.t2, 1.0, det, nd);
+      ew(G.t2, 1.0, G.etay, G.pMm2, 0.0, nullptr, nd);
+      dealias(2, G.t2, KCOS, KSIN);
+      ew(det, -1.0, G.phisy, G.t2, 1.0, det, nd);
+      ew(G.t2, 1.0, G.etax, G.pMm1, 0.0, nullptr, nd);
+      dealias(2, G.t2, KSIN, KCOS);
+      ew(det, -1.0, pxe, G.t2, 1.0, det, nd);
+      ew(G.t2, 1.0, G.etay, G.pMm1, 0.0, nullptr, nd);
+      dealias(2, G.t2, KCOS, KSIN);
+      ew(det, -1.0, pye, G.t2, 1.0, det, nd);
+      ew(det, 1.0, pza, pze, 1.0, det, nd);
+      double* phim[4] = {G.b2, G.b3, G.b4, G.b0};  // x, y, z, t at z = eta (base, physical)
+      filter_ext(det, KCOS, KCOS, G.b0);
+      ew(G.b0, -1.0, G.b0, nullptr, 1.0, dphis3, nsb);
+      base64(KCOS, KCOS, INV, src_plain<double, double>(P, n1, n1, n2), n1, n2, Dst<double>{G.b4, n1, 0});
+      ew(G.t2, -1.0, G.etax, G.pMm1, 1.0, G.phisx, nd);
+      dealias(2, G.t2, KSIN, KCOS);
+      filter_ext(G.t2, KSIN, KCOS, G.b2);
+      if (n2 > 1) {
+        ew(G.t2, -1.0, G.etay, G.pMm1, 1.0, G.phisy, nd);
+        dealias(2, G.t2, KCOS, KSIN);
+        filter_ext(G.t2, KCOS, KSIN, G.b3);
+      } else {
+        zero(G.b3, nsb);
+      }
+      // --- HOSvel2_SWEET (velocities.f90:223-492): meta2 - 1 levels from z = eta down
+      for (int f = 0; f < 4; ++f) {
+        base64(KX[f], KY[f], FWD, src_plain<double, double>(phim[f], n1, n1, n2), n1, n2,
+               Dst<double>{G.b1, n1, 0});
+        hipLaunchKernelGGL(k_extend, dim3(nblk(nd)), dim3(256), 0, st, n1, n2, N1, N2, (const double*)G.b1,
+                           G.aH[f][1]);
+        T2(KX[f], KY[f], INV, ext_src(G.b1, n1, n1, n2), N1, N2, G.vit[f]);
+      }
+      std::vector<double*>* ea = &G.ep;   // powers of the level left
+      std::vector<double*>* eb = &G.ep2;  // powers of the level reached
+      auto level = [&](const std::vector<double*>* ela, const std::vector<double*>* elb, int shift) {
+        for (int iH = 1; iH <= M; ++iH) {
+          for (int f = 0; f < 4; ++f) {
+            zero(G.pa[f], nd);
+            if (elb) zero(G.pb[f], nd);
+          }
+          for (int j = iH; j >= 1; --j) {
+            if (j == 1 && iH > 1) continue;
+            const int jj = j, j2 = iH - (j - 1);
+            for (int f = 0; f < 4; ++f) {
+              T2(KX[f], KY[f], INV, ext_src(G.aH[f][j2], N1, N1, N2, f == 2 ? MUL_KTH2 : MUL_KTH, jj), N1, N2,
+                 G.phizi);
+              ew(G.pa[f], -1.0, G.phizi, (*ela)[jj], 1.0, G.pa[f], nd);
+              if (elb) ew(G.pb[f], 1.0, G.phizi, (*elb)[jj], 1.0, G.pb[f], nd);
+            }
+          }
+          if (iH > 1)
+            for (int f = 0; f < 4; ++f) {
+              T2(KX[f], KY[f], INV, ext_src(G.aH[f][iH], N1, N1, N2, f == 2 ? MUL_KTH2 : MUL_KTH, 1), N1, N2,
+                 G.phizi);
+              ew(G.pa[f], -1.0, G.phizi, (*ela)[1], 1.0, G.pa[f], nd);
+              if (elb) ew(G.pb[f], 1.0, G.phizi, (*elb)[1], 1.0, G.pb[f], nd);
+            }
+          for (int f = 0; f < 4; ++f) {
+            ew(G.vit[f], 1.0, G.pa[f], nullptr, 1.0, G.vit[f], nd);
+            if (elb) ew(G.vit[f], 1.0, G.pb[f], nullptr, 1.0, G.vit[f], nd);
+            dealias(iH + shift, G.vit[f], KX[f], KY[f]);
+            if (iH < M) {
+              dealias(iH + 1, G.pa[f], KX[f], KY[f]);
+              T2(KX[f], KY[f], FWD, ext_src(G.pa[f], N1, N1, N2), N1, N2, G.aH[f][iH + 1]);
+            }
+          }
+        }
+      };
+      for (int l = 1; l < meta2; ++l) {
+        // eta2 = eta (meta2 - l) / meta2 and its powers (dealiased at order j, :296-315)
+        ew(G.b1, (double)(meta2 - l) / (double)meta2, C.eta_hat, nullptr, 0.0, nullptr, nsb);
+        T2(KCOS, KCOS, INV, ext_src(G.b1, n1, n1, n2), N1, N2, (*eb)[1]);
+        for (int j = 1; j < M; ++j) {
+          ew((*eb)[j + 1], 1.0 / (j + 1), (*eb)[j], (*eb)[1], 0.0, nullptr, nd);
+          dealias(j, (*eb)[j + 1], KCOS, KCOS);
+        }
+        level(ea, eb, 0);
+        for (int f = 0; f < 4; ++f) T2(KX[f], KY[f], FWD, ext_src(G.vit[f], N1, N1, N2), N1, N2, G.aH[f][1]);
+        std::swap(ea, eb);
+      }
+      // --- HOSvel_SWEET (velocities.f90:40-216): from the last level to z = 0
+      for (int f = 0; f < 4; ++f) T2(KX[f], KY[f], FWD, ext_src(G.vit[f], N1, N1, N2), N1, N2, G.aH[f][1]);
+      level(ea, nullptr, 1);
+      // filter_ext then space_2_Fourier (:177-185): the base sine transforms drop the end modes
+      for (int f = 0; f < 4; ++f) {
+        filter_ext(G.vit[f], KX[f], KY[f], G.b1);
+        base64(KX[f], KY[f], FWD, src_plain<double, double>(G.b1, n1, n1, n2), n1, n2,
+               Dst<double>{G.out[f], n1, 0});
+      }
+      for (int f = 0; f < 5; ++f) out_dev[f] = G.out[f];
+      HIP_OK(hipGetLastError());
+    }
+  }
 
   void wavemaker(double t, const double* a1, const double* e1, const double* aad, double* ka1, double* ke1,
                  double* kad1, double* kadr, const S2* s2) {
@@ -5038,6 +5407,47 @@ int hos_get_wmk_motion(hos_ctx* c, double* alpha, double* alphat, double* x_top,
     for (int i = 0; i < n2; ++i) {
       x_top[i] = c->last_alpha * X[i];
       v_top[i] = c->last_alphat * X[i] + c->last_alpha * Xt[i];
+    }
+  });
+}
+
+int hos_swense_modes(hos_ctx* c, double t, int meta2, double* specx, double* specy, double* specz, double* spect,
+                     double* modes_fs, double* modes_fst, double* modes_add, double* modes_addt) {
+  return guard([&] {
+    if (!c->wmk) throw std::runtime_error("the SWENSE output is built for wavemaker tanks (igeom 1 or 2)");
+    if (filter_active(c->p.coeffilt[0]) || (c->n2 > 1 && filter_active(c->p.coeffilt[1])) ||
+        filter_active(c->p.coeffilt[2]))
+      throw std::runtime_error("the SWENSE output is built without filters (coeffilt = 1)");
+    if (meta2 < 1) throw std::runtime_error("meta2 must be >= 1");
+    const size_t ns = (size_t)c->n1 * c->n2, na = (size_t)c->n3a * c->n2;
+    hipStream_t st = c->st;
+    // init_velocity_comp's right-hand side at the committed state and time t
+    // (velocities.f90:571-577), into the stage-0 buffers
+    hipLaunchKernelGGL(k_step_begin, dim3(nblk(ns)), dim3(256), 0, st, c->n1, c->n2, c->pix, c->piy,
+                       (const double*)c->eta_hat, (const double*)c->phis_hat, c->G1, c->G2);
+    rhs_eval(c, t, 0.0, c->G1, c->G2, c->a1st, c->eta1st, c->a1st_add, 0, c->a2nd, c->eta2nd, c->a2nd_add,
+             c->a3rd_add);
+    double* outs[5];
+    c->eng->swense(meta2, outs);
+    HIP_OK(hipStreamSynchronize(st));
+    double* host[5] = {specx, specy, specz, spect, modes_fst};
+    for (int f = 0; f < 5; ++f) HIP_OK(hipMemcpy(host[f], outs[f], ns * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(modes_fs, c->eta_hat, ns * sizeof(double), hipMemcpyDeviceToHost));
+    // modesadd, modesaddt: the paddle modes of every order and their derivatives at the
+    // committed state (velocities.f90:623-628)
+    std::vector<double> h(na);
+    const double* adds[3] = {c->a1st_add, c->a2nd_add, c->a3rd_add};
+    const double* dadds[3] = {c->kad1[0], c->wmk2 ? c->kad2[0] : nullptr, c->wmk3 ? c->kad3[0] : nullptr};
+    for (size_t i = 0; i < na; ++i) modes_add[i] = modes_addt[i] = 0.0;
+    for (int q = 0; q < 3; ++q) {
+      if (adds[q]) {
+        HIP_OK(hipMemcpy(h.data(), adds[q], na * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < na; ++i) modes_add[i] += h[i];
+      }
+      if (dadds[q]) {
+        HIP_OK(hipMemcpy(h.data(), dadds[q], na * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < na; ++i) modes_addt[i] += h[i];
+      }
     }
   });
 }

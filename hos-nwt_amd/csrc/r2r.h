@@ -46,7 +46,7 @@ using hosfft::FftPlan;
 
 enum Kind { KCOS = 0, KSIN = 1 };
 enum Dir { FWD = 1, INV = -1 };
-enum Mul { MUL_NONE = 0, MUL_NEG_KX = 1, MUL_NEG_KY = 2, MUL_KTH = 3 };
+enum Mul { MUL_NONE = 0, MUL_NEG_KX = 1, MUL_NEG_KY = 2, MUL_KTH = 3, MUL_KTH2 = 4 };
 
 // ---------------------------------------------------------------------------
 // source / destination functors
@@ -72,12 +72,17 @@ struct Src {
     if (mul == MUL_NONE) return 1.0;
     if (mul == MUL_NEG_KX) return -(double)ix * pix;
     if (mul == MUL_NEG_KY) return -(double)iy * piy;
-    // kth: k^j tanh(k) for odd j, k^j for even j (HOS-NWT.f90:311-327)
+    // kth: k^j tanh(k) for odd j, k^j for even j (HOS-NWT.f90:311-327);
+    // kth2: k^j / tanh(k) for odd j (0 at k = 0), k^j for even j (:316-324)
     const double kx = (double)ix * pix, ky = (double)iy * piy;
     const double k = sqrt(kx * kx + ky * ky);
     double kj = k;
     for (int q = 1; q < jpow; ++q) kj *= k;
-    if (jpow & 1) kj *= (k < 20.0 ? tanh(k) : 1.0);
+    if (jpow & 1) {
+      const double th = (k < 20.0 ? tanh(k) : 1.0);
+      if (mul == MUL_KTH2) return (ix == 0 && iy == 0) ? 0.0 : kj / th;
+      kj *= th;
+    }
     return kj;
   }
   __device__ __forceinline__ T apply(U v, int ix, int iy) const {

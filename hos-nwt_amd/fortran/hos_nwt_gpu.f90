@@ -51,6 +51,10 @@ REAL(RP), ALLOCATABLE, TARGET :: dfr(:), dmxr(:), dmxi(:), dmzr(:), dmzi(:), dzd
 REAL(RP), PARAMETER :: z_dipmono = -0.25_rp, x_dip_si = 2.5_rp
 INTEGER, PARAMETER :: n_side = 6
 REAL(RP) :: x_dip
+! SWENSE output (i_sw = 1): the modes of the last computation (zero before the first)
+REAL(RP), ALLOCATABLE :: sw_x(:,:), sw_y(:,:), sw_z(:,:), sw_t(:,:), sw_fs(:,:), sw_fst(:,:), sw_add(:,:), &
+     sw_addt(:,:)
+INTEGER :: it_out
 TYPE(hos_params) :: p
 TYPE(C_PTR) :: ctx
 INTEGER(C_INT) :: n_acc, n_rej, fx, fy
@@ -93,6 +97,11 @@ theta_mono = theta_mono * PI / 180.0_rp
 IF (iramp == 0) Tramp = 0.0_rp
 IF (i_wmk < 1 .OR. i_wmk > 3) ERROR STOP 'i_wmk = 1, 2 or 3 is built here'
 IF (iprobes == 2) ERROR STOP 'pressure probes (iprobes = 2) are not built here'
+IF (i_sw == 1 .AND. .NOT. ((igeom == 1 .OR. igeom == 2) .AND. icase /= 1)) &
+   ERROR STOP 'the SWENSE output (i_sw = 1) is built for wavemaker tanks (igeom 1, 2)'
+IF (i_sw == 1 .AND. nargs >= 5) THEN
+   IF (prec /= 0) ERROR STOP 'the SWENSE output (i_sw = 1) runs in FP64 (precision 0)'
+END IF
 ! output files, before the nondimensionalisation as in main/HOS-NWT.f90:92
 CALL init_output()
 !
@@ -227,12 +236,18 @@ IF (idim == 1) THEN
    x_adim = h
 END IF
 ALLOCATE(etam(n1,n2), phism(n1,n2), xtop(n2), vtop(n2))
+ALLOCATE(sw_x(n1,n2), sw_y(n1,n2), sw_z(n1,n2), sw_t(n1,n2), sw_fs(n1,n2), sw_fst(n1,n2), &
+         sw_add(2*(n3-1),n2), sw_addt(2*(n3-1),n2))
+sw_x = 0.0_rp; sw_y = 0.0_rp; sw_z = 0.0_rp; sw_t = 0.0_rp; sw_fs = 0.0_rp; sw_fst = 0.0_rp
+sw_add = 0.0_rp; sw_addt = 0.0_rp
+it_out = 1  ! initialize sets it = 1, the time loop adds 1 before each output (HOS-NWT.f90:450)
 !
 time_cur = 0.0_rp
 n_rk_tot = 0; n_er_tot = 0; n_acc = 0; n_rej = 0
 DO WHILE (time_cur < T_stop + delt_out)
    n_rk_tot = n_rk_tot + n_acc
    n_er_tot = n_er_tot + n_rej
+   it_out = it_out + 1
    CALL hos_check(hos_get_physical(ctx, eta, phis, deta, phi_add), 'hos_get_physical')
    CALL hos_check(hos_get_wmk(ctx, alpha, pos_top), 'hos_get_wmk')
    CALL volume_energy()
@@ -254,6 +269,10 @@ DO WHILE (time_cur < T_stop + delt_out)
    h_rk = delt
    CALL hos_check(hos_advance(ctx, time_cur, time_next, h_rk, toler, dt_stab, delt_out, n_acc, n_rej), 'hos_advance')
    delt = h_rk
+   ! SWENSE: the H2 operator after the attempt that ends on the output time
+   ! (runge_kutta.f90:327-333), 10 levels
+   IF (i_sw == 1) CALL hos_check(hos_swense_modes(ctx, time_cur, 10_C_INT, sw_x, sw_y, sw_z, sw_t, sw_fs, sw_fst, &
+        sw_add, sw_addt), 'hos_swense_modes')
    ! check_slope (initial_condition.f90:687-716)
    CALL hos_check(hos_check_slope(ctx, 8.0_rp, 0.7_rp, fx, fy), 'hos_check_slope')
    IF (fx /= 0 .OR. fy /= 0) THEN
@@ -266,6 +285,7 @@ IF (i3d == 1) CLOSE(1)
 IF (imodes == 1) CLOSE(5)
 IF (iwmk /= 0 .AND. igeom /= 3) CLOSE(8)
 IF (iprobes == 1) CLOSE(9)
+IF (i_sw == 1) CLOSE(123)
 CALL hos_check(hos_destroy(ctx), 'hos_destroy')
 !
 CONTAINS
@@ -412,6 +432,12 @@ SUBROUTINE init_output()
          WRITE(5,'(15A)') 'VARIABLES = "kx", "ky","a_eta","a_phis","LOG10-a-eta","LOG10-a-phis"'
       END IF
    END IF
+   ! modes of the volumic fields for coupling (SWENSE), output.f90:219-228
+   IF (i_sw == 1) THEN
+      OPEN(123, FILE='Results/modes_HOS_SWENSE.dat', STATUS='REPLACE', FORM='FORMATTED', ACCESS='DIRECT', &
+           RECL=18*(n1))
+      IF (2 * (n3 - 1) > n1) ERROR STOP 'Change the value of n1 and/or n3 so that n1>=n3_add'
+   END IF
    IF (iwmk /= 0 .AND. igeom /= 3) THEN
       OPEN(8, FILE='Results/wmk_motion.dat', STATUS='UNKNOWN')
       CALL write_input(8)
@@ -546,7 +572,41 @@ SUBROUTINE output_time_step()
          END DO
       END IF
    END IF
+   ! spectral modes for SWENSE (output.f90:417-467): a header record at t = 0, then n2 x 8
+   ! records per output time (it_out is the reference's it)
+   IF (i_sw == 1) THEN
+      IF (ABS(time_cur) < TINY) THEN
+         WRITE(123,'(5000(ES17.10,1X))',REC=1) 1.0_rp*n1, 1.0_rp*2*(n3-1), 1.0_rp*n2, 1.0_rp/f_out, T_stop, &
+              xlen, ylen, h, (sw_x(i1,1), i1=9,n1)
+         WRITE(123,'(5000(ES17.10,1X))',REC=2) (sw_y(i1,1), i1=1,n1)
+         WRITE(123,'(5000(ES17.10,1X))',REC=3) (sw_z(i1,1), i1=1,n1)
+         WRITE(123,'(5000(ES17.10,1X))',REC=4) (sw_t(i1,1), i1=1,n1)
+         WRITE(123,'(5000(ES17.10,1X))',REC=5) (sw_fs(i1,1), i1=1,n1)
+         WRITE(123,'(5000(ES17.10,1X))',REC=6) (sw_fst(i1,1), i1=1,n1)
+         WRITE(123,'(5000(ES17.10,1X))',REC=7) (sw_add(j,1), j=1,2*(n3-1))
+         WRITE(123,'(5000(ES17.10,1X))',REC=8) (sw_addt(j,1), j=1,2*(n3-1))
+         DO i2 = 2, n2
+            CALL sw_block(8 * (i2 - 1), i2)
+         END DO
+      END IF
+      DO i2 = 1, n2
+         CALL sw_block((it_out - 1) * n2 * 8 + 8 * (i2 - 1), i2)
+      END DO
+   END IF
 END SUBROUTINE output_time_step
+!
+SUBROUTINE sw_block(r0, i2b)
+   INTEGER, INTENT(IN) :: r0, i2b
+   INTEGER :: j
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+1) (sw_x(j,i2b), j=1,n1)
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+2) (sw_y(j,i2b), j=1,n1)
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+3) (sw_z(j,i2b), j=1,n1)
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+4) (sw_t(j,i2b), j=1,n1)
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+5) (sw_fs(j,i2b), j=1,n1)
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+6) (sw_fst(j,i2b), j=1,n1)
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+7) (sw_add(j,i2b), j=1,2*(n3-1))
+   WRITE(123,'(5000(ES17.10,1X))',REC=r0+8) (sw_addt(j,i2b), j=1,2*(n3-1))
+END SUBROUTINE sw_block
 !
 ! file2wave (wave/wave_def.f90:181-278, with wave_init :282-329): <file_name>.cfg and
 ! .dat, components between the cut-off frequencies, nondimensionalised by the file's depth

@@ -157,6 +157,16 @@ INTERFACE
       TYPE(C_PTR), VALUE :: ctx
       REAL(C_DOUBLE), INTENT(OUT) :: eta_modes(*), phis_modes(*)
    END FUNCTION hos_get_modes
+   ! SWENSE output modes at an output time (IO/velocities.f90:223-784, output.f90:417-467)
+   INTEGER(C_INT) FUNCTION hos_swense_modes(ctx, t, meta2, specx, specy, specz, spect, modes_fs, modes_fst, &
+        modes_add, modes_addt) BIND(C, NAME='hos_swense_modes')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), VALUE :: t
+      INTEGER(C_INT), VALUE :: meta2
+      REAL(C_DOUBLE), INTENT(OUT) :: specx(*), specy(*), specz(*), spect(*), modes_fs(*), modes_fst(*)
+      REAL(C_DOUBLE), INTENT(OUT) :: modes_add(*), modes_addt(*)
+   END FUNCTION hos_swense_modes
    INTEGER(C_INT) FUNCTION hos_probes(ctx, np, xp, yp, eta_p) BIND(C, NAME='hos_probes')
       IMPORT :: C_INT, C_PTR, C_DOUBLE
       TYPE(C_PTR), VALUE :: ctx

@@ -103,6 +103,7 @@ def load() -> C.CDLL:
     lib.hos_get_wmk.argtypes = [vp, _dp, _dp]
     lib.hos_get_wmk_motion.argtypes = [vp, _dp, _dp, _dp, _dp]
     lib.hos_get_modes.argtypes = [vp, _dp, _dp]
+    lib.hos_swense_modes.argtypes = [vp, C.c_double, C.c_int] + [_dp] * 8
     lib.hos_probes.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     lib.hos_get_stats.argtypes = [vp, C.POINTER(HosStats)]
     lib.hos_synchronize.argtypes = [vp]

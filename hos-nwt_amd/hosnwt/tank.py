@@ -196,6 +196,15 @@ class Tank:
         check(self.lib.hos_get_modes(self.h, *[ptr(a) for a in out]))
         return dict(zip(("eta", "phis"), out))
 
+    def swense_modes(self, t: float, meta2: int = 10):
+        """SWENSE output of the committed state at time t (velocities.f90:223-784): the modes of
+        the velocities and of phi_t at z = 0 (specx, specy, specz, spect), modesFS, modesFSt
+        (n1 x n2) and modesadd, modesaddt (n3_add x n2), as modes_HOS_SWENSE.dat holds them."""
+        n1, n2, na = self.n1, self.n2, self.n3a
+        out = [np.zeros((n1, n2), order="F") for _ in range(6)] + [np.zeros((na, n2), order="F") for _ in range(2)]
+        check(self.lib.hos_swense_modes(self.h, t, meta2, *[ptr(a) for a in out]))
+        return dict(zip(("specx", "specy", "specz", "spect", "FS", "FSt", "add", "addt"), out))
+
     def probes(self, xp, yp=None):
         """Elevation at wave probes (nondimensional positions; output.f90:266-276)."""
         xp = np.ascontiguousarray(xp, dtype=np.float64)

@@ -223,6 +223,18 @@ int hos_get_modes(hos_ctx* c, double* eta_modes, double* phis_modes);
  * NULL in a 2-D tank) of the committed state: the mat_coscos sums of
  * output.f90:266-276 (initial_condition.f90:502-510). */
 int hos_probes(hos_ctx* c, int np, const double* xp, const double* yp, double* eta_p);
+/* SWENSE output at an output time (IO/velocities.f90:223-784, written to
+ * modes_HOS_SWENSE.dat by IO/output.f90:417-467): from the committed state at time t
+ * (the attempt that ended on the output time, runge_kutta.f90:327-333), the modes of the
+ * velocities and of phi_t at z = 0 after meta2 H2 levels (the reference passes 10):
+ * specx (sin-cos), specy (cos-sin), specz, spect (cos-cos), and modesFS (modal eta),
+ * modesFSt (modal d eta / dt): n1*n2 each; modesadd, modesaddt (paddle modes of every
+ * order and their derivatives): n3_add*n2.  FP64 contexts of wavemaker tanks (igeom 1, 2)
+ * without filters.  The reference's direct fallback for 2-D tanks whose free-surface
+ * check fails (HOSvel2_direct, a LAPACK solve) is not built. */
+int hos_swense_modes(hos_ctx* c, double t, int meta2, double* specx, double* specy, double* specz, double* spect,
+                     double* modes_fs, double* modes_fst, double* modes_add, double* modes_addt);
+
 int hos_get_stats(hos_ctx* c, hos_stats* s);
 /* Measurement helper (no reference counterpart): average duration in ms and
  * algorithmic HBM bytes of one extended-grid x pass and one y pass of a full

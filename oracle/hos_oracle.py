@@ -282,6 +282,17 @@ class Tables:
         kye = np.arange(p.Nd2) * piylen
         ke = np.sqrt(kxe[:, None] ** 2 + kye[None, :] ** 2)
         self.kth_ext = self._kth(ke, M)
+        # kth2_ext: k^j / tanh k for odd j (0 at k = 0), k^j for even j (HOS-NWT.f90:310-327)
+        self.kth2_ext = np.zeros(ke.shape + (M,))
+        thk = np.tanh(ke)
+        for j in range(1, M + 1):
+            if j % 2 == 1:
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    v = ke ** j / thk
+                v[0, 0] = 0.0
+                self.kth2_ext[..., j - 1] = v
+            else:
+                self.kth2_ext[..., j - 1] = ke ** j
         # mesh
         self.delx = p.xlen / (n1 - 1)
         self.dely = p.ylen / (n2 - 1) if n2 != 1 else 0.0
@@ -912,15 +923,27 @@ class Oracle:
             if iHOS < M:
                 phimext = self.dealias(iHOS + 1, phimext)
                 aHOS[iHOS + 1] = self.s2f_big(phimext)
+        phizMm2 = np.zeros((p.Nd1, p.Nd2))
+        phizm2 = [None] + [np.zeros((p.Nd1, p.Nd2)) for _ in range(M)]
         if M >= 2:
-            phizMm2 = np.zeros((p.Nd1, p.Nd2))
             for j in range(1, M - 1):
                 phizMm2 = phizMm2 + phizm[j]
+                phizm2[j] = phizm[j].copy()
             phizMm1 = phizMm2 + phizm[M - 1]
             phiz = self.filter_ext(phizMm1.copy()) + self.filter_ext(phizm[M].copy())
         else:
             phizMm1 = np.zeros((p.Nd1, p.Nd2))
             phiz = self.filter_ext(phizm[1].copy())
+        # phiz2Mm2, read only by the SWENSE output (resol_HOS.f90:807-823; mHOSm3div2,
+        # mHOSdiv2m1 of common_vars.f90:49); phizm2(j+1) accumulates in place as there
+        phiz2Mm2 = np.zeros((p.Nd1, p.Nd2))
+        if M >= 4:
+            m3d2 = max(1, (M - 3) // 2)
+            d2m1 = max(1, M // 2 - 1)
+            for j in range(1, m3d2 + 1):
+                phiz2Mm2 = phiz2Mm2 + 2.0 * phizm2[j] * phizm2[M - 2 - j]
+                phizm2[j + 1] = phizm2[j + 1] + phizm2[j]
+            phiz2Mm2 = phiz2Mm2 + phizm2[d2m1] ** 2
         W1 = self.filter_ext(phizm[1].copy())
         geta2phiz = np.zeros((p.Nd1, p.Nd2))
         for j in range(1, M - 2):
@@ -948,7 +971,7 @@ class Oracle:
         phiz2 = temp1
         self.last_aHOS = aHOS  # modal potentials of orders 1..M on the extended grid
         return dict(phiz=phiz, W1=W1, phizMm1=phizMm1, geta2phiz=geta2phiz,
-                    geta2phiz2=geta2phiz2, phiz2=phiz2)
+                    geta2phiz2=geta2phiz2, phiz2=phiz2, phizMm2=phizMm2, phiz2Mm2=phiz2Mm2)
 
     # ----- wavemaker first order (resol_wmkr.f90:109-210) ----------------
     SECOND_FIELDS = ("phi", "phit", "phix", "phiy", "phiz", "phizz", "phixt", "phiyt", "phizt", "phixx",
@@ -1160,6 +1183,8 @@ class Oracle:
         etapm, etax, etay, gradeta2, phisx, phisy = self.phisxy_etaxy(phisrk, etark)
         self.etax, self.etay = etax, etay
         h = self.hosphis_modes(phisrk, etapm, gradeta2)
+        # module globals of this evaluation read by the SWENSE output (velocities.f90:501-784)
+        self.mod = dict(etapm=etapm, etax=etax, etay=etay, phisx=phisx, phisy=phisy, h=h)
         detarkext = -phisx * etax - phisy * etay
         dphisrkext = -phisx * phisx - phisy * phisy
         detarkext = self.dealias(2, detarkext)
@@ -1221,6 +1246,7 @@ class Oracle:
         phix_add = self.f2s(phix_add)
         phiy_add = self.f2s(phiy_add)
         phiz_add = self.f2s(phiz_add)
+        self.mod.update(phix_add=phix_add, phiy_add=phiy_add, phiz_add=phiz_add, phit_add=phit_add)
         phix_add_ext = self.f2s_big(phix_add_ext)
         phiy_add_ext = self.f2s_big(phiy_add_ext)
         phiz_add_ext = self.f2s_big(phiz_add_ext)
@@ -1416,6 +1442,138 @@ class Oracle:
             self.alpha, self.alphat, self.alphatt = ramp(0.0, p)
             self.phi_add = phi_dipoles(eta, 0.0, self.alpha, self.alphat, p, tb.cx[:, None] + 0.0 * eta)[0]
         return State(eta, phis, za, za.copy(), z.copy(), z.copy())
+
+    # ----- SWENSE output (velocities.f90:40-784, runge_kutta.f90:327-333) ------------
+    VEL_KINDS = (("sin", "cos"), ("cos", "sin"), ("cos", "cos"), ("cos", "cos"))  # x, y, z, t
+
+    def init_velocity_comp(self, t, h, G1, G2, eta_l, phis_l, a1st, eta1st, a_add, a2nd=None, eta2nd=None):
+        """init_velocity_comp (velocities.f90:501-784) for a wavemaker tank (igeom 1, 2): the
+        right-hand side re-evaluated at the state (G1, G2) of an attempt of step h ending at t,
+        the full time derivatives, and the free-surface velocities and phi_t at z = eta
+        (phimx, phimy, phimz, phimt; n1 x n2, physical).  Also sets modesFSt, modesadd and
+        modesaddt.  a_add: the attempt's paddle modes (a_add_1, a_add_2, a_add_3; None = 0)."""
+        p, tb = self.p, self.tb
+        if not p.has_wmk:
+            raise ValueError("the SWENSE output is restated for wavemaker tanks (igeom 1, 2)")
+        self.time = t
+        a1, a2, a3 = (np.zeros((p.n3_add, p.n2)) if a is None else a for a in a_add)
+        z = np.zeros((p.n1, p.n2))
+        a2nd = z if a2nd is None else a2nd
+        eta2nd = z if eta2nd is None else eta2nd
+        dG1, dG2, da1, deta1, d1, dr = self.solve_hos2(a1st, eta1st, G2, G1, a1, h,
+                                                       a2nd=a2nd if p.i_wmk >= 2 else None,
+                                                       eta2nd=eta2nd if p.i_wmk >= 2 else None,
+                                                       a_add_2=a2 if p.i_wmk >= 2 else None,
+                                                       a_add_3=a3 if p.i_wmk >= 3 else None)
+        d2 = self.second[2] if self.second is not None else np.zeros_like(d1)
+        d3 = self.third if self.third is not None else np.zeros_like(d1)
+        m = self.mod
+        hm = m["h"]
+        dG2 = dG2.copy()
+        dphis3_1 = dG2[0, 0]
+        dG2[0, 0] = 0.0
+        eta3 = self.s2f(eta_l)
+        phis3 = self.s2f(phis_l)
+        deta3, dphis3 = self.gtof(dG1, dG2, h)
+        om = tb.omega
+        d3p = dphis3.copy()
+        de3 = deta3.copy()
+        mask = np.ones_like(om, bool)
+        mask[0, 0] = False
+        d3p[mask] = dphis3[mask] / om[mask] - eta3[mask]
+        de3[mask] = deta3[mask] + phis3[mask] * om[mask] ** 2
+        d3p[0, 0] = dphis3_1 - eta3[0, 0]
+        self.modesFSt = de3.copy()
+        dphis3 = self.f2s(d3p)
+        self.modesadd = a1 + a2 + a3
+        self.modesaddt = d1 + d2 + d3
+        etax, etay, phisx, phisy = m["etax"], m["etay"], m["phisx"], m["phisy"]
+        etax2 = self.dealias(2, etax * etax)
+        etay2 = self.dealias(2, etay * etay)
+        phiz_etax = self.dealias(2, etax * hm["phizMm1"], "sin", "cos")
+        phiz_etay = self.dealias(2, etay * hm["phizMm1"], "cos", "sin")
+        pm2_etax = self.dealias(2, etax * hm["phizMm2"], "sin", "cos")
+        pm2_etay = self.dealias(2, etay * hm["phizMm2"], "cos", "sin")
+        del etax2, etay2  # they feed only the pressure check of the reference
+        pxa, pya, pza = (self.s2f(m[k]) for k in ("phix_add", "phiy_add", "phiz_add"))
+        phimz = self.s2f(hm["phiz"])
+        pxa_e, pya_e, pza_e, pz_e = (self.f2s_big(self.extend(a)) for a in (pxa, pya, pza, phimz))
+        phimz = self.f2s(phimz)
+        phimx_ext = phisx - etax * hm["phizMm1"]
+        phimy_ext = phisy - etay * hm["phizMm1"]
+        detaphiz = (hm["geta2phiz2"] + hm["phiz2"] - phisx * pm2_etax - phisy * pm2_etay - pxa_e * phiz_etax
+                    - pya_e * phiz_etay + pza_e * pz_e)
+        phimx_ext = self.dealias(2, phimx_ext, "sin", "cos")
+        phimy_ext = self.dealias(2, phimy_ext, "cos", "sin")
+        phimt = dphis3 - self.filter_ext(detaphiz)
+        phimx = self.filter_ext(phimx_ext, "sin", "cos")
+        phimy = self.filter_ext(phimy_ext, "cos", "sin")
+        return phimx, phimy, phimz, phimt
+
+    def _vel_level(self, aH, vit, etapm_a, etapm_b, iH_dealias_shift):
+        """One HOS re-expansion of the four velocity fields (the order loop shared by
+        HOSvel_SWEET and HOSvel2_SWEET): aH[f][j] modal potentials by order, vit[f] physical
+        extended fields; etapm_a subtracted (the level left), etapm_b added (the level reached,
+        HOSvel2_SWEET only)."""
+        p, tb = self.p, self.tb
+        M = p.mHOS
+        for iH in range(1, M + 1):
+            pa = [np.zeros((p.Nd1, p.Nd2)) for _ in range(4)]
+            pb = [np.zeros((p.Nd1, p.Nd2)) for _ in range(4)]
+            for j in list(range(iH, 1, -1)) + [None]:
+                if j is None:
+                    jj, j2 = 1, iH
+                else:
+                    jj, j2 = j, iH - (j - 1)
+                for f, (cx, cy) in enumerate(self.VEL_KINDS):
+                    mult = tb.kth2_ext[:, :, jj - 1] if f == 2 else tb.kth_ext[:, :, jj - 1]
+                    phizi = self.f2s_big(aH[f][j2] * mult, cx, cy)
+                    pa[f] = pa[f] - phizi * etapm_a[jj]
+                    if etapm_b is not None:
+                        pb[f] = pb[f] + phizi * etapm_b[jj]
+            for f, (cx, cy) in enumerate(self.VEL_KINDS):
+                vit[f] = vit[f] + pa[f] + (pb[f] if etapm_b is not None else 0.0)
+                vit[f] = self.dealias(iH + iH_dealias_shift, vit[f], cx, cy)
+                if iH < M:
+                    aH[f][iH + 1] = self.s2f_big(self.dealias(iH + 1, pa[f], cx, cy), cx, cy)
+        return vit
+
+    def hosvel2_sweet(self, meta2, t, h, G1, G2, eta_l, phis_l, a1st, eta1st, a_add, a2nd=None, eta2nd=None):
+        """HOSvel2_SWEET (velocities.f90:223-492) + HOSvel_SWEET (:40-216): the velocity and
+        phi_t modes at z = 0, moved from z = eta in meta2 levels (eta (meta2 - l) / meta2).
+        Returns (modesspecx, modesspecy, modesspecz, modesspect), n1 x n2 modal; the coeffilt
+        filters of the reference's last step are not restated (coeffilt = 1)."""
+        p = self.p
+        M = p.mHOS
+        if min(p.coeffilt) < 1.0:
+            raise ValueError("the SWENSE output is restated without filters")
+        phim = self.init_velocity_comp(t, h, G1, G2, eta_l, phis_l, a1st, eta1st, a_add, a2nd, eta2nd)
+        etapm2 = [None] + [self.mod["etapm"][j].copy() for j in range(1, M + 1)]
+        aH = [[None] * (M + 2) for _ in range(4)]
+        vit = []
+        for f, (cx, cy) in enumerate(self.VEL_KINDS):
+            a = self.extend(self.s2f(phim[f], cx, cy))
+            aH[f][1] = a
+            vit.append(self.f2s_big(a, cx, cy))
+        for ii1 in range(1, meta2):
+            etapm = [None] + [etapm2[j].copy() for j in range(1, M + 1)]
+            eta2 = eta_l * (meta2 - 1 - (ii1 - 1)) / meta2
+            as2 = self.s2f(eta2)
+            etapm2 = [None] * (M + 1)
+            etapm2[1] = self.f2s_big(self.extend(as2))
+            for j in range(1, M):
+                etapm2[j + 1] = self.dealias(j, etapm2[j] / (j + 1) * etapm2[1])
+            vit = self._vel_level(aH, vit, etapm, etapm2, 0)
+            for f, (cx, cy) in enumerate(self.VEL_KINDS):
+                aH[f][1] = self.s2f_big(vit[f], cx, cy)
+        # HOSvel_SWEET: from the last level to z = 0
+        for f, (cx, cy) in enumerate(self.VEL_KINDS):
+            aH[f][1] = self.s2f_big(vit[f], cx, cy)
+        vit = self._vel_level(aH, vit, etapm2, None, 1)
+        out = []
+        for f, (cx, cy) in enumerate(self.VEL_KINDS):
+            out.append(self.s2f(self.filter_ext(vit[f], cx, cy), cx, cy))
+        return tuple(out)
 
     # ----- volume / energy (vol_energy.f90:36-119) ----------------------------
     def volume(self, eta):

@@ -160,6 +160,72 @@ def test_fortran_driver_dipole_matches_oracle(tmp_path):
     np.testing.assert_allclose(got[:, 2:5], ref[:, 2:5], rtol=0, atol=2e-4 * scale)
 
 
+def swense_records(path, n1):
+    """Records of modes_HOS_SWENSE.dat (formatted direct access, RECL = 18 n1; output.f90:219-220,
+    :417-467): one list of floats per record."""
+    raw = path.read_bytes()
+    rl = 18 * n1
+    step = rl + 1 if len(raw) % (rl + 1) == 0 and raw[rl:rl + 1] == b"\n" else rl
+    assert len(raw) % step == 0
+    return [[float(v) for v in raw[i:i + rl].decode().split()] for i in range(0, len(raw), step)]
+
+
+@pytest.mark.gpu
+def test_fortran_driver_writes_swense_modes(tmp_path):
+    """i_sw = 1: modes_HOS_SWENSE.dat holds the header record, a zero block at t = 0 and, at
+    each output time, the n2 x 8 records of the H2 operator's modes computed after the last
+    attempt (10 levels), equal to hos_swense_modes of the same run to the file's 10 digits."""
+    assert os.path.exists(EXE), "build the Fortran driver with hos-nwt_amd/build.sh"
+    T, n1, M, tstop, toler = 1.5, 64, 3, 1.0, 1e-7
+    s = make_setup(n1=n1, n2=1, M=M, n3=9, xlen=20.0, T=T, T_stop=tstop, f_out=2.0 / T, toler=toler)
+    text = INPUT.format(amp=f"{s.cfg.amp_mono:.16e}", nu=f"{1.0 / T:.16e}", tstop=f"{tstop:.16e}",
+                        fout=f"{2.0 / T:.16e}", i_wmk=1, toler=f"{toler:.16e}", iprobes=0, flag=0, icase=2,
+                        file_name="none", igeom=1, d_hinge=f"{0.0:.16e}")
+    text = text.replace("i_sw             :: 0", "i_sw             :: 1")
+    (tmp_path / "input_HOS-NWT.dat").write_text(text)
+    r = subprocess.run([EXE, str(n1), "1", "9", str(M)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    recs = swense_records(tmp_path / "Results" / "modes_HOS_SWENSE.dat", n1)
+    n3a = 16
+    # header: n1, n3_add, n2, 1/f_out, T_stop, xlen, ylen, h, then modes 9..n1 of the t = 0 block
+    hdr = recs[0]
+    assert hdr[:3] == [n1, n3a, 1.0]
+    np.testing.assert_allclose(hdr[3:8], [1.0 / s.f_out, s.T_stop, s.xlen, 1.0, 1.0], rtol=1e-9)
+    from hosnwt.tank import Tank
+    tank = Tank(s, precision=0)
+    try:
+        tank.initialize()
+        t, h = 0.0, min(0.95 * s.dt_stab, s.delt_out)
+        # the driver's loop (HOS-NWT.f90:446-581): an output, then the advance whose last
+        # attempt computes the next output's modes; none before the first (zeros)
+        blocks, cur = [], None
+        while t < s.T_stop + s.delt_out:
+            blocks.append(cur)
+            if abs(t - s.T_stop) <= 2.2e-16:
+                break
+            tn = min(t + s.delt_out, s.T_stop + s.delt_out)
+            h, _, _ = tank.advance(tn, h)
+            t = tank.time
+            cur = tank.swense_modes(t, 10)
+        assert len(recs) == 8 * (len(blocks) + 1)
+        for ib, b in enumerate(blocks):
+            rr = recs[8 * (ib + 1):8 * (ib + 2)]
+            if b is None:
+                assert all(not any(x) for x in rr)
+                continue
+            ref = [b["specx"][:, 0], b["specy"][:, 0], b["specz"][:, 0], b["spect"][:, 0], b["FS"][:, 0],
+                   b["FSt"][:, 0], b["add"][:, 0], b["addt"][:, 0]]
+            for q in range(8):
+                got = np.array(rr[q])
+                assert got.shape == ref[q].shape, (ib, q)
+                np.testing.assert_allclose(got, ref[q], rtol=0, atol=1e-8 * max(np.abs(ref[q]).max(), 1e-30),
+                                           err_msg=f"block {ib} record {q}")
+            assert np.abs(ref[0]).max() > 0
+    finally:
+        tank.close()
+
+
 def data_rows(path):
     """Numeric rows of a Tecplot file (skips the '#' echo, TITLE, VARIABLES and ZONE lines)."""
     rows = []
