@@ -63,6 +63,7 @@ EXPORTS = [
     "hos_mlp_forward", "hos_mlp_loss_grad", "hos_mlp_set_data", "hos_mlp_train",
     "hos_advance_learned", "hos_learned_last", "hos_learned_reset", "hos_mlp_forward_noalloc",
     "hos_advance_graph", "hos_advance_ensemble_graph", "hos_dense_product", "hos_get_schedule",
+    "hos_swense_modes",
 ]
 
 _lib = None
