@@ -159,6 +159,49 @@ __device__ __forceinline__ void run_passes(C2<T>* v, T* line, int es, int tl, co
   }
 }
 
+// FP32 lines exchange both components at once: a complex value is one 8-byte
+// LDS access, at the positions of the 8-byte scheme (W = 32), so each exchange
+// takes one write, one read and two barriers instead of two of each and four.
+// (FP64 keeps one component at a time: a whole complex line of 6144 points
+// would hold 96 KiB of LDS and one workgroup per CU.)
+template <typename T, int V, int R, int Ns, bool YP>
+__device__ __forceinline__ void pass_write_cx(const C2<T>* v, C2<T>* line, int es, int tl, int ntl) {
+#pragma unroll
+  for (int b = 0; b < V / R; ++b) {
+    const int j = tl + b * ntl;
+    const int k = j % Ns;
+    const int o = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) line[(YP ? es : 1) * xpos<double, YP, R, Ns>(o + r * Ns)] = v[b * R + r];
+  }
+}
+
+template <typename T, int V, int R2, int RW, int NsW, bool YP>
+__device__ __forceinline__ void pass_read_cx(C2<T>* v, const C2<T>* line, int es, int tl, int ntl, int L) {
+  const int stride = L / R2;
+#pragma unroll
+  for (int b = 0; b < V / R2; ++b) {
+    const int j = tl + b * ntl;
+#pragma unroll
+    for (int r = 0; r < R2; ++r) v[b * R2 + r] = line[(YP ? es : 1) * xpos<double, YP, RW, NsW>(j + r * stride)];
+  }
+}
+
+template <typename T, int V, bool YP, int L, int Ns, int R, int... Rest>
+__device__ __forceinline__ void run_passes_cx(C2<T>* v, C2<T>* line, int es, int tl, const C2<T>* t0,
+                                              const C2<T>* t1) {
+  constexpr int ntl = L / V;
+  pass_compute<T, V, R>(v, tl, ntl, Ns, L, t0, t1);
+  if constexpr (sizeof...(Rest) > 0) {
+    constexpr int R2 = hosfft2_first<Rest...>::value;
+    pass_write_cx<T, V, R, Ns, YP>(v, line, es, tl, ntl);
+    __syncthreads();
+    pass_read_cx<T, V, R2, R, Ns, YP>(v, line, es, tl, ntl, L);
+    __syncthreads();
+    run_passes_cx<T, V, YP, L, Ns * R, Rest...>(v, line, es, tl, t0, t1);
+  }
+}
+
 // Position of value i (= b*R + r) in the layout of radix R.
 template <int R>
 __device__ __forceinline__ int layout_pos(int i, int tl, int ntl, int L) {
@@ -292,13 +335,18 @@ struct LineBatch {
   D dst[kMaxBatch];
 };
 
-template <typename T, int V, bool YP, class S, class D, int... Rs>
+// NC > 0 fixes the lines per workgroup at compile time (the y tiles: NC complex
+// lines of 2 columns interleaved in LDS), so that the LDS positions of the
+// exchanges are shifts and constant offsets rather than products with a runtime
+// stride; NC = 0 takes ncl from the launch (x passes).
+template <typename T, int V, bool YP, class S, class D, int NC, int... Rs>
 __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
                                       ? 1024
                                       : ((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256),
                                   sizeof(T) == 8 ? 2 : (!YP ? 1 : (sizeof(typename D::Pre) > 2 * sizeof(T) ? 3 : 4)))
-    r2r_seq_kernel(int N, int dir, int ncl, LineBatch<S, D> bt,
+    r2r_seq_kernel(int N, int dir, int ncl_arg, LineBatch<S, D> bt,
                                                                  Plan2<T> pl) {
+  const int ncl = NC > 0 ? NC : ncl_arg;
   constexpr int L = hosfft2_prod<Rs...>::value;
   const int item = blockIdx.y;
   const int kind = bt.kind[item], n_in = bt.n_in[item], n_out = bt.n_out[item], nlines = bt.nlines[item];
@@ -307,9 +355,13 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
   constexpr int ntl = L / V;
   constexpr int R0 = hosfft2_first<Rs...>::value;
   constexpr int Rl = hosfft2_last<Rs...>::value;
+  // FP32: complex exchanges (pl.lp complex values per line); FP64: one component
+  // at a time (pl.lp reals per line)
+  constexpr bool CX = sizeof(T) == 4;
+  constexpr int XW = CX ? 2 : 1;  // reals per exchanged element
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  C2<T>* t0 = reinterpret_cast<C2<T>*>(lds + (size_t)ncl * pl.lp);
+  C2<T>* t0 = reinterpret_cast<C2<T>*>(lds + (size_t)ncl * pl.lp * XW);
   C2<T>* t1 = t0 + 64;
   const int tid = threadIdx.x;
   int cl, tl;
@@ -320,6 +372,9 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
     cl = tid / ntl;
     tl = tid - cl * ntl;
   }
+  // (lane ranges the compiler may use to fold the mirror tests and LDS positions)
+  __builtin_assume(tl >= 0 && tl < ntl);
+  __builtin_assume(cl >= 0 && cl < ncl);
   int grp = blockIdx.x;
   if (YP) {
     const int nb = gridDim.x;
@@ -331,6 +386,7 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
   // exchange buffer of this lane's line: x lines are contiguous regions of lp
   // values; the lines of a y tile are interleaved element by element
   T* line = YP ? lds + cl : lds + (size_t)cl * pl.lp;
+  C2<T>* line2 = YP ? reinterpret_cast<C2<T>*>(lds) + cl : reinterpret_cast<C2<T>*>(lds) + (size_t)cl * pl.lp;
   const T* mult = nullptr;
   if (src.mul != 0) {
     // multipliers of the two real lines of every complex line: line[q*N + e]
@@ -338,17 +394,23 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
       const int c2 = idx / (2 * n_in), rem = idx - c2 * 2 * n_in;
       const int h = rem / n_in, e = rem - h * n_in;
       const int ln = 2 * (grp * ncl + c2) + h;
-      lds[(size_t)c2 * pl.lp + h * N + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
+      lds[(size_t)c2 * pl.lp * XW + h * N + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
     }
     __syncthreads();
-    mult = lds + (size_t)cl * pl.lp;
+    mult = lds + (size_t)cl * pl.lp * XW;
   }
   C2<T> v[V];
   load_line<T, V, R0, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb, mult);
   for (int m = tid; m < 64; m += blockDim.x) t0[m] = pl.tw0[m];
   for (int m = tid; m < (L + 63) / 64; m += blockDim.x) t1[m] = pl.tw1[m];
   __syncthreads();
-  run_passes<T, V, YP, L, 1, Rs...>(v, line, ncl, tl, t0, t1);
+  if constexpr (CX) {
+    (void)line;
+    run_passes_cx<T, V, YP, L, 1, Rs...>(v, line2, ncl, tl, t0, t1);
+  } else {
+    (void)line2;
+    run_passes<T, V, YP, L, 1, Rs...>(v, line, ncl, tl, t0, t1);
+  }
   store_line<T, V, Rl, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb);
 }
 

@@ -454,7 +454,8 @@ struct LinePlan {
     dev2.npass = (int)rs.size();
     for (size_t i = 0; i < rs.size(); ++i) dev2.radix[i] = rs[i];
     dev2.ntl = L / V;
-    dev2.lp = seq_lds_stride(L, rs, (int)sizeof(T));
+    // FP32 lines exchange complex values (8 bytes), FP64 lines one component (8 bytes)
+    dev2.lp = seq_lds_stride(L, rs, 8);
     std::vector<C2<T>> t0(64), t1((L + 63) / 64);
     for (int m = 0; m < 64; ++m) {
       const double ang = -2.0 * M_PI * (double)m / (double)L;
@@ -569,7 +570,7 @@ struct Grid2 {
       // right-hand side run 2-7 % faster with 8, whose smaller LDS footprint
       // keeps two tiles per CU (FP32: 8 and 16 equal on the full step)
       TX = sc.tx > 0 ? sc.tx : 8;
-      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * sizeof(T) > 128 * 1024 || (TX / 2) * py.dev2.ntl > 512))
+      while (TX > 2 && ((size_t)(TX / 2) * py.dev2.lp * 8 > 128 * 1024 || (TX / 2) * py.dev2.ntl > 512))
         TX /= 2;
     }
     sched = GridSched{px.V2 && px.sched == 1 && seq_alt_values(px.dev2.L) ? 1 : 0,
@@ -632,8 +633,8 @@ inline int batch_max_lines(const LineBatch<S, D>& b) {
   return m;
 }
 
-template <typename T, int V, bool YP, class S, class D, int... Rs>
-void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+template <typename T, int V, bool YP, class S, class D, int NC, int... Rs>
+void launch_seq_nc(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   static_assert(hosfft2::hosfft2_prod<Rs...>::value % V == 0, "V must divide the line length");
   {
     constexpr int rs[] = {Rs...};
@@ -642,16 +643,33 @@ void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<
     if (!same) throw std::runtime_error("compiled radix sequence does not match the plan of length " +
                                         std::to_string(pl.dev2.L));
   }
-  const size_t lds = (size_t)ncl * pl.dev2.lp * sizeof(T) + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
+  // exchange elements of 8 bytes: a component in FP64, a complex value in FP32
+  const size_t lds = (size_t)ncl * pl.dev2.lp * 8 + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
   const int nt = ncl * pl.dev2.ntl;
   const int nl = batch_max_lines(b);
   if (nl <= 0) return;
   int groups = (nl + 2 * ncl - 1) / (2 * ncl);
   if (YP) groups = (groups + 7) / 8 * 8;  // XCD-aware tile order needs a multiple of 8
-  auto kern = hosfft2::r2r_seq_kernel<T, V, YP, S, D, Rs...>;
+  auto kern = hosfft2::r2r_seq_kernel<T, V, YP, S, D, NC, Rs...>;
   allow_lds(kern, lds);
   hipLaunchKernelGGL(kern, dim3(groups, b.n), dim3(nt), lds, st, N, dir, ncl, b, pl.dev2);
   HIP_OK(hipGetLastError());
+}
+
+// y tiles: the lines per workgroup (tile width / 2) as a compile-time constant
+template <typename T, int V, bool YP, class S, class D, int... Rs>
+void launch_seq(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
+  if constexpr (YP) {
+    switch (ncl) {
+      case 1: launch_seq_nc<T, V, YP, S, D, 1, Rs...>(pl, ncl, N, dir, b, st); return;
+      case 2: launch_seq_nc<T, V, YP, S, D, 2, Rs...>(pl, ncl, N, dir, b, st); return;
+      case 4: launch_seq_nc<T, V, YP, S, D, 4, Rs...>(pl, ncl, N, dir, b, st); return;
+      case 8: launch_seq_nc<T, V, YP, S, D, 8, Rs...>(pl, ncl, N, dir, b, st); return;
+      default: throw std::runtime_error("y tile of " + std::to_string(2 * ncl) + " columns is not compiled");
+    }
+  } else {
+    launch_seq_nc<T, V, YP, S, D, 0, Rs...>(pl, ncl, N, dir, b, st);
+  }
 }
 
 // The compiled schedules: length, values per thread, radix sequence.
