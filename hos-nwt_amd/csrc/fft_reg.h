@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "fft_lds.h"
 
 namespace hosfft2 {
@@ -236,10 +238,19 @@ __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int nt
     const bool use = (e < n_in) & !(kind == hosnwt_kind_sin && edge);
     T a = (T)ra[i], b = (T)rb[i];
     if (mult) {
-      // multipliers of both real lines, precomputed in LDS: [2][N]
+      // multipliers of both real lines, precomputed in LDS: [2][N].  With a
+      // source of the transform type the product is formed in that type: the
+      // product of two floats is exact in double, so rounding it once to float
+      // gives the float product; a double source is multiplied in double and
+      // rounded once.
       const int ec = e < n_in ? e : 0;
-      a = (T)((double)mult[ec] * (double)ra[i]);
-      b = (T)((double)mult[N + ec] * (double)rb[i]);
+      if constexpr (std::is_same<decltype(src.raw(0, 0)), T>::value) {
+        a = mult[ec] * ra[i];
+        b = mult[N + ec] * rb[i];
+      } else {
+        a = (T)((double)mult[ec] * (double)ra[i]);
+        b = (T)((double)mult[N + ec] * (double)rb[i]);
+      }
     }
     T f = (dir < 0 && !edge) ? (T)0.5 : (T)1;
     if (mir && kind == hosnwt_kind_sin) f = -f;
