@@ -211,22 +211,32 @@ __device__ __forceinline__ int layout_pos(int i, int tl, int ntl, int L) {
   return tl + b * ntl + r * (L / R);
 }
 
-template <typename T, int V, int R, bool YP, class S>
+// N is a compile-time constant at the call (L / 2 + 1), so that the mirror and
+// edge tests of most values fold.  A value outside the input extent, or of a
+// line beyond the batch, is loaded as zero (the load's offset lies beyond the
+// field's buffer resource).
+// P2 (y tiles whose line pairs all hold data): the two columns of a line pair
+// are adjacent, so each row's pair is one load.
+template <typename T, int V, int R, bool YP, bool P2, class S>
 __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int ntl, int L, int N, int kind, int dir,
                                           int n_in, int la, int lb, bool ha, bool hb, const T* mult) {
-  // phase 1: issue every load (branch-free); phase 2: masks, multipliers, signs
+  // phase 1: issue every load (branch-free); phase 2: multipliers, signs
   decltype(src.raw(0, 0)) ra[V], rb[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     const int p = layout_pos<R>(i, tl, ntl, L);
     const int e = (p >= N) ? L - p : p;
-    const int ec = e < n_in ? e : 0;
-    if (YP) {
-      ra[i] = src.raw(ha ? la : 0, ec);
-      rb[i] = src.raw(hb ? lb : 0, ec);
+    const bool in = e < n_in;
+    if constexpr (YP && P2) {
+      const auto q = src.raw2_if(la, e, in);
+      ra[i] = q.x;
+      rb[i] = q.y;
+    } else if (YP) {
+      ra[i] = src.raw_if(la, e, in & ha);
+      rb[i] = src.raw_if(lb, e, in & hb);
     } else {
-      ra[i] = src.raw(ec, ha ? la : 0);
-      rb[i] = src.raw(ec, hb ? lb : 0);
+      ra[i] = src.raw_if(e, la, in & ha);
+      rb[i] = src.raw_if(e, lb, in & hb);
     }
   }
 #pragma unroll
@@ -235,7 +245,6 @@ __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int nt
     const bool mir = p >= N;
     const int e = mir ? L - p : p;
     const bool edge = (e == 0 || e == N - 1);
-    const bool use = (e < n_in) & !(kind == hosnwt_kind_sin && edge);
     T a = (T)ra[i], b = (T)rb[i];
     if (mult) {
       // multipliers of both real lines, precomputed in LDS: [2][N].  With a
@@ -254,15 +263,51 @@ __device__ __forceinline__ void load_line(C2<T>* v, const S& src, int tl, int nt
     }
     T f = (dir < 0 && !edge) ? (T)0.5 : (T)1;
     if (mir && kind == hosnwt_kind_sin) f = -f;
-    a = (use && ha) ? a * f : (T)0;
-    b = (use && hb) ? b * f : (T)0;
-    v[i] = mk<T>(a, b);
+    const bool zero = kind == hosnwt_kind_sin && edge;  // sine ends are zero whatever they hold
+    v[i] = mk<T>(zero ? (T)0 : a * f, zero ? (T)0 : b * f);
   }
 }
 
-template <typename T, int V, int R, bool YP, class D>
+template <typename T, int V, int R, bool YP, bool P2, class D>
 __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl, int ntl, int L, int N, int kind,
                                            int dir, int n_out, int la, int lb, bool ha, bool hb) {
+  if constexpr (YP && P2) {
+    // both columns of every line pair: one access per row
+#pragma unroll
+    for (int i0 = 0; i0 < V; i0 += 8) {
+      typename D::Pre2 q[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int i = i0 + c;
+        if (i >= V) break;
+        const int k = layout_pos<R>(i, tl, ntl, L);
+        q[c] = dst.pre2(la, k, k < n_out);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int i = i0 + c;
+        if (i >= V) break;
+        const int k = layout_pos<R>(i, tl, ntl, L);
+        const C2<T> Z = v[i];
+        T A, B;
+        if (kind == hosnwt_kind_cos) {
+          A = Z.x;
+          B = Z.y;
+        } else {
+          A = -Z.y;
+          B = Z.x;
+          if (k == 0 || k == N - 1) A = B = (T)0;
+        }
+        if (dir > 0) {
+          const T f = (k == 0 || k == N - 1) ? (T)(1.0 / (double)L) : (T)(2.0 / (double)L);
+          A *= f;
+          B *= f;
+        }
+        dst.put2_if(la, k, A, B, q[c], k < n_out);
+      }
+    }
+    return;
+  }
   const T inv = (T)(1.0 / (double)L);
   // outputs whose loads are issued together: 8, or 4 for FP64 destinations
   // that read back several values per output (fewer spills at two waves per SIMD)
@@ -289,7 +334,7 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
       const int i = i0 + c;
       if (i >= V) break;
       const int k = layout_pos<R>(i, tl, ntl, L);
-      if (k >= n_out) continue;
+      const bool ok = k < n_out;
       const C2<T> Z = v[i];
       T A, B;
       if (kind == hosnwt_kind_cos) {
@@ -305,12 +350,13 @@ __device__ __forceinline__ void store_line(const C2<T>* v, const D& dst, int tl,
         A *= f;
         B *= f;
       }
+      // (outputs not stored take the offset beyond the destination's resource)
       if (YP) {
-        if (ha) dst.put(la, k, A, qa[c]);
-        if (hb) dst.put(lb, k, B, qb[c]);
+        dst.put_if(la, k, A, qa[c], ok && ha);
+        dst.put_if(lb, k, B, qb[c], ok && hb);
       } else {
-        if (ha) dst.put(k, la, A, qa[c]);
-        if (hb) dst.put(k, lb, B, qb[c]);
+        dst.put_if(k, la, A, qa[c], ok && ha);
+        dst.put_if(k, lb, B, qb[c], ok && hb);
       }
     }
   }
@@ -354,11 +400,13 @@ template <typename T, int V, bool YP, class S, class D, int NC, int... Rs>
 __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
                                       ? 1024
                                       : ((YP || hosfft2_prod<Rs...>::value / V > 256) ? 512 : 256),
-                                  sizeof(T) == 8 ? 2 : (!YP ? 1 : (sizeof(typename D::Pre) > 2 * sizeof(T) ? 3 : 4)))
+                                  sizeof(T) == 8 || (YP && V >= 24) ? 2 : (!YP ? 1 : (sizeof(typename D::Pre) > 2 * sizeof(T) ? 3 : 4)))
     r2r_seq_kernel(int N, int dir, int ncl_arg, LineBatch<S, D> bt,
                                                                  Plan2<T> pl) {
   const int ncl = NC > 0 ? NC : ncl_arg;
   constexpr int L = hosfft2_prod<Rs...>::value;
+  constexpr int NL = L / 2 + 1;  // points per line (the launch checks N == NL)
+  (void)N;
   const int item = blockIdx.y;
   const int kind = bt.kind[item], n_in = bt.n_in[item], n_out = bt.n_out[item], nlines = bt.nlines[item];
   const S& src = bt.src[item];
@@ -405,13 +453,19 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
       const int c2 = idx / (2 * n_in), rem = idx - c2 * 2 * n_in;
       const int h = rem / n_in, e = rem - h * n_in;
       const int ln = 2 * (grp * ncl + c2) + h;
-      lds[(size_t)c2 * pl.lp * XW + h * N + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
+      lds[(size_t)c2 * pl.lp * XW + h * NL + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
     }
     __syncthreads();
     mult = lds + (size_t)cl * pl.lp * XW;
   }
   C2<T> v[V];
-  load_line<T, V, R0, YP>(v, src, tl, ntl, L, N, kind, dir, n_in, la, lb, ha, hb, mult);
+  // y tiles whose line pairs all hold data (all but an item's last tile) take
+  // one access per row and line pair (uniform over the workgroup)
+  const bool pairs = YP && 2 * (grp + 1) * ncl <= nlines;
+  if (pairs && 2 * (grp + 1) * ncl <= src.nx)
+    load_line<T, V, R0, YP, true>(v, src, tl, ntl, L, NL, kind, dir, n_in, la, lb, ha, hb, mult);
+  else
+    load_line<T, V, R0, YP, false>(v, src, tl, ntl, L, NL, kind, dir, n_in, la, lb, ha, hb, mult);
   for (int m = tid; m < 64; m += blockDim.x) t0[m] = pl.tw0[m];
   for (int m = tid; m < (L + 63) / 64; m += blockDim.x) t1[m] = pl.tw1[m];
   __syncthreads();
@@ -422,7 +476,10 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
     (void)line2;
     run_passes<T, V, YP, L, 1, Rs...>(v, line, ncl, tl, t0, t1);
   }
-  store_line<T, V, Rl, YP>(v, dst, tl, ntl, L, N, kind, dir, n_out, la, lb, ha, hb);
+  if (pairs)
+    store_line<T, V, Rl, YP, true>(v, dst, tl, ntl, L, NL, kind, dir, n_out, la, lb, ha, hb);
+  else
+    store_line<T, V, Rl, YP, false>(v, dst, tl, ntl, L, NL, kind, dir, n_out, la, lb, ha, hb);
 }
 
 }  // namespace hosfft2

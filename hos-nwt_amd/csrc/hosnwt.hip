@@ -2244,6 +2244,7 @@ U* dense_matrix(Arena& ar, int N, int kind, int dir, bool transposed) {
 
 template <typename U>
 void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
+  check_buffer_grid(n1, n2);
   g.n1 = n1;
   g.n2 = n2;
   // stored transposed (k-major): row r of the matrix is contiguous, the layout

@@ -51,6 +51,66 @@ enum Mul { MUL_NONE = 0, MUL_NEG_KX = 1, MUL_NEG_KY = 2, MUL_KTH = 3, MUL_KTH2 =
 // ---------------------------------------------------------------------------
 // source / destination functors
 // ---------------------------------------------------------------------------
+// Fields are read and written through buffer resources: the address is the
+// field's base (scalar registers) plus a 32-bit byte offset, and the hardware
+// returns zero for a load and drops a store whose offset lies at or beyond the
+// resource's extent.  An element outside the valid extents takes the offset
+// kBufOff (2^31), beyond every extent, so that masked loads and stores need
+// neither a branch nor a select on the value.  Every field a transform touches
+// holds fewer than 2^31 bytes (checked where a context is created).
+constexpr unsigned kBufOff = 0x80000000u;
+constexpr int kBufFlags = 0x00020000;  // descriptor word 3 of a raw buffer on gfx9-family chips
+
+template <typename U>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const U* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<U*>(p), (short)0, (int)bytes, kBufFlags);
+}
+template <typename U>
+__device__ __forceinline__ U buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  static_assert(sizeof(U) == 4 || sizeof(U) == 8, "4- or 8-byte elements");
+  if constexpr (sizeof(U) == 4)
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(U, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+// two adjacent elements in one access (4-byte aligned for FP32 pairs: the
+// chip's buffer accesses need dword alignment only)
+template <typename U>
+struct BufPair {
+  U x, y;
+};
+template <typename U>
+__device__ __forceinline__ BufPair<U> buf_load2(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  if constexpr (sizeof(U) == 4)
+    return __builtin_bit_cast(BufPair<U>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(BufPair<U>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+template <typename U>
+__device__ __forceinline__ void buf_store2(__amdgpu_buffer_rsrc_t r, unsigned off, BufPair<U> v) {
+  typedef unsigned u2 __attribute__((ext_vector_type(2)));
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  if constexpr (sizeof(U) == 4)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, off, 0, 0);
+}
+template <typename U>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, U v) {
+  static_assert(sizeof(U) == 4 || sizeof(U) == 8, "4- or 8-byte elements");
+  if constexpr (sizeof(U) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+  } else {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, off, 0, 0);
+  }
+}
+// byte offset of element (ix, iy) at leading dimension ld (24-bit factors)
+template <typename U>
+__device__ __forceinline__ unsigned elem_off(int ix, int iy, int ld) {
+  return ((unsigned)ix + __umul24((unsigned)ld, (unsigned)iy)) * (unsigned)sizeof(U);
+}
+
 // Real field on a column-major grid, zero outside [0,nx) x [0,ny); optional
 // spectral multiplier evaluated from the mode indices (kx = ix*pix, ky = iy*piy).
 // U is the stored type, T the transform type.
@@ -60,12 +120,23 @@ struct Src {
   int ld, nx, ny;
   int mul, jpow;
   double pix, piy;
-  // Branch-free load: the index is clamped to a valid element and the value
-  // masked, so that a thread can keep many loads in flight.
-  __device__ __forceinline__ U raw(int ix, int iy) const {
-    const bool ok = (ix < nx) & (iy < ny);
-    const U v = p[ok ? ix + (size_t)ld * iy : 0];
-    return ok ? v : (U)0;
+  // the valid extent in bytes: elements ix < nx of rows iy < ny
+  __device__ __forceinline__ unsigned extent() const {
+    return (nx > 0 && ny > 0) ? ((unsigned)ld * (unsigned)(ny - 1) + (unsigned)nx) * (unsigned)sizeof(U) : 0u;
+  }
+  // Branch-free load: an element outside the extents reads as zero (its offset
+  // is beyond the resource), so that a thread can keep many loads in flight.
+  __device__ __forceinline__ U raw(int ix, int iy) const { return raw_if(ix, iy, true); }
+  // the same, zero also where ok is false
+  __device__ __forceinline__ U raw_if(int ix, int iy, bool ok) const {
+    ok = ok & (ix < nx) & (iy < ny);
+    return buf_load<U>(buf_rsrc(p, extent()), ok ? elem_off<U>(ix, iy, ld) : kBufOff);
+  }
+  // elements (ix, iy) and (ix + 1, iy) in one access; both zero where ok is false.
+  // The caller guarantees ix + 1 < nx.
+  __device__ __forceinline__ BufPair<U> raw2_if(int ix, int iy, bool ok) const {
+    ok = ok & (iy < ny);
+    return buf_load2<U>(buf_rsrc(p, extent()), ok ? elem_off<U>(ix, iy, ld) : kBufOff);
   }
   // Spectral multiplier of mode (ix, iy).
   __device__ __forceinline__ double mulval(int ix, int iy) const {
@@ -103,7 +174,10 @@ inline Src<T, U> src_mul(const U* p, int ld, int nx, int ny, int mul, int jpow, 
 
 // Plain store: p[ix + ld*iy] = v (mode 0) or += v (mode 1).
 // Stores go through a two-phase interface: pre() issues the loads an output
-// needs (branch-free, clamped), put() combines and stores.
+// needs (branch-free), put() combines and stores; put_if() stores only when ok
+// (an output that is not stored takes the offset beyond the resource).  The
+// destination's resource spans 2^31 - 1 bytes from its base: the offsets of
+// valid outputs lie in the field, and kBufOff lies beyond.
 template <typename T>
 struct Dst {
   T* __restrict__ p;
@@ -111,15 +185,30 @@ struct Dst {
   struct Pre {
     T old;
   };
+  __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc() const { return buf_rsrc(p, 0x7fffffffu); }
   __device__ __forceinline__ Pre pre(int ix, int iy, bool ok) const {
     Pre q;
-    q.old = (mode == 0) ? (T)0 : p[ok ? ix + (size_t)ld * iy : 0];
+    q.old = (mode == 0) ? (T)0 : buf_load<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff);
     return q;
   }
-  __device__ __forceinline__ void put(int ix, int iy, T v, const Pre& q) const {
-    p[ix + (size_t)ld * iy] = (mode == 0) ? v : q.old + v;
+  __device__ __forceinline__ void put_if(int ix, int iy, T v, const Pre& q, bool ok) const {
+    buf_store<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff, (mode == 0) ? v : q.old + v);
   }
+  __device__ __forceinline__ void put(int ix, int iy, T v, const Pre& q) const { put_if(ix, iy, v, q, true); }
   __device__ __forceinline__ void put(int ix, int iy, T v) const { put(ix, iy, v, pre(ix, iy, true)); }
+  // outputs (ix, iy) and (ix + 1, iy) in one access
+  struct Pre2 {
+    BufPair<T> old;
+  };
+  __device__ __forceinline__ Pre2 pre2(int ix, int iy, bool ok) const {
+    Pre2 q;
+    q.old = (mode == 0) ? BufPair<T>{(T)0, (T)0} : buf_load2<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff);
+    return q;
+  }
+  __device__ __forceinline__ void put2_if(int ix, int iy, T a, T b, const Pre2& q, bool ok) const {
+    const BufPair<T> v = (mode == 0) ? BufPair<T>{a, b} : BufPair<T>{q.old.x + a, q.old.y + b};
+    buf_store2<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff, v);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -544,6 +633,14 @@ struct GridSched {
   bool operator==(const GridSched& o) const { return xs == o.xs && ys == o.ys && tx == o.tx; }
 };
 
+// Fields on an n1 x n2 grid are addressed through buffer resources with 32-bit
+// byte offsets (Src, Dst): such a grid holds fewer than 2^31 bytes per FP64 field.
+inline void check_buffer_grid(long n1, long n2) {
+  if (n1 * n2 >= (1L << 31) / 8)
+    throw std::runtime_error("grid " + std::to_string(n1) + " x " + std::to_string(n2) +
+                             ": a field of 2^31 bytes or more is beyond the transforms' 32-bit offsets");
+}
+
 // Transforms on one N1 x N2 grid (N2 == 1: 2-D tank, x lines only).
 template <typename T>
 struct Grid2 {
@@ -552,6 +649,7 @@ struct Grid2 {
   GridSched sched{};
   LinePlan<T> px, py;
   void build(int n1, int n2, GridSched sc = {}) {
+    check_buffer_grid(n1, n2);
     N1 = n1;
     N2 = n2;
     px.sched = sc.xs;
@@ -645,6 +743,7 @@ void launch_seq_nc(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBat
   }
   // exchange elements of 8 bytes: a component in FP64, a complex value in FP32
   const size_t lds = (size_t)ncl * pl.dev2.lp * 8 + (size_t)(64 + (pl.dev2.L + 63) / 64) * sizeof(C2<T>);
+  if (2 * (N - 1) != pl.dev2.L) throw std::runtime_error("line plan does not match N=" + std::to_string(N));
   const int nt = ncl * pl.dev2.ntl;
   const int nl = batch_max_lines(b);
   if (nl <= 0) return;
