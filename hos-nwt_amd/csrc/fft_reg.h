@@ -449,11 +449,12 @@ __global__ void __launch_bounds__(hosfft2_prod<Rs...>::value / V > 512
   const T* mult = nullptr;
   if (src.mul != 0) {
     // multipliers of the two real lines of every complex line: line[q*N + e]
-    for (int idx = tid; idx < ncl * 2 * n_in; idx += blockDim.x) {
-      const int c2 = idx / (2 * n_in), rem = idx - c2 * 2 * n_in;
-      const int h = rem / n_in, e = rem - h * n_in;
-      const int ln = 2 * (grp * ncl + c2) + h;
-      lds[(size_t)c2 * pl.lp * XW + h * NL + e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
+    // (real line q of the workgroup: complex line q / 2, half q % 2; the
+    // threads stride over the elements, so no index is divided)
+    for (int q = 0; q < 2 * ncl; ++q) {
+      const int ln = 2 * grp * ncl + q;
+      T* m = lds + (size_t)(q >> 1) * pl.lp * XW + (q & 1) * NL;
+      for (int e = tid; e < n_in; e += blockDim.x) m[e] = (T)(YP ? src.mulval(ln, e) : src.mulval(e, ln));
     }
     __syncthreads();
     mult = lds + (size_t)cl * pl.lp * XW;
