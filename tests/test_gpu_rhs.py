@@ -95,3 +95,55 @@ def test_check_slope_filters_committed_state(n2):
         assert rel(st["eta"], e_ref) < TOL and rel(st["phis"], p_ref) < TOL
     finally:
         tank.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n2", [1, 10])
+def test_slope_max_against_the_reference_grid(n2):
+    """The reference takes MAXVAL(ABS(etax)) over its extended grid Nd = (p1+1) n1 / 2
+    (initial_condition.f90:694-696); the GPU samples the same trigonometric polynomial on
+    its padded grid N1b x N2b (a smooth length).  Both maxima lie within
+    (hx^2 max|f_xx| + 2 hx hy max|f_xy| + hy^2 max|f_yy|) / 8 of the true maximum (at an
+    interior maximum the gradient vanishes and a node lies within half a cell per axis),
+    with the second derivatives bounded by the sums of |modes| times kx^2, kx ky, ky^2.
+    The test checks the two maxima against the sum of both grids' bounds."""
+    kw = dict(n1=32, n2=n2, M=3, n3=9, xlen=6.0, ylen=3.0)
+    s = make_setup(**kw)
+    orc = ho.Oracle(oracle_params(s))
+    tank = Tank(s, precision=0)
+    try:
+        G1, G2, a1, e1, aad = random_state(s, 11, amp=0.05)
+        orc.time = 0.4
+        orc.solve_hos2(a1, e1, G2, G1, aad, 0.02)
+        tank.solve_hos2(0.4, 0.02, G1, G2, a1, e1, aad)
+        sx, sy = tank.slope_max()
+        st = tank.stats()
+        Nb1, Nb2 = st["big_n1"], st["big_n2"]
+        p = orc.p
+        assert (Nb1, Nb2) != (p.Nd1, p.Nd2)  # the grids differ
+        kx = np.arange(p.Nd1) * np.pi / s.xlen
+        ky = np.arange(p.Nd2) * np.pi / s.ylen if n2 > 1 else np.zeros(1)
+
+        def bound(hx, hy, b):
+            ab = np.abs(b)
+            return (hx * hx * (ab * kx[:, None] ** 2).sum() + 2 * hx * hy * (ab * np.outer(kx, ky)).sum()
+                    + hy * hy * (ab * ky[None, :] ** 2).sum()) / 8.0
+
+        hx_r, hx_g = s.xlen / (p.Nd1 - 1), s.xlen / (Nb1 - 1)
+        hy_r = s.ylen / (p.Nd2 - 1) if n2 > 1 else 0.0
+        hy_g = s.ylen / (Nb2 - 1) if n2 > 1 else 0.0
+        bx = orc.s2f_big(orc.etax, "sin", "cos")
+        sx_ref = float(np.abs(orc.etax).max())
+        lim_x = bound(hx_r, hy_r, bx) + bound(hx_g, hy_g, bx)
+        print(f"n2={n2} grids {p.Nd1}x{p.Nd2} / {Nb1}x{Nb2}: max|eta_x| {sx:.6e} vs {sx_ref:.6e} "
+              f"(diff {abs(sx - sx_ref):.2e}, bound {lim_x:.2e})", flush=True)
+        assert abs(sx - sx_ref) <= lim_x + 1e-12 * sx_ref
+        if n2 > 1:
+            by = orc.s2f_big(orc.etay, "cos", "sin")
+            sy_ref = float(np.abs(orc.etay).max())
+            lim_y = bound(hx_r, hy_r, by) + bound(hx_g, hy_g, by)
+            print(f"   max|eta_y| {sy:.6e} vs {sy_ref:.6e} (diff {abs(sy - sy_ref):.2e}, bound {lim_y:.2e})",
+                  flush=True)
+            assert abs(sy - sy_ref) <= lim_y + 1e-12 * sy_ref
+    finally:
+        tank.close()
