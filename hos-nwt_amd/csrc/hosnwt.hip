@@ -2209,6 +2209,12 @@ std::vector<double> transposed(const std::vector<double>& a, int r, int c) {
   return t;
 }
 
+// W[j + n m] = s nu[m] M[j + n m]: the rows of a stored n x n matrix scaled
+__global__ void k_scale_rows(int n, double s, const double* __restrict__ nu, const double* __restrict__ M, double* W) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < (size_t)n * n) W[i] = s * nu[i / n] * M[i];
+}
+
 template <typename U>
 U* upload_matrix(Arena& ar, const std::vector<double>& m) {
   std::vector<U> h(m.begin(), m.end());
@@ -2453,6 +2459,8 @@ struct hos_ctx {
   // host copies of the wavemaker inputs (pos_ini, dposdy_ini: n2 x n_freq; d nu/dx,
   // d nu/dy: n1 x n2; empty when absent), from which every engine is built
   std::vector<double> h_pos_re, h_pos_im, h_dy_re, h_dy_im, h_nux, h_nuy;
+  // the beach nu(x) when it does not depend on y (empty otherwise)
+  std::vector<double> h_nu_x;
   double gz_top = 1.0;
   // wavemaker globals of the last evaluation (alpha, pos1st(n3,1))
   double last_alpha = 0.0, last_pos_top = 0.0, last_alphat = 0.0;
@@ -2617,6 +2625,8 @@ struct Engine : EngineBase {
   double *part2 = nullptr, *zf[11] = {};
   double *modes8 = nullptr, *T8 = nullptr, *a1f = nullptr;
   double *phit2 = nullptr, *phiz2 = nullptr, *w2 = nullptr, *tmp2 = nullptr, *atot = nullptr, *dtot = nullptr;
+  // mixed path: the beach operator of the orders >= 2 (n1 x n1, column-major), or null
+  T* Bnu = nullptr;
   // base grid as dense products (lengths without a Bluestein-free FFT plan)
   bool dense = false;
   DenseGrid<T> dT;
@@ -2800,6 +2810,30 @@ struct Engine : EngineBase {
       }
     } else {
       PA = RA = T1 = nullptr;
+    }
+    // Mixed path, 3-D dense base grid, beach of x alone: the beach term of the
+    // orders >= 2, S2F(-nu F2S(Q)), is one product Bnu Q with the n1 x n1 matrix
+    // Bnu = X_fwd diag(-nu) X_inv (the y transforms of F2S and S2F are each other's
+    // inverse and nu does not depend on y), formed once in FP64 on the matrix cores.
+    if constexpr (kSplit) {
+      if (dense && n2 > 1 && !c->h_nu_x.empty()) {
+        const size_t nn = (size_t)n1 * n1;
+        double *nu = nullptr, *W = nullptr, *B64 = nullptr;
+        HIP_OK(hipMalloc(&nu, n1 * sizeof(double)));
+        HIP_OK(hipMalloc(&W, nn * sizeof(double)));
+        HIP_OK(hipMalloc(&B64, nn * sizeof(double)));
+        HIP_OK(hipMemcpy(nu, c->h_nu_x.data(), n1 * sizeof(double), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_scale_rows, dim3(nblk(nn)), dim3(256), 0, c->st, n1, -1.0, (const double*)nu,
+                           (const double*)d64.mx[KCOS][1], W);
+        gemm(c->st, true, false, n1, n1, n1, (const double*)d64.mx[KCOS][0], n1, (const double*)W, n1, B64, n1);
+        Bnu = ar.get<T>(nn);
+        hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(nn)), dim3(256), 0, c->st, nn, (const double*)B64, Bnu);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(c->st));
+        (void)hipFree(nu);
+        (void)hipFree(W);
+        (void)hipFree(B64);
+      }
     }
     tune();
   }
@@ -3104,12 +3138,17 @@ struct Engine : EngineBase {
       HIP_OK(hipMemcpyAsync(Q, P, ns * sizeof(T), hipMemcpyDeviceToDevice, st));
     }
     // dphin and the final base-grid RHS
-    base(KCOS, KCOS, INV, src_plain<T>(Q, n1, n1, n2), n1, n2, Dst<T>{tmpS, n1, 0});
+    if (!Bnu) base(KCOS, KCOS, INV, src_plain<T>(Q, n1, n1, n2), n1, n2, Dst<T>{tmpS, n1, 0});
     if constexpr (kSplit) {
       // beach term of the orders >= 2 in T; the first-order modes Zh64 come from the
       // wavemaker stream
-      hipLaunchKernelGGL(k_beach<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)C.nu_d, Zp);
-      base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
+      if (Bnu) {
+        gemm(cs, false, true, n1, n2, n1, (const T*)Bnu, n1, (const T*)Q, n1, Zh, n1);
+        C.stats.n_base += 2;  // the two base transforms it stands for
+      } else {
+        hipLaunchKernelGGL(k_beach<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)C.nu_d, Zp);
+        base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
+      }
       hipLaunchKernelGGL((k_expon_out<T, double>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)nullptr,
                          (const double*)phizh64, (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh,
@@ -4820,6 +4859,11 @@ int hos_create(const hos_params* pin, hos_ctx** out) {
     if (p.nu) {
       c->nu_d = ar.get<double>(ns);
       HIP_OK(hipMemcpy(c->nu_d, p.nu, ns * sizeof(double), hipMemcpyHostToDevice));
+      // a beach of x alone (the reference's front beach, initial_condition.f90:724-794)
+      bool sep = true;
+      for (int j = 1; j < p.n2 && sep; ++j)
+        for (int i = 0; i < p.n1 && sep; ++i) sep = p.nu[i + (size_t)p.n1 * j] == p.nu[i];
+      if (sep) c->h_nu_x.assign(p.nu, p.nu + p.n1);
     }
     c->p.nu = nullptr;
     if (filter_active(p.coeffilt[0]) || (p.n2 > 1 && filter_active(p.coeffilt[1]))) {
