@@ -247,14 +247,9 @@ struct PtrSet {
 // over a fixed bound so that every index is known at compile time.
 constexpr int kMaxOrder = 10;
 template <typename T>
-__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm,
-                            const T* __restrict__ ex, const T* __restrict__ ey, T* g2) {
+__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (g2) {  // level 1 also forms |grad eta|^2 from the slopes
-    const T x = ex[i], y = ey[i];
-    g2[i] = x * x + y * y;
-  }
   const bool do_phim = iH < M;
   const int top = min(iH, M - 1);  // highest power read at this level
   T pw[kMaxOrder];
@@ -310,7 +305,7 @@ __global__ void k_narrow(size_t n, const double* __restrict__ in, T* __restrict_
 // sine in y) and converted exactly to the reference grid afterwards.
 template <typename T>
 __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, const T* __restrict__ py,
-                       const T* __restrict__ ex, const T* __restrict__ ey, const T* __restrict__ g2,
+                       const T* __restrict__ ex, const T* __restrict__ ey,
                        const T* __restrict__ ax, const T* __restrict__ ay, const T* __restrict__ az, T* X, T* Y,
                        T* XA, T* YA, T* XB, T* YB, int j0, T* zsum) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -321,7 +316,8 @@ __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, 
   T zs = (j0 <= M) ? z[j0] : (T)0;
   for (int j = j0 + 1; j <= M; ++j) zs += z[j];
   zsum[i] = zs;
-  const T g = g2[i];
+  const T vpx = px[i], vpy = py[i], vex = ex[i], vey = ey[i];
+  const T g = vex * vex + vey * vey;  // |grad eta|^2
   T sgz = (T)0;
   for (int j = 1; j <= M - 3; ++j) sgz += z[j];
   if (M >= 3) sgz += z[M - 2 >= 1 ? M - 2 : 1];
@@ -344,7 +340,6 @@ __global__ void k_fsbc(size_t n, int M, PtrSet<T> zz, const T* __restrict__ px, 
     const int q0 = max(0, min(M - 2 - j, j - 1)) + 1, q1 = min(M - j, j - 1);
     for (int q = q0; q <= q1; ++q) t1 += (T)2 * z[j] * z[q];
   }
-  const T vpx = px[i], vpy = py[i], vex = ex[i], vey = ey[i];
   const T vax = ax[i], vay = ay[i], vaz = az[i];
   const T deta = -vpx * vex - vpy * vey + g * sgz;
   const T dphis = -vpx * vpx - vpy * vpy + gg2 + t1;
@@ -2569,7 +2564,7 @@ struct Engine : EngineBase {
   std::vector<T*> lvl;  // outputs of one HOS level's transforms
   std::vector<T*> tmpX;  // x-pass intermediates of batched transforms
   double* slope_part = nullptr;  // block maxima of |eta_x|, |eta_y| (slope_max)
-  T *etax, *etay, *phisx, *phisy, *g2, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
+  T *etax, *etay, *phisx, *phisy, *phim, *phixa, *phiya, *phiza, *X, *Y, *tmpB;
   // wavemaker / first order (double)
   double *Xp, *Xpt, *Xptt, *f1, *f2, *fz1, *fz2, *gz, *Wz, *Wy, *freq, *pre, *pim, *part;
   int nch = 1;
@@ -2672,7 +2667,7 @@ struct Engine : EngineBase {
     for (int i = 0; i < M; ++i) lvl.push_back(ar.get<T>(nb));
     etax = ar.get<T>(nb); etay = ar.get<T>(nb); phisx = ar.get<T>(nb); phisy = ar.get<T>(nb);
     slope_part = ar.get<double>(2 * (size_t)nblk(nb));
-    g2 = ar.get<T>(nb); phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
+    phim = ar.get<T>(nb); phixa = ar.get<T>(nb); phiya = ar.get<T>(nb);
     phiza = ar.get<T>(nb); X = ar.get<T>(nb); Y = ar.get<T>(nb); tmpB = ar.get<T>(nb);
     // wavemaker
     const int n2 = c->n2, n3 = c->n3, n3a = c->n3a;
@@ -2705,7 +2700,7 @@ struct Engine : EngineBase {
     HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     tmpBw = ar.get<T>(nb);
-    for (int i = 0; i < 3; ++i) tmpXw.push_back(ar.get<T>(nb));
+    for (int i = 0; i < 4; ++i) tmpXw.push_back(ar.get<T>(nb));
     if (const size_t gw = gb.gline_ws_elems()) {
       gwsM = ar.get<T>(gw);
       gwsS = ar.get<T>(gw);
@@ -3038,24 +3033,15 @@ struct Engine : EngineBase {
     // work up to the join, so that the main stream does not wait for the host
     // while the host enqueues the chain ---
     HIP_OK(hipEventRecord(ev_fork, st));
-    // --- phisxy_etaxy: extended-grid eta, slopes, phis gradients ---
+    // --- phisxy_etaxy (resol_HOS.f90:924-1078): the extended-grid eta here, with the
+    // first HOS level's one transform (kth1 phis); the slopes and phis gradients,
+    // read only after the join (the free-surface products), on the side stream ---
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{eta_ext ? eta_ext : tmpB2(), ldb, 0},
             tmpX[0]);
-      b.add(KSIN, KCOS, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0},
-            tmpX[1]);
-      b.add(KSIN, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b,
-            Dst<T>{phisx, ldb, 0}, tmpX[2]);
-      if (n2 > 1) {
-        b.add(KCOS, KSIN, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
-              Dst<T>{etay, ldb, 0}, tmpX[3]);
-        b.add(KCOS, KSIN, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
-              Dst<T>{phisy, ldb, 0}, tmpX[4]);
-      }
-      // the first HOS level's one transform (kth1 phis) rides along
       b.add(KCOS, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, 1, pix, piy), N1b, N2b, Dst<T>{lvl[0], ldb, 0},
-            tmpX[b.n]);
+            tmpX[1]);
       bigB(INV, b);
     }
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
@@ -3083,7 +3069,7 @@ struct Engine : EngineBase {
       else
         bigB(INV, b);
       hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, (const T*)eta_ext, phim,
-                         phizm[iH], (const T*)etax, (const T*)etay, iH == 1 ? g2 : (T*)nullptr);
+                         phizm[iH]);
       if (iH < M) {
         if (iH + 1 == M)
           big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
@@ -3109,6 +3095,20 @@ struct Engine : EngineBase {
         ~Side() { e->use_main(); }
       } side{this};
       use_side();
+      {
+        Batch2<T, Src<T, T>, Dst<T>> b;
+        b.add(KSIN, KCOS, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0},
+              (*ctmpX)[0]);
+        b.add(KSIN, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b,
+              Dst<T>{phisx, ldb, 0}, (*ctmpX)[1]);
+        if (n2 > 1) {
+          b.add(KCOS, KSIN, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+                Dst<T>{etay, ldb, 0}, (*ctmpX)[2]);
+          b.add(KCOS, KSIN, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+                Dst<T>{phisy, ldb, 0}, (*ctmpX)[3]);
+        }
+        bigB(INV, b);
+      }
       wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, s2);
     }
     HIP_OK(hipEventRecord(ev_join, sw));
@@ -3121,7 +3121,7 @@ struct Engine : EngineBase {
       if (M >= 2) {
         // (in a 2-D tank etay, phisy and phiya hold zeros from allocation on: the
         // arena clears every buffer and only the n2 > 1 transforms write them)
-        hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay, g2,
+        hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay,
                            phixa, phiya, phiza, X, Y, XA, YA, XB, YB, j0, zsum);
         b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[1]);
         b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[2]);
