@@ -537,29 +537,32 @@ __global__ void k_paddle_rhs(int n2, int n3, int n3a, double delz, const double*
     c2t = alt;
     c2tt = 0.0;
   }
-  const int i2 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i2 >= n2) return;
+  // one thread per (i3, i2) of the paddle grid (i3 fastest: coalesced); the
+  // continuation nodes recompute the column's last four values
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= (size_t)n3a * n2) return;
+  const int i3 = (int)(idx % n3a), i2 = (int)(idx / n3a);
   for (int which = 0; which < 2; ++which) {
     double* f = which ? f2 : f1;
     if (!f) continue;
     const double cx = which ? c2x : c1x, ct = which ? c2t : c1t, ctt = which ? c2tt : c1tt;
     // evaluation order of the reference: - a Xtt - 2 a' Xt - a'' X
-    for (int i3 = 0; i3 < n3; ++i3) {
-      const size_t o = i3 + (size_t)n3 * i2;
-      f[i3 + (size_t)n3a * i2] = -ctt * Xtt[o] - ct * Xt[o] - cx * X[o];
+    auto val = [&](int q) {
+      const size_t o = q + (size_t)n3 * i2;
+      return -ctt * Xtt[o] - ct * Xt[o] - cx * X[o];
+    };
+    if (i3 < n3) {
+      f[idx] = val(i3);
+      continue;
     }
-    const double* col = f + (size_t)n3a * i2;
-    const double fm3 = col[n3 - 4], fm2 = col[n3 - 3], fm1 = col[n3 - 2], f0 = col[n3 - 1];
+    const double fm3 = val(n3 - 4), fm2 = val(n3 - 3), fm1 = val(n3 - 2), f0 = val(n3 - 1);
     const double df = (fm2 - 4.0 * fm1 + 3.0 * f0) / (2.0 * delz);
     const double d2f = (-fm3 + 4.0 * fm2 - 5.0 * fm1 + 2.0 * f0) / (delz * delz);
     double c[4];
     for (int r = 0; r < 4; ++r) c[r] = cm.m[0][r] * f0 + cm.m[1][r] * df + cm.m[2][r] * d2f;
-    for (int i3 = n3; i3 < n3a; ++i3) {
-      const double xx = (double)i3 / (double)(n3 - 1) - (double)L_ADD;
-      const double xx2 = xx * xx;
-      const double pxx = (((c[0] * xx2 + c[1]) * xx2 + c[2]) * xx2 + c[3]) * xx * exp(-6.0 * (xx + 1.0));
-      f[i3 + (size_t)n3a * i2] = pxx;
-    }
+    const double xx = (double)i3 / (double)(n3 - 1) - (double)L_ADD;
+    const double xx2 = xx * xx;
+    f[idx] = (((c[0] * xx2 + c[1]) * xx2 + c[2]) * xx2 + c[3]) * xx * exp(-6.0 * (xx + 1.0));
   }
 }
 
@@ -3574,7 +3577,7 @@ struct Engine : EngineBase {
     hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
                        (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt, dsc, C.cur_stage);
     // f1 = -alpha X_tt - 2 alpha' X_t - alpha'' X ; f2 = -alpha' X_t - alpha'' X
-    hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
+    hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk((size_t)n3a * n2)), dim3(256), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
                        (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2,
                        dsc, C.cur_stage);
     paddle_modes(2, kad1, kadr);
@@ -3850,7 +3853,7 @@ struct Engine : EngineBase {
     // --- SWEET 1: paddle condition and modes
     hipLaunchKernelGGL(k_pos, dim3(n2), dim3(256), 0, st, n2, n3, C.p.n_freq, t, (const double*)freq,
                        (const double*)pre, (const double*)pim, (const double*)gz, Xp, Xpt, Xptt);
-    hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(n2, 64)), dim3(64), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
+    hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk((size_t)n3a * n2)), dim3(256), 0, st, n2, n3, n3a, C.delz, (const double*)Xp,
                        (const double*)Xpt, (const double*)Xptt, altt, 2.0 * alt, al, altt, alt, 0.0, cont, f1, f2);
     paddle_modes(2, kad1, kadr);
     // phiadd_1st_SWEET at z = 0: the 11 fields (phit, phiz first)
@@ -4964,7 +4967,7 @@ int hos_initialize(hos_ctx* c) {
     auto run = [&](auto* e) {
       hipLaunchKernelGGL(k_pos, dim3(c->n2), dim3(256), 0, st, c->n2, c->n3, c->p.n_freq, 0.0, (const double*)e->freq,
                          (const double*)e->pre, (const double*)e->pim, (const double*)e->gz, e->Xp, e->Xpt, e->Xptt);
-      hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk(c->n2, 64)), dim3(64), 0, st, c->n2, c->n3, c->n3a, c->delz,
+      hipLaunchKernelGGL(k_paddle_rhs, dim3(nblk((size_t)c->n3a * c->n2)), dim3(256), 0, st, c->n2, c->n3, c->n3a, c->delz,
                          (const double*)e->Xp, (const double*)e->Xpt, (const double*)e->Xptt, alt, al, 0.0, 0.0, 0.0,
                          0.0, e->cont, e->f1, (double*)nullptr);
       e->paddle_modes(1, c->a1st_add, (double*)nullptr);
