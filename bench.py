@@ -113,7 +113,8 @@ def time_path(precision, steps, warmup, spin_up, follow=None):
 
 def roofline(tank, precision):
     """Live HIP-event timing of the dominant kernel family (extended-grid r2r passes) in the
-    precision of the headline path, on the context's stream."""
+    precision of the headline path, on the context's stream: the slower pass of a lone full
+    transform, and (``batched4``) the same pass launched with four transforms at once."""
     r = tank.time_big_transform(reps=20)
     achieved = r["bytes"] / (r["ms"] * 1e-3) / 1e9
     traffic = None
@@ -121,9 +122,16 @@ def roofline(tank, precision):
     st = tank.stats()
     if t is not None and list(t["grid"]) == [st["big_n1"], st["big_n2"]]:
         traffic = t["traffic_bytes"][r["pass"]]
+    # the same passes as the right-hand side launches them: four independent full
+    # transforms per launch (each level's and each product batch's shape)
+    b = tank.time_big_batch(reps=20, items=4)
+    bp = b[r["pass"]]
+    ach4 = bp["bytes"] / (bp["ms"] * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": r["kernel"],
-            "bytes_per_launch": r["bytes"], "avg_ms": round(r["ms"], 5)}
+            "bytes_per_launch": r["bytes"], "avg_ms": round(r["ms"], 5),
+            "batched4": {"achieved": round(ach4, 1), "frac": round(ach4 / 8000.0, 4),
+                         "bytes_per_launch": bp["bytes"], "avg_ms": round(bp["ms"], 5)}}
 
 
 def traffic_table(precision=0):

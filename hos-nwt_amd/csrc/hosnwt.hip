@@ -2423,7 +2423,7 @@ struct EngineBase {
   virtual void paddle_top(double* x, double* xt) = 0;
   // average duration (ms) and algorithmic HBM bytes of one extended-grid x pass
   // and one y pass (full inverse cos-cos transform), timed with HIP events
-  virtual void time_big(int reps, double* out4) = 0;
+  virtual void time_big(int reps, int items, double* out4) = 0;
   // SWENSE output (init_velocity_comp + HOSvel2_SWEET + HOSvel_SWEET) after a right-hand
   // side at the committed state: velocity and phi_t modes at z = 0 (out[0..3], n1 x n2),
   // the modal d eta / dt (out[4]); device pointers
@@ -3974,46 +3974,46 @@ struct Engine : EngineBase {
     HIP_OK(hipGetLastError());
   }
 
-  void time_big(int reps, double* out4) override {
+  // `items` independent full inverse cos-cos transforms in one launch per pass
+  // (items = 1: a lone transform; more: the batched launches of a right-hand
+  // side), each item with its own source and destination
+  void time_big(int reps, int items, double* out4) override {
     hos_ctx& C = *c;
+    if (items < 1 || items > 4) throw std::runtime_error("timed batch of 1 to 4 transforms");
+    T* xin[4] = {X, tmpX[0], tmpX[1], tmpX[2]};
+    T* mid[4] = {N2b > 1 ? tmpB : Y, tmpX[3], tmpX[4], tmpX[5]};
+    T* yout[4] = {Y, X, tmpX[0], tmpX[1]};
     hipEvent_t e0, e1, e2;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventCreate(&e2));
-    Src<T, T> sx = src_plain<T>(X, N1b, N1b, N2b);
-    Dst<T> dx{N2b > 1 ? tmpB : Y, N1b, 0};
+    LineBatch<Src<T, T>, Dst<T>> bx, by;
+    bx.n = by.n = items;
+    for (int i = 0; i < items; ++i) {
+      bx.kind[i] = by.kind[i] = KCOS;
+      bx.n_in[i] = bx.n_out[i] = N1b;
+      bx.nlines[i] = N2b;
+      bx.src[i] = src_plain<T>(xin[i], N1b, N1b, N2b);
+      bx.dst[i] = Dst<T>{mid[i], N1b, 0};
+      by.n_in[i] = by.n_out[i] = N2b;
+      by.nlines[i] = N1b;
+      by.src[i] = src_plain<T>(mid[i], N1b, N1b, N2b);
+      by.dst[i] = Dst<T>{yout[i], N1b, 0};
+    }
     HIP_OK(hipEventRecord(e0, C.st));
-    LineBatch<Src<T, T>, Dst<T>> bx;
-    bx.n = 1;
-    bx.kind[0] = KCOS;
-    bx.n_in[0] = N1b;
-    bx.n_out[0] = N1b;
-    bx.nlines[0] = N2b;
-    bx.src[0] = sx;
-    bx.dst[0] = dx;
     for (int r = 0; r < reps; ++r) launch_x<T>(gb.px, N1b, INV, bx, C.st, gwsM);
     HIP_OK(hipEventRecord(e1, C.st));
-    if (N2b > 1) {
-      Src<T, T> sy = src_plain<T>(tmpB, N1b, N1b, N2b);
-      LineBatch<Src<T, T>, Dst<T>> by;
-      by.n = 1;
-      by.kind[0] = KCOS;
-      by.n_in[0] = N2b;
-      by.n_out[0] = N2b;
-      by.nlines[0] = N1b;
-      by.src[0] = sy;
-      by.dst[0] = Dst<T>{Y, N1b, 0};
+    if (N2b > 1)
       for (int r = 0; r < reps; ++r) launch_y<T>(gb.py, gb.TX, N2b, INV, by, C.st);
-    }
     HIP_OK(hipEventRecord(e2, C.st));
     HIP_OK(hipEventSynchronize(e2));
     float mx = 0.f, my = 0.f;
     HIP_OK(hipEventElapsedTime(&mx, e0, e1));
     HIP_OK(hipEventElapsedTime(&my, e1, e2));
     out4[0] = mx / reps;
-    out4[1] = 2.0 * (double)nb * sizeof(T);
+    out4[1] = 2.0 * items * (double)nb * sizeof(T);
     out4[2] = N2b > 1 ? my / reps : 0.0;
-    out4[3] = N2b > 1 ? 2.0 * (double)nb * sizeof(T) : 0.0;
+    out4[3] = N2b > 1 ? 2.0 * items * (double)nb * sizeof(T) : 0.0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
@@ -5562,7 +5562,14 @@ int hos_check_slope(hos_ctx* c, double threshold, double coeff, int* fx, int* fy
 int hos_time_big_transform(hos_ctx* c, int reps, double* out4) {
   return guard([&] {
     if (reps < 1) throw std::runtime_error("reps must be >= 1");
-    c->eng->time_big(reps, out4);
+    c->eng->time_big(reps, 1, out4);
+  });
+}
+
+int hos_time_big_batch(hos_ctx* c, int reps, int items, double* out4) {
+  return guard([&] {
+    if (reps < 1) throw std::runtime_error("reps must be >= 1");
+    c->eng->time_big(reps, items, out4);
   });
 }
 

@@ -277,6 +277,14 @@ class Tank:
         return dict(kernel="r2r_seq_kernel x pass (extended grid, inverse cos)", ms=out[0], bytes=out[1],
                     other=dict(ms=out[2], bytes=out[3]), **{"pass": "x"})
 
+    def time_big_batch(self, reps: int = 20, items: int = 4):
+        """HIP-event timing of the extended-grid passes with `items` full inverse cos-cos
+        transforms per launch (the batched form of a right-hand side): per pass the
+        average launch duration and the items' algorithmic bytes."""
+        out = np.zeros(4)
+        check(self.lib.hos_time_big_batch(self.h, reps, items, ptr(out)))
+        return dict(x=dict(ms=out[0], bytes=out[1]), y=dict(ms=out[2], bytes=out[3]), items=items)
+
     def schedule(self):
         """The active engine's extended-grid transform schedule (see hos_get_schedule)."""
         s8 = (C.c_int * 8)()

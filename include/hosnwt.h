@@ -241,6 +241,10 @@ int hos_get_stats(hos_ctx* c, hos_stats* s);
  * inverse cos-cos transform, timed with HIP events on the context's stream.
  * out4 = {ms_x, bytes_x, ms_y, bytes_y}. */
 int hos_time_big_transform(hos_ctx* c, int reps, double* out4);
+/* The same for `items` (1 to 4) independent full transforms in one launch per
+ * pass, each with its own source and destination: the batched launches of a
+ * right-hand side.  Bytes are the items' total. */
+int hos_time_big_batch(hos_ctx* c, int reps, int items, double* out4);
 int hos_synchronize(hos_ctx* c);
 
 /* Standalone transforms on host arrays (copy in, transform on the GPU, copy
