@@ -775,39 +775,49 @@ __global__ void k_phiadd_reduce(int n1, int n2, int nch, double dx, const double
   phiz1[o] = acc[6];
 }
 
-// 2-D tank (n2 = 1): one thread per x-node, all seven fields.
+// 2-D tank (n2 = 1): one wave per x-node, its lanes over the wavemaker modes
+// (lane, lane + 64, ...), the six sums reduced across the wave in a fixed
+// order (bit-reproducible), all seven fields.
 template <typename T>
-__global__ void k_phiadd2d(int n1, int n3a, double xlen, double dx, const double* __restrict__ a,
-                           const double* __restrict__ da, const T* __restrict__ eta, double* o_phi, double* o_phit,
-                           double* o_phix, double* o_phiy, double* o_phiz, double* o_phit1, double* o_phiz1) {
-  const int i1 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i1 >= n1) return;
+__global__ void __launch_bounds__(256) k_phiadd2d(int n1, int n3a, double xlen, double dx, const double* __restrict__ a,
+                                                  const double* __restrict__ da, const T* __restrict__ eta,
+                                                  double* o_phi, double* o_phit, double* o_phix, double* o_phiy,
+                                                  double* o_phiz, double* o_phit1, double* o_phiz1) {
+  const int lane = threadIdx.x & 63;
+  const int i1 = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (i1 >= n1) return;  // uniform over the wave
   const double x = i1 * dx;
   const double ep1 = (double)eta[i1] + 1.0;
-  double phi = 0, phit = 0, phix = 0, phiz = 0, phit1 = 0, phiz1 = 0;
-  for (int i3 = 0; i3 < n3a; ++i3) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int i3 = lane; i3 < n3a; i3 += 64) {
     const double kxa = kx_add_of(i3);
-    if (x * kxa > kCut) break;
+    if (x * kxa > kCut) break;  // the modes increase with i3
     double csh, sh;
     csh_sh(kxa, x, xlen, csh, sh);
     double sn, cs, s0, c0;
     sincos(kxa * ep1, &sn, &cs);
     sincos(kxa, &s0, &c0);
     const double av = a[i3], dav = da[i3];
-    phi += cs * (av * csh);
-    phit += cs * (dav * csh);
-    phix += cs * (-av * sh * kxa);
-    phiz += sn * (-av * csh * kxa);
-    phit1 += c0 * (dav * csh);
-    phiz1 += s0 * (-av * csh * kxa);
+    acc[0] += cs * (av * csh);
+    acc[1] += cs * (dav * csh);
+    acc[2] += cs * (-av * sh * kxa);
+    acc[3] += sn * (-av * csh * kxa);
+    acc[4] += c0 * (dav * csh);
+    acc[5] += s0 * (-av * csh * kxa);
   }
-  o_phi[i1] = phi;
-  o_phit[i1] = phit;
-  o_phix[i1] = phix;
-  o_phiy[i1] = 0.0;
-  o_phiz[i1] = phiz;
-  o_phit1[i1] = phit1;
-  o_phiz1[i1] = phiz1;
+#pragma unroll
+  for (int f = 0; f < 6; ++f)
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) acc[f] += __shfl_xor(acc[f], d, 64);
+  if (lane == 0) {
+    o_phi[i1] = acc[0];
+    o_phit[i1] = acc[1];
+    o_phix[i1] = acc[2];
+    o_phiy[i1] = 0.0;
+    o_phiz[i1] = acc[3];
+    o_phit1[i1] = acc[4];
+    o_phiz1[i1] = acc[5];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -3037,14 +3047,30 @@ struct Engine : EngineBase {
     // while the host enqueues the chain ---
     HIP_OK(hipEventRecord(ev_fork, st));
     // --- phisxy_etaxy (resol_HOS.f90:924-1078): the extended-grid eta here, with the
-    // first HOS level's one transform (kth1 phis); the slopes and phis gradients,
-    // read only after the join (the free-surface products), on the side stream ---
+    // first HOS level's one transform (kth1 phis).  The slopes and phis gradients
+    // are read only after the join (the free-surface products): in 3-D they go to
+    // the side stream, which has room beside the HOS levels; a 2-D tank, bound by
+    // kernel latency, keeps them in this launch (one launch fewer per pass) ---
+    const bool slopes_side = n2 > 1;
+    auto slopes = [&](Batch2<T, Src<T, T>, Dst<T>>& b, std::vector<T*>& tx) {
+      b.add(KSIN, KCOS, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0},
+            tx[b.n]);
+      b.add(KSIN, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{phisx, ldb, 0},
+            tx[b.n]);
+      if (n2 > 1) {
+        b.add(KCOS, KSIN, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b, Dst<T>{etay, ldb, 0},
+              tx[b.n]);
+        b.add(KCOS, KSIN, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
+              Dst<T>{phisy, ldb, 0}, tx[b.n]);
+      }
+    };
     {
       Batch2<T, Src<T, T>, Dst<T>> b;
       b.add(KCOS, KCOS, src_plain<T>(eta_hat, n1, n1, n2), N1b, N2b, Dst<T>{eta_ext ? eta_ext : tmpB2(), ldb, 0},
             tmpX[0]);
       b.add(KCOS, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_KTH, 1, pix, piy), N1b, N2b, Dst<T>{lvl[0], ldb, 0},
             tmpX[1]);
+      if (!slopes_side) slopes(b, tmpX);
       bigB(INV, b);
     }
     // --- HOSphis_modes recursion (resol_HOS.f90:720-782) ---
@@ -3098,18 +3124,9 @@ struct Engine : EngineBase {
         ~Side() { e->use_main(); }
       } side{this};
       use_side();
-      {
+      if (slopes_side) {
         Batch2<T, Src<T, T>, Dst<T>> b;
-        b.add(KSIN, KCOS, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b, Dst<T>{etax, ldb, 0},
-              (*ctmpX)[0]);
-        b.add(KSIN, KCOS, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KX, 0, pix, piy), N1b, N2b,
-              Dst<T>{phisx, ldb, 0}, (*ctmpX)[1]);
-        if (n2 > 1) {
-          b.add(KCOS, KSIN, src_mul<T>(eta_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
-                Dst<T>{etay, ldb, 0}, (*ctmpX)[2]);
-          b.add(KCOS, KSIN, src_mul<T>(phis_hat, n1, n1, n2, MUL_NEG_KY, 0, pix, piy), N1b, N2b,
-                Dst<T>{phisy, ldb, 0}, (*ctmpX)[3]);
-        }
+        slopes(b, *ctmpX);
         bigB(INV, b);
       }
       wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, s2);
@@ -3958,7 +3975,7 @@ struct Engine : EngineBase {
     hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2, n3a = C.n3a;
     if (n2 == 1) {
-      hipLaunchKernelGGL((k_phiadd2d<T>), dim3(nblk(n1, 64)), dim3(64), 0, st, n1, n3a, C.p.xlen, C.dx, a, da, eta,
+      hipLaunchKernelGGL((k_phiadd2d<T>), dim3(nblk(n1, 4)), dim3(256), 0, st, n1, n3a, C.p.xlen, C.dx, a, da, eta,
                          phi_add, phit_add, phix_add, phiy_add, phiz_add, phit1, phiz1);
     } else {
       // one thread per y-node pair (kPhi3Per pairs per thread beyond 256 threads)
