@@ -398,9 +398,25 @@ __global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* 
   const int i1 = (int)(i % n1), i2 = (int)(i / n1);
   T a, b;
   if (nparts > 0) {
+    // the chunks' partial sums in chunk order; their loads are issued eight
+    // chunks at a time (the sums keep the order)
     a = RA[i];
     b = RA[n1 + i];
-    for (int s = 1; s < nparts; ++s) {
+    int s = 1;
+    for (; s + 8 <= nparts; s += 8) {
+      T pa[8], pb[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        pa[q] = RA[(size_t)(2 * (s + q)) * n1 + i];
+        pb[q] = RA[(size_t)(2 * (s + q) + 1) * n1 + i];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        a += pa[q];
+        b += pb[q];
+      }
+    }
+    for (; s < nparts; ++s) {
       a += RA[(size_t)(2 * s) * n1 + i];
       b += RA[(size_t)(2 * s + 1) * n1 + i];
     }
@@ -3229,7 +3245,8 @@ struct Engine : EngineBase {
       gemm(cs, true, true, n1, 2 * n2, K1, (const T*)T1, K1, (const T*)PA, K1, RA, n1);
       if (n2 > 1) gemm(cs, false, true, 2 * n1, n2, K2, (const T*)PB, 2 * n1, (const T*)T2, K2, RB, 2 * n1);
     }
-    hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, (const T*)RA,
+    // (64-thread blocks: a 2-D tank's few outputs spread over more CUs)
+    hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns, 64)), dim3(64), 0, cs, n1, n2, (const T*)RA,
                        (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q, t1_parts);
   }
 
