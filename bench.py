@@ -105,33 +105,71 @@ def time_path(precision, steps, warmup, spin_up, follow=None):
     st1 = tank.stats()
     eta = tank.get_state()["eta"]
     out = dict(steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, n_acc=na, n_rej=nr, t_warm=t_warm, t_end=t,
-               eta=eta,
+               h_end=h, eta=eta,
                big_per_step=(st1["n_big"] - st0["n_big"]) / steps, base_per_step=(st1["n_base"] - st0["n_base"]) / steps,
                big_grid=(st1["big_n1"], st1["big_n2"]), tank=tank, setup=s)
     return out
 
 
-def roofline(tank, precision):
+def log_passes(r, steps):
+    """Per-launch HIP-event timing of the extended-grid passes as the solver launches them:
+    ``steps`` further attempts of a timed run (same tank, state and controller), every pass
+    launch bracketed by two events on the stream it runs on.  Returns the log and the
+    attempts/s of the logged interval (the events' own cost included)."""
+    tank, s = r["tank"], r["setup"]
+    tank.synchronize()
+    tank.pass_log_begin(32768)
+    t0 = time.perf_counter()
+    _, _, na, nr = run_steps(tank, s, steps, r["t_end"], r["h_end"])
+    tank.synchronize()
+    dt = time.perf_counter() - t0
+    lg = tank.pass_log_end()
+    return lg, (na + nr) / dt
+
+
+def pass_figure(lg):
+    ach = lg["bytes"] / (lg["ms"] * 1e-3) / 1e9
+    n = lg["launches"]
+    return {"achieved": round(ach, 1), "frac": round(ach / 8000.0, 4), "launches": n,
+            "transforms_per_launch": round(lg["transforms"] / n, 3), "bytes_per_launch": round(lg["bytes"] / n),
+            "avg_ms": round(lg["ms"] / n, 5)}
+
+
+def roofline(r, precision, steps):
     """Live HIP-event timing of the dominant kernel family (extended-grid r2r passes) in the
-    precision of the headline path, on the context's stream: the slower pass of a lone full
-    transform, and (``batched4``) the same pass launched with four transforms at once."""
-    r = tank.time_big_transform(reps=20)
-    achieved = r["bytes"] / (r["ms"] * 1e-3) / 1e9
+    precision of the headline path.  Top level: the slower pass of a lone full inverse
+    transform on the context's stream (full-grid bytes, 20 launches between two events).
+    ``batched4``: four such transforms per launch.  ``in_step``: the solver's own launches over
+    ``steps`` attempts that continue the timed run, each bracketed by two events on its stream
+    (the pairs' own cost included; the logged interval's attempts/s shows it), with the
+    measured HBM traffic over algorithmic bytes of those launches."""
+    tank = r["tank"]
+    st = tank.stats()
+    grid = [st["big_n1"], st["big_n2"]]
+    lone = tank.time_big_transform(reps=20)
+    achieved = lone["bytes"] / (lone["ms"] * 1e-3) / 1e9
     traffic = None
     t = traffic_table(precision)
-    st = tank.stats()
-    if t is not None and list(t["grid"]) == [st["big_n1"], st["big_n2"]]:
-        traffic = t["traffic_bytes"][r["pass"]]
-    # the same passes as the right-hand side launches them: four independent full
-    # transforms per launch (each level's and each product batch's shape)
+    if t is not None and list(t["grid"]) == grid:
+        traffic = t["traffic_bytes"][lone["pass"]]
     b = tank.time_big_batch(reps=20, items=4)
-    bp = b[r["pass"]]
+    bp = b[lone["pass"]]
     ach4 = bp["bytes"] / (bp["ms"] * 1e-3) / 1e9
+    lg, logged_rate = log_passes(r, steps)
+    ti = in_step_traffic_table(precision)
+    in_step = {"attempts": steps, "logged_steps_per_s": round(logged_rate, 2), "full": lg["full"]}
+    for p in ("x", "y"):
+        if lg[p]["launches"] > 0:
+            f = pass_figure(lg[p])
+            if ti is not None and list(ti["grid"]) == grid:
+                f["traffic_over_bytes"] = ti["traffic_ratio"][p]
+            in_step[p] = f
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": r["kernel"],
-            "bytes_per_launch": r["bytes"], "avg_ms": round(r["ms"], 5),
+            "frac": round(achieved / 8000.0, 4), "traffic": traffic, "kernel": lone["kernel"],
+            "bytes_per_launch": lone["bytes"], "avg_ms": round(lone["ms"], 5),
             "batched4": {"achieved": round(ach4, 1), "frac": round(ach4 / 8000.0, 4),
-                         "bytes_per_launch": bp["bytes"], "avg_ms": round(bp["ms"], 5)}}
+                         "bytes_per_launch": bp["bytes"], "avg_ms": round(bp["ms"], 5)},
+            "in_step": in_step}
 
 
 def traffic_table(precision=0):
@@ -147,17 +185,43 @@ def traffic_table(precision=0):
         return None
 
 
-def pass_probe(reps, precision=0):
-    """Extended-grid pass timing (the roofline launches) plus a 64 MiB device-to-device copy whose
-    blit kernel calibrates rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
-    import ctypes as C
+def in_step_traffic_table(precision=0):
+    """Measured HBM traffic over algorithmic bytes of the solver's own pass launches (scaled
+    FETCH_SIZE + WRITE_SIZE summed over a short run's launches), from the committed summary
+    profiles/extended_pass_1024x256_M5_in_step.md; None when absent."""
+    path = os.path.join(ROOT, "profiles", "extended_pass_1024x256_M5_in_step.md")
+    try:
+        text = open(path).read()
+        blocks = [b.split("```", 1)[0] for b in text.split("```json")[1:]]
+        for b in blocks:
+            d = json.loads(b)
+            if d.get("precision") == precision:
+                return d
+    except (OSError, IndexError, ValueError):
+        return None
+    return None
+
+
+def pass_log_probe(steps, precision):
+    """The solver's own extended-grid pass launches over ``steps`` attempts from rest (after
+    two untimed ones), each timed with HIP events: for counter and kernel-trace runs."""
     from hosnwt.tank import Tank
     s = make_setup(precision)
     tank = Tank(s)
-    tank.time_big_transform(reps=2)
-    r = tank.time_big_transform(reps=reps)
-    print(json.dumps({"pass_ms": {r["kernel"]: r["ms"], "other": r["other"]["ms"]}}), flush=True)
+    tank.initialize()
+    h0 = min(0.95 * s.dt_stab, s.delt_out)
+    t, h, _, _ = run_steps(tank, s, 2, 0.0, h0)
+    lg, rate = log_passes(dict(tank=tank, setup=s, t_end=t, h_end=h), steps)
+    print(json.dumps({"precision": precision, "attempts": steps, "logged_steps_per_s": round(rate, 2),
+                      "x": lg["x"], "y": lg["y"], "full": lg["full"]}), flush=True)
     tank.close()
+    calibration_copy()
+
+
+def calibration_copy():
+    """Three 64 MiB device-to-device copies: their blit kernel moves known bytes, against which
+    rocprofv3's FETCH_SIZE / WRITE_SIZE counters are scaled."""
+    import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     nbytes = 64 * 1024 * 1024
     a, b = C.c_void_p(), C.c_void_p()
@@ -172,6 +236,18 @@ def pass_probe(reps, precision=0):
     hip.hipFree(a)
     hip.hipFree(b)
     print("copied 64 MiB device-to-device 3 times", flush=True)
+
+
+def pass_probe(reps, precision=0):
+    """Extended-grid pass timing (the lone roofline launches) plus the counter-calibration copy."""
+    from hosnwt.tank import Tank
+    s = make_setup(precision)
+    tank = Tank(s)
+    tank.time_big_transform(reps=2)
+    r = tank.time_big_transform(reps=reps)
+    print(json.dumps({"pass_ms": {r["kernel"]: r["ms"], "other": r["other"]["ms"]}}), flush=True)
+    tank.close()
+    calibration_copy()
 
 
 def mlp_probe(epochs):
@@ -472,6 +548,8 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--pass-probe", type=int, default=0, metavar="REPS",
                     help="only time the roofline passes REPS times and run the counter-calibration copy")
+    ap.add_argument("--pass-log-probe", type=int, default=0, metavar="STEPS",
+                    help="only run STEPS attempts with every extended-grid pass launch timed (profiling)")
     ap.add_argument("--learned", type=float, default=0.0, metavar="T",
                     help="only run the learned-order ensemble (16 x 512x128, M_max = 7) to time T against "
                          "the fixed-order solver")
@@ -487,6 +565,9 @@ def main():
     args = ap.parse_args()
     if args.pass_probe:
         pass_probe(args.pass_probe, args.precision)
+        return
+    if args.pass_log_probe:
+        pass_log_probe(args.pass_log_probe, args.precision)
         return
     if args.ensemble:
         print(json.dumps(ensemble_bench(args.ensemble)), flush=True)
@@ -516,7 +597,7 @@ def main():
           file=sys.stderr, flush=True)
     r64 = time_path(0, args.steps, args.warmup, spin)
     print(f"[bench] FP64 {r64['steps_per_s']:.2f} steps/s", file=sys.stderr, flush=True)
-    roof64 = roofline(r64["tank"], 0)
+    roof64 = roofline(r64, 0, args.steps)
     sched = {"fp64": r64["tank"].schedule()}
     # one live context at a time, the deployment shape: with the FP64 context still alive the
     # mixed path measured 114 instead of 135 steps/s (its two streams then share the process's
@@ -537,7 +618,7 @@ def main():
     best = r32 if use_mixed else r64
     # the roofline object describes the headline path's dominant kernel (the FP32 passes when
     # the mixed path is the headline)
-    roof = roofline(r32["tank"], 1) if use_mixed else roof64
+    roof = roofline(r32, 1, args.steps) if use_mixed else roof64
     if r32 is not None:
         sched["mixed"] = r32["tank"].schedule()
     cpu = None

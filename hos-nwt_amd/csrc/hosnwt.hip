@@ -2478,6 +2478,8 @@ struct hos_ctx {
   // of order M on the extended grid N1b x N2b
   std::unique_ptr<EngineBase> engines[9];
   EngineBase* eng = nullptr;
+  // per-launch pass timing between hos_pass_log_begin and hos_pass_log_end
+  std::unique_ptr<PassLog> plog;
   // host copies for the paddle-top displacement pos1st(n3,1)
   std::vector<double> freq, pre0, pim0;
   // host copies of the wavemaker inputs (pos_ini, dposdy_ini: n2 x n_freq; d nu/dx,
@@ -2678,12 +2680,13 @@ struct Engine : EngineBase {
     na = (size_t)c->n3a * c->n2;
     eta_hat = ar.get<T>(ns); phis_hat = ar.get<T>(ns); eta_phys = ar.get<T>(ns);
     P = ar.get<T>(ns); D1 = ar.get<T>(ns); D2 = ar.get<T>(ns); Q = ar.get<T>(ns);
-    Zp = ar.get<T>(ns); Zh = ar.get<T>(ns); phixh = ar.get<T>(ns); phiyh = ar.get<T>(ns);
-    phizh = ar.get<T>(ns); tmpS = ar.get<T>(ns);
+    // (outputs of one batched base-grid product lie ns apart: one launch writes them)
+    Zp = ar.get<T>(ns); Zh = ar.get<T>(ns); phixh = ar.get<T>(3 * ns); phiyh = phixh + ns;
+    phizh = phiyh + ns; tmpS = ar.get<T>(ns);
     phix_addT = ar.get<T>(ns); phiy_addT = ar.get<T>(ns); phiz_addT = ar.get<T>(ns);
     if (kSplit) {
-      phis64 = ar.get<double>(ns); wlin = ar.get<double>(ns);
-      Zf64 = ar.get<double>(ns); Zh64 = ar.get<double>(ns); phizh64 = ar.get<double>(ns);
+      phis64 = ar.get<double>(ns);
+      Zf64 = ar.get<double>(ns); phizh64 = ar.get<double>(2 * ns); Zh64 = phizh64 + ns;
     }
     if (M >= 2) eta_ext = ar.get<T>(nb);
     phizm.assign(M + 1, nullptr);
@@ -2709,7 +2712,8 @@ struct Engine : EngineBase {
     phi_add = c->phi_add_last;
     phix_add = ar.get<double>(ns); phiy_add = ar.get<double>(ns);
     phiz_add = ar.get<double>(ns); phit_add = ar.get<double>(ns); phit1 = ar.get<double>(ns);
-    phiz1 = ar.get<double>(ns); w1 = ar.get<double>(ns); tmp1 = ar.get<double>(ns);
+    phiz1 = ar.get<double>(ns); w1 = ar.get<double>(kSplit ? 2 * ns : ns); tmp1 = ar.get<double>(ns);
+    if (kSplit) wlin = w1 + ns;
     // REDFT01 along z and REDFT00 along y as column-major matrices
     std::vector<double> h_gz(n3), h_wz((size_t)n3a * n3a), h_wy((size_t)n2 * n2);
     for (int i3 = 0; i3 < n3; ++i3) h_gz[i3] = gzed(i3 * c->delz - 1.0, c->p.igeom, c->p.d_hinge);
@@ -2743,7 +2747,7 @@ struct Engine : EngineBase {
     if (dense) {
       build_dense<T>(dT, ar, c->n1, c->n2);
       if (sizeof(T) != sizeof(double)) build_dense<double>(d64, ar, c->n1, c->n2);
-      if (c->n2 > 1) multi = ar.get<double>((size_t)3 * kMulti * ns);
+      if (c->n2 > 1) multi = ar.get<double>((size_t)2 * kMulti * ns);
     }
     const int nf = c->p.n_freq;
     // (from the context's host copies: engines of other orders are built after hos_create)
@@ -3023,14 +3027,17 @@ struct Engine : EngineBase {
     if (k > kMulti || !multi) throw std::runtime_error("dense_multi: no scratch for this batch");
     U* stage = reinterpret_cast<U*>(multi);
     U* mid = stage + (size_t)kMulti * ns;
-    U* res = mid + (size_t)kMulti * ns;
     for (int f = 0; f < k; ++f)
       hipLaunchKernelGGL((k_dense_in<U, Src<U, double>>), dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, src[f],
                          stage + (size_t)f * ns);
     gemm(cs, true, true, n1, k * n2, n1, (const U*)g.mx[KCOS][di], n1, (const U*)stage, n1, mid, n1);
-    gemm_yb(cs, n1, n2, mid, (long)ns, g.my[KCOS][di], res, k);
-    for (int f = 0; f < k; ++f)
-      HIP_OK(hipMemcpyAsync(out[f], res + (size_t)f * ns, ns * sizeof(U), hipMemcpyDeviceToDevice, cs));
+    // the y products write each run of outputs that lie ns apart in one launch
+    for (int f0 = 0; f0 < k;) {
+      int f1 = f0 + 1;
+      while (f1 < k && out[f1] == out[f0] + (size_t)(f1 - f0) * ns) ++f1;
+      gemm_yb(cs, n1, n2, mid + (size_t)f0 * ns, (long)ns, g.my[KCOS][di], out[f0], f1 - f0);
+      f0 = f1;
+    }
     c->stats.n_base += k;
   }
 
@@ -3775,10 +3782,10 @@ struct Engine : EngineBase {
     if constexpr (kSplit) {
       hipLaunchKernelGGL(k_forcing, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phiz_add,
                          (const double*)phit_add, (const double*)wlin, (const double*)C.nu_d, Zf64);
-      Src<double, double> in[3] = {src_plain<double, double>(tmp1, n1, n1, n2),
-                                   src_plain<double, double>(phiz_add, n1, n1, n2),
-                                   src_plain<double, double>(Zf64, n1, n1, n2)};
-      double* out[3] = {ka1, phizh64, Zh64};
+      Src<double, double> in[3] = {src_plain<double, double>(phiz_add, n1, n1, n2),
+                                   src_plain<double, double>(Zf64, n1, n1, n2),
+                                   src_plain<double, double>(tmp1, n1, n1, n2)};
+      double* out[3] = {phizh64, Zh64, ka1};
       dense_multi<double>(grid64(), FWD, 3, in, out);
       hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phizh64, phizh);
       Src<T, double> inT[2] = {src_plain<T, double>(phix_add, n1, n1, n2), src_plain<T, double>(phiy_add, n1, n1, n2)};
@@ -4421,6 +4428,7 @@ void graph_release(hos_ctx* c) {
 }
 
 void graph_prepare(hos_ctx* c) {
+  if (t_pass_log) throw std::runtime_error("graph capture while a pass log is recording");
   if (!c->wmk || c->wmk2)
     throw std::runtime_error("graph-captured attempts need a wavemaker tank with i_wmk = 1");
   if (c->feat_mode == 2) throw std::runtime_error("graph-captured attempts record features in mode 0 or 1 only");
@@ -4949,6 +4957,8 @@ int hos_destroy(hos_ctx* c) {
   return guard([&] {
     if (!c) return;
     (void)hipStreamSynchronize(c->st);
+    if (c->plog && t_pass_log == c->plog.get()) t_pass_log = nullptr;
+    c->plog.reset();
     for (auto& e : c->engines) e.reset();
     c->eng = nullptr;
     c->gs64.release();
@@ -5604,6 +5614,45 @@ int hos_time_big_batch(hos_ctx* c, int reps, int items, double* out4) {
   return guard([&] {
     if (reps < 1) throw std::runtime_error("reps must be >= 1");
     c->eng->time_big(reps, items, out4);
+  });
+}
+
+int hos_pass_log_begin(hos_ctx* c, int max_launches) {
+  return guard([&] {
+    if (max_launches < 1 || max_launches > (1 << 16)) throw std::runtime_error("pass log of 1 to 65536 launches");
+    if (t_pass_log) throw std::runtime_error("a pass log is already recording on this host thread");
+    auto lg = std::make_unique<PassLog>();
+    lg->N[0] = c->N1b;
+    lg->N[1] = c->N2b > 1 ? c->N2b : 0;
+    lg->cap = (size_t)max_launches;
+    lg->ev.resize(2 * lg->cap, nullptr);
+    for (auto& e : lg->ev) HIP_OK(hipEventCreate(&e));
+    lg->bytes.reserve(lg->cap);
+    lg->pass.reserve(lg->cap);
+    lg->items.reserve(lg->cap);
+    c->plog = std::move(lg);
+    t_pass_log = c->plog.get();
+  });
+}
+
+int hos_pass_log_end(hos_ctx* c, double* out9) {
+  return guard([&] {
+    if (!c->plog) throw std::runtime_error("no pass log is recording");
+    if (t_pass_log != c->plog.get()) throw std::runtime_error("pass log ended on another host thread");
+    t_pass_log = nullptr;
+    std::unique_ptr<PassLog> lg = std::move(c->plog);
+    for (int k = 0; k < 9; ++k) out9[k] = 0.0;
+    for (size_t i = 0; i < lg->bytes.size(); ++i) {
+      HIP_OK(hipEventSynchronize(lg->ev[2 * i + 1]));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, lg->ev[2 * i], lg->ev[2 * i + 1]));
+      double* o = out9 + 4 * lg->pass[i];
+      o[0] += 1.0;
+      o[1] += lg->bytes[i];
+      o[2] += ms;
+      o[3] += lg->items[i];
+    }
+    out9[8] = lg->full ? 1.0 : 0.0;
   });
 }
 

@@ -731,6 +731,44 @@ inline int batch_max_lines(const LineBatch<S, D>& b) {
   return m;
 }
 
+// Measurement hook: while a PassLog is installed on the calling host thread, each
+// register-engine launch of x lines of N[0] points or y lines of N[1] points is
+// bracketed by two events on its stream, with its algorithmic bytes.
+struct PassLog {
+  int N[2] = {0, 0};
+  size_t cap = 0;
+  std::vector<hipEvent_t> ev;  // 2 per launch, created before the measured interval
+  std::vector<double> bytes;
+  std::vector<int> pass;   // 0 x, 1 y
+  std::vector<int> items;  // transforms in the launch
+  bool full = false;
+  PassLog() = default;
+  PassLog(const PassLog&) = delete;
+  PassLog& operator=(const PassLog&) = delete;
+  ~PassLog() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+inline thread_local PassLog* t_pass_log = nullptr;
+
+// Algorithmic HBM bytes of one launch: every valid source element read once
+// (zeros beyond a source's extents are not read), every output written once
+// (read and written where the destination accumulates).
+template <bool YP, class S, class D>
+double batch_bytes(const LineBatch<S, D>& b, int N) {
+  double s = 0.0;
+  for (int i = 0; i < b.n; ++i) {
+    if (b.nlines[i] <= 0 || b.n_out[i] <= 0) continue;
+    const S& src = b.src[i];
+    const long lines_in = std::max(0, std::min(b.nlines[i], YP ? src.nx : src.ny));
+    const long per_line = std::max(0, std::min(N, YP ? src.ny : src.nx));
+    s += (double)lines_in * per_line * sizeof(*src.p);
+    s += (double)b.nlines[i] * b.n_out[i] * sizeof(*b.dst[i].p) * (b.dst[i].mode ? 2 : 1);
+  }
+  return s;
+}
+
 template <typename T, int V, bool YP, class S, class D, int NC, int... Rs>
 void launch_seq_nc(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBatch<S, D>& b, hipStream_t st) {
   static_assert(hosfft2::hosfft2_prod<Rs...>::value % V == 0, "V must divide the line length");
@@ -751,8 +789,24 @@ void launch_seq_nc(const LinePlan<T>& pl, int ncl, int N, int dir, const LineBat
   if (YP) groups = (groups + 7) / 8 * 8;  // XCD-aware tile order needs a multiple of 8
   auto kern = hosfft2::r2r_seq_kernel<T, V, YP, S, D, NC, Rs...>;
   allow_lds(kern, lds);
+  PassLog* lg = t_pass_log;
+  hipEvent_t* ev = nullptr;
+  if (lg && lg->N[YP ? 1 : 0] == N) {
+    if (lg->bytes.size() < lg->cap) {
+      ev = &lg->ev[2 * lg->bytes.size()];
+      lg->bytes.push_back(batch_bytes<YP>(b, N));
+      lg->pass.push_back(YP ? 1 : 0);
+      int items = 0;
+      for (int i = 0; i < b.n; ++i) items += (b.nlines[i] > 0 && b.n_out[i] > 0) ? 1 : 0;
+      lg->items.push_back(items);
+      HIP_OK(hipEventRecord(ev[0], st));
+    } else {
+      lg->full = true;
+    }
+  }
   hipLaunchKernelGGL(kern, dim3(groups, b.n), dim3(nt), lds, st, N, dir, ncl, b, pl.dev2);
   HIP_OK(hipGetLastError());
+  if (ev) HIP_OK(hipEventRecord(ev[1], st));
 }
 
 // y tiles: the lines per workgroup (tile width / 2) as a compile-time constant

@@ -55,7 +55,7 @@ EXPORTS = [
     "hos_create", "hos_destroy", "hos_last_error", "hos_initialize", "hos_set_state",
     "hos_get_state", "hos_rk_step", "hos_commit", "hos_end_ramp", "hos_advance", "hos_rhs",
     "hos_get_physical", "hos_get_slope_max", "hos_get_wmk", "hos_get_stats", "hos_synchronize",
-    "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_time_big_batch", "hos_check_slope",
+    "hos_space_2_fourier", "hos_fourier_2_space", "hos_time_big_transform", "hos_time_big_batch", "hos_pass_log_begin", "hos_pass_log_end", "hos_check_slope",
     "hos_set_state2", "hos_get_state2", "hos_rhs2", "hos_set_state3", "hos_get_state3", "hos_rhs3",
     "hos_get_wmk_motion", "hos_get_modes", "hos_probes", "hos_advance_ensemble",
     "hos_set_order", "hos_get_order", "hos_record_features", "hos_take_features",
@@ -110,6 +110,8 @@ def load() -> C.CDLL:
     lib.hos_synchronize.argtypes = [vp]
     lib.hos_time_big_transform.argtypes = [vp, C.c_int, _dp]
     lib.hos_time_big_batch.argtypes = [vp, C.c_int, C.c_int, _dp]
+    lib.hos_pass_log_begin.argtypes = [vp, C.c_int]
+    lib.hos_pass_log_end.argtypes = [vp, _dp]
     ip, fp = C.POINTER(C.c_int), C.POINTER(C.c_float)
     lib.hos_set_order.argtypes = [vp, C.c_int]
     lib.hos_get_order.argtypes = [vp, ip]

@@ -285,6 +285,20 @@ class Tank:
         check(self.lib.hos_time_big_batch(self.h, reps, items, ptr(out)))
         return dict(x=dict(ms=out[0], bytes=out[1]), y=dict(ms=out[2], bytes=out[3]), items=items)
 
+    def pass_log_begin(self, max_launches: int = 8192):
+        """Start timing every extended-grid pass launch this thread enqueues for the tank
+        (two HIP events per launch on its stream; see hos_pass_log_begin)."""
+        check(self.lib.hos_pass_log_begin(self.h, max_launches))
+
+    def pass_log_end(self):
+        """Stop timing: per pass the launches, their algorithmic bytes, summed durations (ms)
+        and transforms, and whether launches beyond the log's capacity went unrecorded."""
+        out = np.zeros(9)
+        check(self.lib.hos_pass_log_end(self.h, ptr(out)))
+        return dict(x=dict(launches=int(out[0]), bytes=out[1], ms=out[2], transforms=int(out[3])),
+                    y=dict(launches=int(out[4]), bytes=out[5], ms=out[6], transforms=int(out[7])),
+                    full=bool(out[8]))
+
     def schedule(self):
         """The active engine's extended-grid transform schedule (see hos_get_schedule)."""
         s8 = (C.c_int * 8)()

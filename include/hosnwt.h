@@ -245,6 +245,20 @@ int hos_time_big_transform(hos_ctx* c, int reps, double* out4);
  * pass, each with its own source and destination: the batched launches of a
  * right-hand side.  Bytes are the items' total. */
 int hos_time_big_batch(hos_ctx* c, int reps, int items, double* out4);
+/* Per-launch timing of the extended-grid passes as the solver launches them.
+ * Between hos_pass_log_begin and hos_pass_log_end, called on the same host
+ * thread, every register-engine pass launch of this context's extended grid
+ * that the thread enqueues (hos_advance, hos_rhs; not graph-captured attempts,
+ * which refuse to capture meanwhile) is bracketed by two HIP events on its
+ * stream, up to max_launches (1 to 65536) launches; the events are created in
+ * hos_pass_log_begin.  out9 = {x launches, x algorithmic bytes, x ms, x transforms,
+ * y launches, y bytes, y ms, y transforms, 1 if launches beyond max_launches went
+ * unrecorded}; ms sums each launch's event pair (the pair's own cost included).
+ * Algorithmic bytes: each valid source element read once, each output written
+ * once (read and written where the destination accumulates).  A measurement
+ * tool with no counterpart in the reference. */
+int hos_pass_log_begin(hos_ctx* c, int max_launches);
+int hos_pass_log_end(hos_ctx* c, double* out9);
 int hos_synchronize(hos_ctx* c);
 
 /* Standalone transforms on host arrays (copy in, transform on the GPU, copy
