@@ -391,7 +391,7 @@ __global__ void k_t1_partial(int n1, int K1, int kc, const T* __restrict__ T1h, 
 // chunk sums of k_t1_partial, added here in chunk order.
 template <typename T>
 __global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* D2, const T* P, T* Q,
-                            int nparts) {
+                            int nparts, int psplit) {
   const size_t n = (size_t)n1 * n2;
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -420,13 +420,20 @@ __global__ void k_add_mixed(int n1, int n2, const T* RA, const T* RB, T* D1, T* 
       a += RA[(size_t)(2 * s) * n1 + i];
       b += RA[(size_t)(2 * s + 1) * n1 + i];
     }
+  } else if (psplit) {
+    // rows of RA in parity order: even i1 first
+    const size_t r = ((i1 & 1) ? (n1 >> 1) + (i1 >> 1) : (i1 >> 1)) + (size_t)n1 * i2;
+    a = RA[r];
+    b = RA[n + r];
   } else {
     a = RA[i];
     b = RA[n + i];
   }
   if (RB) {
-    a += RB[i1 + (size_t)2 * n1 * i2];
-    b += RB[n1 + i1 + (size_t)2 * n1 * i2];
+    // with psplit the columns of RB in parity order: even i2 first
+    const int c2 = psplit ? ((i2 & 1) ? (n2 >> 1) + (i2 >> 1) : (i2 >> 1)) : i2;
+    a += RB[i1 + (size_t)2 * n1 * c2];
+    b += RB[n1 + i1 + (size_t)2 * n1 * c2];
   }
   const T d1 = D1[i] + a;
   D1[i] = d1;
@@ -2042,6 +2049,23 @@ std::vector<double> sin_to_cos_matrix(int N, int n, int K) {
   return t;
 }
 
+// The two nonzero parity blocks of an n x K map t (column-major, t[k + n s] = 0
+// where s + k is even; n and K even), each k-major for the products: block 0
+// maps the odd inputs s = 2j + 1 to the even outputs k = 2i, block 1 the even
+// inputs s = 2j to the odd outputs k = 2i + 1; element (i, j) of a block at
+// j + (K / 2) i.
+std::vector<double> parity_halves_kmajor(const std::vector<double>& t, int n, int K) {
+  const int h = n / 2, hk = K / 2;
+  std::vector<double> out((size_t)2 * h * hk);
+  for (int b = 0; b < 2; ++b)
+    for (int i = 0; i < h; ++i)
+      for (int j = 0; j < hk; ++j) {
+        const int k = 2 * i + b, s = 2 * j + 1 - b;
+        out[(size_t)b * h * hk + j + (size_t)hk * i] = t[k + (size_t)n * s];
+      }
+  return out;
+}
+
 // C (m x n) = A (m x k) B (k x n), C column-major, on the matrix cores (gemm.h);
 // ak / bk: the operand is stored k-contiguous (A[kk + lda i], B[kk + ldb j])
 template <typename U>
@@ -2617,6 +2641,7 @@ struct Engine : EngineBase {
   int K1 = 0, K2 = 0;
   T *XA, *YA, *XB, *YB, *PA, *PB, *RA, *RB, *T1, *T2;
   int t1_chunk = 0, t1_parts = 0;  // 2-D: K chunk of the T1 product and the number of chunks
+  bool psplit = false;             // the sine-to-cosine products split by parity
   // The wavemaker chain (paddle motion, paddle-grid products, wavemaker sums,
   // first-order and gradient transforms) depends only on the base-grid eta, so
   // it runs on a second stream beside the HOS recursion, with its own scratch;
@@ -2827,14 +2852,23 @@ struct Engine : EngineBase {
         T1 = upload_t(half);
       } else {
         RA = ar.get<T>((size_t)n1 * 2 * n2);
-        T1 = upload_t(transposed(sin_to_cos_matrix(Nd1, n1, K1), n1, K1));  // k-major
+        // the map vanishes where s + k is even: with even n1 and n2 (multiples of 4,
+        // for the products' vector loads) each output parity takes half the inputs
+        psplit = n1 % 4 == 0 && (n2 == 1 || n2 % 4 == 0);
+        if (psplit)
+          T1 = upload_t(parity_halves_kmajor(sin_to_cos_matrix(Nd1, n1, K1), n1, K1));
+        else
+          T1 = upload_t(transposed(sin_to_cos_matrix(Nd1, n1, K1), n1, K1));  // k-major
       }
       if (n2 > 1) {
         K2 = std::min(2 * n2, N2b);
         XB = ar.get<T>(nb); YB = ar.get<T>(nb);
         PB = ar.get<T>((size_t)2 * n1 * K2);
         RB = ar.get<T>((size_t)2 * n1 * n2);
-        T2 = upload_t(transposed(sin_to_cos_matrix(Nd2, n2, K2), n2, K2));  // k-major
+        if (psplit)
+          T2 = upload_t(parity_halves_kmajor(sin_to_cos_matrix(Nd2, n2, K2), n2, K2));
+        else
+          T2 = upload_t(transposed(sin_to_cos_matrix(Nd2, n2, K2), n2, K2));  // k-major
       }
     } else {
       PA = RA = T1 = nullptr;
@@ -2989,6 +3023,15 @@ struct Engine : EngineBase {
     r2r2d_batch<T>(gb, dir, b, cs, yfirst, cgws);
     c->stats.n_big += b.n;
   }
+  // a forward cos-cos transform to truncated extents, y pass first: the y pass runs
+  // on all N1b columns to o2 rows, then o2 x lines instead of N2b (fewer FFT points
+  // when o2 < N2b)
+  template <class S, class D>
+  void bigF(const S& s, int o1, int o2, const D& d) {
+    Batch2<T, S, D> b;
+    b.add(KCOS, KCOS, s, o1, o2, d, ctmp);
+    bigB(FWD, b, true);
+  }
   // base-grid transforms use tmpB (large enough) as their intermediate
   template <class S, class D>
   void base(int kx, int ky, int dir, const S& s, int o1, int o2, const D& d) {
@@ -3124,7 +3167,7 @@ struct Engine : EngineBase {
                          phizm[iH]);
       if (iH < M) {
         if (iH + 1 == M)
-          big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
+          bigF(src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
         else
           big(KCOS, KCOS, FWD, src_plain<T>(phim, ldb, N1b, N2b), N1b, N2b, Dst<T>{aHOS[iH + 1], ldb, 0});
       }
@@ -3157,20 +3200,21 @@ struct Engine : EngineBase {
     HIP_OK(hipEventRecord(ev_join, sw));
     HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
     // --- free-surface products, filtered to the base modes; P, the products and
-    // the mixed-parity spectra in one batch ---
+    // the mixed-parity spectra in one batch (y passes first: the x pass writes the
+    // spectra's rows and columns in the parity order of the sine-to-cosine products) ---
     {
-      Batch2<T, Src<T, T>, Dst<T>> b;
-      b.add(KCOS, KCOS, src_plain<T>(zsum, ldb, N1b, N2b), n1, n2, Dst<T>{P, n1, 0}, tmpX[0]);
+      Batch2<T, Src<T, T>, DstP<T>> b;
+      b.add(KCOS, KCOS, src_plain<T>(zsum, ldb, N1b, N2b), n1, n2, DstP<T>{P, n1, 0}, tmpX[0]);
       if (M >= 2) {
         // (in a 2-D tank etay, phisy and phiya hold zeros from allocation on: the
         // arena clears every buffer and only the n2 > 1 transforms write them)
         hipLaunchKernelGGL(k_fsbc<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, M, zz, phisx, phisy, etax, etay,
                            phixa, phiya, phiza, X, Y, XA, YA, XB, YB, j0, zsum);
-        b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, Dst<T>{D1, n1, 0}, tmpX[1]);
-        b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, Dst<T>{D2, n1, 0}, tmpX[2]);
+        b.add(KCOS, KCOS, src_plain<T>(X, ldb, N1b, N2b), n1, n2, DstP<T>{D1, n1, 0}, tmpX[1]);
+        b.add(KCOS, KCOS, src_plain<T>(Y, ldb, N1b, N2b), n1, n2, DstP<T>{D2, n1, 0}, tmpX[2]);
         if (C.add_pot()) mixed_parity_spectra(b);
       }
-      bigB(FWD, b);
+      bigB(FWD, b, true);
     }
     if (M >= 2) {
       if (C.add_pot())
@@ -3231,14 +3275,17 @@ struct Engine : EngineBase {
   // filter_ext of the mixed-parity products on the reference grid: exact sine
   // spectra on the padded grid (added to the caller's batch), then the
   // reference grid's sine-to-cosine map (mixed_parity_finish).
-  void mixed_parity_spectra(Batch2<T, Src<T, T>, Dst<T>>& b) {
+  void mixed_parity_spectra(Batch2<T, Src<T, T>, DstP<T>>& b) {
     hos_ctx& C = *c;
     const int n1 = C.n1, n2 = C.n2, ldb = N1b;
-    b.add(KSIN, KCOS, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, Dst<T>{PA, K1, 0}, tmpX[b.n]);
-    b.add(KSIN, KCOS, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, Dst<T>{PA + (size_t)K1 * n2, K1, 0}, tmpX[b.n]);
+    // with psplit the sine rows of PA and the sine columns of PB go in parity order
+    const int ha = psplit ? K1 / 2 : 0, hb = psplit ? K2 / 2 : 0;
+    b.add(KSIN, KCOS, src_plain<T>(XA, ldb, N1b, N2b), K1, n2, DstP<T>{PA, K1, 0, ha, 0}, tmpX[b.n]);
+    b.add(KSIN, KCOS, src_plain<T>(YA, ldb, N1b, N2b), K1, n2, DstP<T>{PA + (size_t)K1 * n2, K1, 0, ha, 0},
+          tmpX[b.n]);
     if (n2 > 1) {
-      b.add(KCOS, KSIN, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, Dst<T>{PB, 2 * n1, 0}, tmpX[b.n]);
-      b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, Dst<T>{PB + n1, 2 * n1, 0}, tmpX[b.n]);
+      b.add(KCOS, KSIN, src_plain<T>(XB, ldb, N1b, N2b), n1, K2, DstP<T>{PB, 2 * n1, 0, 0, hb}, tmpX[b.n]);
+      b.add(KCOS, KSIN, src_plain<T>(YB, ldb, N1b, N2b), n1, K2, DstP<T>{PB + n1, 2 * n1, 0, 0, hb}, tmpX[b.n]);
     }
   }
   // (and Q = D1 + P)
@@ -3248,13 +3295,25 @@ struct Engine : EngineBase {
     if (t1_parts > 0) {
       hipLaunchKernelGGL(k_t1_partial<T>, dim3(nblk(n1), t1_parts), dim3(256), 0, cs, n1, K1, t1_chunk,
                          (const T*)T1, (const T*)PA, RA);
+    } else if (psplit) {
+      // each parity class of the outputs from the other class of the inputs: two
+      // half-size products per map in one launch (batch 0: even outputs from odd
+      // inputs, batch 1: odd from even), rows of RA and columns of RB in that order
+      const int h1 = n1 / 2, hk1 = K1 / 2;
+      hosmm::mm<T>(cs, true, true, h1, 2 * n2, hk1, (const T*)T1, hk1, (long)hk1 * h1, (const T*)PA, K1, hk1, RA,
+                   n1, h1, 2);
+      if (n2 > 1) {
+        const int h2 = n2 / 2, hk2 = K2 / 2;
+        hosmm::mm<T>(cs, false, true, 2 * n1, h2, hk2, (const T*)PB, 2 * n1, (long)2 * n1 * hk2, (const T*)T2, hk2,
+                     (long)hk2 * h2, RB, 2 * n1, (long)2 * n1 * h2, 2);
+      }
     } else {
       gemm(cs, true, true, n1, 2 * n2, K1, (const T*)T1, K1, (const T*)PA, K1, RA, n1);
       if (n2 > 1) gemm(cs, false, true, 2 * n1, n2, K2, (const T*)PB, 2 * n1, (const T*)T2, K2, RB, 2 * n1);
     }
     // (64-thread blocks: a 2-D tank's few outputs spread over more CUs)
     hipLaunchKernelGGL(k_add_mixed<T>, dim3(nblk(ns, 64)), dim3(64), 0, cs, n1, n2, (const T*)RA,
-                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q, t1_parts);
+                       (const T*)(n2 > 1 ? RB : nullptr), D1, D2, (const T*)P, Q, t1_parts, psplit ? 1 : 0);
   }
 
   T* tmpB2() { return X; }

@@ -178,38 +178,62 @@ inline Src<T, U> src_mul(const U* p, int ld, int nx, int ny, int mul, int jpow, 
 // (an output that is not stored takes the offset beyond the resource).  The
 // destination's resource spans 2^31 - 1 bytes from its base: the offsets of
 // valid outputs lie in the field, and kBufOff lies beyond.
-template <typename T>
+// Perm: rows (hx > 0) and/or columns (hy > 0) de-interleaved by parity, odd
+// indices first: index i is stored at i / 2 when odd and at h + i / 2 when even
+// (a product with a matrix that vanishes where the parities of its two indices
+// agree then reads each parity's block contiguously).
+template <typename T, bool Perm = false>
 struct Dst {
   T* __restrict__ p;
   int ld, mode;
+  int hx = 0, hy = 0;
+  static constexpr bool kPerm = Perm;
   struct Pre {
     T old;
   };
   __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc() const { return buf_rsrc(p, 0x7fffffffu); }
+  __device__ __forceinline__ unsigned off(int ix, int iy) const {
+    if constexpr (Perm) {
+      if (hx) ix = (ix & 1) ? (ix >> 1) : hx + (ix >> 1);
+      if (hy) iy = (iy & 1) ? (iy >> 1) : hy + (iy >> 1);
+    }
+    return elem_off<T>(ix, iy, ld);
+  }
   __device__ __forceinline__ Pre pre(int ix, int iy, bool ok) const {
     Pre q;
-    q.old = (mode == 0) ? (T)0 : buf_load<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff);
+    q.old = (mode == 0) ? (T)0 : buf_load<T>(rsrc(), ok ? off(ix, iy) : kBufOff);
     return q;
   }
   __device__ __forceinline__ void put_if(int ix, int iy, T v, const Pre& q, bool ok) const {
-    buf_store<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff, (mode == 0) ? v : q.old + v);
+    buf_store<T>(rsrc(), ok ? off(ix, iy) : kBufOff, (mode == 0) ? v : q.old + v);
   }
   __device__ __forceinline__ void put(int ix, int iy, T v, const Pre& q) const { put_if(ix, iy, v, q, true); }
   __device__ __forceinline__ void put(int ix, int iy, T v) const { put(ix, iy, v, pre(ix, iy, true)); }
-  // outputs (ix, iy) and (ix + 1, iy) in one access
+  // outputs (ix, iy) and (ix + 1, iy) in one access (two where rows are permuted)
   struct Pre2 {
     BufPair<T> old;
   };
   __device__ __forceinline__ Pre2 pre2(int ix, int iy, bool ok) const {
     Pre2 q;
-    q.old = (mode == 0) ? BufPair<T>{(T)0, (T)0} : buf_load2<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff);
+    if constexpr (Perm) {
+      q.old = BufPair<T>{pre(ix, iy, ok).old, pre(ix + 1, iy, ok).old};
+    } else {
+      q.old = (mode == 0) ? BufPair<T>{(T)0, (T)0} : buf_load2<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff);
+    }
     return q;
   }
   __device__ __forceinline__ void put2_if(int ix, int iy, T a, T b, const Pre2& q, bool ok) const {
-    const BufPair<T> v = (mode == 0) ? BufPair<T>{a, b} : BufPair<T>{q.old.x + a, q.old.y + b};
-    buf_store2<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff, v);
+    if constexpr (Perm) {
+      put_if(ix, iy, a, Pre{q.old.x}, ok);
+      put_if(ix + 1, iy, b, Pre{q.old.y}, ok);
+    } else {
+      const BufPair<T> v = (mode == 0) ? BufPair<T>{a, b} : BufPair<T>{q.old.x + a, q.old.y + b};
+      buf_store2<T>(rsrc(), ok ? elem_off<T>(ix, iy, ld) : kBufOff, v);
+    }
   }
 };
+template <typename T>
+using DstP = Dst<T, true>;
 
 // ---------------------------------------------------------------------------
 // line kernels
@@ -1070,11 +1094,15 @@ void r2r2d_batch(const Grid2<T>& g, int dir, const Batch2<T, S, D>& b, hipStream
     p2.dst[i] = b.dst[i];
   }
   if (!yfirst) {
-    launch_x<T>(g.px, g.N1, dir, p1, st, gws, g.gline_ws_pairs());
-    launch_second<T, D>(true, g, dir, p2, st, gws);
+    // (permuted destinations are written by x passes only)
+    if constexpr (D::kPerm) throw std::runtime_error("a permuted destination needs the y pass first");
+    else {
+      launch_x<T>(g.px, g.N1, dir, p1, st, gws, g.gline_ws_pairs());
+      launch_second<T, D>(true, g, dir, p2, st, gws);
+    }
   } else {
     launch_y<T>(g.py, g.TX, g.N2, dir, p1, st);
-    launch_second<T, D>(false, g, dir, p2, st, gws);
+    launch_x<T>(g.px, g.N1, dir, p2, st, gws, g.gline_ws_pairs());
   }
 }
 
