@@ -79,7 +79,7 @@ struct Acc<float> {
 template <typename T, bool AK, bool BK, int TM, int TN, bool VEC>
 __global__ void __launch_bounds__(64 * kKW) k_mm(int m, int n, int k, const T* __restrict__ A, int lda, long sa,
                                                  const T* __restrict__ B, int ldb, long sb, T* __restrict__ C, int ldc,
-                                                 long sc) {
+                                                 long sc, int rsc) {
   using V = typename Acc<T>::V;
   constexpr int RM = 16 * TM, RN = 16 * TN;
   __shared__ T red[kKW][RN][RM + 1];
@@ -181,7 +181,7 @@ __global__ void __launch_bounds__(64 * kKW) k_mm(int m, int n, int k, const T* _
     T s = red[0][cc][rr];
 #pragma unroll
     for (int q = 1; q < kKW; ++q) s += red[q][cc][rr];
-    if (row0 + rr < m && col0 + cc < n) C[(size_t)(row0 + rr) + (size_t)ldc * (col0 + cc)] = s;
+    if (row0 + rr < m && col0 + cc < n) C[(size_t)(row0 + rr) * rsc + (size_t)ldc * (col0 + cc)] = s;
   }
 }
 
@@ -194,34 +194,35 @@ inline int default_tile(int m, int n, int batch) {
 
 template <typename T, bool AK, bool BK, int TM, int TN>
 void launch(hipStream_t st, bool vec, dim3 grid, int m, int n, int k, const T* A, int lda, long sa, const T* B, int ldb,
-            long sb, T* C, int ldc, long sc) {
+            long sb, T* C, int ldc, long sc, int rsc) {
   if (vec)
     hipLaunchKernelGGL((k_mm<T, AK, BK, TM, TN, true>), grid, dim3(64 * kKW), 0, st, m, n, k, A, lda, sa, B, ldb, sb,
-                       C, ldc, sc);
+                       C, ldc, sc, rsc);
   else
     hipLaunchKernelGGL((k_mm<T, AK, BK, TM, TN, false>), grid, dim3(64 * kKW), 0, st, m, n, k, A, lda, sa, B, ldb, sb,
-                       C, ldc, sc);
+                       C, ldc, sc, rsc);
 }
 
 template <typename T, bool AK, bool BK>
 void launch_tile(hipStream_t st, int tile, bool vec, int m, int n, int k, int batch, const T* A, int lda, long sa,
-                 const T* B, int ldb, long sb, T* C, int ldc, long sc) {
+                 const T* B, int ldb, long sb, T* C, int ldc, long sc, int rsc) {
   if (tile == 32)
     launch<T, AK, BK, 2, 2>(st, vec, dim3((m + 31) / 32, (n + 31) / 32, batch), m, n, k, A, lda, sa, B, ldb, sb, C,
-                            ldc, sc);
+                            ldc, sc, rsc);
   else
     launch<T, AK, BK, 1, 1>(st, vec, dim3((m + 15) / 16, (n + 15) / 16, batch), m, n, k, A, lda, sa, B, ldb, sb, C,
-                            ldc, sc);
+                            ldc, sc, rsc);
 }
 
 // C_b = A_b B_b for b < batch on stream st.  ak / bk: the operand is k-major
 // (A[kk + lda i], B[kk + ldb j]); else A[i + lda kk], B[j + ldb kk].  Strides
 // sa, sb, sc between the batch's operands (0: shared).  tile 0: the default.
+// Row i of C_b lies at C_b + i * rsc (rsc = 2: every other row).
 template <typename T>
 void mm(hipStream_t st, bool ak, bool bk, int m, int n, int k, const T* A, int lda, long sa, const T* B, int ldb,
-        long sb, T* C, int ldc, long sc, int batch = 1, int tile = 0) {
+        long sb, T* C, int ldc, long sc, int batch = 1, int tile = 0, int rsc = 1) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
-  if (k < 1 || ldc < m || lda < (ak ? k : m) || ldb < (bk ? k : n))
+  if (k < 1 || rsc < 1 || ldc < (long)(m - 1) * rsc + 1 || lda < (ak ? k : m) || ldb < (bk ? k : n))
     throw std::runtime_error("dense product: inconsistent operand shapes");
   if (batch > 65535) throw std::runtime_error("dense product: batch too large");
   if (tile == 0) tile = default_tile(m, n, batch);
@@ -232,13 +233,13 @@ void mm(hipStream_t st, bool ak, bool bk, int m, int n, int k, const T* A, int l
   };
   const bool vec = (k % kS == 0) && (!ak || aligned(A, lda, sa)) && (!bk || aligned(B, ldb, sb)) && (ak || bk);
   if (ak && bk)
-    launch_tile<T, true, true>(st, tile, vec, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    launch_tile<T, true, true>(st, tile, vec, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc, rsc);
   else if (!ak && bk)
-    launch_tile<T, false, true>(st, tile, vec, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    launch_tile<T, false, true>(st, tile, vec, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc, rsc);
   else if (ak && !bk)
-    launch_tile<T, true, false>(st, tile, vec, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    launch_tile<T, true, false>(st, tile, vec, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc, rsc);
   else
-    launch_tile<T, false, false>(st, tile, false, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc);
+    launch_tile<T, false, false>(st, tile, false, m, n, k, batch, A, lda, sa, B, ldb, sb, C, ldc, sc, rsc);
   if (hipGetLastError() != hipSuccess) throw std::runtime_error("dense product: launch failed");
 }
 

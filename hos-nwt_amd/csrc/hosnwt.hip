@@ -2240,10 +2240,38 @@ __global__ void k_dense_in(int n1, int n2, S src, U* __restrict__ out) {
   out[i] = (U)src.get(i1, i2);
 }
 
+// Forward DCT-I / DST-I matrices M[m][k] satisfy M[m][n-1-k] = sig_m M[m][k]
+// (sig_m = (-1)^m for cos, -(-1)^m for sin), so for even n the forward product
+// of the even modes needs only x_k + sig_0 x_{n-1-k} (k < n/2) and that of the
+// odd modes x_k - sig_0 x_{n-1-k}: the staged source holds those two halves
+// (even-mode half first) and two half-size products replace the full one.
+template <typename U, class S>
+__global__ void k_dense_fold(int n1, int n2, int sig0, S src, U* __restrict__ out) {
+  const int h = n1 / 2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)h * n2) return;
+  const int k = (int)(i % h), i2 = (int)(i / h);
+  const U a = (U)src.get(k, i2), b = (U)src.get(n1 - 1 - k, i2);
+  out[k + (size_t)n1 * i2] = sig0 > 0 ? a + b : a - b;
+  out[h + k + (size_t)n1 * i2] = sig0 > 0 ? a - b : a + b;
+}
+// the even-mode rows (b = 0) and the odd-mode rows (b = 1) of a k-major forward
+// matrix, restricted to k < n/2: out[b h h + k + h i] = M[2i + b][k]
+template <typename U>
+__global__ void k_parity_rows(int n1, const U* __restrict__ M, U* __restrict__ out) {
+  const int h = n1 / 2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)2 * h * h) return;
+  const int k = (int)(i % h), r = (int)((i / h) % h), b = (int)(i / ((size_t)h * h));
+  out[i] = M[k + (size_t)n1 * (2 * r + b)];
+}
+
 template <typename U>
 struct DenseGrid {
   int n1 = 0, n2 = 1;
   U* mx[2][2] = {};  // [kind][0 forward, 1 inverse]
+  U* mxh[2] = {};    // [kind]: the forward matrix's parity halves (k_parity_rows), when fold
+  bool fold = false;
   U* my[2][2] = {};
   U* stage = nullptr;  // source after multiplier / conversion
   U* mid = nullptr;    // after the x product
@@ -2308,6 +2336,18 @@ void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
       g.mx[k][d] = dense_matrix<U>(ar, n1, k, d == 0 ? FWD : INV, true);
       if (n2 > 1) g.my[k][d] = dense_matrix<U>(ar, n2, k, d == 0 ? FWD : INV, true);
     }
+  // (n1 a multiple of 8: the half products keep 16-byte operand rows)
+  g.fold = n2 > 1 && n1 % 8 == 0;
+  if (g.fold) {
+    const size_t hh = (size_t)(n1 / 2) * (n1 / 2);
+    for (int k = 0; k < 2; ++k) {
+      g.mxh[k] = ar.get<U>(2 * hh);
+      hipLaunchKernelGGL(k_parity_rows<U>, dim3((unsigned)((2 * hh + 255) / 256)), dim3(256), 0, 0, n1,
+                         (const U*)g.mx[k][0], g.mxh[k]);
+      HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipDeviceSynchronize());
+  }
   g.stage = ar.get<U>((size_t)n1 * n2);
   g.mid = ar.get<U>((size_t)n1 * n2);
 }
@@ -2320,6 +2360,17 @@ void dense2d(const DenseGrid<U>& g, hipStream_t st, int kx, int ky, int dir, con
   if (d.mode != 0) throw std::runtime_error("dense base transform: plain stores only");
   const int n1 = g.n1, n2 = g.n2, di = (dir == FWD) ? 0 : 1;
   const size_t n = (size_t)n1 * n2;
+  if (dir == FWD && g.fold && nout1 == n1 && n2 > 1) {
+    // folded source, then the even and odd modes' half products written to
+    // every other row of mid (one launch)
+    const int h = n1 / 2;
+    hipLaunchKernelGGL((k_dense_fold<U, S>), dim3((unsigned)(((size_t)h * n2 + 255) / 256)), dim3(256), 0, st, n1,
+                       n2, kx == KCOS ? 1 : -1, src, g.stage);
+    hosmm::mm<U>(st, true, true, h, n2, h, (const U*)g.mxh[kx], h, (long)h * h, (const U*)g.stage, n1, h, g.mid,
+                 n1, 1, 2, 0, 2);
+    gemm(st, false, true, nout1, nout2, n2, (const U*)g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
+    return;
+  }
   // a plain source of the product's type and layout is read in place
   const U* in = g.stage;
   bool staged = true;
@@ -3070,10 +3121,20 @@ struct Engine : EngineBase {
     if (k > kMulti || !multi) throw std::runtime_error("dense_multi: no scratch for this batch");
     U* stage = reinterpret_cast<U*>(multi);
     U* mid = stage + (size_t)kMulti * ns;
-    for (int f = 0; f < k; ++f)
-      hipLaunchKernelGGL((k_dense_in<U, Src<U, double>>), dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, src[f],
-                         stage + (size_t)f * ns);
-    gemm(cs, true, true, n1, k * n2, n1, (const U*)g.mx[KCOS][di], n1, (const U*)stage, n1, mid, n1);
+    if (dir == FWD && g.fold) {
+      // folded sources and the two half products (see k_dense_fold)
+      const int h = n1 / 2;
+      for (int f = 0; f < k; ++f)
+        hipLaunchKernelGGL((k_dense_fold<U, Src<U, double>>), dim3(nblk((size_t)h * n2)), dim3(256), 0, cs, n1, n2, 1,
+                           src[f], stage + (size_t)f * ns);
+      hosmm::mm<U>(cs, true, true, h, k * n2, h, (const U*)g.mxh[KCOS], h, (long)h * h, (const U*)stage, n1, h, mid,
+                   n1, 1, 2, 0, 2);
+    } else {
+      for (int f = 0; f < k; ++f)
+        hipLaunchKernelGGL((k_dense_in<U, Src<U, double>>), dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, src[f],
+                           stage + (size_t)f * ns);
+      gemm(cs, true, true, n1, k * n2, n1, (const U*)g.mx[KCOS][di], n1, (const U*)stage, n1, mid, n1);
+    }
     // the y products write each run of outputs that lie ns apart in one launch
     for (int f0 = 0; f0 < k;) {
       int f1 = f0 + 1;
