@@ -2240,11 +2240,12 @@ __global__ void k_dense_in(int n1, int n2, S src, U* __restrict__ out) {
   out[i] = (U)src.get(i1, i2);
 }
 
-// Forward DCT-I / DST-I matrices M[m][k] satisfy M[m][n-1-k] = sig_m M[m][k]
-// (sig_m = (-1)^m for cos, -(-1)^m for sin), so for even n the forward product
-// of the even modes needs only x_k + sig_0 x_{n-1-k} (k < n/2) and that of the
-// odd modes x_k - sig_0 x_{n-1-k}: the staged source holds those two halves
-// (even-mode half first) and two half-size products replace the full one.
+// The DCT-I / DST-I matrices of both directions satisfy M[r][n-1-k] = sig_r M[r][k]
+// (sig_r = (-1)^r for cos, -(-1)^r for sin: cos(pi r k / (n-1)) is symmetric in r
+// and k, and the end weights and normalisations are symmetric), so for even n
+// the product's even rows need only x_k + sig_0 x_{n-1-k} (k < n/2) and its odd
+// rows x_k - sig_0 x_{n-1-k}: the staged source holds those two halves (even-row
+// half first) and two half-size products replace the full one.
 template <typename U, class S>
 __global__ void k_dense_fold(int n1, int n2, int sig0, S src, U* __restrict__ out) {
   const int h = n1 / 2;
@@ -2255,8 +2256,8 @@ __global__ void k_dense_fold(int n1, int n2, int sig0, S src, U* __restrict__ ou
   out[k + (size_t)n1 * i2] = sig0 > 0 ? a + b : a - b;
   out[h + k + (size_t)n1 * i2] = sig0 > 0 ? a - b : a + b;
 }
-// the even-mode rows (b = 0) and the odd-mode rows (b = 1) of a k-major forward
-// matrix, restricted to k < n/2: out[b h h + k + h i] = M[2i + b][k]
+// the even rows (b = 0) and the odd rows (b = 1) of a k-major matrix, restricted
+// to k < n/2: out[b h h + k + h i] = M[2i + b][k]
 template <typename U>
 __global__ void k_parity_rows(int n1, const U* __restrict__ M, U* __restrict__ out) {
   const int h = n1 / 2;
@@ -2270,7 +2271,7 @@ template <typename U>
 struct DenseGrid {
   int n1 = 0, n2 = 1;
   U* mx[2][2] = {};  // [kind][0 forward, 1 inverse]
-  U* mxh[2] = {};    // [kind]: the forward matrix's parity halves (k_parity_rows), when fold
+  U* mxh[2][2] = {};  // [kind][direction]: the parity halves of mx (k_parity_rows), when fold
   bool fold = false;
   U* my[2][2] = {};
   U* stage = nullptr;  // source after multiplier / conversion
@@ -2340,12 +2341,13 @@ void build_dense(DenseGrid<U>& g, Arena& ar, int n1, int n2) {
   g.fold = n2 > 1 && n1 % 8 == 0;
   if (g.fold) {
     const size_t hh = (size_t)(n1 / 2) * (n1 / 2);
-    for (int k = 0; k < 2; ++k) {
-      g.mxh[k] = ar.get<U>(2 * hh);
-      hipLaunchKernelGGL(k_parity_rows<U>, dim3((unsigned)((2 * hh + 255) / 256)), dim3(256), 0, 0, n1,
-                         (const U*)g.mx[k][0], g.mxh[k]);
-      HIP_OK(hipGetLastError());
-    }
+    for (int k = 0; k < 2; ++k)
+      for (int d = 0; d < 2; ++d) {
+        g.mxh[k][d] = ar.get<U>(2 * hh);
+        hipLaunchKernelGGL(k_parity_rows<U>, dim3((unsigned)((2 * hh + 255) / 256)), dim3(256), 0, 0, n1,
+                           (const U*)g.mx[k][d], g.mxh[k][d]);
+        HIP_OK(hipGetLastError());
+      }
     HIP_OK(hipDeviceSynchronize());
   }
   g.stage = ar.get<U>((size_t)n1 * n2);
@@ -2360,13 +2362,13 @@ void dense2d(const DenseGrid<U>& g, hipStream_t st, int kx, int ky, int dir, con
   if (d.mode != 0) throw std::runtime_error("dense base transform: plain stores only");
   const int n1 = g.n1, n2 = g.n2, di = (dir == FWD) ? 0 : 1;
   const size_t n = (size_t)n1 * n2;
-  if (dir == FWD && g.fold && nout1 == n1 && n2 > 1) {
-    // folded source, then the even and odd modes' half products written to
-    // every other row of mid (one launch)
+  if (g.fold && nout1 == n1 && n2 > 1) {
+    // folded source, then the even and odd rows' half products written to every
+    // other row of mid (one launch)
     const int h = n1 / 2;
     hipLaunchKernelGGL((k_dense_fold<U, S>), dim3((unsigned)(((size_t)h * n2 + 255) / 256)), dim3(256), 0, st, n1,
                        n2, kx == KCOS ? 1 : -1, src, g.stage);
-    hosmm::mm<U>(st, true, true, h, n2, h, (const U*)g.mxh[kx], h, (long)h * h, (const U*)g.stage, n1, h, g.mid,
+    hosmm::mm<U>(st, true, true, h, n2, h, (const U*)g.mxh[kx][di], h, (long)h * h, (const U*)g.stage, n1, h, g.mid,
                  n1, 1, 2, 0, 2);
     gemm(st, false, true, nout1, nout2, n2, (const U*)g.mid, n1, g.my[ky][di], n2, d.p, d.ld);
     return;
@@ -3121,14 +3123,14 @@ struct Engine : EngineBase {
     if (k > kMulti || !multi) throw std::runtime_error("dense_multi: no scratch for this batch");
     U* stage = reinterpret_cast<U*>(multi);
     U* mid = stage + (size_t)kMulti * ns;
-    if (dir == FWD && g.fold) {
+    if (g.fold) {
       // folded sources and the two half products (see k_dense_fold)
       const int h = n1 / 2;
       for (int f = 0; f < k; ++f)
         hipLaunchKernelGGL((k_dense_fold<U, Src<U, double>>), dim3(nblk((size_t)h * n2)), dim3(256), 0, cs, n1, n2, 1,
                            src[f], stage + (size_t)f * ns);
-      hosmm::mm<U>(cs, true, true, h, k * n2, h, (const U*)g.mxh[KCOS], h, (long)h * h, (const U*)stage, n1, h, mid,
-                   n1, 1, 2, 0, 2);
+      hosmm::mm<U>(cs, true, true, h, k * n2, h, (const U*)g.mxh[KCOS][di], h, (long)h * h, (const U*)stage, n1, h,
+                   mid, n1, 1, 2, 0, 2);
     } else {
       for (int f = 0; f < k; ++f)
         hipLaunchKernelGGL((k_dense_in<U, Src<U, double>>), dim3(nblk(ns)), dim3(256), 0, cs, n1, n2, src[f],
