@@ -2256,6 +2256,25 @@ __global__ void k_dense_fold(int n1, int n2, int sig0, S src, U* __restrict__ ou
   out[k + (size_t)n1 * i2] = sig0 > 0 ? a + b : a - b;
   out[h + k + (size_t)n1 * i2] = sig0 > 0 ? a - b : a + b;
 }
+// k_dense_fold for up to four sources in one launch (blockIdx.y = source),
+// source f staged at out + f n1 n2
+template <class S>
+struct FoldSet {
+  S src[4];
+};
+template <typename U, class S>
+__global__ void k_dense_fold_multi(int n1, int n2, int sig0, FoldSet<S> fs, U* __restrict__ out) {
+  const int h = n1 / 2;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= (size_t)h * n2) return;
+  const S& src = fs.src[blockIdx.y];
+  U* o = out + (size_t)blockIdx.y * n1 * n2;
+  const int k = (int)(i % h), i2 = (int)(i / h);
+  const U a = (U)src.get(k, i2), b = (U)src.get(n1 - 1 - k, i2);
+  o[k + (size_t)n1 * i2] = sig0 > 0 ? a + b : a - b;
+  o[h + k + (size_t)n1 * i2] = sig0 > 0 ? a - b : a + b;
+}
+
 // the even rows (b = 0) and the odd rows (b = 1) of a k-major matrix, restricted
 // to k < n/2: out[b h h + k + h i] = M[2i + b][k]
 template <typename U>
@@ -3126,9 +3145,11 @@ struct Engine : EngineBase {
     if (g.fold) {
       // folded sources and the two half products (see k_dense_fold)
       const int h = n1 / 2;
-      for (int f = 0; f < k; ++f)
-        hipLaunchKernelGGL((k_dense_fold<U, Src<U, double>>), dim3(nblk((size_t)h * n2)), dim3(256), 0, cs, n1, n2, 1,
-                           src[f], stage + (size_t)f * ns);
+      static_assert(kMulti <= 4, "FoldSet holds four sources");
+      FoldSet<Src<U, double>> fs{};
+      for (int f = 0; f < k; ++f) fs.src[f] = src[f];
+      hipLaunchKernelGGL((k_dense_fold_multi<U, Src<U, double>>), dim3(nblk((size_t)h * n2), k), dim3(256), 0, cs, n1,
+                         n2, 1, fs, stage);
       hosmm::mm<U>(cs, true, true, h, k * n2, h, (const U*)g.mxh[KCOS][di], h, (long)h * h, (const U*)stage, n1, h,
                    mid, n1, 1, 2, 0, 2);
     } else {
