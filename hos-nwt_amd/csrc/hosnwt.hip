@@ -2719,7 +2719,8 @@ struct Engine : EngineBase {
   // it runs on a second stream beside the HOS recursion, with its own scratch;
   // helpers use the current stream / scratch (cs, ctmp, ctmpX).
   hipStream_t sw = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_join2 = nullptr;
+  bool fo_late = false;  // this evaluation's first_order_rest is pending
   T* tmpBw = nullptr;
   std::vector<T*> tmpXw;
   hipStream_t cs = nullptr;
@@ -2829,6 +2830,7 @@ struct Engine : EngineBase {
     HIP_OK(hipStreamCreateWithFlags(&sw, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    HIP_OK(hipEventCreateWithFlags(&ev_join2, hipEventDisableTiming));
     tmpBw = ar.get<T>(nb);
     for (int i = 0; i < 4; ++i) tmpXw.push_back(ar.get<T>(nb));
     if (const size_t gw = gb.gline_ws_elems()) {
@@ -3067,6 +3069,7 @@ struct Engine : EngineBase {
     if (sw) (void)hipStreamSynchronize(sw);
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_join2) (void)hipEventDestroy(ev_join2);
     if (sw) (void)hipStreamDestroy(sw);
   }
 
@@ -3279,10 +3282,22 @@ struct Engine : EngineBase {
         slopes(b, *ctmpX);
         bigB(INV, b);
       }
+      fo_late = false;
       wavemaker(t, a1, e1, aad, ka1, ke1, kad1, kadr, s2);
     }
     HIP_OK(hipEventRecord(ev_join, sw));
     HIP_OK(hipStreamWaitEvent(st, ev_join, 0));
+    // the first-order conditions continue on the wavemaker stream beside the
+    // free-surface products; the main stream waits for them at the end
+    if (fo_late) {
+      struct Side {
+        Engine* e;
+        ~Side() { e->use_main(); }
+      } side{this};
+      use_side();
+      first_order_rest(e1, a1, ka1, ke1);
+      HIP_OK(hipEventRecord(ev_join2, sw));
+    }
     // --- free-surface products, filtered to the base modes; P, the products and
     // the mixed-parity spectra in one batch (y passes first: the x pass writes the
     // spectra's rows and columns in the parity order of the sine-to-cosine products) ---
@@ -3320,6 +3335,7 @@ struct Engine : EngineBase {
         hipLaunchKernelGGL(k_beach<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)C.nu_d, Zp);
         base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
       }
+      if (fo_late) HIP_OK(hipStreamWaitEvent(st, ev_join2, 0));  // Zh64, ka1, ke1
       hipLaunchKernelGGL((k_expon_out<T, double>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)nullptr,
                          (const double*)phizh64, (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh,
@@ -3328,6 +3344,7 @@ struct Engine : EngineBase {
       hipLaunchKernelGGL(k_dphin<T>, dim3(nbs), dim3(256), 0, st, ns, (const T*)tmpS, (const double*)phiz_add,
                          (const double*)phit_add, (const double*)C.nu_d, Zp);
       base(KCOS, KCOS, FWD, src_plain<T>(Zp, n1, n1, n2), n1, n2, Dst<T>{Zh, n1, 0});
+      if (fo_late) HIP_OK(hipStreamWaitEvent(st, ev_join2, 0));  // ka1, ke1
       hipLaunchKernelGGL((k_expon_out<T, T>), dim3(nbs), dim3(256), 0, st, n1, n2, pix, piy, dt,
                          (const T*)(M >= 2 ? D1 : nullptr), (const T*)P, (const T*)phis_hat, (const T*)phizh,
                          (const T*)(M >= 2 ? D2 : nullptr), (const T*)Zh, (const double*)nullptr,
@@ -3768,7 +3785,8 @@ struct Engine : EngineBase {
     // additional potential at z = eta and its first-order part at z = 0
     phiadd(aad, kad1, eta_phys);
     if (multi) {
-      first_order_batched(e1, a1, ka1, ke1);
+      first_order_gradients();
+      fo_late = true;  // first_order_rest after the join (rhs)
     } else {
       first_order_single(e1, a1, ka1, ke1);
     }
@@ -3907,7 +3925,30 @@ struct Engine : EngineBase {
   // precision batched (dense_multi): inverse {kth1 a1st, kth1 phis (mixed)},
   // forward FP64 {phidphiz, phiz_add, forcing (mixed)} or {phidphiz, grad phi_add}
   // (FP64), forward FP32 {phix_add, phiy_add} (mixed).
-  void first_order_batched(const double* e1, const double* a1, double* ka1, double* ke1) {
+  // The same on a dense 3-D base grid in two parts.  first_order_gradients: the
+  // modes of grad phi_add, which the free-surface products read after the join;
+  // first_order_rest (enqueued on the wavemaker stream after the join's event):
+  // the first-order free-surface conditions (ke1, ka1) and, on the mixed path, the
+  // FP64 forcing modes Zh64, read only at the end of the right-hand side.
+  void first_order_gradients() {
+    const int n1 = c->n1, n2 = c->n2;
+    if constexpr (kSplit) {
+      Src<double, double> in[1] = {src_plain<double, double>(phiz_add, n1, n1, n2)};
+      double* out[1] = {phizh64};
+      dense_multi<double>(grid64(), FWD, 1, in, out);
+      hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, cs, ns, (const double*)phizh64, phizh);
+      Src<T, double> inT[2] = {src_plain<T, double>(phix_add, n1, n1, n2), src_plain<T, double>(phiy_add, n1, n1, n2)};
+      T* outT[2] = {phixh, phiyh};
+      dense_multi<T>(dT, FWD, 2, inT, outT);
+    } else {
+      Src<double, double> in[3] = {src_plain<double, double>(phix_add, n1, n1, n2),
+                                   src_plain<double, double>(phiy_add, n1, n1, n2),
+                                   src_plain<double, double>(phiz_add, n1, n1, n2)};
+      double* out[3] = {(double*)phixh, (double*)phiyh, (double*)phizh};
+      dense_multi<double>(grid64(), FWD, 3, in, out);
+    }
+  }
+  void first_order_rest(const double* e1, const double* a1, double* ka1, double* ke1) {
     hos_ctx& C = *c;
     hipStream_t st = cs;
     const int n1 = C.n1, n2 = C.n2;
@@ -3925,22 +3966,14 @@ struct Engine : EngineBase {
     if constexpr (kSplit) {
       hipLaunchKernelGGL(k_forcing, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phiz_add,
                          (const double*)phit_add, (const double*)wlin, (const double*)C.nu_d, Zf64);
-      Src<double, double> in[3] = {src_plain<double, double>(phiz_add, n1, n1, n2),
-                                   src_plain<double, double>(Zf64, n1, n1, n2),
+      Src<double, double> in[2] = {src_plain<double, double>(Zf64, n1, n1, n2),
                                    src_plain<double, double>(tmp1, n1, n1, n2)};
-      double* out[3] = {phizh64, Zh64, ka1};
-      dense_multi<double>(grid64(), FWD, 3, in, out);
-      hipLaunchKernelGGL(k_narrow<T>, dim3(nblk(ns)), dim3(256), 0, st, ns, (const double*)phizh64, phizh);
-      Src<T, double> inT[2] = {src_plain<T, double>(phix_add, n1, n1, n2), src_plain<T, double>(phiy_add, n1, n1, n2)};
-      T* outT[2] = {phixh, phiyh};
-      dense_multi<T>(dT, FWD, 2, inT, outT);
+      double* out[2] = {Zh64, ka1};
+      dense_multi<double>(grid64(), FWD, 2, in, out);
     } else {
-      Src<double, double> in[4] = {src_plain<double, double>(tmp1, n1, n1, n2),
-                                   src_plain<double, double>(phix_add, n1, n1, n2),
-                                   src_plain<double, double>(phiy_add, n1, n1, n2),
-                                   src_plain<double, double>(phiz_add, n1, n1, n2)};
-      double* out[4] = {ka1, (double*)phixh, (double*)phiyh, (double*)phizh};
-      dense_multi<double>(grid64(), FWD, 4, in, out);
+      Src<double, double> in[1] = {src_plain<double, double>(tmp1, n1, n1, n2)};
+      double* out[1] = {ka1};
+      dense_multi<double>(grid64(), FWD, 1, in, out);
     }
   }
 
