@@ -247,9 +247,8 @@ struct PtrSet {
 // over a fixed bound so that every index is known at compile time.
 constexpr int kMaxOrder = 10;
 template <typename T>
-__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void hos_level_point(size_t i, int iH, int M, const PtrSet<T>& lv, const T* __restrict__ eta,
+                                                T* phim, T* phizm) {
   const bool do_phim = iH < M;
   const int top = min(iH, M - 1);  // highest power read at this level
   T pw[kMaxOrder];
@@ -273,6 +272,62 @@ __global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __re
   }
   if (do_phim) phim[i] = pm;
   phizm[i] = pz;
+}
+template <typename T>
+__global__ void k_hos_level(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  hos_level_point(i, iH, M, lv, eta, phim, phizm);
+}
+// The same with 16 bytes of each field per thread (4 FP32 or 2 FP64 points, the
+// loads issued together); every field 16-byte aligned (checked by the caller), the
+// last partial group point by point.
+template <typename T>
+__global__ void k_hos_level_vec(size_t n, int iH, int M, PtrSet<T> lv, const T* __restrict__ eta, T* phim, T* phizm) {
+  constexpr int W = 16 / sizeof(T);
+  typedef T V __attribute__((ext_vector_type(W)));
+  const size_t i0 = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) * W;
+  if (i0 >= n) return;
+  if (i0 + W > n) {
+    for (size_t i = i0; i < n; ++i) hos_level_point(i, iH, M, lv, eta, phim, phizm);
+    return;
+  }
+  const bool do_phim = iH < M;
+  const int top = min(iH, M - 1);
+  // (eta is read only where a power of it enters: with M < 2 it is not allocated)
+  V ev = {};
+  if (top >= 1) ev = *reinterpret_cast<const V*>(eta + i0);
+  V vv[kMaxOrder];
+#pragma unroll
+  for (int j = 1; j < kMaxOrder; ++j)
+    if (j <= iH) vv[j] = *reinterpret_cast<const V*>(lv.p[iH - j] + i0);
+  V pmv, pzv;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    T pw[kMaxOrder];
+#pragma unroll
+    for (int j = 0; j < kMaxOrder; ++j) pw[j] = (T)0;
+    pw[0] = (T)1;
+    if (top >= 1) pw[1] = ev[w];
+#pragma unroll
+    for (int j = 1; j < kMaxOrder - 1; ++j)
+      if (j + 1 <= top) pw[j + 1] = pw[j] / (T)(j + 1) * pw[1];
+    T pm = (T)0, pz = (T)0;
+#pragma unroll
+    for (int j = kMaxOrder - 1; j >= 1; --j) {
+      if (j <= iH) {
+        const T v = vv[j][w];
+        const T e = do_phim ? pw[j] : (T)0;
+        const T e1 = (j >= 2) ? pw[j - 1] : (T)1;
+        pm = pm - v * e;
+        pz = pz + v * e1;
+      }
+    }
+    pmv[w] = pm;
+    pzv[w] = pz;
+  }
+  if (do_phim) *reinterpret_cast<V*>(phim + i0) = pmv;
+  *reinterpret_cast<V*>(phizm + i0) = pzv;
 }
 
 // out = sum_{j0 <= j <= M} z_j (zero when j0 > M)
@@ -3250,8 +3305,19 @@ struct Engine : EngineBase {
         lv.p[0] = lvl[0];  // transformed with the slopes
       else
         bigB(INV, b);
-      hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, (const T*)eta_ext, phim,
-                         phizm[iH]);
+      {
+        bool al = (uintptr_t)eta_ext % 16 == 0 && (uintptr_t)phizm[iH] % 16 == 0;
+        if (iH < M) al = al && (uintptr_t)phim % 16 == 0;
+        for (int q = 0; q < iH; ++q) al = al && (uintptr_t)lv.p[q] % 16 == 0;
+        constexpr int W = 16 / sizeof(T);
+        // (FP32: four points per thread; FP64 measured no faster with two)
+        if (al && sizeof(T) == 4)
+          hipLaunchKernelGGL(k_hos_level_vec<T>, dim3(nblk((nb + W - 1) / W)), dim3(256), 0, st, nb, iH, M, lv,
+                             (const T*)eta_ext, phim, phizm[iH]);
+        else
+          hipLaunchKernelGGL(k_hos_level<T>, dim3(nblk(nb)), dim3(256), 0, st, nb, iH, M, lv, (const T*)eta_ext, phim,
+                             phizm[iH]);
+      }
       if (iH < M) {
         if (iH + 1 == M)
           bigF(src_plain<T>(phim, ldb, N1b, N2b), n1, n2, Dst<T>{aHOS[iH + 1], n1, 0});
